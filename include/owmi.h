@@ -1,0 +1,210 @@
+/*
+ * owmi: MI355X (gfx950) kernels for optweight's conjugate-gradient
+ * Wiener-filter path, exported as a plain C ABI.
+ *
+ * Conventions (all functions):
+ *   - return 0 on success, nonzero on error; owmi_last_error() returns a
+ *     thread-local message for the last failure. No C++ exception crosses
+ *     the ABI.
+ *   - every pointer named *dev* or described as "device" is a device
+ *     (HBM) pointer owned by the caller; host arrays are marked "host".
+ *   - every call is asynchronous on the given hipStream_t (passed as void*;
+ *     NULL means the default stream).
+ *   - dtype: OWMI_F32 (float / complex float) or OWMI_F64 (double /
+ *     complex double). Complex arrays are interleaved (re, im).
+ *   - alm arrays use optweight's triangular m-major layout with
+ *     mmax = lmax: index(l, m) = m (2 lmax + 1 - m) / 2 + l,
+ *     nelem = (lmax^2 + 3 lmax) / 2 + 1, components contiguous blocks of
+ *     nelem (optweight src/optweight_alm_c_utils.c:3-10).
+ *   - maps are ring-major with per-ring offsets (optweight
+ *     map_utils.MapInfo, map_utils.py:19-297).
+ *
+ * Reference interfaces replaced are cited per function (paths relative to
+ * the optweight repository).
+ */
+#ifndef OWMI_H
+#define OWMI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OWMI_F32 0
+#define OWMI_F64 1
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *owmi_last_error(void);
+
+/* ABI version (increments on incompatible change). */
+int owmi_abi_version(void);
+
+/* ------------------------------------------------------------------ */
+/* Streaming kernels on alm                                            */
+/* ------------------------------------------------------------------ */
+
+/*
+ * out[t, i, lm] = sum_j lmat[i, j, l(lm)] in[t, j, lm]   (diag == 0)
+ * out[t, i, lm] = lmat[i, l(lm)] in[t, i, lm]             (diag != 0)
+ * for t < ntrans, arrays (ntrans, ncomp, nelem). lmat has the alm precision
+ * (float for OWMI_F32, double for OWMI_F64), shape (ncomp, ncomp, lmax+1)
+ * or (ncomp, lmax+1). out may equal in (in-place).
+ * Replaces lmul_dp / lmul_sp / lmul_diag_dp / lmul_diag_sp /
+ * lmul_inplace_dp / lmul_inplace_sp / lmul_diag_inplace_dp /
+ * lmul_diag_inplace_sp (include/optweight_alm_c_utils.h:17-90, 116-140)
+ * and the leading-dimension loop of cython/alm_c_utils.pyx:157-250.
+ */
+int owmi_lmul(int dtype, const void *lmat, int diag, const void *in,
+              void *out, int lmax, int ncomp, long long ntrans,
+              void *stream);
+
+/*
+ * Copy l <= lmax_out, m <= lmax_out of (ntrans*ncomp) alm rows at lmax into
+ * rows at lmax_out. Replaces trunc_alm_dp / trunc_alm_sp
+ * (include/optweight_alm_c_utils.h:92-101, 142-146).
+ */
+int owmi_trunc_alm(int dtype, const void *alm, void *alm_out, int lmax,
+                   int lmax_out, long long nrows, void *stream);
+
+/*
+ * alm[r, lm(l, m)] += w_ell[l] wlm[r, lm'(l, m)] for l, m <= lmax_w over
+ * nrows rows; w_ell has the alm precision. Replaces wlm2alm_dp /
+ * wlm2alm_sp (include/optweight_alm_c_utils.h:41-58, 128-133).
+ */
+int owmi_wlm2alm(int dtype, const void *w_ell, const void *wlm, void *alm,
+                 int lmax_w, int lmax_a, long long nrows, void *stream);
+
+/*
+ * Real-field inner product over nrows alm rows at lmax (complex, dtype):
+ *   *dev_out = 2 Re sum a conj(b) - Re sum_{m=0} a conj(b)
+ * accumulated in double; dev_out is a device double. Deterministic
+ * (fixed-order two-pass reduction).
+ * Replaces alm_utils.contract_almxblm (optweight/alm_utils.py:9-44).
+ */
+int owmi_alm_dot(int dtype, const void *a, const void *b, int lmax,
+                 long long nrows, double *dev_out, void *stream);
+
+/* ------------------------------------------------------------------ */
+/* Streaming kernels on maps                                           */
+/* ------------------------------------------------------------------ */
+
+/*
+ * In-place imap[k, offsets[r] + j * stride[r]] *= weight[r] for j < nphi[r],
+ * r < nrow, k < nmaps (maps map_stride elements apart). weight, nphi,
+ * offsets, stride are device arrays. For OWMI_F32 the weight is rounded
+ * to float before the multiply, as the reference does.
+ * Replaces _apply_ringweight_core_sp / _apply_ringweight_core_dp
+ * (include/optweight_map_c_utils.h:10-13).
+ */
+int owmi_apply_ringweight(int dtype, void *imap, const double *weight,
+                          const long long *nphi, const long long *offsets,
+                          const long long *stride, long long nrow,
+                          long long nmaps, long long map_stride,
+                          void *stream);
+
+/*
+ * Pixel-diagonal matrix-vector product, out may equal in:
+ *   kind 0: out[i, p] = mat[i, p] in[i, p]                 (mat (ncomp, npix))
+ *   kind 1: out[i, p] = sum_j mat[i, j, p] in[j, p]        (mat (ncomp, ncomp, npix))
+ * Replaces mat_utils.matvec (optweight/mat_utils.py:475-532) as used by
+ * operators.PixMatVecMap (operators.py:162-204).
+ */
+int owmi_pix_matvec(int dtype, const void *mat, int kind, const void *in,
+                    void *out, int ncomp, long long npix, void *stream);
+
+/* ------------------------------------------------------------------ */
+/* Conjugate-gradient vector updates (scalars stay on the device)      */
+/* ------------------------------------------------------------------ */
+
+/*
+ * alpha = *dev_num / *dev_den (doubles on device);
+ * x += alpha p; r -= alpha q; over n complex elements.
+ * The x/r update of pixell.utils.CG.step as called through
+ * solvers.CGWienerMap (solvers.py:127-141).
+ */
+int owmi_cg_xr(int dtype, void *x, void *r, const void *p, const void *q,
+               const double *dev_num, const double *dev_den, long long n,
+               void *stream);
+
+/* beta = *dev_num / *dev_den; p = z + beta p over n complex elements. */
+int owmi_cg_p(int dtype, void *p, const void *z, const double *dev_num,
+              const double *dev_den, long long n, void *stream);
+
+/* y = a x + b y over n complex elements with host scalars a, b. */
+int owmi_axpby(int dtype, double a, const void *x, double b, void *y,
+               long long n, void *stream);
+
+/* Convert n complex elements between precisions (dst may not alias src). */
+int owmi_convert(int dtype_src, const void *src, int dtype_dst, void *dst,
+                 long long n, void *stream);
+
+/* ------------------------------------------------------------------ */
+/* Spherical harmonic transforms                                       */
+/* ------------------------------------------------------------------ */
+
+typedef struct owmi_sht_plan owmi_sht_plan;
+
+/*
+ * Build a transform plan for a ring geometry (host arrays of length nrings:
+ * theta (colatitude), weight (per-pixel quadrature weight), nphi, offsets,
+ * phi0) and triangular alm with mmax = lmax. Mirror rings
+ * (|theta_i + theta_j - pi| < 1e-12) share one Legendre evaluation. The
+ * plan holds its tables on the current device and grows a workspace as
+ * transforms of more rows are requested.
+ * Replaces the geometry arguments optweight passes to
+ * ducc0.sht.experimental.{synthesis, adjoint_synthesis}
+ * (optweight/sht.py:56-93, 141-174).
+ */
+int owmi_sht_plan_create(owmi_sht_plan **plan, long long nrings,
+                         const double *theta, const double *weight,
+                         const long long *nphi, const long long *offsets,
+                         const double *phi0, int lmax);
+
+int owmi_sht_plan_destroy(owmi_sht_plan *plan);
+
+/* Number of pixels of the plan's geometry. */
+long long owmi_sht_plan_npix(const owmi_sht_plan *plan);
+
+/*
+ * Synthesis of ntrans transforms of one spin group (spin 0: 1 component,
+ * spin 2: 2 components (E, B) -> (Q, U), HEALPix convention).
+ * alm: transform t, component c at alm + (t * alm_tstride + c * nelem);
+ * map: transform t, component c at map + (t * map_tstride + c * npix)
+ * (strides in elements of the complex / real type). weighted != 0
+ * multiplies by the ring weights (W Y).
+ * Replaces sht.alm2map (optweight/sht.py:95-178) for one spin group.
+ */
+int owmi_sht_synthesis(owmi_sht_plan *plan, int dtype, int spin,
+                       const void *alm, long long alm_tstride, void *map,
+                       long long map_tstride, long long ntrans, int weighted,
+                       void *stream);
+
+/*
+ * Adjoint synthesis (Y^T), or Y^T W when weighted != 0; same layouts.
+ * Replaces sht.map2alm (optweight/sht.py:10-93) for one spin group.
+ */
+int owmi_sht_adjoint(owmi_sht_plan *plan, int dtype, int spin,
+                     const void *map, long long map_tstride, void *alm,
+                     long long alm_tstride, long long ntrans, int weighted,
+                     void *stream);
+
+/*
+ * alm_out = Y^T D Y alm_in for one spin group, D a per-component pixel
+ * diagonal (pixdiag: component c of transform t at
+ * pixdiag + c * npix + t * pix_tstride; pix_tstride may be 0 to share D).
+ * The map never leaves on-chip memory: ring synthesis FFT, the pixel
+ * multiply and the ring analysis FFT run in one kernel.
+ * Fuses the map-space middle of operators.PixMatVecAlm.call
+ * (optweight/operators.py:614-643) and of CGWienerMap.a_matrix
+ * (solvers.py:217-242) when mask and inverse noise are pixel-diagonal.
+ */
+int owmi_sht_pixel_sandwich(owmi_sht_plan *plan, int dtype, int spin,
+                            const void *alm_in, long long alm_in_tstride,
+                            void *alm_out, long long alm_out_tstride,
+                            const void *pixdiag, long long pix_tstride,
+                            long long ntrans, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OWMI_H */

@@ -1,0 +1,104 @@
+// Internal structures shared by the SHT plan, the Legendre kernels and the
+// ring-FFT kernels.
+#pragma once
+
+#include "common.hpp"
+
+#include <vector>
+
+namespace owmi {
+
+// Exponent step of the scaled Legendre recurrence: a value is stored as
+// v * 2^(kScale * k), k <= 0. k == 0 is the plain value; k == -1 values are
+// used with the factor 2^-kScale; k <= -2 values are below 2^-2kScale of
+// the largest values and contribute nothing.
+template <typename T> struct ScaleStep;
+template <> struct ScaleStep<double> { static constexpr int bits = 128; };
+template <> struct ScaleStep<float> { static constexpr int bits = 32; };
+
+// Device tables of one precision.
+template <typename T>
+struct PlanTables {
+  T *x = nullptr;          // cos(theta_north) per pair [npairs]
+  T *seed0 = nullptr;      // spin-0 slam_mm value per (m, pair) [(mmax+1) * npairs]
+  int *seed0k = nullptr;   // its scale exponent
+  T *seedp = nullptr;      // spin +2 value at l = max(m, 2) per (m, pair)
+  int *seedpk = nullptr;
+  T *seedm = nullptr;      // spin -2 value at l = max(m, 2)
+  int *seedmk = nullptr;
+  T *coef0 = nullptr;      // spin-0 recurrence (a, c) per alm index [2 * nelem]
+  T *coef2 = nullptr;      // spin-2 recurrence (a, b, c) per alm index [3 * nelem]
+  T *twiddle = nullptr;    // exp(-2 pi i k / Mmax), k < Mmax/2, complex [Mmax]
+  T *chirp = nullptr;      // per distinct ring length: exp(-i pi n^2 / N), complex
+  T *bhat = nullptr;       // per distinct ring length: FFT_M(conj chirp, wrapped) / M
+  T *ringw = nullptr;      // ring weight per ring (rounded to T)
+  T *phi0 = nullptr;       // phi0 per ring
+  bool ready = false;
+};
+
+struct owmi_plan_impl {
+  int device = 0;
+  int lmax = 0, mmax = 0;
+  long long nrings = 0, npix = 0, npairs = 0;
+  int log2_mmax_fft = 0;  // log2 of the largest Bluestein length
+  int mmax_fft = 0;       // largest Bluestein length M
+
+  // host copies
+  std::vector<long long> h_nphi, h_offsets;
+  std::vector<double> h_theta, h_weight, h_phi0;
+  std::vector<int> h_pair_n, h_pair_s;
+
+  // device geometry (precision independent)
+  int *pair_n = nullptr;         // north ring of each pair
+  int *pair_s = nullptr;         // south ring or -1
+  long long *nphi = nullptr;     // per ring
+  long long *offsets = nullptr;  // per ring
+  int *ring_logm = nullptr;      // log2 of the Bluestein length per ring
+  long long *ring_chirp_off = nullptr;  // offset (complex elements) into chirp table
+  long long *ring_bhat_off = nullptr;   // offset (complex elements) into bhat table
+  // FFT jobs: pairs of rings with equal nphi transformed together
+  long long nfft_jobs = 0;
+  int *fft_ring_a = nullptr;
+  int *fft_ring_b = nullptr;  // -1 when the job holds one ring
+
+  PlanTables<double> t64;
+  PlanTables<float> t32;
+
+  // workspace for ring Fourier coefficients (grows)
+  void *work = nullptr;
+  size_t work_bytes = 0;
+  void *ensure_work(size_t bytes);
+
+  size_t nchirp = 0, nbhat = 0;  // complex element counts of the tables
+};
+
+// Launchers (legendre.hip)
+template <typename T>
+void launch_synthesis(owmi_plan_impl *p, int spin, const cplx_t<T> *alm, long long alm_tstride,
+                      cplx_t<T> *G, long long ntrans, hipStream_t st);
+template <typename T>
+void launch_adjoint(owmi_plan_impl *p, int spin, const cplx_t<T> *G, cplx_t<T> *alm,
+                    long long alm_tstride, long long ntrans, hipStream_t st);
+
+// Launchers (ringfft.hip). G rows are (transform * ncomp + comp) * nrings + ring,
+// each of mmax + 1 complex coefficients.
+template <typename T>
+void launch_ring_synth(owmi_plan_impl *p, const cplx_t<T> *G, T *map, long long map_tstride,
+                       int ncomp, long long ntrans, int weighted, hipStream_t st);
+template <typename T>
+void launch_ring_adjoint(owmi_plan_impl *p, const T *map, long long map_tstride, cplx_t<T> *G,
+                         int ncomp, long long ntrans, int weighted, hipStream_t st);
+template <typename T>
+void launch_ring_sandwich(owmi_plan_impl *p, const cplx_t<T> *G, cplx_t<T> *Gout,
+                          const T *diag, long long diag_tstride, int ncomp, long long ntrans,
+                          hipStream_t st);
+// Fill bhat for every distinct ring length from the chirp table (plan setup).
+void launch_bluestein_init(owmi_plan_impl *p, const std::vector<long long> &lens,
+                           const std::vector<int> &logms, const std::vector<long long> &chirp_off,
+                           const std::vector<long long> &bhat_off);
+
+}  // namespace owmi
+
+struct owmi_sht_plan {
+  owmi::owmi_plan_impl impl;
+};

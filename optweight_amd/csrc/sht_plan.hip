@@ -1,0 +1,551 @@
+// SHT plan: geometry analysis, Legendre seed/coefficient tables and
+// Bluestein ring-FFT tables, built on the host in long double and uploaded
+// once; plus the C-ABI transform entry points that chain the Legendre and
+// ring-FFT kernels.
+#include "sht_internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <numeric>
+
+namespace owmi {
+
+namespace {
+
+using LD = long double;
+const LD kPi = 3.141592653589793238462643383279502884L;
+
+// A long double with a separate power-of-two exponent: value = mant * 2^exp,
+// 0.5 <= |mant| < 1 (or mant == 0). Used for the Legendre seeds, which can
+// lie far below the long double range at high lmax.
+struct XLD {
+  LD mant = 0;
+  long exp = 0;
+  static XLD of(LD v) {
+    XLD r;
+    int e = 0;
+    r.mant = frexpl(v, &e);
+    r.exp = e;
+    return r;
+  }
+  XLD mul(LD f) const {
+    XLD r;
+    int e = 0;
+    r.mant = frexpl(mant * f, &e);
+    r.exp = exp + e;
+    if (r.mant == 0) r.exp = 0;
+    return r;
+  }
+  // log2 |value| (-inf for zero)
+  LD log2abs() const { return mant == 0 ? -INFINITY : log2l(fabsl(mant)) + (LD)exp; }
+};
+
+// Split a seed into (v, k) with value = v * 2^(S k): k = 0 when
+// |value| >= 2^-S, else k = floor(log2|value| / S) and v in [1, 2^S).
+template <typename T>
+void split_seed(const XLD &s, T &v, int &k) {
+  const int S = ScaleStep<T>::bits;
+  if (s.mant == 0) {
+    v = 0;
+    k = -1000;
+    return;
+  }
+  LD l2 = s.log2abs();
+  if (l2 >= -(LD)S) {
+    k = 0;
+    v = (T)ldexpl(s.mant, (int)s.exp);
+    return;
+  }
+  long kk = (long)floorl(l2 / S);
+  if (kk < -1000) kk = -1000;
+  k = (int)kk;
+  v = (T)ldexpl(s.mant, (int)(s.exp - (long)S * kk));
+}
+
+template <typename T>
+T *upload(const std::vector<T> &h) {
+  T *d = nullptr;
+  if (h.empty()) return nullptr;
+  OWMI_HIP(hipMalloc(&d, h.size() * sizeof(T)));
+  OWMI_HIP(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  return d;
+}
+
+template <typename T>
+void free_tables(PlanTables<T> &t) {
+  T **ptrs[] = {&t.x, &t.seed0, &t.seedp, &t.seedm, &t.coef0, &t.coef2, &t.twiddle,
+                &t.chirp, &t.bhat, &t.ringw, &t.phi0};
+  for (T **p : ptrs)
+    if (*p) {
+      (void)hipFree(*p);
+      *p = nullptr;
+    }
+  int **iptrs[] = {&t.seed0k, &t.seedpk, &t.seedmk};
+  for (int **p : iptrs)
+    if (*p) {
+      (void)hipFree(*p);
+      *p = nullptr;
+    }
+  t.ready = false;
+}
+
+int ilog2_ceil(long long n) {
+  int k = 0;
+  while ((1LL << k) < n) ++k;
+  return k;
+}
+
+}  // namespace
+
+void *owmi_plan_impl::ensure_work(size_t bytes) {
+  if (bytes <= work_bytes) return work;
+  if (work) OWMI_HIP(hipFree(work));
+  work = nullptr;
+  work_bytes = 0;
+  OWMI_HIP(hipMalloc(&work, bytes));
+  work_bytes = bytes;
+  return work;
+}
+
+// Host-side table construction for one precision.
+template <typename T>
+static void build_tables(owmi_plan_impl &p, PlanTables<T> &t,
+                         const std::vector<LD> &xs, const std::vector<LD> &th_n,
+                         const std::vector<LD> &chirp, const std::vector<long long> &lens,
+                         const std::vector<int> &logms,
+                         const std::vector<long long> &chirp_off,
+                         const std::vector<long long> &bhat_off) {
+  const int lmax = p.lmax, mmax = p.mmax;
+  const long long np = p.npairs;
+  const long long nelem = nelem_of(lmax);
+
+  std::vector<T> hx(np);
+  for (long long i = 0; i < np; ++i) hx[i] = (T)xs[i];
+
+  // Seeds.
+  std::vector<T> s0((size_t)(mmax + 1) * np), sp((size_t)(mmax + 1) * np),
+      sm((size_t)(mmax + 1) * np);
+  std::vector<int> s0k(s0.size()), spk(s0.size()), smk(s0.size());
+  const LD inv_sqrt_4pi = 1.0L / sqrtl(4.0L * kPi);
+  const LD sqrt5_4pi = sqrtl(5.0L / (4.0L * kPi));
+  for (long long i = 0; i < np; ++i) {
+    LD th = th_n[i];
+    LD st = sinl(th);
+    LD ch = cosl(th / 2), sh = sinl(th / 2);
+    // spin 0: slam_mm = -sqrt((2m+1)/(2m)) sin(theta) slam_{m-1,m-1}
+    XLD a = XLD::of(inv_sqrt_4pi);
+    for (int m = 0; m <= mmax; ++m) {
+      if (m > 0) a = a.mul(-sqrtl((LD)(2 * m + 1) / (LD)(2 * m)) * st);
+      split_seed<T>(a, s0[(size_t)m * np + i], s0k[(size_t)m * np + i]);
+    }
+    // spin -2 at l = m >= 2: slam_22 = sqrt(5/4pi) cos^4(theta/2),
+    // slam_mm = -sqrt((2m+1) 2m / ((m+2)(m-2))) (sin(theta)/2) slam_{m-1,m-1};
+    // spin +2 = spin -2 * tan^4(theta/2).
+    LD t4 = powl(sh / ch, 4);
+    for (int m = 0; m <= mmax; ++m) {
+      XLD vp, vm;
+      if (m == 0) {
+        LD v = sqrt5_4pi * sqrtl(6.0L) / 4.0L * st * st;
+        vp = XLD::of(v);
+        vm = XLD::of(v);
+      } else if (m == 1) {
+        vp = XLD::of(-sqrt5_4pi * sh * sh * st);
+        vm = XLD::of(sqrt5_4pi * ch * ch * st);
+      } else {
+        if (m == 2) {
+          a = XLD::of(sqrt5_4pi * powl(ch, 4));
+        } else {
+          a = a.mul(-sqrtl((LD)(2 * m + 1) * (LD)(2 * m) / ((LD)(m + 2) * (LD)(m - 2))) * st /
+                    2.0L);
+        }
+        vm = a;
+        vp = a.mul(t4);
+      }
+      split_seed<T>(vp, sp[(size_t)m * np + i], spk[(size_t)m * np + i]);
+      split_seed<T>(vm, sm[(size_t)m * np + i], smk[(size_t)m * np + i]);
+    }
+  }
+
+  // Recurrence coefficients: slam_l = (A x + B) slam_{l-1} - C slam_{l-2},
+  // (A, B, C) = (alpha, beta, gamma)_{l-1} of the d^l_{m,m'} recurrence.
+  std::vector<T> c0(2 * (size_t)nelem, 0), c2(3 * (size_t)nelem, 0);
+  for (int m = 0; m <= mmax; ++m) {
+    long long ms = mstart_of(m, lmax);
+    LD m2 = (LD)m * m;
+    for (int s = 0; s <= 2; s += 2) {
+      int lstart = std::max(m, s);
+      LD mp2 = (LD)s * s;
+      for (int l = lstart + 1; l <= lmax; ++l) {
+        LD L = l - 1;
+        LD r_L = sqrtl((L * L - m2) * (L * L - mp2));
+        LD r_L1 = sqrtl(((L + 1) * (L + 1) - m2) * ((L + 1) * (L + 1) - mp2));
+        LD alpha = sqrtl((2 * L + 3) * (2 * L + 1)) * (L + 1) / r_L1;
+        LD gamma = (l - 1 == lstart || l - 1 == 0)
+                       ? 0.0L
+                       : sqrtl((2 * L + 3) / (2 * L - 1)) * (L + 1) * r_L / (L * r_L1);
+        size_t idx = (size_t)(ms + l);
+        if (s == 0) {
+          c0[2 * idx] = (T)alpha;
+          c0[2 * idx + 1] = (T)gamma;
+        } else {
+          // s = +2 -> m' = -2: beta = +2 m sqrt((2L+3)(2L+1)) / (L r_{L+1})
+          LD beta = (m == 0) ? 0.0L : 2.0L * m * sqrtl((2 * L + 3) * (2 * L + 1)) / (L * r_L1);
+          c2[3 * idx] = (T)alpha;
+          c2[3 * idx + 1] = (T)beta;
+          c2[3 * idx + 2] = (T)gamma;
+        }
+      }
+    }
+  }
+
+  // Twiddles for the largest FFT length.
+  std::vector<T> tw(2 * (size_t)(p.mmax_fft / 2 + 1));
+  for (long long k = 0; k <= p.mmax_fft / 2; ++k) {
+    LD ang = -2.0L * kPi * (LD)k / (LD)p.mmax_fft;
+    tw[2 * k] = (T)cosl(ang);
+    tw[2 * k + 1] = (T)sinl(ang);
+  }
+  std::vector<T> chirpT(chirp.size());
+  for (size_t i = 0; i < chirp.size(); ++i) chirpT[i] = (T)chirp[i];
+  std::vector<T> rw(p.nrings), ph(p.nrings);
+  for (long long r = 0; r < p.nrings; ++r) {
+    rw[r] = (T)p.h_weight[r];
+    ph[r] = (T)p.h_phi0[r];
+  }
+
+  t.x = upload(hx);
+  t.seed0 = upload(s0);
+  t.seed0k = upload(s0k);
+  t.seedp = upload(sp);
+  t.seedpk = upload(spk);
+  t.seedm = upload(sm);
+  t.seedmk = upload(smk);
+  t.coef0 = upload(c0);
+  t.coef2 = upload(c2);
+  t.twiddle = upload(tw);
+  t.chirp = upload(chirpT);
+  t.ringw = upload(rw);
+  t.phi0 = upload(ph);
+  OWMI_HIP(hipMalloc(&t.bhat, std::max<size_t>(1, p.nbhat) * 2 * sizeof(T)));
+  t.ready = true;
+}
+
+}  // namespace owmi
+
+using namespace owmi;
+
+template <typename T>
+static PlanTables<T> &tables(owmi_plan_impl &p);
+template <>
+PlanTables<double> &tables<double>(owmi_plan_impl &p) { return p.t64; }
+template <>
+PlanTables<float> &tables<float>(owmi_plan_impl &p) { return p.t32; }
+
+// Bluestein kernel spectra are computed once in double on the device; the
+// float tables are converted from them.
+__global__ void bhat_to_float(const double *src, float *dst, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = (float)src[i];
+}
+
+extern "C" {
+
+int owmi_sht_plan_create(owmi_sht_plan **out, long long nrings, const double *theta,
+                         const double *weight, const long long *nphi, const long long *offsets,
+                         const double *phi0, int lmax) {
+  return guard([&] {
+    OWMI_REQUIRE(out != nullptr, "plan pointer is NULL");
+    OWMI_REQUIRE(nrings >= 1, "need at least one ring");
+    OWMI_REQUIRE(lmax >= 0, "lmax must be >= 0");
+    *out = nullptr;
+    auto *plan = new owmi_sht_plan();
+    owmi_plan_impl &p = plan->impl;
+    try {
+      OWMI_HIP(hipGetDevice(&p.device));
+      p.lmax = lmax;
+      p.mmax = lmax;
+      p.nrings = nrings;
+      p.h_theta.assign(theta, theta + nrings);
+      p.h_weight.assign(weight, weight + nrings);
+      p.h_nphi.assign(nphi, nphi + nrings);
+      p.h_offsets.assign(offsets, offsets + nrings);
+      p.h_phi0.assign(phi0, phi0 + nrings);
+      p.npix = 0;
+      for (long long r = 0; r < nrings; ++r) {
+        OWMI_REQUIRE(nphi[r] >= 1, "nphi must be >= 1");
+        p.npix += nphi[r];
+      }
+
+      // Mirror pairs: rings visited in ascending theta; a ring with
+      // theta < pi/2 takes the lowest-index unused ring with
+      // |theta_i + theta_j - pi| < 1e-12 as its south partner.
+      std::vector<long long> order(nrings);
+      std::iota(order.begin(), order.end(), 0);
+      std::stable_sort(order.begin(), order.end(),
+                       [&](long long a, long long b) { return theta[a] < theta[b]; });
+      std::vector<char> used(nrings, 0);
+      const double pi = 3.141592653589793;
+      for (long long oi = 0; oi < nrings; ++oi) {
+        long long i = order[oi];
+        if (used[i]) continue;
+        used[i] = 1;
+        int j = -1;
+        if (theta[i] < pi / 2) {
+          // candidates lie near pi - theta_i in the sorted order
+          double target = pi - theta[i];
+          auto it = std::lower_bound(order.begin(), order.end(), target - 1e-11,
+                                     [&](long long a, double v) { return theta[a] < v; });
+          long long best = -1;
+          for (; it != order.end() && theta[*it] <= target + 1e-11; ++it) {
+            long long c = *it;
+            if (!used[c] && std::fabs(theta[c] + theta[i] - pi) < 1e-12)
+              if (best < 0 || c < best) best = c;
+          }
+          if (best >= 0) {
+            j = (int)best;
+            used[best] = 1;
+          }
+        }
+        p.h_pair_n.push_back((int)i);
+        p.h_pair_s.push_back(j);
+      }
+      p.npairs = (long long)p.h_pair_n.size();
+
+      std::vector<LD> xs(p.npairs), thn(p.npairs);
+      for (long long k = 0; k < p.npairs; ++k) {
+        LD th = (LD)theta[p.h_pair_n[k]];
+        thn[k] = th;
+        xs[k] = (LD)(double)cosl(th);  // x rounded to double, shared by the pair
+      }
+
+      // Ring FFT tables: one Bluestein length per distinct nphi.
+      std::map<long long, int> len_index;
+      std::vector<long long> lens, chirp_off, bhat_off;
+      std::vector<int> logms;
+      std::vector<LD> chirp;
+      size_t nbhat = 0;
+      std::vector<int> ring_logm(nrings);
+      std::vector<long long> ring_coff(nrings), ring_boff(nrings);
+      p.mmax_fft = 2;
+      for (long long r = 0; r < nrings; ++r) {
+        long long n = nphi[r];
+        auto it = len_index.find(n);
+        int li;
+        if (it == len_index.end()) {
+          li = (int)lens.size();
+          len_index[n] = li;
+          lens.push_back(n);
+          int lm = ilog2_ceil(2 * n - 1);
+          if (lm < 1) lm = 1;
+          logms.push_back(lm);
+          chirp_off.push_back((long long)chirp.size() / 2);
+          bhat_off.push_back((long long)nbhat);
+          nbhat += (size_t)1 << lm;
+          for (long long k = 0; k < n; ++k) {
+            long long q = (k * k) % (2 * n);
+            LD ang = -kPi * (LD)q / (LD)n;
+            chirp.push_back(cosl(ang));
+            chirp.push_back(sinl(ang));
+          }
+          p.mmax_fft = std::max(p.mmax_fft, 1 << lm);
+        } else {
+          li = it->second;
+        }
+        ring_logm[r] = logms[li];
+        ring_coff[r] = chirp_off[li];
+        ring_boff[r] = bhat_off[li];
+      }
+      p.nchirp = chirp.size() / 2;
+      p.nbhat = nbhat;
+      p.log2_mmax_fft = ilog2_ceil(p.mmax_fft);
+
+      // FFT jobs: mirror pairs share nphi on symmetric grids; other rings
+      // are paired with the next ring of equal nphi, leftovers run alone.
+      std::vector<int> ja, jb;
+      std::vector<char> done(nrings, 0);
+      for (long long k = 0; k < p.npairs; ++k) {
+        int a = p.h_pair_n[k], b = p.h_pair_s[k];
+        if (b >= 0 && nphi[a] == nphi[b]) {
+          ja.push_back(a);
+          jb.push_back(b);
+          done[a] = done[b] = 1;
+        }
+      }
+      std::map<long long, int> pending;
+      for (long long r = 0; r < nrings; ++r) {
+        if (done[r]) continue;
+        auto it = pending.find(nphi[r]);
+        if (it == pending.end()) {
+          pending[nphi[r]] = (int)r;
+        } else {
+          ja.push_back(it->second);
+          jb.push_back((int)r);
+          pending.erase(it);
+        }
+      }
+      for (auto &kv : pending) {
+        ja.push_back(kv.second);
+        jb.push_back(-1);
+      }
+      p.nfft_jobs = (long long)ja.size();
+
+      p.pair_n = upload(p.h_pair_n);
+      p.pair_s = upload(p.h_pair_s);
+      p.nphi = upload(p.h_nphi);
+      p.offsets = upload(p.h_offsets);
+      p.ring_logm = upload(ring_logm);
+      p.ring_chirp_off = upload(ring_coff);
+      p.ring_bhat_off = upload(ring_boff);
+      p.fft_ring_a = upload(ja);
+      p.fft_ring_b = upload(jb);
+
+      build_tables<double>(p, p.t64, xs, thn, chirp, lens, logms, chirp_off, bhat_off);
+      launch_bluestein_init(&p, lens, logms, chirp_off, bhat_off);
+      OWMI_HIP(hipDeviceSynchronize());
+      build_tables<float>(p, p.t32, xs, thn, chirp, lens, logms, chirp_off, bhat_off);
+      long long nb = 2 * (long long)p.nbhat;
+      hipLaunchKernelGGL(bhat_to_float, dim3(ceil_div(nb, 256)), dim3(256), 0, 0,
+                         p.t64.bhat, p.t32.bhat, nb);
+      OWMI_HIP(hipDeviceSynchronize());
+    } catch (...) {
+      owmi_sht_plan_destroy(plan);
+      throw;
+    }
+    *out = plan;
+  });
+}
+
+int owmi_sht_plan_destroy(owmi_sht_plan *plan) {
+  if (!plan) return 0;
+  owmi_plan_impl &p = plan->impl;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipSetDevice(p.device);
+  free_tables(p.t64);
+  free_tables(p.t32);
+  void *ptrs[] = {p.pair_n, p.pair_s, p.nphi, p.offsets, p.ring_logm, p.ring_chirp_off,
+                  p.ring_bhat_off, p.fft_ring_a, p.fft_ring_b, p.work};
+  for (void *q : ptrs)
+    if (q) (void)hipFree(q);
+  (void)hipSetDevice(dev);
+  delete plan;
+  return 0;
+}
+
+long long owmi_sht_plan_npix(const owmi_sht_plan *plan) { return plan ? plan->impl.npix : -1; }
+
+}  // extern "C"
+
+template <typename T>
+static void check_fft_fits(owmi_plan_impl &p) {
+  size_t need = (size_t)p.mmax_fft * sizeof(cplx_t<T>);
+  OWMI_REQUIRE(need <= 128 * 1024,
+               "ring length too large for the on-chip FFT at this precision "
+               "(Bluestein length " + std::to_string(p.mmax_fft) + ")");
+}
+
+template <typename T>
+static int do_synthesis(owmi_sht_plan *plan, int spin, const void *alm, long long alm_ts,
+                        void *map, long long map_ts, long long ntrans, int weighted,
+                        void *stream) {
+  return guard([&] {
+    OWMI_REQUIRE(plan != nullptr, "plan is NULL");
+    owmi_plan_impl &p = plan->impl;
+    OWMI_REQUIRE(spin == 0 || spin == 2, "spin must be 0 or 2");
+    OWMI_REQUIRE(spin <= p.lmax, "Spin exceeds lmax");
+    check_fft_fits<T>(p);
+    if (ntrans == 0) return;
+    int nc = spin == 0 ? 1 : 2;
+    size_t rows = (size_t)ntrans * nc * p.nrings;
+    auto *G = static_cast<cplx_t<T> *>(p.ensure_work(rows * (p.mmax + 1) * sizeof(cplx_t<T>)));
+    hipStream_t st = as_stream(stream);
+    launch_synthesis<T>(&p, spin, static_cast<const cplx_t<T> *>(alm), alm_ts, G, ntrans, st);
+    launch_ring_synth<T>(&p, G, static_cast<T *>(map), map_ts, nc, ntrans, weighted, st);
+  });
+}
+
+template <typename T>
+static int do_adjoint(owmi_sht_plan *plan, int spin, const void *map, long long map_ts,
+                      void *alm, long long alm_ts, long long ntrans, int weighted,
+                      void *stream) {
+  return guard([&] {
+    OWMI_REQUIRE(plan != nullptr, "plan is NULL");
+    owmi_plan_impl &p = plan->impl;
+    OWMI_REQUIRE(spin == 0 || spin == 2, "spin must be 0 or 2");
+    OWMI_REQUIRE(spin <= p.lmax, "Spin exceeds lmax");
+    check_fft_fits<T>(p);
+    if (ntrans == 0) return;
+    int nc = spin == 0 ? 1 : 2;
+    size_t rows = (size_t)ntrans * nc * p.nrings;
+    auto *G = static_cast<cplx_t<T> *>(p.ensure_work(rows * (p.mmax + 1) * sizeof(cplx_t<T>)));
+    hipStream_t st = as_stream(stream);
+    launch_ring_adjoint<T>(&p, static_cast<const T *>(map), map_ts, G, nc, ntrans, weighted, st);
+    launch_adjoint<T>(&p, spin, G, static_cast<cplx_t<T> *>(alm), alm_ts, ntrans, st);
+  });
+}
+
+template <typename T>
+static int do_sandwich(owmi_sht_plan *plan, int spin, const void *alm_in, long long in_ts,
+                       void *alm_out, long long out_ts, const void *diag, long long diag_ts,
+                       long long ntrans, void *stream) {
+  return guard([&] {
+    OWMI_REQUIRE(plan != nullptr, "plan is NULL");
+    owmi_plan_impl &p = plan->impl;
+    OWMI_REQUIRE(spin == 0 || spin == 2, "spin must be 0 or 2");
+    OWMI_REQUIRE(spin <= p.lmax, "Spin exceeds lmax");
+    check_fft_fits<T>(p);
+    if (ntrans == 0) return;
+    int nc = spin == 0 ? 1 : 2;
+    size_t rows = (size_t)ntrans * nc * p.nrings;
+    auto *G = static_cast<cplx_t<T> *>(p.ensure_work(rows * (p.mmax + 1) * sizeof(cplx_t<T>)));
+    hipStream_t st = as_stream(stream);
+    launch_synthesis<T>(&p, spin, static_cast<const cplx_t<T> *>(alm_in), in_ts, G, ntrans, st);
+    launch_ring_sandwich<T>(&p, G, G, static_cast<const T *>(diag), diag_ts, nc, ntrans, st);
+    launch_adjoint<T>(&p, spin, G, static_cast<cplx_t<T> *>(alm_out), out_ts, ntrans, st);
+  });
+}
+
+extern "C" {
+
+int owmi_sht_synthesis(owmi_sht_plan *plan, int dtype, int spin, const void *alm,
+                       long long alm_tstride, void *map, long long map_tstride, long long ntrans,
+                       int weighted, void *stream) {
+  if (dtype == OWMI_F64)
+    return do_synthesis<double>(plan, spin, alm, alm_tstride, map, map_tstride, ntrans, weighted,
+                                stream);
+  if (dtype == OWMI_F32)
+    return do_synthesis<float>(plan, spin, alm, alm_tstride, map, map_tstride, ntrans, weighted,
+                               stream);
+  set_error("dtype must be OWMI_F32 or OWMI_F64");
+  return 1;
+}
+
+int owmi_sht_adjoint(owmi_sht_plan *plan, int dtype, int spin, const void *map,
+                     long long map_tstride, void *alm, long long alm_tstride, long long ntrans,
+                     int weighted, void *stream) {
+  if (dtype == OWMI_F64)
+    return do_adjoint<double>(plan, spin, map, map_tstride, alm, alm_tstride, ntrans, weighted,
+                              stream);
+  if (dtype == OWMI_F32)
+    return do_adjoint<float>(plan, spin, map, map_tstride, alm, alm_tstride, ntrans, weighted,
+                             stream);
+  set_error("dtype must be OWMI_F32 or OWMI_F64");
+  return 1;
+}
+
+int owmi_sht_pixel_sandwich(owmi_sht_plan *plan, int dtype, int spin, const void *alm_in,
+                            long long alm_in_tstride, void *alm_out, long long alm_out_tstride,
+                            const void *pixdiag, long long pix_tstride, long long ntrans,
+                            void *stream) {
+  if (dtype == OWMI_F64)
+    return do_sandwich<double>(plan, spin, alm_in, alm_in_tstride, alm_out, alm_out_tstride,
+                               pixdiag, pix_tstride, ntrans, stream);
+  if (dtype == OWMI_F32)
+    return do_sandwich<float>(plan, spin, alm_in, alm_in_tstride, alm_out, alm_out_tstride,
+                              pixdiag, pix_tstride, ntrans, stream);
+  set_error("dtype must be OWMI_F32 or OWMI_F64");
+  return 1;
+}
+
+}  // extern "C"

@@ -1,0 +1,481 @@
+// Streaming (HBM-bound) kernels: per-l multiply, truncation, wavelet
+// accumulate, ring weights, pixel matvec, real-field alm dot product and the
+// CG vector updates. One thread per complex element (or pixel), grid-stride
+// over a fixed grid where a deterministic reduction order matters.
+#include "common.hpp"
+
+#include <map>
+#include <mutex>
+#include <vector>
+
+namespace owmi {
+
+static thread_local std::string g_last_error;
+void set_error(const std::string &msg) { g_last_error = msg; }
+void clear_error() { g_last_error.clear(); }
+
+// (l, m) of flat index idx in a triangular lmax layout:
+// idx = m (2 lmax + 1 - m) / 2 + l with m <= l <= lmax.
+__device__ __forceinline__ void lm_of(long long idx, int lmax, int &l, int &m) {
+  // m is the largest integer with m (2 lmax + 3 - m) / 2 <= idx.
+  double b = 2.0 * lmax + 3.0;
+  double disc = b * b - 8.0 * (double)idx;
+  int mm = (int)((b - sqrt(disc)) * 0.5);
+  if (mm < 0) mm = 0;
+  if (mm > lmax) mm = lmax;
+  while (mm > 0 && mstart_of(mm, lmax) + mm > idx) --mm;
+  while (mm < lmax && mstart_of(mm + 1, lmax) + mm + 1 <= idx) ++mm;
+  m = mm;
+  l = (int)(idx - mstart_of(mm, lmax));
+}
+
+// ---------------------------------------------------------------- lmul
+
+template <typename T, int NC>
+__global__ void lmul_full_kernel(const T *__restrict__ lmat, const cplx_t<T> *in,
+                                 cplx_t<T> *out, int lmax, long long nelem,
+                                 long long ntrans) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long t = blockIdx.y;
+  if (i >= nelem || t >= ntrans) return;
+  int l, m;
+  lm_of(i, lmax, l, m);
+  const int nell = lmax + 1;
+  const cplx_t<T> *src = in + t * NC * nelem + i;
+  cplx_t<T> v[NC];
+#pragma unroll
+  for (int j = 0; j < NC; ++j) v[j] = src[j * nelem];
+  cplx_t<T> *dst = out + t * NC * nelem + i;
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    T re = 0, im = 0;
+#pragma unroll
+    for (int j = 0; j < NC; ++j) {
+      T a = lmat[(c * NC + j) * nell + l];
+      re += a * v[j].x;
+      im += a * v[j].y;
+    }
+    dst[c * nelem] = {re, im};
+  }
+}
+
+template <typename T>
+__global__ void lmul_full_generic_kernel(const T *__restrict__ lmat, const cplx_t<T> *in,
+                                         cplx_t<T> *out, int lmax, int nc,
+                                         long long nelem, long long ntrans) {
+  // Used for ncomp > 4; in-place safe because each thread reads all
+  // components of its element before writing any.
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long t = blockIdx.y;
+  if (i >= nelem || t >= ntrans) return;
+  int l, m;
+  lm_of(i, lmax, l, m);
+  const int nell = lmax + 1;
+  const cplx_t<T> *src = in + t * nc * nelem + i;
+  cplx_t<T> *dst = out + t * nc * nelem + i;
+  cplx_t<T> v[16];
+  for (int j = 0; j < nc; ++j) v[j] = src[j * nelem];
+  for (int c = 0; c < nc; ++c) {
+    T re = 0, im = 0;
+    for (int j = 0; j < nc; ++j) {
+      T a = lmat[(c * nc + j) * nell + l];
+      re += a * v[j].x;
+      im += a * v[j].y;
+    }
+    dst[c * nelem] = {re, im};
+  }
+}
+
+template <typename T>
+__global__ void lmul_diag_kernel(const T *__restrict__ lmat, const cplx_t<T> *in,
+                                 cplx_t<T> *out, int lmax, int nc, long long nelem,
+                                 long long nrows) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long row = blockIdx.y;
+  if (i >= nelem || row >= nrows) return;
+  int l, m;
+  lm_of(i, lmax, l, m);
+  int c = (int)(row % nc);
+  T a = lmat[c * (lmax + 1) + l];
+  cplx_t<T> v = in[row * nelem + i];
+  out[row * nelem + i] = {a * v.x, a * v.y};
+}
+
+template <typename T>
+static void lmul_impl(const void *lmat, int diag, const void *in, void *out, int lmax,
+                      int ncomp, long long ntrans, hipStream_t st) {
+  long long nelem = nelem_of(lmax);
+  dim3 block(256);
+  const T *L = static_cast<const T *>(lmat);
+  auto *I = static_cast<const cplx_t<T> *>(in);
+  auto *O = static_cast<cplx_t<T> *>(out);
+  if (diag) {
+    long long nrows = ntrans * ncomp;
+    OWMI_REQUIRE(nrows < 65536, "too many alm rows for one lmul launch");
+    dim3 grid(ceil_div(nelem, 256), (unsigned)nrows);
+    hipLaunchKernelGGL(lmul_diag_kernel<T>, grid, block, 0, st, L, I, O, lmax, ncomp, nelem, nrows);
+    return;
+  }
+  OWMI_REQUIRE(ntrans < 65536, "too many transforms for one lmul launch");
+  dim3 grid(ceil_div(nelem, 256), (unsigned)ntrans);
+  switch (ncomp) {
+    case 1: hipLaunchKernelGGL((lmul_full_kernel<T, 1>), grid, block, 0, st, L, I, O, lmax, nelem, ntrans); break;
+    case 2: hipLaunchKernelGGL((lmul_full_kernel<T, 2>), grid, block, 0, st, L, I, O, lmax, nelem, ntrans); break;
+    case 3: hipLaunchKernelGGL((lmul_full_kernel<T, 3>), grid, block, 0, st, L, I, O, lmax, nelem, ntrans); break;
+    case 4: hipLaunchKernelGGL((lmul_full_kernel<T, 4>), grid, block, 0, st, L, I, O, lmax, nelem, ntrans); break;
+    default:
+      OWMI_REQUIRE(ncomp <= 16, "ncomp > 16 not supported");
+      hipLaunchKernelGGL(lmul_full_generic_kernel<T>, grid, block, 0, st, L, I, O, lmax, ncomp, nelem, ntrans);
+  }
+}
+
+// ------------------------------------------------------ trunc / wlm2alm
+
+template <typename T>
+__global__ void trunc_kernel(const cplx_t<T> *alm, cplx_t<T> *out, int lmax, int lmax_out,
+                             long long nelem_in, long long nelem_out) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long row = blockIdx.y;
+  if (i >= nelem_out) return;
+  int l, m;
+  lm_of(i, lmax_out, l, m);
+  out[row * nelem_out + i] = alm[row * nelem_in + mstart_of(m, lmax) + l];
+}
+
+template <typename T>
+__global__ void wlm2alm_kernel(const T *__restrict__ w, const cplx_t<T> *wlm, cplx_t<T> *alm,
+                               int lmax_w, int lmax_a, long long nelem_w, long long nelem_a) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long row = blockIdx.y;
+  if (i >= nelem_w) return;
+  int l, m;
+  lm_of(i, lmax_w, l, m);
+  cplx_t<T> v = wlm[row * nelem_w + i];
+  T a = w[l];
+  cplx_t<T> *dst = alm + row * nelem_a + mstart_of(m, lmax_a) + l;
+  cplx_t<T> o = *dst;
+  *dst = {o.x + a * v.x, o.y + a * v.y};
+}
+
+// ---------------------------------------------------------- ring weight
+
+template <typename T>
+__global__ void ringweight_kernel(T *imap, const double *__restrict__ weight,
+                                  const long long *__restrict__ nphi,
+                                  const long long *__restrict__ offsets,
+                                  const long long *__restrict__ stride, long long map_stride) {
+  long long r = blockIdx.x;
+  long long k = blockIdx.y;
+  T w = (T)weight[r];
+  long long n = nphi[r], off = offsets[r], s = stride[r];
+  T *base = imap + k * map_stride + off;
+  for (long long j = threadIdx.x; j < n; j += blockDim.x) base[j * s] *= w;
+}
+
+// ---------------------------------------------------------- pix matvec
+
+template <typename T>
+__global__ void pix_diag_kernel(const T *__restrict__ mat, const T *in, T *out, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = mat[i] * in[i];
+}
+
+template <typename T>
+__global__ void pix_full_kernel(const T *__restrict__ mat, const T *in, T *out, int nc,
+                                long long npix) {
+  long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npix) return;
+  T v[16];
+  for (int j = 0; j < nc; ++j) v[j] = in[j * npix + p];
+  for (int i = 0; i < nc; ++i) {
+    T acc = 0;
+    for (int j = 0; j < nc; ++j) acc += mat[(i * nc + j) * npix + p] * v[j];
+    out[i * npix + p] = acc;
+  }
+}
+
+// --------------------------------------------------------------- dot
+
+constexpr int kDotBlocks = 1024;
+constexpr int kDotThreads = 256;
+
+template <typename T>
+__global__ void __launch_bounds__(kDotThreads)
+dot_partial_kernel(const cplx_t<T> *a, const cplx_t<T> *b, int lmax, long long nelem,
+                   long long nrows, double *partial) {
+  __shared__ double red[kDotThreads];
+  double acc = 0.0;
+  long long total = nelem * nrows;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (long long)gridDim.x * blockDim.x) {
+    long long j = i % nelem;
+    cplx_t<T> x = a[i], y = b[i];
+    double v = (double)x.x * (double)y.x + (double)x.y * (double)y.y;
+    acc += (j <= lmax) ? v : 2.0 * v;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = kDotThreads / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ void __launch_bounds__(kDotThreads)
+dot_final_kernel(const double *partial, int n, double *out) {
+  __shared__ double red[kDotThreads];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) acc += partial[i];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = kDotThreads / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+
+// Per-device scratch for the two-pass reductions (kDotBlocks doubles).
+static double *dot_scratch() {
+  static std::mutex mu;
+  static std::map<int, double *> bufs;
+  int dev = 0;
+  OWMI_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = bufs.find(dev);
+  if (it != bufs.end()) return it->second;
+  double *p = nullptr;
+  OWMI_HIP(hipMalloc(&p, kDotBlocks * sizeof(double)));
+  bufs[dev] = p;
+  return p;
+}
+
+// ------------------------------------------------------------ CG updates
+
+template <typename T>
+__global__ void cg_xr_kernel(cplx_t<T> *x, cplx_t<T> *r, const cplx_t<T> *p,
+                             const cplx_t<T> *q, const double *num, const double *den,
+                             long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  T alpha = (T)(*num / *den);
+  cplx_t<T> pv = p[i], qv = q[i], xv = x[i], rv = r[i];
+  x[i] = {xv.x + alpha * pv.x, xv.y + alpha * pv.y};
+  r[i] = {rv.x - alpha * qv.x, rv.y - alpha * qv.y};
+}
+
+template <typename T>
+__global__ void cg_p_kernel(cplx_t<T> *p, const cplx_t<T> *z, const double *num,
+                            const double *den, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  T beta = (T)(*num / *den);
+  cplx_t<T> pv = p[i], zv = z[i];
+  p[i] = {zv.x + beta * pv.x, zv.y + beta * pv.y};
+}
+
+template <typename T>
+__global__ void axpby_kernel(T a, const cplx_t<T> *x, T b, cplx_t<T> *y, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  cplx_t<T> xv = x[i], yv = y[i];
+  y[i] = {a * xv.x + b * yv.x, a * xv.y + b * yv.y};
+}
+
+template <typename S, typename D>
+__global__ void convert_kernel(const S *src, D *dst, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = (D)src[i];
+}
+
+}  // namespace owmi
+
+using namespace owmi;
+
+extern "C" {
+
+const char *owmi_last_error(void) { return owmi::g_last_error.c_str(); }
+
+int owmi_abi_version(void) { return 1; }
+
+int owmi_lmul(int dtype, const void *lmat, int diag, const void *in, void *out, int lmax,
+              int ncomp, long long ntrans, void *stream) {
+  return guard([&] {
+    check_dtype(dtype);
+    OWMI_REQUIRE(lmax >= 0 && ncomp >= 1 && ntrans >= 0, "invalid lmul sizes");
+    if (ntrans == 0) return;
+    if (dtype == OWMI_F64)
+      lmul_impl<double>(lmat, diag, in, out, lmax, ncomp, ntrans, as_stream(stream));
+    else
+      lmul_impl<float>(lmat, diag, in, out, lmax, ncomp, ntrans, as_stream(stream));
+  });
+}
+
+int owmi_trunc_alm(int dtype, const void *alm, void *alm_out, int lmax, int lmax_out,
+                   long long nrows, void *stream) {
+  return guard([&] {
+    check_dtype(dtype);
+    OWMI_REQUIRE(lmax_out <= lmax && lmax_out >= 0, "lmax_out must be in [0, lmax]");
+    OWMI_REQUIRE(nrows < 65536, "too many rows");
+    if (nrows == 0) return;
+    long long ni = nelem_of(lmax), no = nelem_of(lmax_out);
+    dim3 grid(ceil_div(no, 256), (unsigned)nrows);
+    if (dtype == OWMI_F64)
+      hipLaunchKernelGGL(trunc_kernel<double>, grid, dim3(256), 0, as_stream(stream),
+                         (const double2 *)alm, (double2 *)alm_out, lmax, lmax_out, ni, no);
+    else
+      hipLaunchKernelGGL(trunc_kernel<float>, grid, dim3(256), 0, as_stream(stream),
+                         (const float2 *)alm, (float2 *)alm_out, lmax, lmax_out, ni, no);
+  });
+}
+
+int owmi_wlm2alm(int dtype, const void *w_ell, const void *wlm, void *alm, int lmax_w,
+                 int lmax_a, long long nrows, void *stream) {
+  return guard([&] {
+    check_dtype(dtype);
+    OWMI_REQUIRE(lmax_w <= lmax_a && lmax_w >= 0, "lmax_w must be in [0, lmax_a]");
+    OWMI_REQUIRE(nrows < 65536, "too many rows");
+    if (nrows == 0) return;
+    long long nw = nelem_of(lmax_w), na = nelem_of(lmax_a);
+    dim3 grid(ceil_div(nw, 256), (unsigned)nrows);
+    if (dtype == OWMI_F64)
+      hipLaunchKernelGGL(wlm2alm_kernel<double>, grid, dim3(256), 0, as_stream(stream),
+                         (const double *)w_ell, (const double2 *)wlm, (double2 *)alm, lmax_w,
+                         lmax_a, nw, na);
+    else
+      hipLaunchKernelGGL(wlm2alm_kernel<float>, grid, dim3(256), 0, as_stream(stream),
+                         (const float *)w_ell, (const float2 *)wlm, (float2 *)alm, lmax_w,
+                         lmax_a, nw, na);
+  });
+}
+
+int owmi_alm_dot(int dtype, const void *a, const void *b, int lmax, long long nrows,
+                 double *dev_out, void *stream) {
+  return guard([&] {
+    check_dtype(dtype);
+    long long nelem = nelem_of(lmax);
+    double *part = dot_scratch();
+    hipStream_t st = as_stream(stream);
+    if (dtype == OWMI_F64)
+      hipLaunchKernelGGL(dot_partial_kernel<double>, dim3(kDotBlocks), dim3(kDotThreads), 0, st,
+                         (const double2 *)a, (const double2 *)b, lmax, nelem, nrows, part);
+    else
+      hipLaunchKernelGGL(dot_partial_kernel<float>, dim3(kDotBlocks), dim3(kDotThreads), 0, st,
+                         (const float2 *)a, (const float2 *)b, lmax, nelem, nrows, part);
+    hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(kDotThreads), 0, st, part, kDotBlocks,
+                       dev_out);
+  });
+}
+
+int owmi_apply_ringweight(int dtype, void *imap, const double *weight, const long long *nphi,
+                          const long long *offsets, const long long *stride, long long nrow,
+                          long long nmaps, long long map_stride, void *stream) {
+  return guard([&] {
+    check_dtype(dtype);
+    OWMI_REQUIRE(nmaps < 65536, "too many maps");
+    if (nrow == 0 || nmaps == 0) return;
+    dim3 grid((unsigned)nrow, (unsigned)nmaps);
+    if (dtype == OWMI_F64)
+      hipLaunchKernelGGL(ringweight_kernel<double>, grid, dim3(256), 0, as_stream(stream),
+                         (double *)imap, weight, nphi, offsets, stride, map_stride);
+    else
+      hipLaunchKernelGGL(ringweight_kernel<float>, grid, dim3(256), 0, as_stream(stream),
+                         (float *)imap, weight, nphi, offsets, stride, map_stride);
+  });
+}
+
+int owmi_pix_matvec(int dtype, const void *mat, int kind, const void *in, void *out, int ncomp,
+                    long long npix, void *stream) {
+  return guard([&] {
+    check_dtype(dtype);
+    OWMI_REQUIRE(ncomp >= 1 && ncomp <= 16, "ncomp must be in [1, 16]");
+    hipStream_t st = as_stream(stream);
+    if (kind == 0) {
+      long long n = npix * ncomp;
+      if (dtype == OWMI_F64)
+        hipLaunchKernelGGL(pix_diag_kernel<double>, dim3(ceil_div(n, 256)), dim3(256), 0, st,
+                           (const double *)mat, (const double *)in, (double *)out, n);
+      else
+        hipLaunchKernelGGL(pix_diag_kernel<float>, dim3(ceil_div(n, 256)), dim3(256), 0, st,
+                           (const float *)mat, (const float *)in, (float *)out, n);
+    } else {
+      if (dtype == OWMI_F64)
+        hipLaunchKernelGGL(pix_full_kernel<double>, dim3(ceil_div(npix, 256)), dim3(256), 0, st,
+                           (const double *)mat, (const double *)in, (double *)out, ncomp, npix);
+      else
+        hipLaunchKernelGGL(pix_full_kernel<float>, dim3(ceil_div(npix, 256)), dim3(256), 0, st,
+                           (const float *)mat, (const float *)in, (float *)out, ncomp, npix);
+    }
+  });
+}
+
+int owmi_cg_xr(int dtype, void *x, void *r, const void *p, const void *q, const double *dev_num,
+               const double *dev_den, long long n, void *stream) {
+  return guard([&] {
+    check_dtype(dtype);
+    if (n == 0) return;
+    dim3 g(ceil_div(n, 256));
+    if (dtype == OWMI_F64)
+      hipLaunchKernelGGL(cg_xr_kernel<double>, g, dim3(256), 0, as_stream(stream), (double2 *)x,
+                         (double2 *)r, (const double2 *)p, (const double2 *)q, dev_num, dev_den, n);
+    else
+      hipLaunchKernelGGL(cg_xr_kernel<float>, g, dim3(256), 0, as_stream(stream), (float2 *)x,
+                         (float2 *)r, (const float2 *)p, (const float2 *)q, dev_num, dev_den, n);
+  });
+}
+
+int owmi_cg_p(int dtype, void *p, const void *z, const double *dev_num, const double *dev_den,
+              long long n, void *stream) {
+  return guard([&] {
+    check_dtype(dtype);
+    if (n == 0) return;
+    dim3 g(ceil_div(n, 256));
+    if (dtype == OWMI_F64)
+      hipLaunchKernelGGL(cg_p_kernel<double>, g, dim3(256), 0, as_stream(stream), (double2 *)p,
+                         (const double2 *)z, dev_num, dev_den, n);
+    else
+      hipLaunchKernelGGL(cg_p_kernel<float>, g, dim3(256), 0, as_stream(stream), (float2 *)p,
+                         (const float2 *)z, dev_num, dev_den, n);
+  });
+}
+
+int owmi_axpby(int dtype, double a, const void *x, double b, void *y, long long n, void *stream) {
+  return guard([&] {
+    check_dtype(dtype);
+    if (n == 0) return;
+    dim3 g(ceil_div(n, 256));
+    if (dtype == OWMI_F64)
+      hipLaunchKernelGGL(axpby_kernel<double>, g, dim3(256), 0, as_stream(stream), a,
+                         (const double2 *)x, b, (double2 *)y, n);
+    else
+      hipLaunchKernelGGL(axpby_kernel<float>, g, dim3(256), 0, as_stream(stream), (float)a,
+                         (const float2 *)x, (float)b, (float2 *)y, n);
+  });
+}
+
+int owmi_convert(int dtype_src, const void *src, int dtype_dst, void *dst, long long n,
+                 void *stream) {
+  return guard([&] {
+    check_dtype(dtype_src);
+    check_dtype(dtype_dst);
+    if (n == 0) return;
+    long long nr = 2 * n;
+    dim3 g(ceil_div(nr, 256));
+    hipStream_t st = as_stream(stream);
+    if (dtype_src == OWMI_F64 && dtype_dst == OWMI_F32)
+      hipLaunchKernelGGL((convert_kernel<double, float>), g, dim3(256), 0, st,
+                         (const double *)src, (float *)dst, nr);
+    else if (dtype_src == OWMI_F32 && dtype_dst == OWMI_F64)
+      hipLaunchKernelGGL((convert_kernel<float, double>), g, dim3(256), 0, st,
+                         (const float *)src, (double *)dst, nr);
+    else if (dtype_src == OWMI_F64)
+      hipLaunchKernelGGL((convert_kernel<double, double>), g, dim3(256), 0, st,
+                         (const double *)src, (double *)dst, nr);
+    else
+      hipLaunchKernelGGL((convert_kernel<float, float>), g, dim3(256), 0, st,
+                         (const float *)src, (float *)dst, nr);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,134 @@
+'''
+Small symmetric matrices per pixel / per multipole, mirroring optweight's
+mat_utils: full_matrix (mat_utils.py:125-155), matpow / _eigpow (220-365),
+atleast_nd (445-473), matvec (475-532), and mat_c_utils.eigpow
+(mat_c_utils.c:9-157: V diag(lam^p) V^T, eigenvalues below lim * max set
+to zero, all zero when max < lim0).
+
+Powers are set-up work: they run on the host in float64 (numpy.linalg.eigh,
+as the reference computes them in float64 through LAPACK). matvec on device
+tensors runs through the HIP pixel kernel.
+'''
+import numpy as np
+import torch
+
+from optweight_amd import _lib
+
+
+def atleast_nd(mat, ndim, append=False):
+    mat = np.asanyarray(mat)
+    axes = np.arange(ndim - mat.ndim)
+    if append:
+        axes -= axes.size
+    return np.expand_dims(mat, tuple(axes))
+
+
+def full_matrix(mat):
+    if isinstance(mat, torch.Tensor):
+        if mat.ndim == 2:
+            npol = mat.shape[0]
+            return torch.eye(npol, dtype=mat.dtype, device=mat.device)[:, :, None] * mat
+        return mat
+    mat = np.asarray(mat)
+    if mat.ndim not in (2, 3):
+        raise ValueError(f'Matrix dimension : {mat.ndim} not supported. Requires dim = 2 or 3.')
+    if mat.ndim == 2:
+        npol = mat.shape[0]
+        mat = np.eye(npol, dtype=mat.dtype)[:, :, np.newaxis] * mat
+    return mat
+
+
+def eigpow(imat, power, lim=None, lim0=None):
+    '''(ncomp, ncomp, nsamp) -> V diag(lam^power) V^T per sample.'''
+    imat = np.asarray(imat)
+    ishape = imat.shape
+    if imat.ndim not in (2, 3):
+        raise ValueError(f'Input ndim : {imat.ndim} != 2 or 3.')
+    if imat.ndim == 2:
+        imat = imat[:, :, np.newaxis]
+    if imat.shape[0] != imat.shape[1]:
+        raise ValueError(f'Input shape : {ishape} does not match (ncomp, ncomp, nsamp) shape')
+    if imat.dtype not in (np.float32, np.float64):
+        raise ValueError(f'Input dtype : {imat.dtype} not supported')
+    if lim0 is None:
+        lim0 = np.finfo(imat.dtype).tiny ** 0.5
+    if lim is None:
+        lim = 1e-6
+    m = np.moveaxis(imat.astype(np.float64), -1, 0)
+    w, v = np.linalg.eigh(m)
+    wmax = w[:, -1:]
+    keep = (w >= lim * wmax) & (wmax >= lim0)
+    wp = np.zeros_like(w)
+    wp[keep] = w[keep] ** power
+    out = np.einsum('nij,nj,nkj->nik', v, wp, v)
+    return np.moveaxis(out, 0, -1).astype(imat.dtype).reshape(ishape)
+
+
+def matpow(mat, power, return_diag=False, skip_unit_pow=True, lim=1e-9, lim0=None,
+           chunksize=100_000):
+    '''Positive semidefinite matrix power for (npol, npol, N) or (npol, N)
+    input (diagonal input: elementwise, non-positive entries set to 0).'''
+    mat = np.asarray(mat)
+    if mat.ndim == 1:
+        mat = mat[np.newaxis, :]
+    shape_in = mat.shape
+    if not 2 <= mat.ndim <= 3:
+        raise ValueError(f'Matrix is not (npol, npol, N) or (npol, N), got : {mat.shape}')
+    if power != 1 or not skip_unit_pow:
+        if mat.ndim == 2:
+            mat = mat.copy()
+            mask = mat <= 0
+            mat[mask] = 0
+            np.power(mat, power, out=mat, where=~mask)
+        else:
+            dtype_in = mat.dtype
+            out = np.empty(mat.shape, dtype=dtype_in)
+            for start in range(0, mat.shape[-1], chunksize):
+                sl = slice(start, start + chunksize)
+                out[:, :, sl] = eigpow(mat[:, :, sl].astype(np.float64), power, lim=lim,
+                                       lim0=lim0).astype(dtype_in)
+            mat = out
+    mat = mat.reshape(shape_in)
+    if not return_diag:
+        mat = full_matrix(mat)
+    return np.array(mat, copy=True)
+
+
+def matvec(mat, vec, inplace=False):
+    '''Pixel-diagonal matrix-vector product on device tensors: elementwise
+    when shapes match, else (n, n, N) x (n, N).'''
+    if not isinstance(vec, torch.Tensor):
+        # host arrays: the reference semantics in numpy
+        mat = np.asarray(mat)
+        if mat.shape == vec.shape:
+            return np.multiply(mat, vec, out=vec if inplace else None)
+        out = np.einsum('ijk, jk -> ik', mat, vec, optimize=True)
+        if inplace:
+            vec[...] = out
+            return vec
+        return out
+    _lib.require_cuda(vec)
+    if not isinstance(mat, torch.Tensor):
+        mat = torch.as_tensor(np.asarray(mat), device=vec.device, dtype=vec.dtype)
+    _lib.require_cuda(mat)
+    if mat.dtype != vec.dtype:
+        raise ValueError(f'matrix dtype {mat.dtype} != vector dtype {vec.dtype}')
+    out = vec if inplace else torch.empty_like(vec)
+    if tuple(mat.shape) == tuple(vec.shape):
+        v2 = vec.reshape(1, -1)
+        _lib.call('owmi_pix_matvec', _lib.dtype_code(vec), _lib.ptr(mat), 0, _lib.ptr(vec),
+                  _lib.ptr(out), 1, v2.shape[-1], _lib.stream_ptr(vec.device))
+        return out
+    npol, npix = vec.shape[-2], vec.shape[-1]
+    if tuple(mat.shape) != (npol, npol, npix):
+        raise ValueError(f'matrix shape : {tuple(mat.shape)} does not match up with vector '
+                         f'shape : {tuple(vec.shape)}')
+    if inplace:
+        tmp = torch.empty_like(vec)
+        _lib.call('owmi_pix_matvec', _lib.dtype_code(vec), _lib.ptr(mat), 1, _lib.ptr(vec),
+                  _lib.ptr(tmp), npol, npix, _lib.stream_ptr(vec.device))
+        vec.copy_(tmp)
+        return vec
+    _lib.call('owmi_pix_matvec', _lib.dtype_code(vec), _lib.ptr(mat), 1, _lib.ptr(vec),
+              _lib.ptr(out), npol, npix, _lib.stream_ptr(vec.device))
+    return out

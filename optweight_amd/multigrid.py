@@ -1,0 +1,245 @@
+'''
+Multigrid V-cycle for the masked-pixel system G = M Y d_ell Y^T M on
+equal-area Gauss-Legendre grids, mirroring optweight's multigrid module
+(multigrid.py:13-471): Level, get_levels, get_equal_area_mask_bool,
+restrict / coarsen / interpolate, v_cycle, lowpass_filter.
+
+Level set-up (masks, geometry, spline interpolation of the mask) runs on
+the host with numpy/scipy like the reference; G, the smoothers and the
+inter-level transforms run on the device. The sparse smoother's diagonal
+G_ii is evaluated at one unobserved pixel per component in float64 and
+rounded to float32 (the reference probes it with a float32 transform,
+multigrid.py:82-113). The coarsest level's dense pseudo-inverse is built
+from batched device applications of G and scipy.linalg.pinvh.
+'''
+import numpy as np
+import torch
+from scipy import linalg
+
+from optweight_amd import map_utils, operators, noise_utils, alm_c_utils, sht, alm_utils
+
+
+class Level:
+
+    def __init__(self, mask, minfo, d_ell, spin, dense_smoothing=False, device=None):
+        mask = np.asarray(mask)
+        assert mask.dtype == bool, f'mask must be bool, got {mask.dtype}'
+        if mask.ndim == 1:
+            mask = mask[np.newaxis, :]
+        self.mask_unobs = ~mask
+        self.minfo = minfo
+        self.spin = spin
+        assert d_ell.shape == (self.npol, self.npol, self.lmax + 1), (
+            f'Invalid shape d_ell expected {(self.npol, self.npol, self.lmax + 1)} '
+            f'got {d_ell.shape}')
+        self.d_ell = d_ell
+        if device is None:
+            device = torch.device('cuda', torch.cuda.current_device())
+        self.device = device
+        self._mask = operators._DeviceCache(self.mask_unobs)
+        self.g_op = operators.PixEllPixMatVecMap(self.mask_unobs, self.d_ell, self.minfo,
+                                                 self.spin)
+        self.pinv_g = None
+        if dense_smoothing:
+            self.smoother, self.pinv_g = self._init_dense_smoother()
+        else:
+            self.smoother = self._init_sparse_smoother()
+
+    @property
+    def lmax(self):
+        return map_utils.minfo2lmax(self.minfo)
+
+    @property
+    def npol(self):
+        return self.mask_unobs.shape[0]
+
+    def mask_dev(self, device, dtype):
+        return self._mask.get(device, dtype)
+
+    def _init_sparse_smoother(self):
+        m_prec = np.zeros(self.npol)
+        sample_idx = np.argmax(self.mask_unobs, axis=-1)
+        test = torch.zeros((self.npol, self.minfo.npix), dtype=torch.float64, device=self.device)
+        for p in range(self.npol):
+            test[p, sample_idx[p]] = 1.
+            omap = self.g_op(test)
+            m_prec[p] = 1 / np.float32(omap[p, sample_idx[p]].item())
+            test[p, sample_idx[p]] = 0
+        m_prec *= 0.2
+        self.m_prec = m_prec
+        cache = operators._DeviceCache(m_prec[:, np.newaxis])
+
+        def smoother(x):
+            return cache.get(x.device, x.dtype) * x
+        return smoother
+
+    def dense_g(self, batch=512):
+        '''G restricted to unobserved pixels, (n_unobs, n_unobs) float64 host.'''
+        mu = self.mask_unobs
+        idx = np.nonzero(mu.reshape(-1))[0]
+        n = idx.size
+        npol, npix = mu.shape
+        ainfo = alm_utils.AlmInfo(self.lmax)
+        d = torch.as_tensor(self.d_ell, dtype=torch.float64, device=self.device)
+        mdev = torch.as_tensor(mu, dtype=torch.float64, device=self.device)
+        idx_t = torch.as_tensor(idx, device=self.device)
+        out = np.zeros((n, n))
+        for start in range(0, n, batch):
+            cols = idx[start:start + batch]
+            nb = cols.size
+            u = torch.zeros((nb, npol * npix), dtype=torch.float64, device=self.device)
+            u[torch.arange(nb, device=self.device),
+              torch.as_tensor(cols, device=self.device)] = 1
+            u = u.reshape(nb, npol, npix) * mdev
+            alm = torch.zeros((nb, npol, ainfo.nelem), dtype=torch.complex128, device=self.device)
+            sht.map2alm(u, alm, self.minfo, ainfo, self.spin, adjoint=True)
+            alm = alm_c_utils.lmul(alm, d, ainfo, inplace=True)
+            sht.alm2map(alm, u, ainfo, self.minfo, self.spin)
+            u = (u * mdev).reshape(nb, npol * npix)
+            out[:, start:start + nb] = u[:, idx_t].T.cpu().numpy()
+        return out
+
+    def _init_dense_smoother(self):
+        mat = self.dense_g()
+        pinv_mat = linalg.pinvh(mat)
+        mu_flat = torch.as_tensor(np.nonzero(self.mask_unobs.reshape(-1))[0], device=self.device)
+        pcache = operators._DeviceCache(pinv_mat)
+
+        def smoother(imap):
+            P = pcache.get(imap.device, imap.dtype)
+            flat = imap.reshape(-1)
+            out = torch.zeros_like(flat)
+            out[mu_flat] = P @ flat[mu_flat]
+            return out.reshape(imap.shape)
+        return smoother, pinv_mat
+
+    def _g_op_unobs(self, imap_unobs):
+        imap = torch.zeros(self.mask_unobs.shape, dtype=imap_unobs.dtype,
+                           device=imap_unobs.device)
+        m = torch.as_tensor(self.mask_unobs, device=imap_unobs.device)
+        imap[m] = imap_unobs
+        return self.g_op(imap)[m]
+
+
+def get_levels(mask, minfo, icov_ell, spin, min_pix=1000, lmax_r_ell=6000, device=None):
+    '''Levels halving lmax until a component has fewer than min_pix masked
+    pixels; that level gets the dense smoother.'''
+    icov_ell = np.asarray(icov_ell)
+    if icov_ell.ndim == 1:
+        icov_ell = icov_ell[np.newaxis, :]
+    if icov_ell.ndim == 2:
+        npol, nell = icov_ell.shape
+        icov_ell = icov_ell * np.eye(npol)[:, :, np.newaxis]
+    elif icov_ell.ndim == 3:
+        npol, _, nell = icov_ell.shape
+    else:
+        raise ValueError(f'shape icov_ell : {icov_ell.shape} not supported')
+    mask = np.asarray(mask)
+    if mask.ndim == 1:
+        mask = np.ones((npol, mask.size), dtype=mask.dtype) * mask
+    else:
+        assert mask.shape[0] == npol, (
+            f'shape[0] mask does not match npol {mask.shape[0]} != {npol}')
+    if mask.dtype != bool:
+        raise ValueError(f'Input mask must be bool, got {mask.dtype}')
+    lmax = map_utils.minfo2lmax(minfo)
+    levels = []
+    for idx in range(16):
+        lmax_level = lmax // 2 ** idx
+        mask_level, minfo_level = get_equal_area_mask_bool(
+            mask, minfo, lmax=lmax_level,
+            minfo_fullsky=map_utils.get_fullsky_geometry(minfo) if idx == 0 else None)
+        assert lmax_level <= nell - 1, (
+            f'lmax_level {lmax_level} exceeds lmax of icov_ell {nell - 1}')
+        nmasked = np.sum(~mask_level, axis=1)
+        last = bool(np.any(nmasked < min_pix))
+        if np.any(nmasked == 0):
+            raise ValueError(f'No masked pixels left for min_pix : {min_pix} '
+                             f'in one or more of the masks : {nmasked}')
+        if idx == 0:
+            r_ell = lowpass_filter(lmax_r_ell, lmax=lmax_level)
+            d_ell = icov_ell[:, :, :lmax_level + 1] * r_ell ** 2
+        else:
+            r_ell = noise_utils._band_limit_gauss_beam(lmax_level, fwhm=2 * np.pi / lmax_level)
+            d_ell = levels[idx - 1].d_ell[:, :, :lmax_level + 1] * r_ell ** 2
+        levels.append(Level(mask_level, minfo_level, d_ell, spin, dense_smoothing=last,
+                            device=device))
+        if last:
+            break
+    return levels
+
+
+def get_equal_area_mask_bool(mask_bool, minfo, lmax=None, minfo_fullsky=None):
+    '''Mask on the equal-area GL grid of band limit lmax; a coarse pixel is
+    unobserved only if fully made of unobserved fine pixels.'''
+    if lmax is None:
+        lmax = map_utils.minfo2lmax(minfo)
+    if minfo_fullsky is None:
+        minfo_fullsky = map_utils.MapInfo.map_info_gauss_legendre(lmax + 1, 2 * lmax + 1)
+    minfo_out = map_utils.get_equal_area_minfo(minfo_fullsky, orig_band=0.4, ratio_pow=1)
+    mask_out = map_utils.gauss2map(np.asarray(mask_bool).astype(np.float32), minfo, minfo_out,
+                                   order=1)
+    mask_out[(mask_out > 0) & (mask_out < 1)] = 1.
+    return mask_out.astype(bool), minfo_out
+
+
+def restrict(imap, level_in, level_out, spin, adjoint=False):
+    '''Y R Y^T W between levels (W Y R Y^T if adjoint), R a Gaussian beam
+    at the coarser band limit, masks on both sides.'''
+    lmax_in = level_in.lmax
+    lmax_out = level_out.lmax
+    rdt = imap.dtype
+    cdt = torch.complex128 if rdt == torch.float64 else torch.complex64
+    imap = imap * level_in.mask_dev(imap.device, rdt)
+    lmax_t = lmax_out if lmax_out < lmax_in else lmax_in
+    ainfo = alm_utils.AlmInfo(lmax_t)
+    alm = torch.zeros((imap.shape[0], ainfo.nelem), dtype=cdt, device=imap.device)
+    sht.map2alm(imap, alm, level_in.minfo, ainfo, spin, adjoint=adjoint)
+    if not adjoint:
+        r_ell = noise_utils._band_limit_gauss_beam(lmax_out, fwhm=2 * np.pi / lmax_out)
+    else:
+        r_ell = noise_utils._band_limit_gauss_beam(lmax_in, fwhm=2 * np.pi / lmax_in)
+    alm_c_utils.lmul(alm, r_ell[:lmax_t + 1], ainfo, inplace=True)
+    omap = torch.zeros((imap.shape[0], level_out.minfo.npix), dtype=rdt, device=imap.device)
+    sht.alm2map(alm, omap, ainfo, level_out.minfo, spin, adjoint=adjoint)
+    omap *= level_out.mask_dev(imap.device, rdt)
+    return omap
+
+
+def coarsen(imap, level_in, level_out, spin):
+    return restrict(imap, level_in, level_out, spin, adjoint=False)
+
+
+def interpolate(imap, level_in, level_out, spin):
+    return restrict(imap, level_in, level_out, spin, adjoint=True)
+
+
+def v_cycle(levels, imap, spin, idx=0, n_jacobi=1, x0=None):
+    '''Approximate G^-1 imap by one V-cycle from level idx.'''
+    if idx == len(levels) - 1:
+        return levels[idx].smoother(imap)
+    lev = levels[idx]
+    if x0 is not None:
+        x = x0.clone()
+        x += lev.smoother(imap - lev.g_op(x))
+    else:
+        x = torch.zeros_like(imap)
+        x += lev.smoother(imap)
+    for _ in range(n_jacobi - 1):
+        x += lev.smoother(imap - lev.g_op(x))
+    r_h = imap - lev.g_op(x)
+    r_H = coarsen(r_h, lev, levels[idx + 1], spin)
+    c_H = v_cycle(levels, r_H, spin, idx=idx + 1, n_jacobi=n_jacobi)
+    x += interpolate(c_H, levels[idx + 1], lev, spin)
+    for _ in range(n_jacobi):
+        x += lev.smoother(imap - lev.g_op(x))
+    return x
+
+
+def lowpass_filter(lmax_r_ell, lmax=None):
+    '''r_ell = exp(-beta (l(l+1))^2) with r_{lmax_r_ell/2} = sqrt(0.05).'''
+    if lmax is None:
+        lmax = lmax_r_ell
+    beta = -np.log(np.sqrt(0.05)) / ((lmax_r_ell / 2) * (lmax_r_ell / 2 + 1)) ** 2
+    ells = np.arange(lmax + 1)
+    return np.exp(-beta * (ells * (ells + 1)) ** 2.)

@@ -1,0 +1,307 @@
+'''
+Linear operators as callables on device tensors, mirroring the Wiener-path
+subset of optweight's operators module: MatVecAlm / MatVecMap ABCs (9-47),
+YMatVecAlm (49-103), YTWMatVecMap (105-160), PixMatVecMap (162-204),
+EllMatVecAlm (206-257), PixMatVecAlm (562-643), PixEllPixMatVecMap
+(991-1102), op2mat (1419-1456), add_operators (1458-1489).
+
+Host-side matrices (numpy) are moved to the device once per (device,
+dtype). When a pixel matrix is diagonal, PixMatVecAlm runs the fused
+synthesis -> pixel multiply -> adjoint kernel (sht.pixel_sandwich) instead
+of materialising the map.
+'''
+from abc import ABC, abstractmethod
+
+import numpy as np
+import torch
+
+from optweight_amd import sht, alm_c_utils, mat_utils, map_utils, alm_utils
+
+
+class MatVecAlm(ABC):
+    def __call__(self, alm, *args, **kwargs):
+        return self.call(alm, *args, **kwargs)
+
+    @abstractmethod
+    def call(self, alm, *args, **kwargs):
+        pass
+
+
+class MatVecMap(ABC):
+    def __call__(self, imap, *args, **kwargs):
+        return self.call(imap, *args, **kwargs)
+
+    @abstractmethod
+    def call(self, imap, *args, **kwargs):
+        pass
+
+
+def _real_dtype(dtype):
+    return {torch.complex128: torch.float64, torch.complex64: torch.float32,
+            torch.float64: torch.float64, torch.float32: torch.float32}[dtype]
+
+
+def _complex_dtype(dtype):
+    return {torch.float64: torch.complex128, torch.float32: torch.complex64,
+            torch.complex128: torch.complex128, torch.complex64: torch.complex64}[dtype]
+
+
+class _DeviceCache:
+    '''Device copies of a host array, per (device, dtype).'''
+
+    def __init__(self, arr):
+        self.host = arr
+        self._cache = {}
+
+    def get(self, device, dtype):
+        key = (str(device), dtype)
+        if key not in self._cache:
+            a = self.host
+            if isinstance(a, torch.Tensor):
+                t = a.to(device=device, dtype=dtype)
+            else:
+                t = torch.as_tensor(np.asarray(a), device=device).to(dtype)
+            self._cache[key] = t.contiguous()
+        return self._cache[key]
+
+
+class YMatVecAlm(MatVecAlm):
+    '''Y a (or W Y a with qweight=True).'''
+
+    def __init__(self, ainfo, minfo, spin, qweight=False):
+        self.ainfo = ainfo
+        self.minfo = minfo
+        self.spin = spin
+        self.qweight = qweight
+
+    def call(self, alm, omap=None):
+        if omap is None:
+            omap = torch.zeros(tuple(alm.shape[:-1]) + (self.minfo.npix,),
+                               dtype=_real_dtype(alm.dtype), device=alm.device)
+        sht.alm2map(alm, omap, self.ainfo, self.minfo, self.spin, adjoint=self.qweight)
+        return omap
+
+
+class YTWMatVecMap(MatVecMap):
+    '''Y^T W m (or Y^T m with qweight=False).'''
+
+    def __init__(self, minfo, ainfo, spin, qweight=True):
+        self.minfo = minfo
+        self.ainfo = ainfo
+        self.spin = spin
+        self.qweight = qweight
+
+    def call(self, imap, oalm=None):
+        if oalm is None:
+            oalm = torch.zeros(tuple(imap.shape[:-1]) + (self.ainfo.nelem,),
+                               dtype=_complex_dtype(imap.dtype), device=imap.device)
+        sht.map2alm(imap, oalm, self.minfo, self.ainfo, self.spin, adjoint=not self.qweight)
+        return oalm
+
+
+class PixMatVecMap(MatVecMap):
+    '''M^p m for M diagonal in pixels: (npol, npol, npix) or (npol, npix).'''
+
+    def __init__(self, m_pix, power, inplace=False):
+        if power != 1:
+            m_pix = mat_utils.matpow(m_pix, power, return_diag=True)
+        self.m_pix = m_pix
+        self._dev = _DeviceCache(m_pix)
+        self.inplace = inplace
+
+    def call(self, imap):
+        if not isinstance(imap, torch.Tensor):
+            return mat_utils.matvec(self.m_pix, imap, inplace=self.inplace)
+        m = self._dev.get(imap.device, imap.dtype)
+        return mat_utils.matvec(m, imap, inplace=self.inplace)
+
+    def diagonal(self, device, dtype):
+        '''(npol, npix) device diagonal, or None if the matrix has
+        off-diagonal terms.'''
+        m = np.asarray(self.m_pix)
+        if m.ndim == 2:
+            return self._dev.get(device, dtype)
+        npol = m.shape[0]
+        off = m * (1 - np.eye(npol))[:, :, None]
+        if np.any(off):
+            return None
+        d = np.ascontiguousarray(np.einsum('iip->ip', m))
+        return torch.as_tensor(d, device=device).to(dtype)
+
+
+class EllMatVecAlm(MatVecAlm):
+    '''M^p alm for M diagonal in l, (npol, npol, nell) or (npol, nell).'''
+
+    def __init__(self, ainfo, m_ell, power=1, inplace=False):
+        m_ell = mat_utils.full_matrix(np.asarray(m_ell))
+        if power != 1:
+            m_ell = mat_utils.matpow(m_ell, power)
+        self.m_ell = m_ell
+        self._dev = _DeviceCache(m_ell)
+        self.ainfo = ainfo
+        self.inplace = inplace
+
+    def device_matrix(self, device, dtype):
+        return self._dev.get(device, _real_dtype(dtype))
+
+    def call(self, alm):
+        m = self.device_matrix(alm.device, alm.dtype)
+        if self.inplace:
+            return alm_c_utils.lmul(alm, m, self.ainfo, inplace=True)
+        return alm_c_utils.lmul(alm, m, self.ainfo, alm_out=torch.empty_like(alm))
+
+
+def _pix_alm_weights(adjoint, use_weights):
+    '''(weight after synthesis, weight before adjoint) of PixMatVecAlm.'''
+    a2 = adjoint if use_weights else not adjoint
+    return bool(adjoint), not a2
+
+
+class PixMatVecAlm(MatVecAlm):
+    '''Y^T [W] M^p [W] Y alm with the weight placement of
+    operators.py:627-641 (adjoint -> W after Y; use_weights selects the
+    weight before Y^T).'''
+
+    def __init__(self, ainfo, m_pix, minfo, spin, power=1, inplace=False, adjoint=False,
+                 use_weights=False):
+        m_pix = mat_utils.full_matrix(np.asarray(m_pix))
+        if power != 1:
+            m_pix = mat_utils.matpow(m_pix, power)
+        self.m_pix = m_pix
+        self._dev = _DeviceCache(m_pix)
+        self.ainfo = ainfo
+        self.minfo = minfo
+        self.spin = spin
+        self.inplace = inplace
+        self.adjoint = adjoint
+        self.use_weights = use_weights
+        npol = m_pix.shape[0]
+        off = m_pix * (1 - np.eye(npol))[:, :, None]
+        self._is_diag = not np.any(off)
+        self._diag_cache = {}
+
+    def fused_diagonal(self, device, dtype):
+        '''Per-component pixel diagonal including the quadrature weights,
+        for the fused kernel; None when the matrix is not diagonal.'''
+        if not self._is_diag:
+            return None
+        key = (str(device), dtype)
+        if key not in self._diag_cache:
+            d = np.ascontiguousarray(np.einsum('iip->ip', self.m_pix)).astype(np.float64)
+            w_after, w_before = _pix_alm_weights(self.adjoint, self.use_weights)
+            w = np.ones(self.minfo.nrow)
+            if w_after:
+                w = w * self.minfo.weight
+            if w_before:
+                w = w * self.minfo.weight
+            if w_after or w_before:
+                d = map_utils._ringweight_np(d.copy(), self.minfo, w)
+            self._diag_cache[key] = torch.as_tensor(d, device=device).to(dtype)
+        return self._diag_cache[key]
+
+    def call(self, alm):
+        out = alm if self.inplace else torch.empty_like(alm)
+        rdt = _real_dtype(alm.dtype)
+        d = self.fused_diagonal(alm.device, rdt)
+        if d is not None:
+            if self.inplace:
+                res = torch.empty_like(alm)
+                sht.pixel_sandwich(alm, res, self.ainfo, self.minfo, self.spin, d)
+                alm.copy_(res)
+                return alm
+            return sht.pixel_sandwich(alm, out, self.ainfo, self.minfo, self.spin, d)
+        npol = alm.shape[0]
+        omap = torch.zeros((npol, self.minfo.npix), dtype=rdt, device=alm.device)
+        sht.alm2map(alm, omap, self.ainfo, self.minfo, self.spin, adjoint=self.adjoint)
+        imap = mat_utils.matvec(self._dev.get(alm.device, rdt), omap)
+        a2 = self.adjoint if self.use_weights else not self.adjoint
+        sht.map2alm(imap, out, self.minfo, self.ainfo, self.spin, adjoint=a2)
+        return out
+
+
+class PixEllPixMatVecMap(MatVecMap):
+    '''M_pix^p Y X_ell^q Y^T M_pix^p m (no quadrature weights unless
+    adjoint / use_weights say so), maps (npol, npix).'''
+
+    def __init__(self, m_pix, x_ell, minfo, spin, power_m=1, power_x=1, inplace=False,
+                 adjoint=False, use_weights=False, lmax=None):
+        if m_pix is not None and power_m != 1:
+            m_pix = mat_utils.matpow(m_pix, power_m, return_diag=True)
+        if power_x != 1:
+            x_ell = mat_utils.matpow(x_ell, power_x, return_diag=True)
+        self.m_pix = m_pix
+        self.x_ell = x_ell
+        self._m = _DeviceCache(m_pix) if m_pix is not None else None
+        self._x = _DeviceCache(x_ell)
+        self.npol = np.asarray(x_ell).shape[0]
+        if lmax is None:
+            lmax = map_utils.minfo2lmax(minfo)
+        self.ainfo = alm_utils.AlmInfo(lmax)
+        self.minfo = minfo
+        self.spin = spin
+        self.inplace = inplace
+        self.adjoint = adjoint
+        self.use_weights = use_weights
+
+    def _apply_m(self, out):
+        if self._m is None:
+            return out
+        m = self._m.get(out.device, out.dtype)
+        if m.ndim == 3:
+            return mat_utils.matvec(m, out, inplace=True)
+        if m.dtype == torch.bool:
+            return out.mul_(m)
+        return mat_utils.matvec(m, out, inplace=True)
+
+    def call(self, imap):
+        assert tuple(imap.shape[-2:]) == (self.npol, self.minfo.npix), (
+            f'Wrong imap shape: {tuple(imap.shape)} != {(self.npol, self.minfo.npix)}')
+        out = imap if self.inplace else imap.clone()
+        alm = torch.zeros(tuple(imap.shape[:-1]) + (self.ainfo.nelem,),
+                          dtype=_complex_dtype(imap.dtype), device=imap.device)
+        self._apply_m(out)
+        sht.map2alm(out, alm, self.minfo, self.ainfo, self.spin,
+                    adjoint=self.adjoint is self.use_weights)
+        alm_c_utils.lmul(alm, self._x.get(imap.device, _real_dtype(imap.dtype)), self.ainfo,
+                         inplace=True)
+        sht.alm2map(alm, out, self.ainfo, self.minfo, self.spin, adjoint=self.adjoint)
+        self._apply_m(out)
+        return out
+
+
+def op2mat(op, nrow, dtype, ncol=None, input_shape=None, device=None):
+    '''Matrix of a linear operator from its action on unit vectors (host
+    numpy operator, or device operator when device is given).'''
+    ncol = nrow if ncol is None else ncol
+    if device is None:
+        uvec = np.zeros(ncol, dtype=dtype)
+        shaped = uvec.reshape(input_shape) if input_shape is not None else uvec
+        mat = np.zeros((nrow, ncol), dtype=dtype)
+        for idx in range(ncol):
+            uvec[idx] = 1
+            mat[:, idx] = np.asarray(op(shaped)).reshape(-1)
+            uvec[:] = 0
+        return mat
+    tdt = dtype if isinstance(dtype, torch.dtype) else torch.float64
+    mat = torch.zeros((nrow, ncol), dtype=tdt, device=device)
+    uvec = torch.zeros(ncol, dtype=tdt, device=device)
+    for idx in range(ncol):
+        uvec[idx] = 1
+        shaped = uvec.reshape(input_shape) if input_shape is not None else uvec
+        mat[:, idx] = op(shaped).reshape(-1)
+        uvec[idx] = 0
+    return mat
+
+
+def add_operators(op_1, op_2, slice_2=None):
+    '''Callable acting as op_1(v) + op_2(v[slice_2]) scattered into slice_2.'''
+    if slice_2 is None:
+        slice_2 = slice(None)
+
+    def add(ivec):
+        ovec = op_1(ivec.clone() if isinstance(ivec, torch.Tensor) else ivec.copy())
+        sub = ivec[slice_2]
+        ovec[slice_2] += op_2(sub.clone() if isinstance(sub, torch.Tensor) else sub.copy())
+        return ovec
+
+    return add

@@ -1,0 +1,162 @@
+'''
+Preconditioners on device alm, mirroring optweight's preconditioners:
+HarmonicPreconditioner (preconditioners.py:8-124),
+PseudoInvPreconditioner (126-249), MaskedPreconditioner (558-650),
+get_itau_ell_harm (916-948).
+'''
+import numpy as np
+import torch
+
+from optweight_amd import (operators, mat_utils, map_utils, multigrid, alm_c_utils, sht,
+                           alm_utils)
+
+
+class HarmonicPreconditioner(operators.MatVecAlm):
+    '''M = (S^-1 + B itau B)^-1 per l (eigpow, lim 1e-9).'''
+
+    def __init__(self, ainfo, icov_ell, itau=None, icov_pix=None, minfo=None, icov_wav=None,
+                 w_ell=None, mask_pix=None, b_ell=None, icov_noise_ell=None, sfilt=None):
+        icov_ell = np.asarray(icov_ell)
+        npol, nell = icov_ell.shape[-2:]
+        if itau is None:
+            if icov_pix is None and icov_wav is None and icov_noise_ell is None:
+                raise ValueError('provide atleast itau or icov_{pix/wav/noise_ell}')
+            if icov_wav is not None:
+                raise NotImplementedError('wavelet-domain itau is not supported yet')
+            if icov_pix is not None:
+                assert icov_noise_ell is None, 'icov_pix given, icov_noise_ell is not None'
+                itau = map_utils.get_isotropic_ivar(icov_pix, minfo, mask=mask_pix)
+            if icov_noise_ell is not None:
+                itau = get_itau_ell_harm(icov_noise_ell, w_ell)
+        elif icov_pix is not None or icov_wav is not None or icov_noise_ell is not None:
+            raise ValueError('itau is provided but icov_{pix/wav/noise_ell} are not None.')
+        itau = np.asarray(itau)
+        if itau.ndim == 1:
+            itau = itau * np.eye(npol)
+        if itau.ndim == 2:
+            itau = itau[:, :, np.newaxis]
+        if b_ell is None:
+            b_ell = np.ones((npol, nell))
+        b_ell = np.asarray(b_ell) * np.eye(npol)[:, :, np.newaxis]
+        icov_full = mat_utils.full_matrix(icov_ell)
+        if sfilt is not None:
+            sfilt = mat_utils.full_matrix(np.asarray(sfilt))
+            op = np.einsum('ijl, jkl, kol -> iol', sfilt, icov_full, sfilt)
+            op2 = np.einsum('ijl, jkl, kol -> iol', b_ell, itau, b_ell)
+            op += np.einsum('ijl, jkl, kol -> iol', sfilt, op2, sfilt)
+        else:
+            op = icov_full + np.einsum('ijl, jkl, kol -> iol', b_ell, itau, b_ell)
+        self.preconditioner = operators.EllMatVecAlm(ainfo, op, -1, inplace=False)
+
+    def call(self, alm):
+        return self.preconditioner(alm)
+
+
+class PseudoInvPreconditioner(operators.MatVecAlm):
+    '''Seljebotn et al. (2017) pseudo-inverse:
+    M = H [S^-1 + (tau B) Y^T W N W Y (tau B)] H,  H = (S^-1 + B tau B)^-1.'''
+
+    def __init__(self, ainfo, icov_ell, icov_pix, minfo, spin, itau=None, b_ell=None,
+                 cov_pix=None, mask_pix=None, sfilt=None):
+        icov_ell = np.asarray(icov_ell)
+        npol, nell = icov_ell.shape[-2:]
+        if itau is None:
+            itau = map_utils.get_isotropic_ivar(icov_pix, minfo, mask=mask_pix)
+        itau = np.asarray(itau)
+        if itau.ndim == 2:
+            itau = itau[:, :, np.newaxis]
+        if np.count_nonzero(itau - itau * np.eye(npol)[:, :, np.newaxis]):
+            raise NotImplementedError(
+                'Pinv precondition cannot handle off-diagonal itau elements for now.')
+        if b_ell is None:
+            b_ell = np.ones((npol, nell))
+        b_ell = np.asarray(b_ell) * np.eye(npol)[:, :, np.newaxis]
+        if cov_pix is None:
+            cov_pix = mat_utils.matpow(icov_pix, -1)
+        if sfilt is not None:
+            sfilt = mat_utils.full_matrix(np.asarray(sfilt))
+        self.harmonic_prec = HarmonicPreconditioner(ainfo, icov_ell, itau=itau, b_ell=b_ell,
+                                                    sfilt=sfilt)
+        icov_full = mat_utils.full_matrix(icov_ell)
+        if sfilt is not None:
+            self.icov_signal = operators.EllMatVecAlm(
+                ainfo, np.einsum('ijl, jkl, kol -> iol', sfilt, icov_full, sfilt), inplace=True)
+            self.ivar_noise_iso = operators.EllMatVecAlm(
+                ainfo, np.einsum('ijl, jkl, kol -> iol', itau, b_ell, sfilt), inplace=True)
+            self.ivar_noise_iso_adj = operators.EllMatVecAlm(
+                ainfo, np.einsum('ijl, jkl, kol -> iol', sfilt, b_ell, itau), inplace=True)
+        else:
+            self.icov_signal = operators.EllMatVecAlm(ainfo, icov_full, inplace=True)
+            self.ivar_noise_iso = operators.EllMatVecAlm(ainfo, itau * b_ell, inplace=True)
+            self.ivar_noise_iso_adj = self.ivar_noise_iso
+        self.pcov_noise = operators.PixMatVecAlm(ainfo, cov_pix, minfo, spin, inplace=True,
+                                                 adjoint=True)
+
+    def call(self, alm):
+        alm = self.harmonic_prec(alm)
+        alm_signal = self.icov_signal(alm.clone())
+        alm_noise = self.ivar_noise_iso(alm)
+        alm_noise = self.pcov_noise(alm_noise)
+        alm_noise = self.ivar_noise_iso_adj(alm_noise)
+        alm_noise += alm_signal
+        return self.harmonic_prec(alm_noise)
+
+
+class MaskedPreconditioner(operators.MatVecAlm):
+    '''Multigrid approximate inverse of the masked-pixel system
+    G = M_u Y d_ell Y^T M_u (Seljebotn et al. 2017), applied as
+    r_ell Y^T M_u V-cycle(M_u Y r_ell alm).'''
+
+    def __init__(self, ainfo, icov_ell, spin, mask_bool, minfo, min_pix=1000, n_jacobi=1,
+                 lmax_r_ell=6000, nsteps=1, sfilt=None, device=None):
+        icov_ell = np.asarray(icov_ell)
+        if icov_ell.ndim == 1:
+            icov_ell = icov_ell[np.newaxis, :]
+        if sfilt is not None:
+            sfilt = mat_utils.full_matrix(np.asarray(sfilt))
+            icov_ell = mat_utils.full_matrix(icov_ell)
+            icov_ell = np.einsum('ijl, jkl, kol -> iol', sfilt, icov_ell, sfilt)
+        self.npol = icov_ell.shape[0]
+        self.spin = spin
+        self.ainfo = ainfo
+        self.n_jacobi = n_jacobi
+        self.lmax = ainfo.lmax
+        self.nsteps = nsteps
+        if device is None:
+            device = torch.device('cuda', torch.cuda.current_device())
+        self.levels = multigrid.get_levels(mask_bool, minfo, icov_ell, self.spin,
+                                           min_pix=min_pix, lmax_r_ell=lmax_r_ell, device=device)
+        self.r_ell = multigrid.lowpass_filter(lmax_r_ell, lmax=self.lmax)
+        self._r = operators._DeviceCache(self.r_ell)
+        self.mask_unobs = self.levels[0].mask_unobs
+        self.minfo = self.levels[0].minfo
+
+    def call(self, alm):
+        rdt = operators._real_dtype(alm.dtype)
+        r = self._r.get(alm.device, rdt)
+        alm = alm_c_utils.lmul(alm, r, self.ainfo, alm_out=torch.empty_like(alm))
+        imap = torch.zeros((self.npol, self.minfo.npix), dtype=rdt, device=alm.device)
+        sht.alm2map(alm, imap, self.ainfo, self.minfo, self.spin)
+        mu = self.levels[0].mask_dev(alm.device, rdt)
+        imap *= mu
+        omap = None
+        for _ in range(self.nsteps):
+            omap = multigrid.v_cycle(self.levels, imap, self.spin, n_jacobi=self.n_jacobi, x0=omap)
+        omap *= mu
+        sht.map2alm(omap, alm, self.minfo, self.ainfo, self.spin, adjoint=True)
+        return alm_c_utils.lmul(alm, r, self.ainfo, inplace=True)
+
+
+def get_itau_ell_harm(icov_ell, w_ell):
+    '''Inverse variance per wavelet band, diagonal only (host numpy).'''
+    icov_ell = mat_utils.full_matrix(mat_utils.atleast_nd(np.asarray(icov_ell), 2))
+    itau_ell = np.zeros_like(icov_ell)
+    npol = itau_ell.shape[0]
+    w_ell = np.asarray(w_ell)
+    for widx in range(w_ell.shape[0]):
+        w2 = w_ell[widx] ** 2
+        sw2 = np.sum(w2)
+        for p in range(npol):
+            a_j = np.sum(icov_ell[p, p] * w2) / sw2
+            itau_ell[p, p] += a_j * w2
+    return itau_ell

@@ -1,0 +1,174 @@
+'''
+Spherical harmonic transforms on device tensors, the interface of
+optweight's sht module: map2alm (sht.py:10-93), alm2map (sht.py:95-178),
+default_spin (sht.py:180-215). The transforms run on the HIP Legendre and
+ring-FFT kernels through owmi_sht_* (one plan per geometry and lmax, cached
+on the MapInfo object).
+
+    alm2map(alm, omap, ainfo, minfo, spin, adjoint=False)   Y   (W Y if adjoint)
+    map2alm(imap, alm, minfo, ainfo, spin, adjoint=False)   Y^T W   (Y^T if adjoint)
+'''
+import ctypes
+
+import numpy as np
+import torch
+
+from optweight_amd import _lib
+
+
+class Plan:
+    '''Owns one owmi_sht_plan (geometry + lmax on one device).'''
+
+    def __init__(self, minfo, lmax, device):
+        self.lmax = lmax
+        self.npix = minfo.npix
+        self.device = device
+        self._lib = _lib.load()
+        h = ctypes.c_void_p()
+        theta = _lib.as_np(minfo.theta, np.float64)
+        weight = _lib.as_np(minfo.weight, np.float64)
+        nphi = _lib.as_np(minfo.nphi, np.int64)
+        offsets = _lib.as_np(minfo.offsets, np.int64)
+        phi0 = _lib.as_np(minfo.phi0, np.float64)
+        if np.any(minfo.stride != 1):
+            raise ValueError('only unit pixel stride is supported')
+        with torch.cuda.device(device):
+            _lib.check(self._lib.owmi_sht_plan_create(
+                ctypes.byref(h), minfo.nrow, _lib.np_ptr(theta), _lib.np_ptr(weight),
+                _lib.np_ptr(nphi), _lib.np_ptr(offsets), _lib.np_ptr(phi0), lmax))
+        self.handle = h
+
+    def __del__(self):
+        try:
+            if self.handle:
+                self._lib.owmi_sht_plan_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+def get_plan(minfo, lmax, device):
+    device = torch.device(device)
+    if device.index is None:
+        device = torch.device('cuda', torch.cuda.current_device())
+    cache = minfo.__dict__.setdefault('_plans', {})
+    key = (str(device), int(lmax))
+    if key not in cache:
+        cache[key] = Plan(minfo, int(lmax), device)
+    return cache[key]
+
+
+def default_spin(shape):
+    '''npol 1 -> 0, 2 -> 2, 3 -> [0, 2].'''
+    if len(shape) > 2:
+        raise ValueError(f'Cannot determine default spin for alm/map with more than '
+                         f'one leading dimension. Got shape : {shape}')
+    if len(shape) == 1 or shape[0] == 1:
+        return 0
+    if shape[0] == 2:
+        return 2
+    if shape[0] == 3:
+        return [0, 2]
+    raise ValueError(f'Cannot determine default spin value for shape : {shape}')
+
+
+def spin_groups(spin, npol):
+    '''(s, i1, i2): spin s acts on components i1:i2 (spin 0 takes one
+    component, spin 2 two), walking the spin list over npol components.'''
+    spins = np.atleast_1d(spin).tolist()
+    out = []
+    i = 0
+    k = 0
+    while i < npol:
+        s = int(spins[min(k, len(spins) - 1)])
+        if s not in (0, 2):
+            raise ValueError(f'spin {s} not supported (0 or 2)')
+        n = 1 if s == 0 else 2
+        if i + n > npol:
+            raise ValueError(f'spin {spin} incompatible with npol {npol}')
+        out.append((s, i, i + n))
+        i += n
+        k += 1
+    return out
+
+
+def _lmax_of(ainfo):
+    return ainfo if isinstance(ainfo, (int, np.integer)) else ainfo.lmax
+
+
+def _nelem(lmax):
+    return (lmax * lmax + 3 * lmax) // 2 + 1
+
+
+def _check(alm, imap, ainfo, minfo, spin):
+    lmax = _lmax_of(ainfo)
+    if np.asarray(spin).max() > lmax:
+        raise ValueError('Spin exceeds lmax')
+    if imap.shape[-1] != minfo.npix:
+        raise ValueError(f'Wrong map size, got {imap.shape[-1]} expected {minfo.npix}')
+    if alm.shape[-1] != _nelem(lmax):
+        raise ValueError(f'Wrong alm size, got {alm.shape[-1]} expected {_nelem(lmax)}')
+    _lib.require_cuda(alm, imap)
+    a3 = alm.reshape((-1,) + tuple(alm.shape[-2:])) if alm.ndim >= 2 else alm.reshape(1, 1, -1)
+    m3 = imap.reshape((-1,) + tuple(imap.shape[-2:])) if imap.ndim >= 2 else imap.reshape(1, 1, -1)
+    if a3.shape[:-1] != m3.shape[:-1]:
+        raise ValueError(f'Mismatch leading dimension of map and alm :'
+                         f'{tuple(imap.shape[:-1])} and {tuple(alm.shape[:-1])}')
+    real = torch.float64 if alm.dtype == torch.complex128 else torch.float32
+    if alm.dtype not in (torch.complex64, torch.complex128) or imap.dtype != real:
+        raise ValueError(f'dtype mismatch: alm {alm.dtype}, map {imap.dtype}')
+    return lmax, a3, m3
+
+
+def alm2map(alm, omap, ainfo, minfo, spin, adjoint=False):
+    '''Synthesis Y (or W Y if adjoint) of alm (..., npol, nelem) into omap
+    (..., npol, npix), overwritten.'''
+    lmax, a3, m3 = _check(alm, omap, ainfo, minfo, spin)
+    plan = get_plan(minfo, lmax, alm.device)
+    npol = m3.shape[-2]
+    code = _lib.dtype_code(omap)
+    st = _lib.stream_ptr(alm.device)
+    for s, i1, i2 in spin_groups(spin, npol):
+        _lib.check(plan._lib.owmi_sht_synthesis(
+            plan.handle, code, s, _lib.ptr(a3[:, i1]), npol * a3.shape[-1], _lib.ptr(m3[:, i1]),
+            npol * m3.shape[-1], m3.shape[0], int(bool(adjoint)), st))
+    return omap
+
+
+def map2alm(imap, alm, minfo, ainfo, spin, adjoint=False):
+    '''Adjoint synthesis Y^T W (or Y^T if adjoint) of imap (..., npol, npix)
+    into alm (..., npol, nelem), overwritten.'''
+    lmax, a3, m3 = _check(alm, imap, ainfo, minfo, spin)
+    plan = get_plan(minfo, lmax, alm.device)
+    npol = m3.shape[-2]
+    code = _lib.dtype_code(imap)
+    st = _lib.stream_ptr(alm.device)
+    for s, i1, i2 in spin_groups(spin, npol):
+        _lib.check(plan._lib.owmi_sht_adjoint(
+            plan.handle, code, s, _lib.ptr(m3[:, i1]), npol * m3.shape[-1], _lib.ptr(a3[:, i1]),
+            npol * a3.shape[-1], m3.shape[0], int(not adjoint), st))
+    return alm
+
+
+def pixel_sandwich(alm, oalm, ainfo, minfo, spin, pixdiag):
+    '''oalm = Y^T D Y alm with D = pixdiag (npol, npix) per component; the
+    map stays on chip between the two transforms.'''
+    lmax = _lmax_of(ainfo)
+    _lib.require_cuda(alm, oalm, pixdiag)
+    if tuple(alm.shape) != tuple(oalm.shape) or alm.dtype != oalm.dtype:
+        raise ValueError('alm and oalm must match')
+    a3 = alm.reshape((-1,) + tuple(alm.shape[-2:]))
+    o3 = oalm.reshape((-1,) + tuple(oalm.shape[-2:]))
+    npol = a3.shape[-2]
+    if tuple(pixdiag.shape[-2:]) != (npol, minfo.npix):
+        raise ValueError(f'pixdiag shape {tuple(pixdiag.shape)} != {(npol, minfo.npix)}')
+    p3 = pixdiag.reshape((-1, npol, minfo.npix))
+    pix_ts = 0 if p3.shape[0] == 1 else npol * minfo.npix
+    plan = get_plan(minfo, lmax, alm.device)
+    code = _lib.dtype_code(alm)
+    st = _lib.stream_ptr(alm.device)
+    for s, i1, i2 in spin_groups(spin, npol):
+        _lib.check(plan._lib.owmi_sht_pixel_sandwich(
+            plan.handle, code, s, _lib.ptr(a3[:, i1]), npol * a3.shape[-1], _lib.ptr(o3[:, i1]),
+            npol * o3.shape[-1], _lib.ptr(p3[:, i1]), pix_ts, a3.shape[0], st))
+    return oalm

@@ -1,0 +1,320 @@
+'''
+Conjugate-gradient Wiener filter on device alm, mirroring optweight's
+solvers.CGWienerMap (solvers.py:8-456) and the pixell.utils.CG recurrence
+it inherits (init / step as used at solvers.py:127-141):
+
+  init : x = 0 or x0;  r = b (aliased) or b - A x0;  z = M r;  rz = <r,z>;
+         rz0 = rz;  p = z;  i = 0;  err = inf
+  step : Ap = A p;  alpha = rz/<p,Ap>;  x += alpha p;  r -= alpha Ap;
+         z = M r;  rz' = <r,z>;  err = rz'/rz0;  beta = rz'/rz;  rz = rz';
+         p = z + beta p;  i += 1
+
+The CG scalars stay on the device (float64) and the vector updates are the
+HIP kernels owmi_cg_xr / owmi_cg_p: one iteration needs no host
+synchronisation; reading ``err`` synchronises.
+
+When the data model is pixel-diagonal (mask and inverse noise diagonal per
+component, no swap_bm / lensing / filter / scale filter) the A operator
+runs Y^T (M N^-1 M) Y through the fused synthesis -> pixel multiply ->
+adjoint kernel (sht.pixel_sandwich).
+'''
+import numpy as np
+import torch
+
+from optweight_amd import (operators, alm_utils, map_utils, mat_utils, sht as sht_tools,
+                           alm_c_utils, _lib)
+
+
+def _is_fast(t):
+    return (isinstance(t, torch.Tensor) and t.is_cuda and t.is_contiguous()
+            and t.dtype in (torch.complex64, torch.complex128))
+
+
+class CG:
+    '''Preconditioned conjugate gradients with pixell.utils.CG semantics.'''
+
+    def __init__(self, A, b, x0=None, M=lambda x: x, dot=None):
+        if dot is None:
+            dot = _default_dot(b)
+        self.A = A
+        self.b = b
+        self.M = M
+        self.dot = dot
+        if x0 is None:
+            self.x = b * 0
+            self.r = b
+        else:
+            self.x = x0.clone() if isinstance(x0, torch.Tensor) else x0.copy()
+            self.r = b - self.A(self.x)
+        z = self.M(self.r)
+        self.rz = self.dot(self.r, z)
+        self.rz0 = float(self.rz)
+        self.p = z
+        self.i = 0
+        self._err = np.inf
+
+    @property
+    def err(self):
+        e = self._err
+        if isinstance(e, torch.Tensor):
+            e = float(e.item())
+            self._err = e
+        return e
+
+    @err.setter
+    def err(self, value):
+        self._err = value
+
+    def step(self):
+        Ap = self.A(self.p)
+        pAp = self.dot(self.p, Ap)
+        dev_scalars = isinstance(self.rz, torch.Tensor) and isinstance(pAp, torch.Tensor)
+        if dev_scalars and _is_fast(self.x) and _is_fast(self.r) and _is_fast(self.p) \
+                and _is_fast(Ap):
+            _lib.call('owmi_cg_xr', _lib.dtype_code(self.x), _lib.ptr(self.x), _lib.ptr(self.r),
+                      _lib.ptr(self.p), _lib.ptr(Ap), _lib.ptr(self.rz), _lib.ptr(pAp),
+                      self.x.numel(), _lib.stream_ptr(self.x.device))
+        else:
+            alpha = self.rz / pAp
+            self.x += alpha * self.p
+            self.r -= alpha * Ap
+        del Ap
+        z = self.M(self.r)
+        next_rz = self.dot(self.r, z)
+        self._err = next_rz / self.rz0
+        if dev_scalars and isinstance(next_rz, torch.Tensor) and _is_fast(z) \
+                and _is_fast(self.p) and z.shape == self.p.shape:
+            _lib.call('owmi_cg_p', _lib.dtype_code(self.p), _lib.ptr(self.p), _lib.ptr(z),
+                      _lib.ptr(next_rz), _lib.ptr(self.rz), self.p.numel(),
+                      _lib.stream_ptr(self.p.device))
+        else:
+            beta = next_rz / self.rz
+            self.p = z + beta * self.p
+        self.rz = next_rz
+        self.i += 1
+
+
+def _default_dot(b):
+    if isinstance(b, torch.Tensor):
+        return alm_utils.contract_almxblm_dev
+    return alm_utils.contract_almxblm
+
+
+class CGWienerMap(CG):
+    '''Solve (S^-1 + P^T N^-1 P) x = P^T N^-1 d (+ constrained-realisation
+    terms) with P = M Y B for device alm (npol, nelem) and maps
+    (npol, npix). Same constructor, attributes and methods as optweight's
+    CGWienerMap (lensing and map-space filters are not part of this path).'''
+
+    def __init__(self, imap, icov_signal, icov_noise, sht, beam=None, lens=None, filt=None,
+                 mask=None, rand_isignal=None, rand_inoise=None, swap_bm=False,
+                 scale_filter=None):
+        if lens is not None or filt is not None:
+            raise NotImplementedError('lensing / map filters are not supported')
+        self.imap = imap
+        self.icov_signal = icov_signal
+        self.icov_noise = icov_noise
+        self._beam_op = beam
+        self.beam = beam if beam is not None else (lambda alm: alm)
+        self.sht = sht[0]
+        self.sht_adjoint = sht[1]
+        self.lens = self.lens_adjoint = lambda alm: alm
+        self.filt = self.filt_adjoint = lambda alm: alm
+        self._mask_op = mask
+        self.mask = mask if mask is not None else (lambda imap: imap)
+        self.swap_bm = swap_bm
+        self._sfilt_op = scale_filter
+        self.scale_filter = scale_filter if scale_filter is not None else (lambda alm: alm)
+        self._fast = None
+        self.set_b_vec(self.imap, rand_isignal=rand_isignal, rand_inoise=rand_inoise)
+        self.preconditioner = None
+
+    # --------------------------------------------------------- set-up
+    def init_solver(self, **kwargs):
+        kwargs.setdefault('dot', _default_dot(self.b_vec))
+        if self.preconditioner is not None:
+            kwargs.setdefault('M', self.preconditioner)
+        super().__init__(self.a_matrix, self.b_vec, **kwargs)
+
+    def add_preconditioner(self, prec, sel=None):
+        if self.preconditioner is None:
+            if sel is None:
+                self.preconditioner = prec
+            else:
+                def sliced_prec(alm):
+                    alm[sel] = prec(alm[sel])
+                    return alm
+                self.preconditioner = sliced_prec
+        else:
+            self.preconditioner = operators.add_operators(self.preconditioner, prec,
+                                                          slice_2=sel)
+
+    def reset_preconditioner(self):
+        self.preconditioner = None
+
+    # -------------------------------------------------------- operators
+    def proj(self, alm):
+        if not self.swap_bm:
+            alm = self.lens(alm)
+            alm = self.beam(alm)
+            omap = self.sht(alm)
+            omap = self.mask(omap)
+            return self.filt(omap)
+        alm = self.mask(alm)
+        alm = self.lens(alm)
+        alm = self.beam(alm)
+        omap = self.sht(alm)
+        return self.filt(omap)
+
+    def proj_adjoint(self, imap):
+        if not self.swap_bm:
+            omap = self.filt_adjoint(imap)
+            omap = self.mask(omap)
+            alm = self.sht_adjoint(omap)
+            alm = self.beam(alm)
+            return self.lens_adjoint(alm)
+        omap = self.filt_adjoint(imap)
+        alm = self.sht_adjoint(omap)
+        alm = self.beam(alm)
+        alm = self.lens_adjoint(alm)
+        return self.mask(alm)
+
+    def _fast_plan(self, alm):
+        '''(pixel diagonal, beam matrix, S^-1 matrix, Y op) when A can run
+        fused, else None.'''
+        key = (str(alm.device), alm.dtype)
+        if self._fast is not None and self._fast[0] == key:
+            return self._fast[1]
+        plan = None
+        y, yt = self.sht, self.sht_adjoint
+        ok = (not self.swap_bm and self._sfilt_op is None
+              and isinstance(y, operators.YMatVecAlm) and not y.qweight
+              and isinstance(yt, operators.YTWMatVecMap) and not yt.qweight
+              and isinstance(self.icov_noise, operators.PixMatVecMap)
+              and isinstance(self.icov_signal, operators.EllMatVecAlm)
+              and (self._beam_op is None or isinstance(self._beam_op, operators.EllMatVecAlm))
+              and (self._mask_op is None or isinstance(self._mask_op, operators.PixMatVecMap)))
+        if ok:
+            rdt = operators._real_dtype(alm.dtype)
+            d = self.icov_noise.diagonal(alm.device, rdt)
+            if d is not None and self._mask_op is not None:
+                md = self._mask_op.diagonal(alm.device, rdt)
+                d = None if md is None else (md * d * md).contiguous()
+            if d is not None:
+                beam = (self._beam_op.device_matrix(alm.device, alm.dtype)
+                        if self._beam_op is not None else None)
+                plan = (d, beam, self.icov_signal.device_matrix(alm.device, alm.dtype), y)
+        self._fast = (key, plan)
+        return plan
+
+    def a_matrix(self, alm):
+        fast = self._fast_plan(alm) if isinstance(alm, torch.Tensor) else None
+        if fast is not None:
+            d, beam, icov, y = fast
+            ainfo = y.ainfo
+            t = alm_c_utils.lmul(alm, beam, ainfo) if beam is not None else alm
+            u = torch.empty_like(alm)
+            sht_tools.pixel_sandwich(t, u, ainfo, y.minfo, y.spin, d)
+            if beam is not None:
+                alm_c_utils.lmul(u, beam, ainfo, inplace=True)
+            u += alm_c_utils.lmul(alm, icov, ainfo)
+            return u
+        alm = self.scale_filter(alm)
+        omap = self.proj(alm)
+        omap = self.icov_noise(omap)
+        oalm = self.proj_adjoint(omap)
+        oalm = self.scale_filter(oalm)
+        oalm += self.scale_filter(self.icov_signal(alm))
+        return oalm
+
+    def get_b_vec(self, imap):
+        omap = self.icov_noise(imap)
+        oalm = self.proj_adjoint(omap)
+        return self.scale_filter(oalm)
+
+    def get_b_vec_constr(self, rand_isignal, rand_inoise):
+        oalm = self.scale_filter(self.proj_adjoint(rand_inoise))
+        oalm += self.scale_filter(rand_isignal)
+        return oalm
+
+    def set_b_vec(self, imap, rand_isignal=None, rand_inoise=None):
+        self.b_vec = self.get_b_vec(imap)
+        if rand_isignal is not None and rand_inoise is not None:
+            self.b_vec += self.get_b_vec_constr(rand_isignal, rand_inoise)
+        self.b0 = self.b_vec.clone() if isinstance(self.b_vec, torch.Tensor) \
+            else self.b_vec.copy()
+
+    # ----------------------------------------------------------- output
+    def get_wiener(self):
+        return self.scale_filter(self.x.clone() if isinstance(self.x, torch.Tensor)
+                                 else self.x.copy())
+
+    def get_icov(self):
+        return self.icov_signal(self.get_wiener())
+
+    def get_chisq(self):
+        x_w = self.get_wiener()
+        out = alm_utils.contract_almxblm(x_w, self.icov_signal(x_w))
+        res = self.imap - self.proj(x_w)
+        out += float((res * self.icov_noise(res)).sum())
+        return out
+
+    def get_residual(self):
+        r = self.A(self.x) - self.b0
+        return np.sqrt(alm_utils.contract_almxblm(r, r))
+
+    def get_qform(self):
+        return (0.5 * alm_utils.contract_almxblm(self.x, self.A(self.x))
+                - alm_utils.contract_almxblm(self.x, self.b0))
+
+    @classmethod
+    def from_arrays(cls, imap, minfo, ainfo, icov_ell, icov_pix, *extra_args, b_ell=None,
+                    mask_pix=None, minfo_mask=None, draw_constr=False, spin=None,
+                    swap_bm=False, sfilt=None, lensop=None, seed=None, device=None):
+        '''Build the solver from arrays; host inputs are moved to ``device``
+        (default: current CUDA device), the alm precision follows imap.'''
+        if lensop:
+            raise NotImplementedError('lensing is not supported')
+        if device is None:
+            device = torch.device('cuda', torch.cuda.current_device())
+        imap_np = imap.detach().cpu().numpy() if isinstance(imap, torch.Tensor) else np.asarray(imap)
+        if spin is None:
+            spin = sht_tools.default_spin(imap_np.shape)
+        imap_np = mat_utils.atleast_nd(imap_np, 2)
+        icov_pix = mat_utils.atleast_nd(np.asarray(icov_pix), 2)
+        icov_ell = mat_utils.atleast_nd(np.asarray(icov_ell), 2)
+        if icov_pix.shape[-2:] != imap_np.shape:
+            raise ValueError(f'Mismatch {icov_pix.shape=} and {imap_np.shape=}')
+        if icov_ell.shape[0] != imap_np.shape[0]:
+            raise ValueError(f'Mismatch {icov_ell.shape=} and {imap_np.shape=}')
+        icov_signal = operators.EllMatVecAlm(ainfo, icov_ell)
+        icov_noise = operators.PixMatVecMap(icov_pix, 1, inplace=False)
+        beam = operators.EllMatVecAlm(ainfo, b_ell) if b_ell is not None else None
+        sf = operators.EllMatVecAlm(ainfo, sfilt) if sfilt is not None else None
+        if mask_pix is not None:
+            if minfo_mask is None:
+                minfo_mask = minfo
+            if swap_bm:
+                mask = operators.PixMatVecAlm(ainfo, mask_pix, minfo_mask, spin,
+                                              use_weights=True)
+            else:
+                mask = operators.PixMatVecMap(mask_pix, 1, inplace=False)
+        else:
+            mask = None
+        rdt = torch.float64 if imap_np.dtype == np.float64 else torch.float32
+        cdt = torch.complex128 if rdt == torch.float64 else torch.complex64
+        if draw_constr:
+            rand_isignal = torch.as_tensor(
+                alm_utils.rand_alm(icov_ell, ainfo, seed,
+                                   dtype=np.complex128 if cdt == torch.complex128
+                                   else np.complex64), device=device)
+            rand_inoise = torch.as_tensor(map_utils.rand_map_pix(icov_pix, seed),
+                                          device=device).to(rdt)
+        else:
+            rand_isignal = rand_inoise = None
+        sht = (operators.YMatVecAlm(ainfo, minfo, spin, qweight=False),
+               operators.YTWMatVecMap(minfo, ainfo, spin, qweight=False))
+        imap_dev = torch.as_tensor(np.ascontiguousarray(imap_np), device=device).to(rdt)
+        return cls(imap_dev, icov_signal, icov_noise, sht, beam=beam, mask=mask,
+                   rand_isignal=rand_isignal, rand_inoise=rand_inoise, swap_bm=swap_bm,
+                   scale_filter=sf)

@@ -1,0 +1,151 @@
+'''
+GPU parity of the streaming kernels (per-l multiply, truncation, wavelet
+accumulate, ring weights, pixel matvec, alm dot, CG updates) against the
+oracle, through the C ABI. Integer/layout work is bit-exact; float work
+agrees to 1e-15 relative (float64) / 1e-6 (float32).
+'''
+import numpy as np
+import pytest
+import torch
+
+from oracle import layout, ops, sht as osht, geometry as og
+from optweight_amd import alm_c_utils, map_c_utils, alm_utils, mat_utils, map_utils, _lib
+
+pytestmark = pytest.mark.gpu
+
+
+def rand_alm(lmax, npol, seed, dtype=np.complex128):
+    rng = np.random.default_rng(seed)
+    n = layout.nelem(lmax)
+    return (rng.normal(size=(npol, n)) + 1j * rng.normal(size=(npol, n))).astype(dtype)
+
+
+@pytest.mark.parametrize('dtype', [np.complex128, np.complex64])
+@pytest.mark.parametrize('npol', [1, 3])
+@pytest.mark.parametrize('diag', [False, True])
+@pytest.mark.parametrize('inplace', [False, True])
+def test_lmul(dev, dtype, npol, diag, inplace):
+    lmax = 37
+    a = rand_alm(lmax, npol, 1, dtype)
+    rng = np.random.default_rng(2)
+    lmat = rng.normal(size=(npol, lmax + 1)) if diag else rng.normal(size=(npol, npol, lmax + 1))
+    ref = ops.lmul(a, lmat, lmax)
+    at = torch.as_tensor(a, device=dev)
+    if inplace:
+        out = alm_c_utils.lmul(at, lmat, lmax, inplace=True)
+        assert out.data_ptr() == at.data_ptr()
+    else:
+        out = alm_c_utils.lmul(at, lmat, lmax)
+    tol = 1e-15 if dtype == np.complex128 else 1e-6
+    np.testing.assert_allclose(out.cpu().numpy(), ref, rtol=tol, atol=tol * np.abs(ref).max())
+
+
+def test_lmul_batched_and_1d(dev):
+    lmax = 20
+    a = np.stack([rand_alm(lmax, 3, s) for s in range(5)])
+    lmat = np.random.default_rng(3).normal(size=(3, 3, lmax + 1))
+    out = alm_c_utils.lmul(torch.as_tensor(a, device=dev), lmat, lmax).cpu().numpy()
+    for t in range(5):
+        np.testing.assert_allclose(out[t], ops.lmul(a[t], lmat, lmax), rtol=1e-15, atol=1e-14)
+    w = np.arange(lmax + 1, dtype=float)
+    out = alm_c_utils.lmul(torch.as_tensor(a, device=dev), w, lmax).cpu().numpy()
+    np.testing.assert_allclose(out, ops.lmul(a, w, lmax), rtol=1e-15, atol=1e-14)
+
+
+def test_trunc_alm_golden(dev):
+    '''test_alm_c_utils.py:10-59 literal arrays, bit exact.'''
+    alm = np.zeros((2, 6), dtype=np.complex128)
+    alm[0] = [1, 2, 3, 4, 5, 6]
+    alm[1] = np.asarray([1, 2, 3, 4, 5, 6]) * 2
+    exp = np.zeros((2, 3), dtype=np.complex128)
+    exp[0] = [1, 2, 4]
+    exp[1] = np.asarray([1, 2, 4]) * 2
+    out = torch.zeros((2, 3), dtype=torch.complex128, device=dev)
+    alm_c_utils.trunc_alm(torch.as_tensor(alm, device=dev), out, 2, 1)
+    np.testing.assert_array_equal(out.cpu().numpy(), exp)
+    a = rand_alm(30, 3, 4)
+    out = torch.zeros((3, layout.nelem(11)), dtype=torch.complex128, device=dev)
+    alm_c_utils.trunc_alm(torch.as_tensor(a, device=dev), out, 30, 11)
+    np.testing.assert_array_equal(out.cpu().numpy(), ops.trunc_alm(a, 30, 11))
+
+
+@pytest.mark.parametrize('dtype', [np.complex128, np.complex64])
+def test_wlm2alm_golden(dev, dtype):
+    '''test_alm_c_utils.py:387-441 literal arrays.'''
+    alm = np.ones((3, 10), dtype=dtype)
+    alm[1] *= 2
+    alm[2] *= 3
+    wlm = np.ones((3, 6), dtype=dtype) * 1j
+    wlm[1] *= 10
+    wlm[2] *= 100
+    wlm *= np.arange(6)
+    w_ell = np.arange(4, dtype=float)
+    w_ell[-1] = 0
+    exp = np.zeros_like(alm)
+    exp[0] = [1, 1 + 1j, 1 + 4j, 1, 1 + 3j, 1 + 8j, 1, 1 + 10j, 1, 1]
+    exp[1] = [2, 2 + 10j, 2 + 40j, 2, 2 + 30j, 2 + 80j, 2, 2 + 100j, 2, 2]
+    exp[2] = [3, 3 + 100j, 3 + 400j, 3, 3 + 300j, 3 + 800j, 3, 3 + 1000j, 3, 3]
+    at = torch.as_tensor(alm, device=dev)
+    alm_c_utils.wlm2alm(w_ell, torch.as_tensor(wlm, device=dev), at, 2, 3)
+    np.testing.assert_allclose(at.cpu().numpy(), exp)
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+def test_ringweight(dev, dtype):
+    g = og.gauss_legendre(13, 25)
+    m = map_utils.MapInfo(g.theta, g.weight, nphi=g.nphi)
+    f = np.random.default_rng(5).normal(size=(3, g.npix)).astype(dtype)
+    ref = osht.ringweight(f.copy(), g)
+    ft = torch.as_tensor(f, device=dev)
+    map_c_utils.apply_ringweight(ft, m)
+    np.testing.assert_array_equal(ft.cpu().numpy(), ref)
+
+
+def test_pix_matvec(dev):
+    rng = np.random.default_rng(6)
+    mat = rng.normal(size=(3, 3, 1000))
+    vec = rng.normal(size=(3, 1000))
+    out = mat_utils.matvec(torch.as_tensor(mat, device=dev), torch.as_tensor(vec, device=dev))
+    np.testing.assert_allclose(out.cpu().numpy(), ops.matvec(mat, vec), rtol=1e-14, atol=1e-14)
+    d = rng.normal(size=(3, 1000))
+    out = mat_utils.matvec(torch.as_tensor(d, device=dev), torch.as_tensor(vec, device=dev))
+    np.testing.assert_array_equal(out.cpu().numpy(), d * vec)
+
+
+def test_contract_golden(dev):
+    '''test_alm_utils.py:411-448: -40, -120, -240.'''
+    a = np.asarray([1, 1, 1, 1, 2j, 2j, 2j, 3j, 3j, 4j])
+    b = np.asarray([2, 2, 2, 2, 3, 3, 3, -4j, -4j, 5])
+    A = torch.as_tensor(a, device=dev)
+    B = torch.as_tensor(b, device=dev)
+    assert alm_utils.contract_almxblm(A, B) == -40
+    A3 = torch.as_tensor(np.ones((3, 10)) * a, device=dev)
+    B3 = torch.as_tensor(np.ones((3, 10)) * b, device=dev)
+    assert alm_utils.contract_almxblm(A3, B3) == -120
+    A4 = torch.as_tensor(np.ones((2, 3, 10)) * a, device=dev)
+    B4 = torch.as_tensor(np.ones((2, 3, 10)) * b, device=dev)
+    assert alm_utils.contract_almxblm(A4, B4) == -240
+
+
+def test_contract_large(dev):
+    a = rand_alm(300, 3, 7)
+    b = rand_alm(300, 3, 8)
+    ref = ops.contract_almxblm(a, b)
+    got = alm_utils.contract_almxblm(torch.as_tensor(a, device=dev), torch.as_tensor(b, device=dev))
+    assert abs(got - ref) <= 1e-12 * abs(ref) + 1e-9
+
+
+def test_cg_updates(dev):
+    rng = np.random.default_rng(9)
+    n = 5000
+    x, r, p, q, z = [rng.normal(size=n) + 1j * rng.normal(size=n) for _ in range(5)]
+    X, R, P, Q, Z = [torch.as_tensor(v, device=dev) for v in (x, r, p, q, z)]
+    num = torch.tensor(2.5, dtype=torch.float64, device=dev)
+    den = torch.tensor(-1.25, dtype=torch.float64, device=dev)
+    _lib.call('owmi_cg_xr', _lib.F64, _lib.ptr(X), _lib.ptr(R), _lib.ptr(P), _lib.ptr(Q),
+              _lib.ptr(num), _lib.ptr(den), n, _lib.stream_ptr(dev))
+    np.testing.assert_allclose(X.cpu().numpy(), x - 2 * p, rtol=1e-15, atol=1e-15)
+    np.testing.assert_allclose(R.cpu().numpy(), r + 2 * q, rtol=1e-15, atol=1e-15)
+    _lib.call('owmi_cg_p', _lib.F64, _lib.ptr(P), _lib.ptr(Z), _lib.ptr(num), _lib.ptr(den), n,
+              _lib.stream_ptr(dev))
+    np.testing.assert_allclose(P.cpu().numpy(), z - 2 * p, rtol=1e-15, atol=1e-15)
