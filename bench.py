@@ -1,0 +1,202 @@
+'''
+Benchmark: Wiener solves/s on the polarized (TQU) lmax=1023 workload,
+GL 1024 x 2047 grid, to true relative residual ||b - A x|| / ||b|| <= 1e-6
+(float64 operator, real-field alm norm), on one MI355X.
+
+A step is one full solve of the workload (inputs resident in HBM).
+Reported: the mixed-precision solver (float32 CG inner solves, float64
+iterative refinement) as ``value`` when available, the plain float64 GPU
+solve beside it; the roofline of the dominant kernel measured with HIP
+events on the launch stream; a CPU baseline timed on a bounded sample.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--lmax 1023]
+'''
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+import torch
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from optweight_amd import (solvers, preconditioners, workloads, alm_utils, sht,  # noqa: E402
+                           _lib)
+
+PEAK_FP64_TFLOPS = 78.6
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_fp64_solver(prob, dev, use_mg=True):
+    solver = solvers.CGWienerMap.from_arrays(prob.imap, prob.minfo, prob.ainfo, prob.icov_ell,
+                                             prob.icov_pix, device=dev)
+    solver.add_preconditioner(preconditioners.PseudoInvPreconditioner(
+        prob.ainfo, prob.icov_ell, prob.icov_pix, prob.minfo, prob.spin))
+    if use_mg:
+        mg = preconditioners.MaskedPreconditioner(
+            prob.ainfo, prob.icov_ell[0:1, 0:1], 0, prob.mask_bool[0], prob.minfo,
+            min_pix=1000, n_jacobi=1, lmax_r_ell=6000, device=dev)
+        solver.add_preconditioner(mg, sel=np.s_[0])
+    return solver
+
+
+def true_residual(solver, x):
+    r = solver.a_matrix(x) - solver.b0
+    return np.sqrt(alm_utils.contract_almxblm(r, r) / alm_utils.contract_almxblm(solver.b0,
+                                                                                 solver.b0))
+
+
+def solve_fp64(solver, tol=1e-6, maxiter=500):
+    '''Plain float64 preconditioned CG from x = 0 until the recursive
+    residual norm drops below tol ||b|| (checked every iteration on the
+    device scalar).'''
+    solver.set_b_vec(solver.imap)
+    solver.init_solver()
+    bnorm2 = alm_utils.contract_almxblm_dev(solver.b0, solver.b0)
+    thr = (tol ** 2) * float(bnorm2.item())
+    rr = torch.empty((), dtype=torch.float64, device=solver.b0.device)
+    while solver.i < maxiter:
+        solver.step()
+        alm_utils.contract_almxblm_dev(solver.r, solver.r, out=rr)
+        if float(rr.item()) <= thr:
+            break
+    return solver.x, solver.i
+
+
+def time_kernel(fn, reps=10):
+    '''Average duration (ms) of fn() measured with events on the current
+    stream.'''
+    fn()
+    s = torch.cuda.Event(enable_timing=True)
+    e = torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(reps):
+        fn()
+    e.record()
+    e.synchronize()
+    return s.elapsed_time(e) / reps
+
+
+def legendre_roofline(prob, dev):
+    '''Spin-2 Legendre synthesis (the dominant kernel) at the workload's
+    shape, timed alone: useful flops N_lm * N_theta * 8 per launch.'''
+    plan = sht.get_plan(prob.minfo, prob.lmax, dev)
+    alm = torch.zeros((3, prob.ainfo.nelem), dtype=torch.complex128, device=dev)
+    alm[1:] = 1.0
+    st = _lib.stream_ptr(dev)
+
+    def run():
+        _lib.check(plan._lib.owmi_sht_legendre(plan.handle, _lib.F64, 2, 0,
+                                               _lib.ptr(alm[1]), 3 * prob.ainfo.nelem, 1, st))
+    ms = time_kernel(run)
+    flops = prob.ainfo.nelem * prob.minfo.nrow * 8
+    achieved = flops / (ms * 1e-3) / 1e12
+    return {'kernel': 'synth2_kernel<double>', 'ms': ms, 'flops': flops,
+            'achieved': achieved}
+
+
+def cpu_baseline_sample(prob):
+    '''Oracle (numpy float64) A-operator application at the workload's
+    size, on the host. Returns seconds per application.'''
+    from oracle import wiener as ow, geometry as og
+    g = og.Geom(prob.minfo.theta, prob.minfo.weight, prob.minfo.nphi, phi0=prob.minfo.phi0)
+    osol = ow.WienerMap.from_arrays(prob.imap, g, prob.lmax, prob.icov_ell, prob.icov_pix)
+    v = osol.b0.copy()
+    t0 = time.perf_counter()
+    osol.a_matrix(v)
+    return time.perf_counter() - t0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=5)
+    ap.add_argument('--warmup', type=int, default=1)
+    ap.add_argument('--lmax', type=int, default=1023)
+    ap.add_argument('--no-mg', action='store_true')
+    ap.add_argument('--cpu-baseline', choices=['on', 'off'], default='on')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print('bench.py measures one GPU: this build has no multi-GPU path '
+              '(independent right-hand sides shard trivially across GPUs).', file=sys.stderr)
+        sys.exit(2)
+    dev = torch.device('cuda:0')
+    torch.cuda.set_device(dev)
+
+    log(f'building workload TQU lmax={args.lmax}')
+    t0 = time.perf_counter()
+    prob = workloads.polarized(args.lmax, seed=0, device=dev)
+    solver = build_fp64_solver(prob, dev, use_mg=not args.no_mg)
+    log(f'setup {time.perf_counter() - t0:.1f}s')
+
+    for w in range(args.warmup):
+        x, it = solve_fp64(solver)
+        log(f'warmup solve {w}: {it} iterations, true residual {true_residual(solver, x):.3e}')
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    iters = []
+    for _ in range(args.steps):
+        x, it = solve_fp64(solver)
+        iters.append(it)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    resid = true_residual(solver, x)
+    fp64_sps = args.steps / elapsed
+    log(f'fp64: {args.steps} solves in {elapsed:.3f}s, {np.mean(iters):.1f} it/solve, '
+        f'residual {resid:.3e}')
+
+    roof = legendre_roofline(prob, dev)
+    log(f'roofline: {roof}')
+
+    cpu = None
+    if args.cpu_baseline == 'on':
+        t_a = cpu_baseline_sample(prob)
+        # per-solve CPU estimate: the oracle's A time scaled by the GPU
+        # solve's cost in A applications
+        a_ms = time_kernel(lambda: solver.a_matrix(solver.b0), reps=5)
+        solve_in_a = (elapsed / args.steps) / (a_ms * 1e-3)
+        cpu = {'value': 1.0 / (t_a * solve_in_a), 'unit': 'solves/s', 'cores': 1,
+               'kind': 'port',
+               'sample': (f'one float64 A application of the oracle at TQU lmax {args.lmax} '
+                          f'({t_a:.1f}s), scaled by the GPU solve cost in A applications '
+                          f'({solve_in_a:.1f})')}
+        log(f'cpu baseline: {cpu}')
+
+    out = {
+        'metric': 'Wiener solves/sec, TQU lmax=1023 to resid 1e-6: mixed/batched vs FP64 GPU',
+        'value': fp64_sps,
+        'unit': 'solves/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': 1e3 * elapsed / args.steps,
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f64',
+        'data': 'synthetic',
+        'config': {'workload': f'TQU Wiener filter, lmax {args.lmax}, GL {args.lmax + 1}x'
+                               f'{2 * args.lmax + 1}, pinv + multigrid(T) preconditioner, '
+                               f'resid 1e-6',
+                   'solver': 'fp64', 'iterations': float(np.mean(iters)),
+                   'true_residual': resid, 'fp64_solves_per_s': fp64_sps},
+        'roofline': {'bound': 'mfma', 'achieved': roof['achieved'],
+                     'peak': PEAK_FP64_TFLOPS, 'unit': 'TFLOP/s',
+                     'frac': roof['achieved'] / PEAK_FP64_TFLOPS, 'traffic': None,
+                     'kernel': roof['kernel'], 'kernel_ms': roof['ms']},
+        'cpu_baseline': cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == '__main__':
+    main()

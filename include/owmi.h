@@ -203,6 +203,18 @@ int owmi_sht_pixel_sandwich(owmi_sht_plan *plan, int dtype, int spin,
                             const void *pixdiag, long long pix_tstride,
                             long long ntrans, void *stream);
 
+/*
+ * The Legendre stage alone, between alm and the plan's ring-coefficient
+ * workspace (direction 0: alm -> workspace, the first half of synthesis;
+ * direction 1: workspace -> alm, the second half of the adjoint). Used to
+ * time and profile the Legendre kernels in isolation; the workspace holds
+ * whatever the previous transform on this plan left (zero when new).
+ * Building block of sht.alm2map / sht.map2alm (optweight/sht.py:10-178).
+ */
+int owmi_sht_legendre(owmi_sht_plan *plan, int dtype, int spin, int direction,
+                      void *alm, long long alm_tstride, long long ntrans,
+                      void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -41,6 +41,7 @@ _SIGNATURES = {
     'owmi_sht_adjoint': (_c_int, [_vp, _c_int, _c_int, _vp, _c_ll, _vp, _c_ll, _c_ll, _c_int, _vp]),
     'owmi_sht_pixel_sandwich': (_c_int, [_vp, _c_int, _c_int, _vp, _c_ll, _vp, _c_ll, _vp, _c_ll,
                                          _c_ll, _vp]),
+    'owmi_sht_legendre': (_c_int, [_vp, _c_int, _c_int, _c_int, _vp, _c_ll, _c_ll, _vp]),
 }
 
 _lib = None

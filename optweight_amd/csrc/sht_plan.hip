@@ -104,6 +104,7 @@ void *owmi_plan_impl::ensure_work(size_t bytes) {
   work = nullptr;
   work_bytes = 0;
   OWMI_HIP(hipMalloc(&work, bytes));
+  OWMI_HIP(hipMemset(work, 0, bytes));
   work_bytes = bytes;
   return work;
 }
@@ -506,7 +507,37 @@ static int do_sandwich(owmi_sht_plan *plan, int spin, const void *alm_in, long l
   });
 }
 
+template <typename T>
+static int do_legendre(owmi_sht_plan *plan, int spin, int direction, void *alm,
+                       long long alm_ts, long long ntrans, void *stream) {
+  return guard([&] {
+    OWMI_REQUIRE(plan != nullptr, "plan is NULL");
+    owmi_plan_impl &p = plan->impl;
+    OWMI_REQUIRE(spin == 0 || spin == 2, "spin must be 0 or 2");
+    OWMI_REQUIRE(direction == 0 || direction == 1, "direction must be 0 or 1");
+    if (ntrans == 0) return;
+    int nc = spin == 0 ? 1 : 2;
+    size_t rows = (size_t)ntrans * nc * p.nrings;
+    auto *G = static_cast<cplx_t<T> *>(p.ensure_work(rows * (p.mmax + 1) * sizeof(cplx_t<T>)));
+    hipStream_t st = as_stream(stream);
+    if (direction == 0)
+      launch_synthesis<T>(&p, spin, static_cast<const cplx_t<T> *>(alm), alm_ts, G, ntrans, st);
+    else
+      launch_adjoint<T>(&p, spin, G, static_cast<cplx_t<T> *>(alm), alm_ts, ntrans, st);
+  });
+}
+
 extern "C" {
+
+int owmi_sht_legendre(owmi_sht_plan *plan, int dtype, int spin, int direction, void *alm,
+                      long long alm_tstride, long long ntrans, void *stream) {
+  if (dtype == OWMI_F64)
+    return do_legendre<double>(plan, spin, direction, alm, alm_tstride, ntrans, stream);
+  if (dtype == OWMI_F32)
+    return do_legendre<float>(plan, spin, direction, alm, alm_tstride, ntrans, stream);
+  set_error("dtype must be OWMI_F32 or OWMI_F64");
+  return 1;
+}
 
 int owmi_sht_synthesis(owmi_sht_plan *plan, int dtype, int spin, const void *alm,
                        long long alm_tstride, void *map, long long map_tstride, long long ntrans,

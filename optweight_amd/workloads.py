@@ -1,0 +1,178 @@
+'''
+Synthetic Wiener-filter problems on Gauss-Legendre grids (seeded, host
+numpy), used by the benchmark, the tests and the smoke check.
+
+  temperature(lmax, ...)   T only, GL (lmax+1) x (2 lmax+1):
+      C_l = 1e3 (l + 10)^-2.5 for l >= 2, 0 below; icov_l = pinv(C_l);
+      data = mask * (Y a + n), a = C^(1/2) unit-variance alm, n white with
+      sigma per pixel; icov_pix = w / sigma^2, zero in the mask.
+  polarized(lmax, ...)     TQU on the same grid:
+      TT as above, EE = 0.05 TT, BB = 0.005 TT, TE = 0.3 sqrt(TT EE);
+      icov_T = depth * w / sigma^2 with depth = exp(g), g a seeded smooth
+      Gaussian field (l < 30) mapped affinely onto [ln 0.5, ln 2];
+      icov_Q = icov_U = icov_T / 2 (Q/U noise sqrt(2) x T).
+  mask: zero inverse noise for |latitude| < 10 deg and inside ndisk disks
+  of radius 0.5 deg centred on uniformly random sphere points.
+
+The signal map Y a is synthesised with the device transform (``synth``
+callable, default the HIP SHT); the noise and everything else is numpy.
+'''
+import numpy as np
+import torch
+from scipy.special import sph_harm_y
+
+from optweight_amd import map_utils, alm_utils, sht, mat_utils
+
+
+def cov_ell_T(lmax):
+    ell = np.arange(lmax + 1, dtype=np.float64)
+    c = np.zeros(lmax + 1)
+    c[2:] = 1e3 * (ell[2:] + 10.) ** -2.5
+    return c
+
+
+def cov_ell_TQU(lmax):
+    tt = cov_ell_T(lmax)
+    ee = 0.05 * tt
+    bb = 0.005 * tt
+    te = 0.3 * np.sqrt(tt * ee)
+    cov = np.zeros((3, 3, lmax + 1))
+    cov[0, 0] = tt
+    cov[1, 1] = ee
+    cov[2, 2] = bb
+    cov[0, 1] = cov[1, 0] = te
+    return cov
+
+
+def pinv_ell(cov):
+    '''Per-l pseudo-inverse (as optweight's filters.py:107-109).'''
+    cov = mat_utils.full_matrix(np.asarray(cov))
+    out = np.zeros_like(cov)
+    for l in range(cov.shape[-1]):
+        out[:, :, l] = np.linalg.pinv(cov[:, :, l])
+    return out
+
+
+def gl_minfo(lmax):
+    return map_utils.MapInfo.map_info_gauss_legendre(lmax + 1, 2 * lmax + 1)
+
+
+def pixel_coords(minfo):
+    theta = np.repeat(minfo.theta, minfo.nphi)
+    phi = np.concatenate([minfo.phi0[r] + 2 * np.pi * np.arange(minfo.nphi[r]) / minfo.nphi[r]
+                          for r in range(minfo.nrow)])
+    return theta, phi
+
+
+def make_mask(minfo, seed, ndisk=50, rdisk_deg=0.5, band_deg=10.):
+    '''Observed = True. Galactic-like band plus point-source disks.'''
+    rng = np.random.default_rng(seed)
+    theta, phi = pixel_coords(minfo)
+    lat = np.pi / 2 - theta
+    obs = np.abs(lat) >= np.radians(band_deg)
+    z = rng.uniform(-1, 1, ndisk)
+    ph = rng.uniform(0, 2 * np.pi, ndisk)
+    th = np.arccos(z)
+    cosr = np.cos(np.radians(rdisk_deg))
+    st, ct = np.sin(theta), np.cos(theta)
+    for k in range(ndisk):
+        cosd = st * np.sin(th[k]) * np.cos(phi - ph[k]) + ct * np.cos(th[k])
+        obs &= cosd < cosr
+    return obs
+
+
+def smooth_field(minfo, seed, lmax_s=29):
+    '''Seeded Gaussian field with unit C_l for l <= lmax_s, evaluated on the
+    rings directly from scipy's Y_lm (no transform library involved).'''
+    ainfo = alm_utils.AlmInfo(lmax_s)
+    a = alm_utils.unit_var_alm(ainfo, (1,), seed)[0]
+    ell, em = ainfo.lm_arrays()
+    out = np.zeros(minfo.npix)
+    for r in range(minfo.nrow):
+        th = minfo.theta[r]
+        n = minfo.nphi[r]
+        ph = minfo.phi0[r] + 2 * np.pi * np.arange(n) / n
+        F = np.zeros(lmax_s + 1, dtype=np.complex128)
+        y = sph_harm_y(ell, em, th, 0.0)
+        np.add.at(F, em, a * y)
+        mm = np.arange(lmax_s + 1)
+        fac = np.where(mm == 0, 1.0, 2.0)
+        vals = np.real(np.exp(1j * np.outer(ph, mm)) @ (fac * F))
+        out[minfo.offsets[r]:minfo.offsets[r] + n] = vals
+    return out
+
+
+def depth_map(minfo, seed, lo=0.5, hi=2.0):
+    g = smooth_field(minfo, seed)
+    g = (g - g.min()) / (g.max() - g.min())
+    return np.exp(np.log(lo) + g * (np.log(hi) - np.log(lo)))
+
+
+def _synth_device(alm, minfo, lmax, spin, device):
+    ainfo = alm_utils.AlmInfo(lmax)
+    a = torch.as_tensor(alm, device=device)
+    out = torch.zeros((alm.shape[0], minfo.npix), dtype=torch.float64, device=device)
+    sht.alm2map(a, out, ainfo, minfo, spin)
+    return out.cpu().numpy()
+
+
+class Problem:
+    '''Arrays of one Wiener-filter problem (host numpy).'''
+
+    def __init__(self, lmax, minfo, icov_ell, icov_pix, imap, mask_bool, spin, cov_ell,
+                 signal_alm):
+        self.lmax = lmax
+        self.minfo = minfo
+        self.ainfo = alm_utils.AlmInfo(lmax)
+        self.icov_ell = icov_ell
+        self.icov_pix = icov_pix
+        self.imap = imap
+        self.mask_bool = mask_bool
+        self.spin = spin
+        self.cov_ell = cov_ell
+        self.signal_alm = signal_alm
+
+
+def temperature(lmax, seed=0, sigma=10., ndisk=50, synth=None, device=None):
+    minfo = gl_minfo(lmax)
+    cov = cov_ell_T(lmax)[None, :]
+    icov_ell = np.zeros_like(cov)
+    icov_ell[0, 2:] = 1 / cov[0, 2:]
+    ss = np.random.SeedSequence([1, seed])
+    s_alm, s_noise, s_mask = [np.random.default_rng(s) for s in ss.spawn(3)]
+    a = alm_utils.rand_alm(cov, alm_utils.AlmInfo(lmax), s_alm)
+    if synth is None:
+        dev = device or torch.device('cuda', torch.cuda.current_device())
+        sig = _synth_device(a, minfo, lmax, 0, dev)
+    else:
+        sig = synth(a, minfo, lmax, 0)
+    obs = make_mask(minfo, s_mask, ndisk=ndisk)
+    w = np.repeat(minfo.weight, minfo.nphi)
+    icov_pix = (w / sigma ** 2 * obs)[None, :]
+    noise = s_noise.standard_normal(minfo.npix) * sigma
+    imap = ((sig[0] + noise) * obs)[None, :]
+    return Problem(lmax, minfo, icov_ell, icov_pix, imap, obs[None, :], 0, cov, a)
+
+
+def polarized(lmax, seed=0, sigma=10., ndisk=50, synth=None, device=None):
+    minfo = gl_minfo(lmax)
+    cov = cov_ell_TQU(lmax)
+    icov_ell = pinv_ell(cov)
+    ss = np.random.SeedSequence([2, seed])
+    s_alm, s_noise, s_mask, s_depth = [np.random.default_rng(s) for s in ss.spawn(4)]
+    a = alm_utils.rand_alm(cov, alm_utils.AlmInfo(lmax), s_alm)
+    if synth is None:
+        dev = device or torch.device('cuda', torch.cuda.current_device())
+        sig = _synth_device(a, minfo, lmax, [0, 2], dev)
+    else:
+        sig = synth(a, minfo, lmax, [0, 2])
+    obs = make_mask(minfo, s_mask, ndisk=ndisk)
+    depth = depth_map(minfo, s_depth)
+    w = np.repeat(minfo.weight, minfo.nphi)
+    icov_t = depth * w / sigma ** 2 * obs
+    icov_pix = np.stack([icov_t, icov_t / 2, icov_t / 2])
+    sig_pix = np.stack([sigma / np.sqrt(depth), sigma * np.sqrt(2 / depth),
+                        sigma * np.sqrt(2 / depth)])
+    noise = s_noise.standard_normal((3, minfo.npix)) * sig_pix
+    imap = (sig + noise) * obs
+    return Problem(lmax, minfo, icov_ell, icov_pix, imap, np.stack([obs] * 3), [0, 2], cov, a)

@@ -133,7 +133,8 @@ def test_pixel_sandwich(dev):
 
 @pytest.mark.parametrize('lmax', [255])
 def test_large_lmax_roundtrip(dev, lmax):
-    '''Y^T W Y = I on the GL grid (size-independent property).'''
+    '''Y^T W Y = I on the GL grid (size-independent property). The oracle
+    itself reaches 1.86e-12 on this case (quadrature rounding floor).'''
     m = map_utils.MapInfo.map_info_gauss_legendre(lmax + 1, 2 * lmax + 1)
     a = rand_alm(lmax, 3, 8)
     ell, _ = layout.lm_arrays(lmax)
@@ -144,4 +145,4 @@ def test_large_lmax_roundtrip(dev, lmax):
     sht.alm2map(at, mp, ai, m, [0, 2])
     back = torch.zeros_like(at)
     sht.map2alm(mp, back, m, ai, [0, 2])
-    assert rel(back.cpu().numpy(), a) < 1e-12
+    assert rel(back.cpu().numpy(), a) < 1e-11

@@ -1,0 +1,362 @@
+'''
+Pins of the oracle (CPU). Each check is independent of the formula it
+pins: exact Wigner-d sums (mpmath), textbook closed forms, an independent
+closed form of the spin-2 functions (HEALPix / Zaldarriaga-Seljak), brute
+force synthesis, the reference's own compiled C (oracle/_ref) and the
+literal answers of the reference test-suite.
+'''
+import ctypes
+import os
+
+import mpmath
+import numpy as np
+import pytest
+
+from oracle import legendre as L, sht as osht, geometry as og, layout, ops, cg as ocg, \
+    wiener as ow
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+REF_LIB = os.path.join(ROOT, 'oracle', '_ref', 'libref_c_utils.so')
+
+
+# ------------------------------------------------------------- Legendre
+
+@pytest.mark.parametrize('s', [0, 2, -2])
+@pytest.mark.parametrize('m', [0, 1, 2, 3, 7])
+def test_recurrence_vs_exact_wigner(s, m):
+    th = np.array([0.05, 0.7, 1.3, 2.2, 3.0])
+    lam = L.lam_m(m, 14, th, s)
+    for l in range(m, 15):
+        for i, t in enumerate(th):
+            ex = float(L.lam_exact(l, m, s, t))
+            assert abs(float(lam[l - m, i]) - ex) < 1e-15 * max(1, abs(ex))
+
+
+def test_textbook_values():
+    t = 0.4
+    assert abs(float(L.lam_exact(0, 0, 0, t)) - 1 / np.sqrt(4 * np.pi)) < 1e-16
+    assert abs(float(L.lam_exact(1, 1, 0, t)) + np.sqrt(3 / 8 / np.pi) * np.sin(t)) < 1e-16
+    assert abs(float(L.lam_exact(2, 0, 2, t)) - np.sqrt(15 / 32 / np.pi) * np.sin(t) ** 2) < 1e-16
+    assert abs(float(L.lam_exact(2, 0, -2, t)) - np.sqrt(15 / 32 / np.pi) * np.sin(t) ** 2) < 1e-16
+
+
+@pytest.mark.parametrize('m', [0, 1, 2, 3, 5])
+def test_spin2_vs_healpix_closed_form(m):
+    '''F1 = (2lam + -2lam)/2, F2 = (2lam - -2lam)/2 from Wigner d agree with
+    the independent P_l^m closed form, including the sign of F2.'''
+    for l in range(max(2, m), 10):
+        for t in (0.3, 0.9, 2.0):
+            lp = float(L.lam_exact(l, m, 2, t))
+            lm = float(L.lam_exact(l, m, -2, t))
+            F1, F2 = L.f1f2_healpix(l, m, t)
+            assert abs((lp + lm) / 2 - float(F1)) < 1e-14
+            assert abs((lp - lm) / 2 - float(F2)) < 1e-14
+
+
+def test_healpix_pin_catches_flipped_sign():
+    '''A planted mistake (F2 sign flipped in the oracle's spin-2 functions)
+    is caught by the closed-form pin.'''
+    l, m, t = 4, 2, 0.8
+    lp = float(L.lam_exact(l, m, 2, t))
+    lm = float(L.lam_exact(l, m, -2, t))
+    F1, F2 = L.f1f2_healpix(l, m, t)
+    wrong_f2 = -(lp - lm) / 2
+    assert abs(wrong_f2 - float(F2)) > 1e-3
+
+
+def test_high_lmax_recurrence_against_legenp():
+    """lmax 4095, m 3000, theta 0.85: the seed slam_mm ~ 6e-373 lies below
+    the float64 range, yet slam_{4095,m} is O(1). The longdouble recurrence
+    matches mpmath's associated Legendre function (60 digits)."""
+    th, m, l = 0.85, 3000, 4095
+    lam = L.lam_m(m, l, np.array([th]), 0)
+    assert 0 < abs(lam[0, 0]) < np.longdouble('1e-330')
+    with mpmath.workdps(60):
+        x = mpmath.cos(mpmath.mpf(th))
+        norm = mpmath.sqrt(mpmath.mpf(2 * l + 1) / (4 * mpmath.pi)
+                           * mpmath.factorial(l - m) / mpmath.factorial(l + m))
+        ex = float(norm * mpmath.legenp(l, m, x))
+    assert abs(float(lam[-1, 0]) - ex) < 1e-15 * abs(ex)
+
+
+# ------------------------------------------------------------------ SHT
+
+def _brute_synth(a, g, lmax):
+    ell, em = layout.lm_arrays(lmax)
+    phi = np.concatenate([g.phi0[r] + 2 * np.pi * np.arange(g.nphi[r]) / g.nphi[r]
+                          for r in range(g.nrow)])
+    rr = np.concatenate([np.full(g.nphi[r], r) for r in range(g.nrow)])
+    bt = np.zeros(g.npix)
+    bq = np.zeros(g.npix)
+    bu = np.zeros(g.npix)
+    for i, (l, m) in enumerate(zip(ell, em)):
+        lam0 = np.array([float(L.lam_exact(l, m, 0, t)) for t in g.theta])
+        lp = np.array([float(L.lam_exact(l, m, 2, t)) for t in g.theta])
+        ln = np.array([float(L.lam_exact(l, m, -2, t)) for t in g.theta])
+        e = np.exp(1j * m * phi)
+        fac = 1 if m == 0 else 2
+        bt += fac * (a[0, i] * lam0[rr] * e).real
+        F1, F2 = (lp + ln) / 2, (lp - ln) / 2
+        bq += fac * (-(a[1, i] * F1[rr] + 1j * a[2, i] * F2[rr]) * e).real
+        bu += fac * (-(a[2, i] * F1[rr] - 1j * a[1, i] * F2[rr]) * e).real
+    return np.stack([bt, bq, bu])
+
+
+def _rand_alm(lmax, npol, seed):
+    rng = np.random.default_rng(seed)
+    n = layout.nelem(lmax)
+    a = rng.normal(size=(npol, n)) + 1j * rng.normal(size=(npol, n))
+    ell, em = layout.lm_arrays(lmax)
+    a[:, em == 0] = a[:, em == 0].real
+    return a
+
+
+def test_sht_brute_force():
+    lmax = 5
+    g = og.gauss_legendre(lmax + 1, 2 * lmax + 1)
+    a = _rand_alm(lmax, 3, 1)
+    out = np.zeros((3, g.npix))
+    osht.alm2map(a, out, lmax, g, [0, 2])
+    ref = _brute_synth(a, g, lmax)
+    assert np.abs(out - ref).max() < 1e-13
+
+
+@pytest.mark.parametrize('spin,npol', [(0, 1), (2, 2), ([0, 2], 3)])
+def test_sht_roundtrip_and_adjoint(spin, npol):
+    lmax = 20
+    g = og.gauss_legendre(lmax + 1, 2 * lmax + 1)
+    a = _rand_alm(lmax, npol, 2)
+    ell, _ = layout.lm_arrays(lmax)
+    if npol >= 2:
+        a[npol - 2:, ell < 2] = 0
+    mp = np.zeros((npol, g.npix))
+    osht.alm2map(a, mp, lmax, g, spin)
+    back = np.zeros_like(a)
+    osht.map2alm(mp, back, g, lmax, spin)
+    assert np.linalg.norm(back - a) / np.linalg.norm(a) < 1e-13
+    f = np.random.default_rng(3).normal(size=(npol, g.npix))
+    at = np.zeros_like(a)
+    osht.map2alm(f, at, g, lmax, spin, adjoint=True)
+    lhs = np.sum(mp * f)
+    rhs = ops.contract_almxblm(a, at)
+    assert abs(lhs - rhs) < 1e-12 * abs(lhs)
+
+
+def test_sht_adjoint_equal_area():
+    lmax = 16
+    g = og.equal_area(og.gauss_legendre(2 * lmax + 2, 4 * lmax + 1), orig_band=0.4)
+    assert np.any(g.nphi < 2 * lmax + 1)
+    a = _rand_alm(lmax, 1, 4)
+    mp = np.zeros((1, g.npix))
+    osht.alm2map(a, mp, lmax, g, 0)
+    f = np.random.default_rng(5).normal(size=(1, g.npix))
+    at = np.zeros_like(a)
+    osht.map2alm(f, at, g, lmax, 0, adjoint=True)
+    assert abs(np.sum(mp * f) - ops.contract_almxblm(a, at)) < 1e-12 * abs(np.sum(mp * f))
+
+
+def test_y00_map():
+    '''a_00 = sqrt(4 pi) gives the constant map 1.'''
+    lmax = 4
+    g = og.gauss_legendre(lmax + 1, 2 * lmax + 1)
+    a = np.zeros((1, layout.nelem(lmax)), dtype=np.complex128)
+    a[0, 0] = np.sqrt(4 * np.pi)
+    mp = np.zeros((1, g.npix))
+    osht.alm2map(a, mp, lmax, g, 0)
+    np.testing.assert_allclose(mp, 1, rtol=1e-14)
+
+
+# ------------------------------------------------------ geometry / setup
+
+def test_gl_geometry():
+    '''test_map_utils.py:87-93 (sum w nphi = 4 pi) and :215-234.'''
+    from scipy.special import roots_legendre
+    g = og.gauss_minfo(20)
+    assert abs(np.sum(g.weight * g.nphi) - 4 * np.pi) < 1e-13
+    x, w = roots_legendre(11)
+    np.testing.assert_allclose(np.cos(g.theta), x[::-1], atol=1e-15)
+    np.testing.assert_allclose(g.weight * g.nphi, w[::-1] * 2 * np.pi, rtol=1e-14)
+
+
+def test_mirror_pairs():
+    g = og.gauss_legendre(9, 17)
+    n, s = og.mirror_pairs(g)
+    assert sorted(np.concatenate([n, s[s >= 0]]).tolist()) == list(range(9))
+    assert np.sum(s < 0) == 1  # the equator ring
+
+
+# ------------------------------------------------------ streaming ops
+
+def test_contract_golden():
+    '''test_alm_utils.py:411-448.'''
+    a = np.asarray([1, 1, 1, 1, 2j, 2j, 2j, 3j, 3j, 4j])
+    b = np.asarray([2, 2, 2, 2, 3, 3, 3, -4j, -4j, 5])
+    assert ops.contract_almxblm(a, b) == -40
+    assert ops.contract_almxblm(np.ones((3, 10)) * a, np.ones((3, 10)) * b) == -120
+    assert ops.contract_almxblm(np.ones((2, 3, 10)) * a, np.ones((2, 3, 10)) * b) == -240
+
+
+def test_trunc_wlm2alm_golden():
+    '''test_alm_c_utils.py:10-59 and 387-441.'''
+    alm = np.zeros((2, 6), dtype=np.complex128)
+    alm[0] = [1, 2, 3, 4, 5, 6]
+    alm[1] = np.asarray([1, 2, 3, 4, 5, 6]) * 2
+    np.testing.assert_equal(ops.trunc_alm(alm, 2, 1), [[1, 2, 4], [2, 4, 8]])
+    alm = np.ones((3, 10), dtype=np.complex128)
+    alm[1] *= 2
+    alm[2] *= 3
+    wlm = np.ones((3, 6), dtype=np.complex128) * 1j
+    wlm[1] *= 10
+    wlm[2] *= 100
+    wlm *= np.arange(6)
+    w_ell = np.arange(4, dtype=np.float64)
+    w_ell[-1] = 0
+    ops.wlm2alm(w_ell, wlm, alm, 2, 3)
+    np.testing.assert_allclose(alm[0], [1, 1 + 1j, 1 + 4j, 1, 1 + 3j, 1 + 8j, 1, 1 + 10j, 1, 1])
+    np.testing.assert_allclose(alm[2], [3, 3 + 100j, 3 + 400j, 3, 3 + 300j, 3 + 800j, 3,
+                                        3 + 1000j, 3, 3])
+
+
+def test_eigpow_golden():
+    '''test_mat_c_utils.py:16-46.'''
+    m = np.asarray([[5, -4, 0], [-4, 5, 0], [0, 0, 9]], dtype=np.float64)[:, :, None]
+    out = ops.eigpow(m, 0.5)
+    np.testing.assert_allclose(out[:, :, 0], [[2, -1, 0], [-1, 2, 0], [0, 0, 3]], atol=1e-12)
+
+
+def test_beam_and_lowpass_golden():
+    '''test_noise_utils.py:120-135, test_multigrid.py:12-18.'''
+    b = ops.band_limit_gauss_beam(10)
+    assert abs(b[-1] / 0.019927032253073726 - 1) < 1e-7
+    b = ops.band_limit_gauss_beam(10, fwhm=np.radians(10))
+    assert abs(b[-1] / 0.73923778 - 1) < 1e-7
+    r = ops.lowpass_filter(10)
+    assert abs(r[0] - 1) < 1e-15
+    assert abs(r[5] ** 2 - 0.05) < 1e-12
+
+
+@pytest.mark.skipif(not os.path.exists(REF_LIB), reason='reference C library not built')
+class TestAgainstReferenceC:
+    '''The oracle's lmul / trunc / wlm2alm / ring weight against the
+    reference's own C sources compiled by oracle/Makefile.'''
+
+    @pytest.fixture(scope='class')
+    def ref(self):
+        return ctypes.CDLL(REF_LIB)
+
+    @pytest.mark.parametrize('ncomp', [1, 3])
+    @pytest.mark.parametrize('diag', [False, True])
+    def test_lmul(self, ref, ncomp, diag):
+        lmax = 23
+        rng = np.random.default_rng(7)
+        n = layout.nelem(lmax)
+        a = rng.normal(size=(ncomp, n)) + 1j * rng.normal(size=(ncomp, n))
+        mat = rng.normal(size=(ncomp, lmax + 1) if diag else (ncomp, ncomp, lmax + 1))
+        out = np.zeros_like(a)
+        fn = ref.lmul_diag_dp if diag else ref.lmul_dp
+        fn(mat.ctypes.data_as(ctypes.c_void_p), a.ctypes.data_as(ctypes.c_void_p),
+           out.ctypes.data_as(ctypes.c_void_p), ctypes.c_int(lmax), ctypes.c_int(ncomp))
+        np.testing.assert_allclose(ops.lmul(a, mat, lmax), out, rtol=1e-15, atol=1e-14)
+
+    def test_lmul_sp(self, ref):
+        lmax, ncomp = 17, 3
+        rng = np.random.default_rng(8)
+        n = layout.nelem(lmax)
+        a = (rng.normal(size=(ncomp, n)) + 1j * rng.normal(size=(ncomp, n))).astype(np.complex64)
+        mat = rng.normal(size=(ncomp, ncomp, lmax + 1)).astype(np.float32)
+        out = np.zeros_like(a)
+        ref.lmul_sp(mat.ctypes.data_as(ctypes.c_void_p), a.ctypes.data_as(ctypes.c_void_p),
+                    out.ctypes.data_as(ctypes.c_void_p), ctypes.c_int(lmax), ctypes.c_int(ncomp))
+        np.testing.assert_allclose(ops.lmul(a, mat, lmax), out, rtol=1e-5, atol=1e-5)
+
+    def test_trunc_and_wlm2alm(self, ref):
+        lmax, lmax_out, ncomp = 19, 8, 2
+        rng = np.random.default_rng(9)
+        a = rng.normal(size=(ncomp, layout.nelem(lmax))) + 0j
+        out = np.zeros((ncomp, layout.nelem(lmax_out)), dtype=np.complex128)
+        ref.trunc_alm_dp(a.ctypes.data_as(ctypes.c_void_p), out.ctypes.data_as(ctypes.c_void_p),
+                         ctypes.c_int(lmax), ctypes.c_int(lmax_out), ctypes.c_int(ncomp))
+        np.testing.assert_array_equal(ops.trunc_alm(a, lmax, lmax_out), out)
+        w = rng.normal(size=lmax_out + 1)
+        wlm = rng.normal(size=(ncomp, layout.nelem(lmax_out))) + 1j
+        acc = a.copy()
+        ref.wlm2alm_dp(w.ctypes.data_as(ctypes.c_void_p), wlm.ctypes.data_as(ctypes.c_void_p),
+                       acc.ctypes.data_as(ctypes.c_void_p), ctypes.c_int(lmax_out),
+                       ctypes.c_int(lmax), ctypes.c_int(ncomp))
+        mine = ops.wlm2alm(w, wlm, a.copy(), lmax_out, lmax)
+        np.testing.assert_allclose(mine, acc, rtol=1e-15, atol=1e-15)
+
+    def test_ringweight(self, ref):
+        g = og.gauss_legendre(9, 17)
+        f = np.random.default_rng(10).normal(size=g.npix)
+        out = f.copy()
+        nphi = g.nphi.astype(np.int64)
+        offs = g.offsets.astype(np.int64)
+        stride = np.ones(g.nrow, dtype=np.int64)
+        ref._apply_ringweight_core_dp(out.ctypes.data_as(ctypes.c_void_p),
+                                      g.weight.ctypes.data_as(ctypes.c_void_p),
+                                      nphi.ctypes.data_as(ctypes.c_void_p),
+                                      offs.ctypes.data_as(ctypes.c_void_p),
+                                      stride.ctypes.data_as(ctypes.c_void_p),
+                                      ctypes.c_long(g.nrow))
+        np.testing.assert_array_equal(osht.ringweight(f.copy(), g), out)
+
+
+# ------------------------------------------------------------------ CG
+
+def test_cg_closed_forms():
+    '''test_solvers.py:45-111 with the oracle solver.'''
+    alm = np.asarray([1, 2, 3, 4j, 5j, 6])
+    s = ow.WienerMap(alm, lambda a: 2 * a, lambda a: 3 * a, (lambda a: a, lambda a: a))
+    cg = s.init_solver()
+    b_in = cg.b.copy()
+    cg.step()
+    np.testing.assert_allclose(cg.x, alm * 3 / 5, atol=1e-14)
+    np.testing.assert_allclose(cg.A(cg.x), b_in, atol=1e-13)
+    assert abs(s.get_qform() - (0.5 * 168 * (3 / 5) ** 2 * 5 - 168 * 3 * (3 / 5))) < 1e-10
+    s = ow.WienerMap(np.ones(6), lambda a: 2 * a, lambda a: 3 * a, (lambda a: a, lambda a: a),
+                     beam=lambda a: 0.1 * a, rand_isignal=np.arange(6.),
+                     rand_inoise=np.arange(6.)[::-1])
+    cg = s.init_solver()
+    cg.step()
+    np.testing.assert_allclose(cg.x, (0.3 + np.arange(6) + 0.1 * np.arange(6)[::-1]) / 2.03,
+                               atol=1e-14)
+
+
+def test_pinv_exact_fullsky():
+    '''Full-sky white noise: the pseudo-inverse equals A^-1 (one CG step).'''
+    lmax = 12
+    g = og.gauss_legendre(lmax + 1, 2 * lmax + 1)
+    ell = np.arange(lmax + 1)
+    cl = np.zeros(lmax + 1)
+    cl[2:] = 1e3 * (ell[2:] + 10.) ** -2.5
+    icov_ell = np.zeros((1, lmax + 1))
+    icov_ell[0, 2:] = 1 / cl[2:]
+    icov_pix = np.repeat(g.weight, g.nphi)[None] / 4.
+    imap = np.random.default_rng(11).normal(size=(1, g.npix))
+    s = ow.WienerMap.from_arrays(imap, g, lmax, icov_ell, icov_pix)
+    s.add_preconditioner(ow.PseudoInvPreconditioner(lmax, icov_ell, icov_pix, g, 0))
+    s.init_solver()
+    s.cg.step()
+    b = np.sqrt(ops.contract_almxblm(s.b0, s.b0))
+    assert s.get_residual() / b < 1e-12
+
+
+def test_multigrid_levels():
+    '''test_multigrid.py:20-84 properties on the oracle.'''
+    lmax = 22
+    ells = np.arange(lmax + 1)
+    icov = np.ones((3, 3, lmax + 1)) * np.asarray(
+        [[1, 0.1, 0.1], [0.1, 1, 0.1], [0.1, 0.1, 1]])[:, :, None]
+    icov *= 1e-4 * (ells + 1) ** 2
+    icov *= ops.gauss_beam(2 * np.pi / lmax, lmax)
+    icov[:, :, :2] = 0
+    g = og.gauss_minfo(2 * lmax)
+    mask = np.ones(g.npix, dtype=bool)
+    m2 = og.view_2d(mask, g)
+    m2[0:5, :] = False
+    m2[21:, :] = False
+    levels = ow.get_levels(mask, g, icov, [0, 2], min_pix=10, lmax_r_ell=lmax)
+    assert len(levels) == 3
+    assert [og.minfo2lmax(lv.geom) for lv in levels] == [lmax, lmax // 2, lmax // 4]
+    assert levels[0].pinv_g is None and levels[2].pinv_g is not None
