@@ -24,7 +24,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from optweight_amd import (solvers, preconditioners, workloads, alm_utils, sht,  # noqa: E402
-                           _lib)
+                           mixed, _lib)
 
 PEAK_FP64_TFLOPS = 78.6
 PEAK_FP32_TFLOPS = 157.3
@@ -54,21 +54,28 @@ def true_residual(solver, x):
                                                                                  solver.b0))
 
 
-def solve_fp64(solver, tol=1e-6, maxiter=500):
-    '''Plain float64 preconditioned CG from x = 0 until the recursive
-    residual norm drops below tol ||b|| (checked every iteration on the
-    device scalar).'''
-    solver.set_b_vec(solver.imap)
-    solver.init_solver()
-    bnorm2 = alm_utils.contract_almxblm_dev(solver.b0, solver.b0)
-    thr = (tol ** 2) * float(bnorm2.item())
-    rr = torch.empty((), dtype=torch.float64, device=solver.b0.device)
-    while solver.i < maxiter:
-        solver.step()
-        alm_utils.contract_almxblm_dev(solver.r, solver.r, out=rr)
-        if float(rr.item()) <= thr:
-            break
-    return solver.x, solver.i
+def run_solver(kind, solver, tol=1e-6):
+    b = solver.b0
+    if kind == 'fp64':
+        x, it = mixed.solve_fp64(solver.a_matrix, b, solver.preconditioner, tol=tol)
+        return x, {'iterations': it}
+    x, info = mixed.solve_mixed(solver.a_matrix, b, solver.preconditioner, tol=tol)
+    return x, {'iterations': info.iterations, 'replacements': info.replacements}
+
+
+def time_solves(kind, solver, steps, warmup):
+    for w in range(warmup):
+        x, info = run_solver(kind, solver)
+        log(f'{kind} warmup solve {w}: {info}, true residual {true_residual(solver, x):.3e}')
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    infos = []
+    for _ in range(steps):
+        x, info = run_solver(kind, solver)
+        infos.append(info)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    return x, elapsed, infos
 
 
 def time_kernel(fn, reps=10):
@@ -137,22 +144,22 @@ def main():
     solver = build_fp64_solver(prob, dev, use_mg=not args.no_mg)
     log(f'setup {time.perf_counter() - t0:.1f}s')
 
-    for w in range(args.warmup):
-        x, it = solve_fp64(solver)
-        log(f'warmup solve {w}: {it} iterations, true residual {true_residual(solver, x):.3e}')
-
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    iters = []
-    for _ in range(args.steps):
-        x, it = solve_fp64(solver)
-        iters.append(it)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    resid = true_residual(solver, x)
-    fp64_sps = args.steps / elapsed
-    log(f'fp64: {args.steps} solves in {elapsed:.3f}s, {np.mean(iters):.1f} it/solve, '
-        f'residual {resid:.3e}')
+    x64, el64, inf64 = time_solves('fp64', solver, args.steps, args.warmup)
+    res64 = true_residual(solver, x64)
+    fp64_sps = args.steps / el64
+    it64 = float(np.mean([i['iterations'] for i in inf64]))
+    log(f'fp64: {args.steps} solves in {el64:.3f}s ({fp64_sps:.3f}/s), {it64:.1f} it/solve, '
+        f'true residual {res64:.3e}')
+    xmx, elmx, infmx = time_solves('mixed', solver, args.steps, args.warmup)
+    resmx = true_residual(solver, xmx)
+    mixed_sps = args.steps / elmx
+    itmx = float(np.mean([i['iterations'] for i in infmx]))
+    diff = float(torch.linalg.vector_norm(xmx - x64) / torch.linalg.vector_norm(x64))
+    log(f'mixed: {args.steps} solves in {elmx:.3f}s ({mixed_sps:.3f}/s), {itmx:.1f} it/solve, '
+        f'true residual {resmx:.3e}, rel L2 vs fp64 solution {diff:.3e}, '
+        f'speedup {mixed_sps / fp64_sps:.2f}x')
+    elapsed = elmx
+    resid = resmx
 
     roof = legendre_roofline(prob, dev)
     log(f'roofline: {roof}')
@@ -163,17 +170,17 @@ def main():
         # per-solve CPU estimate: the oracle's A time scaled by the GPU
         # solve's cost in A applications
         a_ms = time_kernel(lambda: solver.a_matrix(solver.b0), reps=5)
-        solve_in_a = (elapsed / args.steps) / (a_ms * 1e-3)
+        solve_in_a = (el64 / args.steps) / (a_ms * 1e-3)
         cpu = {'value': 1.0 / (t_a * solve_in_a), 'unit': 'solves/s', 'cores': 1,
                'kind': 'port',
                'sample': (f'one float64 A application of the oracle at TQU lmax {args.lmax} '
-                          f'({t_a:.1f}s), scaled by the GPU solve cost in A applications '
-                          f'({solve_in_a:.1f})')}
+                          f'({t_a:.1f}s), scaled by the float64 GPU solve cost in A '
+                          f'applications ({solve_in_a:.1f})')}
         log(f'cpu baseline: {cpu}')
 
     out = {
         'metric': 'Wiener solves/sec, TQU lmax=1023 to resid 1e-6: mixed/batched vs FP64 GPU',
-        'value': fp64_sps,
+        'value': mixed_sps,
         'unit': 'solves/s',
         'n_gpus': 1,
         'steps': args.steps,
@@ -182,13 +189,17 @@ def main():
         'higher_is_better': True,
         'scaling': 'weak',
         'vs_baseline': None,
-        'dtype': 'f64',
+        'dtype': 'f32+f64',
         'data': 'synthetic',
         'config': {'workload': f'TQU Wiener filter, lmax {args.lmax}, GL {args.lmax + 1}x'
                                f'{2 * args.lmax + 1}, pinv + multigrid(T) preconditioner, '
-                               f'resid 1e-6',
-                   'solver': 'fp64', 'iterations': float(np.mean(iters)),
-                   'true_residual': resid, 'fp64_solves_per_s': fp64_sps},
+                               f'true resid 1e-6',
+                   'solver': 'mixed (f32 CG, f64 reliable updates)',
+                   'iterations': itmx, 'true_residual': resmx,
+                   'rel_l2_vs_fp64_solution': diff,
+                   'fp64_solves_per_s': fp64_sps, 'fp64_iterations': it64,
+                   'fp64_true_residual': res64,
+                   'speedup_vs_fp64': mixed_sps / fp64_sps},
         'roofline': {'bound': 'mfma', 'achieved': roof['achieved'],
                      'peak': PEAK_FP64_TFLOPS, 'unit': 'TFLOP/s',
                      'frac': roof['achieved'] / PEAK_FP64_TFLOPS, 'traffic': None,

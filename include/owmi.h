@@ -102,14 +102,16 @@ int owmi_apply_ringweight(int dtype, void *imap, const double *weight,
                           void *stream);
 
 /*
- * Pixel-diagonal matrix-vector product, out may equal in:
- *   kind 0: out[i, p] = mat[i, p] in[i, p]                 (mat (ncomp, npix))
- *   kind 1: out[i, p] = sum_j mat[i, j, p] in[j, p]        (mat (ncomp, ncomp, npix))
+ * Pixel-diagonal matrix-vector product on ntrans maps sharing one matrix,
+ * out may equal in:
+ *   kind 0: out[t, i, p] = mat[i, p] in[t, i, p]           (mat (ncomp, npix))
+ *   kind 1: out[t, i, p] = sum_j mat[i, j, p] in[t, j, p]  (mat (ncomp, ncomp, npix))
  * Replaces mat_utils.matvec (optweight/mat_utils.py:475-532) as used by
  * operators.PixMatVecMap (operators.py:162-204).
  */
 int owmi_pix_matvec(int dtype, const void *mat, int kind, const void *in,
-                    void *out, int ncomp, long long npix, void *stream);
+                    void *out, int ncomp, long long npix, long long ntrans,
+                    void *stream);
 
 /* ------------------------------------------------------------------ */
 /* Conjugate-gradient vector updates (scalars stay on the device)      */
@@ -128,6 +130,15 @@ int owmi_cg_xr(int dtype, void *x, void *r, const void *p, const void *q,
 /* beta = *dev_num / *dev_den; p = z + beta p over n complex elements. */
 int owmi_cg_p(int dtype, void *p, const void *z, const double *dev_num,
               const double *dev_den, long long n, void *stream);
+
+/*
+ * Mixed-precision x/r update of a reliable-update CG: alpha = *dev_num /
+ * *dev_den; x += alpha p accumulated in x's precision (dtype_x), r -= alpha q
+ * in dtype_r (p, q and r share dtype_r); n complex elements.
+ */
+int owmi_cg_xr_mixed(int dtype_x, void *x, int dtype_r, void *r, const void *p,
+                     const void *q, const double *dev_num, const double *dev_den,
+                     long long n, void *stream);
 
 /* y = a x + b y over n complex elements with host scalars a, b. */
 int owmi_axpby(int dtype, double a, const void *x, double b, void *y,

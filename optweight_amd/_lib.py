@@ -29,7 +29,8 @@ _SIGNATURES = {
     'owmi_wlm2alm': (_c_int, [_c_int, _vp, _vp, _vp, _c_int, _c_int, _c_ll, _vp]),
     'owmi_alm_dot': (_c_int, [_c_int, _vp, _vp, _c_int, _c_ll, _vp, _vp]),
     'owmi_apply_ringweight': (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _c_ll, _c_ll, _c_ll, _vp]),
-    'owmi_pix_matvec': (_c_int, [_c_int, _vp, _c_int, _vp, _vp, _c_int, _c_ll, _vp]),
+    'owmi_pix_matvec': (_c_int, [_c_int, _vp, _c_int, _vp, _vp, _c_int, _c_ll, _c_ll, _vp]),
+    'owmi_cg_xr_mixed': (_c_int, [_c_int, _vp, _c_int, _vp, _vp, _vp, _vp, _vp, _c_ll, _vp]),
     'owmi_cg_xr': (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _vp, _c_ll, _vp]),
     'owmi_cg_p': (_c_int, [_c_int, _vp, _vp, _vp, _vp, _c_ll, _vp]),
     'owmi_axpby': (_c_int, [_c_int, _c_dbl, _vp, _c_dbl, _vp, _c_ll, _vp]),

@@ -177,20 +177,24 @@ __global__ void ringweight_kernel(T *imap, const double *__restrict__ weight,
 template <typename T>
 __global__ void pix_diag_kernel(const T *__restrict__ mat, const T *in, T *out, long long n) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = mat[i] * in[i];
+  long long t = blockIdx.y;
+  if (i < n) out[t * n + i] = mat[i] * in[t * n + i];
 }
 
 template <typename T>
 __global__ void pix_full_kernel(const T *__restrict__ mat, const T *in, T *out, int nc,
                                 long long npix) {
   long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long t = blockIdx.y;
   if (p >= npix) return;
+  const T *src = in + t * nc * npix;
+  T *dst = out + t * nc * npix;
   T v[16];
-  for (int j = 0; j < nc; ++j) v[j] = in[j * npix + p];
+  for (int j = 0; j < nc; ++j) v[j] = src[j * npix + p];
   for (int i = 0; i < nc; ++i) {
     T acc = 0;
     for (int j = 0; j < nc; ++j) acc += mat[(i * nc + j) * npix + p] * v[j];
-    out[i * npix + p] = acc;
+    dst[i * npix + p] = acc;
   }
 }
 
@@ -263,6 +267,21 @@ __global__ void cg_xr_kernel(cplx_t<T> *x, cplx_t<T> *r, const cplx_t<T> *p,
   cplx_t<T> pv = p[i], qv = q[i], xv = x[i], rv = r[i];
   x[i] = {xv.x + alpha * pv.x, xv.y + alpha * pv.y};
   r[i] = {rv.x - alpha * qv.x, rv.y - alpha * qv.y};
+}
+
+// x (precision X) += alpha p; r (precision R) -= alpha q; alpha in double.
+template <typename X, typename R>
+__global__ void cg_xr_mixed_kernel(cplx_t<X> *x, cplx_t<R> *r, const cplx_t<R> *p,
+                                   const cplx_t<R> *q, const double *num, const double *den,
+                                   long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double alpha = *num / *den;
+  cplx_t<R> pv = p[i], qv = q[i], rv = r[i];
+  cplx_t<X> xv = x[i];
+  x[i] = {(X)(xv.x + alpha * (double)pv.x), (X)(xv.y + alpha * (double)pv.y)};
+  const R a = (R)alpha;
+  r[i] = {rv.x - a * qv.x, rv.y - a * qv.y};
 }
 
 template <typename T>
@@ -386,27 +405,55 @@ int owmi_apply_ringweight(int dtype, void *imap, const double *weight, const lon
 }
 
 int owmi_pix_matvec(int dtype, const void *mat, int kind, const void *in, void *out, int ncomp,
-                    long long npix, void *stream) {
+                    long long npix, long long ntrans, void *stream) {
   return guard([&] {
     check_dtype(dtype);
     OWMI_REQUIRE(ncomp >= 1 && ncomp <= 16, "ncomp must be in [1, 16]");
+    OWMI_REQUIRE(kind == 0 || kind == 1, "kind must be 0 (diagonal) or 1 (full)");
+    OWMI_REQUIRE(ntrans >= 0 && ntrans < 65536, "ntrans must be in [0, 65535]");
+    if (ntrans == 0 || npix == 0) return;
     hipStream_t st = as_stream(stream);
     if (kind == 0) {
       long long n = npix * ncomp;
+      dim3 g(ceil_div(n, 256), (unsigned)ntrans);
       if (dtype == OWMI_F64)
-        hipLaunchKernelGGL(pix_diag_kernel<double>, dim3(ceil_div(n, 256)), dim3(256), 0, st,
+        hipLaunchKernelGGL(pix_diag_kernel<double>, g, dim3(256), 0, st,
                            (const double *)mat, (const double *)in, (double *)out, n);
       else
-        hipLaunchKernelGGL(pix_diag_kernel<float>, dim3(ceil_div(n, 256)), dim3(256), 0, st,
+        hipLaunchKernelGGL(pix_diag_kernel<float>, g, dim3(256), 0, st,
                            (const float *)mat, (const float *)in, (float *)out, n);
     } else {
+      dim3 g(ceil_div(npix, 256), (unsigned)ntrans);
       if (dtype == OWMI_F64)
-        hipLaunchKernelGGL(pix_full_kernel<double>, dim3(ceil_div(npix, 256)), dim3(256), 0, st,
+        hipLaunchKernelGGL(pix_full_kernel<double>, g, dim3(256), 0, st,
                            (const double *)mat, (const double *)in, (double *)out, ncomp, npix);
       else
-        hipLaunchKernelGGL(pix_full_kernel<float>, dim3(ceil_div(npix, 256)), dim3(256), 0, st,
+        hipLaunchKernelGGL(pix_full_kernel<float>, g, dim3(256), 0, st,
                            (const float *)mat, (const float *)in, (float *)out, ncomp, npix);
     }
+  });
+}
+
+int owmi_cg_xr_mixed(int dtype_x, void *x, int dtype_r, void *r, const void *p, const void *q,
+                     const double *dev_num, const double *dev_den, long long n, void *stream) {
+  return guard([&] {
+    check_dtype(dtype_x);
+    check_dtype(dtype_r);
+    if (n == 0) return;
+    dim3 g(ceil_div(n, 256));
+    hipStream_t st = as_stream(stream);
+    if (dtype_x == OWMI_F64 && dtype_r == OWMI_F32)
+      hipLaunchKernelGGL((cg_xr_mixed_kernel<double, float>), g, dim3(256), 0, st, (double2 *)x,
+                         (float2 *)r, (const float2 *)p, (const float2 *)q, dev_num, dev_den, n);
+    else if (dtype_x == OWMI_F64 && dtype_r == OWMI_F64)
+      hipLaunchKernelGGL((cg_xr_mixed_kernel<double, double>), g, dim3(256), 0, st,
+                         (double2 *)x, (double2 *)r, (const double2 *)p, (const double2 *)q,
+                         dev_num, dev_den, n);
+    else if (dtype_x == OWMI_F32 && dtype_r == OWMI_F32)
+      hipLaunchKernelGGL((cg_xr_mixed_kernel<float, float>), g, dim3(256), 0, st, (float2 *)x,
+                         (float2 *)r, (const float2 *)p, (const float2 *)q, dev_num, dev_den, n);
+    else
+      throw Error("x must be at least as precise as r");
   });
 }
 

@@ -110,25 +110,28 @@ def matvec(mat, vec, inplace=False):
     _lib.require_cuda(vec)
     if not isinstance(mat, torch.Tensor):
         mat = torch.as_tensor(np.asarray(mat), device=vec.device, dtype=vec.dtype)
-    _lib.require_cuda(mat)
     if mat.dtype != vec.dtype:
-        raise ValueError(f'matrix dtype {mat.dtype} != vector dtype {vec.dtype}')
+        mat = mat.to(vec.dtype)
+    mat = mat.contiguous()
+    _lib.require_cuda(mat)
+    st = _lib.stream_ptr(vec.device)
+    code = _lib.dtype_code(vec)
     out = vec if inplace else torch.empty_like(vec)
     if tuple(mat.shape) == tuple(vec.shape):
-        v2 = vec.reshape(1, -1)
-        _lib.call('owmi_pix_matvec', _lib.dtype_code(vec), _lib.ptr(mat), 0, _lib.ptr(vec),
-                  _lib.ptr(out), 1, v2.shape[-1], _lib.stream_ptr(vec.device))
+        _lib.call('owmi_pix_matvec', code, _lib.ptr(mat), 0, _lib.ptr(vec), _lib.ptr(out), 1,
+                  vec.numel(), 1, st)
+        return out
+    if vec.ndim >= 2 and tuple(mat.shape) == tuple(vec.shape[-2:]):
+        ntrans = vec.numel() // mat.numel()
+        _lib.call('owmi_pix_matvec', code, _lib.ptr(mat), 0, _lib.ptr(vec), _lib.ptr(out), 1,
+                  mat.numel(), ntrans, st)
         return out
     npol, npix = vec.shape[-2], vec.shape[-1]
     if tuple(mat.shape) != (npol, npol, npix):
         raise ValueError(f'matrix shape : {tuple(mat.shape)} does not match up with vector '
                          f'shape : {tuple(vec.shape)}')
-    if inplace:
-        tmp = torch.empty_like(vec)
-        _lib.call('owmi_pix_matvec', _lib.dtype_code(vec), _lib.ptr(mat), 1, _lib.ptr(vec),
-                  _lib.ptr(tmp), npol, npix, _lib.stream_ptr(vec.device))
-        vec.copy_(tmp)
-        return vec
-    _lib.call('owmi_pix_matvec', _lib.dtype_code(vec), _lib.ptr(mat), 1, _lib.ptr(vec),
-              _lib.ptr(out), npol, npix, _lib.stream_ptr(vec.device))
+    ntrans = vec.numel() // (npol * npix)
+    # each thread reads all components of its pixel before writing: in-place safe
+    _lib.call('owmi_pix_matvec', code, _lib.ptr(mat), 1, _lib.ptr(vec), _lib.ptr(out), npol,
+              npix, ntrans, st)
     return out

@@ -177,8 +177,8 @@ class TestReferenceClosedForms:
 
     def test_constrained(self, dev):
         alm_data = torch.ones(6, dtype=torch.complex128, device=dev)
-        ris = torch.arange(6, dtype=torch.complex128, device=dev)
-        rin = torch.flip(torch.arange(6, dtype=torch.complex128, device=dev), [0])
+        ris = torch.arange(6, dtype=torch.float64).to(torch.complex128).to(dev)
+        rin = torch.flip(ris, [0])
         solver = solvers.CGWienerMap(alm_data, lambda a: 2 * a, lambda a: 3 * a,
                                      (lambda a: a, lambda a: a), beam=lambda a: 0.1 * a,
                                      rand_isignal=ris, rand_inoise=rin)
@@ -189,7 +189,7 @@ class TestReferenceClosedForms:
         np.testing.assert_allclose(solver.x.cpu().numpy(), x_exp.cpu().numpy(), atol=1e-12)
 
     def test_add_preconditioner(self, dev):
-        alm_data = torch.arange(12, dtype=torch.complex64, device=dev).reshape(2, 6)
+        alm_data = torch.arange(12, dtype=torch.float32).to(torch.complex64).to(dev).reshape(2, 6)
         solver = solvers.CGWienerMap(alm_data, lambda a: a, lambda a: a,
                                      (lambda a: a, lambda a: a))
         assert solver.preconditioner is None
