@@ -8,26 +8,36 @@
 
 namespace owmi {
 
-// Exponent step of the scaled Legendre recurrence: a value is stored as
-// v * 2^(kScale * k), k <= 0. k == 0 is the plain value; k == -1 values are
-// used with the factor 2^-kScale; k <= -2 values are below 2^-2kScale of
-// the largest values and contribute nothing.
+// Exponent step of the scaled Legendre recurrence used at plan creation to
+// locate the first non-negligible multipole: a value is stored as
+// v * 2^(kScale * k), k <= 0.
 template <typename T> struct ScaleStep;
 template <> struct ScaleStep<double> { static constexpr int bits = 128; };
 template <> struct ScaleStep<float> { static constexpr int bits = 32; };
 
+// Functions below this magnitude are dropped: |slam_lm| < 1e-30 for every
+// l below the first significant multipole l_f(m, ring) (the functions grow
+// monotonically in l there), far below float64 rounding of O(1) terms.
+constexpr double kNegligible = 1e-30;
+
 // Device tables of one precision.
+//
+// Recurrences are stored normalised: slam_l = c_l mu_l with
+//   mu_l = (A'_l x + B'_l) mu_{l-1} - mu_{l-2},
+// c_l = C_l c_{l-2} (c = 1 at the first two multipoles). Each (m, ring pair,
+// function) starts at its first significant multipole l_f with the seeds
+// mu_{l_f - 1}, mu_{l_f}; l_f = lmax + 1 means negligible throughout.
 template <typename T>
 struct PlanTables {
   T *x = nullptr;          // cos(theta_north) per pair [npairs]
-  T *seed0 = nullptr;      // spin-0 slam_mm value per (m, pair) [(mmax+1) * npairs]
-  int *seed0k = nullptr;   // its scale exponent
-  T *seedp = nullptr;      // spin +2 value at l = max(m, 2) per (m, pair)
-  int *seedpk = nullptr;
-  T *seedm = nullptr;      // spin -2 value at l = max(m, 2)
-  int *seedmk = nullptr;
-  T *coef0 = nullptr;      // spin-0 recurrence (a, c) per alm index [2 * nelem]
-  T *coef2 = nullptr;      // spin-2 recurrence (a, b, c) per alm index [3 * nelem]
+  int *lf0 = nullptr;      // spin-0 l_f per (m, pair) [(mmax+1) * npairs]
+  T *s0 = nullptr;         // spin-0 (mu_{lf-1}, mu_lf) per (m, pair)
+  int *lfp = nullptr;      // spin +2
+  T *sp = nullptr;
+  int *lfm = nullptr;      // spin -2
+  T *sm = nullptr;
+  T *coef0 = nullptr;      // spin-0 (A', c) per alm index [2 * nelem]
+  T *coef2 = nullptr;      // spin-2 (A', B', c) per alm index [3 * nelem]
   T *twiddle = nullptr;    // exp(-2 pi i k / Mmax), k < Mmax/2, complex [Mmax]
   T *chirp = nullptr;      // per distinct ring length: exp(-i pi n^2 / N), complex
   T *bhat = nullptr;       // per distinct ring length: FFT_M(conj chirp, wrapped) / M
@@ -71,6 +81,15 @@ struct owmi_plan_impl {
 
   size_t nchirp = 0, nbhat = 0;  // complex element counts of the tables
 };
+
+// Plan-time search of the first significant multipole (legendre.hip):
+// from seeds v * 2^(128 k) at l = max(m, |s|) and unnormalised recurrence
+// coefficients (spin 0: A, C; spin 2: A, B, C per alm index), writes l_f and
+// the float64 values slam_{l_f - 1}, slam_{l_f} per (m, pair) and function.
+void launch_seed_finder(owmi_plan_impl *p, const double *xs, const double *v0, const int *k0,
+                        const double *vp, const int *kp, const double *vm, const int *km,
+                        const double *U0, const double *U2, int *lf0, double *s0, int *lfp,
+                        double *sp, int *lfm, double *sm);
 
 // Launchers (legendre.hip)
 template <typename T>

@@ -74,14 +74,14 @@ T *upload(const std::vector<T> &h) {
 
 template <typename T>
 void free_tables(PlanTables<T> &t) {
-  T **ptrs[] = {&t.x, &t.seed0, &t.seedp, &t.seedm, &t.coef0, &t.coef2, &t.twiddle,
+  T **ptrs[] = {&t.x, &t.s0, &t.sp, &t.sm, &t.coef0, &t.coef2, &t.twiddle,
                 &t.chirp, &t.bhat, &t.ringw, &t.phi0};
   for (T **p : ptrs)
     if (*p) {
       (void)hipFree(*p);
       *p = nullptr;
     }
-  int **iptrs[] = {&t.seed0k, &t.seedpk, &t.seedmk};
+  int **iptrs[] = {&t.lf0, &t.lfp, &t.lfm};
   for (int **p : iptrs)
     if (*p) {
       (void)hipFree(*p);
@@ -109,25 +109,28 @@ void *owmi_plan_impl::ensure_work(size_t bytes) {
   return work;
 }
 
-// Host-side table construction for one precision.
-template <typename T>
-static void build_tables(owmi_plan_impl &p, PlanTables<T> &t,
-                         const std::vector<LD> &xs, const std::vector<LD> &th_n,
-                         const std::vector<LD> &chirp, const std::vector<long long> &lens,
-                         const std::vector<int> &logms,
-                         const std::vector<long long> &chirp_off,
-                         const std::vector<long long> &bhat_off) {
+// Host-side tables shared by both precisions (long double), built once.
+struct HostTables {
+  // seeds at l = max(m, |s|) as v * 2^(128 k) (float64 v)
+  std::vector<double> v0, vp, vm;
+  std::vector<int> k0, kp, km;
+  // unnormalised recurrence coefficients (float64) for the seed search
+  std::vector<double> U0, U2;
+  // normalisation c_l and normalised coefficients per alm index
+  std::vector<LD> c0, A0, c2, A2, B2;
+  // search results: l_f and slam_{l_f - 1}, slam_{l_f} per (m, pair)
+  std::vector<int> lf0, lfp, lfm;
+  std::vector<double> s0, sp, sm;
+};
+
+static void build_host_tables(owmi_plan_impl &p, const std::vector<LD> &th_n,
+                              const std::vector<LD> &xs, HostTables &h) {
   const int lmax = p.lmax, mmax = p.mmax;
   const long long np = p.npairs;
   const long long nelem = nelem_of(lmax);
-
-  std::vector<T> hx(np);
-  for (long long i = 0; i < np; ++i) hx[i] = (T)xs[i];
-
-  // Seeds.
-  std::vector<T> s0((size_t)(mmax + 1) * np), sp((size_t)(mmax + 1) * np),
-      sm((size_t)(mmax + 1) * np);
-  std::vector<int> s0k(s0.size()), spk(s0.size()), smk(s0.size());
+  const size_t nseed = (size_t)(mmax + 1) * np;
+  h.v0.resize(nseed); h.vp.resize(nseed); h.vm.resize(nseed);
+  h.k0.resize(nseed); h.kp.resize(nseed); h.km.resize(nseed);
   const LD inv_sqrt_4pi = 1.0L / sqrtl(4.0L * kPi);
   const LD sqrt5_4pi = sqrtl(5.0L / (4.0L * kPi));
   for (long long i = 0; i < np; ++i) {
@@ -138,11 +141,12 @@ static void build_tables(owmi_plan_impl &p, PlanTables<T> &t,
     XLD a = XLD::of(inv_sqrt_4pi);
     for (int m = 0; m <= mmax; ++m) {
       if (m > 0) a = a.mul(-sqrtl((LD)(2 * m + 1) / (LD)(2 * m)) * st);
-      split_seed<T>(a, s0[(size_t)m * np + i], s0k[(size_t)m * np + i]);
+      split_seed<double>(a, h.v0[(size_t)m * np + i], h.k0[(size_t)m * np + i]);
     }
     // spin -2 at l = m >= 2: slam_22 = sqrt(5/4pi) cos^4(theta/2),
     // slam_mm = -sqrt((2m+1) 2m / ((m+2)(m-2))) (sin(theta)/2) slam_{m-1,m-1};
-    // spin +2 = spin -2 * tan^4(theta/2).
+    // spin +2 = spin -2 * tan^4(theta/2). For m < 2 the l = 2 values of
+    // (-1)^s sqrt(5/4pi) d^2_{m,-s}(theta).
     LD t4 = powl(sh / ch, 4);
     for (int m = 0; m <= mmax; ++m) {
       XLD vp, vm;
@@ -163,20 +167,28 @@ static void build_tables(owmi_plan_impl &p, PlanTables<T> &t,
         vm = a;
         vp = a.mul(t4);
       }
-      split_seed<T>(vp, sp[(size_t)m * np + i], spk[(size_t)m * np + i]);
-      split_seed<T>(vm, sm[(size_t)m * np + i], smk[(size_t)m * np + i]);
+      split_seed<double>(vp, h.vp[(size_t)m * np + i], h.kp[(size_t)m * np + i]);
+      split_seed<double>(vm, h.vm[(size_t)m * np + i], h.km[(size_t)m * np + i]);
     }
   }
 
-  // Recurrence coefficients: slam_l = (A x + B) slam_{l-1} - C slam_{l-2},
-  // (A, B, C) = (alpha, beta, gamma)_{l-1} of the d^l_{m,m'} recurrence.
-  std::vector<T> c0(2 * (size_t)nelem, 0), c2(3 * (size_t)nelem, 0);
+  // slam_l = (A x + B) slam_{l-1} - C slam_{l-2}, (A, B, C) = (alpha, beta,
+  // gamma)_{l-1} of the d^l_{m,m'} recurrence (m' = 0 or -2); normalised by
+  // c_l = C c_{l-2}, c = 1 at the first two multipoles of each function.
+  h.U0.assign(2 * (size_t)nelem, 0);
+  h.U2.assign(3 * (size_t)nelem, 0);
+  h.c0.assign((size_t)nelem, 1);
+  h.A0.assign((size_t)nelem, 0);
+  h.c2.assign((size_t)nelem, 1);
+  h.A2.assign((size_t)nelem, 0);
+  h.B2.assign((size_t)nelem, 0);
   for (int m = 0; m <= mmax; ++m) {
     long long ms = mstart_of(m, lmax);
     LD m2 = (LD)m * m;
     for (int s = 0; s <= 2; s += 2) {
       int lstart = std::max(m, s);
       LD mp2 = (LD)s * s;
+      std::vector<LD> &cc = s == 0 ? h.c0 : h.c2;
       for (int l = lstart + 1; l <= lmax; ++l) {
         LD L = l - 1;
         LD r_L = sqrtl((L * L - m2) * (L * L - mp2));
@@ -185,19 +197,100 @@ static void build_tables(owmi_plan_impl &p, PlanTables<T> &t,
         LD gamma = (l - 1 == lstart || l - 1 == 0)
                        ? 0.0L
                        : sqrtl((2 * L + 3) / (2 * L - 1)) * (L + 1) * r_L / (L * r_L1);
+        // s = +2 -> m' = -2: beta = +2 m sqrt((2L+3)(2L+1)) / (L r_{L+1})
+        LD beta = (s == 0 || m == 0) ? 0.0L
+                                     : 2.0L * m * sqrtl((2 * L + 3) * (2 * L + 1)) / (L * r_L1);
         size_t idx = (size_t)(ms + l);
+        LD c = (l == lstart + 1) ? 1.0L : gamma * cc[idx - 2];
+        cc[idx] = c;
+        LD ratio = cc[idx - 1] / c;
         if (s == 0) {
-          c0[2 * idx] = (T)alpha;
-          c0[2 * idx + 1] = (T)gamma;
+          h.U0[2 * idx] = (double)alpha;
+          h.U0[2 * idx + 1] = (double)gamma;
+          h.A0[idx] = alpha * ratio;
         } else {
-          // s = +2 -> m' = -2: beta = +2 m sqrt((2L+3)(2L+1)) / (L r_{L+1})
-          LD beta = (m == 0) ? 0.0L : 2.0L * m * sqrtl((2 * L + 3) * (2 * L + 1)) / (L * r_L1);
-          c2[3 * idx] = (T)alpha;
-          c2[3 * idx + 1] = (T)beta;
-          c2[3 * idx + 2] = (T)gamma;
+          h.U2[3 * idx] = (double)alpha;
+          h.U2[3 * idx + 1] = (double)beta;
+          h.U2[3 * idx + 2] = (double)gamma;
+          h.A2[idx] = alpha * ratio;
+          h.B2[idx] = beta * ratio;
         }
       }
     }
+  }
+
+  // First significant multipole and the seeds there (float64 device search).
+  double *d_v0 = upload(h.v0), *d_vp = upload(h.vp), *d_vm = upload(h.vm);
+  int *d_k0 = upload(h.k0), *d_kp = upload(h.kp), *d_km = upload(h.km);
+  double *d_U0 = upload(h.U0), *d_U2 = upload(h.U2);
+  std::vector<double> xd(np);
+  for (long long i = 0; i < np; ++i) xd[i] = (double)xs[i];
+  double *d_x = upload(xd);
+  int *d_lf0 = nullptr, *d_lfp = nullptr, *d_lfm = nullptr;
+  double *d_s0 = nullptr, *d_sp = nullptr, *d_sm = nullptr;
+  OWMI_HIP(hipMalloc(&d_lf0, nseed * sizeof(int)));
+  OWMI_HIP(hipMalloc(&d_lfp, nseed * sizeof(int)));
+  OWMI_HIP(hipMalloc(&d_lfm, nseed * sizeof(int)));
+  OWMI_HIP(hipMalloc(&d_s0, 2 * nseed * sizeof(double)));
+  OWMI_HIP(hipMalloc(&d_sp, 2 * nseed * sizeof(double)));
+  OWMI_HIP(hipMalloc(&d_sm, 2 * nseed * sizeof(double)));
+  launch_seed_finder(&p, d_x, d_v0, d_k0, d_vp, d_kp, d_vm, d_km, d_U0, d_U2, d_lf0, d_s0,
+                     d_lfp, d_sp, d_lfm, d_sm);
+  OWMI_HIP(hipDeviceSynchronize());
+  h.lf0.resize(nseed); h.lfp.resize(nseed); h.lfm.resize(nseed);
+  h.s0.resize(2 * nseed); h.sp.resize(2 * nseed); h.sm.resize(2 * nseed);
+  OWMI_HIP(hipMemcpy(h.lf0.data(), d_lf0, nseed * sizeof(int), hipMemcpyDeviceToHost));
+  OWMI_HIP(hipMemcpy(h.lfp.data(), d_lfp, nseed * sizeof(int), hipMemcpyDeviceToHost));
+  OWMI_HIP(hipMemcpy(h.lfm.data(), d_lfm, nseed * sizeof(int), hipMemcpyDeviceToHost));
+  OWMI_HIP(hipMemcpy(h.s0.data(), d_s0, 2 * nseed * sizeof(double), hipMemcpyDeviceToHost));
+  OWMI_HIP(hipMemcpy(h.sp.data(), d_sp, 2 * nseed * sizeof(double), hipMemcpyDeviceToHost));
+  OWMI_HIP(hipMemcpy(h.sm.data(), d_sm, 2 * nseed * sizeof(double), hipMemcpyDeviceToHost));
+  void *tmp[] = {d_x, d_v0, d_vp, d_vm, d_k0, d_kp, d_km, d_U0, d_U2,
+                 d_lf0, d_lfp, d_lfm, d_s0, d_sp, d_sm};
+  for (void *q : tmp) (void)hipFree(q);
+}
+
+// Per-precision device tables from the host tables.
+template <typename T>
+static void build_tables(owmi_plan_impl &p, PlanTables<T> &t, const HostTables &h,
+                         const std::vector<LD> &xs, const std::vector<LD> &chirp) {
+  const int lmax = p.lmax, mmax = p.mmax;
+  const long long np = p.npairs;
+  const long long nelem = nelem_of(lmax);
+  const size_t nseed = (size_t)(mmax + 1) * np;
+
+  std::vector<T> hx(np);
+  for (long long i = 0; i < np; ++i) hx[i] = (T)xs[i];
+
+  // seeds in normalised units mu = slam / c
+  std::vector<T> s0(2 * nseed), sp(2 * nseed), sm(2 * nseed);
+  for (int m = 0; m <= mmax; ++m) {
+    long long ms = mstart_of(m, lmax);
+    for (long long i = 0; i < np; ++i) {
+      size_t k = (size_t)m * np + i;
+      struct F { const std::vector<int> &lf; const std::vector<double> &sv;
+                 const std::vector<LD> &c; std::vector<T> &out; int lstart; };
+      F fs[3] = {{h.lf0, h.s0, h.c0, s0, m}, {h.lfp, h.sp, h.c2, sp, std::max(m, 2)},
+                 {h.lfm, h.sm, h.c2, sm, std::max(m, 2)}};
+      for (F &f : fs) {
+        int lf = f.lf[k];
+        T a = 0, b = 0;
+        if (lf <= lmax) {
+          b = (T)((LD)f.sv[2 * k + 1] / f.c[(size_t)(ms + lf)]);
+          if (lf - 1 >= f.lstart) a = (T)((LD)f.sv[2 * k] / f.c[(size_t)(ms + lf - 1)]);
+        }
+        f.out[2 * k] = a;
+        f.out[2 * k + 1] = b;
+      }
+    }
+  }
+  std::vector<T> c0(2 * (size_t)nelem), c2(3 * (size_t)nelem);
+  for (size_t i = 0; i < (size_t)nelem; ++i) {
+    c0[2 * i] = (T)h.A0[i];
+    c0[2 * i + 1] = (T)h.c0[i];
+    c2[3 * i] = (T)h.A2[i];
+    c2[3 * i + 1] = (T)h.B2[i];
+    c2[3 * i + 2] = (T)h.c2[i];
   }
 
   // Twiddles for the largest FFT length.
@@ -216,12 +309,12 @@ static void build_tables(owmi_plan_impl &p, PlanTables<T> &t,
   }
 
   t.x = upload(hx);
-  t.seed0 = upload(s0);
-  t.seed0k = upload(s0k);
-  t.seedp = upload(sp);
-  t.seedpk = upload(spk);
-  t.seedm = upload(sm);
-  t.seedmk = upload(smk);
+  t.lf0 = upload(h.lf0);
+  t.s0 = upload(s0);
+  t.lfp = upload(h.lfp);
+  t.sp = upload(sp);
+  t.lfm = upload(h.lfm);
+  t.sm = upload(sm);
   t.coef0 = upload(c0);
   t.coef2 = upload(c2);
   t.twiddle = upload(tw);
@@ -401,10 +494,12 @@ int owmi_sht_plan_create(owmi_sht_plan **out, long long nrings, const double *th
       p.fft_ring_a = upload(ja);
       p.fft_ring_b = upload(jb);
 
-      build_tables<double>(p, p.t64, xs, thn, chirp, lens, logms, chirp_off, bhat_off);
+      HostTables ht;
+      build_host_tables(p, thn, xs, ht);
+      build_tables<double>(p, p.t64, ht, xs, chirp);
       launch_bluestein_init(&p, lens, logms, chirp_off, bhat_off);
       OWMI_HIP(hipDeviceSynchronize());
-      build_tables<float>(p, p.t32, xs, thn, chirp, lens, logms, chirp_off, bhat_off);
+      build_tables<float>(p, p.t32, ht, xs, chirp);
       long long nb = 2 * (long long)p.nbhat;
       hipLaunchKernelGGL(bhat_to_float, dim3(ceil_div(nb, 256)), dim3(256), 0, 0,
                          p.t64.bhat, p.t32.bhat, nb);
