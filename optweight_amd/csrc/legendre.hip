@@ -300,16 +300,31 @@ synth2_kernel(LegArgs g, const T *__restrict__ xs, const int *__restrict__ lfps,
 }
 
 // --------------------------------------------------------------- adjoint
+//
+// One wavefront per work item: an m paired with lmax - m (uniform work per
+// item) and a group of 64 K ring pairs (K per lane). The wave sums over its
+// pairs in registers and across its lanes through a wave-private LDS region
+// after every CH multipoles; no block-wide barrier is used. Ring-pair groups
+// write separate partial sums that a second kernel adds in fixed order.
 
-constexpr int kAdjThreads = 256;
+constexpr int kAdjWaves = 4;  // independent waves per block
 
-// Sum of red[row][*] over the block: thread tid serves row tid / R (R =
-// blockDim / CH threads per row) and returns the row total in lanes with
-// tid % R == 0.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ int wave_min(int v) {
+  for (int off = 32; off >= 1; off >>= 1) v = min(v, __shfl_xor(v, off));
+  return v;
+}
+
+// Row totals of red[i][lane] (i < CH): lane L serves row L / R (R = 64 / CH
+// lanes per row); lanes with L % R == 0 return the row total.
 template <typename T, int CH>
-__device__ __forceinline__ cplx_t<T> row_sum(cplx_t<T> (*red)[kAdjThreads], int nthr) {
-  const int R = nthr / CH;
-  const int row = threadIdx.x / R, seg = threadIdx.x % R;
+__device__ __forceinline__ cplx_t<T> wave_rows(const cplx_t<T> (*red)[64], int lane) {
+  constexpr int R = 64 / CH;
+  const int row = lane / R, seg = lane % R;
   cplx_t<T> acc = {0, 0};
 #pragma unroll
   for (int j = 0; j < CH; ++j) {
@@ -317,6 +332,7 @@ __device__ __forceinline__ cplx_t<T> row_sum(cplx_t<T> (*red)[kAdjThreads], int 
     acc.x += v.x;
     acc.y += v.y;
   }
+#pragma unroll
   for (int off = R >> 1; off >= 1; off >>= 1) {
     acc.x += __shfl_xor(acc.x, off);
     acc.y += __shfl_xor(acc.y, off);
@@ -324,37 +340,44 @@ __device__ __forceinline__ cplx_t<T> row_sum(cplx_t<T> (*red)[kAdjThreads], int 
   return acc;
 }
 
-template <typename T, int K, int CH>
-__global__ void __launch_bounds__(kAdjThreads)
-adj0_kernel(LegArgs g, const T *__restrict__ xs, const int *__restrict__ lfs,
-            const T *__restrict__ seeds, const T *__restrict__ coef,
-            const cplx_t<T> *__restrict__ G, cplx_t<T> *alm, long long alm_ts) {
-  __shared__ T sA[CH];
-  __shared__ T sC[CH];
-  __shared__ cplx_t<T> red[CH][kAdjThreads];
-  __shared__ int smin;
-  const int m = blockIdx.x;
-  const long long t = blockIdx.y;
-  const int nthr = blockDim.x;
-  const int lmax = g.lmax;
-  const long long ms = mstart_of(m, lmax);
-  const T *cf = coef + 2 * ms;
-  cplx_t<T> *out = alm + t * alm_ts + ms;
-  const long long stride = g.mmax + 1;
-  const int R = nthr / CH;
-  const int row = threadIdx.x / R;
-  const bool writer = (threadIdx.x % R) == 0;
+struct AdjItems {
+  int nmpairs;        // ceil((mmax + 1) / 2)
+  int ngroups;        // ring-pair groups of 64 K pairs
+  long long grp_ts;   // elements between the partial buffers of two groups
+};
 
-  for (long long base = 0; base < g.npairs; base += (long long)nthr * K) {
+template <typename T, int K, int CH>
+__global__ void __launch_bounds__(64 * kAdjWaves)
+adj0_kernel(LegArgs g, AdjItems it, const T *__restrict__ xs, const int *__restrict__ lfs,
+            const T *__restrict__ seeds, const T *__restrict__ coef,
+            const cplx_t<T> *__restrict__ G, cplx_t<T> *out, long long out_ts) {
+  __shared__ cplx_t<T> red_all[kAdjWaves][CH][64];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const long long item = (long long)blockIdx.x * kAdjWaves + wave;
+  if (item >= (long long)it.nmpairs * it.ngroups) return;
+  cplx_t<T>(*red)[64] = red_all[wave];
+  const int q = (int)(item / it.ngroups), grp = (int)(item % it.ngroups);
+  const long long t = blockIdx.y;
+  const int lmax = g.lmax;
+  const long long stride = g.mmax + 1;
+  constexpr int R = 64 / CH;
+  const int row = lane / R;
+  const bool writer = (lane % R) == 0;
+
+  for (int pass = 0; pass < 2; ++pass) {
+    const int m = pass == 0 ? q : g.mmax - q;
+    if (pass == 1 && m == q) break;
+    const long long ms = mstart_of(m, lmax);
+    const T *cf = coef + 2 * ms;
+    cplx_t<T> *om = out + grp * it.grp_ts + t * out_ts + ms;
     T x[K], l1[K], l2[K], sa0[K], sb0[K];
     int lf[K];
     cplx_t<T> He[K], Ho[K];
     int mylf = INT_MAX;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      const long long ip = base + threadIdx.x + (long long)k * nthr;
-      x[k] = 0;
-      l1[k] = l2[k] = sa0[k] = sb0[k] = 0;
+      const long long ip = (long long)grp * 64 * K + lane + 64 * k;
+      x[k] = l1[k] = l2[k] = sa0[k] = sb0[k] = 0;
       lf[k] = INT_MAX;
       He[k] = Ho[k] = {0, 0};
       if (ip < g.npairs) {
@@ -378,97 +401,111 @@ adj0_kernel(LegArgs g, const T *__restrict__ xs, const int *__restrict__ lfs,
         mylf = min(mylf, lf[k]);
       }
     }
-    const int lmin = block_min(mylf, &smin);
-    if (base == 0)
-      for (int l = m + threadIdx.x; l < min(lmin, lmax + 1); l += nthr) out[l] = {0, 0};
+    const int lmin = __builtin_amdgcn_readfirstlane(wave_min(mylf));
+    for (int l = m + lane; l < min(lmin, lmax + 1); l += 64) om[l] = {0, 0};
 
     for (int l0 = lmin; l0 <= lmax; l0 += CH) {
       const int nl = min(CH, lmax - l0 + 1);
-      __syncthreads();
-      if (threadIdx.x < nl) {
-        sA[threadIdx.x] = cf[2 * (l0 + threadIdx.x)];
-        sC[threadIdx.x] = cf[2 * (l0 + threadIdx.x) + 1];
-      }
-      __syncthreads();
       bool inj = false;
 #pragma unroll
       for (int k = 0; k < K; ++k) inj |= (lf[k] >= l0 && lf[k] < l0 + nl);
-      const bool slow = __syncthreads_or(inj);
+      // wave-uniform coefficients of the chunk (clamped index: scalar loads)
+      T Ac[CH];
+#pragma unroll
+      for (int i = 0; i < CH; ++i) Ac[i] = cf[2 * min(l0 + i, lmax)];
+      // weights of the first and second multipole of the chunk (parity
+      // alternates with l, so the selection is made once per chunk)
+      const bool odd0 = (l0 - m) & 1;
+      cplx_t<T> H0[K], H1[K];
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        H0[k] = odd0 ? Ho[k] : He[k];
+        H1[k] = odd0 ? He[k] : Ho[k];
+      }
       cplx_t<T> s[CH];
 #pragma unroll
-      for (int i = 0; i < CH; ++i) {
-        s[i] = {0, 0};
-        if (i < nl) {
-          const int l = l0 + i;
-          const bool odd = (l - m) & 1;
+      for (int i = 0; i < CH; ++i) s[i] = {0, 0};
+      if (nl == CH && !__any(inj)) {
+        // steady state: plain recurrence over a full chunk
+#pragma unroll
+        for (int i = 0; i < CH; ++i) {
 #pragma unroll
           for (int k = 0; k < K; ++k) {
-            const T lam = fma(sA[i] * x[k], l1[k], -l2[k]);
-            if (slow && l == lf[k]) {
-              l2[k] = sa0[k];
-              l1[k] = sb0[k];
-            } else {
-              l2[k] = l1[k];
-              l1[k] = lam;
+            const T lam = fma(Ac[i] * x[k], l1[k], -l2[k]);
+            l2[k] = l1[k];
+            l1[k] = lam;
+            s[i] = cfma(lam, (i & 1) ? H1[k] : H0[k], s[i]);
+          }
+        }
+      } else {
+        // chunk holding a start multipole of some lane, or the last chunk
+#pragma unroll
+        for (int i = 0; i < CH; ++i) {
+          if (i < nl) {
+            const int l = l0 + i;
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+              const T lam = fma(Ac[i] * x[k], l1[k], -l2[k]);
+              const bool st = (l == lf[k]);
+              l2[k] = st ? sa0[k] : l1[k];
+              l1[k] = st ? sb0[k] : lam;
+              s[i] = cfma(l1[k], (i & 1) ? H1[k] : H0[k], s[i]);
             }
-            s[i] = cfma(l1[k], odd ? Ho[k] : He[k], s[i]);
           }
         }
       }
+      wave_sync();
 #pragma unroll
-      for (int i = 0; i < CH; ++i) red[i][threadIdx.x] = s[i];
-      __syncthreads();
-      const cplx_t<T> tot = row_sum<T, CH>(red, nthr);
+      for (int i = 0; i < CH; ++i) red[i][lane] = s[i];
+      wave_sync();
+      const cplx_t<T> tot = wave_rows<T, CH>(red, lane);
       if (writer && row < nl) {
         const int l = l0 + row;
-        const T c = sC[row];
-        if (base == 0) {
-          out[l] = {c * tot.x, c * tot.y};
-        } else {
-          const cplx_t<T> o = out[l];
-          out[l] = {o.x + c * tot.x, o.y + c * tot.y};
-        }
+        const T c = cf[2 * l + 1];
+        om[l] = {c * tot.x, c * tot.y};
       }
     }
   }
 }
 
 template <typename T, int K, int CH>
-__global__ void __launch_bounds__(kAdjThreads)
-adj2_kernel(LegArgs g, const T *__restrict__ xs, const int *__restrict__ lfps,
+__global__ void __launch_bounds__(64 * kAdjWaves)
+adj2_kernel(LegArgs g, AdjItems it, const T *__restrict__ xs, const int *__restrict__ lfps,
             const T *__restrict__ sps, const int *__restrict__ lfms, const T *__restrict__ sms,
-            const T *__restrict__ coef, const cplx_t<T> *__restrict__ G, cplx_t<T> *alm,
-            long long alm_ts) {
-  __shared__ T sA[CH];
-  __shared__ T sB[CH];
-  __shared__ T sC[CH];
-  __shared__ cplx_t<T> redP[CH][kAdjThreads];
-  __shared__ cplx_t<T> redM[CH][kAdjThreads];
-  __shared__ int smin;
-  const int m = blockIdx.x;
+            const T *__restrict__ coef, const cplx_t<T> *__restrict__ G, cplx_t<T> *out,
+            long long out_ts) {
+  __shared__ cplx_t<T> redP_all[kAdjWaves][CH][64];
+  __shared__ cplx_t<T> redM_all[kAdjWaves][CH][64];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const long long item = (long long)blockIdx.x * kAdjWaves + wave;
+  if (item >= (long long)it.nmpairs * it.ngroups) return;
+  cplx_t<T>(*redP)[64] = redP_all[wave];
+  cplx_t<T>(*redM)[64] = redM_all[wave];
+  const int q = (int)(item / it.ngroups), grp = (int)(item % it.ngroups);
   const long long t = blockIdx.y;
-  const int nthr = blockDim.x;
   const int lmax = g.lmax;
   const long long nelem = nelem_of(lmax);
-  const long long ms = mstart_of(m, lmax);
-  const T *cf = coef + 3 * ms;
-  cplx_t<T> *outE = alm + t * alm_ts + ms;
-  cplx_t<T> *outB = outE + nelem;
   const long long stride = g.mmax + 1;
   const cplx_t<T> *Gq = G + (t * 2 * g.nrings) * stride;
   const cplx_t<T> *Gu = Gq + g.nrings * stride;
-  const int R = nthr / CH;
-  const int row = threadIdx.x / R;
-  const bool writer = (threadIdx.x % R) == 0;
+  constexpr int R = 64 / CH;
+  const int row = lane / R;
+  const bool writer = (lane % R) == 0;
 
-  for (long long base = 0; base < g.npairs; base += (long long)nthr * K) {
+  for (int pass = 0; pass < 2; ++pass) {
+    const int m = pass == 0 ? q : g.mmax - q;
+    if (pass == 1 && m == q) break;
+    const long long ms = mstart_of(m, lmax);
+    const T *cf = coef + 3 * ms;
+    cplx_t<T> *oE = out + grp * it.grp_ts + t * out_ts + ms;
+    cplx_t<T> *oB = oE + nelem;
     T x[K], p1[K], p2[K], q1[K], q2[K], pa[K], pb[K], qa[K], qb[K];
     int lfp[K], lfm[K];
     cplx_t<T> ZpN[K], ZmN[K], ZpS[K], ZmS[K];
     int mylf = INT_MAX;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      const long long ip = base + threadIdx.x + (long long)k * nthr;
+      const long long ip = (long long)grp * 64 * K + lane + 64 * k;
       x[k] = p1[k] = p2[k] = q1[k] = q2[k] = pa[k] = pb[k] = qa[k] = qb[k] = 0;
       lfp[k] = lfm[k] = INT_MAX;
       ZpN[k] = ZmN[k] = ZpS[k] = ZmS[k] = {0, 0};
@@ -495,88 +532,120 @@ adj2_kernel(LegArgs g, const T *__restrict__ xs, const int *__restrict__ lfps,
         mylf = min(mylf, min(lfp[k], lfm[k]));
       }
     }
-    const int lmin = block_min(mylf, &smin);
-    if (base == 0)
-      for (int l = m + threadIdx.x; l < min(lmin, lmax + 1); l += nthr) {
-        outE[l] = {0, 0};
-        outB[l] = {0, 0};
-      }
+    const int lmin = __builtin_amdgcn_readfirstlane(wave_min(mylf));
+    for (int l = m + lane; l < min(lmin, lmax + 1); l += 64) {
+      oE[l] = {0, 0};
+      oB[l] = {0, 0};
+    }
 
     for (int l0 = lmin; l0 <= lmax; l0 += CH) {
       const int nl = min(CH, lmax - l0 + 1);
-      __syncthreads();
-      if (threadIdx.x < nl) {
-        const int l = l0 + threadIdx.x;
-        sA[threadIdx.x] = cf[3 * l];
-        sB[threadIdx.x] = cf[3 * l + 1];
-        sC[threadIdx.x] = cf[3 * l + 2];
-      }
-      __syncthreads();
       bool inj = false;
 #pragma unroll
       for (int k = 0; k < K; ++k)
         inj |= (lfp[k] >= l0 && lfp[k] < l0 + nl) || (lfm[k] >= l0 && lfm[k] < l0 + nl);
-      const bool slow = __syncthreads_or(inj);
-      cplx_t<T> sPl[CH], sMl[CH];
+      // wave-uniform coefficients of the chunk (clamped index: scalar loads)
+      T Ac[CH], Bc[CH];
 #pragma unroll
       for (int i = 0; i < CH; ++i) {
-        sPl[i] = sMl[i] = {0, 0};
-        if (i < nl) {
-          const int l = l0 + i;
-          const T sg = ((l - m) & 1) ? (T)-1 : (T)1;
+        const int l = min(l0 + i, lmax);
+        Ac[i] = cf[3 * l];
+        Bc[i] = cf[3 * l + 1];
+      }
+      // south weights with the sign of the chunk's first multipole; the
+      // sign alternates with l
+      const T sg0 = ((l0 - m) & 1) ? (T)-1 : (T)1;
+      cplx_t<T> PS[K], MS[K];
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        PS[k] = {sg0 * ZpS[k].x, sg0 * ZpS[k].y};
+        MS[k] = {sg0 * ZmS[k].x, sg0 * ZmS[k].y};
+      }
+      cplx_t<T> sP[CH], sM[CH];
+#pragma unroll
+      for (int i = 0; i < CH; ++i) sP[i] = sM[i] = {0, 0};
+      if (nl == CH && !__any(inj)) {
+        // steady state: plain recurrences over a full chunk
+        // even l-m: S+ += lam+ ZpN + lam- ZpS, S- += lam- ZmN + lam+ ZmS
+        // odd l-m : the south terms change sign
+#pragma unroll
+        for (int i = 0; i < CH; ++i) {
 #pragma unroll
           for (int k = 0; k < K; ++k) {
-            const T ax = sA[i] * x[k];
-            const T np_ = fma(ax + sB[i], p1[k], -p2[k]);
-            const T nq_ = fma(ax - sB[i], q1[k], -q2[k]);
-            if (slow && l == lfp[k]) {
-              p2[k] = pa[k];
-              p1[k] = pb[k];
-            } else {
-              p2[k] = p1[k];
-              p1[k] = np_;
+            const T ax = Ac[i] * x[k];
+            const T np_ = fma(ax + Bc[i], p1[k], -p2[k]);
+            const T nq_ = fma(ax - Bc[i], q1[k], -q2[k]);
+            p2[k] = p1[k];
+            p1[k] = np_;
+            q2[k] = q1[k];
+            q1[k] = nq_;
+            const T nqs = (i & 1) ? -nq_ : nq_, nps = (i & 1) ? -np_ : np_;
+            sP[i] = cfma(np_, ZpN[k], sP[i]);
+            sP[i] = cfma(nqs, PS[k], sP[i]);
+            sM[i] = cfma(nq_, ZmN[k], sM[i]);
+            sM[i] = cfma(nps, MS[k], sM[i]);
+          }
+        }
+      } else {
+        // chunk holding a start multipole of some lane, or the last chunk
+#pragma unroll
+        for (int i = 0; i < CH; ++i) {
+          if (i < nl) {
+            const int l = l0 + i;
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+              const T ax = Ac[i] * x[k];
+              const T np_ = fma(ax + Bc[i], p1[k], -p2[k]);
+              const T nq_ = fma(ax - Bc[i], q1[k], -q2[k]);
+              const bool sp_ = (l == lfp[k]), sq_ = (l == lfm[k]);
+              p2[k] = sp_ ? pa[k] : p1[k];
+              p1[k] = sp_ ? pb[k] : np_;
+              q2[k] = sq_ ? qa[k] : q1[k];
+              q1[k] = sq_ ? qb[k] : nq_;
+              const T nqs = (i & 1) ? -q1[k] : q1[k], nps = (i & 1) ? -p1[k] : p1[k];
+              sP[i] = cfma(p1[k], ZpN[k], sP[i]);
+              sP[i] = cfma(nqs, PS[k], sP[i]);
+              sM[i] = cfma(q1[k], ZmN[k], sM[i]);
+              sM[i] = cfma(nps, MS[k], sM[i]);
             }
-            if (slow && l == lfm[k]) {
-              q2[k] = qa[k];
-              q1[k] = qb[k];
-            } else {
-              q2[k] = q1[k];
-              q1[k] = nq_;
-            }
-            // even: S+ = 2lam ZpN + (-2)lam ZpS, S- = (-2)lam ZmN + 2lam ZmS
-            // odd : S+ = 2lam ZpN - (-2)lam ZpS, S- = (-2)lam ZmN - 2lam ZmS
-            sPl[i] = cfma(p1[k], ZpN[k], sPl[i]);
-            sPl[i] = cfma(sg * q1[k], ZpS[k], sPl[i]);
-            sMl[i] = cfma(q1[k], ZmN[k], sMl[i]);
-            sMl[i] = cfma(sg * p1[k], ZmS[k], sMl[i]);
           }
         }
       }
+      wave_sync();
 #pragma unroll
       for (int i = 0; i < CH; ++i) {
-        redP[i][threadIdx.x] = sPl[i];
-        redM[i][threadIdx.x] = sMl[i];
+        redP[i][lane] = sP[i];
+        redM[i][lane] = sM[i];
       }
-      __syncthreads();
-      const cplx_t<T> sp = row_sum<T, CH>(redP, nthr);
-      const cplx_t<T> sm = row_sum<T, CH>(redM, nthr);
+      wave_sync();
+      const cplx_t<T> tp = wave_rows<T, CH>(redP, lane);
+      const cplx_t<T> tm = wave_rows<T, CH>(redM, lane);
       if (writer && row < nl) {
         const int l = l0 + row;
-        const T c = (T)0.5 * sC[row];
+        const T c = (T)0.5 * cf[3 * l + 2];
         // aE = -(S+ + S-)/2, aB = (i/2)(S+ - S-)
-        const cplx_t<T> e = {-c * (sp.x + sm.x), -c * (sp.y + sm.y)};
-        const cplx_t<T> b = {-c * (sp.y - sm.y), c * (sp.x - sm.x)};
-        if (base == 0) {
-          outE[l] = e;
-          outB[l] = b;
-        } else {
-          const cplx_t<T> oe = outE[l], ob = outB[l];
-          outE[l] = {oe.x + e.x, oe.y + e.y};
-          outB[l] = {ob.x + b.x, ob.y + b.y};
-        }
+        oE[l] = {-c * (tp.x + tm.x), -c * (tp.y + tm.y)};
+        oB[l] = {-c * (tp.y - tm.y), c * (tp.x - tm.x)};
       }
     }
   }
+}
+
+// out[t, c, :] = sum over groups of partial[grp, t, c, :], fixed order.
+template <typename T>
+__global__ void sum_groups_kernel(const cplx_t<T> *part, long long grp_ts, int ngroups,
+                                  cplx_t<T> *out, long long out_ts, long long part_ts,
+                                  long long rowlen, long long nrows_per_t) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long t = blockIdx.y;
+  if (i >= rowlen * nrows_per_t) return;
+  cplx_t<T> acc = part[t * part_ts + i];
+  for (int gi = 1; gi < ngroups; ++gi) {
+    const cplx_t<T> v = part[gi * grp_ts + t * part_ts + i];
+    acc.x += v.x;
+    acc.y += v.y;
+  }
+  out[t * out_ts + i] = acc;
 }
 
 template <typename T>
@@ -597,11 +666,10 @@ LegArgs leg_args(owmi_plan_impl *p) {
   return a;
 }
 
-// Chunk lengths of the adjoint: LDS holds CH x 256 complex values per sum
-// (32 KB).
+// Chunk lengths of the adjoint (multipoles between wave reductions).
 template <typename T> struct AdjCH;
-template <> struct AdjCH<double> { static constexpr int s0 = 8, s2 = 4; };
-template <> struct AdjCH<float> { static constexpr int s0 = 16, s2 = 8; };
+template <> struct AdjCH<double> { static constexpr int s0 = 16, s2 = 8; };
+template <> struct AdjCH<float> { static constexpr int s0 = 16, s2 = 16; };
 
 }  // namespace
 
@@ -631,29 +699,48 @@ void launch_synthesis(owmi_plan_impl *p, int spin, const cplx_t<T> *alm, long lo
 }
 
 template <typename T, int K>
-static void launch_adj_k(owmi_plan_impl *p, int spin, int bs, const cplx_t<T> *G, cplx_t<T> *alm,
+static void launch_adj_k(owmi_plan_impl *p, int spin, const cplx_t<T> *G, cplx_t<T> *alm,
                          long long alm_ts, long long ntrans, hipStream_t st) {
   PlanTables<T> &tb = tab<T>(p);
-  dim3 grid((unsigned)(p->mmax + 1), (unsigned)ntrans);
+  AdjItems it;
+  it.nmpairs = (p->mmax + 2) / 2;
+  it.ngroups = (int)((p->npairs + 64LL * K - 1) / (64LL * K));
+  const long long nelem = nelem_of(p->lmax);
+  const int nc = spin == 0 ? 1 : 2;
+  cplx_t<T> *dst = alm;
+  long long dst_ts = alm_ts;
+  if (it.ngroups > 1) {
+    // partial sums [group][transform][comp][nelem] in the plan's second workspace
+    const long long part_ts = nc * nelem;
+    it.grp_ts = ntrans * part_ts;
+    dst = static_cast<cplx_t<T> *>(
+        p->ensure_work2((size_t)it.ngroups * it.grp_ts * sizeof(cplx_t<T>)));
+    dst_ts = part_ts;
+  } else {
+    it.grp_ts = 0;
+  }
+  const long long nitems = (long long)it.nmpairs * it.ngroups;
+  dim3 grid(ceil_div(nitems, kAdjWaves), (unsigned)ntrans);
   if (spin == 0)
-    hipLaunchKernelGGL((adj0_kernel<T, K, AdjCH<T>::s0>), grid, dim3(bs), 0, st, leg_args(p),
-                       tb.x, tb.lf0, tb.s0, tb.coef0, G, alm, alm_ts);
+    hipLaunchKernelGGL((adj0_kernel<T, K, AdjCH<T>::s0>), grid, dim3(64 * kAdjWaves), 0, st,
+                       leg_args(p), it, tb.x, tb.lf0, tb.s0, tb.coef0, G, dst, dst_ts);
   else
-    hipLaunchKernelGGL((adj2_kernel<T, K, AdjCH<T>::s2>), grid, dim3(bs), 0, st, leg_args(p),
-                       tb.x, tb.lfp, tb.sp, tb.lfm, tb.sm, tb.coef2, G, alm, alm_ts);
+    hipLaunchKernelGGL((adj2_kernel<T, K, AdjCH<T>::s2>), grid, dim3(64 * kAdjWaves), 0, st,
+                       leg_args(p), it, tb.x, tb.lfp, tb.sp, tb.lfm, tb.sm, tb.coef2, G, dst,
+                       dst_ts);
+  if (it.ngroups > 1) {
+    const long long n = nc * nelem;
+    dim3 g2(ceil_div(n, 256), (unsigned)ntrans);
+    hipLaunchKernelGGL(sum_groups_kernel<T>, g2, dim3(256), 0, st, dst, it.grp_ts, it.ngroups,
+                       alm, alm_ts, dst_ts, nelem, (long long)nc);
+  }
 }
 
 template <typename T>
 void launch_adjoint(owmi_plan_impl *p, int spin, const cplx_t<T> *G, cplx_t<T> *alm,
                     long long alm_ts, long long ntrans, hipStream_t st) {
   OWMI_REQUIRE(ntrans < 65536, "too many transforms for one launch");
-  // threads: a power of two in [64, 256] covering the pairs with K <= 2
-  int bs = 64;
-  while (bs < kAdjThreads && bs * 2 < p->npairs) bs *= 2;
-  if (p->npairs > (long long)bs)
-    launch_adj_k<T, 2>(p, spin, bs, G, alm, alm_ts, ntrans, st);
-  else
-    launch_adj_k<T, 1>(p, spin, bs, G, alm, alm_ts, ntrans, st);
+  launch_adj_k<T, 2>(p, spin, G, alm, alm_ts, ntrans, st);
 }
 
 template void launch_synthesis<double>(owmi_plan_impl *, int, const double2 *, long long,

@@ -78,6 +78,10 @@ struct owmi_plan_impl {
   void *work = nullptr;
   size_t work_bytes = 0;
   void *ensure_work(size_t bytes);
+  // workspace for per-group partial sums of the adjoint Legendre transform
+  void *work2 = nullptr;
+  size_t work2_bytes = 0;
+  void *ensure_work2(size_t bytes);
 
   size_t nchirp = 0, nbhat = 0;  // complex element counts of the tables
 };

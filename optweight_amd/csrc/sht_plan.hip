@@ -109,6 +109,16 @@ void *owmi_plan_impl::ensure_work(size_t bytes) {
   return work;
 }
 
+void *owmi_plan_impl::ensure_work2(size_t bytes) {
+  if (bytes <= work2_bytes) return work2;
+  if (work2) OWMI_HIP(hipFree(work2));
+  work2 = nullptr;
+  work2_bytes = 0;
+  OWMI_HIP(hipMalloc(&work2, bytes));
+  work2_bytes = bytes;
+  return work2;
+}
+
 // Host-side tables shared by both precisions (long double), built once.
 struct HostTables {
   // seeds at l = max(m, |s|) as v * 2^(128 k) (float64 v)
@@ -521,7 +531,7 @@ int owmi_sht_plan_destroy(owmi_sht_plan *plan) {
   free_tables(p.t64);
   free_tables(p.t32);
   void *ptrs[] = {p.pair_n, p.pair_s, p.nphi, p.offsets, p.ring_logm, p.ring_chirp_off,
-                  p.ring_bhat_off, p.fft_ring_a, p.fft_ring_b, p.work};
+                  p.ring_bhat_off, p.fft_ring_a, p.fft_ring_b, p.work, p.work2};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   (void)hipSetDevice(dev);
