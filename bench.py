@@ -35,8 +35,12 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build_fp64_solver(prob, dev, use_mg=True):
-    solver = solvers.CGWienerMap.from_arrays(prob.imap, prob.minfo, prob.ainfo, prob.icov_ell,
+def build_solver(prob, dev, imap=None, use_mg=True):
+    """CGWienerMap of the workload (imap (npol, npix), or (nbatch, npol, npix)
+    for a batch of right-hand sides) with the pinv preconditioner on TQU and
+    the multigrid preconditioner on T."""
+    imap = prob.imap if imap is None else imap
+    solver = solvers.CGWienerMap.from_arrays(imap, prob.minfo, prob.ainfo, prob.icov_ell,
                                              prob.icov_pix, device=dev)
     solver.add_preconditioner(preconditioners.PseudoInvPreconditioner(
         prob.ainfo, prob.icov_ell, prob.icov_pix, prob.minfo, prob.spin))
@@ -49,29 +53,34 @@ def build_fp64_solver(prob, dev, use_mg=True):
 
 
 def true_residual(solver, x):
-    r = solver.a_matrix(x) - solver.b0
+    """float64 true relative residual ||b - A x|| / ||b|| per right-hand side."""
+    r = solver.b0 - solver.a_matrix(x)
+    if r.ndim == 3:
+        rr = alm_utils.contract_almxblm_batch(r, r).cpu().numpy()
+        bb = alm_utils.contract_almxblm_batch(solver.b0, solver.b0).cpu().numpy()
+        return np.sqrt(rr / bb)
     return np.sqrt(alm_utils.contract_almxblm(r, r) / alm_utils.contract_almxblm(solver.b0,
                                                                                  solver.b0))
 
 
-def run_solver(kind, solver, tol=1e-6):
+def run_solver(kind, solver, tol=1e-6, delta=0.1):
     b = solver.b0
     if kind == 'fp64':
-        x, it = mixed.solve_fp64(solver.a_matrix, b, solver.preconditioner, tol=tol)
-        return x, {'iterations': it}
-    x, info = mixed.solve_mixed(solver.a_matrix, b, solver.preconditioner, tol=tol)
-    return x, {'iterations': info.iterations, 'replacements': info.replacements}
+        return mixed.solve_fp64(solver.a_matrix, b, solver.preconditioner, tol=tol)
+    return mixed.solve_mixed(solver.a_matrix, b, solver.preconditioner, tol=tol, delta=delta)
 
 
-def time_solves(kind, solver, steps, warmup):
+def time_solves(kind, solver, steps, warmup, delta=0.1):
     for w in range(warmup):
-        x, info = run_solver(kind, solver)
-        log(f'{kind} warmup solve {w}: {info}, true residual {true_residual(solver, x):.3e}')
+        x, info = run_solver(kind, solver, delta=delta)
+        log(f'{kind} warmup solve {w}: iterations {info.iterations} '
+            f'(per RHS {list(info.iterations_rhs)}), replacements {info.replacements}, '
+            f'true residual {np.max(true_residual(solver, x)):.3e}')
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     infos = []
     for _ in range(steps):
-        x, info = run_solver(kind, solver)
+        x, info = run_solver(kind, solver, delta=delta)
         infos.append(info)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
@@ -130,6 +139,10 @@ def main():
     ap.add_argument('--lmax', type=int, default=1023)
     ap.add_argument('--no-mg', action='store_true')
     ap.add_argument('--cpu-baseline', choices=['on', 'off'], default='on')
+    ap.add_argument('--batch', type=int, default=8,
+                    help='right-hand sides per mixed-precision batched solve')
+    ap.add_argument('--delta', type=float, default=0.1,
+                    help='residual reduction between float64 residual replacements')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py measures one GPU: this build has no multi-GPU path '
@@ -138,29 +151,35 @@ def main():
     dev = torch.device('cuda:0')
     torch.cuda.set_device(dev)
 
-    log(f'building workload TQU lmax={args.lmax}')
+    log(f'building workload TQU lmax={args.lmax}, batch {args.batch}')
     t0 = time.perf_counter()
     prob = workloads.polarized(args.lmax, seed=0, device=dev)
-    solver = build_fp64_solver(prob, dev, use_mg=not args.no_mg)
+    solver = build_solver(prob, dev, use_mg=not args.no_mg)
+    data = workloads.batch_data(prob, args.batch, seed=0, device=dev)
+    data[0] = prob.imap
+    bsolver = build_solver(prob, dev, imap=data, use_mg=not args.no_mg)
     log(f'setup {time.perf_counter() - t0:.1f}s')
 
+    # plain float64 GPU solve, one right-hand side per step
     x64, el64, inf64 = time_solves('fp64', solver, args.steps, args.warmup)
-    res64 = true_residual(solver, x64)
+    res64 = float(true_residual(solver, x64))
     fp64_sps = args.steps / el64
-    it64 = float(np.mean([i['iterations'] for i in inf64]))
+    it64 = float(np.mean([i.iterations for i in inf64]))
     log(f'fp64: {args.steps} solves in {el64:.3f}s ({fp64_sps:.3f}/s), {it64:.1f} it/solve, '
         f'true residual {res64:.3e}')
-    xmx, elmx, infmx = time_solves('mixed', solver, args.steps, args.warmup)
-    resmx = true_residual(solver, xmx)
-    mixed_sps = args.steps / elmx
-    itmx = float(np.mean([i['iterations'] for i in infmx]))
-    diff = float(torch.linalg.vector_norm(xmx - x64) / torch.linalg.vector_norm(x64))
-    log(f'mixed: {args.steps} solves in {elmx:.3f}s ({mixed_sps:.3f}/s), {itmx:.1f} it/solve, '
-        f'true residual {resmx:.3e}, rel L2 vs fp64 solution {diff:.3e}, '
+    # mixed precision, a batch of right-hand sides per step
+    xmx, elmx, infmx = time_solves('mixed', bsolver, args.steps, args.warmup, delta=args.delta)
+    resmx = true_residual(bsolver, xmx)
+    nsolve = args.steps * args.batch
+    mixed_sps = nsolve / elmx
+    itmx = float(np.mean([i.iterations for i in infmx]))
+    nrep = float(np.mean([i.replacements for i in infmx]))
+    diff = float(torch.linalg.vector_norm(xmx[0] - x64) / torch.linalg.vector_norm(x64))
+    log(f'mixed batch {args.batch}: {nsolve} solves in {elmx:.3f}s ({mixed_sps:.3f}/s), '
+        f'{itmx:.1f} it/batch, {nrep:.1f} replacements, max true residual '
+        f'{resmx.max():.3e}, RHS 0 rel L2 vs fp64 solution {diff:.3e}, '
         f'speedup {mixed_sps / fp64_sps:.2f}x')
     elapsed = elmx
-    resid = resmx
-
     roof = legendre_roofline(prob, dev)
     log(f'roofline: {roof}')
 
@@ -194,8 +213,10 @@ def main():
         'config': {'workload': f'TQU Wiener filter, lmax {args.lmax}, GL {args.lmax + 1}x'
                                f'{2 * args.lmax + 1}, pinv + multigrid(T) preconditioner, '
                                f'true resid 1e-6',
-                   'solver': 'mixed (f32 CG, f64 reliable updates)',
-                   'iterations': itmx, 'true_residual': resmx,
+                   'solver': f'mixed (f32 CG, f64 reliable updates, delta {args.delta}), '
+                             f'{args.batch} right-hand sides per batch',
+                   'batch': args.batch, 'iterations': itmx, 'replacements': nrep,
+                   'true_residual_max': float(resmx.max()),
                    'rel_l2_vs_fp64_solution': diff,
                    'fp64_solves_per_s': fp64_sps, 'fp64_iterations': it64,
                    'fp64_true_residual': res64,

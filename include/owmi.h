@@ -140,6 +140,34 @@ int owmi_cg_xr_mixed(int dtype_x, void *x, int dtype_r, void *r, const void *p,
                      const void *q, const double *dev_num, const double *dev_den,
                      long long n, void *stream);
 
+/*
+ * nbatch independent real-field inner products (as owmi_alm_dot), product b
+ * over rows [b * nrows, (b + 1) * nrows) of a and b; dev_out[b] (device
+ * doubles). 1 <= nbatch <= 1024. The per-right-hand-side scalars of a
+ * batch of pixell CG solves (solvers.py:127-141) and of
+ * alm_utils.contract_almxblm (alm_utils.py:9-44) applied per RHS.
+ */
+int owmi_alm_dot_batch(int dtype, const void *a, const void *b, int lmax,
+                       long long nrows, long long nbatch, double *dev_out,
+                       void *stream);
+
+/*
+ * Batched mixed-precision x/r update: for each b < nbatch, alpha_b =
+ * dev_num[b] / dev_den[b] (0 when dev_den[b] == 0); x_b += alpha_b p_b in
+ * dtype_x, r_b -= alpha_b q_b in dtype_r, each over n complex elements
+ * (arrays (nbatch, n)). The x/r update of pixell.utils.CG.step per RHS.
+ */
+int owmi_cg_xr_batch(int dtype_x, void *x, int dtype_r, void *r,
+                     const void *p, const void *q, const double *dev_num,
+                     const double *dev_den, long long n, long long nbatch,
+                     void *stream);
+
+/* Batched p_b = z_b + beta_b p_b, beta_b = dev_num[b] / dev_den[b] (0 when
+ * dev_den[b] == 0); arrays (nbatch, n). */
+int owmi_cg_p_batch(int dtype, void *p, const void *z, const double *dev_num,
+                    const double *dev_den, long long n, long long nbatch,
+                    void *stream);
+
 /* y = a x + b y over n complex elements with host scalars a, b. */
 int owmi_axpby(int dtype, double a, const void *x, double b, void *y,
                long long n, void *stream);

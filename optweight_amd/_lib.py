@@ -33,6 +33,10 @@ _SIGNATURES = {
     'owmi_cg_xr_mixed': (_c_int, [_c_int, _vp, _c_int, _vp, _vp, _vp, _vp, _vp, _c_ll, _vp]),
     'owmi_cg_xr': (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _vp, _c_ll, _vp]),
     'owmi_cg_p': (_c_int, [_c_int, _vp, _vp, _vp, _vp, _c_ll, _vp]),
+    'owmi_alm_dot_batch': (_c_int, [_c_int, _vp, _vp, _c_int, _c_ll, _c_ll, _vp, _vp]),
+    'owmi_cg_xr_batch': (_c_int, [_c_int, _vp, _c_int, _vp, _vp, _vp, _vp, _vp, _c_ll, _c_ll,
+                                  _vp]),
+    'owmi_cg_p_batch': (_c_int, [_c_int, _vp, _vp, _vp, _vp, _c_ll, _c_ll, _vp]),
     'owmi_axpby': (_c_int, [_c_int, _c_dbl, _vp, _c_dbl, _vp, _c_ll, _vp]),
     'owmi_convert': (_c_int, [_c_int, _vp, _c_int, _vp, _c_ll, _vp]),
     'owmi_sht_plan_create': (_c_int, [ctypes.POINTER(_vp), _c_ll, _vp, _vp, _vp, _vp, _vp, _c_int]),

@@ -58,6 +58,25 @@ def contract_almxblm_dev(alm, blm, out=None):
     return out
 
 
+def contract_almxblm_batch(alm, blm, out=None):
+    '''Per-right-hand-side real-field inner products of alm, blm with shape
+    (nbatch, ..., nelem): float64 device tensor (nbatch,), no host
+    synchronisation.'''
+    if tuple(blm.shape) != tuple(alm.shape):
+        raise ValueError(f'Shape alm ({tuple(alm.shape)}) != shape blm ({tuple(blm.shape)})')
+    _lib.require_cuda(alm, blm)
+    if alm.dtype != blm.dtype:
+        raise ValueError('dtype mismatch')
+    lmax = getlmax(alm.shape[-1])
+    nb = alm.shape[0]
+    if out is None:
+        out = torch.empty((nb,), dtype=torch.float64, device=alm.device)
+    nrows = alm.numel() // (alm.shape[-1] * nb)
+    _lib.call('owmi_alm_dot_batch', _lib.dtype_code(alm), _lib.ptr(alm), _lib.ptr(blm), lmax,
+              nrows, nb, _lib.ptr(out), _lib.stream_ptr(alm.device))
+    return out
+
+
 def contract_almxblm(alm, blm):
     '''Python float of the real-field inner product (synchronises).'''
     if not isinstance(alm, torch.Tensor):

@@ -240,6 +240,22 @@ dot_final_kernel(const double *partial, int n, double *out) {
   if (threadIdx.x == 0) *out = red[0];
 }
 
+// out[b] = sum of the n partials of product b (one block per product).
+__global__ void __launch_bounds__(kDotThreads)
+dot_batch_final_kernel(const double *partial, int n, double *out) {
+  __shared__ double red[kDotThreads];
+  const double *src = partial + (long long)blockIdx.x * n;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) acc += src[i];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = kDotThreads / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
 // Per-device scratch for the two-pass reductions (kDotBlocks doubles).
 static double *dot_scratch() {
   static std::mutex mu;
@@ -253,6 +269,31 @@ static double *dot_scratch() {
   OWMI_HIP(hipMalloc(&p, kDotBlocks * sizeof(double)));
   bufs[dev] = p;
   return p;
+}
+
+// Batched dot: nbatch independent products, each over `per` complex elements
+// (rows of nelem); block (x, b) reduces a strided slice of product b.
+template <typename T>
+__global__ void __launch_bounds__(kDotThreads)
+dot_batch_partial_kernel(const cplx_t<T> *a, const cplx_t<T> *b, int lmax, long long nelem,
+                         long long per, double *partial) {
+  __shared__ double red[kDotThreads];
+  const long long base = (long long)blockIdx.y * per;
+  double acc = 0.0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < per;
+       i += (long long)gridDim.x * blockDim.x) {
+    const long long j = i % nelem;
+    const cplx_t<T> x = a[base + i], y = b[base + i];
+    const double v = (double)x.x * (double)y.x + (double)x.y * (double)y.y;
+    acc += (j <= lmax) ? v : 2.0 * v;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = kDotThreads / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[(long long)blockIdx.y * gridDim.x + blockIdx.x] = red[0];
 }
 
 // ------------------------------------------------------------ CG updates
@@ -292,6 +333,40 @@ __global__ void cg_p_kernel(cplx_t<T> *p, const cplx_t<T> *z, const double *num,
   T beta = (T)(*num / *den);
   cplx_t<T> pv = p[i], zv = z[i];
   p[i] = {zv.x + beta * pv.x, zv.y + beta * pv.y};
+}
+
+// Batched updates: transform b (blockIdx.y) uses its own scalars
+// num[b] / den[b]; a zero denominator gives a zero coefficient, which
+// freezes a right-hand side whose residual and direction were cleared.
+__device__ __forceinline__ double safe_ratio(const double *num, const double *den, long long b) {
+  const double d = den[b];
+  return d != 0.0 ? num[b] / d : 0.0;
+}
+
+template <typename X, typename R>
+__global__ void cg_xr_batch_kernel(cplx_t<X> *x, cplx_t<R> *r, const cplx_t<R> *p,
+                                   const cplx_t<R> *q, const double *num, const double *den,
+                                   long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const long long b = blockIdx.y, o = b * n + i;
+  const double alpha = safe_ratio(num, den, b);
+  const cplx_t<R> pv = p[o], qv = q[o], rv = r[o];
+  const cplx_t<X> xv = x[o];
+  x[o] = {(X)(xv.x + alpha * (double)pv.x), (X)(xv.y + alpha * (double)pv.y)};
+  const R a = (R)alpha;
+  r[o] = {rv.x - a * qv.x, rv.y - a * qv.y};
+}
+
+template <typename T>
+__global__ void cg_p_batch_kernel(cplx_t<T> *p, const cplx_t<T> *z, const double *num,
+                                  const double *den, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const long long b = blockIdx.y, o = b * n + i;
+  const T beta = (T)safe_ratio(num, den, b);
+  const cplx_t<T> pv = p[o], zv = z[o];
+  p[o] = {zv.x + beta * pv.x, zv.y + beta * pv.y};
 }
 
 template <typename T>
@@ -384,6 +459,69 @@ int owmi_alm_dot(int dtype, const void *a, const void *b, int lmax, long long nr
                          (const float2 *)a, (const float2 *)b, lmax, nelem, nrows, part);
     hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(kDotThreads), 0, st, part, kDotBlocks,
                        dev_out);
+  });
+}
+
+int owmi_alm_dot_batch(int dtype, const void *a, const void *b, int lmax, long long nrows,
+                       long long nbatch, double *dev_out, void *stream) {
+  return guard([&] {
+    check_dtype(dtype);
+    OWMI_REQUIRE(nbatch >= 1 && nbatch <= kDotBlocks, "nbatch must be in [1, 1024]");
+    const long long nelem = nelem_of(lmax);
+    const long long per = nelem * nrows;
+    const int bpb = kDotBlocks / (int)nbatch;
+    double *part = dot_scratch();
+    hipStream_t st = as_stream(stream);
+    dim3 g(bpb, (unsigned)nbatch);
+    if (dtype == OWMI_F64)
+      hipLaunchKernelGGL(dot_batch_partial_kernel<double>, g, dim3(kDotThreads), 0, st,
+                         (const double2 *)a, (const double2 *)b, lmax, nelem, per, part);
+    else
+      hipLaunchKernelGGL(dot_batch_partial_kernel<float>, g, dim3(kDotThreads), 0, st,
+                         (const float2 *)a, (const float2 *)b, lmax, nelem, per, part);
+    hipLaunchKernelGGL(dot_batch_final_kernel, dim3((unsigned)nbatch), dim3(kDotThreads), 0, st,
+                       part, bpb, dev_out);
+  });
+}
+
+int owmi_cg_xr_batch(int dtype_x, void *x, int dtype_r, void *r, const void *p, const void *q,
+                     const double *dev_num, const double *dev_den, long long n,
+                     long long nbatch, void *stream) {
+  return guard([&] {
+    check_dtype(dtype_x);
+    check_dtype(dtype_r);
+    OWMI_REQUIRE(nbatch >= 1 && nbatch < 65536, "nbatch must be in [1, 65535]");
+    if (n == 0) return;
+    dim3 g(ceil_div(n, 256), (unsigned)nbatch);
+    hipStream_t st = as_stream(stream);
+    if (dtype_x == OWMI_F64 && dtype_r == OWMI_F32)
+      hipLaunchKernelGGL((cg_xr_batch_kernel<double, float>), g, dim3(256), 0, st, (double2 *)x,
+                         (float2 *)r, (const float2 *)p, (const float2 *)q, dev_num, dev_den, n);
+    else if (dtype_x == OWMI_F64 && dtype_r == OWMI_F64)
+      hipLaunchKernelGGL((cg_xr_batch_kernel<double, double>), g, dim3(256), 0, st,
+                         (double2 *)x, (double2 *)r, (const double2 *)p, (const double2 *)q,
+                         dev_num, dev_den, n);
+    else if (dtype_x == OWMI_F32 && dtype_r == OWMI_F32)
+      hipLaunchKernelGGL((cg_xr_batch_kernel<float, float>), g, dim3(256), 0, st, (float2 *)x,
+                         (float2 *)r, (const float2 *)p, (const float2 *)q, dev_num, dev_den, n);
+    else
+      throw Error("x must be at least as precise as r");
+  });
+}
+
+int owmi_cg_p_batch(int dtype, void *p, const void *z, const double *dev_num,
+                    const double *dev_den, long long n, long long nbatch, void *stream) {
+  return guard([&] {
+    check_dtype(dtype);
+    OWMI_REQUIRE(nbatch >= 1 && nbatch < 65536, "nbatch must be in [1, 65535]");
+    if (n == 0) return;
+    dim3 g(ceil_div(n, 256), (unsigned)nbatch);
+    if (dtype == OWMI_F64)
+      hipLaunchKernelGGL(cg_p_batch_kernel<double>, g, dim3(256), 0, as_stream(stream),
+                         (double2 *)p, (const double2 *)z, dev_num, dev_den, n);
+    else
+      hipLaunchKernelGGL(cg_p_batch_kernel<float>, g, dim3(256), 0, as_stream(stream),
+                         (float2 *)p, (const float2 *)z, dev_num, dev_den, n);
   });
 }
 

@@ -1,19 +1,29 @@
 '''
-Mixed-precision Wiener solve: float32 operators and Krylov vectors with a
-float64 solution and float64 residual replacement ("reliable updates",
-Sleijpen & van der Vorst 1996; the lattice-QCD mixed-precision CG of Clark
-et al. 2010).
+Wiener solves of one or many right-hand sides on the device.
 
-One iteration is the pixell CG step of optweight's solver (solvers.py
-:127-141) with A and M applied in float32:
+``solve_fp64``: the preconditioned CG recurrence of optweight's solver
+(pixell CG as used at solvers.py:127-141), everything float64.
+
+``solve_mixed``: float32 operators and Krylov vectors with a float64
+solution and float64 residual replacement ("reliable updates", Sleijpen &
+van der Vorst 1996; the mixed-precision CG of Clark et al. 2010). One
+iteration is the pixell step with A and M applied in float32:
     q = A32 p;  alpha = <r,z>/<p,q>;  x64 += alpha p;  r32 -= alpha q;
     z = M32 r32;  beta = <r,z>'/<r,z>;  p = z + beta p
-All inner products accumulate in float64 on the device. Whenever the
-recursive residual has dropped by ``delta`` since the last replacement (or
-reached the target), the true residual r64 = b - A64 x64 is formed with the
-float64 operator and replaces r32; the solve stops when that float64
-residual satisfies ||r64|| <= tol ||b||. The solution therefore meets the
-same float64 residual criterion as the plain float64 solve.
+Inner products accumulate in float64 on the device. Whenever the recursive
+residual of a right-hand side has dropped by ``delta`` since its last
+replacement (or reached the target), the true residual r64 = b - A64 x64 of
+the batch is formed with the float64 operator and replaces r32; a
+right-hand side stops when that float64 residual satisfies
+||r64|| <= tol ||b||, the same criterion the float64 solve is verified
+against.
+
+Batches: ``b`` of shape (nbatch, ncomp, nelem) runs nbatch independent
+solves through one operator application per iteration; every right-hand
+side has its own CG scalars (device vectors). A right-hand side that has
+converged is frozen (its residual and search direction are cleared, its
+coefficients become 0) while the others continue. An unbatched ``b``
+(ncomp, nelem) is a batch of one.
 '''
 import numpy as np
 import torch
@@ -28,90 +38,155 @@ def _convert(src, dtype):
     return out
 
 
-class MixedSolveInfo:
-    def __init__(self, iterations, replacements, residual):
+class SolveInfo:
+    '''iterations: CG iterations of the batch; iterations_rhs: per
+    right-hand side; replacements: float64 residual replacements;
+    residual: final float64 true relative residual per right-hand side
+    (None when not computed).'''
+
+    def __init__(self, iterations, iterations_rhs, replacements, residual):
         self.iterations = iterations
+        self.iterations_rhs = iterations_rhs
         self.replacements = replacements
         self.residual = residual
 
+    def as_dict(self):
+        return {'iterations': self.iterations,
+                'iterations_rhs': [int(i) for i in self.iterations_rhs],
+                'replacements': self.replacements,
+                'residual': None if self.residual is None
+                else [float(r) for r in self.residual]}
+
+
+# kept for callers of the single right-hand-side interface
+MixedSolveInfo = SolveInfo
+
+
+def _as_batch(b):
+    if b.ndim == 3:
+        return b, False
+    if b.ndim == 2:
+        return b[None], True
+    raise ValueError(f'b must be (ncomp, nelem) or (nbatch, ncomp, nelem), got {tuple(b.shape)}')
+
+
+def _xr(x, r, p, q, rz, pq, nb):
+    _lib.call('owmi_cg_xr_batch', _lib.dtype_code(x), _lib.ptr(x), _lib.dtype_code(r),
+              _lib.ptr(r), _lib.ptr(p), _lib.ptr(q), _lib.ptr(rz), _lib.ptr(pq),
+              r.numel() // nb, nb, _lib.stream_ptr(r.device))
+
+
+def _pupd(p, z, rz_new, rz, nb):
+    _lib.call('owmi_cg_p_batch', _lib.dtype_code(p), _lib.ptr(p), _lib.ptr(z), _lib.ptr(rz_new),
+              _lib.ptr(rz), p.numel() // nb, nb, _lib.stream_ptr(p.device))
+
+
+def _identity(v):
+    return v.clone()
+
 
 def solve_mixed(a_matrix, b, precond=None, tol=1e-6, delta=0.1, maxiter=1000,
-                low=torch.complex64):
-    '''Solve A x = b (b complex128 device alm) to float64 true relative
-    residual tol. a_matrix and precond must accept both complex128 and
-    ``low`` inputs. Returns (x complex128, MixedSolveInfo).'''
-    if precond is None:
-        precond = lambda v: v.clone()
-    dot = alm_utils.contract_almxblm_dev
-    dev = b.device
-    st = _lib.stream_ptr(dev)
-    code_lo = _lib.dtype_code(torch.empty(0, dtype=low))
-    x = torch.zeros_like(b)
-    bn2 = float(dot(b, b).item())
+                low=torch.complex64, a_matrix_low=None, precond_low=None):
+    '''Solve A x = b (b complex128 device alm, (ncomp, nelem) or (nbatch,
+    ncomp, nelem)) to float64 true relative residual tol per right-hand
+    side. a_matrix (and precond) must accept complex128 and ``low`` input,
+    or the ``low`` versions are given separately. Returns (x complex128,
+    SolveInfo).'''
+    b3, squeeze = _as_batch(b)
+    precond = precond or _identity
+    a_lo = a_matrix_low or a_matrix
+    m_lo = precond_low or precond
+    dot = alm_utils.contract_almxblm_batch
+    nb = b3.shape[0]
+    x = torch.zeros_like(b3)
+    bn2 = dot(b3, b3).cpu().numpy()
     target = tol * tol * bn2
-    r = _convert(b, low)
-    z = precond(r)
+    r = _convert(b3, low)
+    z = m_lo(r)
     rz = dot(r, z)
     p = z
-    ref2 = bn2
-    rr = torch.empty((), dtype=torch.float64, device=dev)
+    ref2 = bn2.copy()
+    rr = torch.empty((nb,), dtype=torch.float64, device=b3.device)
+    active = np.ones(nb, dtype=bool)
+    it_rhs = np.zeros(nb, dtype=np.int64)
+    res = np.full(nb, np.nan)
     it = nrep = 0
-    res = 1.0
-    while it < maxiter:
-        q = a_matrix(p)
+    while it < maxiter and active.any():
+        q = a_lo(p)
         pq = dot(p, q)
-        _lib.call('owmi_cg_xr_mixed', _lib.F64, _lib.ptr(x), code_lo, _lib.ptr(r), _lib.ptr(p),
-                  _lib.ptr(q), _lib.ptr(rz), _lib.ptr(pq), x.numel(), st)
+        _xr(x, r, p, q, rz, pq, nb)
+        del q
         it += 1
-        dot(r, r, out=rr)
-        rr_h = float(rr.item())
-        if rr_h <= delta * delta * ref2 or rr_h <= target:
-            r64 = a_matrix(x)
-            _lib.call('owmi_axpby', _lib.F64, 1.0, _lib.ptr(b), -1.0, _lib.ptr(r64), b.numel(), st)
-            rr64 = float(dot(r64, r64).item())
+        it_rhs[active] += 1
+        rr_h = dot(r, r, out=rr).cpu().numpy()
+        if np.any(active & ((rr_h <= delta * delta * ref2) | (rr_h <= target))):
+            r64 = b3 - a_matrix(x)
+            rr64 = dot(r64, r64).cpu().numpy()
             nrep += 1
-            res = np.sqrt(rr64 / bn2)
-            if rr64 <= target:
+            res[active] = np.sqrt(rr64[active] / bn2[active])
+            active &= ~(rr64 <= target)
+            if not active.any():
                 break
             r = _convert(r64, low)
+            del r64
             ref2 = rr64
-        z = precond(r)
+            done = torch.as_tensor(~active, device=b3.device)
+            if done.any():
+                r[done] = 0
+                p[done] = 0
+        z = m_lo(r)
         rz_new = dot(r, z)
-        _lib.call('owmi_cg_p', code_lo, _lib.ptr(p), _lib.ptr(z), _lib.ptr(rz_new), _lib.ptr(rz),
-                  p.numel(), st)
+        _pupd(p, z, rz_new, rz, nb)
+        del z
         rz = rz_new
-    return x, MixedSolveInfo(it, nrep, res)
+    info = SolveInfo(it, it_rhs, nrep, res)
+    return (x[0] if squeeze else x), info
 
 
-def solve_fp64(a_matrix, b, precond=None, tol=1e-6, maxiter=1000):
+def solve_fp64(a_matrix, b, precond=None, tol=1e-6, maxiter=1000, verify=False):
     '''Plain float64 preconditioned CG (the same recurrence, all float64),
-    stopping on the recursive residual ||r|| <= tol ||b||.'''
-    if precond is None:
-        precond = lambda v: v.clone()
-    dot = alm_utils.contract_almxblm_dev
-    dev = b.device
-    st = _lib.stream_ptr(dev)
-    x = torch.zeros_like(b)
-    bn2 = float(dot(b, b).item())
+    stopping each right-hand side on its recursive residual
+    ||r|| <= tol ||b||. With verify=True the float64 true residual of every
+    right-hand side is computed at the end. Returns (x, SolveInfo).'''
+    b3, squeeze = _as_batch(b)
+    precond = precond or _identity
+    dot = alm_utils.contract_almxblm_batch
+    nb = b3.shape[0]
+    x = torch.zeros_like(b3)
+    bn2 = dot(b3, b3).cpu().numpy()
     target = tol * tol * bn2
-    r = b.clone()
+    r = b3.clone()
     z = precond(r)
     rz = dot(r, z)
     p = z
-    rr = torch.empty((), dtype=torch.float64, device=dev)
+    rr = torch.empty((nb,), dtype=torch.float64, device=b3.device)
+    active = np.ones(nb, dtype=bool)
+    it_rhs = np.zeros(nb, dtype=np.int64)
     it = 0
-    while it < maxiter:
+    while it < maxiter and active.any():
         q = a_matrix(p)
         pq = dot(p, q)
-        _lib.call('owmi_cg_xr', _lib.F64, _lib.ptr(x), _lib.ptr(r), _lib.ptr(p), _lib.ptr(q),
-                  _lib.ptr(rz), _lib.ptr(pq), x.numel(), st)
+        _xr(x, r, p, q, rz, pq, nb)
+        del q
         it += 1
-        dot(r, r, out=rr)
-        if float(rr.item()) <= target:
-            break
+        it_rhs[active] += 1
+        rr_h = dot(r, r, out=rr).cpu().numpy()
+        newly = active & (rr_h <= target)
+        if newly.any():
+            active &= ~newly
+            if not active.any():
+                break
+            done = torch.as_tensor(~active, device=b3.device)
+            r[done] = 0
+            p[done] = 0
         z = precond(r)
         rz_new = dot(r, z)
-        _lib.call('owmi_cg_p', _lib.F64, _lib.ptr(p), _lib.ptr(z), _lib.ptr(rz_new), _lib.ptr(rz),
-                  p.numel(), st)
+        _pupd(p, z, rz_new, rz, nb)
+        del z
         rz = rz_new
-    return x, it
+    res = None
+    if verify:
+        r64 = b3 - a_matrix(x)
+        res = np.sqrt(dot(r64, r64).cpu().numpy() / bn2)
+    info = SolveInfo(it, it_rhs, 0, res)
+    return (x[0] if squeeze else x), info

@@ -105,11 +105,14 @@ class Level:
         mu_flat = torch.as_tensor(np.nonzero(self.mask_unobs.reshape(-1))[0], device=self.device)
         pcache = operators._DeviceCache(pinv_mat)
 
+        npp = self.mask_unobs.size
+
         def smoother(imap):
+            # rows: right-hand sides (one for an unbatched map)
             P = pcache.get(imap.device, imap.dtype)
-            flat = imap.reshape(-1)
+            flat = imap.reshape(-1, npp)
             out = torch.zeros_like(flat)
-            out[mu_flat] = P @ flat[mu_flat]
+            out[:, mu_flat] = flat[:, mu_flat] @ P.T
             return out.reshape(imap.shape)
         return smoother, pinv_mat
 
@@ -193,14 +196,15 @@ def restrict(imap, level_in, level_out, spin, adjoint=False):
     imap = imap * level_in.mask_dev(imap.device, rdt)
     lmax_t = lmax_out if lmax_out < lmax_in else lmax_in
     ainfo = alm_utils.AlmInfo(lmax_t)
-    alm = torch.zeros((imap.shape[0], ainfo.nelem), dtype=cdt, device=imap.device)
+    alm = torch.zeros(tuple(imap.shape[:-1]) + (ainfo.nelem,), dtype=cdt, device=imap.device)
     sht.map2alm(imap, alm, level_in.minfo, ainfo, spin, adjoint=adjoint)
     if not adjoint:
         r_ell = noise_utils._band_limit_gauss_beam(lmax_out, fwhm=2 * np.pi / lmax_out)
     else:
         r_ell = noise_utils._band_limit_gauss_beam(lmax_in, fwhm=2 * np.pi / lmax_in)
     alm_c_utils.lmul(alm, r_ell[:lmax_t + 1], ainfo, inplace=True)
-    omap = torch.zeros((imap.shape[0], level_out.minfo.npix), dtype=rdt, device=imap.device)
+    omap = torch.zeros(tuple(imap.shape[:-1]) + (level_out.minfo.npix,), dtype=rdt,
+                       device=imap.device)
     sht.alm2map(alm, omap, ainfo, level_out.minfo, spin, adjoint=adjoint)
     omap *= level_out.mask_dev(imap.device, rdt)
     return omap

@@ -293,15 +293,35 @@ def op2mat(op, nrow, dtype, ncol=None, input_shape=None, device=None):
     return mat
 
 
+def component_index(vec, sel):
+    """Index taking components ``sel`` of ``vec``: ``vec[sel]`` for one
+    right-hand side (ncomp, nelem), as the reference indexes; for a batch
+    (nbatch, ncomp, nelem) the same components of every right-hand side,
+    keeping the component axis (an integer selects a length-1 slice)."""
+    if sel is None or vec.ndim <= 2:
+        return sel
+    if vec.ndim != 3:
+        raise ValueError(f'expected (nbatch, ncomp, nelem), got {tuple(vec.shape)}')
+    if isinstance(sel, (int, np.integer)):
+        return (slice(None), slice(int(sel), int(sel) + 1))
+    if isinstance(sel, slice):
+        return (slice(None), sel)
+    raise NotImplementedError(f'component selection {sel!r} on a batch of right-hand sides')
+
+
 def add_operators(op_1, op_2, slice_2=None):
-    '''Callable acting as op_1(v) + op_2(v[slice_2]) scattered into slice_2.'''
-    if slice_2 is None:
-        slice_2 = slice(None)
+    """Callable acting as op_1(v) + op_2(v[slice_2]) scattered into slice_2
+    (per right-hand side for a batch, see component_index)."""
 
     def add(ivec):
         ovec = op_1(ivec.clone() if isinstance(ivec, torch.Tensor) else ivec.copy())
-        sub = ivec[slice_2]
-        ovec[slice_2] += op_2(sub.clone() if isinstance(sub, torch.Tensor) else sub.copy())
+        idx = component_index(ivec, slice_2)
+        if idx is None:
+            idx = slice(None)
+        sub = ivec[idx]
+        sub = (sub.clone(memory_format=torch.contiguous_format)
+               if isinstance(sub, torch.Tensor) else sub.copy())
+        ovec[idx] += op_2(sub)
         return ovec
 
     return add

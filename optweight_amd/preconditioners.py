@@ -135,7 +135,9 @@ class MaskedPreconditioner(operators.MatVecAlm):
         rdt = operators._real_dtype(alm.dtype)
         r = self._r.get(alm.device, rdt)
         alm = alm_c_utils.lmul(alm, r, self.ainfo, alm_out=torch.empty_like(alm))
-        imap = torch.zeros((self.npol, self.minfo.npix), dtype=rdt, device=alm.device)
+        # one right-hand side (nelem,) / (npol, nelem), or a batch (..., npol, nelem)
+        mshape = (self.npol,) if alm.ndim == 1 else tuple(alm.shape[:-1])
+        imap = torch.zeros(mshape + (self.minfo.npix,), dtype=rdt, device=alm.device)
         sht.alm2map(alm, imap, self.ainfo, self.minfo, self.spin)
         mu = self.levels[0].mask_dev(alm.device, rdt)
         imap *= mu

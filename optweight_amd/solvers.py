@@ -13,6 +13,11 @@ The CG scalars stay on the device (float64) and the vector updates are the
 HIP kernels owmi_cg_xr / owmi_cg_p: one iteration needs no host
 synchronisation; reading ``err`` synchronises.
 
+A batch of right-hand sides, b of shape (nbatch, ncomp, nelem), runs as
+nbatch independent CG recurrences sharing every operator application: the
+default inner product is then per right-hand side (a device vector), the
+updates are owmi_cg_xr_batch / owmi_cg_p_batch, and ``err`` is an array.
+
 When the data model is pixel-diagonal (mask and inverse noise diagonal per
 component, no swap_bm / lensing / filter / scale filter) the A operator
 runs Y^T (M N^-1 M) Y through the fused synthesis -> pixel multiply ->
@@ -48,7 +53,7 @@ class CG:
             self.r = b - self.A(self.x)
         z = self.M(self.r)
         self.rz = self.dot(self.r, z)
-        self.rz0 = float(self.rz)
+        self.rz0 = _host(self.rz)
         self.p = z
         self.i = 0
         self._err = np.inf
@@ -57,7 +62,7 @@ class CG:
     def err(self):
         e = self._err
         if isinstance(e, torch.Tensor):
-            e = float(e.item())
+            e = _host(e)
             self._err = e
         return e
 
@@ -69,8 +74,12 @@ class CG:
         Ap = self.A(self.p)
         pAp = self.dot(self.p, Ap)
         dev_scalars = isinstance(self.rz, torch.Tensor) and isinstance(pAp, torch.Tensor)
-        if dev_scalars and _is_fast(self.x) and _is_fast(self.r) and _is_fast(self.p) \
-                and _is_fast(Ap):
+        fast = (dev_scalars and _is_fast(self.x) and _is_fast(self.r) and _is_fast(self.p)
+                and _is_fast(Ap))
+        if fast and self.rz.ndim == 1:
+            self._step_batch(Ap, pAp)
+            return
+        if fast:
             _lib.call('owmi_cg_xr', _lib.dtype_code(self.x), _lib.ptr(self.x), _lib.ptr(self.r),
                       _lib.ptr(self.p), _lib.ptr(Ap), _lib.ptr(self.rz), _lib.ptr(pAp),
                       self.x.numel(), _lib.stream_ptr(self.x.device))
@@ -94,8 +103,37 @@ class CG:
         self.i += 1
 
 
+    def _step_batch(self, Ap, pAp):
+        nb = self.rz.numel()
+        n = self.x.numel() // nb
+        st = _lib.stream_ptr(self.x.device)
+        code = _lib.dtype_code(self.x)
+        _lib.call('owmi_cg_xr_batch', code, _lib.ptr(self.x), code, _lib.ptr(self.r),
+                  _lib.ptr(self.p), _lib.ptr(Ap), _lib.ptr(self.rz), _lib.ptr(pAp), n, nb, st)
+        del Ap
+        z = self.M(self.r)
+        next_rz = self.dot(self.r, z)
+        if getattr(self, '_rz0_dev', None) is None:
+            self._rz0_dev = torch.as_tensor(self.rz0, device=next_rz.device)
+        self._err = next_rz / self._rz0_dev
+        _lib.call('owmi_cg_p_batch', code, _lib.ptr(self.p), _lib.ptr(z), _lib.ptr(next_rz),
+                  _lib.ptr(self.rz), n, nb, st)
+        self.rz = next_rz
+        self.i += 1
+
+
+def _host(v):
+    '''float of a 0-d device scalar, numpy array of a vector of them.'''
+    if isinstance(v, torch.Tensor):
+        v = v.detach().cpu().numpy()
+        return float(v) if v.ndim == 0 else v
+    return v
+
+
 def _default_dot(b):
     if isinstance(b, torch.Tensor):
+        if b.ndim == 3:
+            return alm_utils.contract_almxblm_batch
         return alm_utils.contract_almxblm_dev
     return alm_utils.contract_almxblm
 
@@ -142,7 +180,11 @@ class CGWienerMap(CG):
                 self.preconditioner = prec
             else:
                 def sliced_prec(alm):
-                    alm[sel] = prec(alm[sel])
+                    idx = operators.component_index(alm, sel)
+                    sub = alm[idx]
+                    if isinstance(sub, torch.Tensor) and not sub.is_contiguous():
+                        sub = sub.contiguous()
+                    alm[idx] = prec(sub)
                     return alm
                 self.preconditioner = sliced_prec
         else:
@@ -278,14 +320,16 @@ class CGWienerMap(CG):
         if device is None:
             device = torch.device('cuda', torch.cuda.current_device())
         imap_np = imap.detach().cpu().numpy() if isinstance(imap, torch.Tensor) else np.asarray(imap)
+        # (npol, npix) for one right-hand side, (nbatch, npol, npix) for a batch
+        batched = imap_np.ndim == 3
         if spin is None:
-            spin = sht_tools.default_spin(imap_np.shape)
+            spin = sht_tools.default_spin(imap_np.shape[-2:] if batched else imap_np.shape)
         imap_np = mat_utils.atleast_nd(imap_np, 2)
         icov_pix = mat_utils.atleast_nd(np.asarray(icov_pix), 2)
         icov_ell = mat_utils.atleast_nd(np.asarray(icov_ell), 2)
-        if icov_pix.shape[-2:] != imap_np.shape:
+        if icov_pix.shape[-2:] != imap_np.shape[-2:]:
             raise ValueError(f'Mismatch {icov_pix.shape=} and {imap_np.shape=}')
-        if icov_ell.shape[0] != imap_np.shape[0]:
+        if icov_ell.shape[0] != imap_np.shape[-2]:
             raise ValueError(f'Mismatch {icov_ell.shape=} and {imap_np.shape=}')
         icov_signal = operators.EllMatVecAlm(ainfo, icov_ell)
         icov_noise = operators.PixMatVecMap(icov_pix, 1, inplace=False)
@@ -304,11 +348,14 @@ class CGWienerMap(CG):
         rdt = torch.float64 if imap_np.dtype == np.float64 else torch.float32
         cdt = torch.complex128 if rdt == torch.float64 else torch.complex64
         if draw_constr:
-            rand_isignal = torch.as_tensor(
-                alm_utils.rand_alm(icov_ell, ainfo, seed,
-                                   dtype=np.complex128 if cdt == torch.complex128
-                                   else np.complex64), device=device)
-            rand_inoise = torch.as_tensor(map_utils.rand_map_pix(icov_pix, seed),
+            # a batch draws right-hand side i from the seed sequence [seed, i]
+            seeds = ([None if seed is None else [seed, i] for i in range(imap_np.shape[0])]
+                     if batched else [seed])
+            cnp = np.complex128 if cdt == torch.complex128 else np.complex64
+            rs = [alm_utils.rand_alm(icov_ell, ainfo, sd, dtype=cnp) for sd in seeds]
+            rn = [map_utils.rand_map_pix(icov_pix, sd) for sd in seeds]
+            rand_isignal = torch.as_tensor(np.stack(rs) if batched else rs[0], device=device)
+            rand_inoise = torch.as_tensor(np.stack(rn) if batched else rn[0],
                                           device=device).to(rdt)
         else:
             rand_isignal = rand_inoise = None

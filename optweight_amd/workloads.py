@@ -176,3 +176,27 @@ def polarized(lmax, seed=0, sigma=10., ndisk=50, synth=None, device=None):
     noise = s_noise.standard_normal((3, minfo.npix)) * sig_pix
     imap = (sig + noise) * obs
     return Problem(lmax, minfo, icov_ell, icov_pix, imap, np.stack([obs] * 3), [0, 2], cov, a)
+
+
+def batch_data(prob, nbatch, seed=0, synth=None, device=None):
+    """(nbatch, npol, npix) data maps of the problem's signal and noise
+    model: right-hand side i has its own signal draw from prob.cov_ell and
+    white noise of variance w / icov_pix on observed pixels (seed sequence
+    [3, seed, i])."""
+    npol = prob.icov_pix.shape[0]
+    w = np.repeat(prob.minfo.weight, prob.minfo.nphi)
+    obs = prob.icov_pix > 0
+    sig_pix = np.zeros_like(prob.icov_pix)
+    sig_pix[obs] = np.sqrt((w * np.ones((npol, 1)))[obs] / prob.icov_pix[obs])
+    out = np.zeros((nbatch, npol, prob.minfo.npix))
+    for i in range(nbatch):
+        s_alm, s_noise = [np.random.default_rng(s)
+                          for s in np.random.SeedSequence([3, seed, i]).spawn(2)]
+        a = alm_utils.rand_alm(prob.cov_ell, prob.ainfo, s_alm)
+        if synth is None:
+            dev = device or torch.device('cuda', torch.cuda.current_device())
+            sig = _synth_device(a, prob.minfo, prob.lmax, prob.spin, dev)
+        else:
+            sig = synth(a, prob.minfo, prob.lmax, prob.spin)
+        out[i] = (sig + s_noise.standard_normal((npol, prob.minfo.npix)) * sig_pix) * obs
+    return out

@@ -1,0 +1,169 @@
+'''
+Batched right-hand sides and the mixed-precision solver on the GPU.
+
+  * per-RHS dots (owmi_alm_dot_batch) against single dots: relative 1e-12;
+  * batched CG updates against the update formula in float64, and the
+    zero-denominator freeze (x, r, p of that RHS unchanged / cleared);
+  * a batch of 3 RHS through CGWienerMap (pinv + multigrid on T) against 3
+    single-RHS solvers: relative L2 of x <= 1e-12 at every step;
+  * solve_fp64 / solve_mixed on a batch: every RHS reaches float64 true
+    relative residual <= 1e-6, and both solutions agree with the oracle's
+    converged float64 solution to relative L2 <= 1e-4 (a true residual of
+    1e-6 bounds the solution error by cond(A) * 1e-6).
+'''
+import numpy as np
+import pytest
+import torch
+
+from oracle import geometry as og, wiener as ow, ops as oops, sht as osht
+from optweight_amd import solvers, preconditioners, workloads, alm_utils, mixed, _lib
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a = a.cpu().numpy() if isinstance(a, torch.Tensor) else np.asarray(a)
+    b = b.cpu().numpy() if isinstance(b, torch.Tensor) else np.asarray(b)
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def oracle_geom(minfo):
+    return og.Geom(minfo.theta, minfo.weight, minfo.nphi, phi0=minfo.phi0)
+
+
+def oracle_synth(a, minfo, lmax, spin):
+    out = np.zeros((a.shape[0], minfo.npix))
+    osht.alm2map(a, out, lmax, oracle_geom(minfo), spin)
+    return out
+
+
+@pytest.fixture(scope='module')
+def prob():
+    return workloads.polarized(24, seed=5, ndisk=10, synth=oracle_synth)
+
+
+@pytest.fixture(scope='module')
+def data(prob):
+    return workloads.batch_data(prob, 3, seed=1, synth=oracle_synth)
+
+
+def add_precs(solver, prob, dev):
+    solver.add_preconditioner(preconditioners.PseudoInvPreconditioner(
+        prob.ainfo, prob.icov_ell, prob.icov_pix, prob.minfo, prob.spin))
+    mg = preconditioners.MaskedPreconditioner(
+        prob.ainfo, prob.icov_ell[0:1, 0:1], 0, prob.mask_bool[0], prob.minfo,
+        min_pix=100, n_jacobi=1, lmax_r_ell=6000, device=dev)
+    solver.add_preconditioner(mg, sel=np.s_[0])
+    return solver
+
+
+def rand_alm_t(rng, shape, lmax, dtype, dev):
+    v = rng.normal(size=shape) + 1j * rng.normal(size=shape)
+    v[..., :lmax + 1] = v[..., :lmax + 1].real
+    return torch.as_tensor(v, device=dev).to(dtype)
+
+
+@pytest.mark.parametrize('dtype', [torch.complex128, torch.complex64])
+def test_dot_batch(dev, dtype):
+    lmax = 40
+    nelem = alm_utils.AlmInfo(lmax).nelem
+    rng = np.random.default_rng(0)
+    a = rand_alm_t(rng, (5, 3, nelem), lmax, dtype, dev)
+    b = rand_alm_t(rng, (5, 3, nelem), lmax, dtype, dev)
+    got = alm_utils.contract_almxblm_batch(a, b).cpu().numpy()
+    for i in range(5):
+        ref = alm_utils.contract_almxblm(a[i], b[i])
+        assert abs(got[i] - ref) <= 1e-12 * abs(ref)
+    # closed form: <a, a> = sum_{m=0} |a|^2 + 2 sum_{m>0} |a|^2
+    an = a.to(torch.complex128).cpu().numpy()
+    cf = 2 * np.sum(np.abs(an[1]) ** 2) - np.sum(np.abs(an[1, :, :lmax + 1]) ** 2)
+    assert abs(alm_utils.contract_almxblm_batch(a, a).cpu().numpy()[1] - cf) <= 1e-12 * cf
+
+
+def test_cg_batch_updates(dev):
+    rng = np.random.default_rng(1)
+    nb, n = 4, 1000
+    x, r, p, q = (torch.as_tensor(rng.normal(size=(nb, n)) + 1j * rng.normal(size=(nb, n)),
+                                  device=dev) for _ in range(4))
+    num = torch.as_tensor(rng.uniform(1, 2, nb), device=dev)
+    den = torch.as_tensor(rng.uniform(1, 2, nb), device=dev)
+    den[2] = 0
+    x0, r0, p0 = x.clone(), r.clone(), p.clone()
+    st = _lib.stream_ptr(dev)
+    _lib.call('owmi_cg_xr_batch', _lib.F64, _lib.ptr(x), _lib.F64, _lib.ptr(r), _lib.ptr(p),
+              _lib.ptr(q), _lib.ptr(num), _lib.ptr(den), n, nb, st)
+    alpha = (num / den).cpu().numpy()
+    alpha[2] = 0
+    for i in range(nb):
+        assert rel(x[i], x0[i].cpu().numpy() + alpha[i] * p0[i].cpu().numpy()) < 1e-15
+        assert rel(r[i], r0[i].cpu().numpy() - alpha[i] * q[i].cpu().numpy()) < 1e-15
+    z = torch.as_tensor(rng.normal(size=(nb, n)) + 1j * rng.normal(size=(nb, n)), device=dev)
+    _lib.call('owmi_cg_p_batch', _lib.F64, _lib.ptr(p), _lib.ptr(z), _lib.ptr(num),
+              _lib.ptr(den), n, nb, st)
+    for i in range(nb):
+        exp = z[i].cpu().numpy() + alpha[i] * p0[i].cpu().numpy()
+        assert rel(p[i], exp) < 1e-15
+
+
+def test_batched_cg_matches_single(dev, prob, data):
+    batch = add_precs(solvers.CGWienerMap.from_arrays(
+        data, prob.minfo, prob.ainfo, prob.icov_ell, prob.icov_pix, device=dev), prob, dev)
+    singles = [add_precs(solvers.CGWienerMap.from_arrays(
+        data[i], prob.minfo, prob.ainfo, prob.icov_ell, prob.icov_pix, device=dev), prob, dev)
+        for i in range(data.shape[0])]
+    batch.init_solver()
+    for s in singles:
+        s.init_solver()
+    assert batch.b_vec.shape == (3, 3, prob.ainfo.nelem)
+    for step in range(5):
+        batch.step()
+        err = batch.err
+        for i, s in enumerate(singles):
+            s.step()
+            assert rel(batch.x[i], s.x) < 1e-12, f'rhs {i} step {step}'
+            assert abs(err[i] - s.err) <= 1e-10 * abs(s.err)
+
+
+def true_res(solver, x):
+    r = solver.b0 - solver.a_matrix(x)
+    rr = alm_utils.contract_almxblm_batch(r, r).cpu().numpy()
+    bb = alm_utils.contract_almxblm_batch(solver.b0, solver.b0).cpu().numpy()
+    return np.sqrt(rr / bb)
+
+
+def test_solve_fp64_and_mixed_batch(dev, prob, data):
+    solver = add_precs(solvers.CGWienerMap.from_arrays(
+        data, prob.minfo, prob.ainfo, prob.icov_ell, prob.icov_pix, device=dev), prob, dev)
+    x64, info64 = mixed.solve_fp64(solver.a_matrix, solver.b0, solver.preconditioner,
+                                   tol=1e-6, verify=True)
+    assert np.all(info64.residual <= 1.5e-6)
+    xm, infom = mixed.solve_mixed(solver.a_matrix, solver.b0, solver.preconditioner, tol=1e-6)
+    res = true_res(solver, xm)
+    assert np.all(res <= 1e-6), res
+    assert np.allclose(res, infom.residual, rtol=1e-6)
+    assert infom.replacements >= 1
+    # against the oracle's converged float64 solution, RHS 0
+    g = oracle_geom(prob.minfo)
+    osol = ow.WienerMap.from_arrays(data[0], g, prob.lmax, prob.icov_ell, prob.icov_pix)
+    osol.add_preconditioner(ow.PseudoInvPreconditioner(
+        prob.lmax, prob.icov_ell, prob.icov_pix, g, prob.spin))
+    cg = osol.init_solver()
+    b0 = np.sqrt(oops.contract_almxblm(osol.b0, osol.b0))
+    for _ in range(300):
+        cg.step()
+        if osol.get_residual() / b0 < 1e-11:
+            break
+    em, e64 = rel(xm[0], cg.x), rel(x64[0], cg.x)
+    print(f'relative L2 vs oracle: mixed {em:.3e}, fp64 {e64:.3e}')
+    assert em < 1e-4 and e64 < 1e-4
+
+
+def test_mixed_single_rhs(dev, prob):
+    solver = add_precs(solvers.CGWienerMap.from_arrays(
+        prob.imap, prob.minfo, prob.ainfo, prob.icov_ell, prob.icov_pix, device=dev), prob, dev)
+    x, info = mixed.solve_mixed(solver.a_matrix, solver.b0, solver.preconditioner, tol=1e-6,
+                                delta=1e-2)
+    assert x.shape == solver.b0.shape
+    r = solver.b0 - solver.a_matrix(x)
+    assert np.sqrt(alm_utils.contract_almxblm(r, r)
+                   / alm_utils.contract_almxblm(solver.b0, solver.b0)) <= 1e-6
