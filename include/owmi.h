@@ -198,6 +198,22 @@ int owmi_sht_plan_create(owmi_sht_plan **plan, long long nrings,
                          const long long *nphi, const long long *offsets,
                          const double *phi0, int lmax);
 
+/*
+ * Plan over a subset of a map's rings: the arrays describe nrings rings of a
+ * larger ring-major map whose components are map_npix (>= 1) pixels apart
+ * (offsets index that map). Synthesis writes only these rings' pixels;
+ * the adjoint reads only them, which equals the full adjoint when the
+ * map is zero on every other ring. Mirror pairs are formed within the
+ * subset. Serves masked operators whose input and output vanish outside
+ * the given rings (optweight's multigrid G = M Y d_ell Y^T M on
+ * the unobserved pixels, multigrid.py:13-471, operators.py:991-1102).
+ */
+int owmi_sht_plan_create_rings(owmi_sht_plan **plan, long long nrings,
+                               const double *theta, const double *weight,
+                               const long long *nphi, const long long *offsets,
+                               const double *phi0, int lmax,
+                               long long map_npix);
+
 int owmi_sht_plan_destroy(owmi_sht_plan *plan);
 
 /* Number of pixels of the plan's geometry. */

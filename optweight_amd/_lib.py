@@ -40,6 +40,8 @@ _SIGNATURES = {
     'owmi_axpby': (_c_int, [_c_int, _c_dbl, _vp, _c_dbl, _vp, _c_ll, _vp]),
     'owmi_convert': (_c_int, [_c_int, _vp, _c_int, _vp, _c_ll, _vp]),
     'owmi_sht_plan_create': (_c_int, [ctypes.POINTER(_vp), _c_ll, _vp, _vp, _vp, _vp, _vp, _c_int]),
+    'owmi_sht_plan_create_rings': (_c_int, [ctypes.POINTER(_vp), _c_ll, _vp, _vp, _vp, _vp, _vp,
+                                            _c_int, _c_ll]),
     'owmi_sht_plan_destroy': (_c_int, [_vp]),
     'owmi_sht_plan_npix': (_c_ll, [_vp]),
     'owmi_sht_synthesis': (_c_int, [_vp, _c_int, _c_int, _vp, _c_ll, _vp, _c_ll, _c_ll, _c_int, _vp]),

@@ -353,15 +353,19 @@ __global__ void bhat_to_float(const double *src, float *dst, long long n) {
   if (i < n) dst[i] = (float)src[i];
 }
 
-extern "C" {
+namespace {
 
-int owmi_sht_plan_create(owmi_sht_plan **out, long long nrings, const double *theta,
-                         const double *weight, const long long *nphi, const long long *offsets,
-                         const double *phi0, int lmax) {
+// Plan over the given rings of a map whose components are map_npix pixels
+// apart (map_npix < 0: the rings make up the whole map, map_npix = sum nphi;
+// map_npix == 0 is rejected).
+int create_plan(owmi_sht_plan **out, long long nrings, const double *theta,
+                const double *weight, const long long *nphi, const long long *offsets,
+                const double *phi0, int lmax, long long map_npix) {
   return guard([&] {
     OWMI_REQUIRE(out != nullptr, "plan pointer is NULL");
     OWMI_REQUIRE(nrings >= 1, "need at least one ring");
     OWMI_REQUIRE(lmax >= 0, "lmax must be >= 0");
+    OWMI_REQUIRE(map_npix != 0, "map_npix must be >= 1");
     *out = nullptr;
     auto *plan = new owmi_sht_plan();
     owmi_plan_impl &p = plan->impl;
@@ -375,11 +379,15 @@ int owmi_sht_plan_create(owmi_sht_plan **out, long long nrings, const double *th
       p.h_nphi.assign(nphi, nphi + nrings);
       p.h_offsets.assign(offsets, offsets + nrings);
       p.h_phi0.assign(phi0, phi0 + nrings);
-      p.npix = 0;
+      long long sum_nphi = 0;
       for (long long r = 0; r < nrings; ++r) {
         OWMI_REQUIRE(nphi[r] >= 1, "nphi must be >= 1");
-        p.npix += nphi[r];
+        OWMI_REQUIRE(offsets[r] >= 0, "offsets must be >= 0");
+        sum_nphi += nphi[r];
       }
+      p.npix = map_npix < 0 ? sum_nphi : map_npix;
+      for (long long r = 0; r < nrings; ++r)
+        OWMI_REQUIRE(offsets[r] + nphi[r] <= p.npix, "ring extends past the map");
 
       // Mirror pairs: rings visited in ascending theta; a ring with
       // theta < pi/2 takes the lowest-index unused ring with
@@ -520,6 +528,24 @@ int owmi_sht_plan_create(owmi_sht_plan **out, long long nrings, const double *th
     }
     *out = plan;
   });
+}
+
+}  // namespace
+
+extern "C" {
+
+int owmi_sht_plan_create(owmi_sht_plan **out, long long nrings, const double *theta,
+                         const double *weight, const long long *nphi, const long long *offsets,
+                         const double *phi0, int lmax) {
+  return create_plan(out, nrings, theta, weight, nphi, offsets, phi0, lmax, -1);
+}
+
+int owmi_sht_plan_create_rings(owmi_sht_plan **out, long long nrings, const double *theta,
+                               const double *weight, const long long *nphi,
+                               const long long *offsets, const double *phi0, int lmax,
+                               long long map_npix) {
+  return create_plan(out, nrings, theta, weight, nphi, offsets, phi0, lmax,
+                     map_npix < 1 ? 0 : map_npix);
 }
 
 int owmi_sht_plan_destroy(owmi_sht_plan *plan) {

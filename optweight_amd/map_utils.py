@@ -166,6 +166,17 @@ def minfo2lmax(minfo):
     return int(0.5 * (np.max(minfo.nphi) - 1))
 
 
+def ring_support(mask, minfo):
+    '''Sorted indices of the rings of minfo on which ``mask`` (..., npix)
+    is nonzero for some pixel and leading index (int64 array).'''
+    m = np.asarray(mask).reshape(-1, minfo.npix) != 0
+    anyp = m.any(axis=0)
+    off = np.asarray(minfo.offsets, dtype=np.int64)
+    nphi = np.asarray(minfo.nphi, dtype=np.int64)
+    hit = np.array([anyp[o:o + n].any() for o, n in zip(off, nphi)], dtype=bool)
+    return np.nonzero(hit)[0].astype(np.int64)
+
+
 def view_2d(imap, minfo):
     nphi = minfo.nphi[0]
     if not np.all(minfo.nphi == nphi):

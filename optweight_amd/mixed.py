@@ -12,8 +12,9 @@ iteration is the pixell step with A and M applied in float32:
     z = M32 r32;  beta = <r,z>'/<r,z>;  p = z + beta p
 Inner products accumulate in float64 on the device. Whenever the recursive
 residual of a right-hand side has dropped by ``delta`` since its last
-replacement (or reached the target), the true residual r64 = b - A64 x64 of
-the batch is formed with the float64 operator and replaces r32; a
+replacement (or reached the target), its true residual r64 = b - A64 x64 is
+formed with the float64 operator (on the sub-batch of such right-hand
+sides) and replaces r32; a
 right-hand side stops when that float64 residual satisfies
 ||r64|| <= tol ||b||, the same criterion the float64 solve is verified
 against.
@@ -56,10 +57,6 @@ class SolveInfo:
                 'replacements': self.replacements,
                 'residual': None if self.residual is None
                 else [float(r) for r in self.residual]}
-
-
-# kept for callers of the single right-hand-side interface
-MixedSolveInfo = SolveInfo
 
 
 def _as_batch(b):
@@ -119,19 +116,24 @@ def solve_mixed(a_matrix, b, precond=None, tol=1e-6, delta=0.1, maxiter=1000,
         it += 1
         it_rhs[active] += 1
         rr_h = dot(r, r, out=rr).cpu().numpy()
-        if np.any(active & ((rr_h <= delta * delta * ref2) | (rr_h <= target))):
-            r64 = b3 - a_matrix(x)
+        trig = active & ((rr_h <= delta * delta * ref2) | (rr_h <= target))
+        if trig.any():
+            # float64 true residual of the triggered right-hand sides only
+            idx = np.nonzero(trig)[0]
+            it_t = torch.as_tensor(idx, device=b3.device)
+            r64 = b3[it_t] - a_matrix(x[it_t].contiguous())
             rr64 = dot(r64, r64).cpu().numpy()
             nrep += 1
-            res[active] = np.sqrt(rr64[active] / bn2[active])
-            active &= ~(rr64 <= target)
+            res[idx] = np.sqrt(rr64 / bn2[idx])
+            conv = rr64 <= target[idx]
+            active[idx[conv]] = False
             if not active.any():
                 break
-            r = _convert(r64, low)
+            r[it_t] = _convert(r64, low)
             del r64
-            ref2 = rr64
-            done = torch.as_tensor(~active, device=b3.device)
-            if done.any():
+            ref2[idx] = rr64
+            if conv.any():
+                done = torch.as_tensor(idx[conv], device=b3.device)
                 r[done] = 0
                 p[done] = 0
         z = m_lo(r)

@@ -37,6 +37,9 @@ class Level:
             device = torch.device('cuda', torch.cuda.current_device())
         self.device = device
         self._mask = operators._DeviceCache(self.mask_unobs)
+        # rings holding unobserved pixels: every transform of this level acts
+        # on maps that vanish elsewhere (input) or are masked after (output)
+        self.rings = map_utils.ring_support(self.mask_unobs, minfo)
         self.g_op = operators.PixEllPixMatVecMap(self.mask_unobs, self.d_ell, self.minfo,
                                                  self.spin)
         self.pinv_g = None
@@ -197,7 +200,7 @@ def restrict(imap, level_in, level_out, spin, adjoint=False):
     lmax_t = lmax_out if lmax_out < lmax_in else lmax_in
     ainfo = alm_utils.AlmInfo(lmax_t)
     alm = torch.zeros(tuple(imap.shape[:-1]) + (ainfo.nelem,), dtype=cdt, device=imap.device)
-    sht.map2alm(imap, alm, level_in.minfo, ainfo, spin, adjoint=adjoint)
+    sht.map2alm(imap, alm, level_in.minfo, ainfo, spin, adjoint=adjoint, rings=level_in.rings)
     if not adjoint:
         r_ell = noise_utils._band_limit_gauss_beam(lmax_out, fwhm=2 * np.pi / lmax_out)
     else:
@@ -205,7 +208,8 @@ def restrict(imap, level_in, level_out, spin, adjoint=False):
     alm_c_utils.lmul(alm, r_ell[:lmax_t + 1], ainfo, inplace=True)
     omap = torch.zeros(tuple(imap.shape[:-1]) + (level_out.minfo.npix,), dtype=rdt,
                        device=imap.device)
-    sht.alm2map(alm, omap, ainfo, level_out.minfo, spin, adjoint=adjoint)
+    sht.alm2map(alm, omap, ainfo, level_out.minfo, spin, adjoint=adjoint,
+                rings=level_out.rings)
     omap *= level_out.mask_dev(imap.device, rdt)
     return omap
 

@@ -242,6 +242,9 @@ class PixEllPixMatVecMap(MatVecMap):
         self.inplace = inplace
         self.adjoint = adjoint
         self.use_weights = use_weights
+        # M vanishes off these rings: the transforms skip the others exactly
+        self.rings = (map_utils.ring_support(m_pix, minfo) if m_pix is not None
+                      and np.asarray(m_pix).shape[-1] == minfo.npix else None)
 
     def _apply_m(self, out):
         if self._m is None:
@@ -261,10 +264,12 @@ class PixEllPixMatVecMap(MatVecMap):
                           dtype=_complex_dtype(imap.dtype), device=imap.device)
         self._apply_m(out)
         sht.map2alm(out, alm, self.minfo, self.ainfo, self.spin,
-                    adjoint=self.adjoint is self.use_weights)
+                    adjoint=self.adjoint is self.use_weights, rings=self.rings)
         alm_c_utils.lmul(alm, self._x.get(imap.device, _real_dtype(imap.dtype)), self.ainfo,
                          inplace=True)
-        sht.alm2map(alm, out, self.ainfo, self.minfo, self.spin, adjoint=self.adjoint)
+        # pixels off self.rings keep their masked (zero) input values
+        sht.alm2map(alm, out, self.ainfo, self.minfo, self.spin, adjoint=self.adjoint,
+                    rings=self.rings)
         self._apply_m(out)
         return out
 

@@ -138,14 +138,15 @@ class MaskedPreconditioner(operators.MatVecAlm):
         # one right-hand side (nelem,) / (npol, nelem), or a batch (..., npol, nelem)
         mshape = (self.npol,) if alm.ndim == 1 else tuple(alm.shape[:-1])
         imap = torch.zeros(mshape + (self.minfo.npix,), dtype=rdt, device=alm.device)
-        sht.alm2map(alm, imap, self.ainfo, self.minfo, self.spin)
+        rings = self.levels[0].rings
+        sht.alm2map(alm, imap, self.ainfo, self.minfo, self.spin, rings=rings)
         mu = self.levels[0].mask_dev(alm.device, rdt)
         imap *= mu
         omap = None
         for _ in range(self.nsteps):
             omap = multigrid.v_cycle(self.levels, imap, self.spin, n_jacobi=self.n_jacobi, x0=omap)
         omap *= mu
-        sht.map2alm(omap, alm, self.minfo, self.ainfo, self.spin, adjoint=True)
+        sht.map2alm(omap, alm, self.minfo, self.ainfo, self.spin, adjoint=True, rings=rings)
         return alm_c_utils.lmul(alm, r, self.ainfo, inplace=True)
 
 

@@ -7,6 +7,11 @@ on the MapInfo object).
 
     alm2map(alm, omap, ainfo, minfo, spin, adjoint=False)   Y   (W Y if adjoint)
     map2alm(imap, alm, minfo, ainfo, spin, adjoint=False)   Y^T W   (Y^T if adjoint)
+
+Both take an optional ``rings`` (sorted ring indices of minfo): the
+transform then runs over those rings only. Synthesis leaves the pixels of
+other rings untouched; the adjoint reads only the given rings, which is the
+full adjoint for a map that vanishes on all other rings (masked operators).
 '''
 import ctypes
 
@@ -19,23 +24,33 @@ from optweight_amd import _lib
 class Plan:
     '''Owns one owmi_sht_plan (geometry + lmax on one device).'''
 
-    def __init__(self, minfo, lmax, device):
+    def __init__(self, minfo, lmax, device, rings=None):
         self.lmax = lmax
         self.npix = minfo.npix
         self.device = device
         self._lib = _lib.load()
+        self.handle = None
         h = ctypes.c_void_p()
-        theta = _lib.as_np(minfo.theta, np.float64)
-        weight = _lib.as_np(minfo.weight, np.float64)
-        nphi = _lib.as_np(minfo.nphi, np.int64)
-        offsets = _lib.as_np(minfo.offsets, np.int64)
-        phi0 = _lib.as_np(minfo.phi0, np.float64)
+        sel = slice(None) if rings is None else np.asarray(rings, dtype=np.int64)
+        theta = _lib.as_np(np.asarray(minfo.theta)[sel], np.float64)
+        weight = _lib.as_np(np.asarray(minfo.weight)[sel], np.float64)
+        nphi = _lib.as_np(np.asarray(minfo.nphi)[sel], np.int64)
+        offsets = _lib.as_np(np.asarray(minfo.offsets)[sel], np.int64)
+        phi0 = _lib.as_np(np.asarray(minfo.phi0)[sel], np.float64)
         if np.any(minfo.stride != 1):
             raise ValueError('only unit pixel stride is supported')
+        if theta.size == 0:
+            raise ValueError('empty ring selection')
         with torch.cuda.device(device):
-            _lib.check(self._lib.owmi_sht_plan_create(
-                ctypes.byref(h), minfo.nrow, _lib.np_ptr(theta), _lib.np_ptr(weight),
-                _lib.np_ptr(nphi), _lib.np_ptr(offsets), _lib.np_ptr(phi0), lmax))
+            if rings is None:
+                _lib.check(self._lib.owmi_sht_plan_create(
+                    ctypes.byref(h), theta.size, _lib.np_ptr(theta), _lib.np_ptr(weight),
+                    _lib.np_ptr(nphi), _lib.np_ptr(offsets), _lib.np_ptr(phi0), lmax))
+            else:
+                _lib.check(self._lib.owmi_sht_plan_create_rings(
+                    ctypes.byref(h), theta.size, _lib.np_ptr(theta), _lib.np_ptr(weight),
+                    _lib.np_ptr(nphi), _lib.np_ptr(offsets), _lib.np_ptr(phi0), lmax,
+                    int(minfo.npix)))
         self.handle = h
 
     def __del__(self):
@@ -47,14 +62,20 @@ class Plan:
             pass
 
 
-def get_plan(minfo, lmax, device):
+def get_plan(minfo, lmax, device, rings=None):
+    '''Cached plan of (minfo, lmax, device), optionally restricted to the
+    sorted ring indices ``rings``.'''
     device = torch.device(device)
     if device.index is None:
         device = torch.device('cuda', torch.cuda.current_device())
     cache = minfo.__dict__.setdefault('_plans', {})
-    key = (str(device), int(lmax))
+    if rings is not None:
+        rings = np.asarray(rings, dtype=np.int64)
+        if rings.size == minfo.nrow:
+            rings = None
+    key = (str(device), int(lmax), None if rings is None else rings.tobytes())
     if key not in cache:
-        cache[key] = Plan(minfo, int(lmax), device)
+        cache[key] = Plan(minfo, int(lmax), device, rings=rings)
     return cache[key]
 
 
@@ -120,11 +141,11 @@ def _check(alm, imap, ainfo, minfo, spin):
     return lmax, a3, m3
 
 
-def alm2map(alm, omap, ainfo, minfo, spin, adjoint=False):
+def alm2map(alm, omap, ainfo, minfo, spin, adjoint=False, rings=None):
     '''Synthesis Y (or W Y if adjoint) of alm (..., npol, nelem) into omap
-    (..., npol, npix), overwritten.'''
+    (..., npol, npix), overwritten (on ``rings`` only, if given).'''
     lmax, a3, m3 = _check(alm, omap, ainfo, minfo, spin)
-    plan = get_plan(minfo, lmax, alm.device)
+    plan = get_plan(minfo, lmax, alm.device, rings)
     npol = m3.shape[-2]
     code = _lib.dtype_code(omap)
     st = _lib.stream_ptr(alm.device)
@@ -135,11 +156,12 @@ def alm2map(alm, omap, ainfo, minfo, spin, adjoint=False):
     return omap
 
 
-def map2alm(imap, alm, minfo, ainfo, spin, adjoint=False):
+def map2alm(imap, alm, minfo, ainfo, spin, adjoint=False, rings=None):
     '''Adjoint synthesis Y^T W (or Y^T if adjoint) of imap (..., npol, npix)
-    into alm (..., npol, nelem), overwritten.'''
+    into alm (..., npol, nelem), overwritten (reading ``rings`` only, if
+    given).'''
     lmax, a3, m3 = _check(alm, imap, ainfo, minfo, spin)
-    plan = get_plan(minfo, lmax, alm.device)
+    plan = get_plan(minfo, lmax, alm.device, rings)
     npol = m3.shape[-2]
     code = _lib.dtype_code(imap)
     st = _lib.stream_ptr(alm.device)
