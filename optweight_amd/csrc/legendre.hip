@@ -115,70 +115,73 @@ __device__ __forceinline__ int block_min(int v, int *slot) {
 }
 
 // ------------------------------------------------------------- synthesis
+//
+// NB transforms per block share one recurrence per ring pair: the staged
+// (scaled) alm of the NB transforms sit side by side in LDS, and each
+// recurrence value feeds NB accumulations.
 
 constexpr int kSynCH = 64;
 
-template <typename T>
+template <typename T, int NB>
 __global__ void __launch_bounds__(256)
 synth0_kernel(LegArgs g, const T *__restrict__ xs, const int *__restrict__ lfs,
               const T *__restrict__ seeds, const T *__restrict__ coef,
-              const cplx_t<T> *__restrict__ alm, long long alm_ts, cplx_t<T> *G) {
+              const cplx_t<T> *__restrict__ alm, long long alm_ts, long long ntrans,
+              cplx_t<T> *G) {
   __shared__ T sA[kSynCH];
-  __shared__ cplx_t<T> sa[kSynCH];
+  __shared__ cplx_t<T> sa[kSynCH][NB];
   __shared__ int smin;
   const long long ip = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int m = blockIdx.y;
-  const long long t = blockIdx.z;
+  const long long t0 = (long long)blockIdx.z * NB;
+  const int nt = (int)min((long long)NB, ntrans - t0);
   const bool active = ip < g.npairs;
   const int lmax = g.lmax;
   const long long ms = mstart_of(m, lmax);
   const long long si = (long long)m * g.npairs + ip;
   const int lf = active ? lfs[si] : INT_MAX;
   const int lmin = block_min(lf, &smin);
-  cplx_t<T> E = {0, 0}, O = {0, 0};
+  cplx_t<T> E[NB], O[NB];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) E[b] = O[b] = {0, 0};
   if (lmin <= lmax) {
     const T x = active ? xs[ip] : (T)0;
     const T sA0 = active ? seeds[2 * si] : (T)0, sB0 = active ? seeds[2 * si + 1] : (T)0;
-    const cplx_t<T> *a = alm + t * alm_ts + ms;
+    const cplx_t<T> *a = alm + t0 * alm_ts + ms;
     const T *cf = coef + 2 * ms;
     T l1 = 0, l2 = 0;
     for (int l0 = lmin; l0 <= lmax; l0 += kSynCH) {
       const int nl = min(kSynCH, lmax - l0 + 1);
       __syncthreads();
-      for (int j = threadIdx.x; j < nl; j += blockDim.x) {
-        const int l = l0 + j;
-        sA[j] = cf[2 * l];
-        const T c = cf[2 * l + 1];
-        const cplx_t<T> av = a[l];
-        sa[j] = {av.x * c, av.y * c};
+      for (int j = threadIdx.x; j < kSynCH * NB; j += blockDim.x) {
+        const int jl = j % kSynCH, b = j / kSynCH;
+        if (jl < nl) {
+          const int l = l0 + jl;
+          if (b == 0) sA[jl] = cf[2 * l];
+          const T c = cf[2 * l + 1];
+          const cplx_t<T> av = b < nt ? a[b * alm_ts + l] : cplx_t<T>{0, 0};
+          sa[jl][b] = {av.x * c, av.y * c};
+        }
       }
       __syncthreads();
       const bool inj = active && lf >= l0 && lf < l0 + nl;
-      if (__syncthreads_or(inj)) {
-        for (int i = 0; i < nl; ++i) {
-          const int l = l0 + i;
-          if (l == lf) {
-            l2 = sA0;
-            l1 = sB0;
-          } else {
-            const T lam = fma(sA[i] * x, l1, -l2);
-            l2 = l1;
-            l1 = lam;
-          }
-          if ((l - m) & 1)
-            O = cfma(l1, sa[i], O);
-          else
-            E = cfma(l1, sa[i], E);
-        }
-      } else {
-        for (int i = 0; i < nl; ++i) {
-          const T lam = fma(sA[i] * x, l1, -l2);
+      const bool slow = __syncthreads_or(inj);
+      for (int i = 0; i < nl; ++i) {
+        const int l = l0 + i;
+        const T lam = fma(sA[i] * x, l1, -l2);
+        if (slow && l == lf) {
+          l2 = sA0;
+          l1 = sB0;
+        } else {
           l2 = l1;
           l1 = lam;
-          if ((l0 + i - m) & 1)
-            O = cfma(l1, sa[i], O);
-          else
-            E = cfma(l1, sa[i], E);
+        }
+        if ((l - m) & 1) {
+#pragma unroll
+          for (int b = 0; b < NB; ++b) O[b] = cfma(l1, sa[i][b], O[b]);
+        } else {
+#pragma unroll
+          for (int b = 0; b < NB; ++b) E[b] = cfma(l1, sa[i][b], E[b]);
         }
       }
     }
@@ -186,24 +189,31 @@ synth0_kernel(LegArgs g, const T *__restrict__ xs, const int *__restrict__ lfs,
   if (!active) return;
   const int rn = g.pair_n[ip], rs = g.pair_s[ip];
   const long long stride = g.mmax + 1;
-  G[(t * g.nrings + rn) * stride + m] = {E.x + O.x, E.y + O.y};
-  if (rs >= 0) G[(t * g.nrings + rs) * stride + m] = {E.x - O.x, E.y - O.y};
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    if (b < nt) {
+      const long long t = t0 + b;
+      G[(t * g.nrings + rn) * stride + m] = {E[b].x + O[b].x, E[b].y + O[b].y};
+      if (rs >= 0) G[(t * g.nrings + rs) * stride + m] = {E[b].x - O[b].x, E[b].y - O[b].y};
+    }
+  }
 }
 
-template <typename T>
+template <typename T, int NB>
 __global__ void __launch_bounds__(256)
 synth2_kernel(LegArgs g, const T *__restrict__ xs, const int *__restrict__ lfps,
               const T *__restrict__ sps, const int *__restrict__ lfms, const T *__restrict__ sms,
               const T *__restrict__ coef, const cplx_t<T> *__restrict__ alm, long long alm_ts,
-              cplx_t<T> *G) {
+              long long ntrans, cplx_t<T> *G) {
   __shared__ T sA[kSynCH];
   __shared__ T sB[kSynCH];
-  __shared__ cplx_t<T> sP[kSynCH];
-  __shared__ cplx_t<T> sM[kSynCH];
+  __shared__ cplx_t<T> sP[kSynCH][NB];
+  __shared__ cplx_t<T> sM[kSynCH][NB];
   __shared__ int smin;
   const long long ip = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int m = blockIdx.y;
-  const long long t = blockIdx.z;
+  const long long t0 = (long long)blockIdx.z * NB;
+  const int nt = (int)min((long long)NB, ntrans - t0);
   const bool active = ip < g.npairs;
   const int lmax = g.lmax;
   const long long nelem = nelem_of(lmax);
@@ -213,28 +223,38 @@ synth2_kernel(LegArgs g, const T *__restrict__ xs, const int *__restrict__ lfps,
   const int lfm = active ? lfms[si] : INT_MAX;
   const int lmin = block_min(min(lfp, lfm), &smin);
   // accumulators by parity of (l - m): P*2lam, P*(-2)lam, Mm*2lam, Mm*(-2)lam
-  cplx_t<T> PPe = {0, 0}, PMe = {0, 0}, MPe = {0, 0}, MMe = {0, 0};
-  cplx_t<T> PPo = {0, 0}, PMo = {0, 0}, MPo = {0, 0}, MMo = {0, 0};
+  cplx_t<T> PPe[NB], PMe[NB], MPe[NB], MMe[NB], PPo[NB], PMo[NB], MPo[NB], MMo[NB];
+#pragma unroll
+  for (int b = 0; b < NB; ++b)
+    PPe[b] = PMe[b] = MPe[b] = MMe[b] = PPo[b] = PMo[b] = MPo[b] = MMo[b] = {0, 0};
   if (lmin <= lmax) {
     const T x = active ? xs[ip] : (T)0;
     const T pa = active ? sps[2 * si] : (T)0, pb = active ? sps[2 * si + 1] : (T)0;
     const T qa = active ? sms[2 * si] : (T)0, qb = active ? sms[2 * si + 1] : (T)0;
-    const cplx_t<T> *aE = alm + t * alm_ts + ms;
-    const cplx_t<T> *aB = aE + nelem;
+    const cplx_t<T> *aE = alm + t0 * alm_ts + ms;
     const T *cf = coef + 3 * ms;
     T p1 = 0, p2 = 0, q1 = 0, q2 = 0;
     for (int l0 = lmin; l0 <= lmax; l0 += kSynCH) {
       const int nl = min(kSynCH, lmax - l0 + 1);
       __syncthreads();
-      for (int j = threadIdx.x; j < nl; j += blockDim.x) {
-        const int l = l0 + j;
-        sA[j] = cf[3 * l];
-        sB[j] = cf[3 * l + 1];
-        const T c = (T)-0.5 * cf[3 * l + 2];
-        const cplx_t<T> e = aE[l], b = aB[l];
-        // P = -(e + i b)/2, Mm = -(e - i b)/2, scaled by c_l
-        sP[j] = {c * (e.x - b.y), c * (e.y + b.x)};
-        sM[j] = {c * (e.x + b.y), c * (e.y - b.x)};
+      for (int j = threadIdx.x; j < kSynCH * NB; j += blockDim.x) {
+        const int jl = j % kSynCH, b = j / kSynCH;
+        if (jl < nl) {
+          const int l = l0 + jl;
+          if (b == 0) {
+            sA[jl] = cf[3 * l];
+            sB[jl] = cf[3 * l + 1];
+          }
+          const T c = (T)-0.5 * cf[3 * l + 2];
+          cplx_t<T> e = {0, 0}, bb = {0, 0};
+          if (b < nt) {
+            e = aE[b * alm_ts + l];
+            bb = aE[b * alm_ts + nelem + l];
+          }
+          // P = -(e + i b)/2, Mm = -(e - i b)/2, scaled by c_l
+          sP[jl][b] = {c * (e.x - bb.y), c * (e.y + bb.x)};
+          sM[jl][b] = {c * (e.x + bb.y), c * (e.y - bb.x)};
+        }
       }
       __syncthreads();
       const bool inj = active && ((lfp >= l0 && lfp < l0 + nl) || (lfm >= l0 && lfm < l0 + nl));
@@ -244,38 +264,38 @@ synth2_kernel(LegArgs g, const T *__restrict__ xs, const int *__restrict__ lfps,
         const T ax = sA[i] * x;
         const T np_ = fma(ax + sB[i], p1, -p2);
         const T nq_ = fma(ax - sB[i], q1, -q2);
-        if (slow) {
-          if (l == lfp) {
-            p2 = pa;
-            p1 = pb;
-          } else {
-            p2 = p1;
-            p1 = np_;
-          }
-          if (l == lfm) {
-            q2 = qa;
-            q1 = qb;
-          } else {
-            q2 = q1;
-            q1 = nq_;
-          }
+        if (slow && l == lfp) {
+          p2 = pa;
+          p1 = pb;
         } else {
           p2 = p1;
           p1 = np_;
+        }
+        if (slow && l == lfm) {
+          q2 = qa;
+          q1 = qb;
+        } else {
           q2 = q1;
           q1 = nq_;
         }
-        const cplx_t<T> P = sP[i], Mm = sM[i];
         if ((l - m) & 1) {
-          PPo = cfma(p1, P, PPo);
-          PMo = cfma(q1, P, PMo);
-          MPo = cfma(p1, Mm, MPo);
-          MMo = cfma(q1, Mm, MMo);
+#pragma unroll
+          for (int b = 0; b < NB; ++b) {
+            const cplx_t<T> P = sP[i][b], Mm = sM[i][b];
+            PPo[b] = cfma(p1, P, PPo[b]);
+            PMo[b] = cfma(q1, P, PMo[b]);
+            MPo[b] = cfma(p1, Mm, MPo[b]);
+            MMo[b] = cfma(q1, Mm, MMo[b]);
+          }
         } else {
-          PPe = cfma(p1, P, PPe);
-          PMe = cfma(q1, P, PMe);
-          MPe = cfma(p1, Mm, MPe);
-          MMe = cfma(q1, Mm, MMe);
+#pragma unroll
+          for (int b = 0; b < NB; ++b) {
+            const cplx_t<T> P = sP[i][b], Mm = sM[i][b];
+            PPe[b] = cfma(p1, P, PPe[b]);
+            PMe[b] = cfma(q1, P, PMe[b]);
+            MPe[b] = cfma(p1, Mm, MPe[b]);
+            MMe[b] = cfma(q1, Mm, MMe[b]);
+          }
         }
       }
     }
@@ -283,19 +303,24 @@ synth2_kernel(LegArgs g, const T *__restrict__ xs, const int *__restrict__ lfps,
   if (!active) return;
   const int rn = g.pair_n[ip], rs = g.pair_s[ip];
   const long long stride = g.mmax + 1;
-  cplx_t<T> *Gq = G + (t * 2 * g.nrings) * stride;
-  cplx_t<T> *Gu = Gq + g.nrings * stride;
-  // North: A+ = sum P 2lam, A- = sum Mm (-2)lam; Q = A+ + A-, U = -i (A+ - A-).
-  const cplx_t<T> ApN = {PPe.x + PPo.x, PPe.y + PPo.y};
-  const cplx_t<T> AmN = {MMe.x + MMo.x, MMe.y + MMo.y};
-  Gq[rn * stride + m] = {ApN.x + AmN.x, ApN.y + AmN.y};
-  Gu[rn * stride + m] = {ApN.y - AmN.y, -(ApN.x - AmN.x)};
-  if (rs >= 0) {
-    // South: A+ = sum (-1)^(l+m) P (-2)lam, A- = sum (-1)^(l+m) Mm 2lam.
-    const cplx_t<T> ApS = {PMe.x - PMo.x, PMe.y - PMo.y};
-    const cplx_t<T> AmS = {MPe.x - MPo.x, MPe.y - MPo.y};
-    Gq[rs * stride + m] = {ApS.x + AmS.x, ApS.y + AmS.y};
-    Gu[rs * stride + m] = {ApS.y - AmS.y, -(ApS.x - AmS.x)};
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    if (b >= nt) continue;
+    const long long t = t0 + b;
+    cplx_t<T> *Gq = G + (t * 2 * g.nrings) * stride;
+    cplx_t<T> *Gu = Gq + g.nrings * stride;
+    // North: A+ = sum P 2lam, A- = sum Mm (-2)lam; Q = A+ + A-, U = -i (A+ - A-).
+    const cplx_t<T> ApN = {PPe[b].x + PPo[b].x, PPe[b].y + PPo[b].y};
+    const cplx_t<T> AmN = {MMe[b].x + MMo[b].x, MMe[b].y + MMo[b].y};
+    Gq[rn * stride + m] = {ApN.x + AmN.x, ApN.y + AmN.y};
+    Gu[rn * stride + m] = {ApN.y - AmN.y, -(ApN.x - AmN.x)};
+    if (rs >= 0) {
+      // South: A+ = sum (-1)^(l+m) P (-2)lam, A- = sum (-1)^(l+m) Mm 2lam.
+      const cplx_t<T> ApS = {PMe[b].x - PMo[b].x, PMe[b].y - PMo[b].y};
+      const cplx_t<T> AmS = {MPe[b].x - MPo[b].x, MPe[b].y - MPo[b].y};
+      Gq[rs * stride + m] = {ApS.x + AmS.x, ApS.y + AmS.y};
+      Gu[rs * stride + m] = {ApS.y - AmS.y, -(ApS.x - AmS.x)};
+    }
   }
 }
 
@@ -346,22 +371,25 @@ struct AdjItems {
   long long grp_ts;   // elements between the partial buffers of two groups
 };
 
-template <typename T, int K, int CH>
+template <typename T, int K, int CH, int NB>
 __global__ void __launch_bounds__(64 * kAdjWaves)
 adj0_kernel(LegArgs g, AdjItems it, const T *__restrict__ xs, const int *__restrict__ lfs,
             const T *__restrict__ seeds, const T *__restrict__ coef,
-            const cplx_t<T> *__restrict__ G, cplx_t<T> *out, long long out_ts) {
-  __shared__ cplx_t<T> red_all[kAdjWaves][CH][64];
+            const cplx_t<T> *__restrict__ G, long long ntrans, cplx_t<T> *out,
+            long long out_ts) {
+  constexpr int ROWS = CH * NB;  // reduction rows per chunk: (multipole, transform)
+  __shared__ cplx_t<T> red_all[kAdjWaves][ROWS][64];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const long long item = (long long)blockIdx.x * kAdjWaves + wave;
   if (item >= (long long)it.nmpairs * it.ngroups) return;
   cplx_t<T>(*red)[64] = red_all[wave];
   const int q = (int)(item / it.ngroups), grp = (int)(item % it.ngroups);
-  const long long t = blockIdx.y;
+  const long long t0 = (long long)blockIdx.y * NB;
+  const int nt = (int)min((long long)NB, ntrans - t0);
   const int lmax = g.lmax;
   const long long stride = g.mmax + 1;
-  constexpr int R = 64 / CH;
-  const int row = lane / R;
+  constexpr int R = 64 / ROWS;
+  const int row = lane / R, ri = row / NB, rb = row % NB;
   const bool writer = (lane % R) == 0;
 
   for (int pass = 0; pass < 2; ++pass) {
@@ -369,17 +397,18 @@ adj0_kernel(LegArgs g, AdjItems it, const T *__restrict__ xs, const int *__restr
     if (pass == 1 && m == q) break;
     const long long ms = mstart_of(m, lmax);
     const T *cf = coef + 2 * ms;
-    cplx_t<T> *om = out + grp * it.grp_ts + t * out_ts + ms;
+    cplx_t<T> *om = out + grp * it.grp_ts + t0 * out_ts + ms;
     T x[K], l1[K], l2[K], sa0[K], sb0[K];
     int lf[K];
-    cplx_t<T> He[K], Ho[K];
+    cplx_t<T> He[K][NB], Ho[K][NB];
     int mylf = INT_MAX;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const long long ip = (long long)grp * 64 * K + lane + 64 * k;
       x[k] = l1[k] = l2[k] = sa0[k] = sb0[k] = 0;
       lf[k] = INT_MAX;
-      He[k] = Ho[k] = {0, 0};
+#pragma unroll
+      for (int b = 0; b < NB; ++b) He[k][b] = Ho[k][b] = {0, 0};
       if (ip < g.npairs) {
         const long long si = (long long)m * g.npairs + ip;
         lf[k] = lfs[si];
@@ -388,21 +417,28 @@ adj0_kernel(LegArgs g, AdjItems it, const T *__restrict__ xs, const int *__restr
           sa0[k] = seeds[2 * si];
           sb0[k] = seeds[2 * si + 1];
           const int rn = g.pair_n[ip], rs = g.pair_s[ip];
-          const cplx_t<T> gn = G[(t * g.nrings + rn) * stride + m];
-          if (rs >= 0) {
-            const cplx_t<T> gs = G[(t * g.nrings + rs) * stride + m];
-            He[k] = {gn.x + gs.x, gn.y + gs.y};
-            Ho[k] = {gn.x - gs.x, gn.y - gs.y};
-          } else {
-            He[k] = gn;
-            Ho[k] = gn;
+#pragma unroll
+          for (int b = 0; b < NB; ++b) {
+            if (b < nt) {
+              const long long t = t0 + b;
+              const cplx_t<T> gn = G[(t * g.nrings + rn) * stride + m];
+              if (rs >= 0) {
+                const cplx_t<T> gs = G[(t * g.nrings + rs) * stride + m];
+                He[k][b] = {gn.x + gs.x, gn.y + gs.y};
+                Ho[k][b] = {gn.x - gs.x, gn.y - gs.y};
+              } else {
+                He[k][b] = gn;
+                Ho[k][b] = gn;
+              }
+            }
           }
         }
         mylf = min(mylf, lf[k]);
       }
     }
     const int lmin = __builtin_amdgcn_readfirstlane(wave_min(mylf));
-    for (int l = m + lane; l < min(lmin, lmax + 1); l += 64) om[l] = {0, 0};
+    for (int b = 0; b < nt; ++b)
+      for (int l = m + lane; l < min(lmin, lmax + 1); l += 64) om[b * out_ts + l] = {0, 0};
 
     for (int l0 = lmin; l0 <= lmax; l0 += CH) {
       const int nl = min(CH, lmax - l0 + 1);
@@ -416,15 +452,19 @@ adj0_kernel(LegArgs g, AdjItems it, const T *__restrict__ xs, const int *__restr
       // weights of the first and second multipole of the chunk (parity
       // alternates with l, so the selection is made once per chunk)
       const bool odd0 = (l0 - m) & 1;
-      cplx_t<T> H0[K], H1[K];
+      cplx_t<T> H0[K][NB], H1[K][NB];
 #pragma unroll
-      for (int k = 0; k < K; ++k) {
-        H0[k] = odd0 ? Ho[k] : He[k];
-        H1[k] = odd0 ? He[k] : Ho[k];
-      }
-      cplx_t<T> s[CH];
+      for (int k = 0; k < K; ++k)
 #pragma unroll
-      for (int i = 0; i < CH; ++i) s[i] = {0, 0};
+        for (int b = 0; b < NB; ++b) {
+          H0[k][b] = odd0 ? Ho[k][b] : He[k][b];
+          H1[k][b] = odd0 ? He[k][b] : Ho[k][b];
+        }
+      cplx_t<T> s[CH][NB];
+#pragma unroll
+      for (int i = 0; i < CH; ++i)
+#pragma unroll
+        for (int b = 0; b < NB; ++b) s[i][b] = {0, 0};
       if (nl == CH && !__any(inj)) {
         // steady state: plain recurrence over a full chunk
 #pragma unroll
@@ -434,7 +474,9 @@ adj0_kernel(LegArgs g, AdjItems it, const T *__restrict__ xs, const int *__restr
             const T lam = fma(Ac[i] * x[k], l1[k], -l2[k]);
             l2[k] = l1[k];
             l1[k] = lam;
-            s[i] = cfma(lam, (i & 1) ? H1[k] : H0[k], s[i]);
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+              s[i][b] = cfma(lam, (i & 1) ? H1[k][b] : H0[k][b], s[i][b]);
           }
         }
       } else {
@@ -449,47 +491,51 @@ adj0_kernel(LegArgs g, AdjItems it, const T *__restrict__ xs, const int *__restr
               const bool st = (l == lf[k]);
               l2[k] = st ? sa0[k] : l1[k];
               l1[k] = st ? sb0[k] : lam;
-              s[i] = cfma(l1[k], (i & 1) ? H1[k] : H0[k], s[i]);
+#pragma unroll
+              for (int b = 0; b < NB; ++b)
+                s[i][b] = cfma(l1[k], (i & 1) ? H1[k][b] : H0[k][b], s[i][b]);
             }
           }
         }
       }
       wave_sync();
 #pragma unroll
-      for (int i = 0; i < CH; ++i) red[i][lane] = s[i];
+      for (int i = 0; i < CH; ++i)
+#pragma unroll
+        for (int b = 0; b < NB; ++b) red[i * NB + b][lane] = s[i][b];
       wave_sync();
-      const cplx_t<T> tot = wave_rows<T, CH>(red, lane);
-      if (writer && row < nl) {
-        const int l = l0 + row;
+      const cplx_t<T> tot = wave_rows<T, ROWS>(red, lane);
+      if (writer && ri < nl && rb < nt) {
+        const int l = l0 + ri;
         const T c = cf[2 * l + 1];
-        om[l] = {c * tot.x, c * tot.y};
+        om[rb * out_ts + l] = {c * tot.x, c * tot.y};
       }
     }
   }
 }
 
-template <typename T, int K, int CH>
+template <typename T, int K, int CH, int NB>
 __global__ void __launch_bounds__(64 * kAdjWaves)
 adj2_kernel(LegArgs g, AdjItems it, const T *__restrict__ xs, const int *__restrict__ lfps,
             const T *__restrict__ sps, const int *__restrict__ lfms, const T *__restrict__ sms,
-            const T *__restrict__ coef, const cplx_t<T> *__restrict__ G, cplx_t<T> *out,
-            long long out_ts) {
-  __shared__ cplx_t<T> redP_all[kAdjWaves][CH][64];
-  __shared__ cplx_t<T> redM_all[kAdjWaves][CH][64];
+            const T *__restrict__ coef, const cplx_t<T> *__restrict__ G, long long ntrans,
+            cplx_t<T> *out, long long out_ts) {
+  constexpr int ROWS = CH * NB;
+  __shared__ cplx_t<T> redP_all[kAdjWaves][ROWS][64];
+  __shared__ cplx_t<T> redM_all[kAdjWaves][ROWS][64];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const long long item = (long long)blockIdx.x * kAdjWaves + wave;
   if (item >= (long long)it.nmpairs * it.ngroups) return;
   cplx_t<T>(*redP)[64] = redP_all[wave];
   cplx_t<T>(*redM)[64] = redM_all[wave];
   const int q = (int)(item / it.ngroups), grp = (int)(item % it.ngroups);
-  const long long t = blockIdx.y;
+  const long long t0 = (long long)blockIdx.y * NB;
+  const int nt = (int)min((long long)NB, ntrans - t0);
   const int lmax = g.lmax;
   const long long nelem = nelem_of(lmax);
   const long long stride = g.mmax + 1;
-  const cplx_t<T> *Gq = G + (t * 2 * g.nrings) * stride;
-  const cplx_t<T> *Gu = Gq + g.nrings * stride;
-  constexpr int R = 64 / CH;
-  const int row = lane / R;
+  constexpr int R = 64 / ROWS;
+  const int row = lane / R, ri = row / NB, rb = row % NB;
   const bool writer = (lane % R) == 0;
 
   for (int pass = 0; pass < 2; ++pass) {
@@ -497,18 +543,18 @@ adj2_kernel(LegArgs g, AdjItems it, const T *__restrict__ xs, const int *__restr
     if (pass == 1 && m == q) break;
     const long long ms = mstart_of(m, lmax);
     const T *cf = coef + 3 * ms;
-    cplx_t<T> *oE = out + grp * it.grp_ts + t * out_ts + ms;
-    cplx_t<T> *oB = oE + nelem;
+    cplx_t<T> *oE = out + grp * it.grp_ts + t0 * out_ts + ms;
     T x[K], p1[K], p2[K], q1[K], q2[K], pa[K], pb[K], qa[K], qb[K];
     int lfp[K], lfm[K];
-    cplx_t<T> ZpN[K], ZmN[K], ZpS[K], ZmS[K];
+    cplx_t<T> ZpN[K][NB], ZmN[K][NB], ZpS[K][NB], ZmS[K][NB];
     int mylf = INT_MAX;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const long long ip = (long long)grp * 64 * K + lane + 64 * k;
       x[k] = p1[k] = p2[k] = q1[k] = q2[k] = pa[k] = pb[k] = qa[k] = qb[k] = 0;
       lfp[k] = lfm[k] = INT_MAX;
-      ZpN[k] = ZmN[k] = ZpS[k] = ZmS[k] = {0, 0};
+#pragma unroll
+      for (int b = 0; b < NB; ++b) ZpN[k][b] = ZmN[k][b] = ZpS[k][b] = ZmS[k][b] = {0, 0};
       if (ip < g.npairs) {
         const long long si = (long long)m * g.npairs + ip;
         lfp[k] = lfps[si];
@@ -520,23 +566,31 @@ adj2_kernel(LegArgs g, AdjItems it, const T *__restrict__ xs, const int *__restr
           qa[k] = sms[2 * si];
           qb[k] = sms[2 * si + 1];
           const int rn = g.pair_n[ip], rs = g.pair_s[ip];
-          const cplx_t<T> qn = Gq[rn * stride + m], un = Gu[rn * stride + m];
-          ZpN[k] = {qn.x - un.y, qn.y + un.x};
-          ZmN[k] = {qn.x + un.y, qn.y - un.x};
-          if (rs >= 0) {
-            const cplx_t<T> qs = Gq[rs * stride + m], us = Gu[rs * stride + m];
-            ZpS[k] = {qs.x - us.y, qs.y + us.x};
-            ZmS[k] = {qs.x + us.y, qs.y - us.x};
+#pragma unroll
+          for (int b = 0; b < NB; ++b) {
+            if (b < nt) {
+              const cplx_t<T> *Gq = G + ((t0 + b) * 2 * g.nrings) * stride;
+              const cplx_t<T> *Gu = Gq + g.nrings * stride;
+              const cplx_t<T> qn = Gq[rn * stride + m], un = Gu[rn * stride + m];
+              ZpN[k][b] = {qn.x - un.y, qn.y + un.x};
+              ZmN[k][b] = {qn.x + un.y, qn.y - un.x};
+              if (rs >= 0) {
+                const cplx_t<T> qs = Gq[rs * stride + m], us = Gu[rs * stride + m];
+                ZpS[k][b] = {qs.x - us.y, qs.y + us.x};
+                ZmS[k][b] = {qs.x + us.y, qs.y - us.x};
+              }
+            }
           }
         }
         mylf = min(mylf, min(lfp[k], lfm[k]));
       }
     }
     const int lmin = __builtin_amdgcn_readfirstlane(wave_min(mylf));
-    for (int l = m + lane; l < min(lmin, lmax + 1); l += 64) {
-      oE[l] = {0, 0};
-      oB[l] = {0, 0};
-    }
+    for (int b = 0; b < nt; ++b)
+      for (int l = m + lane; l < min(lmin, lmax + 1); l += 64) {
+        oE[b * out_ts + l] = {0, 0};
+        oE[b * out_ts + nelem + l] = {0, 0};
+      }
 
     for (int l0 = lmin; l0 <= lmax; l0 += CH) {
       const int nl = min(CH, lmax - l0 + 1);
@@ -555,15 +609,19 @@ adj2_kernel(LegArgs g, AdjItems it, const T *__restrict__ xs, const int *__restr
       // south weights with the sign of the chunk's first multipole; the
       // sign alternates with l
       const T sg0 = ((l0 - m) & 1) ? (T)-1 : (T)1;
-      cplx_t<T> PS[K], MS[K];
+      cplx_t<T> PS[K][NB], MS[K][NB];
 #pragma unroll
-      for (int k = 0; k < K; ++k) {
-        PS[k] = {sg0 * ZpS[k].x, sg0 * ZpS[k].y};
-        MS[k] = {sg0 * ZmS[k].x, sg0 * ZmS[k].y};
-      }
-      cplx_t<T> sP[CH], sM[CH];
+      for (int k = 0; k < K; ++k)
 #pragma unroll
-      for (int i = 0; i < CH; ++i) sP[i] = sM[i] = {0, 0};
+        for (int b = 0; b < NB; ++b) {
+          PS[k][b] = {sg0 * ZpS[k][b].x, sg0 * ZpS[k][b].y};
+          MS[k][b] = {sg0 * ZmS[k][b].x, sg0 * ZmS[k][b].y};
+        }
+      cplx_t<T> sP[CH][NB], sM[CH][NB];
+#pragma unroll
+      for (int i = 0; i < CH; ++i)
+#pragma unroll
+        for (int b = 0; b < NB; ++b) sP[i][b] = sM[i][b] = {0, 0};
       if (nl == CH && !__any(inj)) {
         // steady state: plain recurrences over a full chunk
         // even l-m: S+ += lam+ ZpN + lam- ZpS, S- += lam- ZmN + lam+ ZmS
@@ -580,10 +638,13 @@ adj2_kernel(LegArgs g, AdjItems it, const T *__restrict__ xs, const int *__restr
             q2[k] = q1[k];
             q1[k] = nq_;
             const T nqs = (i & 1) ? -nq_ : nq_, nps = (i & 1) ? -np_ : np_;
-            sP[i] = cfma(np_, ZpN[k], sP[i]);
-            sP[i] = cfma(nqs, PS[k], sP[i]);
-            sM[i] = cfma(nq_, ZmN[k], sM[i]);
-            sM[i] = cfma(nps, MS[k], sM[i]);
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+              sP[i][b] = cfma(np_, ZpN[k][b], sP[i][b]);
+              sP[i][b] = cfma(nqs, PS[k][b], sP[i][b]);
+              sM[i][b] = cfma(nq_, ZmN[k][b], sM[i][b]);
+              sM[i][b] = cfma(nps, MS[k][b], sM[i][b]);
+            }
           }
         }
       } else {
@@ -603,29 +664,35 @@ adj2_kernel(LegArgs g, AdjItems it, const T *__restrict__ xs, const int *__restr
               q2[k] = sq_ ? qa[k] : q1[k];
               q1[k] = sq_ ? qb[k] : nq_;
               const T nqs = (i & 1) ? -q1[k] : q1[k], nps = (i & 1) ? -p1[k] : p1[k];
-              sP[i] = cfma(p1[k], ZpN[k], sP[i]);
-              sP[i] = cfma(nqs, PS[k], sP[i]);
-              sM[i] = cfma(q1[k], ZmN[k], sM[i]);
-              sM[i] = cfma(nps, MS[k], sM[i]);
+#pragma unroll
+              for (int b = 0; b < NB; ++b) {
+                sP[i][b] = cfma(p1[k], ZpN[k][b], sP[i][b]);
+                sP[i][b] = cfma(nqs, PS[k][b], sP[i][b]);
+                sM[i][b] = cfma(q1[k], ZmN[k][b], sM[i][b]);
+                sM[i][b] = cfma(nps, MS[k][b], sM[i][b]);
+              }
             }
           }
         }
       }
       wave_sync();
 #pragma unroll
-      for (int i = 0; i < CH; ++i) {
-        redP[i][lane] = sP[i];
-        redM[i][lane] = sM[i];
-      }
+      for (int i = 0; i < CH; ++i)
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          redP[i * NB + b][lane] = sP[i][b];
+          redM[i * NB + b][lane] = sM[i][b];
+        }
       wave_sync();
-      const cplx_t<T> tp = wave_rows<T, CH>(redP, lane);
-      const cplx_t<T> tm = wave_rows<T, CH>(redM, lane);
-      if (writer && row < nl) {
-        const int l = l0 + row;
+      const cplx_t<T> tp = wave_rows<T, ROWS>(redP, lane);
+      const cplx_t<T> tm = wave_rows<T, ROWS>(redM, lane);
+      if (writer && ri < nl && rb < nt) {
+        const int l = l0 + ri;
         const T c = (T)0.5 * cf[3 * l + 2];
         // aE = -(S+ + S-)/2, aB = (i/2)(S+ - S-)
-        oE[l] = {-c * (tp.x + tm.x), -c * (tp.y + tm.y)};
-        oB[l] = {-c * (tp.y - tm.y), c * (tp.x - tm.x)};
+        cplx_t<T> *o = oE + rb * out_ts;
+        o[l] = {-c * (tp.x + tm.x), -c * (tp.y + tm.y)};
+        o[nelem + l] = {-c * (tp.y - tm.y), c * (tp.x - tm.x)};
       }
     }
   }
@@ -666,10 +733,16 @@ LegArgs leg_args(owmi_plan_impl *p) {
   return a;
 }
 
-// Chunk lengths of the adjoint (multipoles between wave reductions).
-template <typename T> struct AdjCH;
-template <> struct AdjCH<double> { static constexpr int s0 = 16, s2 = 8; };
-template <> struct AdjCH<float> { static constexpr int s0 = 16, s2 = 16; };
+// Reduction rows per adjoint chunk (multipoles x transforms) and the
+// widest transform batch per launch, bounded by LDS (4 waves x rows x 64
+// lanes x complex, twice for spin 2) and registers.
+template <typename T> struct AdjCfg;
+template <> struct AdjCfg<double> {
+  static constexpr int rows0 = 16, rows2 = 8, nb0 = 4, nb2 = 2;
+};
+template <> struct AdjCfg<float> {
+  static constexpr int rows0 = 16, rows2 = 16, nb0 = 4, nb2 = 4;
+};
 
 }  // namespace
 
@@ -683,22 +756,34 @@ void launch_seed_finder(owmi_plan_impl *p, const double *xs, const double *v0, c
   OWMI_HIP(hipGetLastError());
 }
 
+template <typename T, int NB>
+static void launch_synth_nb(owmi_plan_impl *p, int spin, const cplx_t<T> *alm, long long alm_ts,
+                            cplx_t<T> *G, long long ntrans, hipStream_t st) {
+  PlanTables<T> &tb = tab<T>(p);
+  int bs = (int)std::min<long long>(256, ((p->npairs + 63) / 64) * 64);
+  dim3 grid(ceil_div(p->npairs, bs), (unsigned)(p->mmax + 1), ceil_div(ntrans, NB));
+  if (spin == 0)
+    hipLaunchKernelGGL((synth0_kernel<T, NB>), grid, dim3(bs), 0, st, leg_args(p), tb.x, tb.lf0,
+                       tb.s0, tb.coef0, alm, alm_ts, ntrans, G);
+  else
+    hipLaunchKernelGGL((synth2_kernel<T, NB>), grid, dim3(bs), 0, st, leg_args(p), tb.x, tb.lfp,
+                       tb.sp, tb.lfm, tb.sm, tb.coef2, alm, alm_ts, ntrans, G);
+}
+
+// Transforms per block: the widest of 4, 2, 1 not exceeding ntrans.
 template <typename T>
 void launch_synthesis(owmi_plan_impl *p, int spin, const cplx_t<T> *alm, long long alm_ts,
                       cplx_t<T> *G, long long ntrans, hipStream_t st) {
-  OWMI_REQUIRE(ntrans < 65536, "too many transforms for one launch");
-  PlanTables<T> &tb = tab<T>(p);
-  int bs = (int)std::min<long long>(256, ((p->npairs + 63) / 64) * 64);
-  dim3 grid(ceil_div(p->npairs, bs), (unsigned)(p->mmax + 1), (unsigned)ntrans);
-  if (spin == 0)
-    hipLaunchKernelGGL(synth0_kernel<T>, grid, dim3(bs), 0, st, leg_args(p), tb.x, tb.lf0, tb.s0,
-                       tb.coef0, alm, alm_ts, G);
+  OWMI_REQUIRE(ntrans >= 1 && ntrans < 65536, "ntrans must be in [1, 65535]");
+  if (ntrans >= 4)
+    launch_synth_nb<T, 4>(p, spin, alm, alm_ts, G, ntrans, st);
+  else if (ntrans >= 2)
+    launch_synth_nb<T, 2>(p, spin, alm, alm_ts, G, ntrans, st);
   else
-    hipLaunchKernelGGL(synth2_kernel<T>, grid, dim3(bs), 0, st, leg_args(p), tb.x, tb.lfp, tb.sp,
-                       tb.lfm, tb.sm, tb.coef2, alm, alm_ts, G);
+    launch_synth_nb<T, 1>(p, spin, alm, alm_ts, G, ntrans, st);
 }
 
-template <typename T, int K>
+template <typename T, int K, int NB>
 static void launch_adj_k(owmi_plan_impl *p, int spin, const cplx_t<T> *G, cplx_t<T> *alm,
                          long long alm_ts, long long ntrans, hipStream_t st) {
   PlanTables<T> &tb = tab<T>(p);
@@ -720,14 +805,15 @@ static void launch_adj_k(owmi_plan_impl *p, int spin, const cplx_t<T> *G, cplx_t
     it.grp_ts = 0;
   }
   const long long nitems = (long long)it.nmpairs * it.ngroups;
-  dim3 grid(ceil_div(nitems, kAdjWaves), (unsigned)ntrans);
+  dim3 grid(ceil_div(nitems, kAdjWaves), ceil_div(ntrans, NB));
   if (spin == 0)
-    hipLaunchKernelGGL((adj0_kernel<T, K, AdjCH<T>::s0>), grid, dim3(64 * kAdjWaves), 0, st,
-                       leg_args(p), it, tb.x, tb.lf0, tb.s0, tb.coef0, G, dst, dst_ts);
+    hipLaunchKernelGGL((adj0_kernel<T, K, AdjCfg<T>::rows0 / NB, NB>), grid,
+                       dim3(64 * kAdjWaves), 0, st, leg_args(p), it, tb.x, tb.lf0, tb.s0,
+                       tb.coef0, G, ntrans, dst, dst_ts);
   else
-    hipLaunchKernelGGL((adj2_kernel<T, K, AdjCH<T>::s2>), grid, dim3(64 * kAdjWaves), 0, st,
-                       leg_args(p), it, tb.x, tb.lfp, tb.sp, tb.lfm, tb.sm, tb.coef2, G, dst,
-                       dst_ts);
+    hipLaunchKernelGGL((adj2_kernel<T, K, AdjCfg<T>::rows2 / NB, NB>), grid,
+                       dim3(64 * kAdjWaves), 0, st, leg_args(p), it, tb.x, tb.lfp, tb.sp, tb.lfm,
+                       tb.sm, tb.coef2, G, ntrans, dst, dst_ts);
   if (it.ngroups > 1) {
     const long long n = nc * nelem;
     dim3 g2(ceil_div(n, 256), (unsigned)ntrans);
@@ -736,11 +822,26 @@ static void launch_adj_k(owmi_plan_impl *p, int spin, const cplx_t<T> *G, cplx_t
   }
 }
 
+template <typename T, int NB>
+static void launch_adj_spin(owmi_plan_impl *p, int spin, const cplx_t<T> *G, cplx_t<T> *alm,
+                            long long alm_ts, long long ntrans, hipStream_t st) {
+  constexpr int nbmax = NB;
+  if (ntrans >= 4 && nbmax >= 4)
+    launch_adj_k<T, 2, (nbmax >= 4 ? 4 : 1)>(p, spin, G, alm, alm_ts, ntrans, st);
+  else if (ntrans >= 2 && nbmax >= 2)
+    launch_adj_k<T, 2, (nbmax >= 2 ? 2 : 1)>(p, spin, G, alm, alm_ts, ntrans, st);
+  else
+    launch_adj_k<T, 2, 1>(p, spin, G, alm, alm_ts, ntrans, st);
+}
+
 template <typename T>
 void launch_adjoint(owmi_plan_impl *p, int spin, const cplx_t<T> *G, cplx_t<T> *alm,
                     long long alm_ts, long long ntrans, hipStream_t st) {
-  OWMI_REQUIRE(ntrans < 65536, "too many transforms for one launch");
-  launch_adj_k<T, 2>(p, spin, G, alm, alm_ts, ntrans, st);
+  OWMI_REQUIRE(ntrans >= 1 && ntrans < 65536, "ntrans must be in [1, 65535]");
+  if (spin == 0)
+    launch_adj_spin<T, AdjCfg<T>::nb0>(p, spin, G, alm, alm_ts, ntrans, st);
+  else
+    launch_adj_spin<T, AdjCfg<T>::nb2>(p, spin, G, alm, alm_ts, ntrans, st);
 }
 
 template void launch_synthesis<double>(owmi_plan_impl *, int, const double2 *, long long,

@@ -6,14 +6,17 @@
 // complex sequence (z = a + i b). Any N is handled with Bluestein's
 // algorithm: DFT_N(x)_k = w_k sum_j (x_j w_j) conj(w_{k-j}), w_n =
 // exp(-i pi n^2 / N), evaluated as a cyclic convolution of length
-// M = 2^ceil(log2(2N - 1)) with a radix-2 decimation-in-time FFT
-// (bit-reversed input, natural output) and a decimation-in-frequency
-// inverse (natural input, bit-reversed output), both in LDS. The kernel
-// spectrum FFT_M(conj w) / M is precomputed per distinct N.
+// M = 2^ceil(log2(2N - 1)) (at least 16) with radix-16 Stockham FFTs whose
+// data stay in registers (16 elements per thread) between passes through
+// LDS. The kernel spectrum FFT_M(conj w) / M is precomputed per distinct N.
+// The sandwich mode runs synthesis, the pixel multiply and the analysis of
+// one ring pair without leaving the workgroup.
 //
 // Frequencies m > N/2 alias onto bin m mod N (reduced rings of equal-area
 // grids), and the first-pixel longitude phi0 enters as exp(i m phi0).
 #include "sht_internal.hpp"
+
+#include <algorithm>
 
 namespace owmi {
 
@@ -66,46 +69,11 @@ __device__ void fft_dit(cplx_t<T> *buf, int logm, const cplx_t<T> *tw, int log_m
   }
 }
 
-// In-place radix-2 DIF inverse FFT (exp(+2 pi i / M), unnormalised) on
-// natural-order input; output in bit-reversed order.
-template <typename T>
-__device__ void ifft_dif(cplx_t<T> *buf, int logm, const cplx_t<T> *tw, int log_mmax) {
-  const int M = 1 << logm;
-  const int half = M >> 1;
-  for (int s = logm - 1; s >= 0; --s) {
-    const int h = 1 << s;
-    const int twshift = log_mmax - s - 1;
-    for (int b = threadIdx.x; b < half; b += blockDim.x) {
-      int j = b & (h - 1);
-      int i0 = ((b >> s) << (s + 1)) + j;
-      int i1 = i0 + h;
-      cplx_t<T> w = cconj<T>(tw[j << twshift]);
-      cplx_t<T> u = buf[i0];
-      cplx_t<T> v = buf[i1];
-      buf[i0] = cadd<T>(u, v);
-      buf[i1] = cmul<T>(csub<T>(u, v), w);
-    }
-    __syncthreads();
-  }
-}
-
-// After loading buf[bitrev(j)] = z_j w_j (zero for j >= N): leaves
-// DFT_N(z)_k / w_k ... i.e. the Bluestein convolution c_k at buf[bitrev(k)].
-template <typename T>
-__device__ void bluestein_core(cplx_t<T> *buf, int logm, const cplx_t<T> *tw, int log_mmax,
-                               const cplx_t<T> *bhat) {
-  fft_dit<T>(buf, logm, tw, log_mmax);
-  const int M = 1 << logm;
-  for (int k = threadIdx.x; k < M; k += blockDim.x) buf[k] = cmul<T>(buf[k], bhat[k]);
-  __syncthreads();
-  ifft_dif<T>(buf, logm, tw, log_mmax);
-}
-
 // Folded, phase-shifted Hermitian spectrum bin k of one ring:
 // Z_k = sum_{m = k mod N} C_m + sum_{m >= 1, m = -k mod N} conj(C_m),
 // C_0 = Re F_0, C_m = F_m exp(i m phi0).
 template <typename T>
-__device__ cplx_t<T> fold_bin(const cplx_t<T> *F, long long k, long long N, int mmax, T phi0) {
+__device__ __attribute__((noinline)) cplx_t<T> fold_bin(const cplx_t<T> *F, long long k, long long N, int mmax, T phi0) {
   cplx_t<T> acc = {0, 0};
   for (long long m = k; m <= mmax; m += N) {
     cplx_t<T> c = F[m];
@@ -130,6 +98,185 @@ __device__ cplx_t<T> fold_bin(const cplx_t<T> *F, long long k, long long N, int 
   return acc;
 }
 
+// ------------------------------------------------------------------ FFT
+//
+// Stockham autosort FFT of length M = 2^logm (logm >= 4) held by Q = M/16
+// threads, thread q owning the 16 elements q + s Q (s < 16) in registers
+// on entry and on exit. Passes: a first pass of radix 2^(logm mod 4) (if
+// nonzero) then radix-16 passes; pass p with radix R and span Ns applies
+// the twiddles W_{Ns R}^{(j mod Ns) r} to the inputs of butterfly j, a
+// radix-R DFT in registers, and stores to (j / Ns) Ns R + (j mod Ns) + r Ns
+// (Govindaraju et al. 2008). Between passes the data go through LDS
+// (padded by one element per 16 against bank conflicts); the last
+// radix-16 pass leaves its outputs in the thread's own slots.
+// DIR = -1: exp(-2 pi i jk / M); DIR = +1: exp(+2 pi i jk / M), unscaled.
+
+__device__ __forceinline__ int pad16(int i) { return i + (i >> 4); }
+
+template <typename T, int DIR>
+__device__ __forceinline__ cplx_t<T> rot_mi(cplx_t<T> a) {
+  // a * (DIR i): forward -i, inverse +i
+  return DIR < 0 ? cplx_t<T>{a.y, -a.x} : cplx_t<T>{-a.y, a.x};
+}
+
+// In-register DFT of the R values v[o], v[o + st], ..., v[o + (R-1) st].
+template <typename T, int DIR, int R>
+struct Dft;
+
+template <typename T, int DIR>
+struct Dft<T, DIR, 2> {
+  __device__ static __forceinline__ void run(cplx_t<T> *v, int o, int st) {
+    const cplx_t<T> a = v[o], b = v[o + st];
+    v[o] = cadd<T>(a, b);
+    v[o + st] = csub<T>(a, b);
+  }
+};
+
+template <typename T, int DIR>
+struct Dft<T, DIR, 4> {
+  __device__ static __forceinline__ void run(cplx_t<T> *v, int o, int st) {
+    const cplx_t<T> x0 = v[o], x1 = v[o + st], x2 = v[o + 2 * st], x3 = v[o + 3 * st];
+    const cplx_t<T> a0 = cadd<T>(x0, x2), a1 = csub<T>(x0, x2);
+    const cplx_t<T> a2 = cadd<T>(x1, x3), a3 = rot_mi<T, DIR>(csub<T>(x1, x3));
+    v[o] = cadd<T>(a0, a2);
+    v[o + 2 * st] = csub<T>(a0, a2);
+    v[o + st] = cadd<T>(a1, a3);
+    v[o + 3 * st] = csub<T>(a1, a3);
+  }
+};
+
+// W_R^q for R in {8, 16} with sign DIR (q < R).
+template <typename T, int DIR>
+__device__ __forceinline__ cplx_t<T> wr16(int q) {
+  // cos / sin of 2 pi q / 16
+  constexpr T c1 = (T)0.92387953251128675613, s1 = (T)0.38268343236508977173;
+  constexpr T h = (T)0.70710678118654752440;
+  const T cs[16] = {1, c1, h, s1, 0, -s1, -h, -c1, -1, -c1, -h, -s1, 0, s1, h, c1};
+  const T sn[16] = {0, s1, h, c1, 1, c1, h, s1, 0, -s1, -h, -c1, -1, -c1, -h, -s1};
+  return {cs[q & 15], (T)DIR * sn[q & 15]};
+}
+
+// R = R1 R2 with R1 = 4: input n = R2 n1 + n2, output k = k1 + 4 k2.
+template <typename T, int DIR, int R2>
+__device__ __forceinline__ void dft4xr2(cplx_t<T> *v, int o, int st) {
+  constexpr int R = 4 * R2;
+  cplx_t<T> y[R];
+#pragma unroll
+  for (int n = 0; n < R; ++n) y[n] = v[o + n * st];
+  // inner radix-4 DFTs over n1 for each n2 (stride R2 in y)
+#pragma unroll
+  for (int n2 = 0; n2 < R2; ++n2) Dft<T, DIR, 4>::run(y, n2, R2);
+  // now y[R2 k1 + n2]; twiddle W_R^{n2 k1}
+#pragma unroll
+  for (int n2 = 1; n2 < R2; ++n2)
+#pragma unroll
+    for (int k1 = 1; k1 < 4; ++k1)
+      y[R2 * k1 + n2] = cmul<T>(y[R2 * k1 + n2], wr16<T, DIR>((16 / R) * n2 * k1));
+  // outer radix-R2 DFTs over n2 for each k1 (contiguous in y)
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1) Dft<T, DIR, R2>::run(y, R2 * k1, 1);
+  // y[R2 k1 + k2] holds X[k1 + 4 k2]
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1)
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) v[o + (k1 + 4 * k2) * st] = y[R2 * k1 + k2];
+}
+
+template <typename T, int DIR>
+struct Dft<T, DIR, 8> {
+  __device__ static __forceinline__ void run(cplx_t<T> *v, int o, int st) {
+    dft4xr2<T, DIR, 2>(v, o, st);
+  }
+};
+
+template <typename T, int DIR>
+struct Dft<T, DIR, 16> {
+  __device__ static __forceinline__ void run(cplx_t<T> *v, int o, int st) {
+    dft4xr2<T, DIR, 4>(v, o, st);
+  }
+};
+
+// First pass of radix R (< 16): butterfly j = q + u Q (u < 16 / R) takes
+// v[u + r (16 / R)], Ns = 1, stores at j R + r.
+template <typename T, int DIR, int R>
+__device__ __forceinline__ void first_pass_small(cplx_t<T> *v, cplx_t<T> *buf, int q, int Q) {
+  constexpr int U = 16 / R;
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    cplx_t<T> y[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) y[r] = v[u + r * U];
+    Dft<T, DIR, R>::run(y, 0, 1);
+    const int j = q + u * Q;
+#pragma unroll
+    for (int r = 0; r < R; ++r) buf[pad16(j * R + r)] = y[r];
+  }
+}
+
+// Full FFT of the thread's 16 slots (see above). All threads of the block
+// call it (barriers); threads with q >= Q pass through.
+template <typename T, int DIR>
+__device__ __forceinline__ void fft16(cplx_t<T> *v, cplx_t<T> *buf, int logm, int q, const cplx_t<T> *tw,
+                      int log_mmax) {
+  const int Q = 1 << (logm - 4);
+  const bool on = q < Q;
+  const int r0 = logm & 3;
+  int Ns;
+  __syncthreads();  // the previous user of buf has finished reading it
+  if (r0 == 0) {
+    if (logm == 4) {
+      if (on) Dft<T, DIR, 16>::run(v, 0, 1);
+      return;
+    }
+    if (on) {
+      Dft<T, DIR, 16>::run(v, 0, 1);
+#pragma unroll
+      for (int r = 0; r < 16; ++r) buf[pad16(q * 16 + r)] = v[r];
+    }
+    Ns = 16;
+  } else {
+    if (on) {
+      if (r0 == 1)
+        first_pass_small<T, DIR, 2>(v, buf, q, Q);
+      else if (r0 == 2)
+        first_pass_small<T, DIR, 4>(v, buf, q, Q);
+      else
+        first_pass_small<T, DIR, 8>(v, buf, q, Q);
+    }
+    Ns = 1 << r0;
+  }
+  // radix-16 passes; the last one keeps its outputs in registers
+  int lns = r0 == 0 ? 4 : r0;  // log2 Ns
+  while (true) {
+    __syncthreads();
+    const bool last = lns + 4 == logm;
+    const int k = q & (Ns - 1);
+    if (on) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) v[r] = buf[pad16(q + r * Q)];
+      if (k != 0) {
+        const int step = k << (log_mmax - lns - 4);  // k Mmax / (16 Ns)
+#pragma unroll
+        for (int r = 1; r < 16; ++r) {
+          cplx_t<T> w = tw[r * step];
+          if (DIR > 0) w.y = -w.y;
+          v[r] = cmul<T>(v[r], w);
+        }
+      }
+      Dft<T, DIR, 16>::run(v, 0, 1);
+    }
+    if (last) return;
+    __syncthreads();
+    if (on) {
+      const int base = ((q >> lns) << (lns + 4)) + k;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) buf[pad16(base + (r << lns))] = v[r];
+    }
+    Ns <<= 4;
+    lns += 4;
+  }
+}
+
 enum Mode { kSynth = 0, kAdjoint = 1, kSandwich = 2 };
 
 struct RingArgs {
@@ -148,8 +295,25 @@ struct RingArgs {
   int weighted;
 };
 
-template <typename T, int MODE>
-__global__ void __launch_bounds__(256)
+// Cyclic convolution with the ring's Bluestein kernel, in the thread's
+// slots: v <- IDFT_M(DFT_M(v) * bhat) (bhat includes 1/M).
+template <typename T>
+__device__ __forceinline__ void bluestein_conv(cplx_t<T> *v, cplx_t<T> *buf, int logm, int q,
+                                               const cplx_t<T> *tw, int log_mmax,
+                                               const cplx_t<T> *bhat) {
+  const int Q = 1 << (logm - 4);
+  fft16<T, -1>(v, buf, logm, q, tw, log_mmax);
+  if (q < Q) {
+#pragma unroll
+    for (int s = 0; s < 16; ++s) v[s] = cmul<T>(v[s], bhat[q + s * Q]);
+  }
+  fft16<T, 1>(v, buf, logm, q, tw, log_mmax);
+}
+
+// MAXT: the launch's workgroup size bound (M / 16 threads rounded up to
+// 256, 512 or 1024), which sets the register budget per thread.
+template <typename T, int MODE, int MAXT>
+__global__ void __launch_bounds__(MAXT)
 ring_kernel(RingArgs a, const cplx_t<T> *twiddle, const cplx_t<T> *chirp,
             const cplx_t<T> *bhat_all, const T *ringw, const T *phi0s, const cplx_t<T> *Gin,
             cplx_t<T> *Gout, const T *map_in, T *map_out, long long map_ts, const T *diag,
@@ -165,7 +329,9 @@ ring_kernel(RingArgs a, const cplx_t<T> *twiddle, const cplx_t<T> *chirp,
   const int rb = a.fft_b[job];
   const long long N = a.nphi[ra];
   const int logm = a.ring_logm[ra];
-  const int M = 1 << logm;
+  const int Q = 1 << (logm - 4);
+  const int q = threadIdx.x;
+  const bool on = q < Q;
   const cplx_t<T> *w = chirp + a.chirp_off[ra];
   const cplx_t<T> *bhat = bhat_all + a.bhat_off[ra];
   const int mmax = a.mmax;
@@ -173,96 +339,105 @@ ring_kernel(RingArgs a, const cplx_t<T> *twiddle, const cplx_t<T> *chirp,
   const long long G_row_b = tc * a.nrings + (rb >= 0 ? rb : 0);
   const T pa = phi0s[ra];
   const T pb = rb >= 0 ? phi0s[rb] : (T)0;
-
-  // Pointers of the two rings in map space.
   const long long off_a = a.offsets[ra];
   const long long off_b = rb >= 0 ? a.offsets[rb] : 0;
+
+  // slot s of thread q holds index k = q + s Q of the length-M sequences
+  cplx_t<T> v[16];
 
   if (MODE == kSynth || MODE == kSandwich) {
     const cplx_t<T> *Fa = Gin + G_row_a * (mmax + 1);
     const cplx_t<T> *Fb = Gin + G_row_b * (mmax + 1);
     // Inverse DFT of Z = Za + i Zb through conj: x = conj(DFT(conj Z)).
-    for (int k = threadIdx.x; k < M; k += blockDim.x) {
-      cplx_t<T> v = {0, 0};
-      if (k < N) {
-        cplx_t<T> za = fold_bin<T>(Fa, k, N, mmax, pa);
-        cplx_t<T> zb = rb >= 0 ? fold_bin<T>(Fb, k, N, mmax, pb) : cplx_t<T>{0, 0};
-        cplx_t<T> Z = {za.x - zb.y, za.y + zb.x};
-        v = cmul<T>(cconj<T>(Z), w[k]);
-      }
-      buf[bitrev(k, logm)] = v;
-    }
-    __syncthreads();
-    bluestein_core<T>(buf, logm, twiddle, a.log_mmax_fft, bhat);
-    // c_j sits at buf[bitrev(j)]; DFT(conj Z)_j = w_j c_j and x = conj of
-    // it, so ring a gets Re(w_j c_j) and ring b gets -Im(w_j c_j).
-    for (int j = threadIdx.x; j < N; j += blockDim.x) {
-      unsigned bj = bitrev(j, logm);
-      cplx_t<T> o = cmul<T>(w[j], buf[bj]);
-      T va = o.x;
-      T vb = -o.y;
-      if (MODE == kSynth) {
-        T *row = map_out + t * map_ts + (long long)c * a.npix;
-        if (a.weighted) {
-          va *= ringw[ra];
-          if (rb >= 0) vb *= ringw[rb];
+    if (on) {
+#pragma unroll
+      for (int s = 0; s < 16; ++s) {
+        const long long k = q + s * Q;
+        cplx_t<T> val = {0, 0};
+        if (k < N) {
+          const cplx_t<T> za = fold_bin<T>(Fa, k, N, mmax, pa);
+          const cplx_t<T> zb = rb >= 0 ? fold_bin<T>(Fb, k, N, mmax, pb) : cplx_t<T>{0, 0};
+          const cplx_t<T> Z = {za.x - zb.y, za.y + zb.x};
+          val = cmul<T>(cconj<T>(Z), w[k]);
         }
-        row[off_a + j] = va;
-        if (rb >= 0) row[off_b + j] = vb;
-      } else {
-        // Pixel multiply; each thread rewrites only the slot it read.
-        const T *drow = diag + t * diag_ts + (long long)c * a.npix;
-        va *= drow[off_a + j];
-        vb = rb >= 0 ? vb * drow[off_b + j] : (T)0;
-        buf[bj] = cplx_t<T>{va, vb};
+        v[s] = val;
       }
     }
-    if (MODE == kSandwich) {
-      __syncthreads();
-      // Forward-DFT input: buf[bitrev(j)] = z'_j w_j, zero for j >= N.
-      for (int j = threadIdx.x; j < M; j += blockDim.x) {
-        unsigned bj = bitrev(j, logm);
-        buf[bj] = j < N ? cmul<T>(buf[bj], w[j]) : cplx_t<T>{0, 0};
+    bluestein_conv<T>(v, buf, logm, q, twiddle, a.log_mmax_fft, bhat);
+    // DFT(conj Z)_j = w_j c_j and x = conj of it: ring a gets Re(w_j c_j),
+    // ring b gets -Im(w_j c_j).
+    if (on) {
+      T *row = map_out + t * map_ts + (long long)c * a.npix;
+      const T *drow = diag + t * diag_ts + (long long)c * a.npix;
+#pragma unroll
+      for (int s = 0; s < 16; ++s) {
+        const long long j = q + s * Q;
+        if (j < N) {
+          const cplx_t<T> o = cmul<T>(w[j], v[s]);
+          T va = o.x;
+          T vb = -o.y;
+          if (MODE == kSynth) {
+            if (a.weighted) {
+              va *= ringw[ra];
+              if (rb >= 0) vb *= ringw[rb];
+            }
+            row[off_a + j] = va;
+            if (rb >= 0) row[off_b + j] = vb;
+          } else {
+            // pixel multiply, then the forward-DFT input z'_j w_j
+            va *= drow[off_a + j];
+            vb = rb >= 0 ? vb * drow[off_b + j] : (T)0;
+            v[s] = cmul<T>(cplx_t<T>{va, vb}, w[j]);
+          }
+        } else if (MODE == kSandwich) {
+          v[s] = {0, 0};
+        }
       }
-      __syncthreads();
     }
   }
 
   if (MODE == kAdjoint) {
-    const T *row = map_in + t * map_ts + (long long)c * a.npix;
-    for (int j = threadIdx.x; j < M; j += blockDim.x) {
-      cplx_t<T> v = {0, 0};
-      if (j < N) {
-        T va = row[off_a + j];
-        T vb = rb >= 0 ? row[off_b + j] : (T)0;
-        if (a.weighted) {
-          va *= ringw[ra];
-          if (rb >= 0) vb *= ringw[rb];
+    if (on) {
+      const T *row = map_in + t * map_ts + (long long)c * a.npix;
+#pragma unroll
+      for (int s = 0; s < 16; ++s) {
+        const long long j = q + s * Q;
+        cplx_t<T> val = {0, 0};
+        if (j < N) {
+          T va = row[off_a + j];
+          T vb = rb >= 0 ? row[off_b + j] : (T)0;
+          if (a.weighted) {
+            va *= ringw[ra];
+            if (rb >= 0) vb *= ringw[rb];
+          }
+          val = cmul<T>(cplx_t<T>{va, vb}, w[j]);
         }
-        v = cmul<T>(cplx_t<T>{va, vb}, w[j]);
+        v[s] = val;
       }
-      buf[bitrev(j, logm)] = v;
     }
-    __syncthreads();
   }
 
   if (MODE == kAdjoint || MODE == kSandwich) {
-    bluestein_core<T>(buf, logm, twiddle, a.log_mmax_fft, bhat);
-    // X_k = w_k c_k with c_k at buf[bitrev(k)]; finish in place.
-    for (int k = threadIdx.x; k < N; k += blockDim.x) {
-      unsigned bk = bitrev(k, logm);
-      buf[bk] = cmul<T>(w[k], buf[bk]);
+    bluestein_conv<T>(v, buf, logm, q, twiddle, a.log_mmax_fft, bhat);
+    // X_k = w_k c_k, gathered in LDS (natural order) for the Hermitian split
+    __syncthreads();
+    if (on) {
+#pragma unroll
+      for (int s = 0; s < 16; ++s) {
+        const long long k = q + s * Q;
+        if (k < N) buf[pad16((int)k)] = cmul<T>(w[k], v[s]);
+      }
     }
     __syncthreads();
     cplx_t<T> *Ga = Gout + G_row_a * (mmax + 1);
     cplx_t<T> *Gb = Gout + G_row_b * (mmax + 1);
     for (int m = threadIdx.x; m <= mmax; m += blockDim.x) {
-      long long k = m % N;
-      long long kk = (N - k) % N;
-      cplx_t<T> X = buf[bitrev((unsigned)k, logm)];
-      cplx_t<T> Xc = cconj<T>(buf[bitrev((unsigned)kk, logm)]);
+      const long long k = m % N;
+      const long long kk = (N - k) % N;
+      const cplx_t<T> X = buf[pad16((int)k)];
+      const cplx_t<T> Xc = cconj<T>(buf[pad16((int)kk)]);
       cplx_t<T> xa = {(T)0.5 * (X.x + Xc.x), (T)0.5 * (X.y + Xc.y)};
-      cplx_t<T> d = {(T)0.5 * (X.x - Xc.x), (T)0.5 * (X.y - Xc.y)};
+      const cplx_t<T> d = {(T)0.5 * (X.x - Xc.x), (T)0.5 * (X.y - Xc.y)};
       cplx_t<T> xb = {d.y, -d.x};  // d / i
       if (k == 0) {
         xa.y = 0;
@@ -313,6 +488,12 @@ bluestein_init_kernel(const long long *lens, const int *logms, const long long *
   for (int k = threadIdx.x; k < M; k += blockDim.x) out[k] = {buf[k].x * inv, buf[k].y * inv};
 }
 
+// LDS of one ring workgroup: M + M/16 (padding) complex elements.
+template <typename T>
+size_t ring_lds_bytes(int mmax_fft) {
+  return (size_t)(mmax_fft + mmax_fft / 16) * sizeof(cplx_t<T>);
+}
+
 template <typename K>
 void allow_lds(K kernel, size_t bytes) {
   if (bytes > 64 * 1024)
@@ -353,12 +534,16 @@ void launch_ring(owmi_plan_impl *p, const cplx_t<T> *Gin, cplx_t<T> *Gout, const
   long long nrows = ntrans * ncomp;
   OWMI_REQUIRE(nrows < 65536, "too many rows for one ring-FFT launch");
   OWMI_REQUIRE(p->nfft_jobs < (1LL << 31), "too many rings");
-  size_t lds = (size_t)p->mmax_fft * sizeof(cplx_t<T>);
-  auto kern = ring_kernel<T, MODE>;
+  const size_t lds = ring_lds_bytes<T>(p->mmax_fft);
+  const int threads = std::max(64, p->mmax_fft / 16);
+  OWMI_REQUIRE(threads <= 1024, "ring too long for one workgroup");
+  auto kern = threads <= 256   ? ring_kernel<T, MODE, 256>
+              : threads <= 512 ? ring_kernel<T, MODE, 512>
+                               : ring_kernel<T, MODE, 1024>;
   allow_lds(kern, lds);
   PlanTables<T> &tb = tab<T>(p);
   dim3 grid((unsigned)p->nfft_jobs, (unsigned)nrows);
-  hipLaunchKernelGGL(kern, grid, dim3(256), lds, st, ring_args<T>(p, ncomp, weighted),
+  hipLaunchKernelGGL(kern, grid, dim3(threads), lds, st, ring_args<T>(p, ncomp, weighted),
                      reinterpret_cast<const cplx_t<T> *>(tb.twiddle),
                      reinterpret_cast<const cplx_t<T> *>(tb.chirp),
                      reinterpret_cast<const cplx_t<T> *>(tb.bhat), tb.ringw, tb.phi0, Gin, Gout,

@@ -38,7 +38,7 @@ struct PlanTables {
   T *sm = nullptr;
   T *coef0 = nullptr;      // spin-0 (A', c) per alm index [2 * nelem]
   T *coef2 = nullptr;      // spin-2 (A', B', c) per alm index [3 * nelem]
-  T *twiddle = nullptr;    // exp(-2 pi i k / Mmax), k < Mmax/2, complex [Mmax]
+  T *twiddle = nullptr;    // exp(-2 pi i k / Mmax), k < Mmax, complex [Mmax]
   T *chirp = nullptr;      // per distinct ring length: exp(-i pi n^2 / N), complex
   T *bhat = nullptr;       // per distinct ring length: FFT_M(conj chirp, wrapped) / M
   T *ringw = nullptr;      // ring weight per ring (rounded to T)

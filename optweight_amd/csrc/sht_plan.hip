@@ -303,9 +303,9 @@ static void build_tables(owmi_plan_impl &p, PlanTables<T> &t, const HostTables &
     c2[3 * i + 2] = (T)h.c2[i];
   }
 
-  // Twiddles for the largest FFT length.
-  std::vector<T> tw(2 * (size_t)(p.mmax_fft / 2 + 1));
-  for (long long k = 0; k <= p.mmax_fft / 2; ++k) {
+  // Twiddles exp(-2 pi i k / Mmax), k < Mmax, for the largest FFT length.
+  std::vector<T> tw(2 * (size_t)p.mmax_fft);
+  for (long long k = 0; k < p.mmax_fft; ++k) {
     LD ang = -2.0L * kPi * (LD)k / (LD)p.mmax_fft;
     tw[2 * k] = (T)cosl(ang);
     tw[2 * k + 1] = (T)sinl(ang);
@@ -439,7 +439,7 @@ int create_plan(owmi_sht_plan **out, long long nrings, const double *theta,
       size_t nbhat = 0;
       std::vector<int> ring_logm(nrings);
       std::vector<long long> ring_coff(nrings), ring_boff(nrings);
-      p.mmax_fft = 2;
+      p.mmax_fft = 16;
       for (long long r = 0; r < nrings; ++r) {
         long long n = nphi[r];
         auto it = len_index.find(n);
@@ -449,7 +449,7 @@ int create_plan(owmi_sht_plan **out, long long nrings, const double *theta,
           len_index[n] = li;
           lens.push_back(n);
           int lm = ilog2_ceil(2 * n - 1);
-          if (lm < 1) lm = 1;
+          if (lm < 4) lm = 4;  // the ring FFT works on 16 slots per thread
           logms.push_back(lm);
           chirp_off.push_back((long long)chirp.size() / 2);
           bhat_off.push_back((long long)nbhat);
@@ -571,8 +571,9 @@ long long owmi_sht_plan_npix(const owmi_sht_plan *plan) { return plan ? plan->im
 
 template <typename T>
 static void check_fft_fits(owmi_plan_impl &p) {
-  size_t need = (size_t)p.mmax_fft * sizeof(cplx_t<T>);
-  OWMI_REQUIRE(need <= 128 * 1024,
+  // M + M/16 complex elements of LDS per ring workgroup, at most 160 KiB
+  const size_t need = (size_t)(p.mmax_fft + p.mmax_fft / 16) * sizeof(cplx_t<T>);
+  OWMI_REQUIRE(need <= 160 * 1024,
                "ring length too large for the on-chip FFT at this precision "
                "(Bluestein length " + std::to_string(p.mmax_fft) + ")");
 }

@@ -179,3 +179,35 @@ def test_ring_subset_transforms(dev, spin, npol):
     sht.map2alm(torch.as_tensor(f, device=dev), got, m, alm_utils.AlmInfo(lmax), spin,
                 rings=rings)
     assert rel(got.cpu().numpy(), aref) < 1e-12
+
+
+@pytest.mark.parametrize('ntrans', [2, 3, 5, 9])
+@pytest.mark.parametrize('dtype', [torch.complex128, torch.complex64])
+def test_batched_legendre_tails(dev, ntrans, dtype):
+    '''Batches that are not multiples of the transforms-per-block widths
+    (4, 2) against one-at-a-time transforms: synthesis and adjoint, spin 0
+    and 2, float64 (relative 1e-13) and float32 (2e-6).'''
+    lmax = 70
+    print(f'batched tails ntrans={ntrans} {dtype}', flush=True)
+    g, m = geoms(lmax)
+    ai = alm_utils.AlmInfo(lmax)
+    rdt = torch.float64 if dtype == torch.complex128 else torch.float32
+    tol = 1e-13 if dtype == torch.complex128 else 2e-6
+    a = torch.as_tensor(np.stack([rand_alm(lmax, 3, 10 + s) for s in range(ntrans)]),
+                        device=dev).to(dtype)
+    out = torch.zeros((ntrans, 3, m.npix), dtype=rdt, device=dev)
+    sht.alm2map(a, out, ai, m, [0, 2])
+    back = torch.zeros_like(a)
+    sht.map2alm(out, back, m, ai, [0, 2])
+    for t in range(ntrans):
+        o1 = torch.zeros((3, m.npix), dtype=rdt, device=dev)
+        sht.alm2map(a[t].contiguous(), o1, ai, m, [0, 2])
+        assert rel(out[t].cpu().numpy(), o1.cpu().numpy()) < tol
+        b1 = torch.zeros_like(a[t])
+        sht.map2alm(out[t].contiguous(), b1, m, ai, [0, 2])
+        assert rel(back[t].cpu().numpy(), b1.cpu().numpy()) < tol
+    # and the float64 batch against the oracle
+    if dtype == torch.complex128:
+        ref = np.zeros((3, g.npix))
+        osht.alm2map(a[ntrans - 1].cpu().numpy(), ref, lmax, g, [0, 2])
+        assert rel(out[ntrans - 1].cpu().numpy(), ref) < 1e-12
