@@ -101,34 +101,51 @@ def time_kernel(fn, reps=10):
     return s.elapsed_time(e) / reps
 
 
-def legendre_roofline(prob, dev):
-    '''Spin-2 Legendre synthesis (the dominant kernel) at the workload's
-    shape, timed alone: useful flops N_lm * N_theta * 8 per launch.'''
+def legendre_roofline(prob, dev, batch, reps=10):
+    """Dominant kernel of the timed (mixed, batched) solve: the float32
+    spin-2 Legendre adjoint at the workload's shape (TQU lmax 1023 on the
+    1024-ring GL grid, ``batch`` right-hand sides), launched alone through
+    owmi_sht_legendre and timed with HIP events on its stream. Useful flops
+    per launch: N_lm * N_theta * 8 * batch."""
     plan = sht.get_plan(prob.minfo, prob.lmax, dev)
-    alm = torch.zeros((3, prob.ainfo.nelem), dtype=torch.complex128, device=dev)
-    alm[1:] = 1.0
+    nelem = prob.ainfo.nelem
+    alm = torch.zeros((batch, 3, nelem), dtype=torch.complex64, device=dev)
     st = _lib.stream_ptr(dev)
+    # fill the ring-coefficient workspace with a synthesis of unit alm first
+    alm[:, 1:] = 1.0
+    _lib.check(plan._lib.owmi_sht_legendre(plan.handle, _lib.F32, 2, 0, _lib.ptr(alm[0, 1]),
+                                           3 * nelem, batch, st))
 
     def run():
-        _lib.check(plan._lib.owmi_sht_legendre(plan.handle, _lib.F64, 2, 0,
-                                               _lib.ptr(alm[1]), 3 * prob.ainfo.nelem, 1, st))
-    ms = time_kernel(run)
-    flops = prob.ainfo.nelem * prob.minfo.nrow * 8
+        _lib.check(plan._lib.owmi_sht_legendre(plan.handle, _lib.F32, 2, 1,
+                                               _lib.ptr(alm[0, 1]), 3 * nelem, batch, st))
+    ms = time_kernel(run, reps=reps)
+    flops = nelem * prob.minfo.nrow * 8 * batch
     achieved = flops / (ms * 1e-3) / 1e12
-    return {'kernel': 'synth2_kernel<double>', 'ms': ms, 'flops': flops,
+    return {'kernel': 'adj2_kernel<float, 2, 4, 4>', 'ms': ms, 'flops': flops,
             'achieved': achieved}
 
 
-def cpu_baseline_sample(prob):
-    '''Oracle (numpy float64) A-operator application at the workload's
-    size, on the host. Returns seconds per application.'''
+# HBM bytes per launch of the roofline kernel: FETCH_SIZE (x 2.0) +
+# WRITE_SIZE (x 1.0), factors from a 256 MiB copy in the same process,
+# measured with `rocprofv3 --pmc` on `python bench.py --roofline-only`
+# (profiles/adj2_f32_tqu1023_b16_roofline.md).
+ROOFLINE_TRAFFIC_BYTES = 2.495e9
+
+
+def cpu_baseline_sample(dev, lmax_sample=255, reps=3):
+    """The oracle's float64 A application (numpy, long-double Legendre
+    recurrence, one thread) on the same workload construction at a reduced
+    band limit; returns (seconds per application, lmax_sample)."""
     from oracle import wiener as ow, geometry as og
+    prob = workloads.polarized(lmax_sample, seed=0, device=dev)
     g = og.Geom(prob.minfo.theta, prob.minfo.weight, prob.minfo.nphi, phi0=prob.minfo.phi0)
     osol = ow.WienerMap.from_arrays(prob.imap, g, prob.lmax, prob.icov_ell, prob.icov_pix)
     v = osol.b0.copy()
     t0 = time.perf_counter()
-    osol.a_matrix(v)
-    return time.perf_counter() - t0
+    for _ in range(reps):
+        osol.a_matrix(v)
+    return (time.perf_counter() - t0) / reps, lmax_sample
 
 
 def main():
@@ -139,10 +156,13 @@ def main():
     ap.add_argument('--lmax', type=int, default=1023)
     ap.add_argument('--no-mg', action='store_true')
     ap.add_argument('--cpu-baseline', choices=['on', 'off'], default='on')
-    ap.add_argument('--batch', type=int, default=8,
+    ap.add_argument('--batch', type=int, default=16,
                     help='right-hand sides per mixed-precision batched solve')
-    ap.add_argument('--delta', type=float, default=0.1,
+    ap.add_argument('--delta', type=float, default=0.01,
                     help='residual reduction between float64 residual replacements')
+    ap.add_argument('--roofline-only', action='store_true',
+                    help='only build the workload geometry and time the roofline kernel '
+                         '(for counter runs), after one copy of known size')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py measures one GPU: this build has no multi-GPU path '
@@ -150,6 +170,18 @@ def main():
         sys.exit(2)
     dev = torch.device('cuda:0')
     torch.cuda.set_device(dev)
+
+    if args.roofline_only:
+        # copy of known size first (256 MiB read + 256 MiB written): the
+        # calibration dispatch for FETCH_SIZE / WRITE_SIZE counters
+        src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        prob = workloads.polarized(args.lmax, seed=0, device=dev)
+        roof = legendre_roofline(prob, dev, args.batch, reps=3)
+        print(json.dumps({'roofline_only': roof}))
+        return
 
     log(f'building workload TQU lmax={args.lmax}, batch {args.batch}')
     t0 = time.perf_counter()
@@ -180,21 +212,23 @@ def main():
         f'{resmx.max():.3e}, RHS 0 rel L2 vs fp64 solution {diff:.3e}, '
         f'speedup {mixed_sps / fp64_sps:.2f}x')
     elapsed = elmx
-    roof = legendre_roofline(prob, dev)
+    roof = legendre_roofline(prob, dev, args.batch)
     log(f'roofline: {roof}')
 
     cpu = None
     if args.cpu_baseline == 'on':
-        t_a = cpu_baseline_sample(prob)
-        # per-solve CPU estimate: the oracle's A time scaled by the GPU
-        # solve's cost in A applications
+        t_a, lms = cpu_baseline_sample(dev)
+        # scale to lmax 1023 by the Legendre work (N_lm N_theta ~ lmax^3),
+        # then to one solve by the float64 GPU solve's cost in A applications
+        scale = ((args.lmax + 1) / (lms + 1)) ** 3
         a_ms = time_kernel(lambda: solver.a_matrix(solver.b0), reps=5)
         solve_in_a = (el64 / args.steps) / (a_ms * 1e-3)
-        cpu = {'value': 1.0 / (t_a * solve_in_a), 'unit': 'solves/s', 'cores': 1,
+        cpu = {'value': 1.0 / (t_a * scale * solve_in_a), 'unit': 'solves/s', 'cores': 1,
                'kind': 'port',
-               'sample': (f'one float64 A application of the oracle at TQU lmax {args.lmax} '
-                          f'({t_a:.1f}s), scaled by the float64 GPU solve cost in A '
-                          f'applications ({solve_in_a:.1f})')}
+               'sample': (f'oracle float64 A application (numpy, 1 thread) on the TQU '
+                          f'workload at lmax {lms} ({t_a:.2f}s), scaled by '
+                          f'((lmax+1)/{lms + 1})^3 = {scale:.0f} to lmax {args.lmax} and by '
+                          f'the float64 solve cost in A applications ({solve_in_a:.1f})')}
         log(f'cpu baseline: {cpu}')
 
     out = {
@@ -222,9 +256,11 @@ def main():
                    'fp64_true_residual': res64,
                    'speedup_vs_fp64': mixed_sps / fp64_sps},
         'roofline': {'bound': 'mfma', 'achieved': roof['achieved'],
-                     'peak': PEAK_FP64_TFLOPS, 'unit': 'TFLOP/s',
-                     'frac': roof['achieved'] / PEAK_FP64_TFLOPS, 'traffic': None,
-                     'kernel': roof['kernel'], 'kernel_ms': roof['ms']},
+                     'peak': PEAK_FP32_TFLOPS, 'unit': 'TFLOP/s',
+                     'frac': roof['achieved'] / PEAK_FP32_TFLOPS,
+                     'traffic': ROOFLINE_TRAFFIC_BYTES,
+                     'kernel': roof['kernel'], 'kernel_ms': roof['ms'],
+                     'flops_per_launch': roof['flops']},
         'cpu_baseline': cpu,
     }
     print(json.dumps(out))

@@ -783,15 +783,15 @@ void launch_synthesis(owmi_plan_impl *p, int spin, const cplx_t<T> *alm, long lo
     launch_synth_nb<T, 1>(p, spin, alm, alm_ts, G, ntrans, st);
 }
 
-template <typename T, int K, int NB>
-static void launch_adj_k(owmi_plan_impl *p, int spin, const cplx_t<T> *G, cplx_t<T> *alm,
+template <typename T, int SPIN, int K, int NB>
+static void launch_adj_k(owmi_plan_impl *p, const cplx_t<T> *G, cplx_t<T> *alm,
                          long long alm_ts, long long ntrans, hipStream_t st) {
   PlanTables<T> &tb = tab<T>(p);
   AdjItems it;
   it.nmpairs = (p->mmax + 2) / 2;
   it.ngroups = (int)((p->npairs + 64LL * K - 1) / (64LL * K));
   const long long nelem = nelem_of(p->lmax);
-  const int nc = spin == 0 ? 1 : 2;
+  const int nc = SPIN == 0 ? 1 : 2;
   cplx_t<T> *dst = alm;
   long long dst_ts = alm_ts;
   if (it.ngroups > 1) {
@@ -806,7 +806,7 @@ static void launch_adj_k(owmi_plan_impl *p, int spin, const cplx_t<T> *G, cplx_t
   }
   const long long nitems = (long long)it.nmpairs * it.ngroups;
   dim3 grid(ceil_div(nitems, kAdjWaves), ceil_div(ntrans, NB));
-  if (spin == 0)
+  if constexpr (SPIN == 0)
     hipLaunchKernelGGL((adj0_kernel<T, K, AdjCfg<T>::rows0 / NB, NB>), grid,
                        dim3(64 * kAdjWaves), 0, st, leg_args(p), it, tb.x, tb.lf0, tb.s0,
                        tb.coef0, G, ntrans, dst, dst_ts);
@@ -822,26 +822,34 @@ static void launch_adj_k(owmi_plan_impl *p, int spin, const cplx_t<T> *G, cplx_t
   }
 }
 
-template <typename T, int NB>
-static void launch_adj_spin(owmi_plan_impl *p, int spin, const cplx_t<T> *G, cplx_t<T> *alm,
-                            long long alm_ts, long long ntrans, hipStream_t st) {
-  constexpr int nbmax = NB;
-  if (ntrans >= 4 && nbmax >= 4)
-    launch_adj_k<T, 2, (nbmax >= 4 ? 4 : 1)>(p, spin, G, alm, alm_ts, ntrans, st);
-  else if (ntrans >= 2 && nbmax >= 2)
-    launch_adj_k<T, 2, (nbmax >= 2 ? 2 : 1)>(p, spin, G, alm, alm_ts, ntrans, st);
-  else
-    launch_adj_k<T, 2, 1>(p, spin, G, alm, alm_ts, ntrans, st);
+// Transforms per launch block: the widest of 4, 2, 1 allowed for the spin
+// and precision and not exceeding ntrans.
+template <typename T, int SPIN, int K>
+static void launch_adj_nb(owmi_plan_impl *p, const cplx_t<T> *G, cplx_t<T> *alm,
+                          long long alm_ts, long long ntrans, hipStream_t st) {
+  constexpr int nbmax = SPIN == 0 ? AdjCfg<T>::nb0 : AdjCfg<T>::nb2;
+  if constexpr (nbmax >= 4) {
+    if (ntrans >= 4) return launch_adj_k<T, SPIN, K, 4>(p, G, alm, alm_ts, ntrans, st);
+  }
+  if (ntrans >= 2) return launch_adj_k<T, SPIN, K, 2>(p, G, alm, alm_ts, ntrans, st);
+  launch_adj_k<T, SPIN, K, 1>(p, G, alm, alm_ts, ntrans, st);
 }
 
 template <typename T>
 void launch_adjoint(owmi_plan_impl *p, int spin, const cplx_t<T> *G, cplx_t<T> *alm,
                     long long alm_ts, long long ntrans, hipStream_t st) {
   OWMI_REQUIRE(ntrans >= 1 && ntrans < 65536, "ntrans must be in [1, 65535]");
-  if (spin == 0)
-    launch_adj_spin<T, AdjCfg<T>::nb0>(p, spin, G, alm, alm_ts, ntrans, st);
-  else
-    launch_adj_spin<T, AdjCfg<T>::nb2>(p, spin, G, alm, alm_ts, ntrans, st);
+  // Ring pairs per lane: 2, or 1 on geometries of at most 64 ring pairs
+  // (coarse or ring-subset plans), where two would leave most slots empty.
+  // More pairs per lane amortise the cross-lane reduction further but cost
+  // registers and waves (measured slower at lmax 1023).
+  const long long np = p->npairs;
+  if (spin == 0) {
+    if (np <= 64) return launch_adj_nb<T, 0, 1>(p, G, alm, alm_ts, ntrans, st);
+    return launch_adj_nb<T, 0, 2>(p, G, alm, alm_ts, ntrans, st);
+  }
+  if (np <= 64) return launch_adj_nb<T, 2, 1>(p, G, alm, alm_ts, ntrans, st);
+  launch_adj_nb<T, 2, 2>(p, G, alm, alm_ts, ntrans, st);
 }
 
 template void launch_synthesis<double>(owmi_plan_impl *, int, const double2 *, long long,
