@@ -1,6 +1,7 @@
 '''
 alm layout and algebra, mirroring optweight's alm_utils: contract_almxblm
-(alm_utils.py:9-44), unit_var_alm (247-307), rand_alm (309-338); plus the
+(alm_utils.py:9-44), alm2wlm_axisym (46-103), wlm2alm_axisym (105-155),
+trunc_alm (180-241), unit_var_alm (247-307), rand_alm (309-338); plus the
 pixell alm_info layout optweight relies on (triangular, mmax = lmax,
 m-major, stride 1).
 '''
@@ -39,6 +40,59 @@ def getlmax(nelem):
     if (lmax * lmax + 3 * lmax) // 2 + 1 != nelem:
         raise ValueError(f'{nelem} is not a triangular alm size')
     return lmax
+
+
+def trunc_alm(alm, ainfo, lmax):
+    '''Device alm truncated to l, m <= lmax: (alm_trunc, ainfo_trunc).'''
+    from optweight_amd import alm_c_utils
+    if lmax > ainfo.lmax:
+        raise ValueError(f'New lmax {lmax} exceeds old lmax {ainfo.lmax}')
+    winfo = AlmInfo(lmax)
+    out = torch.empty(tuple(alm.shape[:-1]) + (winfo.nelem,), dtype=alm.dtype,
+                      device=alm.device)
+    alm_c_utils.trunc_alm(alm.contiguous(), out, ainfo.lmax, lmax)
+    return out, winfo
+
+
+def alm2wlm_axisym(alm, ainfo, w_ell, lmaxs=None):
+    '''Wavelet coefficients w^j_lm = w_ell[j, l] a_lm, l <= lmax_j (lmax_j:
+    last multipole with w_ell[j] > 0, or lmaxs[j]): (wlms, winfos).'''
+    from optweight_amd import alm_c_utils
+    w_ell = np.asarray(w_ell)
+    if w_ell.ndim == 1:
+        w_ell = w_ell[np.newaxis, :]
+    lmax = ainfo.lmax
+    if w_ell.shape[1] != lmax + 1:
+        raise ValueError(f'lmax alm : {lmax} and lmax of w_ell : {w_ell.shape[1] - 1} '
+                         'do not match')
+    wlms, winfos = [], []
+    for idx in range(w_ell.shape[0]):
+        lmax_w = lmaxs[idx] if lmaxs is not None else lmax - np.argmax(w_ell[idx, ::-1] > 0)
+        wlm, winfo = trunc_alm(alm, ainfo, int(lmax_w))
+        alm_c_utils.lmul(wlm, w_ell[idx, :winfo.lmax + 1], winfo, inplace=True)
+        wlms.append(wlm)
+        winfos.append(winfo)
+    return wlms, winfos
+
+
+def wlm2alm_axisym(wlms, winfos, w_ell, alm=None, ainfo=None):
+    '''alm += sum_j w_ell[j, l] w^j_lm (added to alm when given): (alm, ainfo).'''
+    from optweight_amd import alm_c_utils
+    w_ell = np.asarray(w_ell)
+    if w_ell.ndim == 1:
+        w_ell = w_ell[np.newaxis, :]
+    lmax_w = max(w.lmax for w in winfos)
+    if ainfo is None:
+        ainfo = AlmInfo(lmax_w)
+    elif ainfo.lmax < lmax_w:
+        raise ValueError(f'lmax alm {ainfo.lmax} < lmax wavelets {lmax_w}')
+    if alm is None:
+        alm = torch.zeros(tuple(wlms[0].shape[:-1]) + (ainfo.nelem,), dtype=wlms[0].dtype,
+                          device=wlms[0].device)
+    for idx, (wlm, winfo) in enumerate(zip(wlms, winfos)):
+        alm_c_utils.wlm2alm(w_ell[idx, :winfo.lmax + 1], wlm.contiguous(), alm, winfo.lmax,
+                            ainfo.lmax)
+    return alm, ainfo
 
 
 def contract_almxblm_dev(alm, blm, out=None):
@@ -125,3 +179,42 @@ def rand_alm(cov_ell, ainfo, seed, out=None, dtype=np.complex128):
     res = np.einsum('ijn,jn->in', sq[:, :, ell].astype(rtype), a).astype(dtype)
     a[...] = res
     return a
+
+
+def rand_alm_pix(cov_pix, ainfo, minfo, spin, seed, adjoint=False, device=None):
+    """Y^T W N^(1/2) r (Y^T N^(1/2) r if adjoint), r ~ N(0, 1) drawn on the
+    host (alm_utils.py:340-373); device complex alm (npol, nelem)."""
+    from optweight_amd import map_utils, sht
+    cov_pix = np.asarray(cov_pix)
+    if device is None:
+        device = torch.device('cuda', torch.cuda.current_device())
+    noise = torch.as_tensor(map_utils.rand_map_pix(cov_pix, seed), device=device)
+    cdt = torch.complex64 if noise.dtype == torch.float32 else torch.complex128
+    alm = torch.zeros((noise.shape[0], ainfo.nelem), dtype=cdt, device=device)
+    sht.map2alm(noise, alm, minfo, ainfo, spin, adjoint=adjoint)
+    return alm
+
+
+def rand_alm_wav(cov_wav, ainfo, w_ell, spin, seed, adjoint=False, device=None):
+    """sum_j K_j^T Y^T [W] N_j^(1/2) r_j with one draw per wavelet scale
+    from a block-diagonal wavelet covariance (alm_utils.py:375-419)."""
+    from optweight_amd import map_utils, sht, wavtrans
+    if device is None:
+        device = torch.device('cuda', torch.cuda.current_device())
+    npol = wavtrans.preshape2npol(cov_wav.preshape)
+    rng = np.random.default_rng(seed)
+    w_ell = np.asarray(w_ell)
+    out = None
+    for jidx in range(cov_wav.shape[0]):
+        cov = cov_wav.maps[(jidx, jidx)]
+        cov = cov.detach().cpu().numpy() if isinstance(cov, torch.Tensor) else np.asarray(cov)
+        minfo = cov_wav.minfos[(jidx, jidx)]
+        rmap = torch.as_tensor(map_utils.rand_map_pix(cov, rng), device=device)
+        cdt = torch.complex64 if rmap.dtype == torch.float32 else torch.complex128
+        if out is None:
+            out = torch.zeros((npol, ainfo.nelem), dtype=cdt, device=device)
+        winfo = AlmInfo((int(minfo.nphi[0]) - 1) // 2)
+        wlm = torch.zeros((npol, winfo.nelem), dtype=cdt, device=device)
+        sht.map2alm(rmap, wlm, minfo, winfo, spin, adjoint=adjoint)
+        wlm2alm_axisym([wlm], [winfo], w_ell[jidx:jidx + 1], alm=out, ainfo=ainfo)
+    return out

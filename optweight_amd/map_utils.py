@@ -217,6 +217,45 @@ def gauss2map(imap, minfo_in, minfo_out, order=3):
     return omap
 
 
+def gauss2gauss(imap, minfo_in, minfo_out, order=3, area_pow=0):
+    '''Spline interpolation between constant-nphi Gauss-Legendre maps
+    (map_utils.py:672-717): values scaled by (pixel area)^area_pow via the
+    ring weights on both sides.'''
+    imap = np.asarray(imap)
+    thetas_in, phis_in = _gauss_coords(minfo_in)
+    thetas_out, phis_out = _gauss_coords(minfo_out)
+    im2 = view_2d(imap, minfo_in)
+    if area_pow != 0:
+        im2 = im2 * (np.asarray(minfo_in.weight) ** -float(area_pow))[:, np.newaxis]
+    om2 = np.zeros(imap.shape[:-1] + (minfo_out.nrow, int(minfo_out.nphi[0])), dtype=imap.dtype)
+    for idx in np.ndindex(im2.shape[:-2]):
+        rbs = RectBivariateSpline(thetas_in, phis_in, im2[idx], kx=order, ky=order)
+        om2[idx] = rbs(thetas_out, phis_out)
+    if area_pow != 0:
+        om2 *= (np.asarray(minfo_out.weight) ** float(area_pow))[:, np.newaxis]
+    return view_1d(om2, minfo_out)
+
+
+def get_ivar_ell(icov_wav, w_ell, mask=None, minfo_mask=None):
+    '''Inverse-variance spectrum of a wavelet-diagonal inverse covariance
+    (map_utils.py:1220-1266): per scale the isotropic inverse variance of
+    block (j, j) (masked by the mask interpolated onto that scale's grid),
+    weighted by w_ell[j]^2. Diagonal only, (npol, npol, nell).'''
+    from optweight_amd import wavtrans
+    npol = wavtrans.preshape2npol(icov_wav.preshape)
+    w_ell = np.asarray(w_ell)
+    itaus = np.zeros((npol, npol, w_ell.shape[0]))
+    for jidx in range(w_ell.shape[0]):
+        minfo_j = icov_wav.minfos[(jidx, jidx)]
+        m = icov_wav.maps[(jidx, jidx)]
+        m = m.detach().cpu().numpy() if hasattr(m, 'detach') else np.asarray(m)
+        mask_j = None
+        if mask is not None:
+            mask_j = gauss2gauss(np.asarray(mask), minfo_mask, minfo_j, order=1, area_pow=0)
+        itaus[:, :, jidx] = get_isotropic_ivar(m, minfo_j, mask=mask_j)
+    return np.einsum('ijk, kl -> ijl', itaus, w_ell ** 2, optimize=True)
+
+
 def _ringweight_np(imap, minfo, weight):
     for t in range(minfo.nrow):
         sl = slice(minfo.offsets[t], minfo.offsets[t] + minfo.nphi[t] * minfo.stride[t],

@@ -1,6 +1,7 @@
 '''
 Small symmetric matrices per pixel / per multipole, mirroring optweight's
 mat_utils: full_matrix (mat_utils.py:125-155), matpow / _eigpow (220-365),
+wavmatpow (367-411),
 atleast_nd (445-473), matvec (475-532), and mat_c_utils.eigpow
 (mat_c_utils.c:9-157: V diag(lam^p) V^T, eigenvalues below lim * max set
 to zero, all zero when max < lim0).
@@ -92,6 +93,23 @@ def matpow(mat, power, return_diag=False, skip_unit_pow=True, lim=1e-9, lim0=Non
     if not return_diag:
         mat = full_matrix(mat)
     return np.array(mat, copy=True)
+
+
+def wavmatpow(m_wav, power, **matpow_kwargs):
+    '''Power of a block-diagonal wavelet matrix (mat_utils.py:367-411):
+    each diagonal block through matpow (host float64).'''
+    from optweight_amd import wavtrans
+    if not np.array_equal(m_wav.indices[:, 0], m_wav.indices[:, 1]):
+        raise NotImplementedError('Can only raise diagonal block matrix to a power')
+    out = wavtrans.Wav(2, dtype=m_wav.dtype)
+    for jidx in range(m_wav.shape[0]):
+        key = (jidx, jidx)
+        mat = m_wav.maps[key]
+        if isinstance(mat, torch.Tensor):
+            mat = mat.detach().cpu().numpy()
+        out.add(key, matpow(mat, power, **matpow_kwargs).astype(m_wav.dtype, copy=False),
+                m_wav.minfos[key])
+    return out
 
 
 def matvec(mat, vec, inplace=False):

@@ -210,13 +210,111 @@ class PixMatVecAlm(MatVecAlm):
                 alm.copy_(res)
                 return alm
             return sht.pixel_sandwich(alm, out, self.ainfo, self.minfo, self.spin, d)
-        npol = alm.shape[0]
-        omap = torch.zeros((npol, self.minfo.npix), dtype=rdt, device=alm.device)
+        omap = torch.zeros(tuple(alm.shape[:-1]) + (self.minfo.npix,), dtype=rdt,
+                           device=alm.device)
         sht.alm2map(alm, omap, self.ainfo, self.minfo, self.spin, adjoint=self.adjoint)
         imap = mat_utils.matvec(self._dev.get(alm.device, rdt), omap)
         a2 = self.adjoint if self.use_weights else not self.adjoint
         sht.map2alm(imap, out, self.minfo, self.ainfo, self.spin, adjoint=a2)
         return out
+
+
+def _diag_of(mat):
+    """(npol, npix) diagonal of a pixel matrix (npol, npix) or
+    (npol, npol, npix), or None if the latter has off-diagonal terms."""
+    mat = np.asarray(mat)
+    if mat.ndim == 2:
+        return mat
+    npol = mat.shape[0]
+    if np.any(mat * (1 - np.eye(npol))[:, :, None]):
+        return None
+    return np.ascontiguousarray(np.einsum('iip->ip', mat))
+
+
+class WavMatVecAlm(MatVecAlm):
+    """K^T Y^T M Y K alm (adjoint: K^T Y^T W M W Y K) for a wavelet block
+    matrix M (wavtrans.Wav) and axisymmetric kernels K = w_ell
+    (operators.py:705-815). Scale j' of the input is synthesised on the
+    Gauss-Legendre grid of block (j', j'); block (j, j') multiplies it and
+    the result goes back through K_j. A diagonal block with a per-component
+    pixel diagonal runs as one fused synthesis -> multiply -> adjoint
+    transform (sht.pixel_sandwich)."""
+
+    def __init__(self, ainfo, m_wav, w_ell, spin, power=1, adjoint=False):
+        from optweight_amd import wavtrans
+        self.ainfo = ainfo
+        self.w_ell = np.asarray(w_ell)
+        self.spin = spin
+        self.adjoint = adjoint
+        if power != 1:
+            m_wav = mat_utils.wavmatpow(m_wav, power)
+        self.m_wav = m_wav
+        self.npol = wavtrans.preshape2npol(m_wav.preshape)
+        self.blocks = []
+        for key in m_wav.maps:
+            j, jp = key
+            minfo = m_wav.minfos[(jp, jp)]
+            lmax_g = (int(minfo.nphi[0]) - 1) // 2
+            lmax_w = wavtrans.wav_lmax(self.w_ell, jp)
+            mat = m_wav.maps[key]
+            mat = mat.detach().cpu().numpy() if isinstance(mat, torch.Tensor) else np.asarray(mat)
+            diag = _diag_of(mat) if j == jp else None
+            fused = diag is not None and lmax_w <= lmax_g <= ainfo.lmax
+            if fused:
+                d = np.array(diag, dtype=np.float64)
+                if adjoint:
+                    w = np.asarray(minfo.weight, dtype=np.float64) ** 2
+                    d = map_utils._ringweight_np(d, minfo, w)
+                cache = _DeviceCache(d)
+            else:
+                cache = _DeviceCache(mat)
+            self.blocks.append({'j': j, 'jp': jp, 'minfo': minfo, 'lmax_g': lmax_g,
+                                'lmax_w': lmax_w, 'fused': fused, 'mat': cache})
+
+    def call(self, alm):
+        out = torch.zeros_like(alm)
+        rdt = _real_dtype(alm.dtype)
+        wlm_cache = {}
+        for b in self.blocks:
+            j, jp, minfo = b['j'], b['jp'], b['minfo']
+            m = b['mat'].get(alm.device, rdt)
+            if b['fused']:
+                L = b['lmax_g']
+                winfo = alm_utils.AlmInfo(L)
+                wlm, _ = alm_utils.trunc_alm(alm, self.ainfo, L)
+                alm_c_utils.lmul(wlm, self.w_ell[jp, :L + 1], winfo, inplace=True)
+                res = torch.empty_like(wlm)
+                sht.pixel_sandwich(wlm, res, winfo, minfo, self.spin, m)
+            else:
+                if jp not in wlm_cache:
+                    wlms, winfos = alm_utils.alm2wlm_axisym(
+                        alm, self.ainfo, self.w_ell[jp:jp + 1], lmaxs=[b['lmax_w']])
+                    wlm_cache[jp] = (wlms[0], winfos[0])
+                wlm, winf = wlm_cache[jp]
+                vmap = torch.zeros(tuple(alm.shape[:-1]) + (minfo.npix,), dtype=rdt,
+                                   device=alm.device)
+                sht.alm2map(wlm, vmap, winf, minfo, self.spin, adjoint=self.adjoint)
+                prod = mat_utils.matvec(m, vmap)
+                winfo = alm_utils.AlmInfo(b['lmax_g'])
+                res = torch.zeros(tuple(alm.shape[:-1]) + (winfo.nelem,), dtype=alm.dtype,
+                                  device=alm.device)
+                sht.map2alm(prod, res, minfo, winfo, self.spin, adjoint=not self.adjoint)
+            alm_utils.wlm2alm_axisym([res], [winfo], self.w_ell[j:j + 1], alm=out,
+                                     ainfo=self.ainfo)
+        return out
+
+
+class EllWavEllMatVecAlm(MatVecAlm):
+    """X_ell^q M_wav^p X_ell^q alm (operators.py:645-703)."""
+
+    def __init__(self, ainfo, m_wav, w_ell, x_ell, spin, power_m=1, power_x=1, adjoint=False):
+        self.x_ell_op = EllMatVecAlm(ainfo, x_ell, power=power_x, inplace=False)
+        self.m_wav_op = WavMatVecAlm(ainfo, m_wav, w_ell, spin, power=power_m, adjoint=adjoint)
+
+    def call(self, alm):
+        alm = self.x_ell_op(alm)
+        alm = self.m_wav_op(alm)
+        return self.x_ell_op(alm)
 
 
 class PixEllPixMatVecMap(MatVecMap):

@@ -1,8 +1,8 @@
 '''
 Preconditioners on device alm, mirroring optweight's preconditioners:
 HarmonicPreconditioner (preconditioners.py:8-124),
-PseudoInvPreconditioner (126-249), MaskedPreconditioner (558-650),
-get_itau_ell_harm (916-948).
+PseudoInvPreconditioner (126-249), PseudoInvPreconditionerWav (251-395),
+MaskedPreconditioner (558-650), get_itau_ell_harm (916-948).
 '''
 import numpy as np
 import torch
@@ -22,7 +22,9 @@ class HarmonicPreconditioner(operators.MatVecAlm):
             if icov_pix is None and icov_wav is None and icov_noise_ell is None:
                 raise ValueError('provide atleast itau or icov_{pix/wav/noise_ell}')
             if icov_wav is not None:
-                raise NotImplementedError('wavelet-domain itau is not supported yet')
+                assert icov_pix is None, 'icov_wav given, icov_pix is not None'
+                assert icov_noise_ell is None, 'icov_wav given, icov_noise_ell is not None'
+                itau = map_utils.get_ivar_ell(icov_wav, w_ell, mask=mask_pix, minfo_mask=minfo)
             if icov_pix is not None:
                 assert icov_noise_ell is None, 'icov_pix given, icov_noise_ell is not None'
                 itau = map_utils.get_isotropic_ivar(icov_pix, minfo, mask=mask_pix)
@@ -98,6 +100,76 @@ class PseudoInvPreconditioner(operators.MatVecAlm):
         alm_noise = self.ivar_noise_iso(alm)
         alm_noise = self.pcov_noise(alm_noise)
         alm_noise = self.ivar_noise_iso_adj(alm_noise)
+        alm_noise += alm_signal
+        return self.harmonic_prec(alm_noise)
+
+
+class PseudoInvPreconditionerWav(operators.MatVecAlm):
+    """Pseudo-inverse preconditioner for a wavelet-domain noise model
+    (Seljebotn et al. 2017 adapted; preconditioners.py:251-395):
+        M = H [S^-1 + F B M^-1 tau_l N_wav^-1' tau_l M^-1 B F] H
+    with H = (S^-1 + B tau_l B)^-1, tau_l the isotropic inverse-variance
+    spectrum of the wavelet inverse covariance (map_utils.get_ivar_ell, or
+    get_itau_ell_harm from icov_noise_ell), N_wav^-1' the wavelet matrix to
+    the power -1 applied with quadrature weights (WavMatVecAlm(power=-1,
+    adjoint=True)), M^-1 = Y^T W mask^-1 Y and F the optional scale filter.
+    Without sfilt F is the identity (the reference leaves its scale-filter
+    operator unset in that case and fails when applied)."""
+
+    def __init__(self, ainfo, icov_ell, icov_wav, w_ell, spin, mask_pix=None, minfo_mask=None,
+                 itau_ell=None, b_ell=None, icov_noise_ell=None, sfilt=None):
+        icov_ell = mat_utils.full_matrix(np.asarray(icov_ell))
+        npol, nell = icov_ell.shape[-2:]
+        if itau_ell is None:
+            if icov_noise_ell is None:
+                itau_ell = map_utils.get_ivar_ell(icov_wav, w_ell, mask=mask_pix,
+                                                  minfo_mask=minfo_mask)
+            else:
+                itau_ell = get_itau_ell_harm(icov_noise_ell, w_ell)
+        itau_ell = np.asarray(itau_ell)
+        if itau_ell.ndim != 3:
+            raise ValueError(f'Wrong dimensions itau_ell : expected 3, got {itau_ell.ndim}')
+        if b_ell is None:
+            b_ell = np.ones((npol, nell))
+        b_ell = np.asarray(b_ell) * np.eye(npol)[:, :, np.newaxis]
+        if mask_pix is None:
+            self.imask = lambda alm: alm
+        else:
+            self.imask = operators.PixMatVecAlm(ainfo, mask_pix, minfo_mask, spin, power=-1,
+                                                use_weights=True)
+        if sfilt is not None:
+            sfilt = mat_utils.full_matrix(np.asarray(sfilt))
+        self.harmonic_prec = HarmonicPreconditioner(ainfo, icov_ell, itau=itau_ell, b_ell=b_ell,
+                                                    sfilt=sfilt)
+        if sfilt is not None:
+            self.icov_signal = operators.EllMatVecAlm(
+                ainfo, np.einsum('ijl, jkl, kol -> iol', sfilt, icov_ell, sfilt), inplace=True)
+            self.sfilt = operators.EllMatVecAlm(ainfo, sfilt, inplace=True)
+        else:
+            self.icov_signal = operators.EllMatVecAlm(ainfo, icov_ell, inplace=True)
+            self.sfilt = lambda alm: alm
+        self.beam = operators.EllMatVecAlm(ainfo, b_ell, inplace=True)
+        self.ivar_noise_iso = operators.EllMatVecAlm(ainfo, itau_ell, inplace=True)
+        if icov_noise_ell is None:
+            self.pcov_noise = operators.WavMatVecAlm(ainfo, icov_wav, w_ell, spin, power=-1,
+                                                     adjoint=True)
+        else:
+            self.pcov_noise = operators.EllWavEllMatVecAlm(ainfo, icov_wav, w_ell,
+                                                           icov_noise_ell, spin, power_m=-1,
+                                                           power_x=-0.5, adjoint=True)
+
+    def call(self, alm):
+        alm = self.harmonic_prec(alm.clone())
+        alm_signal = self.icov_signal(alm.clone())
+        alm_noise = self.sfilt(alm)
+        alm_noise = self.beam(alm_noise)
+        alm_noise = self.imask(alm_noise)
+        alm_noise = self.ivar_noise_iso(alm_noise)
+        alm_noise = self.pcov_noise(alm_noise)
+        alm_noise = self.ivar_noise_iso(alm_noise)
+        alm_noise = self.imask(alm_noise)
+        alm_noise = self.beam(alm_noise)
+        alm_noise = self.sfilt(alm_noise)
         alm_noise += alm_signal
         return self.harmonic_prec(alm_noise)
 

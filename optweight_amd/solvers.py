@@ -365,3 +365,169 @@ class CGWienerMap(CG):
         return cls(imap_dev, icov_signal, icov_noise, sht, beam=beam, mask=mask,
                    rand_isignal=rand_isignal, rand_inoise=rand_inoise, swap_bm=swap_bm,
                    scale_filter=sf)
+
+
+class CGWiener(CG):
+    """Solve (S^-1 + B F^T M N^-1 M F B) x = B F^T M N^-1 a (+ w_s + B F^T M
+    w_n for constrained realisations) entirely in harmonic space, with the
+    constructor, attributes and methods of optweight's CGWiener
+    (solvers.py:697-1139). Operators are callables on device alm
+    (..., npol, nelem); a leading batch axis solves independent right-hand
+    sides together. Map-space filters (2D Fourier) are not part of this
+    path."""
+
+    def __init__(self, alm_data, icov_signal, icov_noise, beam=None, filt=None, mask=None,
+                 rand_isignal=None, rand_inoise=None, swap_bm=False):
+        self.alm_data = alm_data
+        self.icov_signal = icov_signal
+        self.icov_noise = icov_noise
+        self.beam = beam if beam is not None else (lambda alm: alm)
+        if filt is None:
+            self.filt = self.filt_adjoint = lambda alm: alm
+        else:
+            self.filt, self.filt_adjoint = filt
+        self.mask = mask if mask is not None else (lambda alm: alm)
+        self.rand_isignal = rand_isignal
+        self.rand_inoise = rand_inoise
+        self.swap_bm = swap_bm
+        if self.rand_isignal is not None and self.rand_inoise is not None:
+            self.b_vec = self.get_b_vec_constr(self.alm_data)
+        else:
+            self.b_vec = self.get_b_vec(self.alm_data)
+        self.b0 = self.b_vec.clone()
+        self.preconditioner = None
+
+    def init_solver(self, **kwargs):
+        kwargs.setdefault('dot', _default_dot(self.b_vec))
+        if self.preconditioner is not None:
+            kwargs.setdefault('M', self.preconditioner)
+        CG.__init__(self, self.a_matrix, self.b_vec, **kwargs)
+
+    add_preconditioner = CGWienerMap.add_preconditioner
+    reset_preconditioner = CGWienerMap.reset_preconditioner
+
+    def a_matrix(self, alm):
+        if self.swap_bm:
+            n = self.filt(self.beam(self.mask(alm.clone())))
+        else:
+            n = self.mask(self.filt(self.beam(alm.clone())))
+        n = self.icov_noise(n)
+        if self.swap_bm:
+            n = self.mask(self.beam(self.filt_adjoint(n)))
+        else:
+            n = self.beam(self.filt_adjoint(self.mask(n)))
+        return self.icov_signal(alm.clone()) + n
+
+    def get_b_vec(self, alm):
+        alm = self.icov_noise(alm.clone())
+        if self.swap_bm:
+            return self.mask(self.beam(self.filt_adjoint(alm)))
+        return self.beam(self.filt_adjoint(self.mask(alm)))
+
+    def get_b_vec_constr(self, alm):
+        alm = self.get_b_vec(alm)
+        if self.swap_bm:
+            alm += self.mask(self.beam(self.filt_adjoint(self.rand_inoise.clone())))
+        else:
+            alm += self.beam(self.filt_adjoint(self.mask(self.rand_inoise.clone())))
+        alm += self.rand_isignal
+        return alm
+
+    def get_wiener(self):
+        return self.x.clone()
+
+    def get_icov(self):
+        return self.icov_signal(self.x.clone())
+
+    def get_chisq(self):
+        x_w = self.get_wiener()
+        out = _host(self.dot(x_w, self.icov_signal(x_w)))
+        d = self.alm_data - x_w
+        out += _host(self.dot(self.beam(self.filt_adjoint(self.mask(d))),
+                              self.icov_noise(self.beam(self.filt(self.mask(d))))))
+        return out
+
+    def get_residual(self):
+        r = self.A(self.x) - self.b0
+        return np.sqrt(_host(self.dot(r, r)))
+
+    def get_qform(self):
+        return (0.5 * _host(self.dot(self.x, self.A(self.x)))
+                - _host(self.dot(self.x, self.b0)))
+
+    @classmethod
+    def from_arrays(cls, alm_data, ainfo, icov_ell, icov_pix, minfo, *extra_args, b_ell=None,
+                    mask_pix=None, draw_constr=False, spin=None, icov_noise_flat_ell=None,
+                    swap_bm=False, kfilt=None, minfo_kfilt=None, seed=None, device=None):
+        """Harmonic-space solver with a pixel-diagonal noise model
+        (solvers.py:958-1046): N^-1 = Y^T N^-1_pix Y, mask = Y^T W M Y."""
+        if icov_noise_flat_ell is not None:
+            raise NotImplementedError('icov_noise_flat_ell is not supported')
+        if kfilt is not None:
+            raise NotImplementedError('Fourier filters are not part of this path')
+        alm_data = _to_device_alm(alm_data, device)
+        if spin is None:
+            spin = sht_tools.default_spin(tuple(alm_data.shape[-2:]) if alm_data.ndim == 3
+                                          else tuple(alm_data.shape))
+        icov_signal = operators.EllMatVecAlm(ainfo, icov_ell)
+        icov_noise = operators.PixMatVecAlm(ainfo, icov_pix, minfo, spin)
+        beam = operators.EllMatVecAlm(ainfo, b_ell) if b_ell is not None else None
+        mask = (operators.PixMatVecAlm(ainfo, mask_pix, minfo, spin, use_weights=True)
+                if mask_pix is not None else None)
+        if draw_constr:
+            cnp = np.complex128 if alm_data.dtype == torch.complex128 else np.complex64
+            rand_isignal = torch.as_tensor(alm_utils.rand_alm(icov_ell, ainfo, seed, dtype=cnp),
+                                           device=alm_data.device)
+            rand_inoise = alm_utils.rand_alm_pix(np.asarray(icov_pix), ainfo, minfo, spin, seed,
+                                                 adjoint=True, device=alm_data.device)
+            rand_inoise = rand_inoise.to(alm_data.dtype)
+        else:
+            rand_isignal = rand_inoise = None
+        return cls(alm_data, icov_signal, icov_noise, *extra_args, beam=beam, mask=mask,
+                   rand_isignal=rand_isignal, rand_inoise=rand_inoise, swap_bm=swap_bm)
+
+    @classmethod
+    def from_arrays_wav(cls, alm_data, ainfo, icov_ell, icov_wav, w_ell, *extra_args,
+                        b_ell=None, mask_pix=None, minfo_mask=None, draw_constr=False,
+                        spin=None, swap_bm=False, icov_noise_ell=None, kfilt=None,
+                        minfo_kfilt=None, seed=None, device=None):
+        """Harmonic-space solver with a wavelet-domain noise model
+        (solvers.py:1048-1139): N^-1 = K^T Y^T N^-1_wav Y K (WavMatVecAlm),
+        optionally flattened by icov_noise_ell^(1/2) on both sides."""
+        if kfilt is not None:
+            raise NotImplementedError('Fourier filters are not part of this path')
+        alm_data = _to_device_alm(alm_data, device)
+        if spin is None:
+            spin = sht_tools.default_spin(tuple(alm_data.shape[-2:]) if alm_data.ndim == 3
+                                          else tuple(alm_data.shape))
+        icov_signal = operators.EllMatVecAlm(ainfo, icov_ell)
+        if icov_noise_ell is None:
+            icov_noise = operators.WavMatVecAlm(ainfo, icov_wav, w_ell, spin)
+        else:
+            icov_noise = operators.EllWavEllMatVecAlm(ainfo, icov_wav, w_ell, icov_noise_ell,
+                                                      spin, power_x=0.5)
+        beam = operators.EllMatVecAlm(ainfo, b_ell) if b_ell is not None else None
+        mask = (operators.PixMatVecAlm(ainfo, mask_pix, minfo_mask, spin, use_weights=True)
+                if mask_pix is not None else None)
+        if draw_constr:
+            cnp = np.complex128 if alm_data.dtype == torch.complex128 else np.complex64
+            rand_isignal = torch.as_tensor(alm_utils.rand_alm(icov_ell, ainfo, seed, dtype=cnp),
+                                           device=alm_data.device)
+            rand_inoise = alm_utils.rand_alm_wav(icov_wav, ainfo, w_ell, spin, seed,
+                                                 adjoint=True, device=alm_data.device)
+            rand_inoise = rand_inoise.to(alm_data.dtype)
+            if icov_noise_ell is not None:
+                rand_inoise = operators.EllMatVecAlm(ainfo, icov_noise_ell, power=0.5)(
+                    rand_inoise)
+        else:
+            rand_isignal = rand_inoise = None
+        return cls(alm_data, icov_signal, icov_noise, *extra_args, beam=beam, mask=mask,
+                   rand_isignal=rand_isignal, rand_inoise=rand_inoise, swap_bm=swap_bm)
+
+
+def _to_device_alm(alm, device):
+    if isinstance(alm, torch.Tensor):
+        return alm if device is None else alm.to(device)
+    if device is None:
+        device = torch.device('cuda', torch.cuda.current_device())
+    return torch.as_tensor(np.ascontiguousarray(alm), device=device)

@@ -1,0 +1,173 @@
+'''
+Wavelet block vectors and matrices of maps, and the axisymmetric wavelet
+transforms between alm and per-scale maps, mirroring optweight's wavtrans:
+Wav (wavtrans.py:10-369), wav2alm (406-465), alm2wav (467-515),
+preshape2npol (517-546).
+
+A Wav holds one map per index (an int for a block vector, an (i, j) tuple
+for a block matrix) with its own MapInfo; maps are numpy arrays or device
+tensors (the transforms need device tensors). Wavelet scale j of an alm is
+w^j_lm = w_ell[j, l] a_lm for l <= lmax_j, mapped onto the Gauss-Legendre
+grid of band limit lmax_j (lmax_j + 1 rings, 2 lmax_j + 1 pixels per ring).
+'''
+import numpy as np
+import torch
+
+from optweight_amd import map_utils, alm_utils, sht
+
+
+def _np_dtype(dtype):
+    if isinstance(dtype, torch.dtype):
+        return {torch.float32: np.dtype(np.float32), torch.float64: np.dtype(np.float64),
+                torch.complex64: np.dtype(np.complex64),
+                torch.complex128: np.dtype(np.complex128)}[dtype]
+    return np.dtype(dtype)
+
+
+class Wav:
+    '''Block vector (ndim 1) or matrix (ndim 2) of wavelet maps.'''
+
+    def __init__(self, ndim, indices=None, minfos=None, preshape=None, dtype=np.float64):
+        if ndim not in (1, 2):
+            raise NotImplementedError(f'{ndim}-D matrices not supported')
+        if indices is None:
+            indices = np.zeros((0, ndim), dtype=int)
+        else:
+            indices = np.asarray(indices)
+            if indices.ndim == 1:
+                indices = indices[:, np.newaxis]
+        self.ndim = ndim
+        self.preshape = () if preshape is None else tuple(preshape)
+        self.dtype = _np_dtype(dtype)
+        self.maps = {}
+        self.minfos = {}
+        self.indices = np.zeros((0, ndim), dtype=int)
+        for idx in range(indices.shape[0]):
+            minfo = minfos[idx]
+            self.add(indices[idx], np.zeros(self.preshape + (minfo.npix,), dtype=self.dtype),
+                     minfo)
+
+    @property
+    def shape(self):
+        '''Shape of the corresponding dense block vector / matrix.'''
+        if self.indices.shape[0] == 0:
+            return (0,) * self.indices.shape[1]
+        return tuple(int(1 + np.max(self.indices[:, i])) for i in range(self.indices.shape[1]))
+
+    def add(self, index, m_arr, minfo):
+        '''Place m_arr (preshape + (npix,)) at index.'''
+        index = np.asarray(index)
+        if index.ndim == 0:
+            index = index[np.newaxis]
+        if index.size != self.ndim:
+            raise ValueError(f'Wrong dimension for index, got : {index.size} '
+                             f'expected : {self.ndim}')
+        if not issubclass(index.dtype.type, np.integer):
+            raise ValueError(f'Index should be integer, instead got : {index.dtype}')
+        if np.any(index < 0):
+            raise ValueError(f'Index cannot be negative, got {index}')
+        key = int(index[0]) if index.size == 1 else tuple(int(i) for i in index)
+        if _np_dtype(m_arr.dtype) != self.dtype:
+            raise ValueError(f'Adding array of wrong dtype : {m_arr.dtype}, '
+                             f'expected {self.dtype}')
+        preshape = tuple(m_arr.shape[:-1])
+        if self.maps:
+            if preshape != self.preshape:
+                raise ValueError(f'Wrong leading dimensions of map array: expected : '
+                                 f'{self.preshape}, got {preshape}')
+        else:
+            self.preshape = preshape
+        if m_arr.shape[-1] != minfo.npix:
+            raise ValueError(f'map has {m_arr.shape[-1]} pixels, minfo {minfo.npix}')
+        self.maps[key] = m_arr
+        self.minfos[key] = minfo
+        self.indices = np.unique(np.append(self.indices, index[np.newaxis, :], axis=0), axis=0)
+
+    def get_indices_diag(self):
+        '''Keys of the diagonal (every key for a vector).'''
+        if self.ndim == 1:
+            return [int(i) for i in self.indices[:, 0]]
+        return [(int(i), int(j)) for i, j in self.indices if i == j]
+
+    def diag(self):
+        '''New block vector holding the diagonal (or a copy of a vector).'''
+        out = Wav(1, preshape=self.preshape, dtype=self.dtype)
+        for key in self.get_indices_diag():
+            m = self.maps[key]
+            m = m.clone() if isinstance(m, torch.Tensor) else m.copy()
+            out.add(key if self.ndim == 1 else key[0], m, self.minfos[key])
+        return out
+
+    def astype(self, dtype):
+        '''Change the map dtype (copies when needed).'''
+        for key in self.maps:
+            m = self.maps[key]
+            if isinstance(m, torch.Tensor):
+                tdt = {np.dtype(np.float32): torch.float32,
+                       np.dtype(np.float64): torch.float64}[np.dtype(dtype)]
+                self.maps[key] = m.to(tdt)
+            else:
+                self.maps[key] = m.astype(dtype, copy=False)
+        self.dtype = np.dtype(dtype)
+
+    def to(self, device):
+        '''Copy with every map as a tensor on device.'''
+        out = Wav(self.ndim, preshape=self.preshape, dtype=self.dtype)
+        for key in self.maps:
+            m = self.maps[key]
+            t = m.to(device) if isinstance(m, torch.Tensor) else torch.as_tensor(
+                np.ascontiguousarray(m), device=device)
+            out.add(np.asarray(key), t, self.minfos[key])
+        return out
+
+
+def preshape2npol(preshape):
+    '''npol of a block vector (preshape (npol,)) or matrix ((npol, npol)).'''
+    if len(preshape) > 2 or len(set(preshape)) > 1:
+        raise ValueError(f'Could not determine npol from preshape : {preshape}')
+    return 1 if len(preshape) == 0 else preshape[0]
+
+
+def wav_lmax(w_ell, idx):
+    '''Band limit of kernel idx: the last multipole where it is positive.'''
+    lmax = w_ell.shape[-1] - 1
+    return int(lmax - np.argmax(np.asarray(w_ell)[idx, ::-1] > 0))
+
+
+def alm2wav(alm, ainfo, spin, w_ell, wav=None, adjoint=False, lmaxs=None):
+    '''Wavelet maps of device alm (..., npol, nelem): scale j is Y (or W Y
+    if adjoint) of w_ell[j] a_lm on its Gauss-Legendre grid.'''
+    wlms, winfos = alm_utils.alm2wlm_axisym(alm, ainfo, w_ell, lmaxs=lmaxs)
+    indices = np.arange(len(wlms))
+    rdt = torch.float64 if alm.dtype == torch.complex128 else torch.float32
+    if wav is None:
+        minfos = [map_utils.MapInfo.map_info_gauss_legendre(w.lmax + 1, 2 * w.lmax + 1)
+                  for w in winfos]
+        wav = Wav(1, preshape=tuple(alm.shape[:-1]),
+                  dtype=np.float64 if rdt == torch.float64 else np.float32)
+        for idx in indices:
+            wav.add(idx, torch.zeros(tuple(alm.shape[:-1]) + (minfos[idx].npix,), dtype=rdt,
+                                     device=alm.device), minfos[idx])
+    for idx in indices:
+        sht.alm2map(wlms[idx], wav.maps[int(idx)], winfos[idx], wav.minfos[int(idx)], spin,
+                    adjoint=adjoint)
+    return wav
+
+
+def wav2alm(wav, alm, ainfo, spin, w_ell, adjoint=False):
+    '''alm = sum_j w_ell[j] Y^T W (or Y^T if adjoint) of wavelet map j;
+    alm is overwritten. Returns (alm, ainfo).'''
+    if len(wav.shape) != 1:
+        raise ValueError('Input wavelet container is not a vector')
+    alm.zero_()
+    for idx in range(w_ell.shape[0]):
+        lmax_w = wav_lmax(w_ell, idx)
+        imap = wav.maps[idx]
+        if imap.ndim == 1:
+            imap = imap[None, :]
+        winfo = alm_utils.AlmInfo(lmax_w)
+        cdt = torch.complex128 if imap.dtype == torch.float64 else torch.complex64
+        wlm = torch.zeros(tuple(alm.shape[:-1]) + (winfo.nelem,), dtype=cdt, device=alm.device)
+        sht.map2alm(imap, wlm, wav.minfos[idx], winfo, spin, adjoint=adjoint)
+        alm_utils.wlm2alm_axisym([wlm], [winfo], w_ell[idx:idx + 1, :], alm=alm, ainfo=ainfo)
+    return alm, ainfo

@@ -28,6 +28,12 @@ optweight repository):
              ``noise_utils._band_limit_gauss_beam`` 401-421,
              ``multigrid.lowpass_filter`` 449-471
   cg         ``pixell.utils.CG`` as used by ``solvers.py:127-141``
+  wavelet    ``alm_utils.alm2wlm_axisym``/``wlm2alm_axisym`` 46-155,
+             ``operators.WavMatVecAlm`` 705-815, ``EllWavEllMatVecAlm``
+             645-703, ``map_utils.gauss2gauss`` 672-717, ``get_ivar_ell``
+             1220-1266, ``preconditioners.PseudoInvPreconditionerWav``
+             251-395, ``get_itau_ell_harm`` 916-948, ``solvers.CGWiener``
+             697-1139
   wiener     ``solvers.CGWienerMap`` 8-456, ``preconditioners``
              (Harmonic 8-124, PseudoInv 126-249, Masked 558-650),
              ``multigrid`` 13-447, ``operators`` (PixEllPixMatVecMap 991-1102,
@@ -49,4 +55,8 @@ How each part is pinned (none is "parity unpinned" unless listed here):
     and full-sky white-noise exact inverses (CG converges in one step).
   * Gaussian beam: ``test_noise_utils.py:120-135`` golden values.
   * multigrid: ``test_multigrid.py`` / ``test_preconditioners.py`` properties.
+  * wavelet noise model and harmonic-space solver: per-multipole closed
+    forms for per-scale inverse covariance proportional to the quadrature
+    weights (WavMatVecAlm, its weighted inverse, get_ivar_ell, one-step
+    exact CG, the wavelet pseudo-inverse), a planted missing-weight check.
 '''
