@@ -148,6 +148,53 @@ def cpu_baseline_sample(dev, lmax_sample=255, reps=3):
     return (time.perf_counter() - t0) / reps, lmax_sample
 
 
+def ragged_bench(args, dev):
+    '''Constrained realisations: per band limit one batched mixed-precision
+    solve of `batch` right-hand sides (data + S^-1/2 w + P^T N^-1/2 w', seeds
+    [k, i]), pinv + multigrid(T) preconditioning, FP64-verified residual.'''
+    lmaxs = [int(v) for v in args.ragged_lmaxs.split(',')]
+    t0 = time.perf_counter()
+    work, solvers_ = [], []
+    for k, (prob, data) in enumerate(workloads.ragged(lmaxs, per=args.batch, seed=0,
+                                                      device=dev)):
+        sol = solvers.CGWienerMap.from_arrays(data, prob.minfo, prob.ainfo, prob.icov_ell,
+                                              prob.icov_pix, draw_constr=True, seed=100 + k,
+                                              device=dev)
+        sol.add_preconditioner(preconditioners.PseudoInvPreconditioner(
+            prob.ainfo, prob.icov_ell, prob.icov_pix, prob.minfo, prob.spin))
+        if not args.no_mg:
+            sol.add_preconditioner(preconditioners.MaskedPreconditioner(
+                prob.ainfo, prob.icov_ell[0:1, 0:1], 0, prob.mask_bool[0], prob.minfo,
+                min_pix=1000, n_jacobi=1, lmax_r_ell=6000, device=dev), sel=np.s_[0])
+        solvers_.append(sol)
+        work.append((sol.a_matrix, sol.b0, sol.preconditioner))
+        log(f'bucket lmax {prob.lmax}: setup done ({time.perf_counter() - t0:.1f}s)')
+    for w in range(args.warmup):
+        mixed.solve_ragged(work, delta=args.delta)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        res = mixed.solve_ragged(work, delta=args.delta)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t1
+    worst = 0.0
+    for sol, (x, info) in zip(solvers_, res):
+        worst = max(worst, float(np.max(true_residual(sol, x))))
+        log(f'lmax {sol.sht.ainfo.lmax}: iterations {info.iterations}, '
+            f'replacements {info.replacements}')
+    n = args.steps * args.batch * len(lmaxs)
+    out = {'metric': 'constrained realisations/sec, TQU, uneven lmax, resid 1e-6',
+           'value': n / el, 'unit': 'solves/s', 'n_gpus': 1, 'steps': args.steps,
+           'warmup': args.warmup, 'ms_per_step': 1e3 * el / args.steps,
+           'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+           'dtype': 'f32+f64', 'data': 'synthetic',
+           'config': {'workload': f'{args.batch} right-hand sides at each lmax {lmaxs}, '
+                                  'pinv + multigrid(T), mixed precision, one batched solve '
+                                  'per band limit',
+                      'true_residual_max': worst}}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -160,6 +207,11 @@ def main():
                     help='right-hand sides per mixed-precision batched solve')
     ap.add_argument('--delta', type=float, default=0.01,
                     help='residual reduction between float64 residual replacements')
+    ap.add_argument('--ragged', action='store_true',
+                    help='time the batched constrained realisations of uneven band limit '
+                         '(16 right-hand sides at each of lmax 255, 511, 1023, 2047) instead '
+                         'of the headline workload; prints its own JSON line')
+    ap.add_argument('--ragged-lmaxs', default='255,511,1023,2047')
     ap.add_argument('--roofline-only', action='store_true',
                     help='only build the workload geometry and time the roofline kernel '
                          '(for counter runs), after one copy of known size')
@@ -170,6 +222,10 @@ def main():
         sys.exit(2)
     dev = torch.device('cuda:0')
     torch.cuda.set_device(dev)
+
+    if args.ragged:
+        ragged_bench(args, dev)
+        return
 
     if args.roofline_only:
         # copy of known size first (256 MiB read + 256 MiB written): the

@@ -192,3 +192,17 @@ def solve_fp64(a_matrix, b, precond=None, tol=1e-6, maxiter=1000, verify=False):
         res = np.sqrt(dot(r64, r64).cpu().numpy() / bn2)
     info = SolveInfo(it, it_rhs, 0, res)
     return (x[0] if squeeze else x), info
+
+
+def solve_ragged(buckets, tol=1e-6, delta=0.01, fp64=False):
+    """Right-hand sides of uneven band limit: one batched solve per bucket
+    (the operators differ per band limit). buckets: list of
+    (a_matrix, b, precond) with b (nbatch, ncomp, nelem) at the bucket's
+    lmax. Mixed precision unless fp64. Returns [(x, SolveInfo)]."""
+    out = []
+    for a_matrix, b, precond in buckets:
+        if fp64:
+            out.append(solve_fp64(a_matrix, b, precond, tol=tol, verify=True))
+        else:
+            out.append(solve_mixed(a_matrix, b, precond, tol=tol, delta=delta))
+    return out

@@ -200,3 +200,16 @@ def batch_data(prob, nbatch, seed=0, synth=None, device=None):
             sig = synth(a, prob.minfo, prob.lmax, prob.spin)
         out[i] = (sig + s_noise.standard_normal((npol, prob.minfo.npix)) * sig_pix) * obs
     return out
+
+
+def ragged(lmaxs=(255, 511, 1023, 2047), per=16, seed=0, device=None, synth=None):
+    """Batched constrained realisations of uneven band limit: for each lmax a
+    polarized problem (same spectra, depth and mask construction) and `per`
+    data maps (batch_data); the solvers draw S^-1/2 w and N^-1/2 w' per
+    right-hand side from the seed sequence [seed, i]. Returns
+    [(problem, data (per, 3, npix))]."""
+    out = []
+    for k, lmax in enumerate(lmaxs):
+        prob = polarized(lmax, seed=seed + k, synth=synth, device=device)
+        out.append((prob, batch_data(prob, per, seed=seed + k, synth=synth, device=device)))
+    return out
