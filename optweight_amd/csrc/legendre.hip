@@ -33,6 +33,14 @@ namespace owmi {
 
 namespace {
 
+// The recurrence (x, coefficients, seeds, mu) always runs in float64: near
+// the poles 1 - x ~ 1e-6 at lmax 2047 is not resolved by float32 and the
+// three-term recurrence is ill-conditioned there (float32 transforms were
+// off by up to 3e-2 on the polar ring). The float32 kernels keep the alm,
+// ring coefficients and accumulations in float32; float64 VALU runs at the
+// float32 (non-packed) rate on this hardware.
+using R = double;
+
 struct LegArgs {
   const int *pair_n;
   const int *pair_s;
@@ -124,11 +132,11 @@ constexpr int kSynCH = 64;
 
 template <typename T, int NB>
 __global__ void __launch_bounds__(256)
-synth0_kernel(LegArgs g, const T *__restrict__ xs, const int *__restrict__ lfs,
-              const T *__restrict__ seeds, const T *__restrict__ coef,
+synth0_kernel(LegArgs g, const R *__restrict__ xs, const int *__restrict__ lfs,
+              const R *__restrict__ seeds, const R *__restrict__ coef,
               const cplx_t<T> *__restrict__ alm, long long alm_ts, long long ntrans,
               cplx_t<T> *G) {
-  __shared__ T sA[kSynCH];
+  __shared__ R sA[kSynCH];
   __shared__ cplx_t<T> sa[kSynCH][NB];
   __shared__ int smin;
   const long long ip = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -145,11 +153,11 @@ synth0_kernel(LegArgs g, const T *__restrict__ xs, const int *__restrict__ lfs,
 #pragma unroll
   for (int b = 0; b < NB; ++b) E[b] = O[b] = {0, 0};
   if (lmin <= lmax) {
-    const T x = active ? xs[ip] : (T)0;
-    const T sA0 = active ? seeds[2 * si] : (T)0, sB0 = active ? seeds[2 * si + 1] : (T)0;
+    const R x = active ? xs[ip] : (R)0;
+    const R sA0 = active ? seeds[2 * si] : (R)0, sB0 = active ? seeds[2 * si + 1] : (R)0;
     const cplx_t<T> *a = alm + t0 * alm_ts + ms;
-    const T *cf = coef + 2 * ms;
-    T l1 = 0, l2 = 0;
+    const R *cf = coef + 2 * ms;
+    R l1 = 0, l2 = 0;
     for (int l0 = lmin; l0 <= lmax; l0 += kSynCH) {
       const int nl = min(kSynCH, lmax - l0 + 1);
       __syncthreads();
@@ -158,7 +166,7 @@ synth0_kernel(LegArgs g, const T *__restrict__ xs, const int *__restrict__ lfs,
         if (jl < nl) {
           const int l = l0 + jl;
           if (b == 0) sA[jl] = cf[2 * l];
-          const T c = cf[2 * l + 1];
+          const T c = (T)cf[2 * l + 1];
           const cplx_t<T> av = b < nt ? a[b * alm_ts + l] : cplx_t<T>{0, 0};
           sa[jl][b] = {av.x * c, av.y * c};
         }
@@ -168,7 +176,7 @@ synth0_kernel(LegArgs g, const T *__restrict__ xs, const int *__restrict__ lfs,
       const bool slow = __syncthreads_or(inj);
       for (int i = 0; i < nl; ++i) {
         const int l = l0 + i;
-        const T lam = fma(sA[i] * x, l1, -l2);
+        const R lam = fma(sA[i] * x, l1, -l2);
         if (slow && l == lf) {
           l2 = sA0;
           l1 = sB0;
@@ -176,12 +184,13 @@ synth0_kernel(LegArgs g, const T *__restrict__ xs, const int *__restrict__ lfs,
           l2 = l1;
           l1 = lam;
         }
+        const T lt = (T)l1;
         if ((l - m) & 1) {
 #pragma unroll
-          for (int b = 0; b < NB; ++b) O[b] = cfma(l1, sa[i][b], O[b]);
+          for (int b = 0; b < NB; ++b) O[b] = cfma(lt, sa[i][b], O[b]);
         } else {
 #pragma unroll
-          for (int b = 0; b < NB; ++b) E[b] = cfma(l1, sa[i][b], E[b]);
+          for (int b = 0; b < NB; ++b) E[b] = cfma(lt, sa[i][b], E[b]);
         }
       }
     }
@@ -201,12 +210,12 @@ synth0_kernel(LegArgs g, const T *__restrict__ xs, const int *__restrict__ lfs,
 
 template <typename T, int NB>
 __global__ void __launch_bounds__(256)
-synth2_kernel(LegArgs g, const T *__restrict__ xs, const int *__restrict__ lfps,
-              const T *__restrict__ sps, const int *__restrict__ lfms, const T *__restrict__ sms,
-              const T *__restrict__ coef, const cplx_t<T> *__restrict__ alm, long long alm_ts,
+synth2_kernel(LegArgs g, const R *__restrict__ xs, const int *__restrict__ lfps,
+              const R *__restrict__ sps, const int *__restrict__ lfms, const R *__restrict__ sms,
+              const R *__restrict__ coef, const cplx_t<T> *__restrict__ alm, long long alm_ts,
               long long ntrans, cplx_t<T> *G) {
-  __shared__ T sA[kSynCH];
-  __shared__ T sB[kSynCH];
+  __shared__ R sA[kSynCH];
+  __shared__ R sB[kSynCH];
   __shared__ cplx_t<T> sP[kSynCH][NB];
   __shared__ cplx_t<T> sM[kSynCH][NB];
   __shared__ int smin;
@@ -228,12 +237,12 @@ synth2_kernel(LegArgs g, const T *__restrict__ xs, const int *__restrict__ lfps,
   for (int b = 0; b < NB; ++b)
     PPe[b] = PMe[b] = MPe[b] = MMe[b] = PPo[b] = PMo[b] = MPo[b] = MMo[b] = {0, 0};
   if (lmin <= lmax) {
-    const T x = active ? xs[ip] : (T)0;
-    const T pa = active ? sps[2 * si] : (T)0, pb = active ? sps[2 * si + 1] : (T)0;
-    const T qa = active ? sms[2 * si] : (T)0, qb = active ? sms[2 * si + 1] : (T)0;
+    const R x = active ? xs[ip] : (R)0;
+    const R pa = active ? sps[2 * si] : (R)0, pb = active ? sps[2 * si + 1] : (R)0;
+    const R qa = active ? sms[2 * si] : (R)0, qb = active ? sms[2 * si + 1] : (R)0;
     const cplx_t<T> *aE = alm + t0 * alm_ts + ms;
-    const T *cf = coef + 3 * ms;
-    T p1 = 0, p2 = 0, q1 = 0, q2 = 0;
+    const R *cf = coef + 3 * ms;
+    R p1 = 0, p2 = 0, q1 = 0, q2 = 0;
     for (int l0 = lmin; l0 <= lmax; l0 += kSynCH) {
       const int nl = min(kSynCH, lmax - l0 + 1);
       __syncthreads();
@@ -245,7 +254,7 @@ synth2_kernel(LegArgs g, const T *__restrict__ xs, const int *__restrict__ lfps,
             sA[jl] = cf[3 * l];
             sB[jl] = cf[3 * l + 1];
           }
-          const T c = (T)-0.5 * cf[3 * l + 2];
+          const T c = (T)(-0.5 * cf[3 * l + 2]);
           cplx_t<T> e = {0, 0}, bb = {0, 0};
           if (b < nt) {
             e = aE[b * alm_ts + l];
@@ -261,9 +270,9 @@ synth2_kernel(LegArgs g, const T *__restrict__ xs, const int *__restrict__ lfps,
       const bool slow = __syncthreads_or(inj);
       for (int i = 0; i < nl; ++i) {
         const int l = l0 + i;
-        const T ax = sA[i] * x;
-        const T np_ = fma(ax + sB[i], p1, -p2);
-        const T nq_ = fma(ax - sB[i], q1, -q2);
+        const R ax = sA[i] * x;
+        const R np_ = fma(ax + sB[i], p1, -p2);
+        const R nq_ = fma(ax - sB[i], q1, -q2);
         if (slow && l == lfp) {
           p2 = pa;
           p1 = pb;
@@ -278,23 +287,24 @@ synth2_kernel(LegArgs g, const T *__restrict__ xs, const int *__restrict__ lfps,
           q2 = q1;
           q1 = nq_;
         }
+        const T pt = (T)p1, qt = (T)q1;
         if ((l - m) & 1) {
 #pragma unroll
           for (int b = 0; b < NB; ++b) {
             const cplx_t<T> P = sP[i][b], Mm = sM[i][b];
-            PPo[b] = cfma(p1, P, PPo[b]);
-            PMo[b] = cfma(q1, P, PMo[b]);
-            MPo[b] = cfma(p1, Mm, MPo[b]);
-            MMo[b] = cfma(q1, Mm, MMo[b]);
+            PPo[b] = cfma(pt, P, PPo[b]);
+            PMo[b] = cfma(qt, P, PMo[b]);
+            MPo[b] = cfma(pt, Mm, MPo[b]);
+            MMo[b] = cfma(qt, Mm, MMo[b]);
           }
         } else {
 #pragma unroll
           for (int b = 0; b < NB; ++b) {
             const cplx_t<T> P = sP[i][b], Mm = sM[i][b];
-            PPe[b] = cfma(p1, P, PPe[b]);
-            PMe[b] = cfma(q1, P, PMe[b]);
-            MPe[b] = cfma(p1, Mm, MPe[b]);
-            MMe[b] = cfma(q1, Mm, MMe[b]);
+            PPe[b] = cfma(pt, P, PPe[b]);
+            PMe[b] = cfma(qt, P, PMe[b]);
+            MPe[b] = cfma(pt, Mm, MPe[b]);
+            MMe[b] = cfma(qt, Mm, MMe[b]);
           }
         }
       }
@@ -373,8 +383,8 @@ struct AdjItems {
 
 template <typename T, int K, int CH, int NB>
 __global__ void __launch_bounds__(64 * kAdjWaves)
-adj0_kernel(LegArgs g, AdjItems it, const T *__restrict__ xs, const int *__restrict__ lfs,
-            const T *__restrict__ seeds, const T *__restrict__ coef,
+adj0_kernel(LegArgs g, AdjItems it, const R *__restrict__ xs, const int *__restrict__ lfs,
+            const R *__restrict__ seeds, const R *__restrict__ coef,
             const cplx_t<T> *__restrict__ G, long long ntrans, cplx_t<T> *out,
             long long out_ts) {
   constexpr int ROWS = CH * NB;  // reduction rows per chunk: (multipole, transform)
@@ -388,17 +398,17 @@ adj0_kernel(LegArgs g, AdjItems it, const T *__restrict__ xs, const int *__restr
   const int nt = (int)min((long long)NB, ntrans - t0);
   const int lmax = g.lmax;
   const long long stride = g.mmax + 1;
-  constexpr int R = 64 / ROWS;
-  const int row = lane / R, ri = row / NB, rb = row % NB;
-  const bool writer = (lane % R) == 0;
+  constexpr int RL = 64 / ROWS;  // lanes per reduction row
+  const int row = lane / RL, ri = row / NB, rb = row % NB;
+  const bool writer = (lane % RL) == 0;
 
   for (int pass = 0; pass < 2; ++pass) {
     const int m = pass == 0 ? q : g.mmax - q;
     if (pass == 1 && m == q) break;
     const long long ms = mstart_of(m, lmax);
-    const T *cf = coef + 2 * ms;
+    const R *cf = coef + 2 * ms;
     cplx_t<T> *om = out + grp * it.grp_ts + t0 * out_ts + ms;
-    T x[K], l1[K], l2[K], sa0[K], sb0[K];
+    R x[K], l1[K], l2[K], sa0[K], sb0[K];
     int lf[K];
     cplx_t<T> He[K][NB], Ho[K][NB];
     int mylf = INT_MAX;
@@ -446,7 +456,7 @@ adj0_kernel(LegArgs g, AdjItems it, const T *__restrict__ xs, const int *__restr
 #pragma unroll
       for (int k = 0; k < K; ++k) inj |= (lf[k] >= l0 && lf[k] < l0 + nl);
       // wave-uniform coefficients of the chunk (clamped index: scalar loads)
-      T Ac[CH];
+      R Ac[CH];
 #pragma unroll
       for (int i = 0; i < CH; ++i) Ac[i] = cf[2 * min(l0 + i, lmax)];
       // weights of the first and second multipole of the chunk (parity
@@ -471,12 +481,13 @@ adj0_kernel(LegArgs g, AdjItems it, const T *__restrict__ xs, const int *__restr
         for (int i = 0; i < CH; ++i) {
 #pragma unroll
           for (int k = 0; k < K; ++k) {
-            const T lam = fma(Ac[i] * x[k], l1[k], -l2[k]);
+            const R lam = fma(Ac[i] * x[k], l1[k], -l2[k]);
             l2[k] = l1[k];
             l1[k] = lam;
+            const T lt = (T)lam;
 #pragma unroll
             for (int b = 0; b < NB; ++b)
-              s[i][b] = cfma(lam, (i & 1) ? H1[k][b] : H0[k][b], s[i][b]);
+              s[i][b] = cfma(lt, (i & 1) ? H1[k][b] : H0[k][b], s[i][b]);
           }
         }
       } else {
@@ -487,13 +498,14 @@ adj0_kernel(LegArgs g, AdjItems it, const T *__restrict__ xs, const int *__restr
             const int l = l0 + i;
 #pragma unroll
             for (int k = 0; k < K; ++k) {
-              const T lam = fma(Ac[i] * x[k], l1[k], -l2[k]);
+              const R lam = fma(Ac[i] * x[k], l1[k], -l2[k]);
               const bool st = (l == lf[k]);
               l2[k] = st ? sa0[k] : l1[k];
               l1[k] = st ? sb0[k] : lam;
+              const T lt = (T)l1[k];
 #pragma unroll
               for (int b = 0; b < NB; ++b)
-                s[i][b] = cfma(l1[k], (i & 1) ? H1[k][b] : H0[k][b], s[i][b]);
+                s[i][b] = cfma(lt, (i & 1) ? H1[k][b] : H0[k][b], s[i][b]);
             }
           }
         }
@@ -507,7 +519,7 @@ adj0_kernel(LegArgs g, AdjItems it, const T *__restrict__ xs, const int *__restr
       const cplx_t<T> tot = wave_rows<T, ROWS>(red, lane);
       if (writer && ri < nl && rb < nt) {
         const int l = l0 + ri;
-        const T c = cf[2 * l + 1];
+        const T c = (T)cf[2 * l + 1];
         om[rb * out_ts + l] = {c * tot.x, c * tot.y};
       }
     }
@@ -516,9 +528,9 @@ adj0_kernel(LegArgs g, AdjItems it, const T *__restrict__ xs, const int *__restr
 
 template <typename T, int K, int CH, int NB>
 __global__ void __launch_bounds__(64 * kAdjWaves)
-adj2_kernel(LegArgs g, AdjItems it, const T *__restrict__ xs, const int *__restrict__ lfps,
-            const T *__restrict__ sps, const int *__restrict__ lfms, const T *__restrict__ sms,
-            const T *__restrict__ coef, const cplx_t<T> *__restrict__ G, long long ntrans,
+adj2_kernel(LegArgs g, AdjItems it, const R *__restrict__ xs, const int *__restrict__ lfps,
+            const R *__restrict__ sps, const int *__restrict__ lfms, const R *__restrict__ sms,
+            const R *__restrict__ coef, const cplx_t<T> *__restrict__ G, long long ntrans,
             cplx_t<T> *out, long long out_ts) {
   constexpr int ROWS = CH * NB;
   __shared__ cplx_t<T> redP_all[kAdjWaves][ROWS][64];
@@ -534,17 +546,17 @@ adj2_kernel(LegArgs g, AdjItems it, const T *__restrict__ xs, const int *__restr
   const int lmax = g.lmax;
   const long long nelem = nelem_of(lmax);
   const long long stride = g.mmax + 1;
-  constexpr int R = 64 / ROWS;
-  const int row = lane / R, ri = row / NB, rb = row % NB;
-  const bool writer = (lane % R) == 0;
+  constexpr int RL = 64 / ROWS;  // lanes per reduction row
+  const int row = lane / RL, ri = row / NB, rb = row % NB;
+  const bool writer = (lane % RL) == 0;
 
   for (int pass = 0; pass < 2; ++pass) {
     const int m = pass == 0 ? q : g.mmax - q;
     if (pass == 1 && m == q) break;
     const long long ms = mstart_of(m, lmax);
-    const T *cf = coef + 3 * ms;
+    const R *cf = coef + 3 * ms;
     cplx_t<T> *oE = out + grp * it.grp_ts + t0 * out_ts + ms;
-    T x[K], p1[K], p2[K], q1[K], q2[K], pa[K], pb[K], qa[K], qb[K];
+    R x[K], p1[K], p2[K], q1[K], q2[K], pa[K], pb[K], qa[K], qb[K];
     int lfp[K], lfm[K];
     cplx_t<T> ZpN[K][NB], ZmN[K][NB], ZpS[K][NB], ZmS[K][NB];
     int mylf = INT_MAX;
@@ -599,7 +611,7 @@ adj2_kernel(LegArgs g, AdjItems it, const T *__restrict__ xs, const int *__restr
       for (int k = 0; k < K; ++k)
         inj |= (lfp[k] >= l0 && lfp[k] < l0 + nl) || (lfm[k] >= l0 && lfm[k] < l0 + nl);
       // wave-uniform coefficients of the chunk (clamped index: scalar loads)
-      T Ac[CH], Bc[CH];
+      R Ac[CH], Bc[CH];
 #pragma unroll
       for (int i = 0; i < CH; ++i) {
         const int l = min(l0 + i, lmax);
@@ -630,19 +642,20 @@ adj2_kernel(LegArgs g, AdjItems it, const T *__restrict__ xs, const int *__restr
         for (int i = 0; i < CH; ++i) {
 #pragma unroll
           for (int k = 0; k < K; ++k) {
-            const T ax = Ac[i] * x[k];
-            const T np_ = fma(ax + Bc[i], p1[k], -p2[k]);
-            const T nq_ = fma(ax - Bc[i], q1[k], -q2[k]);
+            const R ax = Ac[i] * x[k];
+            const R np_ = fma(ax + Bc[i], p1[k], -p2[k]);
+            const R nq_ = fma(ax - Bc[i], q1[k], -q2[k]);
             p2[k] = p1[k];
             p1[k] = np_;
             q2[k] = q1[k];
             q1[k] = nq_;
-            const T nqs = (i & 1) ? -nq_ : nq_, nps = (i & 1) ? -np_ : np_;
+            const T pt = (T)np_, qt = (T)nq_;
+            const T nqs = (i & 1) ? -qt : qt, nps = (i & 1) ? -pt : pt;
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
-              sP[i][b] = cfma(np_, ZpN[k][b], sP[i][b]);
+              sP[i][b] = cfma(pt, ZpN[k][b], sP[i][b]);
               sP[i][b] = cfma(nqs, PS[k][b], sP[i][b]);
-              sM[i][b] = cfma(nq_, ZmN[k][b], sM[i][b]);
+              sM[i][b] = cfma(qt, ZmN[k][b], sM[i][b]);
               sM[i][b] = cfma(nps, MS[k][b], sM[i][b]);
             }
           }
@@ -655,20 +668,21 @@ adj2_kernel(LegArgs g, AdjItems it, const T *__restrict__ xs, const int *__restr
             const int l = l0 + i;
 #pragma unroll
             for (int k = 0; k < K; ++k) {
-              const T ax = Ac[i] * x[k];
-              const T np_ = fma(ax + Bc[i], p1[k], -p2[k]);
-              const T nq_ = fma(ax - Bc[i], q1[k], -q2[k]);
+              const R ax = Ac[i] * x[k];
+              const R np_ = fma(ax + Bc[i], p1[k], -p2[k]);
+              const R nq_ = fma(ax - Bc[i], q1[k], -q2[k]);
               const bool sp_ = (l == lfp[k]), sq_ = (l == lfm[k]);
               p2[k] = sp_ ? pa[k] : p1[k];
               p1[k] = sp_ ? pb[k] : np_;
               q2[k] = sq_ ? qa[k] : q1[k];
               q1[k] = sq_ ? qb[k] : nq_;
-              const T nqs = (i & 1) ? -q1[k] : q1[k], nps = (i & 1) ? -p1[k] : p1[k];
+              const T pt = (T)p1[k], qt = (T)q1[k];
+              const T nqs = (i & 1) ? -qt : qt, nps = (i & 1) ? -pt : pt;
 #pragma unroll
               for (int b = 0; b < NB; ++b) {
-                sP[i][b] = cfma(p1[k], ZpN[k][b], sP[i][b]);
+                sP[i][b] = cfma(pt, ZpN[k][b], sP[i][b]);
                 sP[i][b] = cfma(nqs, PS[k][b], sP[i][b]);
-                sM[i][b] = cfma(q1[k], ZmN[k][b], sM[i][b]);
+                sM[i][b] = cfma(qt, ZmN[k][b], sM[i][b]);
                 sM[i][b] = cfma(nps, MS[k][b], sM[i][b]);
               }
             }
@@ -688,7 +702,7 @@ adj2_kernel(LegArgs g, AdjItems it, const T *__restrict__ xs, const int *__restr
       const cplx_t<T> tm = wave_rows<T, ROWS>(redM, lane);
       if (writer && ri < nl && rb < nt) {
         const int l = l0 + ri;
-        const T c = (T)0.5 * cf[3 * l + 2];
+        const T c = (T)(0.5 * cf[3 * l + 2]);
         // aE = -(S+ + S-)/2, aB = (i/2)(S+ - S-)
         cplx_t<T> *o = oE + rb * out_ts;
         o[l] = {-c * (tp.x + tm.x), -c * (tp.y + tm.y)};
@@ -714,13 +728,6 @@ __global__ void sum_groups_kernel(const cplx_t<T> *part, long long grp_ts, int n
   }
   out[t * out_ts + i] = acc;
 }
-
-template <typename T>
-PlanTables<T> &tab(owmi_plan_impl *p);
-template <>
-PlanTables<double> &tab<double>(owmi_plan_impl *p) { return p->t64; }
-template <>
-PlanTables<float> &tab<float>(owmi_plan_impl *p) { return p->t32; }
 
 LegArgs leg_args(owmi_plan_impl *p) {
   LegArgs a;
@@ -759,7 +766,7 @@ void launch_seed_finder(owmi_plan_impl *p, const double *xs, const double *v0, c
 template <typename T, int NB>
 static void launch_synth_nb(owmi_plan_impl *p, int spin, const cplx_t<T> *alm, long long alm_ts,
                             cplx_t<T> *G, long long ntrans, hipStream_t st) {
-  PlanTables<T> &tb = tab<T>(p);
+  PlanTables<R> &tb = p->t64;  // recurrence tables (float64 for every precision)
   int bs = (int)std::min<long long>(256, ((p->npairs + 63) / 64) * 64);
   dim3 grid(ceil_div(p->npairs, bs), (unsigned)(p->mmax + 1), ceil_div(ntrans, NB));
   if (spin == 0)
@@ -786,7 +793,7 @@ void launch_synthesis(owmi_plan_impl *p, int spin, const cplx_t<T> *alm, long lo
 template <typename T, int SPIN, int K, int NB>
 static void launch_adj_k(owmi_plan_impl *p, const cplx_t<T> *G, cplx_t<T> *alm,
                          long long alm_ts, long long ntrans, hipStream_t st) {
-  PlanTables<T> &tb = tab<T>(p);
+  PlanTables<R> &tb = p->t64;  // recurrence tables (float64 for every precision)
   AdjItems it;
   it.nmpairs = (p->mmax + 2) / 2;
   it.ngroups = (int)((p->npairs + 64LL * K - 1) / (64LL * K));

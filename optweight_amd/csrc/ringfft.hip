@@ -98,6 +98,17 @@ __device__ __attribute__((noinline)) cplx_t<T> fold_bin(const cplx_t<T> *F, long
   return acc;
 }
 
+// Workgroup barrier for the FFT buffer: LDS, or (GB) a global-memory
+// region private to the workgroup for rings too long for LDS, where the
+// barrier is wrapped in device-scope release / acquire fences so that the
+// other waves' stores are visible.
+template <bool GB>
+__device__ __forceinline__ void block_sync() {
+  if (GB) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  if (GB) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
 // ------------------------------------------------------------------ FFT
 //
 // Stockham autosort FFT of length M = 2^logm (logm >= 4) held by Q = M/16
@@ -215,14 +226,14 @@ __device__ __forceinline__ void first_pass_small(cplx_t<T> *v, cplx_t<T> *buf, i
 
 // Full FFT of the thread's 16 slots (see above). All threads of the block
 // call it (barriers); threads with q >= Q pass through.
-template <typename T, int DIR>
-__device__ __forceinline__ void fft16(cplx_t<T> *v, cplx_t<T> *buf, int logm, int q, const cplx_t<T> *tw,
-                      int log_mmax) {
+template <typename T, int DIR, bool GB>
+__device__ __forceinline__ void fft16(cplx_t<T> *v, cplx_t<T> *buf, int logm, int q,
+                                      const cplx_t<T> *tw, int log_mmax) {
   const int Q = 1 << (logm - 4);
   const bool on = q < Q;
   const int r0 = logm & 3;
   int Ns;
-  __syncthreads();  // the previous user of buf has finished reading it
+  block_sync<GB>();  // the previous user of buf has finished reading it
   if (r0 == 0) {
     if (logm == 4) {
       if (on) Dft<T, DIR, 16>::run(v, 0, 1);
@@ -248,7 +259,7 @@ __device__ __forceinline__ void fft16(cplx_t<T> *v, cplx_t<T> *buf, int logm, in
   // radix-16 passes; the last one keeps its outputs in registers
   int lns = r0 == 0 ? 4 : r0;  // log2 Ns
   while (true) {
-    __syncthreads();
+    block_sync<GB>();
     const bool last = lns + 4 == logm;
     const int k = q & (Ns - 1);
     if (on) {
@@ -266,7 +277,7 @@ __device__ __forceinline__ void fft16(cplx_t<T> *v, cplx_t<T> *buf, int logm, in
       Dft<T, DIR, 16>::run(v, 0, 1);
     }
     if (last) return;
-    __syncthreads();
+    block_sync<GB>();
     if (on) {
       const int base = ((q >> lns) << (lns + 4)) + k;
 #pragma unroll
@@ -297,29 +308,32 @@ struct RingArgs {
 
 // Cyclic convolution with the ring's Bluestein kernel, in the thread's
 // slots: v <- IDFT_M(DFT_M(v) * bhat) (bhat includes 1/M).
-template <typename T>
+template <typename T, bool GB>
 __device__ __forceinline__ void bluestein_conv(cplx_t<T> *v, cplx_t<T> *buf, int logm, int q,
                                                const cplx_t<T> *tw, int log_mmax,
                                                const cplx_t<T> *bhat) {
   const int Q = 1 << (logm - 4);
-  fft16<T, -1>(v, buf, logm, q, tw, log_mmax);
+  fft16<T, -1, GB>(v, buf, logm, q, tw, log_mmax);
   if (q < Q) {
 #pragma unroll
     for (int s = 0; s < 16; ++s) v[s] = cmul<T>(v[s], bhat[q + s * Q]);
   }
-  fft16<T, 1>(v, buf, logm, q, tw, log_mmax);
+  fft16<T, 1, GB>(v, buf, logm, q, tw, log_mmax);
 }
 
 // MAXT: the launch's workgroup size bound (M / 16 threads rounded up to
-// 256, 512 or 1024), which sets the register budget per thread.
-template <typename T, int MODE, int MAXT>
+// 256, 512 or 1024), which sets the register budget per thread. GB: the
+// FFT buffer of each workgroup is a region of gbuf (region elements) in
+// global memory instead of LDS.
+template <typename T, int MODE, int MAXT, bool GB>
 __global__ void __launch_bounds__(MAXT)
 ring_kernel(RingArgs a, const cplx_t<T> *twiddle, const cplx_t<T> *chirp,
             const cplx_t<T> *bhat_all, const T *ringw, const T *phi0s, const cplx_t<T> *Gin,
             cplx_t<T> *Gout, const T *map_in, T *map_out, long long map_ts, const T *diag,
-            long long diag_ts) {
+            long long diag_ts, cplx_t<T> *gbuf, long long region) {
   extern __shared__ unsigned char smem_raw[];
-  cplx_t<T> *buf = reinterpret_cast<cplx_t<T> *>(smem_raw);
+  cplx_t<T> *buf = GB ? gbuf + ((long long)blockIdx.y * gridDim.x + blockIdx.x) * region
+                      : reinterpret_cast<cplx_t<T> *>(smem_raw);
 
   const long long job = blockIdx.x;
   const long long tc = blockIdx.y;  // transform * ncomp + comp
@@ -363,7 +377,7 @@ ring_kernel(RingArgs a, const cplx_t<T> *twiddle, const cplx_t<T> *chirp,
         v[s] = val;
       }
     }
-    bluestein_conv<T>(v, buf, logm, q, twiddle, a.log_mmax_fft, bhat);
+    bluestein_conv<T, GB>(v, buf, logm, q, twiddle, a.log_mmax_fft, bhat);
     // DFT(conj Z)_j = w_j c_j and x = conj of it: ring a gets Re(w_j c_j),
     // ring b gets -Im(w_j c_j).
     if (on) {
@@ -418,9 +432,9 @@ ring_kernel(RingArgs a, const cplx_t<T> *twiddle, const cplx_t<T> *chirp,
   }
 
   if (MODE == kAdjoint || MODE == kSandwich) {
-    bluestein_conv<T>(v, buf, logm, q, twiddle, a.log_mmax_fft, bhat);
-    // X_k = w_k c_k, gathered in LDS (natural order) for the Hermitian split
-    __syncthreads();
+    bluestein_conv<T, GB>(v, buf, logm, q, twiddle, a.log_mmax_fft, bhat);
+    // X_k = w_k c_k, gathered in the buffer (natural order) for the Hermitian split
+    block_sync<GB>();
     if (on) {
 #pragma unroll
       for (int s = 0; s < 16; ++s) {
@@ -428,7 +442,7 @@ ring_kernel(RingArgs a, const cplx_t<T> *twiddle, const cplx_t<T> *chirp,
         if (k < N) buf[pad16((int)k)] = cmul<T>(w[k], v[s]);
       }
     }
-    __syncthreads();
+    block_sync<GB>();
     cplx_t<T> *Ga = Gout + G_row_a * (mmax + 1);
     cplx_t<T> *Gb = Gout + G_row_b * (mmax + 1);
     for (int m = threadIdx.x; m <= mmax; m += blockDim.x) {
@@ -488,7 +502,10 @@ bluestein_init_kernel(const long long *lens, const int *logms, const long long *
   for (int k = threadIdx.x; k < M; k += blockDim.x) out[k] = {buf[k].x * inv, buf[k].y * inv};
 }
 
-// LDS of one ring workgroup: M + M/16 (padding) complex elements.
+// LDS of one workgroup (160 KiB on gfx950).
+constexpr size_t kMaxLds = 160 * 1024;
+
+// FFT buffer of one ring workgroup: M + M/16 (padding) complex elements.
 template <typename T>
 size_t ring_lds_bytes(int mmax_fft) {
   return (size_t)(mmax_fft + mmax_fft / 16) * sizeof(cplx_t<T>);
@@ -537,17 +554,27 @@ void launch_ring(owmi_plan_impl *p, const cplx_t<T> *Gin, cplx_t<T> *Gout, const
   const size_t lds = ring_lds_bytes<T>(p->mmax_fft);
   const int threads = std::max(64, p->mmax_fft / 16);
   OWMI_REQUIRE(threads <= 1024, "ring too long for one workgroup");
-  auto kern = threads <= 256   ? ring_kernel<T, MODE, 256>
-              : threads <= 512 ? ring_kernel<T, MODE, 512>
-                               : ring_kernel<T, MODE, 1024>;
-  allow_lds(kern, lds);
   PlanTables<T> &tb = tab<T>(p);
   dim3 grid((unsigned)p->nfft_jobs, (unsigned)nrows);
-  hipLaunchKernelGGL(kern, grid, dim3(threads), lds, st, ring_args<T>(p, ncomp, weighted),
+  const bool gb = lds > kMaxLds;
+  cplx_t<T> *gbuf = nullptr;
+  const long long region = p->mmax_fft + p->mmax_fft / 16;
+  if (gb) {
+    const size_t bytes = (size_t)p->nfft_jobs * nrows * region * sizeof(cplx_t<T>);
+    OWMI_REQUIRE(bytes <= (size_t)32 << 30, "ring-FFT scratch above 32 GiB: split the batch");
+    gbuf = static_cast<cplx_t<T> *>(p->ensure_work3(bytes));
+  }
+  auto kern = gb ? ring_kernel<T, MODE, 1024, true>
+              : threads <= 256 ? ring_kernel<T, MODE, 256, false>
+              : threads <= 512 ? ring_kernel<T, MODE, 512, false>
+                               : ring_kernel<T, MODE, 1024, false>;
+  const size_t dyn = gb ? 0 : lds;
+  allow_lds(kern, dyn);
+  hipLaunchKernelGGL(kern, grid, dim3(threads), dyn, st, ring_args<T>(p, ncomp, weighted),
                      reinterpret_cast<const cplx_t<T> *>(tb.twiddle),
                      reinterpret_cast<const cplx_t<T> *>(tb.chirp),
                      reinterpret_cast<const cplx_t<T> *>(tb.bhat), tb.ringw, tb.phi0, Gin, Gout,
-                     map_in, map_out, map_ts, diag, diag_ts);
+                     map_in, map_out, map_ts, diag, diag_ts, gbuf, region);
 }
 
 }  // namespace
@@ -589,8 +616,10 @@ void launch_bluestein_init(owmi_plan_impl *p, const std::vector<long long> &lens
   OWMI_HIP(hipMemcpy(d_boff, bhat_off.data(), lens.size() * sizeof(long long),
                      hipMemcpyHostToDevice));
   OWMI_HIP(hipMemcpy(d_logm, logms.data(), lens.size() * sizeof(int), hipMemcpyHostToDevice));
-  size_t lds = (size_t)p->mmax_fft * sizeof(double2);
-  OWMI_REQUIRE(lds <= 160 * 1024, "Bluestein length exceeds LDS");
+  int max_logm = 0;
+  for (int lm : logms) max_logm = std::max(max_logm, lm);
+  size_t lds = ((size_t)1 << max_logm) * sizeof(double2);
+  OWMI_REQUIRE(lds <= kMaxLds, "Bluestein length exceeds LDS");
   allow_lds(bluestein_init_kernel, lds);
   hipLaunchKernelGGL(bluestein_init_kernel, dim3((unsigned)lens.size()), dim3(256), lds, 0, d_lens,
                      d_logm, d_coff, d_boff, reinterpret_cast<const double2 *>(p->t64.chirp),

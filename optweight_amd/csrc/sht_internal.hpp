@@ -82,6 +82,10 @@ struct owmi_plan_impl {
   void *work2 = nullptr;
   size_t work2_bytes = 0;
   void *ensure_work2(size_t bytes);
+  // FFT buffers of rings whose Bluestein length exceeds LDS (grows)
+  void *work3 = nullptr;
+  size_t work3_bytes = 0;
+  void *ensure_work3(size_t bytes);
 
   size_t nchirp = 0, nbhat = 0;  // complex element counts of the tables
 };

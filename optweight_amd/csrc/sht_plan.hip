@@ -119,6 +119,70 @@ void *owmi_plan_impl::ensure_work2(size_t bytes) {
   return work2;
 }
 
+void *owmi_plan_impl::ensure_work3(size_t bytes) {
+  if (bytes <= work3_bytes) return work3;
+  if (work3) OWMI_HIP(hipFree(work3));
+  work3 = nullptr;
+  work3_bytes = 0;
+  OWMI_HIP(hipMalloc(&work3, bytes));
+  work3_bytes = bytes;
+  return work3;
+}
+
+// LDS budget of the device Bluestein-spectrum kernel (complex double).
+constexpr size_t kInitLds = 160 * 1024;
+
+// Bluestein kernel spectrum FFT_M(b) / M on the host in long double, for
+// lengths whose FFT does not fit LDS: b_n = conj(w_n) for n < N,
+// conj(w_{M-n}) for n > M - N, else 0, w_n = exp(-i pi n^2 / N); returns
+// interleaved (re, im) doubles.
+std::vector<double> host_bhat(long long N, int logm) {
+  const long long M = 1LL << logm;
+  std::vector<LD> re(M, 0), im(M, 0);
+  auto wconj = [&](long long n, LD *r, LD *i) {
+    const long long q = (n * n) % (2 * N);
+    const LD ang = kPi * (LD)q / (LD)N;  // conj of exp(-i pi q / N)
+    *r = cosl(ang);
+    *i = sinl(ang);
+  };
+  for (long long n = 0; n < M; ++n) {
+    if (n < N)
+      wconj(n, &re[n], &im[n]);
+    else if (n > M - N)
+      wconj(M - n, &re[n], &im[n]);
+  }
+  // iterative radix-2 FFT, exp(-2 pi i jk / M), bit-reversed input order
+  for (long long i = 1, j = 0; i < M; ++i) {
+    long long bit = M >> 1;
+    for (; j & bit; bit >>= 1) j ^= bit;
+    j ^= bit;
+    if (i < j) {
+      std::swap(re[i], re[j]);
+      std::swap(im[i], im[j]);
+    }
+  }
+  for (long long len = 2; len <= M; len <<= 1) {
+    const LD ang = -2 * kPi / (LD)len;
+    for (long long i = 0; i < M; i += len) {
+      for (long long k = 0; k < len / 2; ++k) {
+        const LD wr = cosl(ang * k), wi = sinl(ang * k);
+        const long long a = i + k, b = a + len / 2;
+        const LD xr = re[b] * wr - im[b] * wi, xi = re[b] * wi + im[b] * wr;
+        re[b] = re[a] - xr;
+        im[b] = im[a] - xi;
+        re[a] += xr;
+        im[a] += xi;
+      }
+    }
+  }
+  std::vector<double> out(2 * M);
+  for (long long k = 0; k < M; ++k) {
+    out[2 * k] = (double)(re[k] / (LD)M);
+    out[2 * k + 1] = (double)(im[k] / (LD)M);
+  }
+  return out;
+}
+
 // Host-side tables shared by both precisions (long double), built once.
 struct HostTables {
   // seeds at l = max(m, |s|) as v * 2^(128 k) (float64 v)
@@ -515,7 +579,24 @@ int create_plan(owmi_sht_plan **out, long long nrings, const double *theta,
       HostTables ht;
       build_host_tables(p, thn, xs, ht);
       build_tables<double>(p, p.t64, ht, xs, chirp);
-      launch_bluestein_init(&p, lens, logms, chirp_off, bhat_off);
+      {
+        // kernel spectra: on the device when the FFT fits LDS, else on the host
+        std::vector<long long> dl, dc, db;
+        std::vector<int> dm;
+        for (size_t li = 0; li < lens.size(); ++li) {
+          if (((size_t)16 << logms[li]) <= kInitLds) {
+            dl.push_back(lens[li]);
+            dm.push_back(logms[li]);
+            dc.push_back(chirp_off[li]);
+            db.push_back(bhat_off[li]);
+          } else {
+            std::vector<double> h = host_bhat(lens[li], logms[li]);
+            OWMI_HIP(hipMemcpy(p.t64.bhat + 2 * bhat_off[li], h.data(), h.size() * sizeof(double),
+                               hipMemcpyHostToDevice));
+          }
+        }
+        launch_bluestein_init(&p, dl, dm, dc, db);
+      }
       OWMI_HIP(hipDeviceSynchronize());
       build_tables<float>(p, p.t32, ht, xs, chirp);
       long long nb = 2 * (long long)p.nbhat;
@@ -557,7 +638,7 @@ int owmi_sht_plan_destroy(owmi_sht_plan *plan) {
   free_tables(p.t64);
   free_tables(p.t32);
   void *ptrs[] = {p.pair_n, p.pair_s, p.nphi, p.offsets, p.ring_logm, p.ring_chirp_off,
-                  p.ring_bhat_off, p.fft_ring_a, p.fft_ring_b, p.work, p.work2};
+                  p.ring_bhat_off, p.fft_ring_a, p.fft_ring_b, p.work, p.work2, p.work3};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   (void)hipSetDevice(dev);
@@ -571,11 +652,11 @@ long long owmi_sht_plan_npix(const owmi_sht_plan *plan) { return plan ? plan->im
 
 template <typename T>
 static void check_fft_fits(owmi_plan_impl &p) {
-  // M + M/16 complex elements of LDS per ring workgroup, at most 160 KiB
-  const size_t need = (size_t)(p.mmax_fft + p.mmax_fft / 16) * sizeof(cplx_t<T>);
-  OWMI_REQUIRE(need <= 160 * 1024,
-               "ring length too large for the on-chip FFT at this precision "
-               "(Bluestein length " + std::to_string(p.mmax_fft) + ")");
+  // M / 16 threads per ring workgroup, at most 1024; the buffer lives in
+  // LDS up to 160 KiB and in a global-memory workspace beyond
+  OWMI_REQUIRE(p.mmax_fft <= 16384,
+               "ring length too large for the ring FFT (Bluestein length " +
+                   std::to_string(p.mmax_fft) + " > 16384)");
 }
 
 template <typename T>

@@ -211,3 +211,23 @@ def test_batched_legendre_tails(dev, ntrans, dtype):
         ref = np.zeros((3, g.npix))
         osht.alm2map(a[ntrans - 1].cpu().numpy(), ref, lmax, g, [0, 2])
         assert rel(out[ntrans - 1].cpu().numpy(), ref) < 1e-12
+
+
+def test_long_rings_global_buffer(dev):
+    '''Rings of 4201 pixels need a Bluestein length of 16384: the float64
+    FFT buffer then lives in global memory and the kernel spectrum is
+    computed on the host. Y^T W Y = I at lmax 2100 (spin 0) in float64, and
+    float64 / float32 (LDS path) syntheses agree.'''
+    lmax = 2100
+    print(f'long rings lmax={lmax}', flush=True)
+    m = map_utils.MapInfo.map_info_gauss_legendre(lmax + 1, 2 * lmax + 1)
+    ai = alm_utils.AlmInfo(lmax)
+    a = torch.as_tensor(rand_alm(lmax, 1, 12), device=dev)
+    mp = torch.zeros((1, m.npix), dtype=torch.float64, device=dev)
+    sht.alm2map(a, mp, ai, m, 0)
+    back = torch.zeros_like(a)
+    sht.map2alm(mp, back, m, ai, 0)
+    assert rel(back.cpu().numpy(), a.cpu().numpy()) < 1e-10
+    mp32 = torch.zeros((1, m.npix), dtype=torch.float32, device=dev)
+    sht.alm2map(a.to(torch.complex64), mp32, ai, m, 0)
+    assert rel(mp32.double().cpu().numpy(), mp.cpu().numpy()) < 1e-5
