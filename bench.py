@@ -148,6 +148,47 @@ def cpu_baseline_sample(dev, lmax_sample=255, reps=3):
     return (time.perf_counter() - t0) / reps, lmax_sample
 
 
+def wavelet_bench(args, dev):
+    '''Harmonic-space temperature solve with the wavelet noise model:
+    CGWiener.from_arrays_wav + PseudoInvPreconditionerWav, mixed precision
+    (float32 operators, float64 residual replacement), FP64-verified.'''
+    lmax = args.wavelet_lmax
+    t0 = time.perf_counter()
+    wp = workloads.wavelet_temperature(lmax=lmax, device=dev)
+    log(f'wavelet workload lmax {lmax}: {time.perf_counter() - t0:.1f}s')
+    sol = solvers.CGWiener.from_arrays_wav(wp.alm_data, wp.ainfo, wp.icov_ell, wp.icov_wav,
+                                           wp.w_ell, mask_pix=wp.mask, minfo_mask=wp.minfo,
+                                           spin=0, device=dev)
+    prec = preconditioners.PseudoInvPreconditionerWav(wp.ainfo, wp.icov_ell, wp.icov_wav,
+                                                      wp.w_ell, 0, mask_pix=wp.mask,
+                                                      minfo_mask=wp.minfo)
+    b = sol.b0.to(torch.complex128)
+    log(f'solver set-up done: {time.perf_counter() - t0:.1f}s')
+    for _ in range(args.warmup):
+        x, info = mixed.solve_mixed(sol.a_matrix, b, prec, tol=1e-6, delta=args.delta)
+        log(f'warmup: iterations {info.iterations}, replacements {info.replacements}, '
+            f'residual {info.residual}')
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        x, info = mixed.solve_mixed(sol.a_matrix, b, prec, tol=1e-6, delta=args.delta)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t1
+    r = b - sol.a_matrix(x)
+    res = float(np.sqrt(alm_utils.contract_almxblm(r, r) / alm_utils.contract_almxblm(b, b)))
+    out = {'metric': 'Wiener solves/sec, T wavelet noise, resid 1e-6 (FP64-verified)',
+           'value': args.steps / el, 'unit': 'solves/s', 'n_gpus': 1, 'steps': args.steps,
+           'warmup': args.warmup, 'ms_per_step': 1e3 * el / args.steps,
+           'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+           'dtype': 'f32+f64', 'data': 'synthetic',
+           'config': {'workload': f'T lmax {lmax}, GL {lmax + 1}x{2 * lmax + 1}, '
+                                  f'{wp.w_ell.shape[0]} needlet scales (lambda 2), galactic '
+                                  'mask + 200 holes, wavelet pinv',
+                      'iterations': info.iterations, 'replacements': info.replacements,
+                      'true_residual': res}}
+    print(json.dumps(out))
+
+
 def ragged_bench(args, dev):
     '''Constrained realisations: per band limit one batched mixed-precision
     solve of `batch` right-hand sides (data + S^-1/2 w + P^T N^-1/2 w', seeds
@@ -212,6 +253,11 @@ def main():
                          '(16 right-hand sides at each of lmax 255, 511, 1023, 2047) instead '
                          'of the headline workload; prints its own JSON line')
     ap.add_argument('--ragged-lmaxs', default='255,511,1023,2047')
+    ap.add_argument('--wavelet', action='store_true',
+                    help='time the temperature solve with the wavelet noise model (GL grid of '
+                         '--wavelet-lmax, 12 needlet scales, float32 maps, mixed precision, '
+                         'wavelet pseudo-inverse preconditioner); prints its own JSON line')
+    ap.add_argument('--wavelet-lmax', type=int, default=4095)
     ap.add_argument('--roofline-only', action='store_true',
                     help='only build the workload geometry and time the roofline kernel '
                          '(for counter runs), after one copy of known size')
@@ -225,6 +271,9 @@ def main():
 
     if args.ragged:
         ragged_bench(args, dev)
+        return
+    if args.wavelet:
+        wavelet_bench(args, dev)
         return
 
     if args.roofline_only:

@@ -65,19 +65,27 @@ def pixel_coords(minfo):
 
 
 def make_mask(minfo, seed, ndisk=50, rdisk_deg=0.5, band_deg=10.):
-    '''Observed = True. Galactic-like band plus point-source disks.'''
+    '''Observed = True. Galactic-like band plus point-source disks (each disk
+    is tested only on the rings within its radius, which holds every
+    pixel it can contain).'''
     rng = np.random.default_rng(seed)
-    theta, phi = pixel_coords(minfo)
+    theta = np.repeat(minfo.theta, minfo.nphi)
     lat = np.pi / 2 - theta
     obs = np.abs(lat) >= np.radians(band_deg)
     z = rng.uniform(-1, 1, ndisk)
     ph = rng.uniform(0, 2 * np.pi, ndisk)
     th = np.arccos(z)
-    cosr = np.cos(np.radians(rdisk_deg))
-    st, ct = np.sin(theta), np.cos(theta)
+    rad = np.radians(rdisk_deg)
+    cosr = np.cos(rad)
     for k in range(ndisk):
-        cosd = st * np.sin(th[k]) * np.cos(phi - ph[k]) + ct * np.cos(th[k])
-        obs &= cosd < cosr
+        rows = np.nonzero(np.abs(minfo.theta - th[k]) <= rad + 1e-9)[0]
+        for r in rows:
+            n = int(minfo.nphi[r])
+            sl = slice(int(minfo.offsets[r]), int(minfo.offsets[r]) + n)
+            phi = minfo.phi0[r] + 2 * np.pi * np.arange(n) / n
+            t = minfo.theta[r]
+            cosd = np.sin(t) * np.sin(th[k]) * np.cos(phi - ph[k]) + np.cos(t) * np.cos(th[k])
+            obs[sl] &= cosd < cosr
     return obs
 
 
@@ -213,3 +221,71 @@ def ragged(lmaxs=(255, 511, 1023, 2047), per=16, seed=0, device=None, synth=None
         prob = polarized(lmax, seed=seed + k, synth=synth, device=device)
         out.append((prob, batch_data(prob, per, seed=seed + k, synth=synth, device=device)))
     return out
+
+
+def needlet_kernels(lmax, nwav=12, lamb=2.0):
+    """Cosine-needlet kernels on a log_lamb(l) axis, partition of unity
+    sum_j w_j(l)^2 = 1: w_j = cos(pi/2 (t - j)) for |t - j| < 1 with
+    t = log_lamb(l); the first kernel is 1 below its peak, the last 1 above."""
+    ells = np.arange(lmax + 1, dtype=np.float64)
+    t = np.full(lmax + 1, -1e300)
+    t[1:] = np.log(ells[1:]) / np.log(lamb)
+    w = np.zeros((nwav, lmax + 1))
+    for j in range(nwav):
+        core = np.where(np.abs(t - j) < 1, np.cos(np.pi / 2 * np.clip(t - j, -1, 1)), 0.0)
+        if j == 0:
+            core = np.where(t <= 0, 1.0, core)
+        if j == nwav - 1:
+            core = np.where(t >= j, 1.0, core)
+        w[j] = core
+    w[np.abs(w) < 1e-14] = 0
+    return w
+
+
+class WaveletProblem:
+    """Arrays of the temperature Wiener problem with a wavelet-domain noise
+    model (host numpy, data alm on the device)."""
+
+    def __init__(self, **kw):
+        self.__dict__.update(kw)
+
+
+def wavelet_temperature(lmax=4095, nwav=12, lamb=2.0, seed=0, sigma=10., ndisk=200,
+                        rdisk_deg=0.1, map_dtype=np.float32, device=None):
+    """T-only problem: per-scale inverse noise maps icov_j = depth_j W_j /
+    sigma^2 on GL(lmax_j + 1, 2 lmax_j + 1), depth log-normal per pixel
+    (sigma 0.3) and 3x the noise variance on the 30 % of the sky with
+    cos(theta) > 0.4; |lat| < 10 deg plus `ndisk` holes masked; data =
+    mask (signal + noise) in harmonic space."""
+    from optweight_amd import wavtrans, operators, alm_utils as au
+    if device is None:
+        device = torch.device('cuda', torch.cuda.current_device())
+    ss = np.random.SeedSequence([4, seed])
+    s_alm, s_noise, s_mask, s_depth = [np.random.default_rng(x) for x in ss.spawn(4)]
+    minfo = gl_minfo(lmax)
+    ainfo = au.AlmInfo(lmax)
+    w_ell = needlet_kernels(lmax, nwav, lamb)
+    icov_wav = wavtrans.Wav(2, preshape=(1,), dtype=map_dtype)
+    cov_wav = wavtrans.Wav(2, preshape=(1,), dtype=map_dtype)
+    for j in range(nwav):
+        lj = wavtrans.wav_lmax(w_ell, j)
+        mi = gl_minfo(lj)
+        w = np.repeat(mi.weight, mi.nphi)
+        depth = np.exp(0.3 * s_depth.standard_normal(mi.npix))
+        th = np.repeat(mi.theta, mi.nphi)
+        depth[np.cos(th) > 0.4] /= 3.0
+        icov = (depth * w / sigma ** 2)[None, :].astype(map_dtype)
+        icov_wav.add((j, j), icov, mi)
+        cov_wav.add((j, j), (1.0 / icov.astype(np.float64)).astype(map_dtype), mi)
+    mask = make_mask(minfo, s_mask, ndisk=ndisk, rdisk_deg=rdisk_deg).astype(np.float64)[None]
+    cov = cov_ell_T(lmax)[None, :]
+    icov_ell = np.zeros_like(cov)
+    icov_ell[0, 2:] = 1 / cov[0, 2:]
+    cdt = np.complex64 if map_dtype == np.float32 else np.complex128
+    sig = torch.as_tensor(au.rand_alm(cov, ainfo, s_alm, dtype=cdt), device=device)
+    noise = au.rand_alm_wav(cov_wav, ainfo, w_ell, 0, s_noise, adjoint=False,
+                            device=device).to(sig.dtype)
+    mask_op = operators.PixMatVecAlm(ainfo, mask, minfo, 0, use_weights=True)
+    alm_data = mask_op(sig + noise)
+    return WaveletProblem(lmax=lmax, minfo=minfo, ainfo=ainfo, w_ell=w_ell, icov_wav=icov_wav,
+                          mask=mask, icov_ell=icov_ell, cov_ell=cov, alm_data=alm_data)
