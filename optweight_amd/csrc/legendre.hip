@@ -853,7 +853,8 @@ void launch_adjoint(owmi_plan_impl *p, int spin, const cplx_t<T> *G, cplx_t<T> *
   const long long np = p->npairs;
   if (spin == 0) {
     if (np <= 64) return launch_adj_nb<T, 0, 1>(p, G, alm, alm_ts, ntrans, st);
-    return launch_adj_nb<T, 0, 2>(p, G, alm, alm_ts, ntrans, st);
+    if (np <= 128) return launch_adj_nb<T, 0, 2>(p, G, alm, alm_ts, ntrans, st);
+    return launch_adj_nb<T, 0, 4>(p, G, alm, alm_ts, ntrans, st);
   }
   if (np <= 64) return launch_adj_nb<T, 2, 1>(p, G, alm, alm_ts, ntrans, st);
   launch_adj_nb<T, 2, 2>(p, G, alm, alm_ts, ntrans, st);
