@@ -28,6 +28,8 @@
 #include "sht_internal.hpp"
 
 #include <climits>
+#include <cstdlib>
+#include <type_traits>
 
 namespace owmi {
 
@@ -330,6 +332,207 @@ synth2_kernel(LegArgs g, const R *__restrict__ xs, const int *__restrict__ lfps,
       const cplx_t<T> AmS = {MPe[b].x - MPo[b].x, MPe[b].y - MPo[b].y};
       Gq[rs * stride + m] = {ApS.x + AmS.x, ApS.y + AmS.y};
       Gu[rs * stride + m] = {ApS.y - AmS.y, -(ApS.x - AmS.x)};
+    }
+  }
+}
+
+// ------------------------------------------------------- matrix-core synthesis
+//
+// float32 batches: the sums over l run on the matrix cores as
+// D[i][j] += sum_k A[i][k] B[k][j] (v_mfma_f32_32x32x2f32) with
+//   rows i    : 32 ring pairs (each wave owns 64 pairs: two row blocks),
+//   depth k   : two consecutive multipoles l0 + k,
+//   columns j : 32 staged alm components.
+// Operand layouts of this instruction: A[i][k] in lane i + 32k, B[k][j] in
+// lane j + 32k, D[i][j] in lane j + 32 ((i / 4) % 2), register
+// (i % 4) + 4 (i / 8).
+// Every lane runs the float64 recurrence of its own ring pair; one
+// v_permlane32_swap per function and multipole pair moves lam_{l0}, lam_{l0+1}
+// of 64 pairs into the A operands of both row blocks. The south ring uses
+// columns that carry s = (-1)^(l-m):
+//   spin 0 (16 transforms; column block cb holds transforms 8cb .. 8cb+7):
+//     j = 4t + c, c = (re, im, s re, s im) of c_l a_lm;
+//   spin 2 (8 transforms): block 0 multiplies 2lam with (P, s Mm), block 1
+//     multiplies (-2)lam with (Mm, s P) (P, Mm as in synth2_kernel), so one
+//     lane ends with the terms of one Q/U component of one ring.
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// Workgroups go to the 8 XCDs (each with its own L2) round-robin by linear
+// block index. Blocks of neighbouring m share the cache lines of the ring
+// coefficients G[row][m] (16 complex64 values of m per 128-byte line), so
+// the (x, m) of a block are chosen such that every XCD runs 16 consecutive
+// m in a row (all x of one m together), while each XCD still gets m from the
+// whole range (the work per m falls with m). m beyond the last full group of
+// 128 keep the plain order.
+__device__ __forceinline__ void xcd_block(int gx, int nm, int &x, int &m) {
+  const int lin = blockIdx.x + gx * blockIdx.y;
+  const int full = (nm / 128) * 128;
+  if (lin >= gx * full) {
+    x = blockIdx.x;
+    m = blockIdx.y;
+    return;
+  }
+  const int xcd = lin & 7, r = lin >> 3;
+  x = r % gx;
+  const int t = r / gx;
+  m = (t >> 4) * 128 + xcd * 16 + (t & 15);
+}
+
+constexpr int kMfCH = 64;  // multipoles staged per chunk (even)
+
+// lo: lanes 0-31 v, lanes 32-63 w of lanes 0-31; hi: lanes 0-31 v of lanes
+// 32-63, lanes 32-63 w.
+__device__ __forceinline__ void swap_halves(float v, float w, float &lo, float &hi) {
+  const auto r =
+      __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(w), false, false);
+  lo = __uint_as_float(r[0]);
+  hi = __uint_as_float(r[1]);
+}
+
+__device__ __forceinline__ f32x16 mfma32(float a, float b, f32x16 c) {
+  return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
+}
+
+// One recurrence step with the start-multipole injection.
+__device__ __forceinline__ R rec_step(R coefx, R &l1, R &l2, int l, int lf, R s0, R s1) {
+  const R lam = fma(coefx, l1, -l2);
+  const bool st = (l == lf);
+  l2 = st ? s0 : l1;
+  l1 = st ? s1 : lam;
+  return l1;
+}
+
+template <int SPIN>
+__global__ void __launch_bounds__(256)
+synth_mfma_kernel(LegArgs g, const R *__restrict__ xs, const int *__restrict__ lfas,
+                  const R *__restrict__ sas, const int *__restrict__ lfbs,
+                  const R *__restrict__ sbs, const R *__restrict__ coef,
+                  const float2 *__restrict__ alm, long long alm_ts, long long ntrans,
+                  float2 *G) {
+  constexpr int NT = SPIN == 0 ? 16 : 8;  // transforms per block
+  constexpr int NCF = SPIN == 0 ? 2 : 3;  // recurrence coefficients per multipole
+  __shared__ R sA[kMfCH];
+  __shared__ R sC[kMfCH];
+  __shared__ __align__(16) float sB[2][kMfCH][32];
+  __shared__ int smin;
+  const int lane = threadIdx.x & 63;
+  int bx, m;
+  xcd_block(gridDim.x, gridDim.y, bx, m);
+  const long long ip = (long long)bx * blockDim.x + threadIdx.x;
+  const long long t0 = (long long)blockIdx.z * NT;
+  const int nt = (int)min((long long)NT, ntrans - t0);
+  const bool active = ip < g.npairs;
+  const int lmax = g.lmax;
+  const long long nelem = nelem_of(lmax);
+  const long long ms = mstart_of(m, lmax);
+  const long long si = (long long)m * g.npairs + ip;
+  const int lfa = active ? lfas[si] : INT_MAX;
+  const int lfb = (SPIN == 2 && active) ? lfbs[si] : INT_MAX;
+  const int lmin = block_min(min(lfa, lfb), &smin);
+  f32x16 acc00 = {}, acc01 = {}, acc10 = {}, acc11 = {};  // [block][row block]
+  if (lmin <= lmax) {
+    const R x = active ? xs[ip] : (R)0;
+    const R a0s = active ? sas[2 * si] : (R)0, a1s = active ? sas[2 * si + 1] : (R)0;
+    R b0s = 0, b1s = 0;
+    if (SPIN == 2 && active) {
+      b0s = sbs[2 * si];
+      b1s = sbs[2 * si + 1];
+    }
+    const float2 *a = alm + t0 * alm_ts + ms;
+    const R *cf = coef + NCF * ms;
+    const int jc = lane & 31, kh = lane >> 5;
+    R a1 = 0, a2 = 0, b1 = 0, b2 = 0;
+    for (int l0 = lmin; l0 <= lmax; l0 += kMfCH) {
+      __syncthreads();
+      // staging: (multipole, transform) items, transform fastest
+      for (int j = threadIdx.x; j < kMfCH * NT; j += blockDim.x) {
+        const int b = j % NT, jl = j / NT;
+        const int l = l0 + jl;
+        const bool in = l <= lmax;
+        if (b == 0) {
+          sA[jl] = in ? cf[NCF * l] : (R)0;
+          if (SPIN == 2) sC[jl] = in ? cf[NCF * l + 1] : (R)0;
+        }
+        const float s = ((l - m) & 1) ? -1.0f : 1.0f;
+        float4 v0 = {0, 0, 0, 0}, v1 = {0, 0, 0, 0};
+        if (SPIN == 0) {
+          if (in && b < nt) {
+            const float c = (float)cf[2 * l + 1];
+            const float2 av = a[b * alm_ts + l];
+            v0 = {c * av.x, c * av.y, s * c * av.x, s * c * av.y};
+          }
+          *reinterpret_cast<float4 *>(&sB[b / 8][jl][4 * (b % 8)]) = v0;
+        } else {
+          if (in && b < nt) {
+            const float c = (float)(-0.5 * cf[3 * l + 2]);
+            const float2 e = a[b * alm_ts + l], bb = a[b * alm_ts + nelem + l];
+            const float2 P = {c * (e.x - bb.y), c * (e.y + bb.x)};
+            const float2 Mm = {c * (e.x + bb.y), c * (e.y - bb.x)};
+            v0 = {P.x, P.y, s * Mm.x, s * Mm.y};
+            v1 = {Mm.x, Mm.y, s * P.x, s * P.y};
+          }
+          *reinterpret_cast<float4 *>(&sB[0][jl][4 * b]) = v0;
+          *reinterpret_cast<float4 *>(&sB[1][jl][4 * b]) = v1;
+        }
+      }
+      __syncthreads();
+      const int nl2 = (min(kMfCH, lmax - l0 + 1) + 1) & ~1;
+      for (int i = 0; i < nl2; i += 2) {
+        const int l = l0 + i;
+        const float bv0 = sB[0][i + kh][jc], bv1 = sB[1][i + kh][jc];
+        float alo, ahi;
+        if (SPIN == 0) {
+          const R u0 = rec_step(sA[i] * x, a1, a2, l, lfa, a0s, a1s);
+          const R u1 = rec_step(sA[i + 1] * x, a1, a2, l + 1, lfa, a0s, a1s);
+          swap_halves((float)u0, (float)u1, alo, ahi);
+          acc00 = mfma32(alo, bv0, acc00);
+          acc01 = mfma32(ahi, bv0, acc01);
+          acc10 = mfma32(alo, bv1, acc10);
+          acc11 = mfma32(ahi, bv1, acc11);
+        } else {
+          const R ax0 = sA[i] * x, ax1 = sA[i + 1] * x;
+          const R p0 = rec_step(ax0 + sC[i], a1, a2, l, lfa, a0s, a1s);
+          const R q0 = rec_step(ax0 - sC[i], b1, b2, l, lfb, b0s, b1s);
+          const R p1 = rec_step(ax1 + sC[i + 1], a1, a2, l + 1, lfa, a0s, a1s);
+          const R q1 = rec_step(ax1 - sC[i + 1], b1, b2, l + 1, lfb, b0s, b1s);
+          float blo, bhi;
+          swap_halves((float)p0, (float)p1, alo, ahi);
+          swap_halves((float)q0, (float)q1, blo, bhi);
+          acc00 = mfma32(alo, bv0, acc00);
+          acc01 = mfma32(ahi, bv0, acc01);
+          acc10 = mfma32(blo, bv1, acc10);
+          acc11 = mfma32(bhi, bv1, acc11);
+        }
+      }
+    }
+  }
+  // D[i][j]: row i = 8 (r / 4) + 4 (lane / 32) + r % 4 of the row block
+  const int jc = lane & 31, c = jc & 3, tt = jc >> 2;
+  const long long wbase = (long long)bx * blockDim.x + (threadIdx.x & ~63);
+  const long long stride = g.mmax + 1;
+  float *Gf = reinterpret_cast<float *>(G);
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const long long pp = wbase + 32 * h + 8 * (r / 4) + 4 * (lane >> 5) + (r % 4);
+      if (pp >= g.npairs) continue;
+      const int ring = c < 2 ? g.pair_n[pp] : g.pair_s[pp];
+      if (ring < 0) continue;
+      const float v0 = h == 0 ? acc00[r] : acc01[r];
+      const float v1 = h == 0 ? acc10[r] : acc11[r];
+      if (SPIN == 0) {
+        const long long ta = t0 + tt, tb = t0 + 8 + tt;
+        if (tt < nt) Gf[2 * ((ta * g.nrings + ring) * stride + m) + (c & 1)] = v0;
+        if (8 + tt < nt) Gf[2 * ((tb * g.nrings + ring) * stride + m) + (c & 1)] = v1;
+      } else if (tt < nt) {
+        const long long t = t0 + tt;
+        const long long qi = (t * 2 * g.nrings + ring) * stride + m;
+        const long long ui = qi + g.nrings * stride;
+        Gf[2 * qi + (c & 1)] = v0 + v1;
+        Gf[2 * ui + 1 - (c & 1)] = (c == 0 || c == 3) ? v1 - v0 : v0 - v1;
+      }
     }
   }
 }
@@ -712,6 +915,319 @@ adj2_kernel(LegArgs g, AdjItems it, const R *__restrict__ xs, const int *__restr
   }
 }
 
+// ---------------------------------------------------- matrix-core adjoint
+//
+// float32 batches: the sums over ring pairs run on the matrix cores as
+// D[i][j] += sum_k A[i][k] B[k][j] (v_mfma_f32_16x16x4f32; A[i][k] in lane
+// i + 16k, B[k][j] in lane j + 16k, D[i][j] in lane j + 16 (i / 4),
+// register i % 4) with
+//   rows i    : 16 multipoles of one parity of l - m (two row tiles per
+//               chunk of 32 multipoles),
+//   depth k   : spin 0: 4 ring pairs; spin 2: 2 ring pairs x (2lam, s (-2)lam),
+//   columns j : 16 per tile; spin 0: (re, im) of 8 transforms with the
+//               weights N + S (even rows) or N - S (odd rows) of the north
+//               and south ring coefficients; spin 2: 4 transforms x
+//               (S+ re, S+ im, s S- re, s S- im), s = (-1)^(l-m).
+// Spin 2 folds s into the (-2)lam operand: S+ = sum 2lam Zp(N) + s (-2)lam Zp(S),
+// s S- = sum 2lam Zm(S) + s (-2)lam Zm(N), so its weights do not depend on the
+// row parity.
+// Each wave owns 64 ring pairs: its lanes run the float64 recurrences of
+// their pairs over a chunk and write the float32 values to a wave-private
+// LDS tile [multipole][pair]; the matrix products read the A operands from
+// that tile and keep the B operands (the weights, fixed per m) in registers.
+// The W waves of a block (64 W ring pairs) add their partial tiles through
+// LDS after every chunk; blocks of further ring pairs write partial sums
+// that sum_groups_kernel adds in fixed order.
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ f32x4 mfma16(float a, float b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+constexpr int kMaCH = 32;  // multipoles per adjoint chunk
+
+template <int SPIN, int W>
+__global__ void __launch_bounds__(64 * W)
+adj_mfma_kernel(LegArgs g, long long grp_ts, const R *__restrict__ xs,
+                const int *__restrict__ lfas, const R *__restrict__ sas,
+                const int *__restrict__ lfbs, const R *__restrict__ sbs,
+                const R *__restrict__ coef, const float2 *__restrict__ G, long long ntrans,
+                float2 *out, long long out_ts) {
+  constexpr int NT = SPIN == 0 ? 16 : 8;    // transforms per block
+  constexpr int NCF = SPIN == 0 ? 2 : 3;    // recurrence coefficients per multipole
+  constexpr int TS = SPIN == 0 ? 66 : 130;  // tile row stride in floats (bank-conflict free)
+  constexpr int TW = kMaCH * TS;            // tile floats per wave
+  static_assert(TW >= 1024, "the reduction buffer overlays the tile");
+  __shared__ __align__(16) float Tall[W * TW];
+  __shared__ R cA[3][kMaCH];      // recurrence coefficient A' per chunk buffer
+  __shared__ R cB[3][kMaCH];      // B' (spin 2)
+  __shared__ float cN[3][kMaCH];  // normalisation of the output
+  __shared__ int smin;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  float *T = Tall + wave * TW;
+  int grp, m;
+  xcd_block(gridDim.x, gridDim.y, grp, m);
+  const long long wbase = (long long)grp * 64 * W + 64 * wave;
+  const long long ip = wbase + lane;
+  const long long t0 = (long long)blockIdx.z * NT;
+  const int nt = (int)min((long long)NT, ntrans - t0);
+  const bool active = ip < g.npairs;
+  const int lmax = g.lmax;
+  const long long nelem = nelem_of(lmax);
+  const long long ms = mstart_of(m, lmax);
+  const long long stride = g.mmax + 1;
+  const long long si = (long long)m * g.npairs + ip;
+  const int lfa = active ? lfas[si] : INT_MAX;
+  const int lfb = (SPIN == 2 && active) ? lfbs[si] : INT_MAX;
+  const int lmin = block_min(min(lfa, lfb), &smin);
+  const R *cf = coef + NCF * ms;
+  float2 *om = out + grp * grp_ts + t0 * out_ts + ms;
+  // chunks start at multipoles with l - m even
+  const int lb0 = lmin <= lmax ? lmin - ((lmin - m) & 1) : lmax + 1;
+  for (int b = 0; b < nt; ++b)
+    for (int l = m + threadIdx.x; l < lb0; l += blockDim.x) {
+      om[b * out_ts + l] = {0, 0};
+      if (SPIN == 2) om[b * out_ts + nelem + l] = {0, 0};
+    }
+  if (lb0 > lmax) return;  // block-uniform
+
+  // B operands: weights of the ring pairs of this wave (lane j + 16k).
+  // The ring coefficients G[row][m] of one m sit one per cache line, so the
+  // wave first gathers them as whole complex values (one 8-byte load per
+  // (pair, ring, transform[, Q/U]) and lane) into its LDS tile, which the
+  // chunks use only later, and builds the operands from there.
+  const int bj = lane & 15, bk = lane >> 4;
+  float Bz[64];
+  {
+    const int rn = active ? g.pair_n[ip] : -1;
+    const int rs = active ? g.pair_s[ip] : -1;
+    float2 *S = reinterpret_cast<float2 *>(T);
+    if constexpr (SPIN == 0) {
+      // two halves of 8 transforms: S[(b * 2 + side) * 64 + pair]
+      static_assert(8 * 2 * 64 * 2 <= TW, "staging fits the tile");
+#pragma unroll
+      for (int ct = 0; ct < 2; ++ct) {
+#pragma unroll
+        for (int bb = 0; bb < 8; ++bb) {
+          const int b = 8 * ct + bb;
+          float2 vn = {0, 0}, vs = {0, 0};
+          if (rn >= 0 && b < nt) {
+            const long long t = t0 + b;
+            vn = G[(t * g.nrings + rn) * stride + m];
+            if (rs >= 0) vs = G[(t * g.nrings + rs) * stride + m];
+          }
+          S[(bb * 2 + 0) * 64 + lane] = vn;
+          S[(bb * 2 + 1) * 64 + lane] = vs;
+        }
+        wave_sync();
+        // Bz[(par * 16 + s) * 2 + ct]: pair 4s + k, transform 8ct + j / 2
+        const float *Sf = reinterpret_cast<const float *>(S);
+#pragma unroll
+        for (int sq = 0; sq < 16; ++sq) {
+          const int pl = 4 * sq + bk, bb = bj >> 1, comp = bj & 1;
+          const float gn = Sf[2 * ((bb * 2 + 0) * 64 + pl) + comp];
+          const float gs = Sf[2 * ((bb * 2 + 1) * 64 + pl) + comp];
+          Bz[(0 * 16 + sq) * 2 + ct] = gn + gs;
+          Bz[(1 * 16 + sq) * 2 + ct] = gn - gs;
+        }
+        wave_sync();
+      }
+    } else {
+      // S[((b * 2 + qu) * 2 + side) * 64 + pair], qu = 0: Q, 1: U
+      static_assert(8 * 2 * 2 * 64 * 2 <= TW, "staging fits the tile");
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+#pragma unroll
+        for (int qu = 0; qu < 2; ++qu) {
+          float2 vn = {0, 0}, vs = {0, 0};
+          if (rn >= 0 && b < nt) {
+            const long long row = (t0 + b) * 2 * g.nrings + qu * g.nrings;
+            vn = G[(row + rn) * stride + m];
+            if (rs >= 0) vs = G[(row + rs) * stride + m];
+          }
+          S[((b * 2 + qu) * 2 + 0) * 64 + lane] = vn;
+          S[((b * 2 + qu) * 2 + 1) * 64 + lane] = vs;
+        }
+      }
+      wave_sync();
+      // Bz[s * 2 + ct]: pair 2s + k / 2, function k % 2, transform 4ct + j / 4;
+      // f = 0: (Zp(N), Zm(S)), f = 1: (Zp(S), Zm(N)) for c < 2 / c >= 2, with
+      // Zp = q + i u = (q.x - u.y, q.y + u.x), Zm = q - i u = (q.x + u.y, q.y - u.x)
+      const int f = bk & 1, c = bj & 3, comp = c & 1;
+      const int side = (f == 0) == (c < 2) ? 0 : 1;
+      const bool minus = (c < 2) == (comp == 0);
+      const float *Sf = reinterpret_cast<const float *>(S);
+#pragma unroll
+      for (int sc = 0; sc < 32; ++sc) {
+        const int pl = 2 * sc + (bk >> 1);
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct) {
+          const int b = 4 * ct + (bj >> 2);
+          const float qv = Sf[2 * (((b * 2 + 0) * 2 + side) * 64 + pl) + comp];
+          const float uv = Sf[2 * (((b * 2 + 1) * 2 + side) * 64 + pl) + 1 - comp];
+          Bz[sc * 2 + ct] = minus ? qv - uv : qv + uv;
+        }
+      }
+      wave_sync();
+    }
+  }
+
+  const R x = active ? xs[ip] : (R)0;
+  const R a0s = active ? sas[2 * si] : (R)0, a1s = active ? sas[2 * si + 1] : (R)0;
+  R b0s = 0, b1s = 0;
+  if (SPIN == 2 && active) {
+    b0s = sbs[2 * si];
+    b1s = sbs[2 * si + 1];
+  }
+  R a1 = 0, a2 = 0, b1 = 0, b2 = 0;
+  // The coefficients of chunk k + 2 are loaded from global memory during
+  // chunk k and staged in the third LDS buffer, so no chunk waits on them.
+  const int nch = (lmax - lb0) / kMaCH + 1;
+  R ca = 0, cb = 0;
+  float cn = 0;
+  auto load_coef = [&](int k) {
+    const int l = lb0 + k * kMaCH + (int)threadIdx.x;
+    const bool in = l <= lmax;
+    ca = in ? cf[NCF * l] : (R)0;
+    cb = (SPIN == 2 && in) ? cf[NCF * l + 1] : (R)0;
+    cn = in ? (float)(SPIN == 0 ? cf[2 * l + 1] : 0.5 * cf[3 * l + 2]) : 0.0f;
+  };
+  auto store_coef = [&](int buf) {
+    cA[buf][threadIdx.x] = ca;
+    cB[buf][threadIdx.x] = cb;
+    cN[buf][threadIdx.x] = cn;
+  };
+  if (threadIdx.x < kMaCH) {
+    load_coef(0);
+    store_coef(0);
+    load_coef(1);
+    store_coef(1);
+  }
+  __syncthreads();
+  for (int k = 0; k < nch; ++k) {
+    const int lb = lb0 + k * kMaCH;
+    const int bcur = k % 3;
+    if (threadIdx.x < kMaCH) load_coef(k + 2);
+    // recurrences of the lane's ring pair over the chunk -> tile
+    const R *ka = cA[bcur];
+    const R *kb = cB[bcur];
+    const bool inj = __any((lfa >= lb && lfa < lb + kMaCH) || (lfb >= lb && lfb < lb + kMaCH));
+    if (SPIN == 0) {
+      if (inj) {
+        for (int i = 0; i < kMaCH; ++i)
+          T[i * TS + lane] = (float)rec_step(ka[i] * x, a1, a2, lb + i, lfa, a0s, a1s);
+      } else {
+#pragma unroll 8
+        for (int i = 0; i < kMaCH; ++i) {
+          const R lam = fma(ka[i] * x, a1, -a2);
+          a2 = a1;
+          a1 = lam;
+          T[i * TS + lane] = (float)lam;
+        }
+      }
+    } else {
+      float2 *T2 = reinterpret_cast<float2 *>(T);
+      if (inj) {
+        for (int i = 0; i < kMaCH; ++i) {
+          const R ax = ka[i] * x;
+          const R p = rec_step(ax + kb[i], a1, a2, lb + i, lfa, a0s, a1s);
+          const R q = rec_step(ax - kb[i], b1, b2, lb + i, lfb, b0s, b1s);
+          const float qs = (i & 1) ? -(float)q : (float)q;
+          T2[i * (TS / 2) + lane] = {(float)p, qs};
+        }
+      } else {
+#pragma unroll 8
+        for (int i = 0; i < kMaCH; ++i) {
+          const R ax = ka[i] * x;
+          const R p = fma(ax + kb[i], a1, -a2);
+          const R q = fma(ax - kb[i], b1, -b2);
+          a2 = a1;
+          a1 = p;
+          b2 = b1;
+          b1 = q;
+          const float qs = (i & 1) ? -(float)q : (float)q;
+          T2[i * (TS / 2) + lane] = {(float)p, qs};
+        }
+      }
+    }
+    wave_sync();
+    // products: D[par][ct] over the wave's ring pairs
+    f32x4 D00 = {0, 0, 0, 0}, D01 = {0, 0, 0, 0}, D10 = {0, 0, 0, 0}, D11 = {0, 0, 0, 0};
+    if (SPIN == 0) {
+#pragma unroll
+      for (int s = 0; s < 16; ++s) {
+        const float ae = T[(2 * bj) * TS + 4 * s + bk];
+        const float ao = T[(2 * bj + 1) * TS + 4 * s + bk];
+        D00 = mfma16(ae, Bz[(0 * 16 + s) * 2 + 0], D00);
+        D01 = mfma16(ae, Bz[(0 * 16 + s) * 2 + 1], D01);
+        D10 = mfma16(ao, Bz[(1 * 16 + s) * 2 + 0], D10);
+        D11 = mfma16(ao, Bz[(1 * 16 + s) * 2 + 1], D11);
+      }
+    } else {
+#pragma unroll
+      for (int s = 0; s < 32; ++s) {
+        const float ae = T[(2 * bj) * TS + 4 * s + bk];
+        const float ao = T[(2 * bj + 1) * TS + 4 * s + bk];
+        D00 = mfma16(ae, Bz[s * 2 + 0], D00);
+        D01 = mfma16(ae, Bz[s * 2 + 1], D01);
+        D10 = mfma16(ao, Bz[s * 2 + 0], D10);
+        D11 = mfma16(ao, Bz[s * 2 + 1], D11);
+      }
+    }
+    // partial tiles -> LDS (over this wave's tile): red[(par * 2 + ct) * 16 + j][i]
+    {
+      const int ib = 4 * bk;
+      *reinterpret_cast<f32x4 *>(&T[((0 * 2 + 0) * 16 + bj) * 16 + ib]) = D00;
+      *reinterpret_cast<f32x4 *>(&T[((0 * 2 + 1) * 16 + bj) * 16 + ib]) = D01;
+      *reinterpret_cast<f32x4 *>(&T[((1 * 2 + 0) * 16 + bj) * 16 + ib]) = D10;
+      *reinterpret_cast<f32x4 *>(&T[((1 * 2 + 1) * 16 + bj) * 16 + ib]) = D11;
+    }
+    if (threadIdx.x < kMaCH) store_coef((k + 2) % 3);
+    __syncthreads();
+    if (SPIN == 0) {
+      // items (par, ct, i, column pair)
+      for (int it = threadIdx.x; it < 512; it += 64 * W) {
+        const int pc = it >> 7, i = (it >> 3) & 15, jp = it & 7;
+        float re = 0, im = 0;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+          re += Tall[w * TW + (pc * 16 + 2 * jp) * 16 + i];
+          im += Tall[w * TW + (pc * 16 + 2 * jp + 1) * 16 + i];
+        }
+        const int row = (pc >> 1) + 2 * i;
+        const int l = lb + row, b = 8 * (pc & 1) + jp;
+        if (l <= lmax && b < nt) {
+          const float c = cN[bcur][row];
+          om[b * out_ts + l] = {c * re, c * im};
+        }
+      }
+    } else {
+      // items (par, ct, i, transform of the tile)
+      for (int it = threadIdx.x; it < 256; it += 64 * W) {
+        const int pc = it >> 6, i = (it >> 2) & 15, tq = it & 3;
+        float v[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int w = 0; w < W; ++w)
+#pragma unroll
+          for (int cc = 0; cc < 4; ++cc) v[cc] += Tall[w * TW + (pc * 16 + 4 * tq + cc) * 16 + i];
+        const int par = pc >> 1;
+        const int row = par + 2 * i;
+        const int l = lb + row, b = 4 * (pc & 1) + tq;
+        if (l <= lmax && b < nt) {
+          const float sg = par ? -1.0f : 1.0f;
+          const float2 sp = {v[0], v[1]}, sm = {sg * v[2], sg * v[3]};
+          const float c = cN[bcur][row];
+          // aE = -(S+ + S-)/2, aB = (i/2)(S+ - S-)
+          om[b * out_ts + l] = {-c * (sp.x + sm.x), -c * (sp.y + sm.y)};
+          om[b * out_ts + nelem + l] = {-c * (sp.y - sm.y), c * (sp.x - sm.x)};
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // out[t, c, :] = sum over groups of partial[grp, t, c, :], fixed order.
 template <typename T>
 __global__ void sum_groups_kernel(const cplx_t<T> *part, long long grp_ts, int ngroups,
@@ -777,11 +1293,38 @@ static void launch_synth_nb(owmi_plan_impl *p, int spin, const cplx_t<T> *alm, l
                        tb.sp, tb.lfm, tb.sm, tb.coef2, alm, alm_ts, ntrans, G);
 }
 
+// Matrix-core Legendre kernels for float32 batches of at least 8 transforms,
+// unless OWMI_LEGENDRE_VALU=1 selects the vector kernels (for comparisons).
+static bool use_mfma(long long ntrans) {
+  static const bool off = [] {
+    const char *e = std::getenv("OWMI_LEGENDRE_VALU");
+    return e != nullptr && e[0] == '1';
+  }();
+  return !off && ntrans >= 8;
+}
+
+static void launch_synth_mfma(owmi_plan_impl *p, int spin, const float2 *alm, long long alm_ts,
+                              float2 *G, long long ntrans, hipStream_t st) {
+  PlanTables<R> &tb = p->t64;
+  const int bs = (int)std::min<long long>(256, ((p->npairs + 63) / 64) * 64);
+  const int nt = spin == 0 ? 16 : 8;
+  dim3 grid(ceil_div(p->npairs, bs), (unsigned)(p->mmax + 1), ceil_div(ntrans, nt));
+  if (spin == 0)
+    hipLaunchKernelGGL(synth_mfma_kernel<0>, grid, dim3(bs), 0, st, leg_args(p), tb.x, tb.lf0,
+                       tb.s0, nullptr, nullptr, tb.coef0, alm, alm_ts, ntrans, G);
+  else
+    hipLaunchKernelGGL(synth_mfma_kernel<2>, grid, dim3(bs), 0, st, leg_args(p), tb.x, tb.lfp,
+                       tb.sp, tb.lfm, tb.sm, tb.coef2, alm, alm_ts, ntrans, G);
+}
+
 // Transforms per block: the widest of 4, 2, 1 not exceeding ntrans.
 template <typename T>
 void launch_synthesis(owmi_plan_impl *p, int spin, const cplx_t<T> *alm, long long alm_ts,
                       cplx_t<T> *G, long long ntrans, hipStream_t st) {
   OWMI_REQUIRE(ntrans >= 1 && ntrans < 65536, "ntrans must be in [1, 65535]");
+  if constexpr (std::is_same<T, float>::value) {
+    if (use_mfma(ntrans)) return launch_synth_mfma(p, spin, alm, alm_ts, G, ntrans, st);
+  }
   if (ntrans >= 4)
     launch_synth_nb<T, 4>(p, spin, alm, alm_ts, G, ntrans, st);
   else if (ntrans >= 2)
@@ -842,10 +1385,57 @@ static void launch_adj_nb(owmi_plan_impl *p, const cplx_t<T> *G, cplx_t<T> *alm,
   launch_adj_k<T, SPIN, K, 1>(p, G, alm, alm_ts, ntrans, st);
 }
 
+template <int SPIN, int W>
+static void launch_adj_mfma_w(owmi_plan_impl *p, const float2 *G, float2 *alm, long long alm_ts,
+                              long long ntrans, hipStream_t st) {
+  PlanTables<R> &tb = p->t64;
+  const int ngroups = (int)ceil_div(p->npairs, 64LL * W);
+  const long long nelem = nelem_of(p->lmax);
+  const int nc = SPIN == 0 ? 1 : 2;
+  float2 *dst = alm;
+  long long dst_ts = alm_ts, grp_ts = 0;
+  if (ngroups > 1) {
+    // partial sums [group][transform][comp][nelem] in the plan's second workspace
+    dst_ts = nc * nelem;
+    grp_ts = ntrans * dst_ts;
+    dst = static_cast<float2 *>(p->ensure_work2((size_t)ngroups * grp_ts * sizeof(float2)));
+  }
+  const int nt = SPIN == 0 ? 16 : 8;
+  dim3 grid(ngroups, (unsigned)(p->mmax + 1), ceil_div(ntrans, nt));
+  if constexpr (SPIN == 0)
+    hipLaunchKernelGGL((adj_mfma_kernel<0, W>), grid, dim3(64 * W), 0, st, leg_args(p), grp_ts,
+                       tb.x, tb.lf0, tb.s0, nullptr, nullptr, tb.coef0, G, ntrans, dst, dst_ts);
+  else
+    hipLaunchKernelGGL((adj_mfma_kernel<2, W>), grid, dim3(64 * W), 0, st, leg_args(p), grp_ts,
+                       tb.x, tb.lfp, tb.sp, tb.lfm, tb.sm, tb.coef2, G, ntrans, dst, dst_ts);
+  if (ngroups > 1) {
+    dim3 g2(ceil_div(nc * nelem, 256), (unsigned)ntrans);
+    hipLaunchKernelGGL(sum_groups_kernel<float>, g2, dim3(256), 0, st, dst, grp_ts, ngroups,
+                       alm, alm_ts, dst_ts, nelem, (long long)nc);
+  }
+}
+
+// Waves per block: enough for every ring pair of the plan, at most 8.
+template <int SPIN>
+static void launch_adj_mfma(owmi_plan_impl *p, const float2 *G, float2 *alm, long long alm_ts,
+                            long long ntrans, hipStream_t st) {
+  const long long np = p->npairs;
+  if (np <= 64) return launch_adj_mfma_w<SPIN, 1>(p, G, alm, alm_ts, ntrans, st);
+  if (np <= 128) return launch_adj_mfma_w<SPIN, 2>(p, G, alm, alm_ts, ntrans, st);
+  if (np <= 256) return launch_adj_mfma_w<SPIN, 4>(p, G, alm, alm_ts, ntrans, st);
+  launch_adj_mfma_w<SPIN, 8>(p, G, alm, alm_ts, ntrans, st);
+}
+
 template <typename T>
 void launch_adjoint(owmi_plan_impl *p, int spin, const cplx_t<T> *G, cplx_t<T> *alm,
                     long long alm_ts, long long ntrans, hipStream_t st) {
   OWMI_REQUIRE(ntrans >= 1 && ntrans < 65536, "ntrans must be in [1, 65535]");
+  if constexpr (std::is_same<T, float>::value) {
+    if (use_mfma(ntrans)) {
+      if (spin == 0) return launch_adj_mfma<0>(p, G, alm, alm_ts, ntrans, st);
+      return launch_adj_mfma<2>(p, G, alm, alm_ts, ntrans, st);
+    }
+  }
   // Ring pairs per lane: 2, or 1 on geometries of at most 64 ring pairs
   // (coarse or ring-subset plans), where two would leave most slots empty.
   // More pairs per lane amortise the cross-lane reduction further but cost

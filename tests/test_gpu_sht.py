@@ -213,6 +213,43 @@ def test_batched_legendre_tails(dev, ntrans, dtype):
         assert rel(out[ntrans - 1].cpu().numpy(), ref) < 1e-12
 
 
+@pytest.mark.parametrize('lmax,nrings', [(70, 71), (200, 201), (60, 401), (60, 1101)])
+@pytest.mark.parametrize('ntrans', [8, 13, 16, 24])
+def test_matrix_core_legendre(dev, lmax, nrings, ntrans):
+    '''float32 batches of at least 8 transforms take the matrix-core Legendre
+    kernels (16 spin-0 or 8 spin-2 transforms per block, partial last
+    blocks at 13 and 24): synthesis and adjoint of every transform against
+    the float64 transforms of the same input (relative 1e-5), and the last
+    transform's synthesis against the oracle (2e-5). The Gauss-Legendre
+    grids cover an odd ring count (an equator ring without a mirror), one
+    partial wave of ring pairs (36), two waves (101), four-wave adjoint
+    blocks (201 pairs) and two blocks of eight waves with partial sums over
+    ring-pair groups (551 pairs).'''
+    print(f'matrix-core legendre lmax={lmax} nrings={nrings} ntrans={ntrans}', flush=True)
+    nphi = 2 * lmax + 1
+    g = og.gauss_legendre(nrings, nphi)
+    m = map_utils.MapInfo.map_info_gauss_legendre(nrings, nphi)
+    ai = alm_utils.AlmInfo(lmax)
+    a64 = torch.as_tensor(np.stack([rand_alm(lmax, 3, 40 + s) for s in range(ntrans)]),
+                          device=dev)
+    a32 = a64.to(torch.complex64)
+    out = torch.zeros((ntrans, 3, m.npix), dtype=torch.float32, device=dev)
+    sht.alm2map(a32, out, ai, m, [0, 2])
+    ref = torch.zeros((ntrans, 3, m.npix), dtype=torch.float64, device=dev)
+    sht.alm2map(a32.to(torch.complex128), ref, ai, m, [0, 2])
+    back = torch.zeros_like(a32)
+    sht.map2alm(out, back, m, ai, [0, 2])
+    bref = torch.zeros_like(a64)
+    sht.map2alm(out.to(torch.float64), bref, m, ai, [0, 2])
+    for t in range(ntrans):
+        for c in range(3):
+            assert rel(out[t, c].cpu().numpy(), ref[t, c].cpu().numpy()) < 1e-5, (t, c)
+            assert rel(back[t, c].cpu().numpy(), bref[t, c].cpu().numpy()) < 1e-5, (t, c)
+    oref = np.zeros((3, g.npix))
+    osht.alm2map(a32[ntrans - 1].cpu().numpy().astype(np.complex128), oref, lmax, g, [0, 2])
+    assert rel(out[ntrans - 1].cpu().numpy(), oref) < 2e-5
+
+
 def test_long_rings_global_buffer(dev):
     '''Rings of 4201 pixels need a Bluestein length of 16384: the float64
     FFT buffer then lives in global memory and the kernel spectrum is
