@@ -341,20 +341,22 @@ synth2_kernel(LegArgs g, const R *__restrict__ xs, const int *__restrict__ lfps,
 // float32 batches: the sums over l run on the matrix cores as
 // D[i][j] += sum_k A[i][k] B[k][j] (v_mfma_f32_32x32x2f32) with
 //   rows i    : 32 ring pairs (each wave owns 64 pairs: two row blocks),
-//   depth k   : two consecutive multipoles l0 + k,
+//   depth k   : two multipoles (spin 0: l and l + 2 of one parity of l - m;
+//               spin 2: l and l + 1),
 //   columns j : 32 staged alm components.
 // Operand layouts of this instruction: A[i][k] in lane i + 32k, B[k][j] in
 // lane j + 32k, D[i][j] in lane j + 32 ((i / 4) % 2), register
 // (i % 4) + 4 (i / 8).
 // Every lane runs the float64 recurrence of its own ring pair; one
-// v_permlane32_swap per function and multipole pair moves lam_{l0}, lam_{l0+1}
-// of 64 pairs into the A operands of both row blocks. The south ring uses
-// columns that carry s = (-1)^(l-m):
-//   spin 0 (16 transforms; column block cb holds transforms 8cb .. 8cb+7):
-//     j = 4t + c, c = (re, im, s re, s im) of c_l a_lm;
-//   spin 2 (8 transforms): block 0 multiplies 2lam with (P, s Mm), block 1
-//     multiplies (-2)lam with (Mm, s P) (P, Mm as in synth2_kernel), so one
-//     lane ends with the terms of one Q/U component of one ring.
+// v_permlane32_swap per function and multipole pair moves the values of 64
+// pairs into the A operands of both row blocks.
+//   spin 0 (16 transforms): j = 2t + (re, im) of c_l a_lm; the even and odd
+//     sums over l - m accumulate separately, north = even + odd,
+//     south = even - odd;
+//   spin 2 (8 transforms): the south ring uses columns that carry
+//     s = (-1)^(l-m): block 0 multiplies 2lam with (P, s Mm), block 1
+//     multiplies (-2)lam with (Mm, s P) (P, Mm as in synth2_kernel), j = 4t + c,
+//     so one lane ends with the terms of one Q/U component of one ring.
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
@@ -403,18 +405,25 @@ __device__ __forceinline__ R rec_step(R coefx, R &l1, R &l2, int l, int lf, R s0
   return l1;
 }
 
-template <int SPIN>
-__global__ void __launch_bounds__(256)
+// PF: staging items per thread when the block has 256 threads, loaded one
+// chunk ahead into registers so that no chunk waits on global memory; 0:
+// stage each chunk directly (blocks of fewer threads).
+template <int SPIN, int PF>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 synth_mfma_kernel(LegArgs g, const R *__restrict__ xs, const int *__restrict__ lfas,
                   const R *__restrict__ sas, const int *__restrict__ lfbs,
                   const R *__restrict__ sbs, const R *__restrict__ coef,
                   const float2 *__restrict__ alm, long long alm_ts, long long ntrans,
                   float2 *G) {
-  constexpr int NT = SPIN == 0 ? 16 : 8;  // transforms per block
-  constexpr int NCF = SPIN == 0 ? 2 : 3;  // recurrence coefficients per multipole
-  __shared__ R sA[kMfCH];
-  __shared__ R sC[kMfCH];
-  __shared__ __align__(16) float sB[2][kMfCH][32];
+  constexpr int NT = 16;                    // transforms per block
+  constexpr int NCF = SPIN == 0 ? 2 : 3;    // recurrence coefficients per multipole
+  constexpr int STEP = SPIN == 0 ? 4 : 2;   // multipoles per group of products
+  constexpr int NCB = SPIN == 0 ? 1 : 4;    // column blocks of B
+  // recurrence coefficients of the current and the next chunk (the
+  // recurrence runs one group of multipoles ahead of the products)
+  __shared__ R sA[2][kMfCH];
+  __shared__ R sC[2][kMfCH];
+  __shared__ __align__(16) float sB[NCB][kMfCH][32];
   __shared__ int smin;
   const int lane = threadIdx.x & 63;
   int bx, m;
@@ -430,7 +439,12 @@ synth_mfma_kernel(LegArgs g, const R *__restrict__ xs, const int *__restrict__ l
   const int lfa = active ? lfas[si] : INT_MAX;
   const int lfb = (SPIN == 2 && active) ? lfbs[si] : INT_MAX;
   const int lmin = block_min(min(lfa, lfb), &smin);
-  f32x16 acc00 = {}, acc01 = {}, acc10 = {}, acc11 = {};  // [block][row block]
+  // acc[c][h]: row block h; spin 0: c = sums over l - lmin even / odd;
+  // spin 2: c = 2 set + function for the transform sets 0-7, 8-15
+  constexpr int NACC = SPIN == 0 ? 2 : 4;
+  f32x16 acc[NACC][2];
+#pragma unroll
+  for (int c = 0; c < NACC; ++c) acc[c][0] = acc[c][1] = f32x16{};
   if (lmin <= lmax) {
     const R x = active ? xs[ip] : (R)0;
     const R a0s = active ? sas[2 * si] : (R)0, a1s = active ? sas[2 * si + 1] : (R)0;
@@ -443,95 +457,245 @@ synth_mfma_kernel(LegArgs g, const R *__restrict__ xs, const int *__restrict__ l
     const R *cf = coef + NCF * ms;
     const int jc = lane & 31, kh = lane >> 5;
     R a1 = 0, a2 = 0, b1 = 0, b2 = 0;
-    for (int l0 = lmin; l0 <= lmax; l0 += kMfCH) {
-      __syncthreads();
-      // staging: (multipole, transform) items, transform fastest
-      for (int j = threadIdx.x; j < kMfCH * NT; j += blockDim.x) {
-        const int b = j % NT, jl = j / NT;
-        const int l = l0 + jl;
+    auto stage_coef = [&](int l0, int buf) {
+      for (int j = threadIdx.x; j < kMfCH; j += blockDim.x) {
+        const int l = l0 + j;
         const bool in = l <= lmax;
-        if (b == 0) {
-          sA[jl] = in ? cf[NCF * l] : (R)0;
-          if (SPIN == 2) sC[jl] = in ? cf[NCF * l + 1] : (R)0;
-        }
-        const float s = ((l - m) & 1) ? -1.0f : 1.0f;
-        float4 v0 = {0, 0, 0, 0}, v1 = {0, 0, 0, 0};
-        if (SPIN == 0) {
-          if (in && b < nt) {
-            const float c = (float)cf[2 * l + 1];
-            const float2 av = a[b * alm_ts + l];
-            v0 = {c * av.x, c * av.y, s * c * av.x, s * c * av.y};
-          }
-          *reinterpret_cast<float4 *>(&sB[b / 8][jl][4 * (b % 8)]) = v0;
-        } else {
-          if (in && b < nt) {
-            const float c = (float)(-0.5 * cf[3 * l + 2]);
-            const float2 e = a[b * alm_ts + l], bb = a[b * alm_ts + nelem + l];
-            const float2 P = {c * (e.x - bb.y), c * (e.y + bb.x)};
-            const float2 Mm = {c * (e.x + bb.y), c * (e.y - bb.x)};
-            v0 = {P.x, P.y, s * Mm.x, s * Mm.y};
-            v1 = {Mm.x, Mm.y, s * P.x, s * P.y};
-          }
-          *reinterpret_cast<float4 *>(&sB[0][jl][4 * b]) = v0;
-          *reinterpret_cast<float4 *>(&sB[1][jl][4 * b]) = v1;
-        }
+        sA[buf][j] = in ? cf[NCF * l] : (R)0;
+        if (SPIN == 2) sC[buf][j] = in ? cf[NCF * l + 1] : (R)0;
       }
-      __syncthreads();
-      const int nl2 = (min(kMfCH, lmax - l0 + 1) + 1) & ~1;
-      for (int i = 0; i < nl2; i += 2) {
-        const int l = l0 + i;
-        const float bv0 = sB[0][i + kh][jc], bv1 = sB[1][i + kh][jc];
-        float alo, ahi;
-        if (SPIN == 0) {
-          const R u0 = rec_step(sA[i] * x, a1, a2, l, lfa, a0s, a1s);
-          const R u1 = rec_step(sA[i + 1] * x, a1, a2, l + 1, lfa, a0s, a1s);
-          swap_halves((float)u0, (float)u1, alo, ahi);
-          acc00 = mfma32(alo, bv0, acc00);
-          acc01 = mfma32(ahi, bv0, acc01);
-          acc10 = mfma32(alo, bv1, acc10);
-          acc11 = mfma32(ahi, bv1, acc11);
+    };
+    // staging item j of a chunk: (multipole j / NT, transform j % NT); the
+    // raw values (alm, and for spin 2 the B alm) and the normalisation
+    struct Item {
+      float2 e, b;
+      float c;
+    };
+    auto load_item = [&](int l0, int j) {
+      const int b = j % NT, l = l0 + j / NT;
+      Item it = {{0, 0}, {0, 0}, 0.0f};
+      if (l <= lmax && b < nt) {
+        it.e = a[b * alm_ts + l];
+        if (SPIN == 2) it.b = a[b * alm_ts + nelem + l];
+        it.c = (float)(SPIN == 0 ? cf[2 * l + 1] : -0.5 * cf[3 * l + 2]);
+      }
+      return it;
+    };
+    auto store_item = [&](int l0, int j, const Item &it) {
+      const int b = j % NT, jl = j / NT;
+      const float c = it.c;
+      if (SPIN == 0) {
+        *reinterpret_cast<float2 *>(&sB[0][jl][2 * b]) = float2{c * it.e.x, c * it.e.y};
+      } else {
+        const float s = ((l0 + jl - m) & 1) ? -1.0f : 1.0f;
+        const float2 e = it.e, bb = it.b;
+        // P = -(e + i b)/2, Mm = -(e - i b)/2 (the -1/2 is in c)
+        const float2 P = {c * (e.x - bb.y), c * (e.y + bb.x)};
+        const float2 Mm = {c * (e.x + bb.y), c * (e.y - bb.x)};
+        const int set = b >> 3, bt = b & 7;
+        *reinterpret_cast<float4 *>(&sB[(2 * set) % NCB][jl][4 * bt]) =
+            float4{P.x, P.y, s * Mm.x, s * Mm.y};
+        *reinterpret_cast<float4 *>(&sB[(2 * set + 1) % NCB][jl][4 * bt]) =
+            float4{Mm.x, Mm.y, s * P.x, s * P.y};
+      }
+    };
+    auto stage_alm = [&](int l0) {
+      for (int j = threadIdx.x; j < kMfCH * NT; j += blockDim.x)
+        store_item(l0, j, load_item(l0, j));
+    };
+    // prefetch registers: items threadIdx.x + 256 k of the next chunk and
+    // the coefficients of the chunk after it
+    Item pre[PF > 0 ? PF : 1];
+    R pca = 0, pcc = 0;
+    auto prefetch = [&](int l0n) {
+#pragma unroll
+      for (int k = 0; k < PF; ++k) pre[k] = load_item(l0n, (int)threadIdx.x + 256 * k);
+      if (threadIdx.x < kMfCH) {
+        const int l = l0n + kMfCH + (int)threadIdx.x;
+        pca = l <= lmax ? cf[NCF * l] : (R)0;
+        pcc = (SPIN == 2 && l <= lmax) ? cf[NCF * l + 1] : (R)0;
+      }
+    };
+    auto commit = [&](int l0n, int cbuf) {
+#pragma unroll
+      for (int k = 0; k < PF; ++k) store_item(l0n, (int)threadIdx.x + 256 * k, pre[k]);
+      if (threadIdx.x < kMfCH) {
+        sA[cbuf][threadIdx.x] = pca;
+        if (SPIN == 2) sC[cbuf][threadIdx.x] = pcc;
+      }
+    };
+    // A operands of one group of STEP multipoles starting at l (coefficients
+    // from kA / kC): spin 0: (alo, ahi) = rows l, l + 2 and (blo, bhi) =
+    // rows l + 1, l + 3; spin 2: (alo, ahi) = 2lam and (blo, bhi) = (-2)lam
+    // at l, l + 1
+    // A operands of one group of STEP multipoles starting at l from its
+    // recurrence coefficients kc (spin 0: A' at l .. l + 3; spin 2: A' at l,
+    // l + 1, B' at l, l + 1). INJ: the group may hold a start multipole.
+    auto operands = [&](auto inj_tag, const R *kc, int l, float &alo, float &ahi, float &blo,
+                        float &bhi) {
+      constexpr bool INJ = decltype(inj_tag)::value;
+      auto step = [&](R cx, R &l1, R &l2, int ll, int lf, R s0, R s1) -> R {
+        if (INJ) return rec_step(cx, l1, l2, ll, lf, s0, s1);
+        const R lam = fma(cx, l1, -l2);
+        l2 = l1;
+        l1 = lam;
+        return lam;
+      };
+      if (SPIN == 0) {
+        const R u0 = step(kc[0] * x, a1, a2, l, lfa, a0s, a1s);
+        const R u1 = step(kc[1] * x, a1, a2, l + 1, lfa, a0s, a1s);
+        const R u2 = step(kc[2] * x, a1, a2, l + 2, lfa, a0s, a1s);
+        const R u3 = step(kc[3] * x, a1, a2, l + 3, lfa, a0s, a1s);
+        swap_halves((float)u0, (float)u2, alo, ahi);
+        swap_halves((float)u1, (float)u3, blo, bhi);
+      } else {
+        const R ax0 = kc[0] * x, ax1 = kc[1] * x;
+        const R p0 = step(ax0 + kc[2], a1, a2, l, lfa, a0s, a1s);
+        const R q0 = step(ax0 - kc[2], b1, b2, l, lfb, b0s, b1s);
+        const R p1 = step(ax1 + kc[3], a1, a2, l + 1, lfa, a0s, a1s);
+        const R q1 = step(ax1 - kc[3], b1, b2, l + 1, lfb, b0s, b1s);
+        swap_halves((float)p0, (float)p1, alo, ahi);
+        swap_halves((float)q0, (float)q1, blo, bhi);
+      }
+    };
+    // coefficients of the group at chunk row gi (rows from kMfCH on belong
+    // to the next chunk, staged one chunk ahead)
+    auto load_coefs = [&](int gi, int cb, R *kc) {
+      const bool wrap = gi >= kMfCH;
+      const int b = wrap ? cb ^ 1 : cb, r = wrap ? gi - kMfCH : gi;
+      if (SPIN == 0) {
+        kc[0] = sA[b][r];
+        kc[1] = sA[b][r + 1];
+        kc[2] = sA[b][r + 2];
+        kc[3] = sA[b][r + 3];
+      } else {
+        kc[0] = sA[b][r];
+        kc[1] = sA[b][r + 1];
+        kc[2] = sC[b][r];
+        kc[3] = sC[b][r + 1];
+      }
+    };
+    // B operands of the group at chunk row gi
+    auto load_bv = [&](int gi, float *bv) {
+      if (SPIN == 0) {
+        // depth k: multipoles l + 2k (acc[0]: l - l0 even) and l + 1 + 2k (acc[1])
+        bv[0] = sB[0][gi + 2 * kh][jc];
+        bv[1] = sB[0][gi + 1 + 2 * kh][jc];
+      } else {
+#pragma unroll
+        for (int c = 0; c < NACC; ++c) bv[c] = sB[c % NCB][gi + kh][jc];
+      }
+    };
+    stage_coef(lmin, 0);
+    stage_coef(lmin + kMfCH, 1);
+    stage_alm(lmin);
+    if (PF > 0) prefetch(lmin + kMfCH);
+    __syncthreads();
+    // software pipeline: the products of group g run with the recurrence of
+    // group g + 1, the coefficients of group g + 2 and the B operands of
+    // group g + 1 in flight from LDS
+    R kc[4];
+    float alo, ahi, blo, bhi;
+    load_coefs(0, 0, kc);
+    operands(std::true_type{}, kc, lmin, alo, ahi, blo, bhi);
+    load_coefs(STEP, 0, kc);
+    // start multipoles of this lane within [lo, lo + n)
+    auto starts_in = [&](int lo, int n) {
+      return active && ((lfa >= lo && lfa < lo + n) || (lfb >= lo && lfb < lo + n));
+    };
+    bool slow = __syncthreads_or(starts_in(lmin, kMfCH + STEP));
+    for (int ch = 0, l0 = lmin; l0 <= lmax; ++ch, l0 += kMfCH) {
+      const int cb = ch & 1;
+      if (ch > 0) {
+        __syncthreads();
+        if (PF > 0) {
+          commit(l0, cb ^ 1);
+          if (l0 + kMfCH <= lmax) prefetch(l0 + kMfCH);
         } else {
-          const R ax0 = sA[i] * x, ax1 = sA[i + 1] * x;
-          const R p0 = rec_step(ax0 + sC[i], a1, a2, l, lfa, a0s, a1s);
-          const R q0 = rec_step(ax0 - sC[i], b1, b2, l, lfb, b0s, b1s);
-          const R p1 = rec_step(ax1 + sC[i + 1], a1, a2, l + 1, lfa, a0s, a1s);
-          const R q1 = rec_step(ax1 - sC[i + 1], b1, b2, l + 1, lfb, b0s, b1s);
-          float blo, bhi;
-          swap_halves((float)p0, (float)p1, alo, ahi);
-          swap_halves((float)q0, (float)q1, blo, bhi);
-          acc00 = mfma32(alo, bv0, acc00);
-          acc01 = mfma32(ahi, bv0, acc01);
-          acc10 = mfma32(blo, bv1, acc10);
-          acc11 = mfma32(bhi, bv1, acc11);
+          stage_coef(l0 + kMfCH, cb ^ 1);
+          stage_alm(l0);
         }
+        slow = __syncthreads_or(starts_in(l0, kMfCH + STEP));
+      }
+      const int nls = (min(kMfCH, lmax - l0 + 1) + STEP - 1) & ~(STEP - 1);
+      float bv[NACC];
+      load_bv(0, bv);
+      for (int i = 0; i < nls; i += STEP) {
+        float nalo, nahi, nblo, nbhi;
+        if (slow)
+          operands(std::true_type{}, kc, l0 + i + STEP, nalo, nahi, nblo, nbhi);
+        else
+          operands(std::false_type{}, kc, l0 + i + STEP, nalo, nahi, nblo, nbhi);
+        load_coefs(i + 2 * STEP, cb, kc);
+        float nbv[NACC];
+        load_bv(min(i + STEP, kMfCH - STEP), nbv);
+#pragma unroll
+        for (int c = 0; c < NACC; c += 2) {
+          acc[c][0] = mfma32(alo, bv[c], acc[c][0]);
+          acc[c][1] = mfma32(ahi, bv[c], acc[c][1]);
+          acc[c + 1][0] = mfma32(blo, bv[c + 1], acc[c + 1][0]);
+          acc[c + 1][1] = mfma32(bhi, bv[c + 1], acc[c + 1][1]);
+        }
+        alo = nalo;
+        ahi = nahi;
+        blo = nblo;
+        bhi = nbhi;
+#pragma unroll
+        for (int c = 0; c < NACC; ++c) bv[c] = nbv[c];
       }
     }
   }
   // D[i][j]: row i = 8 (r / 4) + 4 (lane / 32) + r % 4 of the row block
   const int jc = lane & 31, c = jc & 3, tt = jc >> 2;
+  // spin 0: sums over l - lmin even / odd; the south ring takes their
+  // difference with the sign of (-1)^(lmin - m)
+  const float sgn0 = ((lmin - m) & 1) ? -1.0f : 1.0f;
   const long long wbase = (long long)bx * blockDim.x + (threadIdx.x & ~63);
   const long long stride = g.mmax + 1;
-  float *Gf = reinterpret_cast<float *>(G);
+  const bool odd = jc & 1;
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
       const long long pp = wbase + 32 * h + 8 * (r / 4) + 4 * (lane >> 5) + (r % 4);
-      if (pp >= g.npairs) continue;
-      const int ring = c < 2 ? g.pair_n[pp] : g.pair_s[pp];
-      if (ring < 0) continue;
-      const float v0 = h == 0 ? acc00[r] : acc01[r];
-      const float v1 = h == 0 ? acc10[r] : acc11[r];
+      // lanes 2n and 2n + 1 hold the two components of one complex value;
+      // after the exchange each lane stores one whole value
       if (SPIN == 0) {
-        const long long ta = t0 + tt, tb = t0 + 8 + tt;
-        if (tt < nt) Gf[2 * ((ta * g.nrings + ring) * stride + m) + (c & 1)] = v0;
-        if (8 + tt < nt) Gf[2 * ((tb * g.nrings + ring) * stride + m) + (c & 1)] = v1;
-      } else if (tt < nt) {
-        const long long t = t0 + tt;
-        const long long qi = (t * 2 * g.nrings + ring) * stride + m;
-        const long long ui = qi + g.nrings * stride;
-        Gf[2 * qi + (c & 1)] = v0 + v1;
-        Gf[2 * ui + 1 - (c & 1)] = (c == 0 || c == 3) ? v1 - v0 : v0 - v1;
+        // column jc: transform jc / 2, component jc % 2; the even lane stores
+        // the north value, the odd lane the south value
+        const float v0 = acc[0][h][r], v1 = acc[1][h][r];
+        const float vn = v0 + v1, vs = sgn0 * (v0 - v1);
+        const float send = odd ? vn : vs;
+        const float recv = __shfl_xor(send, 1);
+        const int b = jc >> 1;
+        if (pp >= g.npairs || b >= nt) continue;
+        const long long t = t0 + b;
+        const int ring = odd ? g.pair_s[pp] : g.pair_n[pp];
+        if (ring < 0) continue;
+        const float2 val = odd ? float2{recv, vs} : float2{vn, recv};
+        G[(t * g.nrings + ring) * stride + m] = val;
+      } else {
+        // column c = jc % 4 of transform tt of each set: q component c % 2
+        // and u component 1 - c % 2 of ring (c < 2: north, else south); the
+        // even lane stores Q, the odd lane U
+        const int ring = c < 2 ? (pp < g.npairs ? g.pair_n[pp] : -1)
+                               : (pp < g.npairs ? g.pair_s[pp] : -1);
+#pragma unroll
+        for (int set = 0; set < 2; ++set) {
+          const float v0 = acc[2 * set][h][r], v1 = acc[2 * set + 1][h][r];
+          const float q = v0 + v1;
+          const float u = (c == 0 || c == 3) ? v1 - v0 : v0 - v1;
+          const float send = odd ? q : u;
+          const float recv = __shfl_xor(send, 1);
+          const int b = 8 * set + tt;
+          if (ring < 0 || b >= nt) continue;
+          const long long t = t0 + b;
+          const long long qi = (t * 2 * g.nrings + ring) * stride + m;
+          if (!odd)
+            G[qi] = float2{q, recv};                    // Q = (q of c, q of c + 1)
+          else
+            G[qi + g.nrings * stride] = float2{u, recv};  // U = (u of c + 1, u of c)
+        }
       }
     }
   }
@@ -1307,14 +1471,23 @@ static void launch_synth_mfma(owmi_plan_impl *p, int spin, const float2 *alm, lo
                               float2 *G, long long ntrans, hipStream_t st) {
   PlanTables<R> &tb = p->t64;
   const int bs = (int)std::min<long long>(256, ((p->npairs + 63) / 64) * 64);
-  const int nt = spin == 0 ? 16 : 8;
-  dim3 grid(ceil_div(p->npairs, bs), (unsigned)(p->mmax + 1), ceil_div(ntrans, nt));
-  if (spin == 0)
-    hipLaunchKernelGGL(synth_mfma_kernel<0>, grid, dim3(bs), 0, st, leg_args(p), tb.x, tb.lf0,
-                       tb.s0, nullptr, nullptr, tb.coef0, alm, alm_ts, ntrans, G);
-  else
-    hipLaunchKernelGGL(synth_mfma_kernel<2>, grid, dim3(bs), 0, st, leg_args(p), tb.x, tb.lfp,
-                       tb.sp, tb.lfm, tb.sm, tb.coef2, alm, alm_ts, ntrans, G);
+  dim3 grid(ceil_div(p->npairs, bs), (unsigned)(p->mmax + 1), ceil_div(ntrans, 16));
+  // 256-thread blocks prefetch 4 staging items per thread (64 x 16 items)
+  if (spin == 0) {
+    if (bs == 256)
+      hipLaunchKernelGGL((synth_mfma_kernel<0, 4>), grid, dim3(bs), 0, st, leg_args(p), tb.x,
+                         tb.lf0, tb.s0, nullptr, nullptr, tb.coef0, alm, alm_ts, ntrans, G);
+    else
+      hipLaunchKernelGGL((synth_mfma_kernel<0, 0>), grid, dim3(bs), 0, st, leg_args(p), tb.x,
+                         tb.lf0, tb.s0, nullptr, nullptr, tb.coef0, alm, alm_ts, ntrans, G);
+  } else {
+    if (bs == 256)
+      hipLaunchKernelGGL((synth_mfma_kernel<2, 4>), grid, dim3(bs), 0, st, leg_args(p), tb.x,
+                         tb.lfp, tb.sp, tb.lfm, tb.sm, tb.coef2, alm, alm_ts, ntrans, G);
+    else
+      hipLaunchKernelGGL((synth_mfma_kernel<2, 0>), grid, dim3(bs), 0, st, leg_args(p), tb.x,
+                         tb.lfp, tb.sp, tb.lfm, tb.sm, tb.coef2, alm, alm_ts, ntrans, G);
+  }
 }
 
 // Transforms per block: the widest of 4, 2, 1 not exceeding ntrans.
