@@ -4,6 +4,9 @@ alm_c_utils Cython module (cython/alm_c_utils.pyx): lmul (157-250),
 trunc_alm (84-137) and wlm2alm (4-62), running on the HIP kernels
 owmi_lmul / owmi_trunc_alm / owmi_wlm2alm.
 '''
+import collections
+import hashlib
+
 import numpy as np
 import torch
 
@@ -18,12 +21,31 @@ def _nelem(lmax):
     return (lmax * lmax + 3 * lmax) // 2 + 1
 
 
+# Device copies of host multipole weights, by content: callers pass freshly
+# computed numpy arrays on every call (filters, beams), and a host-to-device
+# copy per call would stall the stream (and cannot be recorded into a HIP
+# graph). Bounded LRU.
+_LMAT_CACHE = collections.OrderedDict()
+_LMAT_CACHE_MAX = 512
+
+
 def _lmat_device(lmat, npol, rdtype, device):
     '''lmat cast to the alm precision (alm_c_utils.pyx:224-238), on device.'''
-    if isinstance(lmat, torch.Tensor):
-        t = lmat.to(device=device, dtype=rdtype)
+    if isinstance(lmat, torch.Tensor) and lmat.device == torch.device(device):
+        t = lmat.to(dtype=rdtype)
     else:
-        t = torch.as_tensor(np.asarray(lmat), device=device).to(rdtype)
+        a = lmat.detach().cpu().numpy() if isinstance(lmat, torch.Tensor) else lmat
+        a = np.ascontiguousarray(a)
+        key = (str(device), rdtype, a.dtype.str, a.shape,
+               hashlib.blake2b(a.tobytes(), digest_size=16).digest())
+        t = _LMAT_CACHE.get(key)
+        if t is None:
+            t = torch.as_tensor(a, device=device).to(rdtype)
+            _LMAT_CACHE[key] = t
+            if len(_LMAT_CACHE) > _LMAT_CACHE_MAX:
+                _LMAT_CACHE.popitem(last=False)
+        else:
+            _LMAT_CACHE.move_to_end(key)
     if t.ndim == 1:
         t = t.expand(npol, t.shape[0])
     return t.contiguous()

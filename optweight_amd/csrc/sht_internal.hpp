@@ -86,6 +86,10 @@ struct owmi_plan_impl {
   void *work3 = nullptr;
   size_t work3_bytes = 0;
   void *ensure_work3(size_t bytes);
+  // Workspaces outgrown by a larger call stay allocated until the plan is
+  // destroyed: kernels recorded into a HIP graph by an earlier call keep
+  // their addresses.
+  std::vector<void *> retired;
 
   size_t nchirp = 0, nbhat = 0;  // complex element counts of the tables
 };

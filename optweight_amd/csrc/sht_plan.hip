@@ -100,7 +100,7 @@ int ilog2_ceil(long long n) {
 
 void *owmi_plan_impl::ensure_work(size_t bytes) {
   if (bytes <= work_bytes) return work;
-  if (work) OWMI_HIP(hipFree(work));
+  if (work) retired.push_back(work);
   work = nullptr;
   work_bytes = 0;
   OWMI_HIP(hipMalloc(&work, bytes));
@@ -111,7 +111,7 @@ void *owmi_plan_impl::ensure_work(size_t bytes) {
 
 void *owmi_plan_impl::ensure_work2(size_t bytes) {
   if (bytes <= work2_bytes) return work2;
-  if (work2) OWMI_HIP(hipFree(work2));
+  if (work2) retired.push_back(work2);
   work2 = nullptr;
   work2_bytes = 0;
   OWMI_HIP(hipMalloc(&work2, bytes));
@@ -121,7 +121,7 @@ void *owmi_plan_impl::ensure_work2(size_t bytes) {
 
 void *owmi_plan_impl::ensure_work3(size_t bytes) {
   if (bytes <= work3_bytes) return work3;
-  if (work3) OWMI_HIP(hipFree(work3));
+  if (work3) retired.push_back(work3);
   work3 = nullptr;
   work3_bytes = 0;
   OWMI_HIP(hipMalloc(&work3, bytes));
@@ -641,6 +641,7 @@ int owmi_sht_plan_destroy(owmi_sht_plan *plan) {
                   p.ring_bhat_off, p.fft_ring_a, p.fft_ring_b, p.work, p.work2, p.work3};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
+  for (void *q : p.retired) (void)hipFree(q);
   (void)hipSetDevice(dev);
   delete plan;
   return 0;

@@ -25,7 +25,22 @@ side has its own CG scalars (device vectors). A right-hand side that has
 converged is frozen (its residual and search direction are cleared, its
 coefficients become 0) while the others continue. An unbatched ``b``
 (ncomp, nelem) is a batch of one.
+
+HIP graphs (opt-in: graphs=True or OWMI_GRAPHS=1): one mixed iteration is
+two device phases separated by the host's look at the residual norms
+(q = A32 p through the x, r update and <r, r>; z = M32 r through the p
+update). After one eager iteration (which sizes every workspace and cache),
+each phase is recorded once into a HIP graph and replayed, so an iteration
+costs two graph launches instead of the operators' many kernel launches
+from Python. The replayed kernels are the eager ones on the same buffers
+(x, r, p and the CG scalars are updated in place), and the residual
+replacements between replays stay eager. On the TQU lmax 1023 batch of 16
+the launches already overlap the kernels and the recording costs more than
+it saves (49.4 against 50.7 solves/s), hence off by default.
 '''
+import os
+import warnings
+
 import numpy as np
 import torch
 
@@ -82,45 +97,101 @@ def _identity(v):
     return v.clone()
 
 
+class _Phases:
+    '''Device phases of a CG iteration: the first call of each runs eagerly,
+    later calls replay a HIP graph recorded from it (use_graph), else they
+    run eagerly every time.'''
+
+    def __init__(self, fns, use_graph):
+        self.fns = fns
+        self.use_graph = use_graph
+        self.graphs = [None] * len(fns)
+        self.calls = [0] * len(fns)
+
+    def run(self, k):
+        self.calls[k] += 1
+        if not self.use_graph or self.calls[k] == 1:
+            self.fns[k]()
+            return
+        if self.graphs[k] is None:
+            g = torch.cuda.CUDAGraph()
+            try:
+                with torch.cuda.graph(g):
+                    self.fns[k]()
+            except RuntimeError as err:
+                # an operator that cannot be recorded (a host copy or sync
+                # inside it): nothing ran during the recording, so the phase
+                # runs eagerly from here on
+                warnings.warn(f'HIP graph recording failed, running eagerly: {err}')
+                self.use_graph = False
+                self.fns[k]()
+                return
+            self.graphs[k] = g
+        self.graphs[k].replay()
+
+
+def _graphs_default(t):
+    return t.is_cuda and os.environ.get('OWMI_GRAPHS', '0') == '1'
+
+
 def solve_mixed(a_matrix, b, precond=None, tol=1e-6, delta=0.1, maxiter=1000,
-                low=torch.complex64, a_matrix_low=None, precond_low=None):
+                low=torch.complex64, a_matrix_low=None, precond_low=None, graphs=None):
     '''Solve A x = b (b complex128 device alm, (ncomp, nelem) or (nbatch,
     ncomp, nelem)) to float64 true relative residual tol per right-hand
     side. a_matrix (and precond) must accept complex128 and ``low`` input,
-    or the ``low`` versions are given separately. Returns (x complex128,
-    SolveInfo).'''
+    or the ``low`` versions are given separately. graphs: replay the
+    iteration phases from HIP graphs (default: OWMI_GRAPHS=1 in the
+    environment). Returns (x complex128, SolveInfo).'''
     b3, squeeze = _as_batch(b)
     precond = precond or _identity
     a_lo = a_matrix_low or a_matrix
     m_lo = precond_low or precond
+    if graphs is None:
+        graphs = _graphs_default(b3)
     dot = alm_utils.contract_almxblm_batch
     nb = b3.shape[0]
+    dev = b3.device
     x = torch.zeros_like(b3)
     bn2 = dot(b3, b3).cpu().numpy()
     target = tol * tol * bn2
     r = _convert(b3, low)
-    z = m_lo(r)
-    rz = dot(r, z)
-    p = z
+    p = m_lo(r)
+    rz = dot(r, p)
+    # CG scalars on the device, updated in place
+    pq = torch.empty((nb,), dtype=torch.float64, device=dev)
+    rr = torch.empty((nb,), dtype=torch.float64, device=dev)
+    rzn = torch.empty((nb,), dtype=torch.float64, device=dev)
+
+    def step_a():
+        # q = A32 p; x += alpha p; r -= alpha q; <r, r>
+        q = a_lo(p)
+        dot(p, q, out=pq)
+        _xr(x, r, p, q, rz, pq, nb)
+        dot(r, r, out=rr)
+
+    def step_m():
+        # z = M32 r; p = z + beta p
+        z = m_lo(r)
+        dot(r, z, out=rzn)
+        _pupd(p, z, rzn, rz, nb)
+        rz.copy_(rzn)
+
+    phases = _Phases((step_a, step_m), graphs)
     ref2 = bn2.copy()
-    rr = torch.empty((nb,), dtype=torch.float64, device=b3.device)
     active = np.ones(nb, dtype=bool)
     it_rhs = np.zeros(nb, dtype=np.int64)
     res = np.full(nb, np.nan)
     it = nrep = 0
     while it < maxiter and active.any():
-        q = a_lo(p)
-        pq = dot(p, q)
-        _xr(x, r, p, q, rz, pq, nb)
-        del q
+        phases.run(0)
         it += 1
         it_rhs[active] += 1
-        rr_h = dot(r, r, out=rr).cpu().numpy()
+        rr_h = rr.cpu().numpy()
         trig = active & ((rr_h <= delta * delta * ref2) | (rr_h <= target))
         if trig.any():
             # float64 true residual of the triggered right-hand sides only
             idx = np.nonzero(trig)[0]
-            it_t = torch.as_tensor(idx, device=b3.device)
+            it_t = torch.as_tensor(idx, device=dev)
             r64 = b3[it_t] - a_matrix(x[it_t].contiguous())
             rr64 = dot(r64, r64).cpu().numpy()
             nrep += 1
@@ -133,15 +204,13 @@ def solve_mixed(a_matrix, b, precond=None, tol=1e-6, delta=0.1, maxiter=1000,
             del r64
             ref2[idx] = rr64
             if conv.any():
-                done = torch.as_tensor(idx[conv], device=b3.device)
+                done = torch.as_tensor(idx[conv], device=dev)
                 r[done] = 0
                 p[done] = 0
-        z = m_lo(r)
-        rz_new = dot(r, z)
-        _pupd(p, z, rz_new, rz, nb)
-        del z
-        rz = rz_new
+        phases.run(1)
     info = SolveInfo(it, it_rhs, nrep, res)
+    # whether the iterations were replayed from recorded graphs
+    info.graphs = phases.use_graph and any(g is not None for g in phases.graphs)
     return (x[0] if squeeze else x), info
 
 

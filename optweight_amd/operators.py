@@ -125,8 +125,11 @@ class PixMatVecMap(MatVecMap):
         off = m * (1 - np.eye(npol))[:, :, None]
         if np.any(off):
             return None
-        d = np.ascontiguousarray(np.einsum('iip->ip', m))
-        return torch.as_tensor(d, device=device).to(dtype)
+        key = ('diag', str(device), dtype)
+        if key not in self._dev._cache:
+            d = np.ascontiguousarray(np.einsum('iip->ip', m))
+            self._dev._cache[key] = torch.as_tensor(d, device=device).to(dtype)
+        return self._dev._cache[key]
 
 
 class EllMatVecAlm(MatVecAlm):

@@ -167,3 +167,31 @@ def test_mixed_single_rhs(dev, prob):
     r = solver.b0 - solver.a_matrix(x)
     assert np.sqrt(alm_utils.contract_almxblm(r, r)
                    / alm_utils.contract_almxblm(solver.b0, solver.b0)) <= 1e-6
+
+
+@pytest.mark.parametrize('nb', [3, 8])
+def test_mixed_graph_replay_matches_eager(dev, prob, nb):
+    '''The HIP-graph replay of the mixed iteration phases (recorded after one
+    eager iteration, replayed every later iteration with host syncs and
+    eager float64 residual replacements in between) gives the eager solve:
+    same iteration and replacement counts, same final residuals and
+    solutions to relative 1e-12 (nb 8 runs the matrix-core Legendre
+    kernels inside the graphs).'''
+    print(f'graph replay nb={nb}', flush=True)
+    dat = workloads.batch_data(prob, nb, seed=3, synth=oracle_synth)
+    solver = add_precs(solvers.CGWienerMap.from_arrays(
+        dat, prob.minfo, prob.ainfo, prob.icov_ell, prob.icov_pix, device=dev), prob, dev)
+    xe, ie = mixed.solve_mixed(solver.a_matrix, solver.b0, solver.preconditioner, tol=1e-6,
+                               delta=1e-2, graphs=False)
+    xg, ig = mixed.solve_mixed(solver.a_matrix, solver.b0, solver.preconditioner, tol=1e-6,
+                               delta=1e-2, graphs=True)
+    assert ig.graphs and not ie.graphs
+    assert ig.iterations == ie.iterations and ig.replacements == ie.replacements
+    assert list(ig.iterations_rhs) == list(ie.iterations_rhs)
+    assert ig.iterations >= 3
+    np.testing.assert_allclose(ig.residual, ie.residual, rtol=1e-10)
+    assert rel(xg, xe) < 1e-12
+    # a second graph solve in the same process (fresh recording)
+    xg2, ig2 = mixed.solve_mixed(solver.a_matrix, solver.b0, solver.preconditioner, tol=1e-6,
+                                 delta=1e-2, graphs=True)
+    assert ig2.iterations == ie.iterations and rel(xg2, xe) < 1e-12
