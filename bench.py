@@ -104,9 +104,12 @@ def time_kernel(fn, reps=10):
 def legendre_roofline(prob, dev, batch, reps=10):
     """Dominant kernel of the timed (mixed, batched) solve: the float32
     spin-2 Legendre adjoint at the workload's shape (TQU lmax 1023 on the
-    1024-ring GL grid, ``batch`` right-hand sides), launched alone through
-    owmi_sht_legendre and timed with HIP events on its stream. Useful flops
-    per launch: N_lm * N_theta * 8 * batch."""
+    1024-ring GL grid, ``batch`` right-hand sides; from 8 right-hand sides
+    the matrix-core kernel adj_mfma_kernel, 8 waves per block over the 512
+    ring pairs), launched alone through owmi_sht_legendre and timed with HIP
+    events on its stream. Flops per launch: N_lm * N_theta * 8 * batch, the
+    products the matrix cores execute (no padding: (2lam, s(-2)lam) x
+    (S+, s S-) columns)."""
     plan = sht.get_plan(prob.minfo, prob.lmax, dev)
     nelem = prob.ainfo.nelem
     alm = torch.zeros((batch, 3, nelem), dtype=torch.complex64, device=dev)
@@ -122,15 +125,16 @@ def legendre_roofline(prob, dev, batch, reps=10):
     ms = time_kernel(run, reps=reps)
     flops = nelem * prob.minfo.nrow * 8 * batch
     achieved = flops / (ms * 1e-3) / 1e12
-    return {'kernel': 'adj2_kernel<float, 2, 4, 4>', 'ms': ms, 'flops': flops,
-            'achieved': achieved}
+    kernel = 'adj_mfma_kernel<2, 8>' if batch >= 8 else 'adj2_kernel<float, 2, 4, 4>'
+    return {'kernel': kernel, 'ms': ms, 'flops': flops, 'achieved': achieved}
 
 
-# HBM bytes per launch of the roofline kernel: FETCH_SIZE (x 2.0) +
-# WRITE_SIZE (x 1.0), factors from a 256 MiB copy in the same process,
-# measured with `rocprofv3 --pmc` on `python bench.py --roofline-only`
-# (profiles/adj2_f32_tqu1023_b16_roofline.md).
-ROOFLINE_TRAFFIC_BYTES = 2.495e9
+# HBM bytes per launch of the roofline kernel (adj_mfma_kernel<2, 8>, 16
+# right-hand sides): FETCH_SIZE (x 2.0) + WRITE_SIZE (x 1.0), factors from a
+# 256 MiB copy in the same process, measured with `rocprofv3 --pmc` on
+# `python bench.py --roofline-only`
+# (profiles/adj_mfma2_f32_tqu1023_b16_roofline.md): 0.337 GB + 0.136 GB.
+ROOFLINE_TRAFFIC_BYTES = 4.731e8
 
 
 def cpu_baseline_sample(dev, lmax_sample=255, reps=3):
@@ -363,7 +367,9 @@ def main():
         'roofline': {'bound': 'mfma', 'achieved': roof['achieved'],
                      'peak': PEAK_FP32_TFLOPS, 'unit': 'TFLOP/s',
                      'frac': roof['achieved'] / PEAK_FP32_TFLOPS,
-                     'traffic': ROOFLINE_TRAFFIC_BYTES,
+                     # counters were read at the default shape only
+                     'traffic': (ROOFLINE_TRAFFIC_BYTES
+                                 if (args.lmax, args.batch) == (1023, 16) else None),
                      'kernel': roof['kernel'], 'kernel_ms': roof['ms'],
                      'flops_per_launch': roof['flops']},
         'cpu_baseline': cpu,
