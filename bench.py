@@ -250,7 +250,7 @@ def main():
     ap.add_argument('--cpu-baseline', choices=['on', 'off'], default='on')
     ap.add_argument('--batch', type=int, default=16,
                     help='right-hand sides per mixed-precision batched solve')
-    ap.add_argument('--delta', type=float, default=0.01,
+    ap.add_argument('--delta', type=float, default=0.001,
                     help='residual reduction between float64 residual replacements')
     ap.add_argument('--ragged', action='store_true',
                     help='time the batched constrained realisations of uneven band limit '
