@@ -70,13 +70,17 @@ def run_solver(kind, solver, tol=1e-6, delta=0.1):
     return mixed.solve_mixed(solver.a_matrix, b, solver.preconditioner, tol=tol, delta=delta)
 
 
-def time_solves(kind, solver, steps, warmup, delta=0.1):
+def time_solves(kind, solver, steps, warmup, delta=0.1, count_flops=False):
+    """Warm-up, then ``steps`` timed solves. With count_flops, also the
+    useful Legendre flops the timed solves issued, by precision
+    (sht.LEGENDRE_FLOPS: N_lm * N_theta * (2 n0 + 8 n2) per direction)."""
     for w in range(warmup):
         x, info = run_solver(kind, solver, delta=delta)
         log(f'{kind} warmup solve {w}: iterations {info.iterations} '
             f'(per RHS {list(info.iterations_rhs)}), replacements {info.replacements}, '
             f'true residual {np.max(true_residual(solver, x)):.3e}')
     torch.cuda.synchronize()
+    sht.reset_legendre_flops()
     t0 = time.perf_counter()
     infos = []
     for _ in range(steps):
@@ -84,6 +88,9 @@ def time_solves(kind, solver, steps, warmup, delta=0.1):
         infos.append(info)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    flops = dict(sht.LEGENDRE_FLOPS)
+    if count_flops:
+        return x, elapsed, infos, flops
     return x, elapsed, infos
 
 
@@ -309,7 +316,8 @@ def main():
     log(f'fp64: {args.steps} solves in {el64:.3f}s ({fp64_sps:.3f}/s), {it64:.1f} it/solve, '
         f'true residual {res64:.3e}')
     # mixed precision, a batch of right-hand sides per step
-    xmx, elmx, infmx = time_solves('mixed', bsolver, args.steps, args.warmup, delta=args.delta)
+    xmx, elmx, infmx, flmx = time_solves('mixed', bsolver, args.steps, args.warmup,
+                                         delta=args.delta, count_flops=True)
     resmx = true_residual(bsolver, xmx)
     nsolve = args.steps * args.batch
     mixed_sps = nsolve / elmx
@@ -320,6 +328,9 @@ def main():
         f'{itmx:.1f} it/batch, {nrep:.1f} replacements, max true residual '
         f'{resmx.max():.3e}, RHS 0 rel L2 vs fp64 solution {diff:.3e}, '
         f'speedup {mixed_sps / fp64_sps:.2f}x')
+    log(f'mixed: {itmx * args.steps / elmx:.1f} batched iterations/s, useful Legendre work '
+        f'{flmx["float32"] / elmx / 1e12:.1f} TFLOP/s float32 + '
+        f'{flmx["float64"] / elmx / 1e12:.2f} TFLOP/s float64 over the solve time')
     elapsed = elmx
     roof = legendre_roofline(prob, dev, args.batch)
     log(f'roofline: {roof}')
@@ -363,7 +374,12 @@ def main():
                    'rel_l2_vs_fp64_solution': diff,
                    'fp64_solves_per_s': fp64_sps, 'fp64_iterations': it64,
                    'fp64_true_residual': res64,
-                   'speedup_vs_fp64': mixed_sps / fp64_sps},
+                   'speedup_vs_fp64': mixed_sps / fp64_sps,
+                   # batched iterations per second, and the useful Legendre
+                   # flops of the timed mixed solves over their wall time
+                   'iterations_per_s': itmx * args.steps / elmx,
+                   'legendre_tflops_per_s_fp32': flmx['float32'] / elmx / 1e12,
+                   'legendre_tflops_per_s_fp64': flmx['float64'] / elmx / 1e12},
         'roofline': {'bound': 'mfma', 'achieved': roof['achieved'],
                      'peak': PEAK_FP32_TFLOPS, 'unit': 'TFLOP/s',
                      'frac': roof['achieved'] / PEAK_FP32_TFLOPS,

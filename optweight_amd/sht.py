@@ -21,12 +21,31 @@ import torch
 from optweight_amd import _lib
 
 
+# Useful Legendre flops issued through this module, by dtype:
+# F = N_lm * N_theta * (2 n0 + 8 n2) per transform direction (N_theta: rings
+# of the plan or of its ring subset; n0 spin-0 components, n2 spin-2 pairs),
+# counted on the host when a transform is launched.
+LEGENDRE_FLOPS = {'float32': 0.0, 'float64': 0.0}
+
+
+def reset_legendre_flops():
+    for k in LEGENDRE_FLOPS:
+        LEGENDRE_FLOPS[k] = 0.0
+
+
+def _count(plan, spin, ncomp, ntrans, dtype, directions=1):
+    per = 2 * ncomp if spin == 0 else 4 * ncomp
+    key = 'float64' if dtype in (torch.float64, torch.complex128) else 'float32'
+    LEGENDRE_FLOPS[key] += float(_nelem(plan.lmax)) * plan.nrings * per * ntrans * directions
+
+
 class Plan:
     '''Owns one owmi_sht_plan (geometry + lmax on one device).'''
 
     def __init__(self, minfo, lmax, device, rings=None):
         self.lmax = lmax
         self.npix = minfo.npix
+        self.nrings = int(minfo.nrow if rings is None else np.asarray(rings).size)
         self.device = device
         self._lib = _lib.load()
         self.handle = None
@@ -153,6 +172,7 @@ def alm2map(alm, omap, ainfo, minfo, spin, adjoint=False, rings=None):
         _lib.check(plan._lib.owmi_sht_synthesis(
             plan.handle, code, s, _lib.ptr(a3[:, i1]), npol * a3.shape[-1], _lib.ptr(m3[:, i1]),
             npol * m3.shape[-1], m3.shape[0], int(bool(adjoint)), st))
+        _count(plan, s, i2 - i1, m3.shape[0], alm.dtype)
     return omap
 
 
@@ -169,6 +189,7 @@ def map2alm(imap, alm, minfo, ainfo, spin, adjoint=False, rings=None):
         _lib.check(plan._lib.owmi_sht_adjoint(
             plan.handle, code, s, _lib.ptr(m3[:, i1]), npol * m3.shape[-1], _lib.ptr(a3[:, i1]),
             npol * a3.shape[-1], m3.shape[0], int(not adjoint), st))
+        _count(plan, s, i2 - i1, m3.shape[0], alm.dtype)
     return alm
 
 
@@ -193,4 +214,5 @@ def pixel_sandwich(alm, oalm, ainfo, minfo, spin, pixdiag):
         _lib.check(plan._lib.owmi_sht_pixel_sandwich(
             plan.handle, code, s, _lib.ptr(a3[:, i1]), npol * a3.shape[-1], _lib.ptr(o3[:, i1]),
             npol * o3.shape[-1], _lib.ptr(p3[:, i1]), pix_ts, a3.shape[0], st))
+        _count(plan, s, i2 - i1, a3.shape[0], alm.dtype, directions=2)
     return oalm

@@ -268,3 +268,27 @@ def test_long_rings_global_buffer(dev):
     mp32 = torch.zeros((1, m.npix), dtype=torch.float32, device=dev)
     sht.alm2map(a.to(torch.complex64), mp32, ai, m, 0)
     assert rel(mp32.double().cpu().numpy(), mp.cpu().numpy()) < 1e-5
+
+
+def test_legendre_flop_counter(dev):
+    '''sht.LEGENDRE_FLOPS counts N_lm * N_theta * (2 n0 + 8 n2) per direction
+    and transform: a TQU batch of 2 through synthesis, adjoint and the
+    sandwich (two directions), and a spin-0 transform on a ring subset.'''
+    lmax = 20
+    g, m = geoms(lmax)
+    ai = alm_utils.AlmInfo(lmax)
+    a = torch.as_tensor(np.stack([rand_alm(lmax, 3, 1), rand_alm(lmax, 3, 2)]), device=dev)
+    mp = torch.zeros((2, 3, m.npix), dtype=torch.float64, device=dev)
+    sht.reset_legendre_flops()
+    sht.alm2map(a, mp, ai, m, [0, 2])
+    sht.map2alm(mp, torch.zeros_like(a), m, ai, [0, 2])
+    d = torch.ones((3, m.npix), dtype=torch.float64, device=dev)
+    sht.pixel_sandwich(a, torch.zeros_like(a), ai, m, [0, 2], d)
+    per = layout.nelem(lmax) * m.nrow * (2 * 1 + 8 * 1) * 2
+    assert sht.LEGENDRE_FLOPS['float64'] == 4 * per
+    assert sht.LEGENDRE_FLOPS['float32'] == 0
+    rings = np.array([0, 3, 7], dtype=np.int64)
+    sht.reset_legendre_flops()
+    sht.alm2map(a[0, :1].to(torch.complex64).contiguous(),
+                torch.zeros((1, m.npix), dtype=torch.float32, device=dev), ai, m, 0, rings=rings)
+    assert sht.LEGENDRE_FLOPS['float32'] == layout.nelem(lmax) * 3 * 2
