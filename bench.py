@@ -94,6 +94,17 @@ def time_solves(kind, solver, steps, warmup, delta=0.1, count_flops=False):
     return x, elapsed, infos
 
 
+def calibration_copy(dev):
+    """One float32 copy of 256 MiB (read once, written once), the first
+    dispatch of a counter run: FETCH_SIZE and WRITE_SIZE of the other
+    kernels are scaled by the bytes this copy moves over what the counters
+    read for it."""
+    src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+
+
 def time_kernel(fn, reps=10):
     '''Average duration (ms) of fn() measured with events on the current
     stream.'''
@@ -200,6 +211,81 @@ def wavelet_bench(args, dev):
     print(json.dumps(out))
 
 
+def streaming_bench(args, dev):
+    """HBM-bound kernels of the CG iteration at the headline shapes (TQU,
+    lmax 1023, GL 1024x2047, 16 right-hand sides in float32; one right-hand
+    side in float64), each launched alone and timed with HIP events on its
+    stream. Algorithmic bytes per launch: every array read and written once.
+    Prints one JSON line per kernel and one summary line."""
+    from optweight_amd import alm_c_utils, mixed
+    lmax, nb = args.lmax, args.batch
+    ainfo = alm_utils.AlmInfo(lmax)
+    nelem = ainfo.nelem
+    npix = (lmax + 1) * (2 * lmax + 1)
+    rng = np.random.default_rng(0)
+    lines = []
+
+    def record(name, fn, nbytes, reps=20):
+        ms = time_kernel(fn, reps=reps)
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        lines.append({'kernel': name, 'ms': ms, 'bytes': nbytes, 'GB/s': gbs,
+                      'frac_hbm': gbs / PEAK_HBM_GBS})
+        log(f'{name}: {ms * 1e3:.1f} us, {nbytes / 1e6:.1f} MB, {gbs:.0f} GB/s')
+
+    for cdt, rdt, n in ((torch.complex64, torch.float32, nb), (torch.complex128, torch.float64, 1)):
+        esz = 8 if cdt == torch.complex64 else 16
+        rsz = esz // 2
+        tag = 'f32 x%d' % n if rdt == torch.float32 else 'f64 x1'
+        alm = torch.randn((n, 3, nelem), dtype=cdt, device=dev)
+        out = torch.empty_like(alm)
+        lmat = np.abs(rng.normal(size=(3, 3, lmax + 1)))
+        lmat = torch.as_tensor(lmat + lmat.transpose(1, 0, 2), dtype=rdt, device=dev)
+        alm_c_utils.lmul(alm, lmat, ainfo, alm_out=out)
+        record(f'lmul 3x3 ({tag})',
+               lambda: _lib.call('owmi_lmul', _lib.dtype_code(alm), _lib.ptr(lmat), 0,
+                                 _lib.ptr(alm), _lib.ptr(out), lmax, 3, n,
+                                 _lib.stream_ptr(dev)),
+               2 * alm.numel() * esz)
+        ldiag = torch.as_tensor(np.abs(rng.normal(size=(3, lmax + 1))), dtype=rdt, device=dev)
+        record(f'lmul diagonal ({tag})',
+               lambda: _lib.call('owmi_lmul', _lib.dtype_code(alm), _lib.ptr(ldiag), 1,
+                                 _lib.ptr(alm), _lib.ptr(out), lmax, 3, n,
+                                 _lib.stream_ptr(dev)),
+               2 * alm.numel() * esz)
+        imap = torch.randn((n, 3, npix), dtype=rdt, device=dev)
+        d = torch.rand((3, npix), dtype=rdt, device=dev)
+        omap = torch.empty_like(imap)
+        record(f'pixel diagonal N^-1 ({tag})',
+               lambda: _lib.call('owmi_pix_matvec', _lib.dtype_code(imap), _lib.ptr(d), 0,
+                                 _lib.ptr(imap), _lib.ptr(omap), 1, d.numel(), n,
+                                 _lib.stream_ptr(dev)),
+               (2 * imap.numel() + d.numel()) * rsz)
+        b = torch.randn_like(alm)
+        dots = alm_utils.contract_almxblm_batch(alm, b)
+        record(f'dot <a, b> per RHS ({tag})',
+               lambda: _lib.call('owmi_alm_dot_batch', _lib.dtype_code(alm), _lib.ptr(alm),
+                                 _lib.ptr(b), lmax, 3, n, _lib.ptr(dots),
+                                 _lib.stream_ptr(dev)),
+               2 * alm.numel() * esz)
+    # the mixed CG updates: x (complex128) += alpha p, r -= alpha q (complex64)
+    x = torch.zeros((nb, 3, nelem), dtype=torch.complex128, device=dev)
+    r = torch.randn((nb, 3, nelem), dtype=torch.complex64, device=dev)
+    pv = torch.randn_like(r)
+    q = torch.randn_like(r)
+    rz = torch.rand((nb,), dtype=torch.float64, device=dev)
+    pq = torch.rand((nb,), dtype=torch.float64, device=dev) + 1
+    record(f'CG x, r update (x f64, r/p/q f32 x{nb})',
+           lambda: mixed._xr(x, r, pv, q, rz, pq, nb),
+           2 * x.numel() * 16 + (2 * r.numel() + pv.numel() + q.numel()) * 8)
+    rzn = torch.rand((nb,), dtype=torch.float64, device=dev)
+    record(f'CG p update (f32 x{nb})', lambda: mixed._pupd(pv, q, rzn, rz, nb),
+           3 * pv.numel() * 8)
+    record(f'convert f64 -> f32 (x{nb})', lambda: mixed._convert(x, torch.complex64),
+           x.numel() * (16 + 8))
+    for ln in lines:
+        print(json.dumps({'streaming': ln}))
+
+
 def ragged_bench(args, dev):
     '''Constrained realisations: per band limit one batched mixed-precision
     solve of `batch` right-hand sides (data + S^-1/2 w + P^T N^-1/2 w', seeds
@@ -269,6 +355,10 @@ def main():
                          '--wavelet-lmax, 12 needlet scales, float32 maps, mixed precision, '
                          'wavelet pseudo-inverse preconditioner); prints its own JSON line')
     ap.add_argument('--wavelet-lmax', type=int, default=4095)
+    ap.add_argument('--streaming', action='store_true',
+                    help='time the HBM-bound kernels of the CG iteration at the headline shapes '
+                         '(lmul, pixel diagonal, dots, CG updates, conversion); prints one '
+                         'JSON line per kernel')
     ap.add_argument('--roofline-only', action='store_true',
                     help='only build the workload geometry and time the roofline kernel '
                          '(for counter runs), after one copy of known size')
@@ -287,13 +377,12 @@ def main():
         wavelet_bench(args, dev)
         return
 
+    if args.streaming:
+        calibration_copy(dev)
+        streaming_bench(args, dev)
+        return
     if args.roofline_only:
-        # copy of known size first (256 MiB read + 256 MiB written): the
-        # calibration dispatch for FETCH_SIZE / WRITE_SIZE counters
-        src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
+        calibration_copy(dev)
         prob = workloads.polarized(args.lmax, seed=0, device=dev)
         roof = legendre_roofline(prob, dev, args.batch, reps=3)
         print(json.dumps({'roofline_only': roof}))

@@ -31,12 +31,18 @@ __device__ __forceinline__ void lm_of(long long idx, int lmax, int &l, int &m) {
 
 // ---------------------------------------------------------------- lmul
 
+// One thread per (multipole, transform): the flat index i gives (l, m)
+// once, and the thread applies the NC x NC matrix of its multipole to the
+// NC components of its transform. A grid over the flat index (blockIdx.x)
+// and the transform (blockIdx.y) keeps the resident waves on one narrow,
+// advancing window of each array; a grid over m with several transforms
+// per thread measured 1.4-1.8x slower at lmax 1023.
 template <typename T, int NC>
-__global__ void lmul_full_kernel(const T *__restrict__ lmat, const cplx_t<T> *in,
-                                 cplx_t<T> *out, int lmax, long long nelem,
-                                 long long ntrans) {
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  long long t = blockIdx.y;
+__global__ void __launch_bounds__(256)
+lmul_full_kernel(const T *__restrict__ lmat, const cplx_t<T> *in, cplx_t<T> *out, int lmax,
+                 long long nelem, long long ntrans) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long t = blockIdx.y;
   if (i >= nelem || t >= ntrans) return;
   int l, m;
   lm_of(i, lmax, l, m);
@@ -51,13 +57,19 @@ __global__ void lmul_full_kernel(const T *__restrict__ lmat, const cplx_t<T> *in
     T re = 0, im = 0;
 #pragma unroll
     for (int j = 0; j < NC; ++j) {
-      T a = lmat[(c * NC + j) * nell + l];
+      const T a = lmat[(c * NC + j) * nell + l];
       re += a * v[j].x;
       im += a * v[j].y;
     }
     dst[c * nelem] = {re, im};
   }
 }
+
+// Loads of kLoadB elements are issued before their stores. in and out may
+// be the same array (in place), so the compiler cannot move a load above an
+// earlier store by itself; every thread reads and writes only its own
+// elements, so the reordering leaves the result unchanged.
+constexpr int kLoadB = 4;
 
 template <typename T>
 __global__ void lmul_full_generic_kernel(const T *__restrict__ lmat, const cplx_t<T> *in,
@@ -86,19 +98,44 @@ __global__ void lmul_full_generic_kernel(const T *__restrict__ lmat, const cplx_
   }
 }
 
+// Diagonal lmat (nc, nell): one thread per (multipole, transform) applies
+// the nc diagonal entries of its l to the nc components of its transform.
 template <typename T>
-__global__ void lmul_diag_kernel(const T *__restrict__ lmat, const cplx_t<T> *in,
-                                 cplx_t<T> *out, int lmax, int nc, long long nelem,
-                                 long long nrows) {
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  long long row = blockIdx.y;
-  if (i >= nelem || row >= nrows) return;
+__global__ void __launch_bounds__(256)
+lmul_diag_kernel(const T *__restrict__ lmat, const cplx_t<T> *in, cplx_t<T> *out, int lmax,
+                 int nc, long long nelem, long long ntrans) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long t = blockIdx.y;
+  if (i >= nelem || t >= ntrans) return;
   int l, m;
   lm_of(i, lmax, l, m);
-  int c = (int)(row % nc);
-  T a = lmat[c * (lmax + 1) + l];
-  cplx_t<T> v = in[row * nelem + i];
-  out[row * nelem + i] = {a * v.x, a * v.y};
+  const int nell = lmax + 1;
+  const cplx_t<T> *src = in + t * nc * nelem + i;
+  cplx_t<T> *dst = out + t * nc * nelem + i;
+  int c = 0;
+  for (; c + kLoadB <= nc; c += kLoadB) {
+    cplx_t<T> v[kLoadB];
+#pragma unroll
+    for (int b = 0; b < kLoadB; ++b) v[b] = src[(c + b) * nelem];
+#pragma unroll
+    for (int b = 0; b < kLoadB; ++b) {
+      const T a = lmat[(c + b) * nell + l];
+      dst[(c + b) * nelem] = {a * v[b].x, a * v[b].y};
+    }
+  }
+  if (nc - c == 3) {
+    const cplx_t<T> v0 = src[c * nelem], v1 = src[(c + 1) * nelem], v2 = src[(c + 2) * nelem];
+    const T a0 = lmat[c * nell + l], a1 = lmat[(c + 1) * nell + l], a2 = lmat[(c + 2) * nell + l];
+    dst[c * nelem] = {a0 * v0.x, a0 * v0.y};
+    dst[(c + 1) * nelem] = {a1 * v1.x, a1 * v1.y};
+    dst[(c + 2) * nelem] = {a2 * v2.x, a2 * v2.y};
+    return;
+  }
+  for (; c < nc; ++c) {
+    const T a = lmat[c * nell + l];
+    const cplx_t<T> v = src[c * nelem];
+    dst[c * nelem] = {a * v.x, a * v.y};
+  }
 }
 
 template <typename T>
@@ -109,15 +146,13 @@ static void lmul_impl(const void *lmat, int diag, const void *in, void *out, int
   const T *L = static_cast<const T *>(lmat);
   auto *I = static_cast<const cplx_t<T> *>(in);
   auto *O = static_cast<cplx_t<T> *>(out);
-  if (diag) {
-    long long nrows = ntrans * ncomp;
-    OWMI_REQUIRE(nrows < 65536, "too many alm rows for one lmul launch");
-    dim3 grid(ceil_div(nelem, 256), (unsigned)nrows);
-    hipLaunchKernelGGL(lmul_diag_kernel<T>, grid, block, 0, st, L, I, O, lmax, ncomp, nelem, nrows);
-    return;
-  }
   OWMI_REQUIRE(ntrans < 65536, "too many transforms for one lmul launch");
   dim3 grid(ceil_div(nelem, 256), (unsigned)ntrans);
+  if (diag) {
+    hipLaunchKernelGGL(lmul_diag_kernel<T>, grid, block, 0, st, L, I, O, lmax, ncomp, nelem,
+                       ntrans);
+    return;
+  }
   switch (ncomp) {
     case 1: hipLaunchKernelGGL((lmul_full_kernel<T, 1>), grid, block, 0, st, L, I, O, lmax, nelem, ntrans); break;
     case 2: hipLaunchKernelGGL((lmul_full_kernel<T, 2>), grid, block, 0, st, L, I, O, lmax, nelem, ntrans); break;
@@ -174,11 +209,59 @@ __global__ void ringweight_kernel(T *imap, const double *__restrict__ weight,
 
 // ---------------------------------------------------------- pix matvec
 
+// Pixel-diagonal multiply of ntrans maps of n elements by one diagonal: each
+// thread holds one 16-byte vector of the diagonal and applies it to up to
+// kPixTG maps (blockIdx.y groups them). Requires 16-byte aligned buffers and
+// n a multiple of the vector width; pix_diag_scalar_kernel otherwise.
+constexpr int kPixTG = 4;
+
+template <typename V, typename T>
+__global__ void __launch_bounds__(256)
+pix_diag_kernel(const T *__restrict__ mat, const T *in, T *out, long long n, long long ntrans) {
+  constexpr int W = sizeof(V) / sizeof(T);
+  const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n / W) return;
+  const V d = reinterpret_cast<const V *>(mat)[v];
+  const long long t1 = min(ntrans, (long long)(blockIdx.y + 1) * kPixTG);
+  long long t = (long long)blockIdx.y * kPixTG;
+  // kLoadB maps' loads before their stores (in place is allowed; each
+  // thread touches only its own vectors)
+  for (; t + kLoadB <= t1; t += kLoadB) {
+    V x[kLoadB];
+#pragma unroll
+    for (int b = 0; b < kLoadB; ++b) x[b] = reinterpret_cast<const V *>(in + (t + b) * n)[v];
+#pragma unroll
+    for (int b = 0; b < kLoadB; ++b) reinterpret_cast<V *>(out + (t + b) * n)[v] = d * x[b];
+  }
+  for (; t < t1; ++t)
+    reinterpret_cast<V *>(out + t * n)[v] = d * reinterpret_cast<const V *>(in + t * n)[v];
+}
+
 template <typename T>
-__global__ void pix_diag_kernel(const T *__restrict__ mat, const T *in, T *out, long long n) {
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  long long t = blockIdx.y;
-  if (i < n) out[t * n + i] = mat[i] * in[t * n + i];
+__global__ void __launch_bounds__(256)
+pix_diag_scalar_kernel(const T *__restrict__ mat, const T *in, T *out, long long n,
+                       long long ntrans) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const T d = mat[i];
+  const long long t1 = min(ntrans, (long long)(blockIdx.y + 1) * kPixTG);
+  for (long long t = (long long)blockIdx.y * kPixTG; t < t1; ++t) out[t * n + i] = d * in[t * n + i];
+}
+
+template <typename T>
+static void launch_pix_diag(const T *mat, const T *in, T *out, long long n, long long ntrans,
+                            hipStream_t st) {
+  typedef T V __attribute__((ext_vector_type(16 / sizeof(T))));
+  constexpr int W = 16 / sizeof(T);
+  auto aligned = [](const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
+  const unsigned tg = (unsigned)ceil_div(ntrans, kPixTG);
+  if (n % W == 0 && aligned(mat) && aligned(in) && aligned(out)) {
+    dim3 g(ceil_div(n / W, 256), tg);
+    hipLaunchKernelGGL((pix_diag_kernel<V, T>), g, dim3(256), 0, st, mat, in, out, n, ntrans);
+  } else {
+    dim3 g(ceil_div(n, 256), tg);
+    hipLaunchKernelGGL(pix_diag_scalar_kernel<T>, g, dim3(256), 0, st, mat, in, out, n, ntrans);
+  }
 }
 
 template <typename T>
@@ -202,29 +285,6 @@ __global__ void pix_full_kernel(const T *__restrict__ mat, const T *in, T *out, 
 
 constexpr int kDotBlocks = 1024;
 constexpr int kDotThreads = 256;
-
-template <typename T>
-__global__ void __launch_bounds__(kDotThreads)
-dot_partial_kernel(const cplx_t<T> *a, const cplx_t<T> *b, int lmax, long long nelem,
-                   long long nrows, double *partial) {
-  __shared__ double red[kDotThreads];
-  double acc = 0.0;
-  long long total = nelem * nrows;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
-       i += (long long)gridDim.x * blockDim.x) {
-    long long j = i % nelem;
-    cplx_t<T> x = a[i], y = b[i];
-    double v = (double)x.x * (double)y.x + (double)x.y * (double)y.y;
-    acc += (j <= lmax) ? v : 2.0 * v;
-  }
-  red[threadIdx.x] = acc;
-  __syncthreads();
-  for (int s = kDotThreads / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
-}
 
 __global__ void __launch_bounds__(kDotThreads)
 dot_final_kernel(const double *partial, int n, double *out) {
@@ -271,21 +331,41 @@ static double *dot_scratch() {
   return p;
 }
 
-// Batched dot: nbatch independent products, each over `per` complex elements
-// (rows of nelem); block (x, b) reduces a strided slice of product b.
+// Batched dot: nbatch independent products, each over nrows rows of nelem
+// complex elements; block (x, b) reduces a strided slice of every row of
+// product b. The first lmax + 1 elements of a row (m = 0) count once, the
+// rest twice (real field, m < 0 implied). kLoadB elements per thread and
+// step are loaded before they are summed.
 template <typename T>
 __global__ void __launch_bounds__(kDotThreads)
 dot_batch_partial_kernel(const cplx_t<T> *a, const cplx_t<T> *b, int lmax, long long nelem,
-                         long long per, double *partial) {
+                         long long nrows, double *partial) {
   __shared__ double red[kDotThreads];
-  const long long base = (long long)blockIdx.y * per;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  const long long j0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   double acc = 0.0;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < per;
-       i += (long long)gridDim.x * blockDim.x) {
-    const long long j = i % nelem;
-    const cplx_t<T> x = a[base + i], y = b[base + i];
-    const double v = (double)x.x * (double)y.x + (double)x.y * (double)y.y;
-    acc += (j <= lmax) ? v : 2.0 * v;
+  for (long long r = 0; r < nrows; ++r) {
+    const long long base = ((long long)blockIdx.y * nrows + r) * nelem;
+    const cplx_t<T> *ar = a + base, *br = b + base;
+    long long j = j0;
+    for (; j + (kLoadB - 1) * stride < nelem; j += kLoadB * stride) {
+      cplx_t<T> x[kLoadB], y[kLoadB];
+#pragma unroll
+      for (int k = 0; k < kLoadB; ++k) {
+        x[k] = ar[j + k * stride];
+        y[k] = br[j + k * stride];
+      }
+#pragma unroll
+      for (int k = 0; k < kLoadB; ++k) {
+        const double v = (double)x[k].x * (double)y[k].x + (double)x[k].y * (double)y[k].y;
+        acc += (j + k * stride <= lmax) ? v : 2.0 * v;
+      }
+    }
+    for (; j < nelem; j += stride) {
+      const cplx_t<T> x = ar[j], y = br[j];
+      const double v = (double)x.x * (double)y.x + (double)x.y * (double)y.y;
+      acc += (j <= lmax) ? v : 2.0 * v;
+    }
   }
   red[threadIdx.x] = acc;
   __syncthreads();
@@ -377,10 +457,15 @@ __global__ void axpby_kernel(T a, const cplx_t<T> *x, T b, cplx_t<T> *y, long lo
   y[i] = {a * xv.x + b * yv.x, a * xv.y + b * yv.y};
 }
 
+// One complex element per thread (a 16- or 8-byte load, an 8- or 16-byte
+// store).
 template <typename S, typename D>
-__global__ void convert_kernel(const S *src, D *dst, long long n) {
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) dst[i] = (D)src[i];
+__global__ void __launch_bounds__(256)
+convert_kernel(const cplx_t<S> *src, cplx_t<D> *dst, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const cplx_t<S> v = src[i];
+  dst[i] = {(D)v.x, (D)v.y};
 }
 
 }  // namespace owmi
@@ -452,10 +537,10 @@ int owmi_alm_dot(int dtype, const void *a, const void *b, int lmax, long long nr
     double *part = dot_scratch();
     hipStream_t st = as_stream(stream);
     if (dtype == OWMI_F64)
-      hipLaunchKernelGGL(dot_partial_kernel<double>, dim3(kDotBlocks), dim3(kDotThreads), 0, st,
+      hipLaunchKernelGGL(dot_batch_partial_kernel<double>, dim3(kDotBlocks), dim3(kDotThreads), 0, st,
                          (const double2 *)a, (const double2 *)b, lmax, nelem, nrows, part);
     else
-      hipLaunchKernelGGL(dot_partial_kernel<float>, dim3(kDotBlocks), dim3(kDotThreads), 0, st,
+      hipLaunchKernelGGL(dot_batch_partial_kernel<float>, dim3(kDotBlocks), dim3(kDotThreads), 0, st,
                          (const float2 *)a, (const float2 *)b, lmax, nelem, nrows, part);
     hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(kDotThreads), 0, st, part, kDotBlocks,
                        dev_out);
@@ -468,17 +553,16 @@ int owmi_alm_dot_batch(int dtype, const void *a, const void *b, int lmax, long l
     check_dtype(dtype);
     OWMI_REQUIRE(nbatch >= 1 && nbatch <= kDotBlocks, "nbatch must be in [1, 1024]");
     const long long nelem = nelem_of(lmax);
-    const long long per = nelem * nrows;
     const int bpb = kDotBlocks / (int)nbatch;
     double *part = dot_scratch();
     hipStream_t st = as_stream(stream);
     dim3 g(bpb, (unsigned)nbatch);
     if (dtype == OWMI_F64)
       hipLaunchKernelGGL(dot_batch_partial_kernel<double>, g, dim3(kDotThreads), 0, st,
-                         (const double2 *)a, (const double2 *)b, lmax, nelem, per, part);
+                         (const double2 *)a, (const double2 *)b, lmax, nelem, nrows, part);
     else
       hipLaunchKernelGGL(dot_batch_partial_kernel<float>, g, dim3(kDotThreads), 0, st,
-                         (const float2 *)a, (const float2 *)b, lmax, nelem, per, part);
+                         (const float2 *)a, (const float2 *)b, lmax, nelem, nrows, part);
     hipLaunchKernelGGL(dot_batch_final_kernel, dim3((unsigned)nbatch), dim3(kDotThreads), 0, st,
                        part, bpb, dev_out);
   });
@@ -553,13 +637,11 @@ int owmi_pix_matvec(int dtype, const void *mat, int kind, const void *in, void *
     hipStream_t st = as_stream(stream);
     if (kind == 0) {
       long long n = npix * ncomp;
-      dim3 g(ceil_div(n, 256), (unsigned)ntrans);
       if (dtype == OWMI_F64)
-        hipLaunchKernelGGL(pix_diag_kernel<double>, g, dim3(256), 0, st,
-                           (const double *)mat, (const double *)in, (double *)out, n);
+        launch_pix_diag<double>((const double *)mat, (const double *)in, (double *)out, n, ntrans,
+                                st);
       else
-        hipLaunchKernelGGL(pix_diag_kernel<float>, g, dim3(256), 0, st,
-                           (const float *)mat, (const float *)in, (float *)out, n);
+        launch_pix_diag<float>((const float *)mat, (const float *)in, (float *)out, n, ntrans, st);
     } else {
       dim3 g(ceil_div(npix, 256), (unsigned)ntrans);
       if (dtype == OWMI_F64)
@@ -645,21 +727,20 @@ int owmi_convert(int dtype_src, const void *src, int dtype_dst, void *dst, long 
     check_dtype(dtype_src);
     check_dtype(dtype_dst);
     if (n == 0) return;
-    long long nr = 2 * n;
-    dim3 g(ceil_div(nr, 256));
+    dim3 g(ceil_div(n, 256));
     hipStream_t st = as_stream(stream);
     if (dtype_src == OWMI_F64 && dtype_dst == OWMI_F32)
       hipLaunchKernelGGL((convert_kernel<double, float>), g, dim3(256), 0, st,
-                         (const double *)src, (float *)dst, nr);
+                         (const double2 *)src, (float2 *)dst, n);
     else if (dtype_src == OWMI_F32 && dtype_dst == OWMI_F64)
       hipLaunchKernelGGL((convert_kernel<float, double>), g, dim3(256), 0, st,
-                         (const float *)src, (double *)dst, nr);
+                         (const float2 *)src, (double2 *)dst, n);
     else if (dtype_src == OWMI_F64)
       hipLaunchKernelGGL((convert_kernel<double, double>), g, dim3(256), 0, st,
-                         (const double *)src, (double *)dst, nr);
+                         (const double2 *)src, (double2 *)dst, n);
     else
       hipLaunchKernelGGL((convert_kernel<float, float>), g, dim3(256), 0, st,
-                         (const float *)src, (float *)dst, nr);
+                         (const float2 *)src, (float2 *)dst, n);
   });
 }
 

@@ -80,6 +80,27 @@ def test_dot_batch(dev, dtype):
     assert abs(alm_utils.contract_almxblm_batch(a, a).cpu().numpy()[1] - cf) <= 1e-12 * cf
 
 
+@pytest.mark.parametrize('dtype', [torch.complex128, torch.complex64])
+@pytest.mark.parametrize('lmax, nb, ncomp', [(500, 16, 3), (700, 3, 3), (1500, 1, 1)])
+def test_dot_batch_sizes(dev, dtype, lmax, nb, ncomp):
+    '''Sizes where every thread of the partial-sum kernel takes several
+    strided steps per row (and a remainder), batched and single, against a
+    float64 numpy sum: <a, b> = 2 sum Re(a conj b) - sum_{m=0} Re(a conj b).'''
+    nelem = alm_utils.AlmInfo(lmax).nelem
+    rng = np.random.default_rng(lmax)
+    a = rand_alm_t(rng, (nb, ncomp, nelem), lmax, dtype, dev)
+    b = rand_alm_t(rng, (nb, ncomp, nelem), lmax, dtype, dev)
+    an = a.to(torch.complex128).cpu().numpy()
+    bn = b.to(torch.complex128).cpu().numpy()
+    prod = (an * np.conj(bn)).real
+    ref = 2 * prod.sum(axis=(1, 2)) - prod[..., :lmax + 1].sum(axis=(1, 2))
+    scale = 2 * np.abs(prod).sum(axis=(1, 2))
+    got = alm_utils.contract_almxblm_batch(a, b).cpu().numpy()
+    assert np.all(np.abs(got - ref) <= 1e-13 * scale)
+    for i in range(nb):
+        assert abs(alm_utils.contract_almxblm(a[i], b[i]) - ref[i]) <= 1e-13 * scale[i]
+
+
 def test_cg_batch_updates(dev):
     rng = np.random.default_rng(1)
     nb, n = 4, 1000

@@ -52,6 +52,78 @@ def test_lmul_batched_and_1d(dev):
     np.testing.assert_allclose(out, ops.lmul(a, w, lmax), rtol=1e-15, atol=1e-14)
 
 
+@pytest.mark.parametrize('dtype', [np.complex128, np.complex64])
+@pytest.mark.parametrize('ntrans', [1, 16, 17, 40])
+def test_lmul_transform_groups(dev, dtype, ntrans):
+    '''Batches of transforms (full 3x3 and diagonal matrices, out of place
+    and in place) against the oracle per transform.'''
+    lmax = 45
+    a = np.stack([rand_alm(lmax, 3, 10 + s, dtype) for s in range(ntrans)])
+    rng = np.random.default_rng(4)
+    full = rng.normal(size=(3, 3, lmax + 1))
+    diag = rng.normal(size=(3, lmax + 1))
+    tol = 1e-15 if dtype == np.complex128 else 1e-6
+    for lmat in (full, diag):
+        ref = np.stack([ops.lmul(a[t], lmat, lmax) for t in range(ntrans)])
+        out = alm_c_utils.lmul(torch.as_tensor(a, device=dev), lmat, lmax).cpu().numpy()
+        np.testing.assert_allclose(out, ref, rtol=tol, atol=tol * np.abs(ref).max())
+        at = torch.as_tensor(a, device=dev)
+        alm_c_utils.lmul(at, lmat, lmax, inplace=True)
+        np.testing.assert_allclose(at.cpu().numpy(), ref, rtol=tol, atol=tol * np.abs(ref).max())
+
+
+@pytest.mark.parametrize('src, dst', [(torch.complex128, torch.complex64),
+                                      (torch.complex64, torch.complex128),
+                                      (torch.complex128, torch.complex128),
+                                      (torch.complex64, torch.complex64)])
+def test_convert(dev, src, dst):
+    '''owmi_convert of n complex elements: round to nearest, bit-exact
+    against numpy's cast.'''
+    rng = np.random.default_rng(3)
+    n = 1001
+    v = (rng.normal(size=n) + 1j * rng.normal(size=n)) * 10.0 ** rng.integers(-30, 30, n)
+    x = torch.as_tensor(v, device=dev).to(src)
+    y = torch.empty(n, dtype=dst, device=dev)
+    _lib.call('owmi_convert', _lib.dtype_code(x), _lib.ptr(x), _lib.dtype_code(y), _lib.ptr(y),
+              n, _lib.stream_ptr(dev))
+    want = x.cpu().numpy().astype({torch.complex64: np.complex64,
+                                   torch.complex128: np.complex128}[dst])
+    np.testing.assert_array_equal(y.cpu().numpy(), want)
+
+
+@pytest.mark.parametrize('npol', [1, 2, 4, 5, 7])
+def test_lmul_components(dev, npol):
+    '''Component counts on either side of the diagonal kernel's load groups
+    of four and of the unrolled full-matrix kernels (1-4; 5 and 7 take the
+    generic one), two transforms, against the oracle.'''
+    lmax = 30
+    a = np.stack([rand_alm(lmax, npol, 20 + s, np.complex128) for s in range(2)])
+    rng = np.random.default_rng(npol)
+    for lmat in (rng.normal(size=(npol, npol, lmax + 1)), rng.normal(size=(npol, lmax + 1))):
+        ref = np.stack([ops.lmul(a[t], lmat, lmax) for t in range(2)])
+        out = alm_c_utils.lmul(torch.as_tensor(a, device=dev), lmat, lmax).cpu().numpy()
+        np.testing.assert_allclose(out, ref, rtol=1e-14, atol=1e-14 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+@pytest.mark.parametrize('npix', [1000, 1001])
+@pytest.mark.parametrize('ntrans', [1, 17, 40])
+def test_pix_diag_batches(dev, dtype, npix, ntrans):
+    '''Pixel-diagonal multiply of a batch of maps by one diagonal: the
+    16-byte vector path (npix 1000) and the scalar path (3 x 1001 elements,
+    not a multiple of the vector width), with map counts that leave partial
+    groups of the maps a thread handles, out of place and in place;
+    bit-exact against numpy.'''
+    rng = np.random.default_rng(7)
+    d = rng.normal(size=(3, npix)).astype(dtype)
+    v = rng.normal(size=(ntrans, 3, npix)).astype(dtype)
+    dt, vt = torch.as_tensor(d, device=dev), torch.as_tensor(v, device=dev)
+    out = mat_utils.matvec(dt, vt)
+    np.testing.assert_array_equal(out.cpu().numpy(), d * v)
+    mat_utils.matvec(dt, vt, inplace=True)
+    np.testing.assert_array_equal(vt.cpu().numpy(), d * v)
+
+
 def test_trunc_alm_golden(dev):
     '''test_alm_c_utils.py:10-59 literal arrays, bit exact.'''
     alm = np.zeros((2, 6), dtype=np.complex128)
