@@ -98,6 +98,21 @@ __device__ __attribute__((noinline)) cplx_t<T> fold_bin(const cplx_t<T> *F, long
   return acc;
 }
 
+// fold_bin of a ring that resolves every m (N > 2 mmax) with phi0 = 0 (the
+// Gauss-Legendre and Clenshaw-Curtis grids at nphi >= 2 lmax + 1): bin k
+// holds C_k for k <= mmax, conj C_{N-k} for k >= N - mmax, zero between.
+template <typename T>
+__device__ __forceinline__ cplx_t<T> plain_bin(const cplx_t<T> *F, long long k, long long N,
+                                               int mmax) {
+  if (k <= mmax) {
+    cplx_t<T> c = F[k];
+    if (k == 0) c.y = 0;
+    return c;
+  }
+  if (N - k <= mmax) return cconj<T>(F[N - k]);
+  return {0, 0};
+}
+
 // Workgroup barrier for the FFT buffer: LDS, or (GB) a global-memory
 // region private to the workgroup for rings too long for LDS, where the
 // barrier is wrapped in device-scope release / acquire fences so that the
@@ -224,11 +239,39 @@ __device__ __forceinline__ void first_pass_small(cplx_t<T> *v, cplx_t<T> *buf, i
   }
 }
 
+// Twiddles W^j = exp(-2 pi i j / Mmax), j < Mmax, from two workgroup
+// tables in LDS: lo[j mod 2^hb] * hi[j >> hb] (one complex product of two
+// correctly rounded values). Per-lane reads of a global table of Mmax
+// entries at scattered indices measured 35-40 % of the sandwich kernel's
+// time at Mmax 4096.
+constexpr int kTwTab = 128;  // entries per table: Mmax <= 2^14
+
+template <typename T>
+struct Twiddles {
+  const cplx_t<T> *lo;
+  const cplx_t<T> *hi;
+  int hb;
+  __device__ __forceinline__ cplx_t<T> operator()(int j) const {
+    return cmul<T>(lo[j & ((1 << hb) - 1)], hi[j >> hb]);
+  }
+};
+
+// Fill the tables of a Twiddles from the plan's table tw (Mmax entries);
+// the caller's next barrier makes them visible.
+template <typename T>
+__device__ __forceinline__ Twiddles<T> load_twiddles(const cplx_t<T> *tw, int log_mmax,
+                                                     cplx_t<T> *lo, cplx_t<T> *hi) {
+  const int hb = (log_mmax + 1) >> 1;
+  for (int i = threadIdx.x; i < (1 << hb); i += blockDim.x) lo[i] = tw[i];
+  for (int i = threadIdx.x; i < (1 << (log_mmax - hb)); i += blockDim.x) hi[i] = tw[i << hb];
+  return Twiddles<T>{lo, hi, hb};
+}
+
 // Full FFT of the thread's 16 slots (see above). All threads of the block
 // call it (barriers); threads with q >= Q pass through.
 template <typename T, int DIR, bool GB>
 __device__ __forceinline__ void fft16(cplx_t<T> *v, cplx_t<T> *buf, int logm, int q,
-                                      const cplx_t<T> *tw, int log_mmax) {
+                                      const Twiddles<T> &tw, int log_mmax) {
   const int Q = 1 << (logm - 4);
   const bool on = q < Q;
   const int r0 = logm & 3;
@@ -269,7 +312,7 @@ __device__ __forceinline__ void fft16(cplx_t<T> *v, cplx_t<T> *buf, int logm, in
         const int step = k << (log_mmax - lns - 4);  // k Mmax / (16 Ns)
 #pragma unroll
         for (int r = 1; r < 16; ++r) {
-          cplx_t<T> w = tw[r * step];
+          cplx_t<T> w = tw(r * step);
           if (DIR > 0) w.y = -w.y;
           v[r] = cmul<T>(v[r], w);
         }
@@ -310,7 +353,7 @@ struct RingArgs {
 // slots: v <- IDFT_M(DFT_M(v) * bhat) (bhat includes 1/M).
 template <typename T, bool GB>
 __device__ __forceinline__ void bluestein_conv(cplx_t<T> *v, cplx_t<T> *buf, int logm, int q,
-                                               const cplx_t<T> *tw, int log_mmax,
+                                               const Twiddles<T> &tw, int log_mmax,
                                                const cplx_t<T> *bhat) {
   const int Q = 1 << (logm - 4);
   fft16<T, -1, GB>(v, buf, logm, q, tw, log_mmax);
@@ -334,6 +377,8 @@ ring_kernel(RingArgs a, const cplx_t<T> *twiddle, const cplx_t<T> *chirp,
   extern __shared__ unsigned char smem_raw[];
   cplx_t<T> *buf = GB ? gbuf + ((long long)blockIdx.y * gridDim.x + blockIdx.x) * region
                       : reinterpret_cast<cplx_t<T> *>(smem_raw);
+  __shared__ cplx_t<T> tw_lo[kTwTab], tw_hi[kTwTab];
+  const Twiddles<T> tws = load_twiddles<T>(twiddle, a.log_mmax_fft, tw_lo, tw_hi);
 
   const long long job = blockIdx.x;
   const long long tc = blockIdx.y;  // transform * ncomp + comp
@@ -363,21 +408,28 @@ ring_kernel(RingArgs a, const cplx_t<T> *twiddle, const cplx_t<T> *chirp,
     const cplx_t<T> *Fa = Gin + G_row_a * (mmax + 1);
     const cplx_t<T> *Fb = Gin + G_row_b * (mmax + 1);
     // Inverse DFT of Z = Za + i Zb through conj: x = conj(DFT(conj Z)).
+    const bool plain = N > 2LL * mmax && pa == (T)0 && pb == (T)0;
     if (on) {
 #pragma unroll
       for (int s = 0; s < 16; ++s) {
         const long long k = q + s * Q;
         cplx_t<T> val = {0, 0};
         if (k < N) {
-          const cplx_t<T> za = fold_bin<T>(Fa, k, N, mmax, pa);
-          const cplx_t<T> zb = rb >= 0 ? fold_bin<T>(Fb, k, N, mmax, pb) : cplx_t<T>{0, 0};
+          cplx_t<T> za, zb = {0, 0};
+          if (plain) {
+            za = plain_bin<T>(Fa, k, N, mmax);
+            if (rb >= 0) zb = plain_bin<T>(Fb, k, N, mmax);
+          } else {
+            za = fold_bin<T>(Fa, k, N, mmax, pa);
+            if (rb >= 0) zb = fold_bin<T>(Fb, k, N, mmax, pb);
+          }
           const cplx_t<T> Z = {za.x - zb.y, za.y + zb.x};
           val = cmul<T>(cconj<T>(Z), w[k]);
         }
         v[s] = val;
       }
     }
-    bluestein_conv<T, GB>(v, buf, logm, q, twiddle, a.log_mmax_fft, bhat);
+    bluestein_conv<T, GB>(v, buf, logm, q, tws, a.log_mmax_fft, bhat);
     // DFT(conj Z)_j = w_j c_j and x = conj of it: ring a gets Re(w_j c_j),
     // ring b gets -Im(w_j c_j).
     if (on) {
@@ -432,7 +484,7 @@ ring_kernel(RingArgs a, const cplx_t<T> *twiddle, const cplx_t<T> *chirp,
   }
 
   if (MODE == kAdjoint || MODE == kSandwich) {
-    bluestein_conv<T, GB>(v, buf, logm, q, twiddle, a.log_mmax_fft, bhat);
+    bluestein_conv<T, GB>(v, buf, logm, q, tws, a.log_mmax_fft, bhat);
     // X_k = w_k c_k, gathered in the buffer (natural order) for the Hermitian split
     block_sync<GB>();
     if (on) {
@@ -556,7 +608,8 @@ void launch_ring(owmi_plan_impl *p, const cplx_t<T> *Gin, cplx_t<T> *Gout, const
   OWMI_REQUIRE(threads <= 1024, "ring too long for one workgroup");
   PlanTables<T> &tb = tab<T>(p);
   dim3 grid((unsigned)p->nfft_jobs, (unsigned)nrows);
-  const bool gb = lds > kMaxLds;
+  OWMI_REQUIRE(p->log2_mmax_fft <= 14, "ring FFT length above 2^14");
+  const bool gb = lds + 2 * kTwTab * sizeof(cplx_t<T>) > kMaxLds;
   cplx_t<T> *gbuf = nullptr;
   const long long region = p->mmax_fft + p->mmax_fft / 16;
   if (gb) {

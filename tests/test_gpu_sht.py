@@ -131,6 +131,44 @@ def test_pixel_sandwich(dev):
     assert rel(out.cpu().numpy(), ref) < 1e-12
 
 
+@pytest.mark.parametrize('dtype', ['float64', 'float32'])
+def test_shifted_rings(dev, dtype):
+    '''Rings with a first-pixel longitude phi0 (rings 1, 5, 9, ...; the
+    others at phi0 = 0), so that one plan has mirror-ring pairs with one
+    shifted ring (general folding path) and pairs with none (unshifted fast
+    path): synthesis, adjoint analysis and the pixel sandwich against the
+    oracle.'''
+    lmax = 24
+    g0 = og.gauss_legendre(lmax + 1, 2 * lmax + 1)
+    phi0 = np.where(np.arange(g0.nrow) % 4 == 1, 0.03 + 0.01 * np.arange(g0.nrow), 0.0)
+    g = og.Geom(g0.theta, g0.weight, g0.nphi, phi0=phi0)
+    m = map_utils.MapInfo(g0.theta, g0.weight, nphi=g0.nphi, phi0=phi0)
+    ai = alm_utils.AlmInfo(lmax)
+    cdt = torch.complex128 if dtype == 'float64' else torch.complex64
+    rdt = torch.float64 if dtype == 'float64' else torch.float32
+    tol = 1e-12 if dtype == 'float64' else 2e-5
+    a = rand_alm(lmax, 3, 11)
+    ref = np.zeros((3, g.npix))
+    osht.alm2map(a, ref, lmax, g, [0, 2])
+    out = torch.zeros((3, m.npix), dtype=rdt, device=dev)
+    sht.alm2map(torch.as_tensor(a, device=dev).to(cdt), out, ai, m, [0, 2])
+    assert rel(out.cpu().numpy(), ref) < tol
+    rng = np.random.default_rng(12)
+    f = rng.normal(size=(3, g.npix))
+    refa = np.zeros((3, layout.nelem(lmax)), dtype=np.complex128)
+    osht.map2alm(f, refa, g, lmax, [0, 2], adjoint=True)
+    outa = torch.zeros((3, layout.nelem(lmax)), dtype=cdt, device=dev)
+    sht.map2alm(torch.as_tensor(f, device=dev).to(rdt), outa, m, ai, [0, 2], adjoint=True)
+    assert rel(outa.cpu().numpy(), refa) < tol
+    d = rng.uniform(0.5, 2, size=(3, g.npix))
+    refs = np.zeros_like(a)
+    osht.map2alm(ref * d, refs, g, lmax, [0, 2], adjoint=True)
+    outs = torch.zeros((3, layout.nelem(lmax)), dtype=cdt, device=dev)
+    sht.pixel_sandwich(torch.as_tensor(a, device=dev).to(cdt), outs, ai, m, [0, 2],
+                       torch.as_tensor(d, device=dev).to(rdt))
+    assert rel(outs.cpu().numpy(), refs) < tol
+
+
 @pytest.mark.parametrize('lmax', [255])
 def test_large_lmax_roundtrip(dev, lmax):
     '''Y^T W Y = I on the GL grid (size-independent property). The oracle
