@@ -2,7 +2,9 @@
 Preconditioners on device alm, mirroring optweight's preconditioners:
 HarmonicPreconditioner (preconditioners.py:8-124),
 PseudoInvPreconditioner (126-249), PseudoInvPreconditionerWav (251-395),
-MaskedPreconditioner (558-650), get_itau_ell_harm (916-948).
+MaskedPreconditioner (558-650), MaskedPreconditionerCG (652-789),
+get_2level_prec / get_prec_adef1 / get_prec_adef2 (791-914),
+get_itau_ell_harm (916-948).
 '''
 import numpy as np
 import torch
@@ -220,6 +222,168 @@ class MaskedPreconditioner(operators.MatVecAlm):
         omap *= mu
         sht.map2alm(omap, alm, self.minfo, self.ainfo, self.spin, adjoint=True, rings=rings)
         return alm_c_utils.lmul(alm, r, self.ainfo, inplace=True)
+
+
+def _map_dot(x, y):
+    '''<x, y> of real maps in float64: a 0-d tensor for one map set
+    (npol, npix), one value per right-hand side (nb,) for a batch.'''
+    if x.ndim >= 3:
+        return (x * y).reshape(x.shape[0], -1).sum(dim=1, dtype=torch.float64)
+    return (x * y).sum(dtype=torch.float64)
+
+
+def _ratio(num, den):
+    '''num / den per right-hand side, 0 where den is 0 (a right-hand side
+    whose input is zero stays zero instead of turning into NaN).'''
+    return torch.where(den != 0, num / torch.where(den != 0, den, torch.ones_like(den)),
+                       torch.zeros_like(num))
+
+
+class MaskedPreconditionerCG(operators.MatVecAlm):
+    '''Approximate inverse of the masked-pixel system by nsteps conjugate
+    gradient steps (preconditioners.py:652-789): with alm_lo the l, m <= lmax
+    part of the input (times r_ell if lmax_r_ell is given), solve
+    G x = M_u Y alm_lo with G = M_u Y d_ell Y^T M_u (d_ell = icov_ell r_ell^2)
+    on the equal-area geometry of band limit lmax, inner preconditioner
+    M_u W Y icov_ell^-1 Y^T W M_u, and return r_ell Y^T M_u x in the l, m <=
+    lmax part of a zero alm. Inputs (npol, nelem) or batches
+    (..., npol, nelem): every right-hand side runs its own CG with its own
+    scalars; a right-hand side that is zero gives zero.'''
+
+    def __init__(self, ainfo, icov_ell, spin, mask_bool, minfo, lmax_r_ell=None, lmax=None,
+                 nsteps=15, verbose=False, sfilt=None):
+        icov_ell = np.asarray(icov_ell)
+        self.spin = spin
+        self.npol = icov_ell.shape[0]
+        self.ainfo = ainfo
+        self.nsteps = nsteps
+        self.verbose = verbose
+        if sfilt is not None:
+            sfilt = mat_utils.full_matrix(np.asarray(sfilt))
+            icov_ell = mat_utils.full_matrix(icov_ell)
+            icov_ell = np.einsum('ijl, jkl, kol -> iol', sfilt, icov_ell, sfilt)
+        if lmax is not None:
+            lmax = min(lmax, ainfo.lmax)
+            icov_ell = icov_ell[..., :lmax + 1].copy()
+        else:
+            lmax = ainfo.lmax
+        self.lmax = lmax
+        self.ainfo_lowres = alm_utils.AlmInfo(lmax)
+        if lmax_r_ell is not None:
+            self.r_ell = multigrid.lowpass_filter(lmax_r_ell, lmax=ainfo.lmax)
+            d_ell = icov_ell * self.r_ell[:lmax + 1] ** 2
+            self._r = operators._DeviceCache(np.ascontiguousarray(self.r_ell[:lmax + 1]))
+        else:
+            self.r_ell = None
+            d_ell = icov_ell.copy()
+        mask_bool = np.asarray(mask_bool)
+        if mask_bool.ndim == 1:
+            mask_bool = np.ones((self.npol, mask_bool.size), dtype=mask_bool.dtype) * mask_bool
+        elif mask_bool.shape[0] != self.npol:
+            raise ValueError(f'shape[0] mask != npol {mask_bool.shape[0]} != {self.npol}')
+        if mask_bool.dtype != bool:
+            raise ValueError(f'Input mask must be bool, got {mask_bool.dtype}')
+        mask_bool, minfo = multigrid.get_equal_area_mask_bool(mask_bool, minfo, lmax=lmax)
+        self.mask_unobs = ~mask_bool
+        self.minfo = minfo
+        self._mu = operators._DeviceCache(self.mask_unobs)
+        self.g_op = operators.PixEllPixMatVecMap(self.mask_unobs, d_ell, self.minfo, self.spin,
+                                                 lmax=self.lmax)
+        self.prec = operators.PixEllPixMatVecMap(self.mask_unobs, icov_ell, self.minfo,
+                                                 self.spin, power_x=-1, lmax=self.lmax,
+                                                 adjoint=True)
+        self.dot = _map_dot
+
+    def _cg(self, b):
+        '''nsteps of preconditioned CG from x = 0 (pixell.utils.CG order of
+        operations), per-right-hand-side scalars.'''
+        bshape = (-1,) + (1,) * (b.ndim - 1) if b.ndim >= 3 else ()
+        x = torch.zeros_like(b)
+        r = b
+        z = self.prec(r)
+        rz = self.dot(r, z)
+        p = z
+        for idx in range(self.nsteps):
+            Ap = self.g_op(p)
+            alpha = _ratio(rz, self.dot(p, Ap)).to(b.dtype).reshape(bshape)
+            x += alpha * p
+            r = r - alpha * Ap
+            del Ap
+            z = self.prec(r)
+            rz_next = self.dot(r, z)
+            beta = _ratio(rz_next, rz).to(b.dtype).reshape(bshape)
+            rz = rz_next
+            p = z + beta * p
+            if self.verbose:
+                print(idx, rz.cpu().numpy())
+        return x
+
+    def call(self, alm):
+        rdt = operators._real_dtype(alm.dtype)
+        alm_lo, _ = alm_utils.trunc_alm(alm, self.ainfo, self.lmax)
+        if self.r_ell is not None:
+            alm_c_utils.lmul(alm_lo, self._r.get(alm.device, rdt), self.ainfo_lowres,
+                             inplace=True)
+        mshape = (self.npol,) if alm.ndim == 1 else tuple(alm.shape[:-1])
+        imap = torch.zeros(mshape + (self.minfo.npix,), dtype=rdt, device=alm.device)
+        rings = self.g_op.rings
+        sht.alm2map(alm_lo, imap, self.ainfo_lowres, self.minfo, self.spin, rings=rings)
+        mu = self._mu.get(alm.device, torch.bool)
+        imap *= mu
+        omap = self._cg(imap)
+        omap *= mu
+        sht.map2alm(omap, alm_lo, self.minfo, self.ainfo_lowres, self.spin, adjoint=True,
+                    rings=rings)
+        if self.r_ell is not None:
+            alm_c_utils.lmul(alm_lo, self._r.get(alm.device, rdt), self.ainfo_lowres,
+                             inplace=True)
+        out = torch.zeros_like(alm)
+        return alm_utils.add_to_alm(out, alm_lo, self.ainfo, self.ainfo_lowres)
+
+
+def get_2level_prec(prec_main, prec_masked, solver, method, sel_masked=None):
+    '''Two-level preconditioner (preconditioners.py:791-822): 'ADEF-1' or
+    'ADEF-2' combination of the main preconditioner M and the masked one Q
+    through the solver's A; sel_masked restricts Q to a slice.'''
+    if method == 'ADEF-1':
+        return get_prec_adef1(prec_main, prec_masked, solver, sel_masked=sel_masked)
+    if method == 'ADEF-2':
+        return get_prec_adef2(prec_main, prec_masked, solver, sel_masked=sel_masked)
+    raise ValueError(f'method : {method} not supported.')
+
+
+def _apply_sliced(prec, alm, sel):
+    if sel is None:
+        return prec(alm)
+    out = torch.zeros_like(alm)
+    out[sel] = prec(alm[sel].clone())
+    return out
+
+
+def get_prec_adef1(prec_main, prec_masked, solver, sel_masked=None):
+    '''ADEF-1 (preconditioners.py:824-865): M2 b = M (b - A Q b) + Q b.'''
+
+    def prec_adef1(alm):
+        q_vec = _apply_sliced(prec_masked, alm, sel_masked)
+        out = prec_main(alm - solver.A(q_vec))
+        out += q_vec
+        return out
+
+    return prec_adef1
+
+
+def get_prec_adef2(prec_main, prec_masked, solver, sel_masked=None):
+    '''ADEF-2 (preconditioners.py:867-914): M2 b = M b - Q A M b + Q b.'''
+
+    def prec_adef2(alm):
+        m_vec = prec_main(alm)
+        qam_vec = _apply_sliced(prec_masked, solver.A(m_vec), sel_masked)
+        m_vec = m_vec - qam_vec
+        del qam_vec
+        m_vec += _apply_sliced(prec_masked, alm, sel_masked)
+        return m_vec
+
+    return prec_adef2
 
 
 def get_itau_ell_harm(icov_ell, w_ell):

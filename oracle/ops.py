@@ -67,6 +67,22 @@ def trunc_alm(alm, lmax, lmax_out):
     return out
 
 
+def add_to_alm(alm, blm, lmax_a, lmax_b, overwrite=False):
+    '''In place: alm[..., lm_a(l, m)] (+)= blm[..., lm_b(l, m)] for l, m
+    <= lmax_b (alm_utils.py:428-480; mmax = lmax on both sides).'''
+    if lmax_b > lmax_a:
+        raise ValueError(f'lmax blm exceeds that of alm : {lmax_b} > {lmax_a}')
+    for m in range(lmax_b + 1):
+        i0 = layout.mstart(m, lmax_a) + m
+        j0 = layout.mstart(m, lmax_b) + m
+        n = lmax_b + 1 - m
+        if overwrite:
+            alm[..., i0:i0 + n] = blm[..., j0:j0 + n]
+        else:
+            alm[..., i0:i0 + n] += blm[..., j0:j0 + n]
+    return alm
+
+
 def wlm2alm(w_ell, wlm, alm, lmax_w, lmax_a):
     '''alm[..., lm] += w_ell[l] wlm[..., lm'] for l, m <= lmax_w
     (alm_c_utils.c:113-144).'''

@@ -8,7 +8,9 @@ Wiener-filter solver, preconditioners and multigrid (oracle), restating:
       YTWMatVecMap 105-160, PixMatVecAlm 562-643, PixEllPixMatVecMap 991-1102,
       add_operators 1458-1489
   preconditioners.HarmonicPreconditioner 8-124 (itau from icov_pix),
-      PseudoInvPreconditioner 126-249, MaskedPreconditioner 558-650
+      PseudoInvPreconditioner 126-249, MaskedPreconditioner 558-650,
+      MaskedPreconditionerCG 652-789, get_2level_prec / get_prec_adef1 /
+      get_prec_adef2 791-914
   multigrid.Level 13-156, get_levels 158-249, get_equal_area_mask_bool
       251-294, restrict/coarsen/interpolate 296-398, v_cycle 400-447
 Only the operator variants that the Wiener path uses are restated (no
@@ -76,11 +78,16 @@ def pix_alm_op(m_pix, lmax, geom, spin, power=1, adjoint=False, use_weights=Fals
     return f
 
 
-def pix_ell_pix_op(m_pix, x_ell, geom, spin, lmax=None):
-    '''PixEllPixMatVecMap (power 1, adjoint=False, use_weights=False):
-    M Y x_ell Y^T M (no quadrature weights).'''
+def pix_ell_pix_op(m_pix, x_ell, geom, spin, lmax=None, power_x=1, adjoint=False,
+                   use_weights=False):
+    '''PixEllPixMatVecMap (operators.py:991-1102): M Y x_ell^p Y^T M, with
+    the quadrature weights W on the analysis when adjoint is use_weights
+    (Y^T W) and on the synthesis when adjoint (W Y); no weights for the
+    defaults.'''
     if lmax is None:
         lmax = geometry.minfo2lmax(geom)
+    if power_x != 1:
+        x_ell = ops.matpow(x_ell, power_x, return_diag=True)
 
     def f(imap):
         out = np.array(imap, copy=True)
@@ -88,9 +95,9 @@ def pix_ell_pix_op(m_pix, x_ell, geom, spin, lmax=None):
             out = out * m_pix
         alm = np.zeros((out.shape[0], layout.nelem(lmax)),
                        dtype=np.complex64 if out.dtype == np.float32 else np.complex128)
-        osht.map2alm(out, alm, geom, lmax, spin, adjoint=True)
+        osht.map2alm(out, alm, geom, lmax, spin, adjoint=adjoint is use_weights)
         alm = ops.lmul(alm, x_ell, lmax)
-        osht.alm2map(alm, out, lmax, geom, spin, adjoint=False)
+        osht.alm2map(alm, out, lmax, geom, spin, adjoint=adjoint)
         if m_pix is not None:
             out = out * m_pix
         return out
@@ -466,3 +473,111 @@ class MaskedPreconditioner:
         out = np.zeros_like(alm)
         osht.map2alm(omap, out, self.geom, self.lmax, self.spin, adjoint=True)
         return ops.lmul(out, self.r_ell, self.lmax)
+
+
+class MaskedPreconditionerCG:
+    '''MaskedPreconditionerCG (preconditioners.py:652-789): nsteps of CG on
+    the masked-pixel system G x = M_u Y r_ell alm_lowres with
+    G = M_u Y d_ell Y^T M_u (d_ell = icov_ell r_ell^2, or icov_ell without
+    r_ell) and inner preconditioner M_u W Y icov_ell^-1 Y^T W M_u, on the
+    equal-area geometry at the preconditioner's lmax; the result
+    r_ell Y^T M_u x is placed in the l, m <= lmax part of a zero alm.'''
+
+    def __init__(self, lmax_in, icov_ell, spin, mask_bool, geom, lmax_r_ell=None, lmax=None,
+                 nsteps=15, sfilt=None):
+        icov_ell = np.asarray(icov_ell)
+        self.spin = spin
+        self.npol = icov_ell.shape[0]
+        self.lmax_in = lmax_in
+        self.nsteps = nsteps
+        if sfilt is not None:
+            sf = ops.full_matrix(np.asarray(sfilt))
+            ic = ops.full_matrix(icov_ell)
+            icov_ell = np.einsum('ijl, jkl, kol -> iol', sf, ic, sf)
+        if lmax is not None:
+            lmax = min(lmax, lmax_in)
+            icov_ell = icov_ell[..., :lmax + 1].copy()
+        else:
+            lmax = lmax_in
+        self.lmax = lmax
+        if lmax_r_ell is not None:
+            self.r_ell = ops.lowpass_filter(lmax_r_ell, lmax=lmax_in)
+            d_ell = icov_ell * self.r_ell[:lmax + 1] ** 2
+        else:
+            self.r_ell = None
+            d_ell = icov_ell.copy()
+        mask_bool = np.asarray(mask_bool)
+        if mask_bool.ndim == 1:
+            mask_bool = np.ones((self.npol, mask_bool.size), dtype=mask_bool.dtype) * mask_bool
+        elif mask_bool.shape[0] != self.npol:
+            raise ValueError(f'shape[0] mask != npol {mask_bool.shape[0]} != {self.npol}')
+        if mask_bool.dtype != bool:
+            raise ValueError(f'Input mask must be bool, got {mask_bool.dtype}')
+        mask_bool, geom = equal_area_mask(mask_bool, geom, lmax)
+        self.mask_unobs = ~mask_bool
+        self.geom = geom
+        self.g_op = pix_ell_pix_op(self.mask_unobs, d_ell, geom, spin, lmax=lmax)
+        self.prec = pix_ell_pix_op(self.mask_unobs, icov_ell, geom, spin, lmax=lmax,
+                                   power_x=-1, adjoint=True)
+        self.dot = lambda x, y: np.dot(x.reshape(-1), y.reshape(-1))
+
+    def __call__(self, alm):
+        imap = np.zeros((self.npol, self.geom.npix),
+                        dtype=np.float32 if alm.dtype == np.complex64 else np.float64)
+        alm_lo = ops.trunc_alm(alm, self.lmax_in, self.lmax)
+        if self.r_ell is not None:
+            alm_lo = ops.lmul(alm_lo, self.r_ell[:self.lmax + 1], self.lmax)
+        osht.alm2map(alm_lo, imap, self.lmax, self.geom, self.spin)
+        imap *= self.mask_unobs
+        cg = CG(self.g_op, imap, dot=self.dot, M=self.prec)
+        for _ in range(self.nsteps):
+            cg.step()
+        omap = cg.x * self.mask_unobs
+        osht.map2alm(omap, alm_lo, self.geom, self.lmax, self.spin, adjoint=True)
+        if self.r_ell is not None:
+            alm_lo = ops.lmul(alm_lo, self.r_ell[:self.lmax + 1], self.lmax)
+        out = np.zeros_like(alm)
+        return ops.add_to_alm(out, alm_lo, self.lmax_in, self.lmax)
+
+
+def prec_adef1(prec_main, prec_masked, a_op, sel_masked=None):
+    '''ADEF-1 (preconditioners.py:824-865): M2 b = M (b - A Q b) + Q b.'''
+    def f(alm):
+        if sel_masked is not None:
+            q = np.zeros_like(alm)
+            q[sel_masked] = prec_masked(alm[sel_masked].copy())
+        else:
+            q = prec_masked(alm)
+        out = prec_main(alm - a_op(q))
+        return out + q
+    return f
+
+
+def prec_adef2(prec_main, prec_masked, a_op, sel_masked=None):
+    '''ADEF-2 (preconditioners.py:867-914): M2 b = M b - Q A M b + Q b.'''
+    def f(alm):
+        m = prec_main(alm)
+        qam = a_op(m)
+        if sel_masked is not None:
+            tmp = qam[sel_masked].copy()
+            qam = np.zeros_like(qam)
+            qam[sel_masked] = prec_masked(tmp)
+        else:
+            qam = prec_masked(qam)
+        m = m - qam
+        if sel_masked is not None:
+            q = np.zeros_like(alm)
+            q[sel_masked] = prec_masked(alm[sel_masked].copy())
+        else:
+            q = prec_masked(alm)
+        return m + q
+    return f
+
+
+def get_2level_prec(prec_main, prec_masked, a_op, method, sel_masked=None):
+    '''get_2level_prec (preconditioners.py:791-822).'''
+    if method == 'ADEF-1':
+        return prec_adef1(prec_main, prec_masked, a_op, sel_masked=sel_masked)
+    if method == 'ADEF-2':
+        return prec_adef2(prec_main, prec_masked, a_op, sel_masked=sel_masked)
+    raise ValueError(f'method : {method} not supported.')

@@ -360,3 +360,167 @@ def test_multigrid_levels():
     assert len(levels) == 3
     assert [og.minfo2lmax(lv.geom) for lv in levels] == [lmax, lmax // 2, lmax // 4]
     assert levels[0].pinv_g is None and levels[2].pinv_g is not None
+
+
+# ------------------------------------------- masked CG and two-level
+
+def test_add_to_alm_golden():
+    '''test_alm_utils.py:191-245 (literal answers).'''
+    alm = np.asarray([1, 2, 3, 4, 5, 6, 7, 8, 9, 10], dtype=np.complex64)
+    blm = np.asarray([1, 2, 3, 4, 5j, 6j, 7j, 8j, 9j, 10j], dtype=np.complex64)
+    ops.add_to_alm(alm, blm, 3, 3)
+    np.testing.assert_array_equal(
+        alm, np.asarray([2, 4, 6, 8, 5 + 5j, 6 + 6j, 7 + 7j, 8 + 8j, 9 + 9j, 10 + 10j]))
+    alm = np.asarray([1, 2, 3, 4, 5, 6, 7, 8, 9, 10], dtype=np.complex64)
+    blm = np.asarray([1, 2, 3, 4j, 5j, 6j], dtype=np.complex64)
+    ops.add_to_alm(alm, blm, 3, 2)
+    np.testing.assert_array_equal(alm, np.asarray([2, 4, 6, 4, 5 + 4j, 6 + 5j, 7, 8 + 6j, 9, 10]))
+    alm = np.asarray([1, 2, 3, 4, 5, 6, 7, 8, 9, 10], dtype=np.complex64)
+    ops.add_to_alm(alm, blm, 3, 2, overwrite=True)
+    np.testing.assert_array_equal(alm, np.asarray([1, 2, 3, 4, 4j, 5j, 7, 6j, 9, 10]))
+    with pytest.raises(ValueError):
+        ops.add_to_alm(blm.copy(), alm, 2, 3)
+
+
+def _spd_icov(npol, lmax, seed):
+    rng = np.random.default_rng(seed)
+    ell = np.arange(lmax + 1)
+    icov = np.zeros((npol, npol, lmax + 1))
+    for l in range(lmax + 1):
+        b = rng.normal(size=(npol, npol)) * 0.2
+        icov[:, :, l] = (np.eye(npol) + b @ b.T) * (ell[l] + 2.0) ** 2
+    return icov
+
+
+def test_pix_ell_pix_inverse_closed_form():
+    '''On a Gauss-Legendre grid (Y^T W Y = I at the band limit) the inner
+    preconditioner P = W Y icov^-1 Y^T W of MaskedPreconditionerCG sends
+    Y icov a to W Y a. A planted mistake (weights on the wrong side, or the
+    power +1 instead of -1) fails the same check.'''
+    lmax = 12
+    g = og.gauss_legendre(lmax + 1, 2 * lmax + 1)
+    icov = _spd_icov(3, lmax, 5)
+    # no T-E/B coupling at l < 2, where the spin-2 harmonics vanish
+    icov[0, 1:, :2] = 0
+    icov[1:, 0, :2] = 0
+    a = np.random.default_rng(6).normal(size=(3, layout.nelem(lmax))) * (1 + 1j)
+    ell, em = layout.lm_arrays(lmax)
+    a[:, em == 0] = a[:, em == 0].real
+    a[1:, ell < 2] = 0
+    x = np.zeros((3, g.npix))
+    ia = ops.lmul(a, icov, lmax)
+    ia[1:, ell < 2] = 0
+    osht.alm2map(ia, x, lmax, g, [0, 2])
+    want = np.zeros((3, g.npix))
+    osht.alm2map(a, want, lmax, g, [0, 2], adjoint=True)
+    ones = np.ones((3, g.npix), dtype=bool)
+    P = ow.pix_ell_pix_op(ones, icov, g, [0, 2], lmax=lmax, power_x=-1, adjoint=True)
+    got = P(x)
+    assert np.linalg.norm(got - want) / np.linalg.norm(want) < 1e-12
+    for bad in (ow.pix_ell_pix_op(ones, icov, g, [0, 2], lmax=lmax, power_x=-1),
+                ow.pix_ell_pix_op(ones, icov, g, [0, 2], lmax=lmax, adjoint=True)):
+        assert np.linalg.norm(bad(x) - want) / np.linalg.norm(want) > 1e-3
+
+
+def _dense_masked_solve(P, alm, lmax_in):
+    '''r Y^T M_u x with x the least-squares solution of G x = M_u Y r a_lo
+    (dense G over the unobserved pixels): the converged masked CG.'''
+    mu = P.mask_unobs
+    npol, npix = mu.shape
+    idx = [(c, k) for c in range(npol) for k in np.nonzero(mu[c])[0]]
+    G = np.zeros((len(idx), len(idx)))
+    for j, (c, k) in enumerate(idx):
+        e = np.zeros((npol, npix))
+        e[c, k] = 1
+        ge = P.g_op(e)
+        G[:, j] = [ge[cc, kk] for cc, kk in idx]
+    a_lo = ops.trunc_alm(alm, lmax_in, P.lmax)
+    if P.r_ell is not None:
+        a_lo = ops.lmul(a_lo, P.r_ell[:P.lmax + 1], P.lmax)
+    imap = np.zeros((npol, npix))
+    osht.alm2map(a_lo, imap, P.lmax, P.geom, P.spin)
+    b = np.asarray([imap[c, k] for c, k in idx])
+    x = np.linalg.lstsq(G, b, rcond=1e-13)[0]
+    xm = np.zeros((npol, npix))
+    for v, (c, k) in zip(x, idx):
+        xm[c, k] = v
+    out_lo = np.zeros_like(a_lo)
+    osht.map2alm(xm, out_lo, P.geom, P.lmax, P.spin, adjoint=True)
+    if P.r_ell is not None:
+        out_lo = ops.lmul(out_lo, P.r_ell[:P.lmax + 1], P.lmax)
+    out = np.zeros_like(alm)
+    for m in range(P.lmax + 1):
+        i0 = layout.mstart(m, lmax_in)
+        j0 = layout.mstart(m, P.lmax)
+        out[..., i0 + m:i0 + P.lmax + 1] = out_lo[..., j0 + m:j0 + P.lmax + 1]
+    return out
+
+
+@pytest.mark.parametrize('spin,npol,lmax_r_ell', [(0, 1, None), ([0, 2], 3, None), ([0, 2], 3, 30)])
+def test_masked_cg_brute_force(spin, npol, lmax_r_ell):
+    '''MaskedPreconditionerCG run for more CG steps than the masked system
+    has unknowns equals the dense least-squares solve of that system (the
+    output r Y^T M_u x is the same for every solution x), with truncation
+    to the preconditioner's lmax and the low-pass filter.'''
+    lmax_in, lmax = 12, 9
+    g = og.gauss_legendre(lmax_in + 1, 2 * lmax_in + 1)
+    th = np.repeat(g.theta, g.nphi)
+    mask = th > 0.8
+    icov = _spd_icov(npol, lmax_in, 7) if npol == 3 else \
+        ((np.arange(lmax_in + 1) + 2.0) ** 2)[None]
+    P = ow.MaskedPreconditionerCG(lmax_in, icov, spin, mask, g, lmax_r_ell=lmax_r_ell,
+                                  lmax=lmax, nsteps=1)
+    nunk = int(P.mask_unobs.sum())
+    P.nsteps = nunk + 2
+    a = np.random.default_rng(8).normal(size=(npol, layout.nelem(lmax_in))) * (1 - 1j)
+    ell, em = layout.lm_arrays(lmax_in)
+    a[:, em == 0] = a[:, em == 0].real
+    if npol == 3:
+        a[1:, ell < 2] = 0
+    got = P(a)
+    ref = _dense_masked_solve(P, a, lmax_in)
+    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-9
+    hi = ell > lmax
+    assert np.all(got[:, hi] == 0)
+
+
+def test_adef_closed_forms():
+    '''Two-level preconditioners on a dense SPD system: with Q = 0 both
+    reduce to the main preconditioner M; with Q = A^-1 both give A^-1 b
+    exactly; with Q on a slice only, the other rows see M alone.'''
+    rng = np.random.default_rng(9)
+    n = 8
+    B = rng.normal(size=(n, n))
+    A = B @ B.T + n * np.eye(n)
+    C = rng.normal(size=(n, n))
+    M = C @ C.T + np.eye(n)
+    Ainv = np.linalg.inv(A)
+    a_op = lambda v: A @ v
+    m_op = lambda v: M @ v
+    b = rng.normal(size=n)
+    for method in ('ADEF-1', 'ADEF-2'):
+        f0 = ow.get_2level_prec(m_op, lambda v: 0 * v, a_op, method)
+        np.testing.assert_allclose(f0(b), M @ b, rtol=1e-13)
+        fi = ow.get_2level_prec(m_op, lambda v: Ainv @ v, a_op, method)
+        np.testing.assert_allclose(fi(b), Ainv @ b, rtol=1e-10, atol=1e-12)
+    # ADEF-2 with a block-diagonal A and Q = (A^-1 on the first block)
+    A2 = A.copy()
+    A2[:4, 4:] = 0
+    A2[4:, :4] = 0
+    Q1 = np.linalg.inv(A2[:4, :4])
+    a2 = lambda v: A2 @ v
+    for method in ('ADEF-1', 'ADEF-2'):
+        f = ow.get_2level_prec(m_op, lambda v: Q1 @ v, a2, method, sel_masked=np.s_[:4])
+        out = f(b)
+        if method == 'ADEF-2':
+            mb = M @ b
+            am = A2 @ mb
+            want = mb.copy()
+            want[:4] += Q1 @ b[:4] - Q1 @ am[:4]
+        else:
+            q = np.zeros(n)
+            q[:4] = Q1 @ b[:4]
+            want = M @ (b - A2 @ q) + q
+        np.testing.assert_allclose(out, want, rtol=1e-12)
+    with pytest.raises(ValueError):
+        ow.get_2level_prec(m_op, m_op, a_op, 'ADEF-3')
