@@ -286,6 +286,80 @@ def streaming_bench(args, dev):
         print(json.dumps({'streaming': ln}))
 
 
+def phase1_bench(args, dev):
+    """Phase 1 of the reference's two-phase filter (filters.py:138-143,
+    221-229): CG with the pseudo-inverse plus MaskedPreconditionerCG (15
+    inner steps, no low-pass, band limit --prec-cg-lmax, mask of T) on the
+    TQU workload. Times the reference's schedule of 5 float64 iterations of
+    one right-hand side, one masked-CG application in float64 (one
+    right-hand side) and in float32 (a batch), and reports the true
+    relative residual after the 5 iterations with and without the masked CG
+    preconditioner. Prints one JSON line."""
+    prob = workloads.polarized(args.lmax, seed=0, device=dev)
+    lmax_cg = args.prec_cg_lmax or args.lmax
+    mask = np.asarray(prob.mask_bool)[0].astype(bool)
+    t0 = time.perf_counter()
+    mcg = preconditioners.MaskedPreconditionerCG(prob.ainfo, prob.icov_ell, prob.spin, mask,
+                                                 prob.minfo, lmax=lmax_cg, nsteps=15,
+                                                 lmax_r_ell=None)
+    pinv = preconditioners.PseudoInvPreconditioner(prob.ainfo, prob.icov_ell, prob.icov_pix,
+                                                   prob.minfo, prob.spin)
+    log(f'masked CG set-up {time.perf_counter() - t0:.1f}s (lmax {lmax_cg}, '
+        f'{int(mcg.mask_unobs[0].sum())} unobserved pixels of {mcg.minfo.npix})')
+    niter = 5
+    res = {}
+    times = {}
+    # first applications build the plans of the equal-area geometry
+    mcg(solvers.CGWienerMap.from_arrays(prob.imap, prob.minfo, prob.ainfo, prob.icov_ell,
+                                        prob.icov_pix, device=dev).b0)
+    for name, precs in (('pinv+masked_cg', (pinv, mcg)), ('pinv', (pinv,))):
+        solver = solvers.CGWienerMap.from_arrays(prob.imap, prob.minfo, prob.ainfo,
+                                                 prob.icov_ell, prob.icov_pix, device=dev)
+        for pr in precs:
+            solver.add_preconditioner(pr)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        solver.init_solver()
+        for _ in range(niter):
+            solver.step()
+        torch.cuda.synchronize()
+        times[name] = time.perf_counter() - t1
+        res[name] = float(true_residual(solver, solver.x))
+        log(f'{name}: {niter} float64 iterations (with the initial preconditioner application) '
+            f'{times[name]:.3f}s, true residual {res[name]:.3e}')
+    b64 = solver.b0
+    mcg(b64)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    mcg(b64)
+    torch.cuda.synchronize()
+    t_app64 = time.perf_counter() - t1
+    data = workloads.batch_data(prob, args.batch, seed=0, device=dev)
+    bsolver = solvers.CGWienerMap.from_arrays(data, prob.minfo, prob.ainfo, prob.icov_ell,
+                                              prob.icov_pix, device=dev)
+    b32 = bsolver.b0.to(torch.complex64)
+    mcg(b32)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    mcg(b32)
+    torch.cuda.synchronize()
+    t_app32 = time.perf_counter() - t1
+    del bsolver, data, b32
+    log(f'masked CG application: float64 x1 {t_app64 * 1e3:.1f} ms, float32 x{args.batch} '
+        f'{t_app32 * 1e3:.1f} ms ({t_app32 * 1e3 / args.batch:.1f} ms per right-hand side)')
+    out = {'phase1': {
+        'workload': f'TQU lmax {args.lmax}, GL {args.lmax + 1}x{2 * args.lmax + 1}, pinv + '
+                    f'MaskedPreconditionerCG (lmax {lmax_cg}, 15 inner steps)',
+        'iterations': niter,
+        'fp64_seconds_per_iteration': times['pinv+masked_cg'] / niter,
+        'fp64_pinv_only_seconds_per_iteration': times['pinv'] / niter,
+        'true_residual_after': res['pinv+masked_cg'],
+        'true_residual_after_pinv_only': res['pinv'],
+        'masked_cg_application_ms_fp64_x1': t_app64 * 1e3,
+        f'masked_cg_application_ms_fp32_x{args.batch}': t_app32 * 1e3}}
+    print(json.dumps(out))
+
+
 def ragged_bench(args, dev):
     '''Constrained realisations: per band limit one batched mixed-precision
     solve of `batch` right-hand sides (data + S^-1/2 w + P^T N^-1/2 w', seeds
@@ -359,6 +433,11 @@ def main():
                     help='time the HBM-bound kernels of the CG iteration at the headline shapes '
                          '(lmul, pixel diagonal, dots, CG updates, conversion); prints one '
                          'JSON line per kernel')
+    ap.add_argument('--phase1', action='store_true',
+                    help='time phase 1 of the two-phase filter (pseudo-inverse plus the masked '
+                         'CG preconditioner) on the TQU workload; prints its own JSON line')
+    ap.add_argument('--prec-cg-lmax', type=int, default=0,
+                    help='band limit of the masked CG preconditioner (default: --lmax)')
     ap.add_argument('--roofline-only', action='store_true',
                     help='only build the workload geometry and time the roofline kernel '
                          '(for counter runs), after one copy of known size')
@@ -377,6 +456,9 @@ def main():
         wavelet_bench(args, dev)
         return
 
+    if args.phase1:
+        phase1_bench(args, dev)
+        return
     if args.streaming:
         calibration_copy(dev)
         streaming_bench(args, dev)
