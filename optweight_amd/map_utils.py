@@ -4,7 +4,15 @@ map_utils: MapInfo (map_utils.py:19-297), map_info_gauss_legendre (150-185),
 get_mtype (299-347), get_fullsky_geometry (349-413), view_2d/view_1d
 (415-475), gauss2map (719-785), get_minfo/get_gauss_minfo (819-941),
 get_equal_area_minfo (943-992), rand_map_pix (1084-1122), inv_qweight_map
-(1124-1170), get_isotropic_ivar (1172-1218), minfo2lmax (1795-1811).
+(1124-1170), get_isotropic_ivar (1172-1218), minfo2lmax (1795-1811),
+map_info_clenshaw_curtis / map_info_fejer1 (187-257), threshold_icov
+(1297-1363), match_enmap_minfo (1490-1591).
+
+The ring weights of the Clenshaw-Curtis and Fejer-1 grids are those that
+ducc0's get_gridweights gives optweight (ducc0 >= 0.31; not used here),
+computed from Waldvogel's closed forms (BIT Numer. Math. 46, 2006). CAR
+geometries are read from any object with astropy's ``wcs.wcs.cdelt / crpix /
+crval`` attributes (a pixell ndmap's ``.wcs``, or CarWCS below).
 
 Geometry and setup quantities are numpy on the host; map-space operations
 on device tensors go through the HIP library.
@@ -85,6 +93,46 @@ class MapInfo:
         offsets = np.arange(theta.size) * stride_lat if stride_lat is not None else None
         return cls(theta, weight, nphi=nphi, phi0=phi0, offsets=offsets, stride=stride_lon)
 
+    @classmethod
+    def map_info_clenshaw_curtis(cls, nrings, nphi, phi0=0, stride_lon=None, stride_lat=None):
+        theta = np.linspace(0, np.pi, num=nrings, endpoint=True)
+        weight = get_gridweights('CC', nrings) / nphi
+        offsets = np.arange(theta.size) * stride_lat if stride_lat is not None else None
+        return cls(theta, weight, nphi=nphi, phi0=phi0, offsets=offsets, stride=stride_lon)
+
+    @classmethod
+    def map_info_fejer1(cls, nrings, nphi, phi0=0, stride_lon=None, stride_lat=None):
+        theta = np.linspace(0, np.pi, num=nrings + 1, endpoint=True)[:-1] + np.pi / nrings / 2
+        weight = get_gridweights('F1', nrings) / nphi
+        offsets = np.arange(theta.size) * stride_lat if stride_lat is not None else None
+        return cls(theta, weight, nphi=nphi, phi0=phi0, offsets=offsets, stride=stride_lon)
+
+
+def get_gridweights(kind, ntheta):
+    '''Ring weights summing to 4 pi: 'CC' (Clenshaw-Curtis, ntheta rings at
+    theta_k = k pi / (ntheta - 1), both poles included) or 'F1' (Fejer's
+    first rule, theta_k = (k + 1/2) pi / ntheta): the rule's weights on
+    x = cos(theta) times 2 pi.'''
+    k = np.arange(ntheta)
+    if kind == 'CC':
+        n = ntheta - 1
+        if n < 1:
+            raise ValueError('Clenshaw-Curtis needs at least 2 rings')
+        w = np.ones(ntheta)
+        for j in range(1, n // 2 + 1):
+            b = 1.0 if 2 * j == n else 2.0
+            w -= b / (4.0 * j * j - 1) * np.cos(2 * j * k * np.pi / n)
+        c = np.full(ntheta, 2.0)
+        c[0] = c[-1] = 1.0
+        return 2 * np.pi * c / n * w
+    if kind == 'F1':
+        th = (2 * k + 1) * np.pi / (2 * ntheta)
+        w = np.ones(ntheta)
+        for j in range(1, ntheta // 2 + 1):
+            w -= 2 * np.cos(2 * j * th) / (4.0 * j * j - 1)
+        return 2 * np.pi * 2.0 / ntheta * w
+    raise ValueError(f'geometry {kind} not supported')
+
 
 def get_gauss_minfo(lmax, theta_min=None, theta_max=None):
     '''GL grid that supports band limit lmax: floor(lmax/2)+1 rings of
@@ -124,11 +172,16 @@ def get_fullsky_geometry(minfo):
     if not all(minfo.nphi == nphi):
         raise ValueError('Geometries with varying nphi not supported')
     mtype = get_mtype(minfo)
-    if mtype != 'GL':
-        raise ValueError(f'{mtype=} not supported')
     diff = np.diff(minfo.theta)
-    nrings = int(np.round(np.min(np.pi / np.abs(diff) - 0.5)))
-    out = MapInfo.map_info_gauss_legendre(nrings, nphi)
+    if mtype == 'GL':
+        nrings = int(np.round(np.min(np.pi / np.abs(diff) - 0.5)))
+        out = MapInfo.map_info_gauss_legendre(nrings, nphi)
+    elif mtype == 'CC':
+        out = MapInfo.map_info_clenshaw_curtis(int(np.round(np.pi / np.abs(diff[0]) + 1)), nphi)
+    elif mtype == 'F1':
+        out = MapInfo.map_info_fejer1(int(np.round(np.pi / np.abs(diff[0]))), nphi)
+    else:
+        raise ValueError(f'{mtype=} not supported')
     ts = np.sort(minfo.theta)
     tf = np.sort(out.theta)
     idx = np.argwhere(np.isclose(tf, ts[0]))
@@ -160,6 +213,117 @@ def get_equal_area_minfo(minfo, orig_band=np.pi / 8, ratio_pow=1.):
         nphi_r[t] = max(2, int(np.round(minfo.nphi[t] * ratio)))
         w_r[t] = area * minfo.nphi[t] / nphi_r[t]
     return MapInfo(minfo.theta, nphi=nphi_r, weight=w_r)
+
+
+class CarWCS:
+    '''The attributes of an astropy WCS that a cylindrical CAR geometry
+    needs (``.wcs.cdelt``, ``.wcs.crpix``, ``.wcs.crval``, degrees and FITS
+    1-based reference pixel), for callers without astropy.'''
+
+    class _Params:
+        def __init__(self, cdelt, crpix, crval):
+            self.cdelt = np.array(cdelt, dtype=np.float64)
+            self.crpix = np.array(crpix, dtype=np.float64)
+            self.crval = np.array(crval, dtype=np.float64)
+
+    def __init__(self, cdelt, crpix, crval=(0., 0.)):
+        self.wcs = CarWCS._Params(cdelt, crpix, crval)
+
+
+def car_pix2sky(wcs, y, x):
+    '''(dec, ra) in radians of pixels (y, x) of a cylindrical CAR map
+    (crval[1] = 0): ra = crval0 + (x + 1 - crpix0) cdelt0 and
+    dec = crval1 + (y + 1 - crpix1) cdelt1 in degrees.'''
+    w = wcs.wcs
+    if abs(float(w.crval[1])) > 1e-12:
+        raise NotImplementedError('only cylindrical CAR (crval[1] = 0) is supported')
+    ra = w.crval[0] + (np.asarray(x, dtype=np.float64) + 1 - w.crpix[0]) * w.cdelt[0]
+    dec = w.crval[1] + (np.asarray(y, dtype=np.float64) + 1 - w.crpix[1]) * w.cdelt[1]
+    return np.radians(dec), np.radians(ra)
+
+
+def match_enmap_minfo(shape, wcs, mtype='CC'):
+    '''MapInfo of a CAR map of shape (..., ny, nx) without moving its
+    pixels: ring r is map row r, its pixel j sits at offsets[r] + j stride[r]
+    of the flattened map (stride -1 when RA decreases with x) at longitude
+    phi0 + 2 pi j / nx. Raises ValueError if the map does not span the full
+    RA range or its rows are not rings of the CC / fejer1 grid.'''
+    if mtype not in ('CC', 'fejer1'):
+        raise NotImplementedError('Only "CC" or "fejer1" `mtype` implemented for now.')
+    yo = 1 if mtype == 'CC' else 0
+    cdelt = wcs.wcs.cdelt
+    nx = int(np.round(np.abs(360 / cdelt[0])))
+    if nx != shape[-1]:
+        raise ValueError(f'Input needs to span full range in phi, {shape[-1]}, {nx} ')
+    ny = int(np.round(abs(180 / cdelt[1]) + yo))
+    phi0 = float(car_pix2sky(wcs, 0, 0)[1])
+    if mtype == 'CC':
+        full = MapInfo.map_info_clenshaw_curtis(ny, nphi=nx)
+    else:
+        full = MapInfo.map_info_fejer1(ny, nphi=nx)
+    ntheta = shape[-2]
+    theta = np.pi / 2 - car_pix2sky(wcs, np.arange(ntheta), np.zeros(ntheta))[0]
+    atol = 1e-6
+    i0 = np.flatnonzero(np.abs(theta[0] - full.theta) < atol)
+    i1 = np.flatnonzero(np.abs(theta[-1] - full.theta) < atol)
+    if i0.size != 1 or i1.size != 1:
+        raise ValueError(f'Input geometry is not {mtype} at atol = {atol}')
+    keep = np.s_[min(i0[0], i1[0]):max(i0[0], i1[0]) + 1]
+    theta_k = full.theta[keep]
+    weight_k = full.weight[keep]
+    offsets = np.arange(theta_k.size) * nx
+    if cdelt[0] < 0:
+        offsets = offsets + nx - 1
+        phi0 -= np.radians(cdelt[0])
+    stride = np.full(theta_k.size, 1 if cdelt[0] > 0 else -1)
+    if cdelt[1] > 0:
+        theta_k = np.ascontiguousarray(theta_k[::-1])
+        weight_k = np.ascontiguousarray(weight_k[::-1])
+    return MapInfo(theta_k, weight_k, nphi=nx, phi0=phi0, offsets=offsets, stride=stride)
+
+
+def unit_stride(minfo):
+    '''(minfo_c, index): the same rings, in the same order, stored one after
+    another with stride 1, and the int64 index with map_c = map[..., index]
+    (map_c[..., index] = map_c scatters back). The transforms run on
+    unit-stride geometries.'''
+    nphi = minfo.nphi
+    off_c = np.concatenate([[0], np.cumsum(nphi[:-1])]).astype(np.int64)
+    index = np.concatenate([minfo.offsets[r] + minfo.stride[r] * np.arange(nphi[r])
+                            for r in range(minfo.nrow)]).astype(np.int64)
+    return MapInfo(minfo.theta, minfo.weight, nphi=nphi, phi0=minfo.phi0, offsets=off_c), index
+
+
+def threshold_icov(icov_pix, q_low=0, q_high=1):
+    '''Copy of icov_pix (npol, npix) or (npol, npol, npix) whose diagonal is
+    clipped per component to its [q_low, q_high] quantiles over the nonzero
+    pixels; a 3d matrix is rescaled by sqrt of the clipping factor on both
+    sides, keeping it symmetric.'''
+    icov_pix = np.asarray(icov_pix)
+    ndim = icov_pix.ndim
+    if ndim not in (2, 3):
+        raise ValueError(f'{icov_pix.ndim=}, expected 2 or 3')
+    npol = icov_pix.shape[0]
+    q_low = q_low * np.ones(npol)
+    q_high = q_high * np.ones(npol)
+    scaling = np.ones((npol, icov_pix.shape[-1]))
+    for pidx in range(npol):
+        diag = icov_pix[pidx, pidx] if ndim == 3 else icov_pix[pidx]
+        nonzero = diag > 0
+        if np.sum(nonzero) == 0:
+            continue
+        val_low = np.quantile(diag[nonzero], q_low[pidx])
+        val_high = np.quantile(diag[nonzero], q_high[pidx])
+        low = (diag < val_low) & nonzero
+        high = (diag > val_high) & nonzero
+        scaling[pidx][low] = val_low / diag[low]
+        scaling[pidx][high] = val_high / diag[high]
+    if ndim == 2:
+        out = icov_pix * scaling
+    else:
+        root = np.sqrt(scaling)
+        out = root[:, np.newaxis, :] * icov_pix * root[np.newaxis, :, :]
+    return out.astype(icov_pix.dtype, copy=False)
 
 
 def minfo2lmax(minfo):

@@ -10,6 +10,7 @@ Streaming and setup operations (oracle), restating:
   inv_qweight_map / get_isotropic_ivar    map_utils.py:1124-1218
   gauss_beam (healpy)                     noise_utils.py:401-421
   lowpass_filter                          multigrid.py:449-471
+  threshold_icov                          map_utils.py:1297-1363
 '''
 import numpy as np
 
@@ -228,3 +229,33 @@ def lowpass_filter(lmax_r_ell, lmax=None):
     beta = -np.log(np.sqrt(0.05)) / ((lmax_r_ell / 2) * (lmax_r_ell / 2 + 1)) ** 2
     ells = np.arange(lmax + 1)
     return np.exp(-beta * (ells * (ells + 1)) ** 2.)
+
+
+def threshold_icov(icov_pix, q_low=0, q_high=1):
+    '''Clip the diagonal of icov_pix (npol, npix) or (npol, npol, npix) per
+    component to its [q_low, q_high] quantiles over the nonzero pixels; a 3d
+    matrix is rescaled symmetrically by sqrt of the per-pixel factors.'''
+    icov_pix = np.asarray(icov_pix)
+    if icov_pix.ndim not in (2, 3):
+        raise ValueError(f'{icov_pix.ndim=}, expected 2 or 3')
+    npol = icov_pix.shape[0]
+    q_low = q_low * np.ones(npol)
+    q_high = q_high * np.ones(npol)
+    scale = np.ones((npol, icov_pix.shape[-1]))
+    for p in range(npol):
+        d = icov_pix[p, p] if icov_pix.ndim == 3 else icov_pix[p]
+        nz = d > 0
+        if not np.any(nz):
+            continue
+        lo = np.quantile(d[nz], q_low[p])
+        hi = np.quantile(d[nz], q_high[p])
+        ml = (d < lo) & nz
+        mh = (d > hi) & nz
+        scale[p][ml] = lo / d[ml]
+        scale[p][mh] = hi / d[mh]
+    if icov_pix.ndim == 2:
+        out = icov_pix * scale
+    else:
+        r = np.sqrt(scale)
+        out = r[:, None, :] * icov_pix * r[None, :, :]
+    return out.astype(icov_pix.dtype, copy=False)

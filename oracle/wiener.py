@@ -581,3 +581,102 @@ def get_2level_prec(prec_main, prec_masked, a_op, method, sel_masked=None):
     if method == 'ADEF-2':
         return prec_adef2(prec_main, prec_masked, a_op, sel_masked=sel_masked)
     raise ValueError(f'method : {method} not supported.')
+
+
+class CGPixFilter:
+    '''filters.CGPixFilter (filters.py:9-301) on a CAR map given by its WCS:
+    TT / TE / EE / BB signal, beam, pixel inverse covariance and mask;
+    phase 1 with the pseudo-inverse plus MaskedPreconditionerCG for
+    niter_masked_cg steps, then the pseudo-inverse plus the T multigrid
+    restarted from the phase-1 solution. The pixels are gathered once into
+    the unit-stride ring order of the map's geometry. scale_a is not
+    restated.'''
+
+    def __init__(self, theory_cls, b_ell, icov_pix, mask_bool, wcs, include_te=True, q_low=0,
+                 q_high=1, swap_bm=False, lmax=None, lmax_prec_cg=None,
+                 no_masked_prec_mg=False, no_masked_prec_cg=False):
+        b_ell = np.asarray(b_ell)
+        icov_pix = np.asarray(icov_pix)
+        if not np.all(np.isfinite(b_ell)) or not np.all(np.isfinite(icov_pix)):
+            raise ValueError('non-finite input')
+        self.shape_in = icov_pix.shape[-2:]
+        icov_pix = icov_pix.reshape((-1,) * (3 - icov_pix.ndim) + icov_pix.shape) \
+            if icov_pix.ndim < 3 else icov_pix
+        mask_bool = np.asarray(mask_bool).astype(bool)
+        mask_bool = mask_bool.reshape((1,) * (3 - mask_bool.ndim) + mask_bool.shape)
+        ncomp = icov_pix.shape[0]
+        if mask_bool.shape[0] == 1:
+            mask_bool = (np.ones(ncomp)[:, None, None] * mask_bool).astype(bool)
+        for mtype in ('CC', 'fejer1'):
+            try:
+                _, geom, index = geometry.match_car(icov_pix.shape, wcs, mtype=mtype)
+            except ValueError:
+                continue
+            break
+        self.geom, self.index = geom, index
+        if lmax is None:
+            lmax = geometry.minfo2lmax(geom)
+        icov_pix = icov_pix.reshape(icov_pix.shape[:-2] + (-1,))[..., index]
+        mask_bool = mask_bool.reshape(mask_bool.shape[:-2] + (-1,))[..., index]
+        if q_low != 0 or q_high != 1:
+            icov_pix = ops.threshold_icov(icov_pix, q_low=q_low, q_high=q_high)
+        if not theory_cls['TT'].size - 1 >= lmax:
+            raise ValueError('theory_cls shorter than lmax')
+        cov_ell = np.zeros((ncomp, ncomp, lmax + 1))
+        cov_ell[0, 0] = theory_cls['TT'][:lmax + 1]
+        if ncomp > 1:
+            if include_te:
+                cov_ell[0, 1] = theory_cls['TE'][:lmax + 1]
+                cov_ell[1, 0] = theory_cls['TE'][:lmax + 1]
+            cov_ell[1, 1] = theory_cls['EE'][:lmax + 1]
+            cov_ell[2, 2] = theory_cls['BB'][:lmax + 1]
+        icov_ell = np.zeros_like(cov_ell)
+        for l in range(icov_ell.shape[-1]):
+            icov_ell[:, :, l] = np.linalg.pinv(cov_ell[:, :, l])
+        if b_ell.ndim == 1:
+            b_ell = b_ell[None] * np.asarray((1, 1, 1)[:ncomp])[:, None]
+        elif b_ell.ndim != 2 or b_ell.shape[0] != ncomp:
+            raise ValueError('b_ell shape does not match ncomp')
+        b_ell = np.ascontiguousarray(b_ell[:, :lmax + 1])
+        spin = 0 if ncomp == 1 else [0, 2]
+        self.prec_pinv = PseudoInvPreconditioner(lmax, icov_ell, icov_pix, geom, spin,
+                                                 b_ell=b_ell)
+        self.prec_masked_cg = None if no_masked_prec_cg else MaskedPreconditionerCG(
+            lmax, icov_ell, spin, mask_bool[0], geom,
+            lmax=lmax_prec_cg if lmax_prec_cg else lmax, nsteps=15, lmax_r_ell=None)
+        self.prec_masked_mg = None if no_masked_prec_mg else MaskedPreconditioner(
+            lmax, icov_ell[0:1, 0:1], 0, mask_bool[0], geom, min_pix=1000, n_jacobi=1,
+            lmax_r_ell=6000)
+        self.icov_ell, self.icov_pix, self.mask_bool = icov_ell, icov_pix, mask_bool
+        self.b_ell, self.ncomp, self.swap_bm, self.lmax, self.spin = \
+            b_ell, ncomp, swap_bm, lmax, spin
+
+    def filter(self, imap, niter=15, niter_masked_cg=5, err_tol=1e-15):
+        imap = np.asarray(imap, dtype=self.icov_pix.dtype)
+        if imap.shape != (self.ncomp,) + tuple(self.shape_in):
+            raise ValueError('imap shape does not match the filter geometry')
+        imap = imap.reshape(self.ncomp, -1)[..., self.index]
+        sol = WienerMap.from_arrays(imap, self.geom, self.lmax, self.icov_ell, self.icov_pix,
+                                    b_ell=self.b_ell, mask_pix=self.mask_bool,
+                                    swap_bm=self.swap_bm, spin=self.spin)
+        sol.add_preconditioner(self.prec_pinv)
+        if self.prec_masked_cg is not None:
+            sol.add_preconditioner(self.prec_masked_cg)
+        cg = sol.init_solver()
+        errors = [np.nan]
+        for idx in range(niter_masked_cg + niter):
+            if idx == niter_masked_cg:
+                sol.reset_preconditioner()
+                sol.add_preconditioner(self.prec_pinv)
+                if self.prec_masked_mg is not None:
+                    sol.add_preconditioner(self.prec_masked_mg, sel=np.s_[0])
+                sol.b_vec = sol.b0
+                cg = sol.init_solver(x0=cg.x)
+            cg.step()
+            if idx >= niter_masked_cg and niter_masked_cg > 0:
+                errors.append(cg.err * errors[niter_masked_cg - 1])
+            else:
+                errors.append(cg.err)
+            if cg.err < err_tol:
+                break
+        return {'walm': sol.get_wiener(), 'ialm': sol.get_icov(), 'errors': errors}
