@@ -360,6 +360,37 @@ def phase1_bench(args, dev):
     print(json.dumps(out))
 
 
+def filter_bench(args, dev):
+    """The two-phase CAR filter of the reference (filters.CGPixFilter,
+    filters.py:9-301) on a full-sky Clenshaw-Curtis CAR map of
+    (lmax + 2) x 2 (lmax + 1) pixels, TQU at band limit lmax (config-2
+    spectra, depth and mask models, 10 arcmin beam): set-up, then the
+    default schedule of 5 phase-1 (pinv + masked CG) and 15 phase-2 (pinv +
+    multigrid on T) float64 iterations of one map. Prints one JSON line."""
+    from optweight_amd import filters
+    ny, nx = args.lmax + 2, 2 * (args.lmax + 1)
+    t0 = time.perf_counter()
+    wl = workloads.car_polarized(ny, nx, args.lmax, seed=0, device=dev)
+    t1 = time.perf_counter()
+    f = filters.CGPixFilter(wl['theory_cls'], wl['b_ell'], wl['icov_pix'], wl['mask'],
+                            wcs=wl['wcs'], lmax=args.lmax, device=dev)
+    f.filter(wl['imap'], niter=1, niter_masked_cg=1, verbose=False)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    log(f'CAR workload {ny}x{nx} built in {t1 - t0:.1f}s, filter set-up and warm-up '
+        f'{t2 - t1:.1f}s')
+    out = f.filter(wl['imap'], niter=15, niter_masked_cg=5, verbose=False)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t2
+    errs = [float(e) for e in out['errors'][1:]]
+    log(f'filter: 5 + 15 iterations in {el:.3f}s, errors after phase 1 {errs[4]:.3e}, '
+        f'final {errs[-1]:.3e}')
+    print(json.dumps({'filter': {
+        'workload': f'CGPixFilter, TQU, CC CAR {ny}x{nx}, lmax {args.lmax}, 10 arcmin beam, '
+                    'pinv + masked CG (5 steps) then pinv + multigrid T (15 steps), float64',
+        'seconds': el, 'iterations': len(errs), 'errors': errs}}))
+
+
 def ragged_bench(args, dev):
     '''Constrained realisations: per band limit one batched mixed-precision
     solve of `batch` right-hand sides (data + S^-1/2 w + P^T N^-1/2 w', seeds
@@ -433,6 +464,9 @@ def main():
                     help='time the HBM-bound kernels of the CG iteration at the headline shapes '
                          '(lmul, pixel diagonal, dots, CG updates, conversion); prints one '
                          'JSON line per kernel')
+    ap.add_argument('--filter', action='store_true',
+                    help='time the two-phase CAR filter (CGPixFilter) on a full-sky CC map at '
+                         '--lmax; prints its own JSON line')
     ap.add_argument('--phase1', action='store_true',
                     help='time phase 1 of the two-phase filter (pseudo-inverse plus the masked '
                          'CG preconditioner) on the TQU workload; prints its own JSON line')
@@ -458,6 +492,9 @@ def main():
 
     if args.phase1:
         phase1_bench(args, dev)
+        return
+    if args.filter:
+        filter_bench(args, dev)
         return
     if args.streaming:
         calibration_copy(dev)

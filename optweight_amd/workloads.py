@@ -186,6 +186,46 @@ def polarized(lmax, seed=0, sigma=10., ndisk=50, synth=None, device=None):
     return Problem(lmax, minfo, icov_ell, icov_pix, imap, np.stack([obs] * 3), [0, 2], cov, a)
 
 
+def car_polarized(ny, nx, lmax, seed=0, sigma=10., ndisk=50, fwhm_arcmin=10., device=None):
+    """Inputs of the two-phase CAR filter (filters.CGPixFilter): a full-sky
+    Clenshaw-Curtis CAR map of ny rows and nx columns in pixell's convention
+    (RA decreasing with x, dec increasing with y), the TQU spectra of
+    cov_ell_TQU as theory spectra, a Gaussian beam, the depth and mask models
+    of ``polarized`` on the map's rings, beam-convolved signal plus white
+    noise. Returns a dict with theory_cls, b_ell, icov_pix (3, ny, nx),
+    mask (ny, nx), imap (3, ny, nx), wcs and lmax."""
+    wcs = map_utils.CarWCS([-360. / nx, 180. / (ny - 1)], [nx / 2 + 1, (ny + 1) / 2])
+    minfo, index = map_utils.unit_stride(map_utils.match_enmap_minfo((ny, nx), wcs, mtype='CC'))
+    cov = cov_ell_TQU(lmax)
+    cls = {'TT': cov[0, 0].copy(), 'EE': cov[1, 1].copy(), 'BB': cov[2, 2].copy(),
+           'TE': cov[0, 1].copy()}
+    ell = np.arange(lmax + 1)
+    sig_b = np.radians(fwhm_arcmin / 60.) / np.sqrt(8 * np.log(2))
+    b_ell = np.exp(-0.5 * ell * (ell + 1) * sig_b ** 2)
+    ss = np.random.SeedSequence([4, seed])
+    s_alm, s_noise, s_mask, s_depth = [np.random.default_rng(s) for s in ss.spawn(4)]
+    a = alm_utils.rand_alm(cov, alm_utils.AlmInfo(lmax), s_alm)
+    a = a * b_ell[alm_utils.AlmInfo(lmax).lm_arrays()[0]]
+    dev = device or torch.device('cuda', torch.cuda.current_device())
+    sig = _synth_device(a, minfo, lmax, [0, 2], dev)
+    obs = make_mask(minfo, s_mask, ndisk=ndisk)
+    depth = depth_map(minfo, s_depth)
+    w = np.repeat(minfo.weight, minfo.nphi)
+    icov_t = depth * w / sigma ** 2 * obs
+    icov_c = np.stack([icov_t, icov_t / 2, icov_t / 2])
+    sig_pix = np.stack([sigma / np.sqrt(depth), sigma * np.sqrt(2 / depth),
+                        sigma * np.sqrt(2 / depth)])
+    imap_c = (sig + s_noise.standard_normal((3, minfo.npix)) * sig_pix) * obs
+
+    def to_map(c):
+        out = np.zeros(c.shape[:-1] + (ny * nx,), dtype=c.dtype)
+        out[..., index] = c
+        return out.reshape(c.shape[:-1] + (ny, nx))
+
+    return {'theory_cls': cls, 'b_ell': b_ell, 'icov_pix': to_map(icov_c),
+            'mask': to_map(obs), 'imap': to_map(imap_c), 'wcs': wcs, 'lmax': lmax}
+
+
 def batch_data(prob, nbatch, seed=0, synth=None, device=None):
     """(nbatch, npol, npix) data maps of the problem's signal and noise
     model: right-hand side i has its own signal draw from prob.cov_ell and
