@@ -147,12 +147,12 @@ def legendre_roofline(prob, dev, batch, reps=10):
     return {'kernel': kernel, 'ms': ms, 'flops': flops, 'achieved': achieved}
 
 
-# HBM bytes per launch of the roofline kernel (adj_mfma_kernel<2, 8>, 16
+# HBM bytes per launch of the roofline kernel (adj_mfma_kernel<2, 8>, 64
 # right-hand sides): FETCH_SIZE (x 2.0) + WRITE_SIZE (x 1.0), factors from a
 # 256 MiB copy in the same process, measured with `rocprofv3 --pmc` on
 # `python bench.py --roofline-only`
-# (profiles/adj_mfma2_f32_tqu1023_b16_roofline.md): 0.337 GB + 0.136 GB.
-ROOFLINE_TRAFFIC_BYTES = 4.731e8
+# (profiles/adj_mfma2_f32_tqu1023_b64_roofline.md): 1.348 GB + 0.544 GB.
+ROOFLINE_TRAFFIC_BYTES = 1.892e9
 
 
 def cpu_baseline_sample(dev, lmax_sample=255, reps=3):
@@ -213,7 +213,7 @@ def wavelet_bench(args, dev):
 
 def streaming_bench(args, dev):
     """HBM-bound kernels of the CG iteration at the headline shapes (TQU,
-    lmax 1023, GL 1024x2047, 16 right-hand sides in float32; one right-hand
+    lmax 1023, GL 1024x2047, --batch right-hand sides in float32; one right-hand
     side in float64), each launched alone and timed with HIP events on its
     stream. Algorithmic bytes per launch: every array read and written once.
     Prints one JSON line per kernel and one summary line."""
@@ -446,7 +446,7 @@ def main():
     ap.add_argument('--lmax', type=int, default=1023)
     ap.add_argument('--no-mg', action='store_true')
     ap.add_argument('--cpu-baseline', choices=['on', 'off'], default='on')
-    ap.add_argument('--batch', type=int, default=16,
+    ap.add_argument('--batch', type=int, default=64,
                     help='right-hand sides per mixed-precision batched solve')
     ap.add_argument('--delta', type=float, default=0.001,
                     help='residual reduction between float64 residual replacements')
@@ -593,7 +593,7 @@ def main():
                      'frac': roof['achieved'] / PEAK_FP32_TFLOPS,
                      # counters were read at the default shape only
                      'traffic': (ROOFLINE_TRAFFIC_BYTES
-                                 if (args.lmax, args.batch) == (1023, 16) else None),
+                                 if (args.lmax, args.batch) == (1023, 64) else None),
                      'kernel': roof['kernel'], 'kernel_ms': roof['ms'],
                      'flops_per_launch': roof['flops']},
         'cpu_baseline': cpu,
