@@ -156,6 +156,9 @@ def solve_mixed(a_matrix, b, precond=None, tol=1e-6, delta=0.1, maxiter=1000,
     target = tol * tol * bn2
     r = _convert(b3, low)
     p = m_lo(r)
+    if p.data_ptr() == r.data_ptr():
+        # a preconditioner that returns its input: p is updated in place
+        p = p.clone()
     rz = dot(r, p)
     # CG scalars on the device, updated in place
     pq = torch.empty((nb,), dtype=torch.float64, device=dev)
@@ -229,7 +232,8 @@ def solve_fp64(a_matrix, b, precond=None, tol=1e-6, maxiter=1000, verify=False):
     r = b3.clone()
     z = precond(r)
     rz = dot(r, z)
-    p = z
+    # p is updated in place: never as the same array as r
+    p = z.clone() if z.data_ptr() == r.data_ptr() else z
     rr = torch.empty((nb,), dtype=torch.float64, device=b3.device)
     active = np.ones(nb, dtype=bool)
     it_rhs = np.zeros(nb, dtype=np.int64)

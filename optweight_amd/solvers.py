@@ -93,6 +93,7 @@ class CG:
         self._err = next_rz / self.rz0
         if dev_scalars and isinstance(next_rz, torch.Tensor) and _is_fast(z) \
                 and _is_fast(self.p) and z.shape == self.p.shape:
+            self._unalias_p(z)
             _lib.call('owmi_cg_p', _lib.dtype_code(self.p), _lib.ptr(self.p), _lib.ptr(z),
                       _lib.ptr(next_rz), _lib.ptr(self.rz), self.p.numel(),
                       _lib.stream_ptr(self.p.device))
@@ -102,6 +103,15 @@ class CG:
         self.rz = next_rz
         self.i += 1
 
+
+    def _unalias_p(self, z):
+        '''The in-place direction update p <- z + beta p must not write
+        through p into r or z: with a preconditioner that returns its input
+        (the identity), p, z and r start as one array, and the reference's
+        out-of-place update leaves r alone.'''
+        ptr = self.p.data_ptr()
+        if ptr == z.data_ptr() or ptr == self.r.data_ptr():
+            self.p = self.p.clone()
 
     def _step_batch(self, Ap, pAp):
         nb = self.rz.numel()
@@ -116,6 +126,7 @@ class CG:
         if getattr(self, '_rz0_dev', None) is None:
             self._rz0_dev = torch.as_tensor(self.rz0, device=next_rz.device)
         self._err = next_rz / self._rz0_dev
+        self._unalias_p(z)
         _lib.call('owmi_cg_p_batch', code, _lib.ptr(self.p), _lib.ptr(z), _lib.ptr(next_rz),
                   _lib.ptr(self.rz), n, nb, st)
         self.rz = next_rz
@@ -365,6 +376,64 @@ class CGWienerMap(CG):
         return cls(imap_dev, icov_signal, icov_noise, sht, beam=beam, mask=mask,
                    rand_isignal=rand_isignal, rand_inoise=rand_inoise, swap_bm=swap_bm,
                    scale_filter=sf)
+
+
+def _from_arrays_const_cor(cls, imap, minfo, ainfo, icov_ell, icov_pix, icov_noise_ell,
+                           nsteps=5, b_ell=None, mask_pix=None, minfo_mask=None,
+                           no_masked_noise=False, draw_constr=False, spin=None, sfilt=None,
+                           lensop=None, seed=None, verbose=False, device=None):
+    '''CGWienerMap with the constant-correlation noise model
+    N = M N_pix^1/2 Y C_ell Y^T N_pix^1/2 M (solvers.py:459-575): N^-1 is
+    applied by nsteps of CG (InvPixEllPixMatVecMap with icov_pix^-1/2 and
+    icov_noise_ell^-1), the constrained-realisation noise draw by its
+    inverse square root on unit-variance alm.'''
+    if lensop:
+        raise NotImplementedError('lensing is not supported')
+    if device is None:
+        device = torch.device('cuda', torch.cuda.current_device())
+    imap_np = imap.detach().cpu().numpy() if isinstance(imap, torch.Tensor) else np.asarray(imap)
+    batched = imap_np.ndim == 3
+    if spin is None:
+        spin = sht_tools.default_spin(imap_np.shape[-2:] if batched else imap_np.shape)
+    imap_np = mat_utils.atleast_nd(imap_np, 2)
+    icov_pix = mat_utils.atleast_nd(np.asarray(icov_pix), 2)
+    icov_ell = mat_utils.atleast_nd(np.asarray(icov_ell), 2)
+    icov_noise_ell = mat_utils.atleast_nd(np.asarray(icov_noise_ell), 2)
+    if icov_pix.shape[-2:] != imap_np.shape[-2:]:
+        raise ValueError(f'Mismatch {icov_pix.shape=} and {imap_np.shape=}')
+    if icov_ell.shape[0] != imap_np.shape[-2]:
+        raise ValueError(f'Mismatch {icov_ell.shape=} and {imap_np.shape=}')
+    icov_signal = operators.EllMatVecAlm(ainfo, icov_ell)
+    noise_mask = None if no_masked_noise else mask_pix
+    icov_noise = operators.InvPixEllPixMatVecMap(icov_pix, icov_noise_ell, minfo, spin,
+                                                 power_m=-0.5, power_x=-1, mask=noise_mask,
+                                                 nsteps=nsteps, verbose=verbose)
+    beam = operators.EllMatVecAlm(ainfo, b_ell) if b_ell is not None else None
+    sf = operators.EllMatVecAlm(ainfo, sfilt) if sfilt is not None else None
+    mask = operators.PixMatVecMap(mask_pix, 1, inplace=False) if mask_pix is not None else None
+    rdt = torch.float64 if imap_np.dtype == np.float64 else torch.float32
+    cdt = torch.complex128 if rdt == torch.float64 else torch.complex64
+    if draw_constr:
+        seeds = ([None if seed is None else [seed, i] for i in range(imap_np.shape[0])]
+                 if batched else [seed])
+        cnp = np.complex128 if cdt == torch.complex128 else np.complex64
+        rs = [alm_utils.rand_alm(icov_ell, ainfo, sd, dtype=cnp) for sd in seeds]
+        rn = [alm_utils.unit_var_alm(ainfo, imap_np.shape[-2:-1], sd, dtype=cnp) for sd in seeds]
+        rand_isignal = torch.as_tensor(np.stack(rs) if batched else rs[0], device=device)
+        sqrt_op = operators.InvSqrtPixEllPixMatVecMap(icov_pix, icov_noise_ell, minfo, spin,
+                                                      power_m=-0.5, power_x=-1, mask=noise_mask,
+                                                      inv_op=icov_noise, verbose=verbose)
+        rand_inoise = sqrt_op(torch.as_tensor(np.stack(rn) if batched else rn[0], device=device))
+    else:
+        rand_isignal = rand_inoise = None
+    sht = (operators.YMatVecAlm(ainfo, minfo, spin, qweight=False),
+           operators.YTWMatVecMap(minfo, ainfo, spin, qweight=False))
+    imap_dev = torch.as_tensor(np.ascontiguousarray(imap_np), device=device).to(rdt)
+    return cls(imap_dev, icov_signal, icov_noise, sht, beam=beam, mask=mask,
+               rand_isignal=rand_isignal, rand_inoise=rand_inoise, scale_filter=sf)
+
+
+CGWienerMap.from_arrays_const_cor = classmethod(_from_arrays_const_cor)
 
 
 class CGWiener(CG):

@@ -115,6 +115,25 @@ def test_cg_trajectory(dev, pol_problem):
         assert abs(solver.err - cg.err) <= 1e-9 * abs(cg.err)
 
 
+@pytest.mark.parametrize('precond', ['none', 'returns_input'])
+def test_cg_trajectory_identity_preconditioner(dev, pol_problem, precond):
+    '''Without a preconditioner (or with one that returns its input) the
+    direction starts as the residual array itself; the in-place direction
+    update must leave the residual alone, as the reference's out-of-place
+    update does (regression test).'''
+    prob = pol_problem
+    solver, osol = build_pair(prob, dev, with_pinv=False)
+    if precond == 'returns_input':
+        solver.add_preconditioner(lambda a: a)
+        osol.add_preconditioner(lambda a: a)
+    solver.init_solver()
+    cg = osol.init_solver()
+    for i in range(4):
+        solver.step()
+        cg.step()
+        assert rel(solver.x, cg.x) < 1e-10, f'step {i}'
+
+
 def test_converged_solution(dev, t_problem):
     '''Both solvers driven to true relative residual <= 1e-12.'''
     prob = t_problem
