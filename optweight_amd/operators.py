@@ -3,7 +3,9 @@ Linear operators as callables on device tensors, mirroring the Wiener-path
 subset of optweight's operators module: MatVecAlm / MatVecMap ABCs (9-47),
 YMatVecAlm (49-103), YTWMatVecMap (105-160), PixMatVecMap (162-204),
 EllMatVecAlm (206-257), PixMatVecAlm (562-643), PixEllPixMatVecMap
-(991-1102), op2mat (1419-1456), add_operators (1458-1489).
+(991-1102), InvSqrtPixEllPixMatVecMap (1104-1200),
+InvPixEllPixMatVecMap (1202-1284), op2mat (1419-1456), add_operators
+(1458-1489).
 
 Host-side matrices (numpy) are moved to the device once per (device,
 dtype). When a pixel matrix is diagonal, PixMatVecAlm runs the fused
@@ -373,6 +375,117 @@ class PixEllPixMatVecMap(MatVecMap):
                     rings=self.rings)
         self._apply_m(out)
         return out
+
+
+def map_dot(x, y):
+    '''<x, y> of real maps in float64: a 0-d tensor for one map set
+    (npol, npix), one value per right-hand side (nb,) for a batch.'''
+    if x.ndim >= 3:
+        return (x * y).reshape(x.shape[0], -1).sum(dim=1, dtype=torch.float64)
+    return (x * y).sum(dtype=torch.float64)
+
+
+def _ratio(num, den):
+    '''num / den per right-hand side, 0 where den is 0 (a right-hand side
+    whose input is zero stays zero instead of turning into NaN).'''
+    return torch.where(den != 0, num / torch.where(den != 0, den, torch.ones_like(den)),
+                       torch.zeros_like(num))
+
+
+def inner_cg(a_op, m_op, b, nsteps, dot=map_dot, verbose=False):
+    '''nsteps of preconditioned CG from x = 0 on maps (the recurrence and
+    order of operations of pixell.utils.CG), with per-right-hand-side
+    scalars for a batch (..., npol, npix) and zero-safe divisions.'''
+    bshape = (-1,) + (1,) * (b.ndim - 1) if b.ndim >= 3 else ()
+    x = torch.zeros_like(b)
+    r = b
+    z = m_op(r)
+    rz = dot(r, z)
+    p = z
+    for idx in range(nsteps):
+        Ap = a_op(p)
+        alpha = _ratio(rz, dot(p, Ap)).to(b.dtype).reshape(bshape)
+        x += alpha * p
+        r = r - alpha * Ap
+        del Ap
+        z = m_op(r)
+        rz_next = dot(r, z)
+        beta = _ratio(rz_next, rz).to(b.dtype).reshape(bshape)
+        rz = rz_next
+        p = z + beta * p
+        if verbose:
+            print(idx, rz.cpu().numpy())
+    return x
+
+
+class InvPixEllPixMatVecMap(MatVecMap):
+    '''(M' M_pix^p Y X_ell^q Y^T M_pix^p M')^-1 m by nsteps of CG
+    (operators.py:1202-1284), preconditioned by the same form with powers
+    -p, -q and the quadrature weights on the other side; M' an optional
+    binary mask applied on both sides of both operators.'''
+
+    def __init__(self, m_pix, x_ell, minfo, spin, power_m=1, power_x=1, mask=None,
+                 adjoint=False, lmax=None, nsteps=5, verbose=False):
+        self._a_mat = PixEllPixMatVecMap(m_pix, x_ell, minfo, spin, power_m=power_m,
+                                         power_x=power_x, inplace=False, adjoint=adjoint,
+                                         lmax=lmax)
+        self._prec = PixEllPixMatVecMap(m_pix, x_ell, minfo, spin, power_m=-power_m,
+                                        power_x=-power_x, inplace=False, adjoint=not adjoint,
+                                        lmax=lmax)
+        self.mask = mask
+        self._mask = _DeviceCache(np.asarray(mask)) if mask is not None else None
+        self.nsteps = nsteps
+        self.verbose = verbose
+
+    def _masked(self, op):
+        if self._mask is None:
+            return op
+
+        def f(imap):
+            m = self._mask.get(imap.device, imap.dtype)
+            return m * op(m * imap)
+        return f
+
+    def call(self, imap):
+        return inner_cg(self._masked(self._a_mat), self._masked(self._prec), imap.clone(),
+                        self.nsteps, verbose=self.verbose)
+
+
+class InvSqrtPixEllPixMatVecMap(MatVecMap):
+    '''Inverse square root of a PixEllPixMatVecMap noise model applied to
+    alm (operators.py:1104-1200): N^-1 (M' M_pix^p Y X_ell^(q/2) alm), the
+    inverse by the CG of InvPixEllPixMatVecMap (inv_op, or a new one).'''
+
+    def __init__(self, m_pix, x_ell, minfo, spin, power_m=1, power_x=1, mask=None,
+                 adjoint=False, lmax=None, inv_op=None, verbose=False):
+        if inv_op:
+            self.inv_op = inv_op
+        else:
+            self.inv_op = InvPixEllPixMatVecMap(m_pix, x_ell, minfo, spin, power_m=power_m,
+                                                power_x=power_x, mask=mask, adjoint=adjoint,
+                                                lmax=lmax, verbose=verbose)
+        self.npol = np.asarray(x_ell).shape[0]
+        self.minfo = minfo
+        self.spin = spin
+        self.adjoint = adjoint
+        self.x_ell_halfq = mat_utils.matpow(x_ell, power_x / 2, return_diag=True)
+        self._xh = _DeviceCache(self.x_ell_halfq)
+        self.mask = mask
+        self._mask = _DeviceCache(np.asarray(mask)) if mask is not None else None
+
+    def _sqrt_n(self, ialm):
+        ainfo = self.inv_op._a_mat.ainfo
+        alm = alm_c_utils.lmul(ialm, self._xh.get(ialm.device, _real_dtype(ialm.dtype)), ainfo)
+        out = torch.zeros(tuple(ialm.shape[:-1]) + (self.minfo.npix,),
+                          dtype=_real_dtype(ialm.dtype), device=ialm.device)
+        sht.alm2map(alm, out, ainfo, self.minfo, self.spin, adjoint=self.adjoint)
+        out = self.inv_op._a_mat._apply_m(out)
+        if self._mask is not None:
+            out *= self._mask.get(out.device, out.dtype)
+        return out
+
+    def call(self, ialm):
+        return self.inv_op(self._sqrt_n(ialm))
 
 
 def op2mat(op, nrow, dtype, ncol=None, input_shape=None, device=None):

@@ -224,21 +224,6 @@ class MaskedPreconditioner(operators.MatVecAlm):
         return alm_c_utils.lmul(alm, r, self.ainfo, inplace=True)
 
 
-def _map_dot(x, y):
-    '''<x, y> of real maps in float64: a 0-d tensor for one map set
-    (npol, npix), one value per right-hand side (nb,) for a batch.'''
-    if x.ndim >= 3:
-        return (x * y).reshape(x.shape[0], -1).sum(dim=1, dtype=torch.float64)
-    return (x * y).sum(dtype=torch.float64)
-
-
-def _ratio(num, den):
-    '''num / den per right-hand side, 0 where den is 0 (a right-hand side
-    whose input is zero stays zero instead of turning into NaN).'''
-    return torch.where(den != 0, num / torch.where(den != 0, den, torch.ones_like(den)),
-                       torch.zeros_like(num))
-
-
 class MaskedPreconditionerCG(operators.MatVecAlm):
     '''Approximate inverse of the masked-pixel system by nsteps conjugate
     gradient steps (preconditioners.py:652-789): with alm_lo the l, m <= lmax
@@ -292,31 +277,7 @@ class MaskedPreconditionerCG(operators.MatVecAlm):
         self.prec = operators.PixEllPixMatVecMap(self.mask_unobs, icov_ell, self.minfo,
                                                  self.spin, power_x=-1, lmax=self.lmax,
                                                  adjoint=True)
-        self.dot = _map_dot
-
-    def _cg(self, b):
-        '''nsteps of preconditioned CG from x = 0 (pixell.utils.CG order of
-        operations), per-right-hand-side scalars.'''
-        bshape = (-1,) + (1,) * (b.ndim - 1) if b.ndim >= 3 else ()
-        x = torch.zeros_like(b)
-        r = b
-        z = self.prec(r)
-        rz = self.dot(r, z)
-        p = z
-        for idx in range(self.nsteps):
-            Ap = self.g_op(p)
-            alpha = _ratio(rz, self.dot(p, Ap)).to(b.dtype).reshape(bshape)
-            x += alpha * p
-            r = r - alpha * Ap
-            del Ap
-            z = self.prec(r)
-            rz_next = self.dot(r, z)
-            beta = _ratio(rz_next, rz).to(b.dtype).reshape(bshape)
-            rz = rz_next
-            p = z + beta * p
-            if self.verbose:
-                print(idx, rz.cpu().numpy())
-        return x
+        self.dot = operators.map_dot
 
     def call(self, alm):
         rdt = operators._real_dtype(alm.dtype)
@@ -330,7 +291,8 @@ class MaskedPreconditionerCG(operators.MatVecAlm):
         sht.alm2map(alm_lo, imap, self.ainfo_lowres, self.minfo, self.spin, rings=rings)
         mu = self._mu.get(alm.device, torch.bool)
         imap *= mu
-        omap = self._cg(imap)
+        omap = operators.inner_cg(self.g_op, self.prec, imap, self.nsteps, dot=self.dot,
+                                  verbose=self.verbose)
         omap *= mu
         sht.map2alm(omap, alm_lo, self.minfo, self.ainfo_lowres, self.spin, adjoint=True,
                     rings=rings)

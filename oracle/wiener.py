@@ -11,6 +11,9 @@ Wiener-filter solver, preconditioners and multigrid (oracle), restating:
       PseudoInvPreconditioner 126-249, MaskedPreconditioner 558-650,
       MaskedPreconditionerCG 652-789, get_2level_prec / get_prec_adef1 /
       get_prec_adef2 791-914
+  operators.InvSqrtPixEllPixMatVecMap 1104-1200, InvPixEllPixMatVecMap
+      1202-1284; solvers.CGWienerMap.from_arrays_const_cor 459-575
+  filters.CGPixFilter 9-301
   multigrid.Level 13-156, get_levels 158-249, get_equal_area_mask_bool
       251-294, restrict/coarsen/interpolate 296-398, v_cycle 400-447
 Only the operator variants that the Wiener path uses are restated (no
@@ -79,7 +82,7 @@ def pix_alm_op(m_pix, lmax, geom, spin, power=1, adjoint=False, use_weights=Fals
 
 
 def pix_ell_pix_op(m_pix, x_ell, geom, spin, lmax=None, power_x=1, adjoint=False,
-                   use_weights=False):
+                   use_weights=False, power_m=1):
     '''PixEllPixMatVecMap (operators.py:991-1102): M Y x_ell^p Y^T M, with
     the quadrature weights W on the analysis when adjoint is use_weights
     (Y^T W) and on the synthesis when adjoint (W Y); no weights for the
@@ -88,19 +91,76 @@ def pix_ell_pix_op(m_pix, x_ell, geom, spin, lmax=None, power_x=1, adjoint=False
         lmax = geometry.minfo2lmax(geom)
     if power_x != 1:
         x_ell = ops.matpow(x_ell, power_x, return_diag=True)
+    if m_pix is not None and power_m != 1:
+        m_pix = ops.matpow(m_pix, power_m, return_diag=True)
+    if m_pix is not None and np.asarray(m_pix).ndim == 3:
+        mp = np.asarray(m_pix)
+        mul = lambda v: np.einsum('abp,bp->ap', mp, v)
+    else:
+        mul = lambda v: v * m_pix
 
     def f(imap):
         out = np.array(imap, copy=True)
         if m_pix is not None:
-            out = out * m_pix
+            out = mul(out)
         alm = np.zeros((out.shape[0], layout.nelem(lmax)),
                        dtype=np.complex64 if out.dtype == np.float32 else np.complex128)
         osht.map2alm(out, alm, geom, lmax, spin, adjoint=adjoint is use_weights)
         alm = ops.lmul(alm, x_ell, lmax)
         osht.alm2map(alm, out, lmax, geom, spin, adjoint=adjoint)
         if m_pix is not None:
-            out = out * m_pix
+            out = mul(out)
         return out
+    f.m_pix = m_pix
+    return f
+
+
+def inv_pix_ell_pix_op(m_pix, x_ell, geom, spin, power_m=1, power_x=1, mask=None,
+                       adjoint=False, lmax=None, nsteps=5):
+    '''InvPixEllPixMatVecMap (operators.py:1202-1284): nsteps of CG on
+    M' M^p Y X^q Y^T M^p M' from x = 0, preconditioned by the powers -p, -q
+    with the weights on the other side; dot = sum(x y).'''
+    a_mat = pix_ell_pix_op(m_pix, x_ell, geom, spin, lmax=lmax, power_x=power_x,
+                           adjoint=adjoint, power_m=power_m)
+    prec = pix_ell_pix_op(m_pix, x_ell, geom, spin, lmax=lmax, power_x=-power_x,
+                          adjoint=not adjoint, power_m=-power_m)
+    if mask is not None:
+        am = lambda v: mask * a_mat(mask * v)
+        pm = lambda v: mask * prec(mask * v)
+    else:
+        am, pm = a_mat, prec
+
+    def f(imap):
+        cg = CG(am, np.array(imap, copy=True), M=pm, dot=lambda x, y: np.sum(x * y))
+        for _ in range(nsteps):
+            cg.step()
+        return cg.x
+    f.a_mat = a_mat
+    return f
+
+
+def inv_sqrt_pix_ell_pix_op(m_pix, x_ell, geom, spin, power_m=1, power_x=1, mask=None,
+                            adjoint=False, lmax=None, inv_op=None, nsteps=5):
+    '''InvSqrtPixEllPixMatVecMap (operators.py:1104-1200): inv_op applied to
+    M' M^p Y X^(q/2) alm.'''
+    if lmax is None:
+        lmax = geometry.minfo2lmax(geom)
+    if inv_op is None:
+        inv_op = inv_pix_ell_pix_op(m_pix, x_ell, geom, spin, power_m=power_m,
+                                    power_x=power_x, mask=mask, adjoint=adjoint, lmax=lmax,
+                                    nsteps=nsteps)
+    xh = ops.matpow(x_ell, power_x / 2, return_diag=True)
+    mp = inv_op.a_mat.m_pix
+
+    def f(alm):
+        a = ops.lmul(alm, xh, lmax)
+        out = np.zeros((a.shape[0], geom.npix))
+        osht.alm2map(a, out, lmax, geom, spin, adjoint=adjoint)
+        if mp is not None:
+            out = np.einsum('abp,bp->ap', mp, out) if np.asarray(mp).ndim == 3 else out * mp
+        if mask is not None:
+            out = out * mask
+        return inv_op(out)
     return f
 
 
@@ -237,6 +297,34 @@ class WienerMap:
         return cls(imap, icov_signal, icov_noise, sht, beam=beam, mask=mask,
                    rand_isignal=rand_isignal, rand_inoise=rand_inoise,
                    swap_bm=swap_bm, scale_filter=sf)
+
+
+def from_arrays_const_cor(imap, geom, lmax, icov_ell, icov_pix, icov_noise_ell, nsteps=5,
+                          b_ell=None, mask_pix=None, no_masked_noise=False, draw_constr=False,
+                          spin=None, seed=None):
+    '''CGWienerMap.from_arrays_const_cor (solvers.py:459-575).'''
+    if spin is None:
+        spin = osht.default_spin(np.shape(imap))
+    imap = np.atleast_2d(imap)
+    icov_pix = np.atleast_2d(icov_pix)
+    icov_ell = np.atleast_2d(icov_ell)
+    icov_noise_ell = np.atleast_2d(icov_noise_ell)
+    nmask = None if no_masked_noise else mask_pix
+    icov_noise = inv_pix_ell_pix_op(icov_pix, icov_noise_ell, geom, spin, power_m=-0.5,
+                                    power_x=-1, mask=nmask, nsteps=nsteps)
+    beam = ell_op(b_ell, lmax) if b_ell is not None else None
+    mask = pix_op(mask_pix) if mask_pix is not None else None
+    if draw_constr:
+        rand_isignal = ops.rand_alm(icov_ell, lmax, seed)
+        rand_inoise = inv_sqrt_pix_ell_pix_op(icov_pix, icov_noise_ell, geom, spin,
+                                              power_m=-0.5, power_x=-1, mask=nmask,
+                                              inv_op=icov_noise)(
+            ops.unit_var_alm(lmax, imap.shape[0], seed))
+    else:
+        rand_isignal = rand_inoise = None
+    sht = (y_op(lmax, geom, spin), yt_op(geom, lmax, spin))
+    return WienerMap(imap, ell_op(icov_ell, lmax), icov_noise, sht, beam=beam, mask=mask,
+                     rand_isignal=rand_isignal, rand_inoise=rand_inoise)
 
 
 # ------------------------------------------------------------ preconditioners

@@ -524,3 +524,78 @@ def test_adef_closed_forms():
         np.testing.assert_allclose(out, want, rtol=1e-12)
     with pytest.raises(ValueError):
         ow.get_2level_prec(m_op, m_op, a_op, 'ADEF-3')
+
+
+# ------------------------------------------- constant-correlation noise
+
+def _gl_pol_setup(lmax, seed):
+    g = og.gauss_legendre(lmax + 1, 2 * lmax + 1)
+    x = _spd_icov(3, lmax, seed)
+    x[0, 1:, :2] = 0
+    x[1:, 0, :2] = 0
+    z = np.random.default_rng(seed + 1).normal(size=(3, layout.nelem(lmax))) * (1 - 1j)
+    ell, em = layout.lm_arrays(lmax)
+    z[:, em == 0] = z[:, em == 0].real
+    z[1:, ell < 2] = 0
+    return g, x, z
+
+
+def test_inv_pix_ell_pix_closed_form():
+    '''With M_pix = c (no mask) on a GL grid, A = c^2p Y X^q Y^T and its
+    preconditioner c^-2p W Y X^-q Y^T W invert each other on range(W Y), so
+    one CG step gives A^-1 (Y z) = c^-2p W Y X^-q z exactly (here p = -1/2,
+    q = -1 as in the constant-correlation noise model), and the inverse
+    square root gives c^-p W Y X^-q/2 a. A preconditioner without the
+    flipped weight side misses after one step.'''
+    lmax = 10
+    g, x, z = _gl_pol_setup(lmax, 20)
+    c = 2.5
+    m_pix = np.full((3, g.npix), c)
+    yz = np.zeros((3, g.npix))
+    osht.alm2map(z, yz, lmax, g, [0, 2])
+    inv = ow.inv_pix_ell_pix_op(m_pix, x, g, [0, 2], power_m=-0.5, power_x=-1, nsteps=1)
+    want = np.zeros((3, g.npix))
+    osht.alm2map(ops.lmul(z, x, lmax), want, lmax, g, [0, 2], adjoint=True)
+    want *= c
+    assert np.linalg.norm(inv(yz) - want) / np.linalg.norm(want) < 1e-12
+    sq = ow.inv_sqrt_pix_ell_pix_op(m_pix, x, g, [0, 2], power_m=-0.5, power_x=-1, nsteps=1)
+    want2 = np.zeros((3, g.npix))
+    osht.alm2map(ops.lmul(z, ops.matpow(x, 0.5, return_diag=True), lmax), want2, lmax, g,
+                 [0, 2], adjoint=True)
+    want2 *= np.sqrt(c)
+    assert np.linalg.norm(sq(z) - want2) / np.linalg.norm(want2) < 1e-12
+    # planted mistake: preconditioner with the weights on the same side as A
+    am = ow.pix_ell_pix_op(m_pix, x, g, [0, 2], power_x=-1, power_m=-0.5)
+    bad_p = ow.pix_ell_pix_op(m_pix, x, g, [0, 2], power_x=1, power_m=0.5)
+    cg = ocg.CG(am, yz.copy(), M=bad_p, dot=lambda u, v: np.sum(u * v))
+    cg.step()
+    assert np.linalg.norm(cg.x - want) / np.linalg.norm(want) > 1e-3
+
+
+def test_inv_pix_ell_pix_masked_brute_force():
+    '''With a mask and a varying pixel matrix, 50 CG steps (more than the
+    34 observed unknowns) solve the dense masked system M' A M' x = b on the
+    observed pixels (b in its range: the system is rank deficient, two rings
+    share the m = lmax mode), and leaves the masked pixels at zero.'''
+    lmax = 8
+    g = og.gauss_legendre(lmax + 1, 2 * lmax + 1)
+    rng = np.random.default_rng(21)
+    m_pix = rng.uniform(0.5, 2.0, size=(1, g.npix))
+    x_ell = ((np.arange(lmax + 1) + 1.0) ** -2)[None]
+    th = np.repeat(g.theta, g.nphi)
+    mask = (th < 0.9)[None].astype(float)
+    idx = np.nonzero(mask[0])[0]
+    a_mat = ow.pix_ell_pix_op(m_pix, x_ell, g, 0, power_x=-1, power_m=-0.5)
+    A = np.zeros((len(idx), len(idx)))
+    for j, k in enumerate(idx):
+        e = np.zeros((1, g.npix))
+        e[0, k] = 1
+        A[:, j] = a_mat(e)[0, idx]
+    b = mask * a_mat(mask * rng.normal(size=(1, g.npix)))
+    inv = ow.inv_pix_ell_pix_op(m_pix, x_ell, g, 0, power_m=-0.5, power_x=-1, mask=mask,
+                                nsteps=50)
+    got = inv(b)
+    res = A @ got[0, idx] - b[0, idx]
+    # residual 3e-3 after 10 steps, 1e-7 after 50 (rank-deficient system)
+    assert np.linalg.norm(res) / np.linalg.norm(b) < 1e-6
+    assert np.all(got[0, mask[0] == 0] == 0)
