@@ -391,6 +391,60 @@ def filter_bench(args, dev):
         'seconds': el, 'iterations': len(errs), 'errors': errs}}))
 
 
+def const_cor_bench(args, dev):
+    """The constant-correlation noise model N = M N_pix^1/2 Y C_ell Y^T
+    N_pix^1/2 M (solvers.py:459-575) on the TQU lmax workload: icov_pix and
+    mask of the headline workload, a 1/f-like noise spectrum C_ell (knee at
+    l = 200, slope -2, TE-correlated), N^-1 by 5 inner CG steps, the
+    harmonic preconditioner with the isotropic inverse variance of icov_pix.
+    Times 10 float64 iterations of one right-hand side and one float32
+    application of A to a batch; prints one JSON line."""
+    prob = workloads.polarized(args.lmax, seed=0, device=dev)
+    ell = np.arange(args.lmax + 1)
+    nl = 1.0 + (np.maximum(ell, 1) / 200.0) ** -2.0
+    cov = np.zeros((3, 3, args.lmax + 1))
+    cov[0, 0] = nl
+    cov[1, 1] = cov[2, 2] = nl
+    cov[0, 1] = cov[1, 0] = 0.2 * nl
+    icov_noise_ell = np.linalg.inv(cov.transpose(2, 0, 1)).transpose(1, 2, 0)
+    mask = prob.mask_bool.astype(np.float64)
+    solver = solvers.CGWienerMap.from_arrays_const_cor(
+        prob.imap, prob.minfo, prob.ainfo, prob.icov_ell, prob.icov_pix, icov_noise_ell,
+        nsteps=5, mask_pix=mask, device=dev)
+    solver.add_preconditioner(preconditioners.HarmonicPreconditioner(
+        prob.ainfo, prob.icov_ell, icov_pix=prob.icov_pix, minfo=prob.minfo))
+    solver.init_solver()
+    solver.step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    niter = 10
+    for _ in range(niter):
+        solver.step()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    res = float(true_residual(solver, solver.x))
+    log(f'const-cor: {niter} float64 iterations {el:.3f}s ({el / niter * 1e3:.1f} ms each), '
+        f'true residual after 11: {res:.3e}')
+    data = workloads.batch_data(prob, args.batch, seed=0, device=dev)
+    bsolver = solvers.CGWienerMap.from_arrays_const_cor(
+        data.astype(np.float32), prob.minfo, prob.ainfo, prob.icov_ell, prob.icov_pix,
+        icov_noise_ell, nsteps=5, mask_pix=mask, device=dev)
+    v = bsolver.b0
+    bsolver.a_matrix(v)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    bsolver.a_matrix(v)
+    torch.cuda.synchronize()
+    t_a32 = time.perf_counter() - t1
+    log(f'const-cor A in float32 for {args.batch} right-hand sides: {t_a32 * 1e3:.1f} ms '
+        f'({t_a32 * 1e3 / args.batch:.2f} ms each)')
+    print(json.dumps({'const_cor': {
+        'workload': f'TQU lmax {args.lmax}, GL {args.lmax + 1}x{2 * args.lmax + 1}, '
+                    'constant-correlation noise (5 inner CG steps), harmonic preconditioner',
+        'fp64_ms_per_iteration': el / niter * 1e3, 'true_residual_after_11': res,
+        f'a_matrix_ms_fp32_x{args.batch}': t_a32 * 1e3}}))
+
+
 def ragged_bench(args, dev):
     '''Constrained realisations: per band limit one batched mixed-precision
     solve of `batch` right-hand sides (data + S^-1/2 w + P^T N^-1/2 w', seeds
@@ -467,6 +521,9 @@ def main():
     ap.add_argument('--filter', action='store_true',
                     help='time the two-phase CAR filter (CGPixFilter) on a full-sky CC map at '
                          '--lmax; prints its own JSON line')
+    ap.add_argument('--const-cor', action='store_true',
+                    help='time the constant-correlation noise model solve on the TQU workload; '
+                         'prints its own JSON line')
     ap.add_argument('--phase1', action='store_true',
                     help='time phase 1 of the two-phase filter (pseudo-inverse plus the masked '
                          'CG preconditioner) on the TQU workload; prints its own JSON line')
@@ -495,6 +552,9 @@ def main():
         return
     if args.filter:
         filter_bench(args, dev)
+        return
+    if args.const_cor:
+        const_cor_bench(args, dev)
         return
     if args.streaming:
         calibration_copy(dev)
