@@ -199,7 +199,8 @@ def restrict(imap, level_in, level_out, spin, adjoint=False):
     imap = imap * level_in.mask_dev(imap.device, rdt)
     lmax_t = lmax_out if lmax_out < lmax_in else lmax_in
     ainfo = alm_utils.AlmInfo(lmax_t)
-    alm = torch.zeros(tuple(imap.shape[:-1]) + (ainfo.nelem,), dtype=cdt, device=imap.device)
+    # the analysis overwrites every coefficient
+    alm = torch.empty(tuple(imap.shape[:-1]) + (ainfo.nelem,), dtype=cdt, device=imap.device)
     sht.map2alm(imap, alm, level_in.minfo, ainfo, spin, adjoint=adjoint, rings=level_in.rings)
     if not adjoint:
         r_ell = noise_utils._band_limit_gauss_beam(lmax_out, fwhm=2 * np.pi / lmax_out)
@@ -231,8 +232,7 @@ def v_cycle(levels, imap, spin, idx=0, n_jacobi=1, x0=None):
         x = x0.clone()
         x += lev.smoother(imap - lev.g_op(x))
     else:
-        x = torch.zeros_like(imap)
-        x += lev.smoother(imap)
+        x = lev.smoother(imap)
     for _ in range(n_jacobi - 1):
         x += lev.smoother(imap - lev.g_op(x))
     r_h = imap - lev.g_op(x)

@@ -362,10 +362,15 @@ class PixEllPixMatVecMap(MatVecMap):
     def call(self, imap):
         assert tuple(imap.shape[-2:]) == (self.npol, self.minfo.npix), (
             f'Wrong imap shape: {tuple(imap.shape)} != {(self.npol, self.minfo.npix)}')
-        out = imap if self.inplace else imap.clone()
-        alm = torch.zeros(tuple(imap.shape[:-1]) + (self.ainfo.nelem,),
+        m = self._m.get(imap.device, imap.dtype) if self._m is not None else None
+        if not self.inplace and m is not None and m.ndim == 2:
+            out = mat_utils.matvec(m, imap.contiguous())  # multiplied copy in one pass
+        else:
+            out = imap if self.inplace else imap.clone()
+            self._apply_m(out)
+        # the analysis overwrites every coefficient
+        alm = torch.empty(tuple(imap.shape[:-1]) + (self.ainfo.nelem,),
                           dtype=_complex_dtype(imap.dtype), device=imap.device)
-        self._apply_m(out)
         sht.map2alm(out, alm, self.minfo, self.ainfo, self.spin,
                     adjoint=self.adjoint is self.use_weights, rings=self.rings)
         alm_c_utils.lmul(alm, self._x.get(imap.device, _real_dtype(imap.dtype)), self.ainfo,
