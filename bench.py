@@ -445,6 +445,34 @@ def const_cor_bench(args, dev):
         f'a_matrix_ms_fp32_x{args.batch}': t_a32 * 1e3}}))
 
 
+def noise_est_bench(args, dev):
+    """Wavelet noise-model estimation (noise_utils.estimate_cov_wav) at the
+    config-4 shape: temperature noise alm at lmax --wavelet-lmax, 12
+    needlet kernels (lambda 2), each scale's map on its Gauss-Legendre grid,
+    squared, smoothed and normalised on the GPU (float64). Prints one JSON
+    line."""
+    from optweight_amd import noise_utils
+    lmax = args.wavelet_lmax
+    w_ell = workloads.needlet_kernels(lmax)
+    ainfo = alm_utils.AlmInfo(lmax)
+    rng = np.random.default_rng(0)
+    a = (rng.normal(size=(1, 1, ainfo.nelem)) + 1j * rng.normal(size=(1, 1, ainfo.nelem)))
+    a[..., :lmax + 1] = a[..., :lmax + 1].real
+    at = torch.as_tensor(a, device=dev)
+    noise_utils.estimate_cov_wav(at, ainfo, w_ell, 0, diag=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    cov = noise_utils.estimate_cov_wav(at, ainfo, w_ell, 0, diag=True)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    npix = sum(int(cov.minfos[k].npix) for k in cov.maps)
+    log(f'estimate_cov_wav: {w_ell.shape[0]} scales, lmax {lmax}, {npix} pixels in total: '
+        f'{el:.3f}s')
+    print(json.dumps({'noise_estimation': {
+        'workload': f'estimate_cov_wav, T, lmax {lmax}, {w_ell.shape[0]} needlet scales, float64',
+        'seconds': el, 'pixels': npix}}))
+
+
 def ragged_bench(args, dev):
     '''Constrained realisations: per band limit one batched mixed-precision
     solve of `batch` right-hand sides (data + S^-1/2 w + P^T N^-1/2 w', seeds
@@ -521,6 +549,9 @@ def main():
     ap.add_argument('--filter', action='store_true',
                     help='time the two-phase CAR filter (CGPixFilter) on a full-sky CC map at '
                          '--lmax; prints its own JSON line')
+    ap.add_argument('--noise-est', action='store_true',
+                    help='time the wavelet noise-model estimation at --wavelet-lmax; prints its '
+                         'own JSON line')
     ap.add_argument('--const-cor', action='store_true',
                     help='time the constant-correlation noise model solve on the TQU workload; '
                          'prints its own JSON line')
@@ -555,6 +586,9 @@ def main():
         return
     if args.const_cor:
         const_cor_bench(args, dev)
+        return
+    if args.noise_est:
+        noise_est_bench(args, dev)
         return
     if args.streaming:
         calibration_copy(dev)

@@ -56,9 +56,13 @@ def norm_cov_est(cov_est, minfo, kernel_ell, inplace=False):
     ells = np.arange(kernel_ell.size)
     norm = np.sum(kernel_ell ** 2 * (2 * ells + 1)) / 4 / np.pi
     out = cov_est if inplace else cov_est.clone()
-    w = torch.as_tensor(np.repeat(1.0 / minfo.weight, minfo.nphi) / norm, dtype=out.dtype,
-                        device=out.device)
-    out *= w
+    cache = minfo.__dict__.setdefault('_inv_area', {})
+    key = (str(out.device), out.dtype)
+    if key not in cache:
+        cache[key] = torch.as_tensor(np.repeat(1.0 / minfo.weight, minfo.nphi), dtype=out.dtype,
+                                     device=out.device)
+    out *= cache[key]
+    out /= norm
     return out
 
 

@@ -134,6 +134,18 @@ def wav_lmax(w_ell, idx):
     return int(lmax - np.argmax(np.asarray(w_ell)[idx, ::-1] > 0))
 
 
+_GL_MINFO = {}
+
+
+def _gl_minfo(lmax):
+    '''The Gauss-Legendre geometry of band limit lmax, one object per lmax:
+    the transform plans are cached on the geometry object, so repeated
+    wavelet transforms reuse them.'''
+    if lmax not in _GL_MINFO:
+        _GL_MINFO[lmax] = map_utils.MapInfo.map_info_gauss_legendre(lmax + 1, 2 * lmax + 1)
+    return _GL_MINFO[lmax]
+
+
 def alm2wav(alm, ainfo, spin, w_ell, wav=None, adjoint=False, lmaxs=None):
     '''Wavelet maps of device alm (..., npol, nelem): scale j is Y (or W Y
     if adjoint) of w_ell[j] a_lm on its Gauss-Legendre grid.'''
@@ -141,8 +153,7 @@ def alm2wav(alm, ainfo, spin, w_ell, wav=None, adjoint=False, lmaxs=None):
     indices = np.arange(len(wlms))
     rdt = torch.float64 if alm.dtype == torch.complex128 else torch.float32
     if wav is None:
-        minfos = [map_utils.MapInfo.map_info_gauss_legendre(w.lmax + 1, 2 * w.lmax + 1)
-                  for w in winfos]
+        minfos = [_gl_minfo(w.lmax) for w in winfos]
         wav = Wav(1, preshape=tuple(alm.shape[:-1]),
                   dtype=np.float64 if rdt == torch.float64 else np.float32)
         for idx in indices:
