@@ -155,6 +155,11 @@ class EllMatVecAlm(MatVecAlm):
             return alm_c_utils.lmul(alm, m, self.ainfo, inplace=True)
         return alm_c_utils.lmul(alm, m, self.ainfo, alm_out=torch.empty_like(alm))
 
+    def apply_copy(self, alm):
+        '''M^p alm into a new array, whatever ``inplace`` says.'''
+        m = self.device_matrix(alm.device, alm.dtype)
+        return alm_c_utils.lmul(alm, m, self.ainfo, alm_out=torch.empty_like(alm))
+
 
 def _pix_alm_weights(adjoint, use_weights):
     '''(weight after synthesis, weight before adjoint) of PixMatVecAlm.'''
@@ -538,7 +543,12 @@ def add_operators(op_1, op_2, slice_2=None):
     (per right-hand side for a batch, see component_index)."""
 
     def add(ivec):
-        ovec = op_1(ivec.clone() if isinstance(ivec, torch.Tensor) else ivec.copy())
+        # operators that never write to their input (preserves_input) get
+        # the input itself; others a copy, as the reference gives them
+        if getattr(op_1, 'preserves_input', False):
+            ovec = op_1(ivec)
+        else:
+            ovec = op_1(ivec.clone() if isinstance(ivec, torch.Tensor) else ivec.copy())
         idx = component_index(ivec, slice_2)
         if idx is None:
             idx = slice(None)
