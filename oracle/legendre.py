@@ -115,7 +115,6 @@ def lam_m(m, lmax, theta, s, seed=None, x=None):
         first = seed
     elif m >= abs(s):
         # d^m_{m,-s} = (-1)^{m+s} sqrt((2m)!/((m-s)!(m+s)!)) c^{m-s} sn^{m+s}
-        from math import lgamma
         c = np.cos(th / LD(2))
         sn = np.sin(th / LD(2))
         # prefactor via a product to keep longdouble precision

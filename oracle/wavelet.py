@@ -20,7 +20,7 @@ THIS MODULE IS TEST INFRASTRUCTURE (see oracle/__init__.py).
 import numpy as np
 from scipy.interpolate import RectBivariateSpline
 
-from oracle import layout, geometry, ops, sht as osht
+from oracle import layout, ops, sht as osht
 from oracle.cg import CG
 from oracle.wiener import ell_op, pix_alm_op, harmonic_matrix
 

@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 import torch
 
-from oracle import geometry as og, sht as osht, layout, wiener as ow
+from oracle import geometry as og, sht as osht, wiener as ow
 from optweight_amd import filters, map_utils, sht, alm_utils
 
 import test_car_geometry as car

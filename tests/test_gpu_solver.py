@@ -13,8 +13,7 @@ import pytest
 import torch
 
 from oracle import geometry as og, wiener as ow, ops as oops, sht as osht
-from optweight_amd import (solvers, preconditioners, multigrid, workloads, alm_utils,
-                           map_utils, operators)
+from optweight_amd import solvers, preconditioners, workloads, alm_utils
 
 pytestmark = pytest.mark.gpu
 

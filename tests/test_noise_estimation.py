@@ -9,9 +9,8 @@ symmetry and positive semi-definiteness of the full estimate, and the same
 statistics per wavelet scale for estimate_cov_wav.
 '''
 import numpy as np
-import pytest
 
-from oracle import geometry as og, sht as osht, layout, noise as on, ops
+from oracle import geometry as og, sht as osht, layout, noise as on
 
 
 def unit_alm(lmax, shape, seed):
