@@ -177,6 +177,77 @@ int owmi_convert(int dtype_src, const void *src, int dtype_dst, void *dst,
                  long long n, void *stream);
 
 /* ------------------------------------------------------------------ */
+/* 2-D Fourier coefficients of flat-sky (CAR) maps                     */
+/* ------------------------------------------------------------------ */
+/* Coefficient arrays are (nlead, ny, nlx) complex, nlx = nx / 2 + 1 for
+ * maps of nx columns. */
+
+/*
+ * out[c, p] = sum_j M[c, j, p] in[j, p] (kind 1, M (ncomp, ncomp, npix)
+ * real, ncomp <= 4) or d[c * diag_stride + p] in[c, p] (kind 0; diag_stride
+ * npix for a per-component diagonal, 0 for one shared by all components),
+ * for ntrans consecutive (ncomp, npix) complex arrays; in may alias out.
+ * Replaces dft.fmul_2d (optweight/dft.py:332-365).
+ */
+int owmi_fmul(int dtype, const void *mat, int kind, const void *in, void *out,
+              int ncomp, long long npix, long long ntrans, long long diag_stride,
+              void *stream);
+
+/* The full-matrix product of owmi_fmul for any ncomp, through a caller
+ * buffer tmp of the output's size (dft.py:332-365). */
+int owmi_fmul_generic(int dtype, const void *mat, const void *in, void *out,
+                      void *tmp, int ncomp, long long npix, long long ntrans,
+                      void *stream);
+
+/*
+ * Kernel slices of the Fourier-wavelet transform: block j of the packed
+ * buffer small (offset nlead * desc[8j], shape (nlead, ny_j = desc[8j+1],
+ * nx_j = desc[8j+2])) receives full[b, rows[desc[8j+4] + r], desc[8j+3] + c] times
+ * kern[desc[8j+5] + r nx_j + c]; max_count = max_j ny_j nx_j. desc, rows
+ * and kern are device arrays. The gather half of wavtrans.f2wav
+ * (optweight/wavtrans.py:687-712, dft.slice_fmap dft.py:498-533).
+ */
+int owmi_fwav_gather(int dtype, const void *full, long long nlead, long long ny,
+                     long long nlx, int nscale, const long long *desc,
+                     const int *rows, const void *kern, long long max_count,
+                     void *small, void *stream);
+
+/*
+ * The adjoint: full[b, y, x] = sum over scales j (in table order) whose
+ * slice holds (y, x) of small_j[b, invrows[j ny + y], x - x0_j] times the
+ * kernel value; invrows[j ny + y] is the block row of full row y or -1.
+ * Writes every element of full. The accumulate half of wavtrans.wav2f
+ * (wavtrans.py:714-745, dft.add_to_fmap dft.py:535-560).
+ */
+int owmi_fwav_scatter(int dtype, const void *small, long long nlead,
+                      long long ny, long long nlx, int nscale,
+                      const long long *desc, const int *invrows,
+                      const void *kern, void *full, void *stream);
+
+/*
+ * Real-field conventions around library FFT passes, in place. mode 0,
+ * after a real-to-complex transform: zero the imaginary parts of the
+ * self-conjugate coefficients (ly in {0, ny/2}, lx in {0, nx/2}). mode 1,
+ * between the inverse transform along y and the complex-to-real pass
+ * along x: zero the imaginary parts of column 0 and, for even nx, of
+ * column nlx - 1 of every row. Used by dft.rfft / dft.irfft
+ * (dft.py:13-67) for the semantics of a 2-D real FFT.
+ */
+int owmi_fourier_fix(int dtype, void *data, long long nlead, long long ny,
+                     long long nlx, int nx_even, int mode, void *stream);
+
+/*
+ * Real-field inner product of half-plane coefficients into *dev_out
+ * (float64, device): 2 sum Re(f g*) - sum over column 0 - [nx even] sum
+ * over column nlx - 1. nx_parity 1 / 0 states the parity of nx; -1 takes
+ * nx as even when f has no imaginary part at (ly = 0, and ly = ny/2 for
+ * even ny; lx = nlx - 1). Replaces dft.contract_fxg (dft.py:449-496).
+ */
+int owmi_fourier_dot(int dtype, const void *f, const void *g, long long nlead,
+                     long long ny, long long nlx, int nx_parity, double *dev_out,
+                     void *stream);
+
+/* ------------------------------------------------------------------ */
 /* Spherical harmonic transforms                                       */
 /* ------------------------------------------------------------------ */
 

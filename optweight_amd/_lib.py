@@ -39,6 +39,14 @@ _SIGNATURES = {
     'owmi_cg_p_batch': (_c_int, [_c_int, _vp, _vp, _vp, _vp, _c_ll, _c_ll, _vp]),
     'owmi_axpby': (_c_int, [_c_int, _c_dbl, _vp, _c_dbl, _vp, _c_ll, _vp]),
     'owmi_convert': (_c_int, [_c_int, _vp, _c_int, _vp, _c_ll, _vp]),
+    'owmi_fmul': (_c_int, [_c_int, _vp, _c_int, _vp, _vp, _c_int, _c_ll, _c_ll, _c_ll, _vp]),
+    'owmi_fmul_generic': (_c_int, [_c_int, _vp, _vp, _vp, _vp, _c_int, _c_ll, _c_ll, _vp]),
+    'owmi_fwav_gather': (_c_int, [_c_int, _vp, _c_ll, _c_ll, _c_ll, _c_int, _vp, _vp, _vp, _c_ll,
+                                  _vp, _vp]),
+    'owmi_fwav_scatter': (_c_int, [_c_int, _vp, _c_ll, _c_ll, _c_ll, _c_int, _vp, _vp, _vp, _vp,
+                                   _vp]),
+    'owmi_fourier_fix': (_c_int, [_c_int, _vp, _c_ll, _c_ll, _c_ll, _c_int, _c_int, _vp]),
+    'owmi_fourier_dot': (_c_int, [_c_int, _vp, _vp, _c_ll, _c_ll, _c_ll, _c_int, _vp, _vp]),
     'owmi_sht_plan_create': (_c_int, [ctypes.POINTER(_vp), _c_ll, _vp, _vp, _vp, _vp, _vp, _c_int]),
     'owmi_sht_plan_create_rings': (_c_int, [ctypes.POINTER(_vp), _c_ll, _vp, _vp, _vp, _vp, _vp,
                                             _c_int, _c_ll]),

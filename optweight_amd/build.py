@@ -20,7 +20,7 @@ BUILD = os.path.join(HERE, '_build')
 LIBDIR = os.path.join(HERE, '_lib')
 LIB = os.path.join(LIBDIR, 'libowmi.so')
 
-SOURCES = ['streaming.hip', 'sht_plan.hip', 'legendre.hip', 'ringfft.hip']
+SOURCES = ['streaming.hip', 'sht_plan.hip', 'legendre.hip', 'ringfft.hip', 'fourier.hip']
 
 HIPCC = os.environ.get('HIPCC', shutil.which('hipcc') or '/opt/rocm/bin/hipcc')
 ARCH = os.environ.get('OWMI_OFFLOAD_ARCH', 'gfx950')

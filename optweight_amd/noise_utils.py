@@ -3,9 +3,10 @@ Noise helpers the Wiener path needs, mirroring optweight's noise_utils:
 _band_limit_gauss_beam (noise_utils.py:401-421, healpy.gauss_beam closed
 form exp(-l(l+1) sigma^2 / 2), sigma = fwhm / sqrt(8 ln 2)),
 get_white_icov_pix (592-641), and the noise-model estimation
-estimate_cov_wav (10-77), estimate_cov_pix (144-226), norm_cov_est
-(228-253) with map_utils.lmul_pix's spin-0 smoothing (map_utils.py:
-1365-1408) on device maps.
+estimate_cov_wav (10-77), estimate_cov_fwav (79-142), estimate_cov_pix
+(144-226), norm_cov_est (228-253) with map_utils.lmul_pix's spin-0
+smoothing (map_utils.py:1365-1408) or the flat-sky one of fmul_pix
+(1751-1793) on device maps, and unit_var_wav (423-458).
 '''
 import numpy as np
 import torch
@@ -81,7 +82,16 @@ def _smooth_spin0(maps, b_ell, minfo):
     return out.reshape(maps.shape)
 
 
-def estimate_cov_pix(imap, minfo, diag=False, fwhm=None, lmax=None, kernel_ell=None):
+def _smooth_flat(maps, fb, minfo):
+    '''map_utils.fmul_pix with a shared 2-D Fourier diagonal fb on a stack
+    of maps (k, npix) of a rectangular grid.'''
+    from optweight_amd import map_utils
+    m2 = maps.reshape(-1, maps.shape[-1]).contiguous()
+    return map_utils.fmul_pix(m2, minfo, fmat2d=fb).reshape(maps.shape)
+
+
+def estimate_cov_pix(imap, minfo, diag=False, fwhm=None, lmax=None, kernel_ell=None,
+                     flatsky=False, modlmap=None):
     '''Pixel covariance of a noise map (ncomp, npol, npix) (device tensor or
     host array, result on the same side as a device tensor): the outer
     product per pixel, (ncomp, ncomp, npol, npix) with diag (no
@@ -90,7 +100,8 @@ def estimate_cov_pix(imap, minfo, diag=False, fwhm=None, lmax=None, kernel_ell=N
     as spin-0 maps (upper triangle, the lower one mirrored), normalised
     with norm_cov_est when the noise is band limited (lmax or kernel_ell),
     and made positive semi-definite per pixel (negative entries clipped to
-    zero with diag).'''
+    zero with diag). flatsky smooths in the 2-D Fourier plane instead, with
+    the beam sampled on modlmap (|l| of the map's rfft coefficients).'''
     if not isinstance(imap, torch.Tensor):
         imap = torch.as_tensor(np.asarray(imap), device=torch.device(
             'cuda', torch.cuda.current_device()))
@@ -112,7 +123,12 @@ def estimate_cov_pix(imap, minfo, diag=False, fwhm=None, lmax=None, kernel_ell=N
         iu = [(i, j) for i in range(n0) for j in range(n1) if i <= j]
         ii = torch.as_tensor([i for i, _ in iu], device=cov.device)
         jj = torch.as_tensor([j for _, j in iu], device=cov.device)
-        sm = _smooth_spin0(cov[ii, jj], b_ell, minfo)
+        if flatsky:
+            from optweight_amd import dft
+            fb = dft.cl2flat(b_ell, np.arange(b_ell.shape[-1]), modlmap)
+            sm = _smooth_flat(cov[ii, jj], fb, minfo)
+        else:
+            sm = _smooth_spin0(cov[ii, jj], b_ell, minfo)
         cov[ii, jj] = sm
         cov[jj, ii] = sm
     if lmax or kernel_ell is not None:
@@ -160,4 +176,40 @@ def estimate_cov_wav(alm, ainfo, w_ell, spin, diag=False, wav_template=None, fwh
         cov_pix = estimate_cov_pix(noise_wav.maps[jidx], minfo, kernel_ell=w_ell[jidx],
                                    diag=diag, fwhm=fact * np.pi / lmax_j)
         cov_wav.add((jidx, jidx), cov_pix, minfo)
+    return cov_wav
+
+
+def unit_var_wav(minfos, preshape, dtype, seed=None):
+    """Block vector of unit-variance Gaussian maps, one per map info, drawn
+    in order from numpy's default_rng(seed) (host arrays)."""
+    from optweight_amd import wavtrans
+    rng = np.random.default_rng(seed)
+    wav = wavtrans.Wav(1, preshape=preshape, dtype=dtype)
+    for widx, minfo in enumerate(minfos):
+        wav.add(widx, rng.normal(size=tuple(preshape) + (minfo.npix,)).astype(dtype), minfo)
+    return wav
+
+
+def estimate_cov_fwav(fmap, fkernelset, wav_template, diag=False, fwhm_fact=2):
+    """Fourier-wavelet noise covariance (nwav, nwav) block diagonal from
+    noise 2-D coefficients fmap (device): the monopole is left out (and put
+    back afterwards), fmap is cut into wavelet maps (wavtrans.f2wav into
+    wav_template), and each scale's pixel covariance is estimated with
+    flat-sky smoothing of FWHM fwhm_fact pi / lmax_j and divided by the
+    mean squared kernel (noise_utils.py:79-142)."""
+    from optweight_amd import wavtrans
+    mono = fmap[..., 0, 0].clone()
+    fmap[..., 0, 0] = 0
+    noise_wav = wavtrans.f2wav(fmap, wav_template, fkernelset)
+    fmap[..., 0, 0] = mono
+    rdt = np.float64 if fmap.dtype == torch.complex128 else np.float32
+    cov_wav = wavtrans.Wav(2, dtype=rdt)
+    for jidx, fk in fkernelset:
+        minfo = noise_wav.minfos[jidx]
+        lmax_j = fk.lmax
+        fact = fwhm_fact(lmax_j) if callable(fwhm_fact) else fwhm_fact
+        cov = estimate_cov_pix(noise_wav.maps[jidx], minfo, diag=diag, fwhm=fact * np.pi / lmax_j,
+                               flatsky=True, modlmap=fk.modlmap())
+        cov /= float(np.mean(np.abs(np.asarray(fk.fkernel, dtype=np.float64)) ** 2))
+        cov_wav.add((jidx, jidx), cov, minfo)
     return cov_wav

@@ -5,7 +5,10 @@ YMatVecAlm (49-103), YTWMatVecMap (105-160), PixMatVecMap (162-204),
 EllMatVecAlm (206-257), PixMatVecAlm (562-643), PixEllPixMatVecMap
 (991-1102), InvSqrtPixEllPixMatVecMap (1104-1200),
 InvPixEllPixMatVecMap (1202-1284), op2mat (1419-1456), add_operators
-(1458-1489).
+(1458-1489), and the 2-D Fourier-wavelet noise operators: FMatVecF
+(259-309), FWavMatVecF (311-362), InvFWavMatVecF (364-421),
+FInvFWavFMatVecMap (423-486), InvSqrtFWavMatVecWav (488-560),
+WavMatVecWav (914-989).
 
 Host-side matrices (numpy) are moved to the device once per (device,
 dtype). When a pixel matrix is diagonal, PixMatVecAlm runs the fused
@@ -17,7 +20,7 @@ from abc import ABC, abstractmethod
 import numpy as np
 import torch
 
-from optweight_amd import sht, alm_c_utils, mat_utils, map_utils, alm_utils
+from optweight_amd import sht, alm_c_utils, mat_utils, map_utils, alm_utils, dft, wavtrans
 
 
 class MatVecAlm(ABC):
@@ -402,11 +405,14 @@ def _ratio(num, den):
                        torch.zeros_like(num))
 
 
-def inner_cg(a_op, m_op, b, nsteps, dot=map_dot, verbose=False):
+def inner_cg(a_op, m_op, b, nsteps, dot=map_dot, verbose=False, batched=None):
     '''nsteps of preconditioned CG from x = 0 on maps (the recurrence and
     order of operations of pixell.utils.CG), with per-right-hand-side
-    scalars for a batch (..., npol, npix) and zero-safe divisions.'''
-    bshape = (-1,) + (1,) * (b.ndim - 1) if b.ndim >= 3 else ()
+    scalars for a batch (nb, npol, npix) (batched None: when b.ndim >= 3)
+    and zero-safe divisions.'''
+    if batched is None:
+        batched = b.ndim >= 3
+    bshape = (-1,) + (1,) * (b.ndim - 1) if batched else ()
     x = torch.zeros_like(b)
     r = b
     z = m_op(r)
@@ -559,3 +565,181 @@ def add_operators(op_1, op_2, slice_2=None):
         return ovec
 
     return add
+
+
+class MatVecF(ABC):
+    def __call__(self, fmap, *args, **kwargs):
+        return self.call(fmap, *args, **kwargs)
+
+    @abstractmethod
+    def call(self, fmap, *args, **kwargs):
+        pass
+
+
+class MatVecWav(ABC):
+    def __call__(self, wav, *args, **kwargs):
+        return self.call(wav, *args, **kwargs)
+
+    @abstractmethod
+    def call(self, wav, *args, **kwargs):
+        pass
+
+
+class WavMatVecWav(MatVecWav):
+    """Wavelet block matrix times a wavelet vector, in place: map j' of the
+    vector is multiplied by block (j, j') for every block present, in key
+    order (operators.py:914-989); blocks are (npol, npol, npix) matrices or
+    (npol, npix) diagonals, raised to power at construction."""
+
+    def __init__(self, m_wav, power=1, inplace=False, op=None):
+        if m_wav.ndim != 2:
+            raise ValueError(f'Expected m_wav.ndim = 2, got : {m_wav.ndim}')
+        if power != 1:
+            m_wav = mat_utils.wavmatpow(m_wav, power, return_diag=True)
+        self.m_wav = m_wav
+        self.inplace = inplace
+        if not self.inplace:
+            raise NotImplementedError()
+        self.op = op
+        self._dev = {key: _DeviceCache(m) for key, m in m_wav.maps.items()}
+
+    def call(self, wav):
+        if wav.ndim != 1:
+            raise ValueError(f'Expected wav.ndim = 1, got : {wav.ndim}')
+        nrow, ncol = self.m_wav.shape
+        for jidx in range(nrow):
+            for jpidx in range(ncol):
+                if (jidx, jpidx) not in self._dev:
+                    continue
+                vec = wav.maps[jpidx]
+                mat = self._dev[(jidx, jpidx)].get(vec.device, vec.dtype)
+                if self.op is None:
+                    mat_utils.matvec(mat, vec, inplace=True)
+                else:
+                    vec.copy_(torch.einsum(self.op, mat, vec))
+        return wav
+
+
+class FMatVecF(MatVecF):
+    """M^p f for M diagonal in the 2-D Fourier plane: (npol, npol, nly, nlx)
+    matrices or (npol, nly, nlx) diagonals (operators.py:259-309)."""
+
+    def __init__(self, fmat2d, power=1, inplace=False):
+        fmat2d = np.asarray(fmat2d)
+        if power != 1:
+            flat = fmat2d.reshape(fmat2d.shape[:-2] + (-1,))
+            fmat2d = mat_utils.matpow(flat, power, return_diag=True).reshape(fmat2d.shape)
+        self.fmat2d = fmat2d
+        self._dev = _DeviceCache(fmat2d)
+        self.inplace = inplace
+
+    def call(self, fmap):
+        m = self._dev.get(fmap.device, _real_dtype(fmap.dtype))
+        return dft.fmul(fmap, fmat2d=m, out=fmap if self.inplace else None)
+
+
+class FWavMatVecF(MatVecF):
+    """K^T F M_wav^p F^T K f: 2-D coefficients to Fourier-wavelet maps
+    (f2wav), the wavelet block matrix, and back (wav2f)
+    (operators.py:311-362). The wavelet maps live in a template vector per
+    (device, dtype), shared with other operators through wav_template."""
+
+    def __init__(self, m_wav, fkernelset, power=1, wav_template=None):
+        self.m_wav_op = WavMatVecWav(m_wav, power=power, inplace=True)
+        self.fkernelset = fkernelset
+        self._preshape = tuple(m_wav.preshape[-1:])
+        if isinstance(wav_template, dict):
+            self.wav_template = wav_template
+        else:
+            self.wav_template = {}
+            if wav_template is not None:
+                self.wav_template['given'] = wav_template
+
+    def _template(self, device, dtype):
+        if 'given' in self.wav_template:
+            return self.wav_template['given']
+        key = (str(device), dtype)
+        if key not in self.wav_template:
+            npdt = np.float64 if dtype == torch.float64 else np.float32
+            self.wav_template[key] = self.fkernelset.get_wav_vec(self._preshape, dtype=npdt,
+                                                                 device=device)
+        return self.wav_template[key]
+
+    def call(self, fmap):
+        tmpl = self._template(fmap.device, _real_dtype(fmap.dtype))
+        wavtrans.f2wav(fmap, tmpl, self.fkernelset)
+        self.m_wav_op(tmpl)
+        out = torch.empty_like(fmap)
+        wavtrans.wav2f(tmpl, out, self.fkernelset)
+        return out
+
+
+class InvFWavMatVecF(MatVecF):
+    """(K^T F M_wav F^T K)^-1 f by nsteps of CG from zero, preconditioned by
+    the same operator with M_wav^-1, under the real-field inner product of
+    2-D coefficients (operators.py:364-421)."""
+
+    def __init__(self, m_wav, fkernelset, nsteps=3):
+        self.fkernelset = fkernelset
+        self.nsteps = nsteps
+        self._a_mat = FWavMatVecF(m_wav, fkernelset, power=1)
+        self._prec = FWavMatVecF(m_wav, fkernelset, power=-1,
+                                 wav_template=self._a_mat.wav_template)
+
+    def call(self, fmap, verbose=False):
+        return inner_cg(self._a_mat, self._prec, fmap.clone(), self.nsteps,
+                        dot=dft.contract_fxg_dev, verbose=verbose, batched=False)
+
+
+class FInvFWavFMatVecMap(MatVecMap):
+    """F^T X^q (K^T F M_wav F^T K)^-1 X^q F m of a map on a rectangular grid
+    (operators.py:423-486): X a 2-D Fourier-diagonal matrix raised to
+    power_x (identity for power_x 0), the inverse by InvFWavMatVecF.
+    Leading dimensions beyond (npol, npix) are independent maps."""
+
+    def __init__(self, minfo, m_wav, fkernelset, fmat2d, power_x=1, nsteps=3):
+        self.minfo = minfo
+        self.inv_op = InvFWavMatVecF(m_wav, fkernelset, nsteps=nsteps)
+        if power_x != 0:
+            self.x_op = FMatVecF(fmat2d, power=power_x, inplace=True)
+        else:
+            self.x_op = lambda x: x
+
+    def _one(self, imap, verbose):
+        imap2 = map_utils.view_2d(imap, self.minfo)
+        fmap = dft.allocate_fmap(tuple(imap2.shape), imap2.dtype, device=imap.device)
+        omap = torch.zeros_like(imap2)
+        dft.rfft(imap2, fmap)
+        self.x_op(fmap)
+        fmap = self.inv_op(fmap, verbose=verbose)
+        self.x_op(fmap)
+        dft.irfft(fmap, omap)
+        return map_utils.view_1d(omap, self.minfo)
+
+    def call(self, imap, verbose=False):
+        imap = imap.contiguous()
+        if imap.ndim >= 3:
+            flat = imap.reshape((-1,) + tuple(imap.shape[-2:]))
+            return torch.stack([self._one(m, verbose) for m in flat]).reshape(imap.shape)
+        return self._one(imap, verbose)
+
+
+class InvSqrtFWavMatVecWav(MatVecWav):
+    """(K^T F M_wav F^T K)^-1 K^T F M_wav^1/2 w of a wavelet vector w
+    (operators.py:488-560); M_wav^1/2 is applied to w in place."""
+
+    def __init__(self, m_wav, fkernelset, inv_op=None):
+        self.sqrt_cov_wav_op = WavMatVecWav(m_wav, power=0.5, inplace=True)
+        self.fkernelset = fkernelset
+        self.inv_op = InvFWavMatVecF(m_wav, fkernelset) if inv_op is None else inv_op
+        self.oshape = tuple(m_wav.preshape[-1:]) + tuple(fkernelset.shape_full)
+
+    def _sqrt_n(self, wav):
+        first = wav.maps[wav.get_indices_diag()[0]]
+        out = torch.zeros(self.oshape, dtype=_complex_dtype(first.dtype), device=first.device)
+        self.sqrt_cov_wav_op(wav)
+        wavtrans.wav2f(wav, out, self.fkernelset)
+        return out
+
+    def call(self, wav, **kwargs):
+        return self.inv_op(self._sqrt_n(wav), **kwargs)

@@ -180,6 +180,78 @@ class PseudoInvPreconditionerWav(operators.MatVecAlm):
         return self.harmonic_prec(alm_noise)
 
 
+class PseudoInvPreconditionerFWav(operators.MatVecAlm):
+    """Pseudo-inverse preconditioner for the 2-D Fourier-wavelet noise
+    model (Seljebotn et al. 2017 adapted; preconditioners.py:397-556):
+        M = H [S^-1 + F B tau_l P tau_l B F] H,  H = (S^-1 + B tau_l B)^-1,
+    P alm = Y^T W mask F^T X^1/2 X^1/2 F mask W Y alm, X = cov_noise_2d (the
+    reference also applies the Fourier-wavelet covariance to the 2-D
+    coefficients between the two X^1/2 but discards its result,
+    preconditioners.py:529-531, so P does not contain it). Without sfilt F
+    is the identity (the reference leaves it unset and fails)."""
+
+    def __init__(self, ainfo, icov_ell, cov_wav, fkernelset, spin, itau_ell, cov_noise_2d,
+                 mask_pix, minfo_mask, b_ell=None, sfilt=None):
+        icov_ell = np.asarray(icov_ell)
+        self.npol, self.nell = icov_ell.shape[-2:]
+        itau_ell = np.asarray(itau_ell)
+        if itau_ell.ndim != 3:
+            raise ValueError(f'Wrong dimensions itau_ell : expected 3, got {itau_ell.ndim}')
+        if b_ell is None:
+            b_ell = np.ones((self.npol, self.nell))
+        b_ell = np.asarray(b_ell) * np.eye(self.npol)[:, :, np.newaxis]
+        self.ainfo = ainfo
+        self.minfo_mask = minfo_mask
+        self.spin = spin
+        self._mask = operators._DeviceCache(np.asarray(mask_pix))
+        if sfilt is not None:
+            sfilt = mat_utils.full_matrix(np.asarray(sfilt))
+        self.harmonic_prec = HarmonicPreconditioner(ainfo, icov_ell, itau=itau_ell, b_ell=b_ell,
+                                                    sfilt=sfilt)
+        if sfilt is not None:
+            self.icov_signal = operators.EllMatVecAlm(
+                ainfo, np.einsum('ijl, jkl, kol -> iol', sfilt, icov_ell, sfilt), inplace=True)
+            self.sfilt = operators.EllMatVecAlm(ainfo, sfilt, inplace=True)
+        else:
+            self.icov_signal = operators.EllMatVecAlm(ainfo, icov_ell, inplace=True)
+            self.sfilt = lambda alm: alm
+        self.beam = operators.EllMatVecAlm(ainfo, b_ell, inplace=True)
+        self.ivar_noise_iso = operators.EllMatVecAlm(ainfo, itau_ell, inplace=True)
+        self.fkernelset = fkernelset
+        self.x_op = operators.FMatVecF(cov_noise_2d, power=0.5, inplace=True)
+
+    def pcov_noise(self, alm):
+        from optweight_amd import dft, sht
+        rdt = operators._real_dtype(alm.dtype)
+        map_tmp = torch.zeros(tuple(alm.shape[:-1]) + (self.minfo_mask.npix,), dtype=rdt,
+                              device=alm.device)
+        sht.alm2map(alm, map_tmp, self.ainfo, self.minfo_mask, self.spin, adjoint=True)
+        mask = self._mask.get(alm.device, rdt)
+        map_tmp *= mask
+        map2d = map_utils.view_2d(map_tmp, self.minfo_mask)
+        fmap = dft.allocate_fmap(tuple(map2d.shape), rdt, device=alm.device)
+        dft.rfft(map2d, fmap)
+        self.x_op(fmap)
+        self.x_op(fmap)
+        dft.irfft(fmap, map2d)
+        map_tmp *= mask
+        sht.map2alm(map_tmp, alm, self.minfo_mask, self.ainfo, self.spin, adjoint=True)
+        return alm
+
+    def call(self, alm):
+        alm = self.harmonic_prec(alm.clone())
+        alm_signal = self.icov_signal(alm.clone())
+        alm_noise = self.sfilt(alm)
+        alm_noise = self.beam(alm_noise)
+        alm_noise = self.ivar_noise_iso(alm_noise)
+        alm_noise = self.pcov_noise(alm_noise)
+        alm_noise = self.ivar_noise_iso(alm_noise)
+        alm_noise = self.beam(alm_noise)
+        alm_noise = self.sfilt(alm_noise)
+        alm_noise += alm_signal
+        return self.harmonic_prec(alm_noise)
+
+
 class MaskedPreconditioner(operators.MatVecAlm):
     '''Multigrid approximate inverse of the masked-pixel system
     G = M_u Y d_ell Y^T M_u (Seljebotn et al. 2017), applied as

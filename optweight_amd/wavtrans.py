@@ -2,7 +2,8 @@
 Wavelet block vectors and matrices of maps, and the axisymmetric wavelet
 transforms between alm and per-scale maps, mirroring optweight's wavtrans:
 Wav (wavtrans.py:10-369), wav2alm (406-465), alm2wav (467-515),
-preshape2npol (517-546).
+preshape2npol (517-546), and the 2-D Fourier-wavelet transforms f2wav /
+wav2f (687-745).
 
 A Wav holds one map per index (an int for a block vector, an (i, j) tuple
 for a block matrix) with its own MapInfo; maps are numpy arrays or device
@@ -88,6 +89,12 @@ class Wav:
         if self.ndim == 1:
             return [int(i) for i in self.indices[:, 0]]
         return [(int(i), int(j)) for i, j in self.indices if i == j]
+
+    def get_minfos_diag(self, copy=True):
+        '''MapInfo objects of the diagonal, in key order (wavtrans.py:253-280).'''
+        import copy as _copy
+        out = [self.minfos[k] for k in sorted(self.get_indices_diag())]
+        return [_copy.deepcopy(m) for m in out] if copy else out
 
     def diag(self):
         '''New block vector holding the diagonal (or a copy of a vector).'''
@@ -182,3 +189,61 @@ def wav2alm(wav, alm, ainfo, spin, w_ell, adjoint=False):
         sht.map2alm(imap, wlm, wav.minfos[idx], winfo, spin, adjoint=adjoint)
         alm_utils.wlm2alm_axisym([wlm], [winfo], w_ell[idx:idx + 1, :], alm=alm, ainfo=ainfo)
     return alm, ainfo
+
+
+def _fourier_plan(fmap, fkernelset):
+    from optweight_amd import dft
+    rdt = dft._REAL[fmap.dtype]
+    plan = fkernelset.device_plan(fmap.device, rdt)
+    ny, nlx = fmap.shape[-2:]
+    if (ny, nlx) != tuple(fkernelset.shape_full):
+        raise ValueError(f'fmap {tuple(fmap.shape)} does not match the kernels '
+                         f'{tuple(fkernelset.shape_full)}')
+    nlead = fmap.numel() // max(ny * nlx, 1)
+    return plan, nlead
+
+
+def f2wav(fmap, wav, fkernelset):
+    """Wavelet maps of 2-D Fourier coefficients fmap (..., ny, nx // 2 + 1):
+    map j = irfft(slice_j(fmap) k_j) on the (ny_j, 2 (nx_j - 1) + 1) grid of
+    wav.maps[j] (wavtrans.py:687-712). One gather kernel cuts and weights
+    every slice; the inverse FFT of each runs per scale."""
+    from optweight_amd import dft, _lib
+    _lib.require_cuda(fmap)
+    fmap = fmap.contiguous()
+    plan, nlead = _fourier_plan(fmap, fkernelset)
+    lead = tuple(fmap.shape[:-2])
+    ny, nlx = fmap.shape[-2:]
+    small = torch.empty(max(nlead * plan['total'], 1), dtype=fmap.dtype, device=fmap.device)
+    _lib.call('owmi_fwav_gather', _lib.dtype_code(fmap), _lib.ptr(fmap), nlead, ny, nlx,
+              len(plan['shapes']), _lib.ptr(plan['desc']), _lib.ptr(plan['rows']),
+              _lib.ptr(plan['kern']), plan['max_count'], _lib.ptr(small),
+              _lib.stream_ptr(fmap.device))
+    for widx, off, ny_j, nx_j in plan['shapes']:
+        blk = small[nlead * off:nlead * (off + ny_j * nx_j)].view(lead + (ny_j, nx_j))
+        map_2d = map_utils.view_2d(wav.maps[widx], wav.minfos[widx])
+        dft.irfft(blk, map_2d)
+    return wav
+
+
+def wav2f(wav, fmap, fkernelset):
+    """2-D Fourier coefficients of a wavelet vector, the adjoint of f2wav:
+    fmap = sum_j pad_j(rfft(wav.maps[j]) k_j), overwriting fmap
+    (wavtrans.py:714-745). The forward FFT of each scale runs per scale;
+    one scatter kernel weights and adds them in scale order."""
+    from optweight_amd import dft, _lib
+    _lib.require_cuda(fmap)
+    if not fmap.is_contiguous():
+        raise ValueError('fmap must be contiguous')
+    plan, nlead = _fourier_plan(fmap, fkernelset)
+    lead = tuple(fmap.shape[:-2])
+    ny, nlx = fmap.shape[-2:]
+    small = torch.empty(max(nlead * plan['total'], 1), dtype=fmap.dtype, device=fmap.device)
+    for widx, off, ny_j, nx_j in plan['shapes']:
+        blk = small[nlead * off:nlead * (off + ny_j * nx_j)].view(lead + (ny_j, nx_j))
+        map_2d = map_utils.view_2d(wav.maps[widx], wav.minfos[widx])
+        dft.rfft(map_2d, blk)
+    _lib.call('owmi_fwav_scatter', _lib.dtype_code(fmap), _lib.ptr(small), nlead, ny, nlx,
+              len(plan['shapes']), _lib.ptr(plan['desc']), _lib.ptr(plan['invrows']),
+              _lib.ptr(plan['kern']), _lib.ptr(fmap), _lib.stream_ptr(fmap.device))
+    return fmap

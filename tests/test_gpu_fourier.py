@@ -1,0 +1,293 @@
+'''
+GPU parity of the 2-D Fourier-wavelet noise model against the oracle,
+through the owmi C ABI (float64 to relative 1e-12 unless stated, float32
+to 2e-5):
+  * dft.rfft / irfft for the four parities of (ny, nx) with leading
+    dimensions, the exact real self-conjugate coefficients, irfft leaving
+    its input unchanged and ignoring the imaginary parts the real-field
+    synthesis drops, and the argument errors of dft.py:13-67;
+  * contract_fxg (data-dependent parity of nx, and stated parity),
+    fmul_2d (shared and per-component diagonals, full matrices, in place);
+  * f2wav / wav2f with digital and smooth scale-discrete kernels;
+  * FWavMatVecF, InvFWavMatVecF (3 CG steps, 1e-10), FInvFWavFMatVecMap with
+    and without a 2-D Fourier matrix, InvSqrtFWavMatVecWav;
+  * estimate_cov_fwav (diag and full, 1e-10);
+  * CGWienerMap.from_arrays_fwav (b vector, A, a 3-step CG trajectory and a
+    constrained-realisation right-hand side from the same wavelet draw,
+    1e-9) and PseudoInvPreconditionerFWav (1e-10).
+'''
+import numpy as np
+import pytest
+import torch
+
+from oracle import fourier as of, geometry as og, layout
+from optweight_amd import (dft, fkernel, wavtrans, operators, noise_utils, solvers, map_utils,
+                           alm_utils, preconditioners)
+
+from test_fourier_host import car_axes, sd_kernel_list
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a = a.cpu().numpy() if isinstance(a, torch.Tensor) else np.asarray(a)
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def cplx(shape, rng):
+    return rng.normal(size=shape) + 1j * rng.normal(size=shape)
+
+
+@pytest.mark.parametrize('ny,nx', [(10, 11), (10, 12), (9, 11), (9, 12)])
+@pytest.mark.parametrize('dtype', ['float64', 'float32'])
+def test_rfft_irfft(dev, ny, nx, dtype):
+    print(f'rfft/irfft {ny}x{nx} {dtype}', flush=True)
+    rng = np.random.default_rng(ny * 100 + nx)
+    m = rng.normal(size=(2, 3, ny, nx))
+    rdt = torch.float64 if dtype == 'float64' else torch.float32
+    tol = 1e-12 if dtype == 'float64' else 2e-6
+    mt = torch.as_tensor(m, device=dev).to(rdt)
+    f = dft.allocate_fmap(mt.shape, rdt, device=dev)
+    dft.rfft(mt, f)
+    ref = of.rfft(m)
+    assert rel(f, ref) < tol
+    fh = f.cpu().numpy()
+    assert np.all(fh[..., 0, 0].imag == 0)
+    if nx % 2 == 0:
+        assert np.all(fh[..., 0, -1].imag == 0)
+        if ny % 2 == 0:
+            assert np.all(fh[..., ny // 2, -1].imag == 0)
+    # irfft of general (non-Hermitian) coefficients: the real-field synthesis
+    fin = cplx((2, 3, ny, nx // 2 + 1), rng)
+    ft = torch.as_tensor(fin, device=dev).to(f.dtype)
+    keep = ft.clone()
+    out = torch.empty_like(mt)
+    dft.irfft(ft, out)
+    assert torch.equal(ft, keep)
+    assert rel(out, of.irfft(fin, nx)) < tol
+    with pytest.raises(TypeError):
+        dft.rfft(mt, torch.empty_like(f, dtype=torch.complex64 if dtype == 'float64'
+                                      else torch.complex128))
+    with pytest.raises(ValueError):
+        dft.irfft(ft[..., :-1, :], out)
+
+
+@pytest.mark.parametrize('ny,nx', [(4, 5), (4, 4), (5, 5), (5, 4), (12, 30)])
+def test_contract_fxg(dev, ny, nx):
+    print(f'contract_fxg {ny}x{nx}', flush=True)
+    rng = np.random.default_rng(7)
+    f, g = rng.normal(size=(2, ny, nx)), rng.normal(size=(2, ny, nx))
+    F, G = of.rfft(f), of.rfft(g)
+    Ft, Gt = [torch.empty(F.shape, dtype=torch.complex128, device=dev) for _ in range(2)]
+    dft.rfft(torch.as_tensor(f, device=dev), Ft)
+    dft.rfft(torch.as_tensor(g, device=dev), Gt)
+    got = dft.contract_fxg(Ft, Gt)
+    assert abs(got - of.contract_fxg(F, G)) < 1e-12 * abs(np.sum(f * g)) + 1e-13
+    assert abs(got - np.sum(f * g)) < 1e-11
+    a, b = cplx(F.shape, rng), cplx(F.shape, rng)
+    at, bt = torch.as_tensor(a, device=dev), torch.as_tensor(b, device=dev)
+    assert abs(dft.contract_fxg(at, bt) - of.contract_fxg(a, b)) < 1e-12 * np.abs(a).sum()
+    for even in (False, True):
+        v = float(dft.contract_fxg_dev(at, bt, nx_even=even))
+        ref = 2 * np.sum(a * np.conj(b)).real - np.sum(a[..., 0] * np.conj(b[..., 0])).real
+        if even:
+            ref -= np.sum(a[..., -1] * np.conj(b[..., -1])).real
+        assert abs(v - ref) < 1e-12 * np.abs(a).sum()
+
+
+@pytest.mark.parametrize('kind', ['shared', 'diag', 'full', 'full_inplace', 'full5'])
+def test_fmul_2d(dev, kind):
+    print(f'fmul_2d {kind}', flush=True)
+    rng = np.random.default_rng(11)
+    npol = 5 if kind == 'full5' else 3
+    f = cplx((npol, 6, 7), rng)
+    mat = {'shared': rng.normal(size=(6, 7)), 'diag': rng.normal(size=(npol, 6, 7))}.get(
+        kind, rng.normal(size=(npol, npol, 6, 7)))
+    ft = torch.as_tensor(f, device=dev)
+    out = dft.fmul_2d(ft, mat, out=ft if kind == 'full_inplace' else None)
+    assert rel(out, of.fmul_2d(f, mat)) < 1e-14
+    if kind == 'full_inplace':
+        assert out.data_ptr() == ft.data_ptr()
+    f32 = torch.as_tensor(f, device=dev).to(torch.complex64)
+    assert rel(dft.fmul_2d(f32, mat), of.fmul_2d(f, mat)) < 1e-6
+    if kind == 'diag':
+        fb = np.stack([f, 2 * f])
+        got = dft.fmul_2d(torch.as_tensor(fb, device=dev), mat)
+        assert rel(got, np.stack([of.fmul_2d(f, mat), 2 * of.fmul_2d(f, mat)])) < 1e-14
+
+
+@pytest.fixture(scope='module', params=['digital', 'smooth'])
+def fsetup(request):
+    shape, wcs = car_axes(20, nx=41)
+    ly, lx = dft.laxes_real(shape, wcs)
+    fks = fkernel.get_sd_kernels_fourier(ly, lx, 2, digital=request.param == 'digital')
+    return shape, fks, sd_kernel_list(fks)
+
+
+def test_f2wav_wav2f(dev, fsetup):
+    shape, fks, kern = fsetup
+    print(f'f2wav/wav2f {len(fks)} kernels', flush=True)
+    rng = np.random.default_rng(1)
+    F = cplx((3, shape[0], shape[1] // 2 + 1), rng)
+    ref = of.f2wav(F, kern)
+    wav = fks.get_wav_vec((3,), dtype=np.float64, device=dev)
+    wavtrans.f2wav(torch.as_tensor(F, device=dev), wav, fks)
+    for j, r in ref.items():
+        assert rel(wav.maps[j].reshape(r.shape), r) < 1e-12, j
+    w = {j: rng.normal(size=r.shape) for j, r in ref.items()}
+    wt = fks.get_wav_vec((3,), dtype=np.float64, device=dev)
+    for j in w:
+        wt.maps[j].copy_(torch.as_tensor(w[j].reshape(3, -1), device=dev))
+    out = torch.full(F.shape, np.nan, dtype=torch.complex128, device=dev)
+    wavtrans.wav2f(wt, out, fks)
+    assert rel(out, of.wav2f(w, kern, F.shape[-2:])) < 1e-12
+    wav32 = fks.get_wav_vec((3,), dtype=np.float32, device=dev)
+    wavtrans.f2wav(torch.as_tensor(F, device=dev).to(torch.complex64), wav32, fks)
+    for j, r in ref.items():
+        assert rel(wav32.maps[j].reshape(r.shape), r) < 2e-6, j
+
+
+def cov_wav_for(fks, npol, full, seed):
+    rng = np.random.default_rng(seed)
+    cov = wavtrans.Wav(2, dtype=np.float64)
+    ref = {}
+    for j, fk in fks:
+        ny_j, nx_j = fk.fkernel.shape
+        minfo = map_utils.get_minfo('CC', ny_j, 2 * (nx_j - 1) + 1)
+        d = np.exp(0.3 * rng.normal(size=(npol, minfo.npix)))
+        if full:
+            m = np.zeros((npol, npol, minfo.npix))
+            for i in range(npol):
+                m[i, i] = d[i]
+            if npol > 1:
+                m[0, 1] = m[1, 0] = 0.3 * np.sqrt(d[0] * d[1])
+        else:
+            m = d
+        cov.add((j, j), m, minfo)
+        ref[j] = m
+    return cov, ref
+
+
+@pytest.mark.parametrize('full', [False, True])
+def test_fwav_operators(dev, fsetup, full):
+    shape, fks, kern = fsetup
+    print(f'FWav operators full={full}', flush=True)
+    ny, nx = shape
+    cov, ref = cov_wav_for(fks, 3, full, 2)
+    rng = np.random.default_rng(3)
+    F = of.rfft(rng.normal(size=(3, ny, nx)))
+    Ft = torch.as_tensor(F, device=dev)
+    shape_f = F.shape[-2:]
+    got = operators.FWavMatVecF(cov, fks)(Ft)
+    assert rel(got, of.fwav_op(kern, shape_f, ref)(F)) < 1e-12
+    assert rel(Ft, F) == 0
+    inv = operators.InvFWavMatVecF(cov, fks, nsteps=3)
+    assert rel(inv(Ft), of.inv_fwav_op(kern, shape_f, ref, nsteps=3)(F)) < 1e-10
+    fmat = np.exp(0.2 * rng.normal(size=(3, ny, nx // 2 + 1)))
+    for fm, px in ((None, 0), (fmat, -0.5)):
+        op = operators.FInvFWavFMatVecMap(map_utils.get_minfo('CC', ny, nx), cov, fks, fm,
+                                          power_x=px, nsteps=3)
+        m = rng.normal(size=(3, ny * nx))
+        oop = of.finv_fwav_map_op(ny, nx, kern, ref, fm, power_x=px, nsteps=3)
+        assert rel(op(torch.as_tensor(m, device=dev)), oop(m)) < 1e-10
+    w = {j: rng.normal(size=(3, k.shape[0], 2 * (k.shape[1] - 1) + 1)) for j, k, _, _ in kern}
+    wt = fks.get_wav_vec((3,), dtype=np.float64, device=dev)
+    for j in w:
+        wt.maps[j].copy_(torch.as_tensor(w[j].reshape(3, -1), device=dev))
+    sq = operators.InvSqrtFWavMatVecWav(cov, fks, inv_op=inv)
+    oinv = of.inv_fwav_op(kern, shape_f, ref, nsteps=3)
+    assert rel(sq(wt), of.inv_sqrt_fwav_op(kern, shape_f, ref, oinv)(w)) < 1e-10
+
+
+@pytest.mark.parametrize('diag', [True, False])
+def test_estimate_cov_fwav(dev, fsetup, diag):
+    shape, fks, kern = fsetup
+    print(f'estimate_cov_fwav diag={diag}', flush=True)
+    ny, nx = shape
+    rng = np.random.default_rng(4)
+    F = of.rfft(rng.normal(size=(3, ny, nx)) * np.linspace(1, 2, nx))
+    Ft = torch.as_tensor(F, device=dev)
+    tmpl = fks.get_wav_vec((3,), dtype=np.float64, device=dev)
+    got = noise_utils.estimate_cov_fwav(Ft, fks, tmpl, diag=diag)
+    assert rel(Ft, F) == 0
+    ref = of.estimate_cov_fwav(F, kern, [fk.lmax for _, fk in fks],
+                               [fk.modlmap() for _, fk in fks], diag=diag)
+    for j, r in ref.items():
+        assert tuple(got.maps[(j, j)].shape) == r.shape
+        assert rel(got.maps[(j, j)], r) < 1e-10, j
+
+
+def solver_problem(fks, lmax, ny, nx, seed=6):
+    rng = np.random.default_rng(seed)
+    minfo = map_utils.get_minfo('CC', ny, nx)
+    geom = og.Geom(minfo.theta, minfo.weight, minfo.nphi, phi0=minfo.phi0)
+    cov, ref = cov_wav_for(fks, 3, True, seed)
+    ell = np.arange(lmax + 1)
+    icov_ell = np.zeros((3, 3, lmax + 1))
+    c = 1e2 * (ell + 10.) ** -2.5
+    icov_ell[0, 0], icov_ell[1, 1], icov_ell[2, 2] = 1 / c, 20 / c, 200 / c
+    icov_ell[1:, 1:, :2] = 0
+    imap = rng.normal(size=(3, minfo.npix))
+    mask = (rng.uniform(size=(3, minfo.npix)) > 0.2).astype(np.float64)
+    return minfo, geom, cov, ref, icov_ell, imap, mask
+
+
+def test_from_arrays_fwav(dev, fsetup, monkeypatch):
+    shape, fks, kern = fsetup
+    print('from_arrays_fwav', flush=True)
+    lmax = 20
+    ny, nx = shape
+    minfo, geom, cov, ref, icov_ell, imap, mask = solver_problem(fks, lmax, ny, nx)
+    ainfo = alm_utils.AlmInfo(lmax)
+    b_ell = np.exp(-0.002 * np.arange(lmax + 1) ** 2) * np.ones((3, 1))
+    fmat = np.exp(0.1 * np.random.default_rng(8).normal(size=(3, ny, nx // 2 + 1)))
+    kw = dict(b_ell=b_ell, mask_pix=mask, cov_noise_2d=fmat)
+    sol = solvers.CGWienerMap.from_arrays_fwav(imap, minfo, ainfo, icov_ell, cov, fks,
+                                               device=dev, **kw)
+    osol = of.from_arrays_fwav(imap, geom, lmax, ny, nx, icov_ell, ref, kern, **kw)
+    assert rel(sol.b0, osol.b0) < 1e-9
+    rng = np.random.default_rng(9)
+    v = cplx((3, layout.nelem(lmax)), rng)
+    v[:, :lmax + 1] = v[:, :lmax + 1].real
+    assert rel(sol.a_matrix(torch.as_tensor(v, device=dev)), osol.a_matrix(v)) < 1e-9
+    sol.init_solver()
+    cg = osol.init_solver()
+    for i in range(3):
+        sol.step()
+        cg.step()
+        assert rel(sol.x, cg.x) < 1e-8, f'step {i}'
+    # constrained realisation: both sides draw the same unit-variance maps
+    draws = {}
+    unseeded = noise_utils.unit_var_wav
+
+    def seeded(minfos, preshape, dtype, seed=None):
+        draws['w'] = unseeded(minfos, preshape, dtype, seed=21)
+        return draws['w']
+    monkeypatch.setattr(noise_utils, 'unit_var_wav', seeded)
+    solc = solvers.CGWienerMap.from_arrays_fwav(imap, minfo, ainfo, icov_ell, cov, fks,
+                                                device=dev, draw_constr=True, seed=5, **kw)
+    wd = {j: draws['w'].maps[j].reshape(3, k.shape[0], -1) for j, k, _, _ in kern}
+    osolc = of.from_arrays_fwav(imap, geom, lmax, ny, nx, icov_ell, ref, kern,
+                                draw_constr=True, seed=5, wav_draw=wd, **kw)
+    assert rel(solc.b0, osolc.b0) < 1e-9
+
+
+def test_pinv_fwav(dev, fsetup):
+    shape, fks, kern = fsetup
+    print('PseudoInvPreconditionerFWav', flush=True)
+    lmax = 20
+    ny, nx = shape
+    minfo, geom, cov, ref, icov_ell, imap, mask = solver_problem(fks, lmax, ny, nx, seed=10)
+    ainfo = alm_utils.AlmInfo(lmax)
+    itau = np.eye(3)[:, :, None] * np.linspace(2, 1, lmax + 1)
+    fmat = np.exp(0.1 * np.random.default_rng(12).normal(size=(3, ny, nx // 2 + 1)))
+    prec = preconditioners.PseudoInvPreconditionerFWav(ainfo, icov_ell, cov, fks, [0, 2], itau,
+                                                       fmat, mask, minfo)
+    oprec = of.PseudoInvPreconditionerFWav(lmax, icov_ell, geom, ny, nx, [0, 2], itau, fmat,
+                                           mask)
+    rng = np.random.default_rng(13)
+    v = cplx((3, layout.nelem(lmax)), rng)
+    v[:, :lmax + 1] = v[:, :lmax + 1].real
+    vt = torch.as_tensor(v, device=dev)
+    assert rel(prec(vt), oprec(v)) < 1e-10
+    assert rel(vt, v) == 0
