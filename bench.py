@@ -473,6 +473,103 @@ def noise_est_bench(args, dev):
         'seconds': el, 'pixels': npix}}))
 
 
+def fwav_bench(args, dev):
+    """The 2-D Fourier-wavelet noise model (solvers.py:577-695) at the
+    full-sky CAR shape of band limit --wavelet-lmax: temperature map on the
+    Clenshaw-Curtis grid of L + 1 rings by 2 L columns (float32), digital
+    scale-discrete kernels (lambda 2) on its Fourier plane, a lognormal
+    per-pixel covariance per scale. Times N^-1 m (FInvFWavFMatVecMap, 3
+    inner CG steps: 7 Fourier-wavelet round trips) and its kernels with HIP
+    events (rfft, irfft, slice gather and scatter, inner product, Fourier
+    diagonal), and the oracle's float64 numpy version of the same operator
+    on a bounded sample (L / 2). Prints one JSON line."""
+    from optweight_amd import dft, fkernel, wavtrans, operators, map_utils
+    L = args.wavelet_lmax + 1
+    ny, nx = L + 1, 2 * L
+    wcs = map_utils.CarWCS(cdelt=[-360. / nx, 180. / L], crpix=[nx // 2 + 1, ny // 2 + 1])
+    t0 = time.perf_counter()
+    ly, lx = dft.laxes_real((ny, nx), wcs)
+    fks = fkernel.get_sd_kernels_fourier(ly, lx, 2, digital=True)
+    rng = np.random.default_rng(0)
+    cov = wavtrans.Wav(2, dtype=np.float32)
+    for j, fk in fks:
+        ny_j, nx_j = fk.fkernel.shape
+        minfo_j = map_utils.get_minfo('CC', ny_j, 2 * (nx_j - 1) + 1)
+        cov.add((j, j), np.exp(0.3 * rng.normal(size=(1, minfo_j.npix))).astype(np.float32),
+                minfo_j)
+    minfo = map_utils.get_minfo('CC', ny, nx)
+    op = operators.FInvFWavFMatVecMap(minfo, cov, fks, None, power_x=0, nsteps=3)
+    m = torch.as_tensor(rng.normal(size=(1, minfo.npix)), dtype=torch.float32, device=dev)
+    op(m)
+    torch.cuda.synchronize()
+    log(f'fwav setup {time.perf_counter() - t0:.1f}s, {len(fks)} scales, '
+        f'kernel blocks {[s[2:] for s in fks.device_plan(dev, torch.float32)["shapes"]]}')
+    reps = max(args.steps, 1)
+    t_apply = time_kernel(lambda: op(m), reps=reps)
+    # kernels at this shape
+    m2 = m.reshape(1, ny, nx)
+    f = dft.allocate_fmap(tuple(m2.shape), torch.float32, device=dev)
+    out = torch.empty_like(m2)
+    t_rfft = time_kernel(lambda: dft.rfft(m2, f))
+    t_irfft = time_kernel(lambda: dft.irfft(f, out))
+    plan = fks.device_plan(dev, torch.float32)
+    nlx = nx // 2 + 1
+    small = torch.empty(plan['total'], dtype=torch.complex64, device=dev)
+    st = _lib.stream_ptr(dev)
+    nsc = len(plan['shapes'])
+    t_gather = time_kernel(lambda: _lib.call(
+        'owmi_fwav_gather', _lib.F32, _lib.ptr(f), 1, ny, nlx, nsc, _lib.ptr(plan['desc']),
+        _lib.ptr(plan['rows']), _lib.ptr(plan['kern']), plan['max_count'], _lib.ptr(small), st))
+    t_scatter = time_kernel(lambda: _lib.call(
+        'owmi_fwav_scatter', _lib.F32, _lib.ptr(small), 1, ny, nlx, nsc, _lib.ptr(plan['desc']),
+        _lib.ptr(plan['invrows']), _lib.ptr(plan['kern']), _lib.ptr(f), st))
+    dot_out = torch.empty((), dtype=torch.float64, device=dev)
+    t_dot = time_kernel(lambda: dft.contract_fxg_dev(f, f, out=dot_out))
+    fdiag = torch.ones((ny, nlx), dtype=torch.float32, device=dev)
+    t_fmul = time_kernel(lambda: dft.fmul_2d(f, fdiag, out=f))
+    nf = ny * nlx
+    ns = plan['total']
+    kern_bytes = {
+        'fwav_gather': (ns * (8 + 4 + 8) + ns * 4, t_gather),
+        'fwav_scatter': (ns * (8 + 4) + nf * 8, t_scatter),
+        'fourier_dot': (2 * nf * 8, t_dot),
+        'fmul_diag': (nf * (8 + 4 + 8), t_fmul),
+    }
+    kernels = {}
+    for name, (nbytes, ms) in kern_bytes.items():
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        kernels[name] = {'ms': ms, 'algorithmic_bytes': nbytes, 'GB/s': gbs,
+                         'frac_hbm': gbs / PEAK_HBM_GBS}
+    log(f'fwav: N^-1 apply {t_apply:.2f} ms; rfft {t_rfft:.3f} ms, irfft {t_irfft:.3f} ms; '
+        + ', '.join(f'{k} {v["ms"]:.3f} ms ({v["GB/s"]:.0f} GB/s)' for k, v in kernels.items()))
+    cpu = None
+    if args.cpu_baseline == 'on':
+        from oracle import fourier as ofour
+        Ls = L // 2
+        nys, nxs = Ls + 1, 2 * Ls
+        wcs_s = map_utils.CarWCS(cdelt=[-360. / nxs, 180. / Ls], crpix=[nxs // 2 + 1, nys // 2 + 1])
+        lys, lxs = dft.laxes_real((nys, nxs), wcs_s)
+        fks_s = fkernel.get_sd_kernels_fourier(lys, lxs, 2, digital=True)
+        kern_s = [(j, np.asarray(fk.fkernel, dtype=np.float64), fk.slices_y, fk.slice_x)
+                  for j, fk in fks_s]
+        cov_s = {j: np.exp(0.3 * rng.normal(size=(1, k.shape[0] * (2 * k.shape[1] - 1))))
+                 for j, k, _, _ in kern_s}
+        oop = ofour.finv_fwav_map_op(nys, nxs, kern_s, cov_s, None, power_x=0, nsteps=3)
+        ms_ = rng.normal(size=(1, nys * nxs))
+        t1 = time.perf_counter()
+        oop(ms_)
+        el = time.perf_counter() - t1
+        cpu = {'value': 1.0 / el, 'unit': 'N^-1 applications/s', 'cores': 1, 'kind': 'port',
+               'sample': f'oracle FInvFWavFMatVecMap (numpy, float64), T, CC {nys}x{nxs}, '
+                         f'{len(kern_s)} scales, one application ({el:.1f} s)'}
+        log(f'cpu baseline: {cpu["sample"]}')
+    print(json.dumps({'fwav': {
+        'workload': f'FInvFWavFMatVecMap (3 inner CG steps), T, CC {ny}x{nx} float32, '
+                    f'{len(fks)} digital scale-discrete scales (lambda 2)',
+        'ms_per_apply': t_apply, 'applications_per_s': 1e3 / t_apply,
+        'rfft_ms': t_rfft, 'irfft_ms': t_irfft, 'kernels': kernels, 'cpu_baseline': cpu}}))
+
+
 def ragged_bench(args, dev):
     '''Constrained realisations: per band limit one batched mixed-precision
     solve of `batch` right-hand sides (data + S^-1/2 w + P^T N^-1/2 w', seeds
@@ -560,6 +657,9 @@ def main():
                          'CG preconditioner) on the TQU workload; prints its own JSON line')
     ap.add_argument('--prec-cg-lmax', type=int, default=0,
                     help='band limit of the masked CG preconditioner (default: --lmax)')
+    ap.add_argument('--fwav', action='store_true',
+                    help='time the 2-D Fourier-wavelet noise model inverse on the full-sky CAR '
+                         'map of band limit --wavelet-lmax; prints its own JSON line')
     ap.add_argument('--roofline-only', action='store_true',
                     help='only build the workload geometry and time the roofline kernel '
                          '(for counter runs), after one copy of known size')
@@ -589,6 +689,9 @@ def main():
         return
     if args.noise_est:
         noise_est_bench(args, dev)
+        return
+    if args.fwav:
+        fwav_bench(args, dev)
         return
     if args.streaming:
         calibration_copy(dev)
