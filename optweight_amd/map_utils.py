@@ -504,3 +504,100 @@ def rand_map_pix(cov_pix, seed):
         return uv * np.sqrt(cov_pix)
     sq = mat_utils.matpow(cov_pix, 0.5)
     return np.einsum('ijk, jk -> ik', sq, uv, optimize=True)
+
+
+# ------------------------------------------------------------ HDF5 files
+# Datasets and attributes of the reference's files (map_utils.py:1902-2101):
+# a map info is the datasets theta, nphi, phi0, offsets, stride, weight; a
+# map file holds dataset 'map' and group 'minfo', and for maps stored as
+# upper-triangular elements the attributes 'symm_axis' and 'shape_orig'.
+# The append_* / *_from_hdf functions take any h5py-like group; opening
+# files needs h5py.
+
+def _h5py():
+    try:
+        import h5py
+    except ImportError as exc:
+        raise ImportError('reading and writing map files needs h5py') from exc
+    return h5py
+
+
+def _host(a):
+    return a.detach().cpu().numpy() if isinstance(a, torch.Tensor) else np.asarray(a)
+
+
+def _with_ext(fname):
+    import os
+    return fname if os.path.splitext(fname)[1] else fname + '.hdf5'
+
+
+def append_minfo_to_hdf(hfile, minfo):
+    for name in ('theta', 'nphi', 'phi0', 'offsets', 'stride', 'weight'):
+        hfile.create_dataset(name, data=np.asarray(getattr(minfo, name)))
+
+
+def minfo_from_hdf(hfile):
+    return MapInfo(theta=hfile['theta'][()], nphi=hfile['nphi'][()], phi0=hfile['phi0'][()],
+                   offsets=hfile['offsets'][()], stride=hfile['stride'][()],
+                   weight=hfile['weight'][()])
+
+
+def write_minfo(fname, minfo):
+    with _h5py().File(_with_ext(fname), 'w') as hfile:
+        append_minfo_to_hdf(hfile, minfo)
+
+
+def read_minfo(fname):
+    with _h5py().File(fname, 'r') as hfile:
+        return minfo_from_hdf(hfile)
+
+
+def _is_seq_of_seq(obj):
+    try:
+        iter(obj)
+    except TypeError:
+        return False
+    for item in obj:
+        try:
+            iter(item)
+        except TypeError:
+            return False
+    return True
+
+
+def append_map_to_hdf(hfile, imap, minfo, symm_axes=None):
+    """Dataset 'map' (only the upper-triangular elements when the map is
+    symmetric in symm_axes, after merging axis groups if symm_axes is a
+    sequence of sequences) and group 'minfo'."""
+    imap = _host(imap)
+    if symm_axes is not None:
+        shape_orig = imap.shape
+        if _is_seq_of_seq(symm_axes):
+            imap, symm_axes = mat_utils.flattened_view(imap, [list(a) for a in symm_axes],
+                                                       return_flat_axes=True)
+        imap = mat_utils.symm2triu(imap, axes=symm_axes)
+        hfile.attrs['symm_axis'] = symm_axes[0]
+        hfile.attrs['shape_orig'] = shape_orig
+    hfile.create_dataset('map', data=imap)
+    append_minfo_to_hdf(hfile.create_group('minfo'), minfo)
+
+
+def map_from_hdf(hfile):
+    omap = hfile['map'][()]
+    symm_axis = hfile.attrs.get('symm_axis', None)
+    if symm_axis is not None:
+        omap = mat_utils.triu2symm(omap, int(symm_axis))
+    shape_orig = hfile.attrs.get('shape_orig', None)
+    if shape_orig is not None:
+        omap = omap.reshape(tuple(int(n) for n in shape_orig))
+    return omap, minfo_from_hdf(hfile['minfo'])
+
+
+def write_map(fname, imap, minfo, symm_axes=None):
+    with _h5py().File(_with_ext(fname), 'w') as hfile:
+        append_map_to_hdf(hfile, imap, minfo, symm_axes=symm_axes)
+
+
+def read_map(fname):
+    with _h5py().File(fname, 'r') as hfile:
+        return map_from_hdf(hfile)

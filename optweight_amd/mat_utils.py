@@ -153,3 +153,63 @@ def matvec(mat, vec, inplace=False):
     _lib.call('owmi_pix_matvec', code, _lib.ptr(mat), 1, _lib.ptr(vec), _lib.ptr(out), npol,
               npix, ntrans, st)
     return out
+
+
+def symm2triu(mat, axes, return_axis=False):
+    """Copy of the upper-triangular elements (row-major) of a matrix that is
+    symmetric in two adjacent axes, which become one axis of N (N + 1) / 2
+    (mat_utils.py:7-65)."""
+    mat = np.asarray(mat)
+    if len(axes) != 2:
+        raise ValueError(f'Need two axes, got {len(axes)} ({axes})')
+    axes = np.sort([a + mat.ndim if a < 0 else a for a in axes])
+    if mat.shape[axes[0]] != mat.shape[axes[1]]:
+        raise ValueError(f'Matrix (shape : {mat.shape}) is not N x N in specified axes : {axes}.')
+    if axes[1] - axes[0] != 1:
+        raise ValueError(f'Axes are not adjacent: {axes}')
+    n = mat.shape[axes[0]]
+    mask = np.triu(np.ones((n, n), dtype=bool))
+    out = np.ascontiguousarray(mat[(slice(None),) * axes[0] + (mask,)])
+    return (out, int(axes[0])) if return_axis else out
+
+
+def triu2symm(mat, axis):
+    """Inverse of symm2triu: the upper-triangular elements in axis expanded to
+    a symmetric N x N pair of axes (mat_utils.py:67-117)."""
+    mat = np.asarray(mat)
+    if axis < 0:
+        axis += mat.ndim
+    n = (-1 + np.sqrt(1 + 8 * mat.shape[axis])) / 2
+    if np.floor(n) != n:
+        raise ValueError(f'Cannot intepret axis : {axis} of shape {mat.shape} '
+                         f'as upper triangular elements')
+    n = int(n)
+    out = np.zeros(mat.shape[:axis] + (n, n) + mat.shape[axis + 1:], dtype=mat.dtype)
+    lead = (slice(None),) * axis
+    for k, (i, j) in enumerate(zip(*np.triu_indices(n))):
+        out[lead + (i, j)] = mat[lead + (k,)]
+        out[lead + (j, i)] = mat[lead + (k,)]
+    return out
+
+
+def flattened_view(mat, axes, return_flat_axes=False):
+    """View of mat with each group of axes merged into one (the first axis
+    of the group keeps its place); also the merged axes' indices in the
+    output (mat_utils.py:157-218)."""
+    shape_in = np.asarray(mat.shape)
+    axes = [[a + mat.ndim if a < 0 else a for a in sub] for sub in axes]
+    flat = [a for sub in axes for a in sub]
+    if np.unique(flat).size != len(flat):
+        raise ValueError(f'Axes cannot contain duplicates, got : {axes}')
+    shape_out = shape_in.copy()
+    merged = np.zeros_like(shape_out)
+    for sub in axes:
+        sub = np.asarray(sub)
+        shape_out[sub[0]] = np.prod(shape_in[sub])
+        shape_out[sub[1:]] = 0
+        merged[sub[0]] = 1
+    keep = shape_out != 0
+    view = mat.reshape(tuple(int(n) for n in shape_out[keep]))
+    if return_flat_axes:
+        return view, [int(i) for i in np.nonzero(merged[keep])[0]]
+    return view

@@ -247,3 +247,48 @@ def wav2f(wav, fmap, fkernelset):
               len(plan['shapes']), _lib.ptr(plan['desc']), _lib.ptr(plan['invrows']),
               _lib.ptr(plan['kern']), _lib.ptr(fmap), _lib.stream_ptr(fmap.device))
     return fmap
+
+
+def _map_key(idx):
+    idx = np.asarray(idx)
+    if idx.size == 1:
+        return f'm_{int(idx.ravel()[0])}', int(idx.ravel()[0])
+    return f'm_{int(idx[0])}_{int(idx[1])}', (int(idx[0]), int(idx[1]))
+
+
+def append_wav_to_hdf(hfile, wav, extra=None, **kwargs):
+    """Attributes ndim, indices, preshape and dtype (numpy type character),
+    optional extra datasets, and one group m_j or m_j_k per map written by
+    map_utils.append_map_to_hdf (wavtrans.py:548-590)."""
+    hfile.attrs['ndim'] = wav.ndim
+    hfile.attrs['indices'] = wav.indices
+    hfile.attrs['preshape'] = wav.preshape
+    hfile.attrs['dtype'] = np.dtype(wav.dtype).char
+    if extra is not None:
+        for key in extra:
+            hfile.create_dataset(key, data=extra[key])
+    for idx in wav.indices:
+        dname, key = _map_key(idx)
+        map_utils.append_map_to_hdf(hfile.create_group(dname), wav.maps[key], wav.minfos[key],
+                                    **kwargs)
+
+
+def wav_from_hdf(hfile, extra=None):
+    extra_dict = {key: hfile[key][()] for key in extra} if extra is not None else None
+    wav = Wav(int(hfile.attrs['ndim']), preshape=tuple(int(n) for n in hfile.attrs['preshape']),
+              dtype=np.dtype(hfile.attrs['dtype']))
+    for idx in hfile.attrs['indices']:
+        dname, key = _map_key(idx)
+        m_arr, minfo = map_utils.map_from_hdf(hfile[dname])
+        wav.add(np.asarray(key), m_arr, minfo)
+    return (wav, extra_dict) if extra is not None else wav
+
+
+def write_wav(fname, wav, extra=None, **kwargs):
+    with map_utils._h5py().File(map_utils._with_ext(fname), 'w') as hfile:
+        append_wav_to_hdf(hfile, wav, extra=extra, **kwargs)
+
+
+def read_wav(fname, extra=None):
+    with map_utils._h5py().File(fname, 'r') as hfile:
+        return wav_from_hdf(hfile, extra=extra)
