@@ -519,7 +519,7 @@ def fwav_bench(args, dev):
     nsc = len(plan['shapes'])
     t_gather = time_kernel(lambda: _lib.call(
         'owmi_fwav_gather', _lib.F32, _lib.ptr(f), 1, ny, nlx, nsc, _lib.ptr(plan['desc']),
-        _lib.ptr(plan['rows']), _lib.ptr(plan['kern']), plan['max_count'], _lib.ptr(small), st))
+        _lib.ptr(plan['rows']), _lib.ptr(plan['kern']), plan['nblocks'], _lib.ptr(small), st))
     t_scatter = time_kernel(lambda: _lib.call(
         'owmi_fwav_scatter', _lib.F32, _lib.ptr(small), 1, ny, nlx, nsc, _lib.ptr(plan['desc']),
         _lib.ptr(plan['invrows']), _lib.ptr(plan['kern']), _lib.ptr(f), st))

@@ -203,13 +203,14 @@ int owmi_fmul_generic(int dtype, const void *mat, const void *in, void *out,
  * Kernel slices of the Fourier-wavelet transform: block j of the packed
  * buffer small (offset nlead * desc[8j], shape (nlead, ny_j = desc[8j+1],
  * nx_j = desc[8j+2])) receives full[b, rows[desc[8j+4] + r], desc[8j+3] + c] times
- * kern[desc[8j+5] + r nx_j + c]; max_count = max_j ny_j nx_j. desc, rows
- * and kern are device arrays. The gather half of wavtrans.f2wav
+ * kern[desc[8j+5] + r nx_j + c]; the launch has nblocks = sum_j
+ * ceil(ny_j nx_j / 256) thread blocks, scale j's first one at desc[8j+6]
+ * (ascending in j). desc, rows and kern are device arrays. The gather half of wavtrans.f2wav
  * (optweight/wavtrans.py:687-712, dft.slice_fmap dft.py:498-533).
  */
 int owmi_fwav_gather(int dtype, const void *full, long long nlead, long long ny,
                      long long nlx, int nscale, const long long *desc,
-                     const int *rows, const void *kern, long long max_count,
+                     const int *rows, const void *kern, long long nblocks,
                      void *small, void *stream);
 
 /*

@@ -11,7 +11,8 @@
 // full row rows[j][r] and its columns from x0_j .. x0_j + nx_j - 1.
 // Descriptor table (int64, kDescLen per scale): block offset in complex
 // elements, ny_j, nx_j, x0_j, offset into the row table, offset into the
-// packed kernel values (ny_j * nx_j reals per scale).
+// packed kernel values (ny_j * nx_j reals per scale), first thread block of
+// the scale in the gather's block list (blocks of kThreads elements).
 
 #include <map>
 #include <mutex>
@@ -97,22 +98,25 @@ fmul_diag_kernel(const T *__restrict__ d, const cplx_t<T> *in, cplx_t<T> *out, l
 // ------------------------------------------------------- gather / scatter
 
 // small_j[b, r, c] = full[b, rows_j[r], x0_j + c] * kern_j[r, c], block j
-// starting at nlead * off_j in small;
-// grid (ceil(max_j ny_j nx_j / 256), nscale, nlead)
+// starting at nlead * off_j in small; grid (sum_j ceil(ny_j nx_j / 256),
+// 1, nlead), each thread block inside one scale (its first block in desc).
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
 fwav_gather_kernel(const cplx_t<T> *__restrict__ full, long long ny, long long nlx,
-                   const long long *__restrict__ desc, const int *__restrict__ rows,
-                   const T *__restrict__ kern, cplx_t<T> *__restrict__ small) {
-  const long long *dj = desc + (long long)blockIdx.y * kDescLen;
-  const long long off = dj[0], nyj = dj[1], nxj = dj[2], x0 = dj[3], roff = dj[4],
-                  koff = dj[5];
-  const long long cnt = nyj * nxj;
-  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+                   int nscale, const long long *__restrict__ desc,
+                   const int *__restrict__ rows, const T *__restrict__ kern,
+                   cplx_t<T> *__restrict__ small) {
+  int j = 0;
+  while (j + 1 < nscale && desc[(long long)(j + 1) * kDescLen + 6] <= (long long)blockIdx.x) ++j;
+  const long long *dj = desc + (long long)j * kDescLen;
+  const long long off = dj[0], x0 = dj[3], roff = dj[4], koff = dj[5];
+  const unsigned nxj = (unsigned)dj[2];
+  const long long cnt = dj[1] * dj[2];
+  const long long e = ((long long)blockIdx.x - dj[6]) * blockDim.x + threadIdx.x;
   if (e >= cnt) return;
   const long long b = blockIdx.z;
-  const long long r = e / nxj;
-  const long long c = e - r * nxj;
+  const unsigned r = (unsigned)e / nxj;
+  const long long c = e - (long long)r * nxj;
   const long long y = rows[roff + r];
   const cplx_t<T> v = full[(b * ny + y) * nlx + x0 + c];
   const T k = kern[koff + e];
@@ -122,19 +126,19 @@ fwav_gather_kernel(const cplx_t<T> *__restrict__ full, long long ny, long long n
 // full[b, y, x] = sum_j [y, x in slice j] small_j[b, r_j(y), x - x0_j] *
 // kern_j[r_j(y), x - x0_j], scales added in table order from zero;
 // invrows[j * ny + y] = r_j(y) or -1. Every element of full is written.
+// grid (ceil(nlx / 256), ny, nlead): the row and the scales' row indices
+// are uniform per thread block.
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
 fwav_scatter_kernel(const cplx_t<T> *__restrict__ small, long long ny, long long nlx,
                     long long nlead, int nscale, const long long *__restrict__ desc,
                     const int *__restrict__ invrows, const T *__restrict__ kern,
                     cplx_t<T> *__restrict__ full) {
-  const long long n = nlead * ny * nlx;
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const long long x = i % nlx;
-  const long long by = i / nlx;
-  const long long y = by % ny;
-  const long long b = by / ny;
+  const long long x = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= nlx) return;
+  const long long y = blockIdx.y;
+  const long long b = blockIdx.z;
+  const long long i = (b * ny + y) * nlx + x;
   T re = 0, im = 0;
   for (int j = 0; j < nscale; ++j) {
     const long long *dj = desc + (long long)j * kDescLen;
@@ -338,21 +342,22 @@ int owmi_fmul_generic(int dtype, const void *mat, const void *in, void *out, voi
 
 int owmi_fwav_gather(int dtype, const void *full, long long nlead, long long ny, long long nlx,
                      int nscale, const long long *desc, const int *rows, const void *kern,
-                     long long max_count, void *small, void *stream) {
+                     long long nblocks, void *small, void *stream) {
   return guard([&] {
     check_dtype(dtype);
-    OWMI_REQUIRE(nscale >= 0 && nscale <= 65535 && nlead >= 0 && nlead <= 65535,
-                 "fwav_gather: nscale and nlead must lie in [0, 65535]");
-    if (nscale == 0 || nlead == 0 || max_count == 0) return;
-    dim3 grid(ceil_div(max_count, kThreads), (unsigned)nscale, (unsigned)nlead);
+    OWMI_REQUIRE(nscale >= 0 && nlead >= 0 && nlead <= 65535,
+                 "fwav_gather: nlead must lie in [0, 65535]");
+    OWMI_REQUIRE(nblocks >= 0 && nblocks < (1LL << 31), "fwav_gather: bad block count");
+    if (nscale == 0 || nlead == 0 || nblocks == 0) return;
+    dim3 grid((unsigned)nblocks, 1, (unsigned)nlead);
     if (dtype == OWMI_F64)
       hipLaunchKernelGGL(fwav_gather_kernel<double>, grid, dim3(kThreads), 0, as_stream(stream),
-                         (const double2 *)full, ny, nlx, desc, rows, (const double *)kern,
-                         (double2 *)small);
+                         (const double2 *)full, ny, nlx, nscale, desc, rows,
+                         (const double *)kern, (double2 *)small);
     else
       hipLaunchKernelGGL(fwav_gather_kernel<float>, grid, dim3(kThreads), 0, as_stream(stream),
-                         (const float2 *)full, ny, nlx, desc, rows, (const float *)kern,
-                         (float2 *)small);
+                         (const float2 *)full, ny, nlx, nscale, desc, rows,
+                         (const float *)kern, (float2 *)small);
   });
 }
 
@@ -361,17 +366,18 @@ int owmi_fwav_scatter(int dtype, const void *small, long long nlead, long long n
                       void *full, void *stream) {
   return guard([&] {
     check_dtype(dtype);
-    OWMI_REQUIRE(nscale >= 0 && nlead >= 0, "bad sizes");
-    const long long n = nlead * ny * nlx;
-    if (n == 0) return;
+    OWMI_REQUIRE(nscale >= 0 && nlead >= 0 && nlead <= 65535 && ny >= 0 && ny <= 65535,
+                 "fwav_scatter: ny and nlead must lie in [0, 65535]");
+    if (nlead * ny * nlx == 0) return;
+    dim3 grid(ceil_div(nlx, kThreads), (unsigned)ny, (unsigned)nlead);
     if (dtype == OWMI_F64)
-      hipLaunchKernelGGL(fwav_scatter_kernel<double>, dim3(ceil_div(n, kThreads)), dim3(kThreads),
-                         0, as_stream(stream), (const double2 *)small, ny, nlx, nlead, nscale,
-                         desc, invrows, (const double *)kern, (double2 *)full);
+      hipLaunchKernelGGL(fwav_scatter_kernel<double>, grid, dim3(kThreads), 0, as_stream(stream),
+                         (const double2 *)small, ny, nlx, nlead, nscale, desc, invrows,
+                         (const double *)kern, (double2 *)full);
     else
-      hipLaunchKernelGGL(fwav_scatter_kernel<float>, dim3(ceil_div(n, kThreads)), dim3(kThreads),
-                         0, as_stream(stream), (const float2 *)small, ny, nlx, nlead, nscale,
-                         desc, invrows, (const float *)kern, (float2 *)full);
+      hipLaunchKernelGGL(fwav_scatter_kernel<float>, grid, dim3(kThreads), 0, as_stream(stream),
+                         (const float2 *)small, ny, nlx, nlead, nscale, desc, invrows,
+                         (const float *)kern, (float2 *)full);
   });
 }
 

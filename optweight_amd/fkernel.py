@@ -9,9 +9,9 @@ A kernel holds the values k[r, c] of rows slices_y[0] then slices_y[1]
 and columns slice_x of the full (ny, nx // 2 + 1) half plane. For the
 device transforms (wavtrans.f2wav / wav2f) a set packs, per device and
 real dtype, a descriptor table (block offset, ny_j, nx_j, first column,
-row-table offset, kernel offset per scale), the full-plane row of every
-kernel row, the inverse row map (ny per scale, -1 outside the slice) and
-the kernel values.
+row-table offset, kernel offset and first gather block per scale), the
+full-plane row of every kernel row, the inverse row map (ny per scale, -1
+outside the slice) and the kernel values.
 
 File output (write / from_hdf) needs h5py.
 '''
@@ -24,6 +24,7 @@ from scipy.interpolate import interp1d
 from optweight_amd import dft, wlm_utils
 
 DESC_LEN = 8
+GATHER_BLOCK = 256
 
 
 class FKernel:
@@ -176,7 +177,7 @@ class FKernelSet:
         ny, nlx = self.shape_full
         desc, rows, kern = [], [], []
         invrows = np.full((len(self), ny), -1, dtype=np.int32)
-        shapes, off, roff, koff = [], 0, 0, 0
+        shapes, off, roff, koff, nblk = [], 0, 0, 0, 0
         for j, (widx, fk) in enumerate(self):
             k = np.asarray(fk.fkernel, dtype=np.float64)
             ny_j, nx_j = k.shape
@@ -189,7 +190,8 @@ class FKernelSet:
             if np.unique(r).size != r.size:
                 raise ValueError(f'kernel {widx}: repeated rows')
             invrows[j, r] = np.arange(ny_j, dtype=np.int32)
-            desc.append([off, ny_j, nx_j, int(c[0]) if nx_j else 0, roff, koff, 0, 0])
+            desc.append([off, ny_j, nx_j, int(c[0]) if nx_j else 0, roff, koff, nblk, 0])
+            nblk += -(-ny_j * nx_j // GATHER_BLOCK)
             shapes.append((widx, off, ny_j, nx_j))
             rows.append(r.astype(np.int32))
             kern.append(k.ravel())
@@ -208,6 +210,7 @@ class FKernelSet:
             'shapes': shapes,
             'total': off,
             'max_count': max([s[2] * s[3] for s in shapes], default=0),
+            'nblocks': nblk,
         }
         self._plans[key] = plan
         return plan

@@ -217,7 +217,7 @@ def f2wav(fmap, wav, fkernelset):
     small = torch.empty(max(nlead * plan['total'], 1), dtype=fmap.dtype, device=fmap.device)
     _lib.call('owmi_fwav_gather', _lib.dtype_code(fmap), _lib.ptr(fmap), nlead, ny, nlx,
               len(plan['shapes']), _lib.ptr(plan['desc']), _lib.ptr(plan['rows']),
-              _lib.ptr(plan['kern']), plan['max_count'], _lib.ptr(small),
+              _lib.ptr(plan['kern']), plan['nblocks'], _lib.ptr(small),
               _lib.stream_ptr(fmap.device))
     for widx, off, ny_j, nx_j in plan['shapes']:
         blk = small[nlead * off:nlead * (off + ny_j * nx_j)].view(lead + (ny_j, nx_j))

@@ -291,3 +291,30 @@ def test_pinv_fwav(dev, fsetup):
     vt = torch.as_tensor(v, device=dev)
     assert rel(prec(vt), oprec(v)) < 1e-10
     assert rel(vt, v) == 0
+
+
+def test_fwav_float32_full_x_and_batch(dev, fsetup):
+    '''float32 operators (2e-5 of the float64 oracle), a full 3x3 Fourier
+    matrix X^-1/2, and a batch of maps equal to maps one at a time.'''
+    shape, fks, kern = fsetup
+    print('FWav float32 / full X / batch', flush=True)
+    ny, nx = shape
+    cov, ref = cov_wav_for(fks, 3, True, 14)
+    rng = np.random.default_rng(15)
+    a = rng.normal(size=(3, 3, ny, nx // 2 + 1)) * 0.1
+    fmat = np.einsum('abyx, cbyx -> acyx', a, a) + np.eye(3)[:, :, None, None]
+    op = operators.FInvFWavFMatVecMap(map_utils.get_minfo('CC', ny, nx), cov, fks, fmat,
+                                      power_x=-0.5, nsteps=3)
+    oop = of.finv_fwav_map_op(ny, nx, kern, ref, fmat, power_x=-0.5, nsteps=3)
+    m = rng.normal(size=(2, 3, ny * nx))
+    mt = torch.as_tensor(m, device=dev)
+    got = op(mt)
+    for i in range(2):
+        assert rel(got[i], oop(m[i])) < 1e-10, i
+        assert rel(got[i], op(mt[i]).cpu().numpy()) < 1e-13, i
+    got32 = op(mt[0].to(torch.float32))
+    assert got32.dtype == torch.float32
+    assert rel(got32, oop(m[0])) < 2e-5
+    F = of.rfft(m[0].reshape(3, ny, nx))
+    got32f = operators.FWavMatVecF(cov, fks)(torch.as_tensor(F, device=dev).to(torch.complex64))
+    assert rel(got32f, of.fwav_op(kern, F.shape[-2:], ref)(F)) < 2e-6
