@@ -147,12 +147,13 @@ def legendre_roofline(prob, dev, batch, reps=10):
     return {'kernel': kernel, 'ms': ms, 'flops': flops, 'achieved': achieved}
 
 
-# HBM bytes per launch of the roofline kernel (adj_mfma_kernel<2, 8>, 64
+# HBM bytes per launch of the roofline kernel (adj_mfma_kernel<2, 8>, 128
 # right-hand sides): FETCH_SIZE (x 2.0) + WRITE_SIZE (x 1.0), factors from a
 # 256 MiB copy in the same process, measured with `rocprofv3 --pmc` on
 # `python bench.py --roofline-only`
-# (profiles/adj_mfma2_f32_tqu1023_b64_roofline.md): 1.348 GB + 0.544 GB.
-ROOFLINE_TRAFFIC_BYTES = 1.892e9
+# (profiles/adj_mfma2_f32_tqu1023_b128_roofline.md): 2.696 GB + 1.088 GB.
+ROOFLINE_TRAFFIC_BYTES = 3.784e9
+ROOFLINE_TRAFFIC_SHAPE = (1023, 128)
 
 
 def cpu_baseline_sample(dev, lmax_sample=255, reps=3):
@@ -625,7 +626,7 @@ def main():
     ap.add_argument('--lmax', type=int, default=1023)
     ap.add_argument('--no-mg', action='store_true')
     ap.add_argument('--cpu-baseline', choices=['on', 'off'], default='on')
-    ap.add_argument('--batch', type=int, default=64,
+    ap.add_argument('--batch', type=int, default=128,
                     help='right-hand sides per mixed-precision batched solve')
     ap.add_argument('--delta', type=float, default=0.001,
                     help='residual reduction between float64 residual replacements')
@@ -790,7 +791,8 @@ def main():
                      'frac': roof['achieved'] / PEAK_FP32_TFLOPS,
                      # counters were read at the default shape only
                      'traffic': (ROOFLINE_TRAFFIC_BYTES
-                                 if (args.lmax, args.batch) == (1023, 64) else None),
+                                 if (args.lmax, args.batch) == ROOFLINE_TRAFFIC_SHAPE
+                                 else None),
                      'kernel': roof['kernel'], 'kernel_ms': roof['ms'],
                      'flops_per_launch': roof['flops']},
         'cpu_baseline': cpu,
