@@ -116,11 +116,13 @@ def test_fmul_2d(dev, kind):
         assert rel(got, np.stack([of.fmul_2d(f, mat), 2 * of.fmul_2d(f, mat)])) < 1e-14
 
 
-@pytest.fixture(scope='module', params=['digital', 'smooth'])
+@pytest.fixture(scope='module', params=['digital', 'smooth', 'digital_even_nx'])
 def fsetup(request):
-    shape, wcs = car_axes(20, nx=41)
+    # even nx: the inner product's parity test reads exact zeros that the
+    # real-field fix-up of the forward transform provides
+    shape, wcs = car_axes(20, nx=40 if request.param.endswith('even_nx') else 41)
     ly, lx = dft.laxes_real(shape, wcs)
-    fks = fkernel.get_sd_kernels_fourier(ly, lx, 2, digital=request.param == 'digital')
+    fks = fkernel.get_sd_kernels_fourier(ly, lx, 2, digital=request.param.startswith('digital'))
     return shape, fks, sd_kernel_list(fks)
 
 
