@@ -97,6 +97,37 @@ def lmul(alm, lmat, ainfo, alm_out=None, inplace=False):
     return out.reshape(shape_in)
 
 
+def lmul_add(a, lmat_a, b, lmat_b, ainfo, out):
+    '''out = lmat_a a + lmat_b b (lmat_a None: a itself) in one pass, full
+    (npol, npol, nell) device matrices of the alm precision; a or b may be
+    out.'''
+    _lib.require_cuda(a, b, out)
+    if not (a.shape == b.shape == out.shape) or not (a.dtype == b.dtype == out.dtype):
+        raise ValueError('a, b and out must have one shape and dtype')
+    lmax = _lmax_of(ainfo)
+    nelem = _nelem(lmax)
+    if a.shape[-1] != nelem:
+        raise ValueError(f'nelem ({nelem}) != alm.shape[-1] ({a.shape[-1]})')
+    npol = a.shape[-2] if a.ndim >= 2 else 1
+    rdtype = torch.float64 if a.dtype == torch.complex128 else torch.float32
+    mats = []
+    for L in (lmat_a, lmat_b):
+        if L is None:
+            mats.append(None)
+            continue
+        L = _lmat_device(L, npol, rdtype, a.device)
+        if tuple(L.shape) != (npol, npol, lmax + 1):
+            raise ValueError(f'lmat shape {tuple(L.shape)} != {(npol, npol, lmax + 1)}')
+        mats.append(L)
+    if mats[1] is None:
+        raise ValueError('lmat_b is required')
+    ntrans = a.numel() // (npol * nelem)
+    _lib.call('owmi_lmul_add', _lib.dtype_code(a),
+              None if mats[0] is None else _lib.ptr(mats[0]), _lib.ptr(a), _lib.ptr(mats[1]),
+              _lib.ptr(b), _lib.ptr(out), lmax, npol, ntrans, _lib.stream_ptr(a.device))
+    return out
+
+
 def trunc_alm(alm, alm_out, lmax_in, lmax_out):
     '''Copy l, m <= lmax_out into alm_out (truncation to a smaller lmax).'''
     _lib.require_cuda(alm, alm_out)

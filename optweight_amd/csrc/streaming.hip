@@ -138,6 +138,84 @@ lmul_diag_kernel(const T *__restrict__ lmat, const cplx_t<T> *in, cplx_t<T> *out
   }
 }
 
+// out = La a + Lb b per (l, m), full (NC, NC, lmax + 1) matrices (La absent:
+// the identity). a and b may alias out: every thread reads all components
+// of both inputs at its element before writing.
+template <typename T, int NC, bool HASA>
+__global__ void __launch_bounds__(256)
+lmul_add_kernel(const T *__restrict__ la, const cplx_t<T> *a, const T *__restrict__ lb,
+                const cplx_t<T> *b, cplx_t<T> *out, int lmax, long long nelem,
+                long long ntrans) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long t = blockIdx.y;
+  if (i >= nelem || t >= ntrans) return;
+  int l, m;
+  lm_of(i, lmax, l, m);
+  const int nell = lmax + 1;
+  const long long base = t * NC * nelem + i;
+  cplx_t<T> va[NC], vb[NC];
+#pragma unroll
+  for (int j = 0; j < NC; ++j) {
+    va[j] = a[base + j * nelem];
+    vb[j] = b[base + j * nelem];
+  }
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    T ra = 0, ia = 0;
+    if (HASA) {
+#pragma unroll
+      for (int j = 0; j < NC; ++j) {
+        const T w = la[(c * NC + j) * nell + l];
+        ra += w * va[j].x;
+        ia += w * va[j].y;
+      }
+    } else {
+      ra = va[c].x;
+      ia = va[c].y;
+    }
+    T rb = 0, ib = 0;
+#pragma unroll
+    for (int j = 0; j < NC; ++j) {
+      const T w = lb[(c * NC + j) * nell + l];
+      rb += w * vb[j].x;
+      ib += w * vb[j].y;
+    }
+    out[base + c * nelem] = {ra + rb, ia + ib};
+  }
+}
+
+template <typename T, int NC>
+static void launch_lmul_add_nc(const T *la, const cplx_t<T> *a, const T *lb, const cplx_t<T> *b,
+                               cplx_t<T> *out, int lmax, long long nelem, long long ntrans,
+                               hipStream_t st) {
+  dim3 grid(ceil_div(nelem, 256), (unsigned)ntrans);
+  if (la)
+    hipLaunchKernelGGL((lmul_add_kernel<T, NC, true>), grid, dim3(256), 0, st, la, a, lb, b, out,
+                       lmax, nelem, ntrans);
+  else
+    hipLaunchKernelGGL((lmul_add_kernel<T, NC, false>), grid, dim3(256), 0, st, la, a, lb, b,
+                       out, lmax, nelem, ntrans);
+}
+
+template <typename T>
+static void lmul_add_impl(const void *lmat_a, const void *a, const void *lmat_b, const void *b,
+                          void *out, int lmax, int ncomp, long long ntrans, hipStream_t st) {
+  const long long nelem = nelem_of(lmax);
+  OWMI_REQUIRE(ntrans < 65536, "too many transforms for one lmul_add launch");
+  const T *La = static_cast<const T *>(lmat_a);
+  const T *Lb = static_cast<const T *>(lmat_b);
+  auto *A = static_cast<const cplx_t<T> *>(a);
+  auto *B = static_cast<const cplx_t<T> *>(b);
+  auto *O = static_cast<cplx_t<T> *>(out);
+  switch (ncomp) {
+    case 1: launch_lmul_add_nc<T, 1>(La, A, Lb, B, O, lmax, nelem, ntrans, st); break;
+    case 2: launch_lmul_add_nc<T, 2>(La, A, Lb, B, O, lmax, nelem, ntrans, st); break;
+    case 3: launch_lmul_add_nc<T, 3>(La, A, Lb, B, O, lmax, nelem, ntrans, st); break;
+    case 4: launch_lmul_add_nc<T, 4>(La, A, Lb, B, O, lmax, nelem, ntrans, st); break;
+    default: throw Error("lmul_add: ncomp must lie in [1, 4]");
+  }
+}
+
 template <typename T>
 static void lmul_impl(const void *lmat, int diag, const void *in, void *out, int lmax,
                       int ncomp, long long ntrans, hipStream_t st) {
@@ -488,6 +566,22 @@ int owmi_lmul(int dtype, const void *lmat, int diag, const void *in, void *out, 
       lmul_impl<double>(lmat, diag, in, out, lmax, ncomp, ntrans, as_stream(stream));
     else
       lmul_impl<float>(lmat, diag, in, out, lmax, ncomp, ntrans, as_stream(stream));
+  });
+}
+
+int owmi_lmul_add(int dtype, const void *lmat_a, const void *a, const void *lmat_b,
+                  const void *b, void *out, int lmax, int ncomp, long long ntrans,
+                  void *stream) {
+  return guard([&] {
+    check_dtype(dtype);
+    OWMI_REQUIRE(lmax >= 0 && ncomp >= 1 && ntrans >= 0, "invalid lmul_add sizes");
+    OWMI_REQUIRE(lmat_b != nullptr && a != nullptr && b != nullptr && out != nullptr,
+                 "lmul_add: lmat_b, a, b and out are required");
+    if (ntrans == 0) return;
+    if (dtype == OWMI_F64)
+      lmul_add_impl<double>(lmat_a, a, lmat_b, b, out, lmax, ncomp, ntrans, as_stream(stream));
+    else
+      lmul_add_impl<float>(lmat_a, a, lmat_b, b, out, lmax, ncomp, ntrans, as_stream(stream));
   });
 }
 

@@ -221,3 +221,30 @@ def test_cg_updates(dev):
     _lib.call('owmi_cg_p', _lib.F64, _lib.ptr(P), _lib.ptr(Z), _lib.ptr(num), _lib.ptr(den), n,
               _lib.stream_ptr(dev))
     np.testing.assert_allclose(P.cpu().numpy(), z - 2 * p, rtol=1e-15, atol=1e-15)
+
+
+@pytest.mark.parametrize('dtype', ['float64', 'float32'])
+@pytest.mark.parametrize('npol', [1, 3, 4])
+@pytest.mark.parametrize('case', ['identity_a', 'both', 'alias_a', 'alias_b'])
+def test_lmul_add(dev, dtype, npol, case):
+    '''out = La a + Lb b against the oracle's lmul, with out aliasing a or b.'''
+    print(f'lmul_add {case} npol={npol} {dtype}', flush=True)
+    from oracle import ops as oops
+    from optweight_amd import alm_c_utils
+    lmax = 37
+    rng = np.random.default_rng(npol)
+    n = layout.nelem(lmax)
+    a = rng.normal(size=(2, npol, n)) + 1j * rng.normal(size=(2, npol, n))
+    b = rng.normal(size=(2, npol, n)) + 1j * rng.normal(size=(2, npol, n))
+    la = rng.normal(size=(npol, npol, lmax + 1))
+    lb = rng.normal(size=(npol, npol, lmax + 1))
+    cdt = torch.complex128 if dtype == 'float64' else torch.complex64
+    at, bt = torch.as_tensor(a, device=dev).to(cdt), torch.as_tensor(b, device=dev).to(cdt)
+    use_a = None if case == 'identity_a' else la
+    ref = np.stack([(a[t] if use_a is None else oops.lmul(a[t], la, lmax)) + oops.lmul(b[t], lb, lmax)
+                    for t in range(2)])
+    out = {'alias_a': at, 'alias_b': bt}.get(case, torch.empty_like(at))
+    got = alm_c_utils.lmul_add(at, use_a, bt, lb, alm_utils.AlmInfo(lmax), out=out)
+    assert got.data_ptr() == out.data_ptr()
+    tol = 1e-14 if dtype == 'float64' else 2e-6
+    assert np.linalg.norm(got.cpu().numpy() - ref) / np.linalg.norm(ref) < tol
