@@ -61,9 +61,8 @@ int owmi_lmul(int dtype, const void *lmat, int diag, const void *in,
  * out[t, i, lm] = sum_j La[i, j, l] a[t, j, lm] + sum_j Lb[i, j, l] b[t, j, lm]
  * with full (ncomp, ncomp, lmax+1) matrices of the alm precision, ncomp <= 4;
  * lmat_a NULL takes La as the identity. a and b may alias out. The sum of
- * two EllMatVecAlm products in one pass: the signal term added to the noise
- * term of CGWienerMap.a_matrix (solvers.py:217-242, beam applied to the
- * noise term) and of PseudoInvPreconditioner (preconditioners.py:126-249).
+ * two EllMatVecAlm products in one pass: the signal term S^-1 p added to
+ * the beam-weighted noise term of CGWienerMap.a_matrix (solvers.py:217-242).
  */
 int owmi_lmul_add(int dtype, const void *lmat_a, const void *a,
                   const void *lmat_b, const void *b, void *out, int lmax,
