@@ -62,7 +62,9 @@ int owmi_lmul(int dtype, const void *lmat, int diag, const void *in,
  * with full (ncomp, ncomp, lmax+1) matrices of the alm precision, ncomp <= 4;
  * lmat_a NULL takes La as the identity. a and b may alias out. The sum of
  * two EllMatVecAlm products in one pass: the signal term S^-1 p added to
- * the beam-weighted noise term of CGWienerMap.a_matrix (solvers.py:217-242).
+ * the beam-weighted noise term of CGWienerMap.a_matrix (solvers.py:217-242),
+ * and H (tau B)^T n + H S^-1 a, the last stages of PseudoInvPreconditioner
+ * (preconditioners.py:126-249).
  */
 int owmi_lmul_add(int dtype, const void *lmat_a, const void *a,
                   const void *lmat_b, const void *b, void *out, int lmax,

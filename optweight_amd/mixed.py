@@ -38,6 +38,7 @@ replacements between replays stay eager. On the TQU lmax 1023 batch of 16
 the launches already overlap the kernels and the recording costs more than
 it saves (49.4 against 50.7 solves/s), hence off by default.
 '''
+import gc
 import os
 import warnings
 
@@ -115,6 +116,12 @@ class _Phases:
             return
         if self.graphs[k] is None:
             g = torch.cuda.CUDAGraph()
+            # no garbage collection inside the recording: a finalizer that
+            # frees device memory (an SHT plan of an unreachable geometry)
+            # would invalidate it
+            gc.collect()
+            gc_was_enabled = gc.isenabled()
+            gc.disable()
             try:
                 with torch.cuda.graph(g):
                     self.fns[k]()
@@ -126,6 +133,9 @@ class _Phases:
                 self.use_graph = False
                 self.fns[k]()
                 return
+            finally:
+                if gc_was_enabled:
+                    gc.enable()
             self.graphs[k] = g
         self.graphs[k].replay()
 

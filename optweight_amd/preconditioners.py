@@ -99,15 +99,23 @@ class PseudoInvPreconditioner(operators.MatVecAlm):
             self.ivar_noise_iso_adj = self.ivar_noise_iso
         self.pcov_noise = operators.PixMatVecAlm(ainfo, cov_pix, minfo, spin, inplace=True,
                                                  adjoint=True)
+        # H (tau B)^T and H S^-1 per multipole (float64, cast to the alm
+        # precision on use): the last three stages of call in one pass
+        h = self.harmonic_prec.preconditioner.m_ell
+        self.ainfo = ainfo
+        self._h_ivar_adj = operators._DeviceCache(
+            np.einsum('ijl, jkl -> ikl', h, self.ivar_noise_iso_adj.m_ell))
+        self._h_icov = operators._DeviceCache(
+            np.einsum('ijl, jkl -> ikl', h, self.icov_signal.m_ell))
 
     def call(self, alm):
         alm = self.harmonic_prec(alm)
-        alm_signal = self.icov_signal.apply_copy(alm)
-        alm_noise = self.ivar_noise_iso(alm)
+        alm_noise = self.ivar_noise_iso.apply_copy(alm)
         alm_noise = self.pcov_noise(alm_noise)
-        alm_noise = self.ivar_noise_iso_adj(alm_noise)
-        alm_noise += alm_signal
-        return self.harmonic_prec(alm_noise)
+        # H [(tau B)^T noise + S^-1 alm], written over alm (a new array here)
+        rdt = operators._real_dtype(alm.dtype)
+        return alm_c_utils.lmul_add(alm_noise, self._h_ivar_adj.get(alm.device, rdt), alm,
+                                    self._h_icov.get(alm.device, rdt), self.ainfo, out=alm)
 
 
 class PseudoInvPreconditionerWav(operators.MatVecAlm):

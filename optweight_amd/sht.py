@@ -75,10 +75,28 @@ class Plan:
     def __del__(self):
         try:
             if self.handle:
-                self._lib.owmi_sht_plan_destroy(self.handle)
+                if torch.cuda.is_current_stream_capturing():
+                    # freeing device memory would invalidate a HIP-graph
+                    # recording in progress: destroy after it
+                    _DEFERRED.append((self._lib, self.handle))
+                else:
+                    self._lib.owmi_sht_plan_destroy(self.handle)
                 self.handle = None
         except Exception:
             pass
+
+
+# plans released during a HIP-graph recording, destroyed by the next
+# get_plan outside one
+_DEFERRED = []
+
+
+def release_deferred():
+    '''Destroy plans whose release was deferred by a graph recording.'''
+    if _DEFERRED and not torch.cuda.is_current_stream_capturing():
+        while _DEFERRED:
+            lib, handle = _DEFERRED.pop()
+            lib.owmi_sht_plan_destroy(handle)
 
 
 def get_plan(minfo, lmax, device, rings=None):
@@ -87,6 +105,7 @@ def get_plan(minfo, lmax, device, rings=None):
     device = torch.device(device)
     if device.index is None:
         device = torch.device('cuda', torch.cuda.current_device())
+    release_deferred()
     cache = minfo.__dict__.setdefault('_plans', {})
     if rings is not None:
         rings = np.asarray(rings, dtype=np.int64)

@@ -216,3 +216,30 @@ def test_mixed_graph_replay_matches_eager(dev, prob, nb):
     xg2, ig2 = mixed.solve_mixed(solver.a_matrix, solver.b0, solver.preconditioner, tol=1e-6,
                                  delta=1e-2, graphs=True)
     assert ig2.iterations == ie.iterations and rel(xg2, xe) < 1e-12
+
+
+def test_plan_released_during_graph_recording(dev):
+    '''An SHT plan whose last reference goes away inside a HIP-graph
+    recording is destroyed after it (freeing device memory during the
+    recording would invalidate it); the recorded graph replays correctly.'''
+    print('plan release during graph recording', flush=True)
+    from optweight_amd import sht, map_utils, alm_utils
+    minfo = map_utils.MapInfo.map_info_gauss_legendre(9, 17)
+    ainfo = alm_utils.AlmInfo(8)
+    plan = sht.get_plan(minfo, 8, dev)
+    del minfo.__dict__['_plans']
+    x = torch.ones(1024, device=dev)
+    y = torch.empty_like(x)
+    y.copy_(x * 2)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        del plan
+        y.copy_(x * 2)
+    assert len(sht._DEFERRED) == 1
+    g.replay()
+    torch.cuda.synchronize()
+    assert torch.equal(y, x * 2)
+    sht.get_plan(map_utils.MapInfo.map_info_gauss_legendre(5, 9), 4, dev)
+    assert len(sht._DEFERRED) == 0
+    del ainfo
