@@ -58,6 +58,18 @@ int owmi_lmul(int dtype, const void *lmat, int diag, const void *in,
               void *stream);
 
 /*
+ * out[t, c, p] = x[t, c, p] + dcomp[c] mask[c, p] (a[t, c, p] - b[t, c, p])
+ * on maps (ntrans, ncomp, npix) of dtype's real type; x, b, dcomp (ncomp) and
+ * mask (ncomp, npix) may be NULL (0, 0, 1, 1); any array may alias out. The
+ * array operations of a multigrid V-cycle in one pass (multigrid.py:
+ * 400-447): the Jacobi update x += D (r - G x), the masked residual
+ * M (r - G x) that is restricted, and x += M (interpolated correction).
+ */
+int owmi_map_fused(int dtype, void *out, const void *x, const void *a,
+                   const void *b, const void *dcomp, const void *mask, int ncomp,
+                   long long npix, long long ntrans, void *stream);
+
+/*
  * out[t, i, lm] = sum_j La[i, j, l] a[t, j, lm] + sum_j Lb[i, j, l] b[t, j, lm]
  * with full (ncomp, ncomp, lmax+1) matrices of the alm precision, ncomp <= 4;
  * lmat_a NULL takes La as the identity. a and b may alias out. The sum of

@@ -184,6 +184,28 @@ lmul_add_kernel(const T *__restrict__ la, const cplx_t<T> *a, const T *__restric
   }
 }
 
+// out[t, c, p] = x[t, c, p] + dcomp[c] * mask[c, p] * (a[t, c, p] - b[t, c, p]),
+// each of x, b, dcomp and mask optional (absent: 0, 0, 1, 1); rows (t, c) of
+// npix pixels. The difference is rounded first, then each factor, then the
+// sum (the order of the separate array operations it replaces). Any array
+// may alias out: every thread reads its element of each before writing.
+template <typename T>
+__global__ void __launch_bounds__(256)
+map_fused_kernel(T *out, const T *x, const T *a, const T *b, const T *__restrict__ dcomp,
+                 const T *__restrict__ mask, int ncomp, long long npix) {
+#pragma clang fp contract(off)
+  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npix) return;
+  const long long row = blockIdx.y;
+  const int c = (int)(row % ncomp);
+  const long long i = row * npix + p;
+  T v = a[i];
+  if (b) v = v - b[i];
+  if (mask) v = v * mask[(long long)c * npix + p];
+  if (dcomp) v = v * dcomp[c];
+  out[i] = x ? x[i] + v : v;
+}
+
 template <typename T, int NC>
 static void launch_lmul_add_nc(const T *la, const cplx_t<T> *a, const T *lb, const cplx_t<T> *b,
                                cplx_t<T> *out, int lmax, long long nelem, long long ntrans,
@@ -566,6 +588,29 @@ int owmi_lmul(int dtype, const void *lmat, int diag, const void *in, void *out, 
       lmul_impl<double>(lmat, diag, in, out, lmax, ncomp, ntrans, as_stream(stream));
     else
       lmul_impl<float>(lmat, diag, in, out, lmax, ncomp, ntrans, as_stream(stream));
+  });
+}
+
+int owmi_map_fused(int dtype, void *out, const void *x, const void *a, const void *b,
+                   const void *dcomp, const void *mask, int ncomp, long long npix,
+                   long long ntrans, void *stream) {
+  return guard([&] {
+    check_dtype(dtype);
+    OWMI_REQUIRE(out != nullptr && a != nullptr, "map_fused: out and a are required");
+    OWMI_REQUIRE(ncomp >= 1 && npix >= 0 && ntrans >= 0, "invalid map_fused sizes");
+    const long long rows = (long long)ncomp * ntrans;
+    OWMI_REQUIRE(rows < 65536, "map_fused: too many rows for one launch");
+    if (rows == 0 || npix == 0) return;
+    dim3 grid(ceil_div(npix, 256), (unsigned)rows);
+    if (dtype == OWMI_F64)
+      hipLaunchKernelGGL(map_fused_kernel<double>, grid, dim3(256), 0, as_stream(stream),
+                         (double *)out, (const double *)x, (const double *)a,
+                         (const double *)b, (const double *)dcomp, (const double *)mask, ncomp,
+                         npix);
+    else
+      hipLaunchKernelGGL(map_fused_kernel<float>, grid, dim3(256), 0, as_stream(stream),
+                         (float *)out, (const float *)x, (const float *)a, (const float *)b,
+                         (const float *)dcomp, (const float *)mask, ncomp, npix);
   });
 }
 

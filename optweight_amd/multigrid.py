@@ -16,7 +16,7 @@ import numpy as np
 import torch
 from scipy import linalg
 
-from optweight_amd import map_utils, operators, noise_utils, alm_c_utils, sht, alm_utils
+from optweight_amd import map_utils, operators, noise_utils, alm_c_utils, sht, alm_utils, map_c_utils
 
 
 class Level:
@@ -48,6 +48,20 @@ class Level:
         else:
             self.smoother = self._init_sparse_smoother()
 
+    def jacobi(self, x, imap):
+        '''x += D (imap - G x) in place (the smoother D of this level).'''
+        gx = self.g_op(x)
+        if self.pinv_g is not None:
+            x += self.smoother(imap - gx)
+            return x
+        d = self._m_prec.get(x.device, x.dtype)
+        return map_c_utils.map_fused(imap, b=gx, x=x, dcomp=d, out=x)
+
+    def masked_residual(self, imap, x):
+        '''M_u (imap - G x) as a new map.'''
+        gx = self.g_op(x)
+        return map_c_utils.map_fused(imap, b=gx, mask=self.mask_dev(x.device, x.dtype), out=gx)
+
     @property
     def lmax(self):
         return map_utils.minfo2lmax(self.minfo)
@@ -71,6 +85,7 @@ class Level:
         m_prec *= 0.2
         self.m_prec = m_prec
         cache = operators._DeviceCache(m_prec[:, np.newaxis])
+        self._m_prec = operators._DeviceCache(m_prec)
 
         def smoother(x):
             return cache.get(x.device, x.dtype) * x
@@ -189,14 +204,17 @@ def get_equal_area_mask_bool(mask_bool, minfo, lmax=None, minfo_fullsky=None):
     return mask_out.astype(bool), minfo_out
 
 
-def restrict(imap, level_in, level_out, spin, adjoint=False):
+def restrict(imap, level_in, level_out, spin, adjoint=False, premasked=False, acc=None):
     '''Y R Y^T W between levels (W Y R Y^T if adjoint), R a Gaussian beam
-    at the coarser band limit, masks on both sides.'''
+    at the coarser band limit, masks on both sides. premasked: imap already
+    carries level_in's mask. acc: add the result to acc (in place) and
+    return acc.'''
     lmax_in = level_in.lmax
     lmax_out = level_out.lmax
     rdt = imap.dtype
     cdt = torch.complex128 if rdt == torch.float64 else torch.complex64
-    imap = imap * level_in.mask_dev(imap.device, rdt)
+    if not premasked:
+        imap = imap * level_in.mask_dev(imap.device, rdt)
     lmax_t = lmax_out if lmax_out < lmax_in else lmax_in
     ainfo = alm_utils.AlmInfo(lmax_t)
     # the analysis overwrites every coefficient
@@ -211,6 +229,9 @@ def restrict(imap, level_in, level_out, spin, adjoint=False):
                        device=imap.device)
     sht.alm2map(alm, omap, ainfo, level_out.minfo, spin, adjoint=adjoint,
                 rings=level_out.rings)
+    if acc is not None:
+        return map_c_utils.map_fused(omap, x=acc, mask=level_out.mask_dev(imap.device, rdt),
+                                     out=acc)
     omap *= level_out.mask_dev(imap.device, rdt)
     return omap
 
@@ -229,18 +250,19 @@ def v_cycle(levels, imap, spin, idx=0, n_jacobi=1, x0=None):
         return levels[idx].smoother(imap)
     lev = levels[idx]
     if x0 is not None:
-        x = x0.clone()
-        x += lev.smoother(imap - lev.g_op(x))
+        x = lev.jacobi(x0.clone(), imap)
     else:
         x = lev.smoother(imap)
     for _ in range(n_jacobi - 1):
-        x += lev.smoother(imap - lev.g_op(x))
-    r_h = imap - lev.g_op(x)
-    r_H = coarsen(r_h, lev, levels[idx + 1], spin)
+        lev.jacobi(x, imap)
+    # the residual carries this level's mask, which the restriction would
+    # apply; the interpolated correction is masked and added in one pass
+    r_H = restrict(lev.masked_residual(imap, x), lev, levels[idx + 1], spin, adjoint=False,
+                   premasked=True)
     c_H = v_cycle(levels, r_H, spin, idx=idx + 1, n_jacobi=n_jacobi)
-    x += interpolate(c_H, levels[idx + 1], lev, spin)
+    restrict(c_H, levels[idx + 1], lev, spin, adjoint=True, acc=x)
     for _ in range(n_jacobi):
-        x += lev.smoother(imap - lev.g_op(x))
+        lev.jacobi(x, imap)
     return x
 
 

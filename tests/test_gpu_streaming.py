@@ -248,3 +248,31 @@ def test_lmul_add(dev, dtype, npol, case):
     assert got.data_ptr() == out.data_ptr()
     tol = 1e-14 if dtype == 'float64' else 2e-6
     assert np.linalg.norm(got.cpu().numpy() - ref) / np.linalg.norm(ref) < tol
+
+
+@pytest.mark.parametrize('dtype', ['float64', 'float32'])
+@pytest.mark.parametrize('case', ['jacobi', 'masked_residual', 'masked_add', 'scale'])
+def test_map_fused(dev, dtype, case):
+    '''out = x + d_c mask (a - b) against the same sequence of torch operations
+    (bit-exact in float64), in place on x or b.'''
+    print(f'map_fused {case} {dtype}', flush=True)
+    rng = np.random.default_rng(7)
+    rdt = torch.float64 if dtype == 'float64' else torch.float32
+    t = lambda a: torch.as_tensor(a, device=dev).to(rdt)
+    a, b, x = (t(rng.normal(size=(4, 2, 1000))) for _ in range(3))
+    d = t(rng.uniform(0.1, 1, size=2))
+    mask = t(rng.uniform(size=(2, 1000)) > 0.3)
+    if case == 'jacobi':
+        ref = x + d[:, None] * (a - b)
+        got = map_c_utils.map_fused(a, b=b, x=x, dcomp=d, out=x)
+    elif case == 'masked_residual':
+        ref = (a - b) * mask
+        got = map_c_utils.map_fused(a, b=b, mask=mask, out=b)
+    elif case == 'masked_add':
+        ref = x + a * mask
+        got = map_c_utils.map_fused(a, x=x, mask=mask, out=x)
+    else:
+        ref = d[:, None] * a
+        got = map_c_utils.map_fused(a, dcomp=d)
+    tol = 0 if dtype == 'float64' else 1e-6
+    assert (got - ref).abs().max().item() <= tol * ref.abs().max().item()
