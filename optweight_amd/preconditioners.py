@@ -110,6 +110,13 @@ class PseudoInvPreconditioner(operators.MatVecAlm):
 
     def call(self, alm):
         alm = self.harmonic_prec(alm)
+        if alm.shape[-2] > 4:
+            # more components than the fused pass takes: stage by stage
+            alm_signal = self.icov_signal.apply_copy(alm)
+            alm_noise = self.pcov_noise(self.ivar_noise_iso.apply_copy(alm))
+            alm_noise = self.ivar_noise_iso_adj(alm_noise)
+            alm_noise += alm_signal
+            return self.harmonic_prec(alm_noise)
         alm_noise = self.ivar_noise_iso.apply_copy(alm)
         alm_noise = self.pcov_noise(alm_noise)
         # H [(tau B)^T noise + S^-1 alm], written over alm (a new array here)

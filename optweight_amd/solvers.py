@@ -268,6 +268,11 @@ class CGWienerMap(CG):
             t = alm_c_utils.lmul(alm, beam, ainfo) if beam is not None else alm
             u = torch.empty_like(alm)
             sht_tools.pixel_sandwich(t, u, ainfo, y.minfo, y.spin, d)
+            if alm.shape[-2] > 4:
+                if beam is not None:
+                    alm_c_utils.lmul(u, beam, ainfo, inplace=True)
+                u += alm_c_utils.lmul(alm, icov, ainfo)
+                return u
             # B u + S^-1 alm in one pass
             return alm_c_utils.lmul_add(u, beam, alm, icov, ainfo, out=u)
         alm = self.scale_filter(alm)
