@@ -418,6 +418,9 @@ def inner_cg(a_op, m_op, b, nsteps, dot=map_dot, verbose=False, batched=None):
     z = m_op(r)
     rz = dot(r, z)
     p = z
+    if b.is_complex() and b.is_cuda and b.is_contiguous():
+        return _inner_cg_complex(a_op, m_op, x, r, p, rz, nsteps, dot, verbose,
+                                 b.shape[0] if batched else 1)
     for idx in range(nsteps):
         Ap = a_op(p)
         alpha = _ratio(rz, dot(p, Ap)).to(b.dtype).reshape(bshape)
@@ -429,6 +432,35 @@ def inner_cg(a_op, m_op, b, nsteps, dot=map_dot, verbose=False, batched=None):
         beta = _ratio(rz_next, rz).to(b.dtype).reshape(bshape)
         rz = rz_next
         p = z + beta * p
+        if verbose:
+            print(idx, rz.cpu().numpy())
+    return x
+
+
+def _inner_cg_complex(a_op, m_op, x, r, p, rz, nsteps, dot, verbose, nb):
+    '''The loop of inner_cg on complex device vectors with the CG update
+    kernels: x += alpha p, r -= alpha A p and p = z + beta p in place, alpha
+    and beta formed on the device (0 for a zero denominator). r is the
+    caller's right-hand side array, updated in place.'''
+    from optweight_amd import _lib
+    code = _lib.dtype_code(x)
+    n = x.numel() // nb
+    st = _lib.stream_ptr(x.device)
+    if p.data_ptr() == r.data_ptr():
+        p = p.clone()
+    for idx in range(nsteps):
+        ap = a_op(p).contiguous()
+        pap = dot(p, ap)
+        _lib.call('owmi_cg_xr_batch', code, _lib.ptr(x), code, _lib.ptr(r), _lib.ptr(p),
+                  _lib.ptr(ap), _lib.ptr(rz), _lib.ptr(pap), n, nb, st)
+        del ap
+        z = m_op(r).contiguous()
+        rz_next = dot(r, z)
+        if z.data_ptr() == p.data_ptr():
+            z = z.clone()
+        _lib.call('owmi_cg_p_batch', code, _lib.ptr(p), _lib.ptr(z), _lib.ptr(rz_next),
+                  _lib.ptr(rz), n, nb, st)
+        rz = rz_next
         if verbose:
             print(idx, rz.cpu().numpy())
     return x
