@@ -252,16 +252,18 @@ def test_lmul_add(dev, dtype, npol, case):
 
 @pytest.mark.parametrize('dtype', ['float64', 'float32'])
 @pytest.mark.parametrize('case', ['jacobi', 'masked_residual', 'masked_add', 'scale'])
-def test_map_fused(dev, dtype, case):
+@pytest.mark.parametrize('npix', [1000, 1001])
+def test_map_fused(dev, dtype, case, npix):
     '''out = x + d_c mask (a - b) against the same sequence of torch operations
-    (bit-exact in float64), in place on x or b.'''
-    print(f'map_fused {case} {dtype}', flush=True)
+    (bit-exact in float64), in place on x or b; 16-byte vectors (npix 1000)
+    and single elements (npix 1001).'''
+    print(f'map_fused {case} {dtype} npix={npix}', flush=True)
     rng = np.random.default_rng(7)
     rdt = torch.float64 if dtype == 'float64' else torch.float32
     t = lambda a: torch.as_tensor(a, device=dev).to(rdt)
-    a, b, x = (t(rng.normal(size=(4, 2, 1000))) for _ in range(3))
+    a, b, x = (t(rng.normal(size=(4, 2, npix))) for _ in range(3))
     d = t(rng.uniform(0.1, 1, size=2))
-    mask = t(rng.uniform(size=(2, 1000)) > 0.3)
+    mask = t(rng.uniform(size=(2, npix)) > 0.3)
     if case == 'jacobi':
         ref = x + d[:, None] * (a - b)
         got = map_c_utils.map_fused(a, b=b, x=x, dcomp=d, out=x)
