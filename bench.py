@@ -155,6 +155,11 @@ def legendre_roofline(prob, dev, batch, reps=10):
 ROOFLINE_TRAFFIC_BYTES = 3.784e9
 ROOFLINE_TRAFFIC_SHAPE = (1023, 128)
 
+# right-hand sides per batch: the headline solve, and the batched timings of
+# the other modes (config 3: 16 per band limit)
+HEADLINE_BATCH = 128
+AUX_BATCH = 16
+
 
 def cpu_baseline_sample(dev, lmax_sample=255, reps=3):
     """The oracle's float64 A application (numpy, long-double Legendre
@@ -626,8 +631,10 @@ def main():
     ap.add_argument('--lmax', type=int, default=1023)
     ap.add_argument('--no-mg', action='store_true')
     ap.add_argument('--cpu-baseline', choices=['on', 'off'], default='on')
-    ap.add_argument('--batch', type=int, default=128,
-                    help='right-hand sides per mixed-precision batched solve')
+    ap.add_argument('--batch', type=int, default=None,
+                    help='right-hand sides per mixed-precision batched solve (default 128 '
+                         'for the headline solve, 16 per band limit for --ragged and for '
+                         'the batched timings of the other modes)')
     ap.add_argument('--delta', type=float, default=0.001,
                     help='residual reduction between float64 residual replacements')
     ap.add_argument('--ragged', action='store_true',
@@ -665,6 +672,10 @@ def main():
                     help='only build the workload geometry and time the roofline kernel '
                          '(for counter runs), after one copy of known size')
     args = ap.parse_args()
+    headline = not (args.ragged or args.wavelet or args.phase1 or args.filter or args.const_cor
+                    or args.noise_est or args.fwav or args.streaming)
+    if args.batch is None:
+        args.batch = HEADLINE_BATCH if headline else AUX_BATCH
     if args.gpus != 1:
         print('bench.py measures one GPU: this build has no multi-GPU path '
               '(independent right-hand sides shard trivially across GPUs).', file=sys.stderr)
