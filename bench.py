@@ -252,6 +252,11 @@ def streaming_bench(args, dev):
                                  _lib.ptr(alm), _lib.ptr(out), lmax, 3, n,
                                  _lib.stream_ptr(dev)),
                2 * alm.numel() * esz)
+        record(f'lmul_add 3x3 + 3x3 ({tag})',
+               lambda: _lib.call('owmi_lmul_add', _lib.dtype_code(alm), _lib.ptr(lmat),
+                                 _lib.ptr(out), _lib.ptr(lmat), _lib.ptr(alm), _lib.ptr(out),
+                                 lmax, 3, n, _lib.stream_ptr(dev)),
+               3 * alm.numel() * esz)
         ldiag = torch.as_tensor(np.abs(rng.normal(size=(3, lmax + 1))), dtype=rdt, device=dev)
         record(f'lmul diagonal ({tag})',
                lambda: _lib.call('owmi_lmul', _lib.dtype_code(alm), _lib.ptr(ldiag), 1,
@@ -261,6 +266,16 @@ def streaming_bench(args, dev):
         imap = torch.randn((n, 3, npix), dtype=rdt, device=dev)
         d = torch.rand((3, npix), dtype=rdt, device=dev)
         omap = torch.empty_like(imap)
+        tmap = torch.randn_like(imap)
+        bmap = torch.randn_like(imap)
+        dcomp = torch.rand((3,), dtype=rdt, device=dev)
+        # x (in place), a, b read, x written: 4 arrays of the map size
+        record(f'map_fused x += d (a - b) ({tag})',
+               lambda: _lib.call('owmi_map_fused', _lib.dtype_code(imap), _lib.ptr(tmap),
+                                 _lib.ptr(tmap), _lib.ptr(imap), _lib.ptr(bmap), _lib.ptr(dcomp),
+                                 None, 3, npix, n, _lib.stream_ptr(dev)),
+               4 * imap.numel() * imap.element_size())
+        del bmap
         record(f'pixel diagonal N^-1 ({tag})',
                lambda: _lib.call('owmi_pix_matvec', _lib.dtype_code(imap), _lib.ptr(d), 0,
                                  _lib.ptr(imap), _lib.ptr(omap), 1, d.numel(), n,
