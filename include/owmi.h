@@ -102,7 +102,11 @@ int owmi_wlm2alm(int dtype, const void *w_ell, const void *wlm, void *alm,
  * Real-field inner product over nrows alm rows at lmax (complex, dtype):
  *   *dev_out = 2 Re sum a conj(b) - Re sum_{m=0} a conj(b)
  * accumulated in double; dev_out is a device double. Deterministic
- * (fixed-order two-pass reduction).
+ * (fixed-order two-pass reduction). The partial sums go to a buffer of the
+ * library private to the (device, stream) pair, allocated on the first call
+ * on that stream (also inside a HIP-graph recording, outside the graph):
+ * dots on different streams may overlap; this and every other reduction
+ * here (owmi_alm_dot_batch, owmi_fourier_dot) follow the same rule.
  * Replaces alm_utils.contract_almxblm (optweight/alm_utils.py:9-44).
  */
 int owmi_alm_dot(int dtype, const void *a, const void *b, int lmax,

@@ -173,9 +173,17 @@ def wiener_filter(d, amps, model):
 def config_loss(d, amps, model):
     '''L = 1/2 d^T (P S P^T + N)^-1 d + 1/2 <x, x>_A with x = x(d, a),
     using (P S P^T + N)^-1 d = N^-1 (d - P x) (Woodbury):
-    d^T (P S P^T + N)^-1 d = d^T N^-1 d - <b, x>_A.'''
+    d^T (P S P^T + N)^-1 d = d^T N^-1 d - <b, x>_A. The inverse noise
+    covariance is per-pixel diagonal (npol, npix) or a full per-pixel
+    matrix (npol, npol, npix).'''
+    ninv = torch.as_tensor(model.icov_pix, device=d.device, dtype=d.dtype)
+    full = ninv.ndim == d.ndim + 1 and tuple(ninv.shape[:2]) == (d.shape[-2], d.shape[-2]) \
+        and ninv.shape[-1] == d.shape[-1]
+    if not full and tuple(ninv.shape) != tuple(d.shape):
+        raise ValueError(f'icov_pix of shape {tuple(ninv.shape)} does not match data '
+                         f'{tuple(d.shape)}: need (npol, npix) or (npol, npol, npix)')
     b = BVec.apply(d, model)
     x = WienerSolve.apply(b, amps, model)
-    ninv = torch.as_tensor(model.icov_pix, device=d.device, dtype=d.dtype)
-    quad = torch.sum(d * ninv * d)
+    # full matrix: sum_p d_i N^-1_ij d_j
+    quad = torch.einsum('ip,ijp,jp->', d, ninv, d) if full else torch.sum(d * ninv * d)
     return 0.5 * (quad - contract(b, x, model.weight)) + 0.5 * contract(x, x, model.weight)

@@ -54,6 +54,14 @@ int guard(F &&f) {
 
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Device buffer of `bytes` bytes private to (current device, stream, tag):
+// the partial sums of a two-pass reduction, so reductions queued on
+// different streams never share one. Allocated on the first call for the
+// key (with the thread's stream-capture mode relaxed, so a first call
+// inside a HIP-graph recording allocates outside the graph) and kept until
+// the process ends; `bytes` must be the same on every call with one tag.
+void *stream_scratch(int tag, size_t bytes, hipStream_t st);
+
 inline void check_dtype(int dtype) {
   OWMI_REQUIRE(dtype == OWMI_F32 || dtype == OWMI_F64, "dtype must be OWMI_F32 or OWMI_F64");
 }

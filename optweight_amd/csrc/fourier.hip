@@ -262,18 +262,11 @@ fdot_final_kernel(const double *partial, int nblk, int nx_parity, double *out) {
   }
 }
 
-double *fdot_scratch() {
-  static std::mutex mu;
-  static std::map<int, double *> bufs;
-  int dev = 0;
-  OWMI_HIP(hipGetDevice(&dev));
-  std::lock_guard<std::mutex> lk(mu);
-  auto it = bufs.find(dev);
-  if (it != bufs.end()) return it->second;
-  double *p = nullptr;
-  OWMI_HIP(hipMalloc(&p, 3 * kDotBlocks * sizeof(double)));
-  bufs[dev] = p;
-  return p;
+// Partial sums of owmi_fourier_dot (3 kDotBlocks doubles), one buffer per
+// device and stream.
+constexpr int kTagFourierDot = 2;
+double *fdot_scratch(hipStream_t st) {
+  return (double *)stream_scratch(kTagFourierDot, 3 * kDotBlocks * sizeof(double), st);
 }
 
 template <typename T>
@@ -402,8 +395,8 @@ int owmi_fourier_dot(int dtype, const void *f, const void *g, long long nlead, l
   return guard([&] {
     check_dtype(dtype);
     OWMI_REQUIRE(nx_parity >= -1 && nx_parity <= 1, "nx_parity must be -1, 0 or 1");
-    double *part = fdot_scratch();
     hipStream_t st = as_stream(stream);
+    double *part = fdot_scratch(st);
     if (dtype == OWMI_F64)
       hipLaunchKernelGGL(fdot_partial_kernel<double>, dim3(kDotBlocks), dim3(kThreads), 0, st,
                          (const double2 *)f, (const double2 *)g, nlead, ny, nlx, part);

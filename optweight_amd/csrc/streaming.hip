@@ -6,6 +6,7 @@
 
 #include <map>
 #include <mutex>
+#include <tuple>
 #include <vector>
 
 namespace owmi {
@@ -13,6 +14,28 @@ namespace owmi {
 static thread_local std::string g_last_error;
 void set_error(const std::string &msg) { g_last_error = msg; }
 void clear_error() { g_last_error.clear(); }
+
+void *stream_scratch(int tag, size_t bytes, hipStream_t st) {
+  static std::mutex mu;
+  static std::map<std::tuple<int, hipStream_t, int>, std::pair<void *, size_t>> bufs;
+  int dev = 0;
+  OWMI_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(mu);
+  const auto key = std::make_tuple(dev, st, tag);
+  auto it = bufs.find(key);
+  if (it != bufs.end()) {
+    OWMI_REQUIRE(it->second.second == bytes, "stream_scratch: size changed for a tag");
+    return it->second.first;
+  }
+  hipStreamCaptureMode mode = hipStreamCaptureModeRelaxed;
+  OWMI_HIP(hipThreadExchangeStreamCaptureMode(&mode));
+  void *p = nullptr;
+  const hipError_t e = hipMalloc(&p, bytes);
+  (void)hipThreadExchangeStreamCaptureMode(&mode);
+  if (e != hipSuccess) throw Error(std::string("stream_scratch: hipMalloc: ") + hipGetErrorString(e));
+  bufs[key] = {p, bytes};
+  return p;
+}
 
 // (l, m) of flat index idx in a triangular lmax layout:
 // idx = m (2 lmax + 1 - m) / 2 + l with m <= l <= lmax.
@@ -471,19 +494,11 @@ dot_batch_final_kernel(const double *partial, int n, double *out) {
   if (threadIdx.x == 0) out[blockIdx.x] = red[0];
 }
 
-// Per-device scratch for the two-pass reductions (kDotBlocks doubles).
-static double *dot_scratch() {
-  static std::mutex mu;
-  static std::map<int, double *> bufs;
-  int dev = 0;
-  OWMI_HIP(hipGetDevice(&dev));
-  std::lock_guard<std::mutex> lk(mu);
-  auto it = bufs.find(dev);
-  if (it != bufs.end()) return it->second;
-  double *p = nullptr;
-  OWMI_HIP(hipMalloc(&p, kDotBlocks * sizeof(double)));
-  bufs[dev] = p;
-  return p;
+// Partial sums of the two-pass reductions (kDotBlocks doubles), one buffer
+// per device and stream.
+constexpr int kTagDot = 1;
+static double *dot_scratch(hipStream_t st) {
+  return (double *)stream_scratch(kTagDot, kDotBlocks * sizeof(double), st);
 }
 
 // Batched dot: nbatch independent products, each over nrows rows of nelem
@@ -726,8 +741,8 @@ int owmi_alm_dot(int dtype, const void *a, const void *b, int lmax, long long nr
   return guard([&] {
     check_dtype(dtype);
     long long nelem = nelem_of(lmax);
-    double *part = dot_scratch();
     hipStream_t st = as_stream(stream);
+    double *part = dot_scratch(st);
     if (dtype == OWMI_F64)
       hipLaunchKernelGGL(dot_batch_partial_kernel<double>, dim3(kDotBlocks), dim3(kDotThreads), 0, st,
                          (const double2 *)a, (const double2 *)b, lmax, nelem, nrows, part);
@@ -739,24 +754,31 @@ int owmi_alm_dot(int dtype, const void *a, const void *b, int lmax, long long nr
   });
 }
 
+// The batched dot through a partial-sum buffer `part` of kDotBlocks doubles.
+static void alm_dot_batch_impl(int dtype, const void *a, const void *b, int lmax,
+                               long long nrows, long long nbatch, double *dev_out,
+                               double *part, hipStream_t st) {
+  const long long nelem = nelem_of(lmax);
+  const int bpb = kDotBlocks / (int)nbatch;
+  dim3 g(bpb, (unsigned)nbatch);
+  if (dtype == OWMI_F64)
+    hipLaunchKernelGGL(dot_batch_partial_kernel<double>, g, dim3(kDotThreads), 0, st,
+                       (const double2 *)a, (const double2 *)b, lmax, nelem, nrows, part);
+  else
+    hipLaunchKernelGGL(dot_batch_partial_kernel<float>, g, dim3(kDotThreads), 0, st,
+                       (const float2 *)a, (const float2 *)b, lmax, nelem, nrows, part);
+  hipLaunchKernelGGL(dot_batch_final_kernel, dim3((unsigned)nbatch), dim3(kDotThreads), 0, st,
+                     part, bpb, dev_out);
+}
+
 int owmi_alm_dot_batch(int dtype, const void *a, const void *b, int lmax, long long nrows,
                        long long nbatch, double *dev_out, void *stream) {
   return guard([&] {
     check_dtype(dtype);
     OWMI_REQUIRE(nbatch >= 1 && nbatch <= kDotBlocks, "nbatch must be in [1, 1024]");
-    const long long nelem = nelem_of(lmax);
-    const int bpb = kDotBlocks / (int)nbatch;
-    double *part = dot_scratch();
+    OWMI_REQUIRE(lmax >= 0 && nrows >= 0, "invalid alm_dot_batch sizes");
     hipStream_t st = as_stream(stream);
-    dim3 g(bpb, (unsigned)nbatch);
-    if (dtype == OWMI_F64)
-      hipLaunchKernelGGL(dot_batch_partial_kernel<double>, g, dim3(kDotThreads), 0, st,
-                         (const double2 *)a, (const double2 *)b, lmax, nelem, nrows, part);
-    else
-      hipLaunchKernelGGL(dot_batch_partial_kernel<float>, g, dim3(kDotThreads), 0, st,
-                         (const float2 *)a, (const float2 *)b, lmax, nelem, nrows, part);
-    hipLaunchKernelGGL(dot_batch_final_kernel, dim3((unsigned)nbatch), dim3(kDotThreads), 0, st,
-                       part, bpb, dev_out);
+    alm_dot_batch_impl(dtype, a, b, lmax, nrows, nbatch, dev_out, dot_scratch(st), st);
   });
 }
 

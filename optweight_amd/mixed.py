@@ -40,7 +40,6 @@ it saves (49.4 against 50.7 solves/s), hence off by default.
 '''
 import gc
 import os
-import warnings
 
 import numpy as np
 import torch
@@ -127,12 +126,10 @@ class _Phases:
                     self.fns[k]()
             except RuntimeError as err:
                 # an operator that cannot be recorded (a host copy or sync
-                # inside it): nothing ran during the recording, so the phase
-                # runs eagerly from here on
-                warnings.warn(f'HIP graph recording failed, running eagerly: {err}')
-                self.use_graph = False
-                self.fns[k]()
-                return
+                # inside it): graphs were asked for, so this is an error, not
+                # a silent switch to eager execution
+                raise RuntimeError(f'HIP graph recording of CG phase {k} failed (run with '
+                                   f'graphs=False for eager execution): {err}') from err
             finally:
                 if gc_was_enabled:
                     gc.enable()

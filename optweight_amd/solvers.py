@@ -153,13 +153,19 @@ class CGWienerMap(CG):
     '''Solve (S^-1 + P^T N^-1 P) x = P^T N^-1 d (+ constrained-realisation
     terms) with P = M Y B for device alm (npol, nelem) and maps
     (npol, npix). Same constructor, attributes and methods as optweight's
-    CGWienerMap (lensing and map-space filters are not part of this path).'''
+    CGWienerMap (lensing is not part of this path). ``filt`` is a pair of
+    map-space callables (F, F^T) applied after the mask in the projection
+    and before it in the adjoint (solvers.py:105-111, 177-215); the fused
+    A operator is used only without one.'''
 
     def __init__(self, imap, icov_signal, icov_noise, sht, beam=None, lens=None, filt=None,
                  mask=None, rand_isignal=None, rand_inoise=None, swap_bm=False,
                  scale_filter=None):
-        if lens is not None or filt is not None:
-            raise NotImplementedError('lensing / map filters are not supported')
+        if lens is not None:
+            raise NotImplementedError('lensing is not supported')
+        if filt is not None and not (len(filt) == 2 and callable(filt[0])
+                                     and callable(filt[1])):
+            raise ValueError('filt must be a pair of callables (filter, filter_adjoint)')
         self.imap = imap
         self.icov_signal = icov_signal
         self.icov_noise = icov_noise
@@ -168,7 +174,11 @@ class CGWienerMap(CG):
         self.sht = sht[0]
         self.sht_adjoint = sht[1]
         self.lens = self.lens_adjoint = lambda alm: alm
-        self.filt = self.filt_adjoint = lambda alm: alm
+        self._filt_op = filt
+        if filt is None:
+            self.filt = self.filt_adjoint = lambda omap: omap
+        else:
+            self.filt, self.filt_adjoint = filt
         self._mask_op = mask
         self.mask = mask if mask is not None else (lambda imap: imap)
         self.swap_bm = swap_bm
@@ -240,7 +250,7 @@ class CGWienerMap(CG):
             return self._fast[1]
         plan = None
         y, yt = self.sht, self.sht_adjoint
-        ok = (not self.swap_bm and self._sfilt_op is None
+        ok = (not self.swap_bm and self._sfilt_op is None and self._filt_op is None
               and isinstance(y, operators.YMatVecAlm) and not y.qweight
               and isinstance(yt, operators.YTWMatVecMap) and not yt.qweight
               and isinstance(self.icov_noise, operators.PixMatVecMap)
@@ -308,20 +318,40 @@ class CGWienerMap(CG):
     def get_icov(self):
         return self.icov_signal(self.get_wiener())
 
+    def _solver_dot(self):
+        '''The inner product the solver was initialised with (pixell CG's
+        ``dot``), the default one before init_solver.'''
+        dot = getattr(self, 'dot', None)
+        return dot if dot is not None else _default_dot(self.b0)
+
     def get_chisq(self):
+        '''x^T S^-1 x + (d - P x)^T N^-1 (d - P x) of the scaled solution
+        (solvers.py:331-338), through the solver's inner product; per
+        right-hand side (an array) for a batch.'''
+        dot = self._solver_dot()
         x_w = self.get_wiener()
-        out = alm_utils.contract_almxblm(x_w, self.icov_signal(x_w))
+        out = _host(dot(x_w, self.icov_signal(x_w)))
         res = self.imap - self.proj(x_w)
-        out += float((res * self.icov_noise(res)).sum())
-        return out
+        prod = res * self.icov_noise(res)
+        if isinstance(prod, torch.Tensor):
+            batched = self.b0.ndim == 3
+            pix = (prod.reshape(prod.shape[0], -1).sum(dim=1) if batched else prod.sum())
+            pix = _host(pix.to(torch.float64))
+        else:
+            pix = np.sum(prod)
+        return out + pix
 
     def get_residual(self):
+        '''sqrt((A x - b)^T (A x - b)) under the solver's inner product
+        (solvers.py:340-344).'''
         r = self.A(self.x) - self.b0
-        return np.sqrt(alm_utils.contract_almxblm(r, r))
+        return np.sqrt(_host(self._solver_dot()(r, r)))
 
     def get_qform(self):
-        return (0.5 * alm_utils.contract_almxblm(self.x, self.A(self.x))
-                - alm_utils.contract_almxblm(self.x, self.b0))
+        '''x^T A x / 2 - x^T b under the solver's inner product
+        (solvers.py:346-349).'''
+        dot = self._solver_dot()
+        return 0.5 * _host(dot(self.x, self.A(self.x))) - _host(dot(self.x, self.b0))
 
     @classmethod
     def from_arrays(cls, imap, minfo, ainfo, icov_ell, icov_pix, *extra_args, b_ell=None,

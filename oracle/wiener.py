@@ -178,12 +178,15 @@ def add_operators(op1, op2, slice_2=None):
 # ------------------------------------------------------------------- solver
 
 class WienerMap:
-    '''CGWienerMap restated (swap_bm=False and swap_bm=True paths).'''
+    '''CGWienerMap restated (swap_bm=False and swap_bm=True paths; filt is
+    the (F, F^T) pair of map operators, solvers.py:105-111, 177-215).'''
 
     def __init__(self, imap, icov_signal, icov_noise, sht, beam=None,
                  mask=None, rand_isignal=None, rand_inoise=None,
-                 swap_bm=False, scale_filter=None):
+                 swap_bm=False, scale_filter=None, filt=None):
         self.imap = imap
+        self.filt, self.filt_adjoint = filt if filt is not None else ((lambda m: m),
+                                                                      (lambda m: m))
         self.icov_signal = icov_signal
         self.icov_noise = icov_noise
         self.beam = beam if beam is not None else (lambda a: a)
@@ -196,10 +199,11 @@ class WienerMap:
 
     def proj(self, alm):
         if not self.swap_bm:
-            return self.mask(self.sht(self.beam(alm)))
-        return self.sht(self.beam(self.mask(alm)))
+            return self.filt(self.mask(self.sht(self.beam(alm))))
+        return self.filt(self.sht(self.beam(self.mask(alm))))
 
     def proj_adjoint(self, imap):
+        imap = self.filt_adjoint(imap)
         if not self.swap_bm:
             return self.beam(self.sht_adjoint(self.mask(imap)))
         return self.mask(self.beam(self.sht_adjoint(imap)))
@@ -271,7 +275,7 @@ class WienerMap:
     @classmethod
     def from_arrays(cls, imap, geom, lmax, icov_ell, icov_pix, b_ell=None,
                     mask_pix=None, draw_constr=False, spin=None, swap_bm=False,
-                    sfilt=None, seed=None):
+                    sfilt=None, seed=None, filt=None):
         if spin is None:
             spin = osht.default_spin(imap.shape)
         imap = np.atleast_2d(imap)
@@ -296,7 +300,7 @@ class WienerMap:
         sht = (y_op(lmax, geom, spin), yt_op(geom, lmax, spin))
         return cls(imap, icov_signal, icov_noise, sht, beam=beam, mask=mask,
                    rand_isignal=rand_isignal, rand_inoise=rand_inoise,
-                   swap_bm=swap_bm, scale_filter=sf)
+                   swap_bm=swap_bm, scale_filter=sf, filt=filt)
 
 
 def from_arrays_const_cor(imap, geom, lmax, icov_ell, icov_pix, icov_noise_ell, nsteps=5,

@@ -141,3 +141,54 @@ def test_batched_forward(dev, setup):
         x1 = autodiff.wiener_filter(torch.as_tensor(d2[i], device=dev), a, model)
         assert (torch.linalg.vector_norm(xb[i] - x1) / torch.linalg.vector_norm(x1)).item() \
             < 1e-10
+
+
+def test_full_matrix_noise(dev, setup):
+    '''icov_pix as a full (npol, npol, npix) matrix with a T-Q correlation:
+    the loss's quadratic term is sum_p d_i N^-1_ij d_j and dL/dd of the first
+    term is N^-1 (d - P x) with the full matrix (relative 1e-10 / 1e-8).'''
+    from optweight_amd import operators, preconditioners, alm_utils
+    prob, _, amps0 = setup
+    npix = prob.minfo.npix
+    diag = prob.icov_pix
+    full = np.zeros((3, 3, npix))
+    for i in range(3):
+        full[i, i] = diag[i]
+    full[0, 1] = full[1, 0] = 0.3 * np.sqrt(diag[0] * diag[1])
+    itau = np.array([np.sum(diag[i]) / np.sum(diag[i] > 0) for i in range(3)])
+
+    class FullModel(autodiff.WienerModel):
+        def set_amps(self, amps):
+            icov = self.scaled_icov(amps)
+            self.base.icov_signal = operators.EllMatVecAlm(self.ainfo, icov)
+            self.base._fast = None
+            self.prec = preconditioners.HarmonicPreconditioner(self.ainfo, icov, itau=itau)
+            return icov
+
+    edges = np.array([2, 4, 7, 11, 17])
+    model = FullModel(prob.minfo, prob.ainfo, prob.icov_ell, full, edges, tol=1e-13,
+                      device=dev)
+    d = torch.as_tensor(prob.imap, device=dev).requires_grad_()
+    a = torch.as_tensor(amps0, device=dev)
+    L = autodiff.config_loss(d, a, model)
+    b = model.b_vec(d.detach().contiguous())
+    x = model.solve(b, amps0)
+    dn = prob.imap
+    quad = np.einsum('ip,ijp,jp->', dn, full, dn)
+    w = model.weight
+    exp = 0.5 * (quad - autodiff.contract(b, x, w).item()) + 0.5 * autodiff.contract(x, x, w).item()
+    assert abs(L.item() - exp) <= 1e-10 * abs(exp)
+    # a diagonal-only quadratic term differs (the check above is not vacuous)
+    assert abs(np.sum(dn * diag * dn) - quad) > 1e-6 * abs(quad)
+    d2 = torch.as_tensor(prob.imap, device=dev).requires_grad_()
+    b2 = autodiff.BVec.apply(d2, model)
+    x2 = autodiff.WienerSolve.apply(b2, a, model)
+    ninv = torch.as_tensor(full, device=dev)
+    L1 = 0.5 * (torch.einsum('ip,ijp,jp->', d2, ninv, d2) - autodiff.contract(b2, x2, w))
+    L1.backward()
+    res = d2.detach() - model.base.proj(x2.detach())
+    exp_d = torch.einsum('ijp,jp->ip', ninv, res)
+    err = (torch.linalg.vector_norm(d2.grad - exp_d) / torch.linalg.vector_norm(exp_d)).item()
+    assert err < 1e-8, err
+    with pytest.raises(ValueError):
+        autodiff.config_loss(d.detach()[:2], a, model)

@@ -1,0 +1,213 @@
+'''
+GPU parity of CGWienerMap's outputs and of its map-space filter pair
+against the oracle (float64, seeded problems of the benchmark's
+construction at small lmax):
+  * filt = (F, F^T) (solvers.py:105-111, 177-215), F a map-space smoothing
+    Y L Y^T W and F^T = W Y L Y^T its exact adjoint: b, A and a 5-step CG
+    trajectory against the oracle (1e-12 / 1e-12 / 1e-10);
+  * get_chisq / get_residual / get_qform (solvers.py:321-349) against the
+    oracle's (1e-10), through the solver's own inner product (a custom dot
+    passed to init_solver is used), per right-hand side for a batch;
+  * get_chisq at the converged solution equals the Woodbury form
+    d^T N^-1 d - <b, x> (x^T S^-1 x + (d - P x)^T N^-1 (d - P x) with
+    A x = b), 1e-9.
+'''
+import numpy as np
+import pytest
+import torch
+
+from oracle import geometry as og, wiener as ow, sht as osht
+from optweight_amd import solvers, preconditioners, workloads, operators, alm_utils
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a = a.cpu().numpy() if isinstance(a, torch.Tensor) else np.asarray(a)
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def oracle_geom(minfo):
+    return og.Geom(minfo.theta, minfo.weight, minfo.nphi, phi0=minfo.phi0)
+
+
+def oracle_synth(a, minfo, lmax, spin):
+    out = np.zeros((a.shape[0], minfo.npix))
+    osht.alm2map(a, out, lmax, oracle_geom(minfo), spin)
+    return out
+
+
+@pytest.fixture(scope='module')
+def prob():
+    return workloads.polarized(20, seed=7, ndisk=8, synth=oracle_synth)
+
+
+def lowpass(prob):
+    ell = np.arange(prob.lmax + 1)
+    return np.exp(-0.5 * ell * (ell + 1) * 0.05 ** 2) * np.ones((prob.icov_pix.shape[0], 1))
+
+
+def gpu_filter_pair(prob):
+    '''F = Y L Y^T W and F^T = W Y L Y^T on the device.'''
+    lp = operators.EllMatVecAlm(prob.ainfo, lowpass(prob))
+    y = operators.YMatVecAlm(prob.ainfo, prob.minfo, prob.spin, qweight=False)
+    wy = operators.YMatVecAlm(prob.ainfo, prob.minfo, prob.spin, qweight=True)
+    ytw = operators.YTWMatVecMap(prob.minfo, prob.ainfo, prob.spin, qweight=True)
+    yt = operators.YTWMatVecMap(prob.minfo, prob.ainfo, prob.spin, qweight=False)
+    return (lambda m: y(lp(ytw(m)))), (lambda m: wy(lp(yt(m))))
+
+
+def oracle_filter_pair(prob):
+    g = oracle_geom(prob.minfo)
+    lp = ow.ell_op(lowpass(prob), prob.lmax)
+    y = ow.y_op(prob.lmax, g, prob.spin)
+    wy = ow.y_op(prob.lmax, g, prob.spin, qweight=True)
+    ytw = ow.yt_op(g, prob.lmax, prob.spin, qweight=True)
+    yt = ow.yt_op(g, prob.lmax, prob.spin)
+    return (lambda m: y(lp(ytw(m)))), (lambda m: wy(lp(yt(m))))
+
+
+def build_filtered(prob, dev):
+    base = solvers.CGWienerMap.from_arrays(prob.imap, prob.minfo, prob.ainfo, prob.icov_ell,
+                                           prob.icov_pix, device=dev)
+    solver = solvers.CGWienerMap(base.imap, base.icov_signal, base.icov_noise,
+                                 (base.sht, base.sht_adjoint), filt=gpu_filter_pair(prob))
+    g = oracle_geom(prob.minfo)
+    osol = ow.WienerMap.from_arrays(prob.imap, g, prob.lmax, prob.icov_ell, prob.icov_pix,
+                                    filt=oracle_filter_pair(prob))
+    return solver, osol
+
+
+def test_filter_pair_is_adjoint(dev, prob):
+    f, ft = gpu_filter_pair(prob)
+    rng = np.random.default_rng(5)
+    m = torch.as_tensor(rng.normal(size=(3, prob.minfo.npix)), device=dev)
+    n = torch.as_tensor(rng.normal(size=(3, prob.minfo.npix)), device=dev)
+    a = float((f(m) * n).sum())
+    b = float((m * ft(n)).sum())
+    assert abs(a - b) <= 1e-12 * abs(a)
+
+
+def test_filt_b_vec_a_matrix_and_trajectory(dev, prob):
+    solver, osol = build_filtered(prob, dev)
+    assert rel(solver.b0, osol.b0) < 1e-12
+    rng = np.random.default_rng(1)
+    v = rng.normal(size=osol.b0.shape) + 1j * rng.normal(size=osol.b0.shape)
+    v[:, :prob.lmax + 1] = v[:, :prob.lmax + 1].real
+    got = solver.a_matrix(torch.as_tensor(v, device=dev))
+    assert solver._fast[1] is None, 'a map filter must disable the fused A'
+    assert rel(got, osol.a_matrix(v)) < 1e-12
+    # the filter changes the system (the comparison above is not vacuous)
+    plain = ow.WienerMap.from_arrays(prob.imap, oracle_geom(prob.minfo), prob.lmax,
+                                     prob.icov_ell, prob.icov_pix)
+    assert rel(osol.b0, plain.b0) > 1e-2
+    noise_f = osol.a_matrix(v) - osol.icov_signal(v)
+    noise_p = plain.a_matrix(v) - plain.icov_signal(v)
+    assert rel(noise_f, noise_p) > 1e-2
+    g = oracle_geom(prob.minfo)
+    solver.add_preconditioner(preconditioners.PseudoInvPreconditioner(
+        prob.ainfo, prob.icov_ell, prob.icov_pix, prob.minfo, prob.spin))
+    osol.add_preconditioner(ow.PseudoInvPreconditioner(
+        prob.lmax, prob.icov_ell, prob.icov_pix, g, prob.spin))
+    solver.init_solver()
+    cg = osol.init_solver()
+    for i in range(5):
+        solver.step()
+        cg.step()
+        assert rel(solver.x, cg.x) < 1e-10, f'step {i}'
+
+
+def test_filt_rejects_non_pair(dev, prob):
+    base = solvers.CGWienerMap.from_arrays(prob.imap, prob.minfo, prob.ainfo, prob.icov_ell,
+                                           prob.icov_pix, device=dev)
+    with pytest.raises(ValueError):
+        solvers.CGWienerMap(base.imap, base.icov_signal, base.icov_noise,
+                            (base.sht, base.sht_adjoint), filt=(lambda m: m,))
+
+
+def gpu_solver(prob, dev, imap):
+    solver = solvers.CGWienerMap.from_arrays(imap, prob.minfo, prob.ainfo, prob.icov_ell,
+                                             prob.icov_pix, device=dev)
+    solver.add_preconditioner(preconditioners.PseudoInvPreconditioner(
+        prob.ainfo, prob.icov_ell, prob.icov_pix, prob.minfo, prob.spin))
+    return solver
+
+
+def solver_pair(prob, dev, imap=None):
+    imap = prob.imap if imap is None else imap
+    solver = gpu_solver(prob, dev, imap)
+    g = oracle_geom(prob.minfo)
+    osol = ow.WienerMap.from_arrays(np.asarray(imap), g, prob.lmax, prob.icov_ell,
+                                    prob.icov_pix)
+    osol.add_preconditioner(ow.PseudoInvPreconditioner(
+        prob.lmax, prob.icov_ell, prob.icov_pix, g, prob.spin))
+    return solver, osol
+
+
+def test_outputs_match_oracle(dev, prob):
+    solver, osol = solver_pair(prob, dev)
+    solver.init_solver()
+    cg = osol.init_solver()
+    for _ in range(3):
+        solver.step()
+        cg.step()
+    for name in ('get_chisq', 'get_residual', 'get_qform'):
+        got, ref = getattr(solver, name)(), getattr(osol, name)()
+        assert isinstance(got, float), name
+        assert abs(got - ref) <= 1e-10 * abs(ref), (name, got, ref)
+    assert rel(solver.get_icov(), osol.get_icov()) < 1e-10
+
+
+def test_outputs_use_solver_dot(dev, prob):
+    '''A custom inner product passed to init_solver is the one the outputs use
+    (the reference's self.dot, solvers.py:335-349).'''
+    solver, osol = solver_pair(prob, dev)
+    dot2 = lambda a, b: 2.0 * alm_utils.contract_almxblm(a, b)
+    solver.init_solver(dot=dot2)
+    cg = osol.init_solver()
+    for _ in range(2):
+        solver.step()
+        cg.step()
+    ref_res = osol.get_residual()
+    assert abs(solver.get_residual() - np.sqrt(2.0) * ref_res) <= 1e-10 * ref_res
+    ref_q = osol.get_qform()
+    assert abs(solver.get_qform() - 2.0 * ref_q) <= 1e-10 * abs(ref_q)
+
+
+def test_chisq_woodbury_at_optimum(dev, prob):
+    solver, _ = solver_pair(prob, dev)
+    solver.init_solver()
+    for _ in range(300):
+        solver.step()
+        if solver.err < 1e-24:
+            break
+    # chisq - (d^T N^-1 d - <b, x>) = <x, A x - b>: small once err ~ |r|^2 is
+    assert solver.err < 1e-20
+    x = solver.x
+    d = solver.imap
+    dnd = float((d * solver.icov_noise(d)).sum())
+    bx = alm_utils.contract_almxblm(solver.b0, x)
+    chisq = solver.get_chisq()
+    assert abs(chisq - (dnd - bx)) <= 1e-9 * abs(dnd - bx)
+
+
+def test_outputs_batched_per_rhs(dev, prob):
+    rng = np.random.default_rng(9)
+    imap2 = np.stack([prob.imap, prob.imap + rng.normal(size=prob.imap.shape) * 0.1])
+    bsolver = gpu_solver(prob, dev, imap2)
+    bsolver.init_solver()
+    singles = []
+    for k in range(2):
+        s = gpu_solver(prob, dev, imap2[k])
+        s.init_solver()
+        singles.append(s)
+    for _ in range(3):
+        bsolver.step()
+        for s in singles:
+            s.step()
+    for name in ('get_chisq', 'get_residual', 'get_qform'):
+        got = np.asarray(getattr(bsolver, name)())
+        assert got.shape == (2,), name
+        for k, s in enumerate(singles):
+            ref = getattr(s, name)()
+            assert abs(got[k] - ref) <= 1e-10 * abs(ref), (name, k, got[k], ref)

@@ -278,3 +278,22 @@ def test_map_fused(dev, dtype, case, npix):
         got = map_c_utils.map_fused(a, dcomp=d)
     tol = 0 if dtype == 'float64' else 1e-6
     assert (got - ref).abs().max().item() <= tol * ref.abs().max().item()
+
+
+def test_dots_on_two_streams(dev):
+    '''Batched dots queued on two streams at once each use their own partial
+    sums (one reduction buffer per stream): both equal the serial results.'''
+    lmax, nb = 200, 8
+    a = [torch.as_tensor(np.stack([rand_alm(lmax, 3, 40 + 10 * k + i) for i in range(nb)]),
+                         device=dev) for k in range(2)]
+    ref = [alm_utils.contract_almxblm_batch(x, x).cpu().numpy() for x in a]
+    streams = [torch.cuda.Stream(dev) for _ in range(2)]
+    outs = [torch.empty(nb, dtype=torch.float64, device=dev) for _ in range(2)]
+    torch.cuda.synchronize()
+    for rep in range(20):
+        for k in range(2):
+            with torch.cuda.stream(streams[k]):
+                alm_utils.contract_almxblm_batch(a[k], a[k], out=outs[k])
+        torch.cuda.synchronize()
+        for k in range(2):
+            assert np.array_equal(outs[k].cpu().numpy(), ref[k]), (rep, k)
