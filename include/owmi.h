@@ -181,6 +181,23 @@ int owmi_alm_dot_batch(int dtype, const void *a, const void *b, int lmax,
                        void *stream);
 
 /*
+ * owmi_alm_dot_batch without any allocation: the same products, kernels and
+ * summation order (results equal owmi_alm_dot_batch bit for bit), with the
+ * partial sums in a fixed 8 KiB device array of the library (one per GPU,
+ * held until the process ends) instead of a buffer allocated on a stream's
+ * first call. No counterpart in the reference: for callers that cannot
+ * allocate device memory (code under HIP-graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). Limit: nbatch <= 1024 (the
+ * array holds 1024 partial sums); larger values fail with an error.
+ * Every call on one GPU shares that array: calls that overlap in time on
+ * one GPU (different streams or threads) race and must be serialised by
+ * the caller.
+ */
+int owmi_alm_dot_batch_noalloc(int dtype, const void *a, const void *b, int lmax,
+                               long long nrows, long long nbatch, double *dev_out,
+                               void *stream);
+
+/*
  * Batched mixed-precision x/r update: for each b < nbatch, alpha_b =
  * dev_num[b] / dev_den[b] (0 when dev_den[b] == 0); x_b += alpha_b p_b in
  * dtype_x, r_b -= alpha_b q_b in dtype_r, each over n complex elements

@@ -37,6 +37,7 @@ _SIGNATURES = {
     'owmi_cg_xr': (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _vp, _c_ll, _vp]),
     'owmi_cg_p': (_c_int, [_c_int, _vp, _vp, _vp, _vp, _c_ll, _vp]),
     'owmi_alm_dot_batch': (_c_int, [_c_int, _vp, _vp, _c_int, _c_ll, _c_ll, _vp, _vp]),
+    'owmi_alm_dot_batch_noalloc': (_c_int, [_c_int, _vp, _vp, _c_int, _c_ll, _c_ll, _vp, _vp]),
     'owmi_cg_xr_batch': (_c_int, [_c_int, _vp, _c_int, _vp, _vp, _vp, _vp, _vp, _c_ll, _c_ll,
                                   _vp]),
     'owmi_cg_p_batch': (_c_int, [_c_int, _vp, _vp, _vp, _vp, _c_ll, _c_ll, _vp]),

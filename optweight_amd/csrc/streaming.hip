@@ -501,6 +501,32 @@ static double *dot_scratch(hipStream_t st) {
   return (double *)stream_scratch(kTagDot, kDotBlocks * sizeof(double), st);
 }
 
+// The partial sums of owmi_alm_dot_batch_noalloc: a fixed device array
+// (one copy per GPU, 8 KiB), shared by every call on that GPU. Host code
+// never passes its name to a kernel: the address each device holds it at
+// comes from hipGetSymbolAddress (noalloc_scratch_ptr).
+static __device__ __align__(256) double scratch[kDotBlocks];
+
+// Device address of `scratch` on the current device, looked up once per
+// device (no allocation, host or device).
+constexpr int kMaxDevices = 64;
+static double *noalloc_scratch_ptr() {
+  static std::mutex mu;
+  static double *ptrs[kMaxDevices] = {};
+  int dev = 0;
+  OWMI_HIP(hipGetDevice(&dev));
+  OWMI_REQUIRE(dev >= 0 && dev < kMaxDevices, "alm_dot_batch_noalloc: device index above 63");
+  std::lock_guard<std::mutex> lk(mu);
+  if (ptrs[dev] != nullptr) return ptrs[dev];
+  double *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || p == nullptr)
+    throw Error(std::string("alm_dot_batch_noalloc: hipGetSymbolAddress(scratch): ") +
+                hipGetErrorString(e));
+  ptrs[dev] = p;
+  return p;
+}
+
 // Batched dot: nbatch independent products, each over nrows rows of nelem
 // complex elements; block (x, b) reduces a strided slice of every row of
 // product b. The first lmax + 1 elements of a row (m = 0) count once, the
@@ -779,6 +805,22 @@ int owmi_alm_dot_batch(int dtype, const void *a, const void *b, int lmax, long l
     OWMI_REQUIRE(lmax >= 0 && nrows >= 0, "invalid alm_dot_batch sizes");
     hipStream_t st = as_stream(stream);
     alm_dot_batch_impl(dtype, a, b, lmax, nrows, nbatch, dev_out, dot_scratch(st), st);
+  });
+}
+
+int owmi_alm_dot_batch_noalloc(int dtype, const void *a, const void *b, int lmax,
+                               long long nrows, long long nbatch, double *dev_out,
+                               void *stream) {
+  return guard([&] {
+    check_dtype(dtype);
+    OWMI_REQUIRE(nbatch >= 1 && nbatch <= kDotBlocks,
+                 "alm_dot_batch_noalloc: nbatch must be in [1, 1024] (the fixed scratch holds "
+                 "1024 partial sums)");
+    OWMI_REQUIRE(lmax >= 0 && nrows >= 0, "invalid alm_dot_batch_noalloc sizes");
+    hipStream_t st = as_stream(stream);
+    // every partial is written by the first pass before the second reads
+    // it: nothing needs clearing between calls
+    alm_dot_batch_impl(dtype, a, b, lmax, nrows, nbatch, dev_out, noalloc_scratch_ptr(), st);
   });
 }
 
