@@ -161,19 +161,58 @@ HEADLINE_BATCH = 128
 AUX_BATCH = 16
 
 
-def cpu_baseline_sample(dev, lmax_sample=255, reps=3):
-    """The oracle's float64 A application (numpy, long-double Legendre
-    recurrence, one thread) on the same workload construction at a reduced
-    band limit; returns (seconds per application, lmax_sample)."""
-    from oracle import wiener as ow, geometry as og
-    prob = workloads.polarized(lmax_sample, seed=0, device=dev)
-    g = og.Geom(prob.minfo.theta, prob.minfo.weight, prob.minfo.nphi, phi0=prob.minfo.phi0)
-    osol = ow.WienerMap.from_arrays(prob.imap, g, prob.lmax, prob.icov_ell, prob.icov_pix)
-    v = osol.b0.copy()
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        osol.a_matrix(v)
-    return (time.perf_counter() - t0) / reps, lmax_sample
+def cpu_model():
+    try:
+        with open('/proc/cpuinfo') as f:
+            for line in f:
+                if line.startswith('model name'):
+                    return line.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return 'unknown'
+
+
+def cpu_baseline_solve(prob, iterations, nstep=5):
+    """The oracle's float64 solve of the same workload on the host cores:
+    pixell's CG recurrence with the pseudo-inverse on TQU and the multigrid
+    preconditioner on T (oracle/wiener.py), transforms with the Legendre
+    stage in C/OpenMP on every core this process may use and the ring FFTs
+    in scipy (oracle/fastsht.py). After one untimed set-up and init, nstep
+    CG iterations are timed one by one; solves/s = 1 / (median iteration
+    time x ``iterations``, the float64 GPU solve's iteration count, the
+    same recurrence to the same stopping rule). Returns (baseline dict,
+    oracle iterate after nstep steps)."""
+    from oracle import wiener as ow, geometry as og, fastsht
+    fastsht.install()
+    try:
+        t0 = time.perf_counter()
+        g = og.Geom(prob.minfo.theta, prob.minfo.weight, prob.minfo.nphi, phi0=prob.minfo.phi0)
+        osol = ow.WienerMap.from_arrays(prob.imap, g, prob.lmax, prob.icov_ell, prob.icov_pix)
+        osol.add_preconditioner(ow.PseudoInvPreconditioner(prob.lmax, prob.icov_ell,
+                                                           prob.icov_pix, g, prob.spin))
+        osol.add_preconditioner(ow.MaskedPreconditioner(
+            prob.lmax, prob.icov_ell[0:1, 0:1], 0, prob.mask_bool[0], g, min_pix=1000,
+            n_jacobi=1, lmax_r_ell=6000), sel=np.s_[0:1])
+        cg = osol.init_solver()
+        setup = time.perf_counter() - t0
+        times = []
+        for _ in range(nstep):
+            t1 = time.perf_counter()
+            cg.step()
+            times.append(time.perf_counter() - t1)
+    finally:
+        fastsht.uninstall()
+    med = float(np.median(times))
+    cores = fastsht.threads()
+    out = {'value': 1.0 / (med * iterations), 'unit': 'solves/s', 'cores': cores,
+           'kind': 'port', 'cpu_model': cpu_model(),
+           'iteration_s_median': med, 'iteration_s': times, 'setup_s': setup,
+           'sample': (f'oracle float64 CG (pixell recurrence, pinv TQU + multigrid T) on the '
+                      f'same TQU lmax {prob.lmax} workload, Legendre stage in C/OpenMP on '
+                      f'{cores} threads, ring FFTs in scipy: median of {nstep} timed '
+                      f'iterations ({med:.2f}s) x {iterations:.0f} iterations of the float64 '
+                      f'solve; set-up ({setup:.1f}s) not included')}
+    return out, cg.x
 
 
 def wavelet_bench(args, dev):
@@ -652,6 +691,8 @@ def main():
                          'the batched timings of the other modes)')
     ap.add_argument('--delta', type=float, default=0.001,
                     help='residual reduction between float64 residual replacements')
+    ap.add_argument('--fp64-batch-steps', type=int, default=1,
+                    help='timed plain float64 solves of the whole batch (after one untimed)')
     ap.add_argument('--ragged', action='store_true',
                     help='time the batched constrained realisations of uneven band limit '
                          '(16 right-hand sides at each of lmax 255, 511, 1023, 2047) instead '
@@ -767,20 +808,28 @@ def main():
     roof = legendre_roofline(prob, dev, args.batch)
     log(f'roofline: {roof}')
 
+    # plain float64 CG on the same batch of right-hand sides (batching
+    # without mixed precision), and the mixed solutions' error against it
+    # per right-hand side
+    x64b, el64b, inf64b = time_solves('fp64', bsolver, args.fp64_batch_steps, 1)
+    fp64b_sps = args.fp64_batch_steps * args.batch / el64b
+    errs = (torch.linalg.vector_norm((xmx - x64b).reshape(args.batch, -1), dim=1)
+            / torch.linalg.vector_norm(x64b.reshape(args.batch, -1), dim=1)).cpu().numpy()
+    res64b = true_residual(bsolver, x64b)
+    log(f'fp64 batch {args.batch}: {fp64b_sps:.3f} solves/s, max true residual '
+        f'{res64b.max():.3e}; mixed vs fp64 per RHS rel L2 max {errs.max():.3e} '
+        f'median {np.median(errs):.3e}')
+
     cpu = None
     if args.cpu_baseline == 'on':
-        t_a, lms = cpu_baseline_sample(dev)
-        # scale to lmax 1023 by the Legendre work (N_lm N_theta ~ lmax^3),
-        # then to one solve by the float64 GPU solve's cost in A applications
-        scale = ((args.lmax + 1) / (lms + 1)) ** 3
-        a_ms = time_kernel(lambda: solver.a_matrix(solver.b0), reps=5)
-        solve_in_a = (el64 / args.steps) / (a_ms * 1e-3)
-        cpu = {'value': 1.0 / (t_a * scale * solve_in_a), 'unit': 'solves/s', 'cores': 1,
-               'kind': 'port',
-               'sample': (f'oracle float64 A application (numpy, 1 thread) on the TQU '
-                          f'workload at lmax {lms} ({t_a:.2f}s), scaled by '
-                          f'((lmax+1)/{lms + 1})^3 = {scale:.0f} to lmax {args.lmax} and by '
-                          f'the float64 solve cost in A applications ({solve_in_a:.1f})')}
+        cpu, xo = cpu_baseline_solve(prob, it64)
+        # the same 5 float64 CG steps on the GPU: full-size parity of the
+        # iterate with the oracle
+        solver.init_solver()
+        for _ in range(len(cpu['iteration_s'])):
+            solver.step()
+        cpu['gpu_vs_oracle_rel_l2_5_steps'] = float(
+            np.linalg.norm(solver.x.cpu().numpy() - xo) / np.linalg.norm(xo))
         log(f'cpu baseline: {cpu}')
 
     out = {
@@ -807,6 +856,15 @@ def main():
                    'fp64_solves_per_s': fp64_sps, 'fp64_iterations': it64,
                    'fp64_true_residual': res64,
                    'speedup_vs_fp64': mixed_sps / fp64_sps,
+                   # the speed-up split into batching and precision: plain
+                   # float64 CG on the same batch, and per-RHS distance of
+                   # the mixed solutions to those float64 solutions
+                   'fp64_batched_solves_per_s': fp64b_sps,
+                   'fp64_batched_true_residual_max': float(res64b.max()),
+                   'speedup_fp64_batched_vs_fp64': fp64b_sps / fp64_sps,
+                   'speedup_mixed_vs_fp64_batched': mixed_sps / fp64b_sps,
+                   'rel_l2_vs_fp64_batched_max': float(errs.max()),
+                   'rel_l2_vs_fp64_batched_median': float(np.median(errs)),
                    # batched iterations per second, and the useful Legendre
                    # flops of the timed mixed solves over their wall time
                    'iterations_per_s': itmx * args.steps / elmx,

@@ -1111,14 +1111,17 @@ __device__ __forceinline__ f32x4 mfma16(float a, float b, f32x4 c) {
 
 constexpr int kMaCH = 32;  // multipoles per adjoint chunk
 
-template <int SPIN, int W>
+// CT: column tiles of 16 per block (transforms per block: 8 CT for spin 0,
+// 4 CT for spin 2). More tiles share one recurrence and one cross-wave
+// reduction pass between more products, at 32 CT registers of weights.
+template <int SPIN, int W, int CT>
 __global__ void __launch_bounds__(64 * W)
 adj_mfma_kernel(LegArgs g, long long grp_ts, const R *__restrict__ xs,
                 const int *__restrict__ lfas, const R *__restrict__ sas,
                 const int *__restrict__ lfbs, const R *__restrict__ sbs,
                 const R *__restrict__ coef, const float2 *__restrict__ G, long long ntrans,
                 float2 *out, long long out_ts) {
-  constexpr int NT = SPIN == 0 ? 16 : 8;    // transforms per block
+  constexpr int NT = (SPIN == 0 ? 8 : 4) * CT;    // transforms per block
   constexpr int NCF = SPIN == 0 ? 2 : 3;    // recurrence coefficients per multipole
   constexpr int TS = SPIN == 0 ? 66 : 130;  // tile row stride in floats (bank-conflict free)
   constexpr int TW = kMaCH * TS;            // tile floats per wave
@@ -1162,16 +1165,16 @@ adj_mfma_kernel(LegArgs g, long long grp_ts, const R *__restrict__ xs,
   // (pair, ring, transform[, Q/U]) and lane) into its LDS tile, which the
   // chunks use only later, and builds the operands from there.
   const int bj = lane & 15, bk = lane >> 4;
-  float Bz[64];
+  float Bz[32 * CT];
   {
     const int rn = active ? g.pair_n[ip] : -1;
     const int rs = active ? g.pair_s[ip] : -1;
     float2 *S = reinterpret_cast<float2 *>(T);
     if constexpr (SPIN == 0) {
-      // two halves of 8 transforms: S[(b * 2 + side) * 64 + pair]
+      // one column tile (8 transforms) at a time: S[(b * 2 + side) * 64 + pair]
       static_assert(8 * 2 * 64 * 2 <= TW, "staging fits the tile");
 #pragma unroll
-      for (int ct = 0; ct < 2; ++ct) {
+      for (int ct = 0; ct < CT; ++ct) {
 #pragma unroll
         for (int bb = 0; bb < 8; ++bb) {
           const int b = 8 * ct + bb;
@@ -1185,37 +1188,23 @@ adj_mfma_kernel(LegArgs g, long long grp_ts, const R *__restrict__ xs,
           S[(bb * 2 + 1) * 64 + lane] = vs;
         }
         wave_sync();
-        // Bz[(par * 16 + s) * 2 + ct]: pair 4s + k, transform 8ct + j / 2
+        // Bz[(par * 16 + s) * CT + ct]: pair 4s + k, transform 8ct + j / 2
         const float *Sf = reinterpret_cast<const float *>(S);
 #pragma unroll
         for (int sq = 0; sq < 16; ++sq) {
           const int pl = 4 * sq + bk, bb = bj >> 1, comp = bj & 1;
           const float gn = Sf[2 * ((bb * 2 + 0) * 64 + pl) + comp];
           const float gs = Sf[2 * ((bb * 2 + 1) * 64 + pl) + comp];
-          Bz[(0 * 16 + sq) * 2 + ct] = gn + gs;
-          Bz[(1 * 16 + sq) * 2 + ct] = gn - gs;
+          Bz[(0 * 16 + sq) * CT + ct] = gn + gs;
+          Bz[(1 * 16 + sq) * CT + ct] = gn - gs;
         }
         wave_sync();
       }
     } else {
+      // two column tiles (8 transforms) at a time:
       // S[((b * 2 + qu) * 2 + side) * 64 + pair], qu = 0: Q, 1: U
       static_assert(8 * 2 * 2 * 64 * 2 <= TW, "staging fits the tile");
-#pragma unroll
-      for (int b = 0; b < 8; ++b) {
-#pragma unroll
-        for (int qu = 0; qu < 2; ++qu) {
-          float2 vn = {0, 0}, vs = {0, 0};
-          if (rn >= 0 && b < nt) {
-            const long long row = (t0 + b) * 2 * g.nrings + qu * g.nrings;
-            vn = G[(row + rn) * stride + m];
-            if (rs >= 0) vs = G[(row + rs) * stride + m];
-          }
-          S[((b * 2 + qu) * 2 + 0) * 64 + lane] = vn;
-          S[((b * 2 + qu) * 2 + 1) * 64 + lane] = vs;
-        }
-      }
-      wave_sync();
-      // Bz[s * 2 + ct]: pair 2s + k / 2, function k % 2, transform 4ct + j / 4;
+      static_assert(CT % 2 == 0, "spin 2 stages column tiles in pairs");
       // f = 0: (Zp(N), Zm(S)), f = 1: (Zp(S), Zm(N)) for c < 2 / c >= 2, with
       // Zp = q + i u = (q.x - u.y, q.y + u.x), Zm = q - i u = (q.x + u.y, q.y - u.x)
       const int f = bk & 1, c = bj & 3, comp = c & 1;
@@ -1223,28 +1212,51 @@ adj_mfma_kernel(LegArgs g, long long grp_ts, const R *__restrict__ xs,
       const bool minus = (c < 2) == (comp == 0);
       const float *Sf = reinterpret_cast<const float *>(S);
 #pragma unroll
-      for (int sc = 0; sc < 32; ++sc) {
-        const int pl = 2 * sc + (bk >> 1);
+      for (int h = 0; h < CT / 2; ++h) {
 #pragma unroll
-        for (int ct = 0; ct < 2; ++ct) {
-          const int b = 4 * ct + (bj >> 2);
-          const float qv = Sf[2 * (((b * 2 + 0) * 2 + side) * 64 + pl) + comp];
-          const float uv = Sf[2 * (((b * 2 + 1) * 2 + side) * 64 + pl) + 1 - comp];
-          Bz[sc * 2 + ct] = minus ? qv - uv : qv + uv;
+        for (int bb = 0; bb < 8; ++bb) {
+          const int b = 8 * h + bb;
+#pragma unroll
+          for (int qu = 0; qu < 2; ++qu) {
+            float2 vn = {0, 0}, vs = {0, 0};
+            if (rn >= 0 && b < nt) {
+              const long long row = (t0 + b) * 2 * g.nrings + qu * g.nrings;
+              vn = G[(row + rn) * stride + m];
+              if (rs >= 0) vs = G[(row + rs) * stride + m];
+            }
+            S[((bb * 2 + qu) * 2 + 0) * 64 + lane] = vn;
+            S[((bb * 2 + qu) * 2 + 1) * 64 + lane] = vs;
+          }
         }
+        wave_sync();
+        // Bz[s * CT + ct]: pair 2s + k / 2, function k % 2, transform
+        // 4ct + j / 4 (ct = 2h, 2h + 1)
+#pragma unroll
+        for (int sc = 0; sc < 32; ++sc) {
+          const int pl = 2 * sc + (bk >> 1);
+#pragma unroll
+          for (int cq = 0; cq < 2; ++cq) {
+            const int bb = 4 * cq + (bj >> 2);
+            const float qv = Sf[2 * (((bb * 2 + 0) * 2 + side) * 64 + pl) + comp];
+            const float uv = Sf[2 * (((bb * 2 + 1) * 2 + side) * 64 + pl) + 1 - comp];
+            Bz[sc * CT + 2 * h + cq] = minus ? qv - uv : qv + uv;
+          }
+        }
+        wave_sync();
       }
-      wave_sync();
     }
   }
 
   const R x = active ? xs[ip] : (R)0;
-  const R a0s = active ? sas[2 * si] : (R)0, a1s = active ? sas[2 * si + 1] : (R)0;
-  R b0s = 0, b1s = 0;
-  if (SPIN == 2 && active) {
-    b0s = sbs[2 * si];
-    b1s = sbs[2 * si + 1];
-  }
   R a1 = 0, a2 = 0, b1 = 0, b2 = 0;
+  // The start values (mu_{lf-1}, mu_lf) are read from global memory only in
+  // the chunk that holds the lane's start multipole, not kept in registers
+  // through the kernel.
+  auto seeds = [&](const R *s, int lf, int lb, R &s0, R &s1) {
+    const bool in = active && lf >= lb && lf < lb + kMaCH;
+    s0 = in ? s[2 * si] : (R)0;
+    s1 = in ? s[2 * si + 1] : (R)0;
+  };
   // The coefficients of chunk k + 2 are loaded from global memory during
   // chunk k and staged in the third LDS buffer, so no chunk waits on them.
   const int nch = (lmax - lb0) / kMaCH + 1;
@@ -1279,6 +1291,8 @@ adj_mfma_kernel(LegArgs g, long long grp_ts, const R *__restrict__ xs,
     const bool inj = __any((lfa >= lb && lfa < lb + kMaCH) || (lfb >= lb && lfb < lb + kMaCH));
     if (SPIN == 0) {
       if (inj) {
+        R a0s, a1s;
+        seeds(sas, lfa, lb, a0s, a1s);
         for (int i = 0; i < kMaCH; ++i)
           T[i * TS + lane] = (float)rec_step(ka[i] * x, a1, a2, lb + i, lfa, a0s, a1s);
       } else {
@@ -1293,6 +1307,9 @@ adj_mfma_kernel(LegArgs g, long long grp_ts, const R *__restrict__ xs,
     } else {
       float2 *T2 = reinterpret_cast<float2 *>(T);
       if (inj) {
+        R a0s, a1s, b0s, b1s;
+        seeds(sas, lfa, lb, a0s, a1s);
+        seeds(sbs, lfb, lb, b0s, b1s);
         for (int i = 0; i < kMaCH; ++i) {
           const R ax = ka[i] * x;
           const R p = rec_step(ax + kb[i], a1, a2, lb + i, lfa, a0s, a1s);
@@ -1317,41 +1334,47 @@ adj_mfma_kernel(LegArgs g, long long grp_ts, const R *__restrict__ xs,
     }
     wave_sync();
     // products: D[par][ct] over the wave's ring pairs
-    f32x4 D00 = {0, 0, 0, 0}, D01 = {0, 0, 0, 0}, D10 = {0, 0, 0, 0}, D11 = {0, 0, 0, 0};
+    f32x4 D[2][CT];
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct) D[0][ct] = D[1][ct] = f32x4{0, 0, 0, 0};
     if (SPIN == 0) {
 #pragma unroll
       for (int s = 0; s < 16; ++s) {
         const float ae = T[(2 * bj) * TS + 4 * s + bk];
         const float ao = T[(2 * bj + 1) * TS + 4 * s + bk];
-        D00 = mfma16(ae, Bz[(0 * 16 + s) * 2 + 0], D00);
-        D01 = mfma16(ae, Bz[(0 * 16 + s) * 2 + 1], D01);
-        D10 = mfma16(ao, Bz[(1 * 16 + s) * 2 + 0], D10);
-        D11 = mfma16(ao, Bz[(1 * 16 + s) * 2 + 1], D11);
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) {
+          D[0][ct] = mfma16(ae, Bz[(0 * 16 + s) * CT + ct], D[0][ct]);
+          D[1][ct] = mfma16(ao, Bz[(1 * 16 + s) * CT + ct], D[1][ct]);
+        }
       }
     } else {
 #pragma unroll
       for (int s = 0; s < 32; ++s) {
         const float ae = T[(2 * bj) * TS + 4 * s + bk];
         const float ao = T[(2 * bj + 1) * TS + 4 * s + bk];
-        D00 = mfma16(ae, Bz[s * 2 + 0], D00);
-        D01 = mfma16(ae, Bz[s * 2 + 1], D01);
-        D10 = mfma16(ao, Bz[s * 2 + 0], D10);
-        D11 = mfma16(ao, Bz[s * 2 + 1], D11);
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) {
+          D[0][ct] = mfma16(ae, Bz[s * CT + ct], D[0][ct]);
+          D[1][ct] = mfma16(ao, Bz[s * CT + ct], D[1][ct]);
+        }
       }
     }
-    // partial tiles -> LDS (over this wave's tile): red[(par * 2 + ct) * 16 + j][i]
+    // partial tiles -> LDS (over this wave's tile): red[(par * CT + ct) * 16 + j][i]
+    static_assert(2 * CT * 16 * 16 <= TW, "partial tiles fit the tile");
     {
       const int ib = 4 * bk;
-      *reinterpret_cast<f32x4 *>(&T[((0 * 2 + 0) * 16 + bj) * 16 + ib]) = D00;
-      *reinterpret_cast<f32x4 *>(&T[((0 * 2 + 1) * 16 + bj) * 16 + ib]) = D01;
-      *reinterpret_cast<f32x4 *>(&T[((1 * 2 + 0) * 16 + bj) * 16 + ib]) = D10;
-      *reinterpret_cast<f32x4 *>(&T[((1 * 2 + 1) * 16 + bj) * 16 + ib]) = D11;
+#pragma unroll
+      for (int par = 0; par < 2; ++par)
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct)
+          *reinterpret_cast<f32x4 *>(&T[((par * CT + ct) * 16 + bj) * 16 + ib]) = D[par][ct];
     }
     if (threadIdx.x < kMaCH) store_coef((k + 2) % 3);
     __syncthreads();
     if (SPIN == 0) {
       // items (par, ct, i, column pair)
-      for (int it = threadIdx.x; it < 512; it += 64 * W) {
+      for (int it = threadIdx.x; it < 256 * CT; it += 64 * W) {
         const int pc = it >> 7, i = (it >> 3) & 15, jp = it & 7;
         float re = 0, im = 0;
 #pragma unroll
@@ -1359,8 +1382,8 @@ adj_mfma_kernel(LegArgs g, long long grp_ts, const R *__restrict__ xs,
           re += Tall[w * TW + (pc * 16 + 2 * jp) * 16 + i];
           im += Tall[w * TW + (pc * 16 + 2 * jp + 1) * 16 + i];
         }
-        const int row = (pc >> 1) + 2 * i;
-        const int l = lb + row, b = 8 * (pc & 1) + jp;
+        const int row = pc / CT + 2 * i;
+        const int l = lb + row, b = 8 * (pc % CT) + jp;
         if (l <= lmax && b < nt) {
           const float c = cN[bcur][row];
           om[b * out_ts + l] = {c * re, c * im};
@@ -1368,16 +1391,16 @@ adj_mfma_kernel(LegArgs g, long long grp_ts, const R *__restrict__ xs,
       }
     } else {
       // items (par, ct, i, transform of the tile)
-      for (int it = threadIdx.x; it < 256; it += 64 * W) {
+      for (int it = threadIdx.x; it < 128 * CT; it += 64 * W) {
         const int pc = it >> 6, i = (it >> 2) & 15, tq = it & 3;
         float v[4] = {0, 0, 0, 0};
 #pragma unroll
         for (int w = 0; w < W; ++w)
 #pragma unroll
           for (int cc = 0; cc < 4; ++cc) v[cc] += Tall[w * TW + (pc * 16 + 4 * tq + cc) * 16 + i];
-        const int par = pc >> 1;
+        const int par = pc / CT;
         const int row = par + 2 * i;
-        const int l = lb + row, b = 4 * (pc & 1) + tq;
+        const int l = lb + row, b = 4 * (pc % CT) + tq;
         if (l <= lmax && b < nt) {
           const float sg = par ? -1.0f : 1.0f;
           const float2 sp = {v[0], v[1]}, sm = {sg * v[2], sg * v[3]};
@@ -1558,7 +1581,7 @@ static void launch_adj_nb(owmi_plan_impl *p, const cplx_t<T> *G, cplx_t<T> *alm,
   launch_adj_k<T, SPIN, K, 1>(p, G, alm, alm_ts, ntrans, st);
 }
 
-template <int SPIN, int W>
+template <int SPIN, int W, int CT>
 static void launch_adj_mfma_w(owmi_plan_impl *p, const float2 *G, float2 *alm, long long alm_ts,
                               long long ntrans, hipStream_t st) {
   PlanTables<R> &tb = p->t64;
@@ -1573,14 +1596,16 @@ static void launch_adj_mfma_w(owmi_plan_impl *p, const float2 *G, float2 *alm, l
     grp_ts = ntrans * dst_ts;
     dst = static_cast<float2 *>(p->ensure_work2((size_t)ngroups * grp_ts * sizeof(float2)));
   }
-  const int nt = SPIN == 0 ? 16 : 8;
+  const int nt = (SPIN == 0 ? 8 : 4) * CT;
   dim3 grid(ngroups, (unsigned)(p->mmax + 1), ceil_div(ntrans, nt));
   if constexpr (SPIN == 0)
-    hipLaunchKernelGGL((adj_mfma_kernel<0, W>), grid, dim3(64 * W), 0, st, leg_args(p), grp_ts,
-                       tb.x, tb.lf0, tb.s0, nullptr, nullptr, tb.coef0, G, ntrans, dst, dst_ts);
+    hipLaunchKernelGGL((adj_mfma_kernel<0, W, CT>), grid, dim3(64 * W), 0, st, leg_args(p),
+                       grp_ts, tb.x, tb.lf0, tb.s0, nullptr, nullptr, tb.coef0, G, ntrans, dst,
+                       dst_ts);
   else
-    hipLaunchKernelGGL((adj_mfma_kernel<2, W>), grid, dim3(64 * W), 0, st, leg_args(p), grp_ts,
-                       tb.x, tb.lfp, tb.sp, tb.lfm, tb.sm, tb.coef2, G, ntrans, dst, dst_ts);
+    hipLaunchKernelGGL((adj_mfma_kernel<2, W, CT>), grid, dim3(64 * W), 0, st, leg_args(p),
+                       grp_ts, tb.x, tb.lfp, tb.sp, tb.lfm, tb.sm, tb.coef2, G, ntrans, dst,
+                       dst_ts);
   if (ngroups > 1) {
     dim3 g2(ceil_div(nc * nelem, 256), (unsigned)ntrans);
     hipLaunchKernelGGL(sum_groups_kernel<float>, g2, dim3(256), 0, st, dst, grp_ts, ngroups,
@@ -1588,15 +1613,35 @@ static void launch_adj_mfma_w(owmi_plan_impl *p, const float2 *G, float2 *alm, l
   }
 }
 
+// Column tiles per block of the matrix-core adjoint: 4 for spin 2 from 16
+// transforms (one recurrence per 16 transforms), else 2; OWMI_ADJ_CT
+// (2 or 4) overrides for measurements.
+static int adj_mfma_ct(int spin, long long ntrans) {
+  static const int forced = [] {
+    const char *e = std::getenv("OWMI_ADJ_CT");
+    return e != nullptr ? std::atoi(e) : 0;
+  }();
+  if (forced == 2 || forced == 4) return forced;
+  return (spin == 2 && ntrans >= 16) ? 4 : 2;
+}
+
+template <int SPIN, int W>
+static void launch_adj_mfma_ct(owmi_plan_impl *p, const float2 *G, float2 *alm,
+                               long long alm_ts, long long ntrans, hipStream_t st) {
+  if (adj_mfma_ct(SPIN, ntrans) == 4)
+    return launch_adj_mfma_w<SPIN, W, 4>(p, G, alm, alm_ts, ntrans, st);
+  launch_adj_mfma_w<SPIN, W, 2>(p, G, alm, alm_ts, ntrans, st);
+}
+
 // Waves per block: enough for every ring pair of the plan, at most 8.
 template <int SPIN>
 static void launch_adj_mfma(owmi_plan_impl *p, const float2 *G, float2 *alm, long long alm_ts,
                             long long ntrans, hipStream_t st) {
   const long long np = p->npairs;
-  if (np <= 64) return launch_adj_mfma_w<SPIN, 1>(p, G, alm, alm_ts, ntrans, st);
-  if (np <= 128) return launch_adj_mfma_w<SPIN, 2>(p, G, alm, alm_ts, ntrans, st);
-  if (np <= 256) return launch_adj_mfma_w<SPIN, 4>(p, G, alm, alm_ts, ntrans, st);
-  launch_adj_mfma_w<SPIN, 8>(p, G, alm, alm_ts, ntrans, st);
+  if (np <= 64) return launch_adj_mfma_ct<SPIN, 1>(p, G, alm, alm_ts, ntrans, st);
+  if (np <= 128) return launch_adj_mfma_ct<SPIN, 2>(p, G, alm, alm_ts, ntrans, st);
+  if (np <= 256) return launch_adj_mfma_ct<SPIN, 4>(p, G, alm, alm_ts, ntrans, st);
+  launch_adj_mfma_ct<SPIN, 8>(p, G, alm, alm_ts, ntrans, st);
 }
 
 template <typename T>

@@ -91,6 +91,55 @@ def seeds_spin0(mmax, theta):
     return out
 
 
+def seed_spin(m, s, theta, pref=None):
+    '''slam_lm at l = lstart = max(m, |s|) for s != 0 (longdouble, theta a
+    longdouble array). pref may pass sqrt(binom(2m, m+s)) for m >= |s|.'''
+    th = np.asarray(theta, dtype=LD)
+    if m >= abs(s):
+        # d^m_{m,-s} = (-1)^{m+s} sqrt((2m)!/((m-s)!(m+s)!)) c^{m-s} sn^{m+s}
+        c = np.cos(th / LD(2))
+        sn = np.sin(th / LD(2))
+        if pref is None:
+            # sqrt((2m)! / ((m-s)! (m+s)!)) = sqrt(binom(2m, m+s)), as a
+            # product to keep longdouble precision
+            pref = LD(1)
+            k = m + s
+            for i in range(1, k + 1):
+                pref *= LD(2 * m - k + i) / LD(i)
+            pref = np.sqrt(pref)
+        return ((-1) ** m * np.sqrt(LD(2 * m + 1) / (LD(4) * LD(np.pi)))
+                * pref * c ** (m - s) * sn ** (m + s))
+    lstart = abs(s)
+    return ((-1) ** s * np.sqrt(LD(2 * lstart + 1) / (LD(4) * LD(np.pi)))
+            * _wigner_d_ld(lstart, m, -s, th))
+
+
+def seed_tables(lmax, theta, s):
+    '''Seeds slam at l = max(m, |s|) for m = 0..lmax on the given colatitudes
+    as float64 mantissas and int32 exponents (value = mant * 2^exp), shape
+    (nfunc, lmax + 1, ntheta): one function for s = 0, two (+|s|, -|s|) for
+    spin |s| > 0. The seeds are the longdouble values ``lam_m`` starts from.'''
+    th = np.asarray(theta, dtype=LD)
+    if s == 0:
+        vals = [seeds_spin0(lmax, th)]
+    else:
+        vals = []
+        for sg in (abs(s), -abs(s)):
+            v = np.zeros((lmax + 1, th.size), dtype=LD)
+            # sqrt(binom(2m, m + sg)) for m >= |s|, by its ratio in m
+            pref2 = LD(1)    # binom(2|s|, |s| + sg) = 1 for both signs
+            for m in range(lmax + 1):
+                if m < abs(s):
+                    v[m] = seed_spin(m, sg, th)
+                    continue
+                if m > abs(s):
+                    pref2 *= LD((2 * m) * (2 * m - 1)) / LD((m + sg) * (m - sg))
+                v[m] = seed_spin(m, sg, th, pref=np.sqrt(pref2))
+            vals.append(v)
+    mant, expo = np.frexp(np.stack(vals))
+    return mant.astype(np.float64), expo.astype(np.int32)
+
+
 def lam_m(m, lmax, theta, s, seed=None, x=None):
     '''slam_lm(theta) for l = m..lmax at fixed m, spin s (0, +2 or -2).
 
@@ -113,22 +162,8 @@ def lam_m(m, lmax, theta, s, seed=None, x=None):
         if seed is None:
             seed = seeds_spin0(m, theta)[m]
         first = seed
-    elif m >= abs(s):
-        # d^m_{m,-s} = (-1)^{m+s} sqrt((2m)!/((m-s)!(m+s)!)) c^{m-s} sn^{m+s}
-        c = np.cos(th / LD(2))
-        sn = np.sin(th / LD(2))
-        # prefactor via a product to keep longdouble precision
-        pref = LD(1)
-        # sqrt((2m)! / ((m-s)! (m+s)!)) = sqrt(binom(2m, m+s))
-        k = m + s
-        for i in range(1, k + 1):
-            pref *= LD(2 * m - k + i) / LD(i)
-        pref = np.sqrt(pref)
-        first = ((-1) ** m * np.sqrt(LD(2 * m + 1) / (LD(4) * LD(np.pi)))
-                 * pref * c ** (m - s) * sn ** (m + s))
     else:
-        first = ((-1) ** s * np.sqrt(LD(2 * lstart + 1) / (LD(4) * LD(np.pi)))
-                 * _wigner_d_ld(lstart, m, mp, th))
+        first = seed_spin(m, s, th)
     out[lstart - m] = first
     lam_prev2 = np.zeros_like(x)
     lam_prev = first

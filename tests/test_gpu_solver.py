@@ -248,9 +248,10 @@ def test_fullsky_white_noise_one_step(dev):
 
 
 def test_multigrid_parity(dev, t_problem):
-    '''Level geometry and masks bit-identical; G diagonal and dense G to
-    1e-12; one preconditioner application to 1e-12 with the smoother
-    values shared.'''
+    '''Level geometry and masks bit-identical; the smoother's probed
+    diagonal equal (both round the float64 probe to float32); dense G to
+    1e-12; one preconditioner application to 1e-10, each side with its own
+    smoother and its own pseudo-inverse of the coarsest G.'''
     prob = t_problem
     mask = prob.mask_bool[0]
     prec = preconditioners.MaskedPreconditioner(prob.ainfo, prob.icov_ell[0:1], 0, mask,
@@ -266,18 +267,20 @@ def test_multigrid_parity(dev, t_problem):
         np.testing.assert_array_equal(lv.minfo.weight, ol.geom.weight)
         np.testing.assert_allclose(lv.d_ell, ol.d_ell, rtol=1e-15)
         if lv.pinv_g is None:
-            np.testing.assert_allclose(lv.m_prec, ol.m_prec, rtol=1e-6)
-            ol.m_prec = lv.m_prec.copy()
+            np.testing.assert_allclose(lv.m_prec, ol.m_prec, rtol=1e-15)
         else:
             assert rel(lv.dense_g(), ol.dense_g()) < 1e-12
-            ol.pinv_g = lv.pinv_g.copy()
+            print(f'coarsest pinv(G) GPU vs oracle: {rel(lv.pinv_g, ol.pinv_g):.3e}',
+                  flush=True)
     oprec = ow.MaskedPreconditioner(prob.lmax, prob.icov_ell[0:1], 0, mask, g,
                                     lmax_r_ell=prob.lmax, levels=olevels)
     rng = np.random.default_rng(6)
     v = rng.normal(size=(1, prob.ainfo.nelem)) + 1j * rng.normal(size=(1, prob.ainfo.nelem))
     v[:, :prob.lmax + 1] = v[:, :prob.lmax + 1].real
     got = prec(torch.as_tensor(v, device=dev))
-    assert rel(got, oprec(v)) < 1e-12
+    err = rel(got, oprec(v))
+    print(f'multigrid application GPU vs oracle: {err:.3e}', flush=True)
+    assert err < 1e-10
     # sliced (1-d) input as used through add_preconditioner(sel=np.s_[0])
     got1 = prec(torch.as_tensor(v[0], device=dev))
-    assert rel(got1, oprec(v)[0]) < 1e-12
+    assert rel(got1, oprec(v)[0]) < 1e-10
