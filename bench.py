@@ -125,9 +125,9 @@ def legendre_roofline(prob, dev, batch, reps=10):
     1024-ring GL grid, ``batch`` right-hand sides; from 8 right-hand sides
     the matrix-core kernel adj_mfma_kernel, 8 waves per block over the 512
     ring pairs), launched alone through owmi_sht_legendre and timed with HIP
-    events on its stream. Flops per launch: N_lm * N_theta * 8 * batch, the
-    products the matrix cores execute (no padding: (2lam, s(-2)lam) x
-    (S+, s S-) columns)."""
+    events on its stream (16 transforms per block from a batch of 16).
+    Flops per launch: N_lm * N_theta * 8 * batch, the products the matrix
+    cores execute (no padding: (2lam, s(-2)lam) x (S+, s S-) columns)."""
     plan = sht.get_plan(prob.minfo, prob.lmax, dev)
     nelem = prob.ainfo.nelem
     alm = torch.zeros((batch, 3, nelem), dtype=torch.complex64, device=dev)
@@ -143,16 +143,19 @@ def legendre_roofline(prob, dev, batch, reps=10):
     ms = time_kernel(run, reps=reps)
     flops = nelem * prob.minfo.nrow * 8 * batch
     achieved = flops / (ms * 1e-3) / 1e12
-    kernel = 'adj_mfma_kernel<2, 8>' if batch >= 8 else 'adj2_kernel<float, 2, 4, 4>'
+    # 512 ring pairs: 8 waves per block; 16 transforms per block from a
+    # batch of 16 (column tiles CT = 4), else 8
+    kernel = ('adj_mfma_kernel<2, 8, 4>' if batch >= 16 else
+              'adj_mfma_kernel<2, 8, 2>' if batch >= 8 else 'adj2_kernel<float, 2, 4, 4>')
     return {'kernel': kernel, 'ms': ms, 'flops': flops, 'achieved': achieved}
 
 
-# HBM bytes per launch of the roofline kernel (adj_mfma_kernel<2, 8>, 128
+# HBM bytes per launch of the roofline kernel (adj_mfma_kernel<2, 8, 4>, 128
 # right-hand sides): FETCH_SIZE (x 2.0) + WRITE_SIZE (x 1.0), factors from a
 # 256 MiB copy in the same process, measured with `rocprofv3 --pmc` on
 # `python bench.py --roofline-only`
-# (profiles/adj_mfma2_f32_tqu1023_b128_roofline.md): 2.696 GB + 1.088 GB.
-ROOFLINE_TRAFFIC_BYTES = 3.784e9
+# (profiles/adj_mfma2_f32_tqu1023_b128_roofline.md): 2.756 GB + 1.476 GB.
+ROOFLINE_TRAFFIC_BYTES = 4.232e9
 ROOFLINE_TRAFFIC_SHAPE = (1023, 128)
 
 # right-hand sides per batch: the headline solve, and the batched timings of
