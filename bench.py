@@ -696,6 +696,9 @@ def main():
                     help='residual reduction between float64 residual replacements')
     ap.add_argument('--fp64-batch-steps', type=int, default=1,
                     help='timed plain float64 solves of the whole batch (after one untimed)')
+    ap.add_argument('--autotune', choices=['on', 'off'], default='on',
+                    help='tune the Legendre launch choices of the workload plan (float32 at '
+                         'the batch, float64 single transforms) before timing; cached')
     ap.add_argument('--ragged', action='store_true',
                     help='time the batched constrained realisations of uneven band limit '
                          '(16 right-hand sides at each of lmax 255, 511, 1023, 2047) instead '
@@ -783,6 +786,14 @@ def main():
     data[0] = prob.imap
     bsolver = build_solver(prob, dev, imap=data, use_mg=not args.no_mg)
     log(f'setup {time.perf_counter() - t0:.1f}s')
+    tuned = {}
+    if args.autotune == 'on':
+        from optweight_amd import autotune
+        for dt, nb in ((torch.float32, args.batch), (torch.float64, 1)):
+            res = autotune.tune(prob.minfo, prob.lmax, dt, nb, device=dev)
+            tuned[f'{str(dt)[6:]}_batch{nb}'] = {k: res[k] for k in ('params', 'cached',
+                                                                        'tune_seconds')}
+        log(f'autotune: {tuned}')
 
     # plain float64 GPU solve, one right-hand side per step
     x64, el64, inf64 = time_solves('fp64', solver, args.steps, args.warmup)
@@ -868,6 +879,7 @@ def main():
                    'speedup_mixed_vs_fp64_batched': mixed_sps / fp64b_sps,
                    'rel_l2_vs_fp64_batched_max': float(errs.max()),
                    'rel_l2_vs_fp64_batched_median': float(np.median(errs)),
+                   'autotune': tuned,
                    # batched iterations per second, and the useful Legendre
                    # flops of the timed mixed solves over their wall time
                    'iterations_per_s': itmx * args.steps / elmx,

@@ -338,6 +338,26 @@ int owmi_sht_plan_destroy(owmi_sht_plan *plan);
 long long owmi_sht_plan_npix(const owmi_sht_plan *plan);
 
 /*
+ * Launch choices of a plan's Legendre kernels, set by an autotuner. No
+ * counterpart in the reference (ducc0 picks its blocking internally); the
+ * values only change which kernel variant runs and the order of float
+ * sums, never the transform computed. A setting applies to transforms of
+ * at least min_batch right-hand sides: the entry with the largest
+ * min_batch <= ntrans is used, and a value of 0 keeps the library's rule.
+ */
+#define OWMI_TUNE_MFMA_MIN 1 /* float32 batches from this size on the matrix cores (>= 1) */
+#define OWMI_TUNE_ADJ_CT0 2  /* matrix-core adjoint, spin 0: 2 or 4 column tiles (16 / 32 transforms per block) */
+#define OWMI_TUNE_ADJ_CT2 3  /* matrix-core adjoint, spin 2: 2 or 4 column tiles (8 / 16 transforms per block) */
+#define OWMI_TUNE_SYNTH_NB 4 /* vector synthesis: at most 1, 2 or 4 transforms per thread */
+#define OWMI_TUNE_ADJ_NB 5   /* vector adjoint: at most 1, 2 or 4 transforms per block row */
+#define OWMI_TUNE_ADJ_K0 6   /* vector adjoint, spin 0: 1, 2 or 4 ring pairs per lane */
+#define OWMI_TUNE_ADJ_K2 7   /* vector adjoint, spin 2: 1 or 2 ring pairs per lane */
+int owmi_sht_plan_tune(owmi_sht_plan *plan, long long min_batch, int param, int value);
+
+/* Drop every tuned setting of a plan (back to the library's rules). */
+int owmi_sht_plan_tune_reset(owmi_sht_plan *plan);
+
+/*
  * Synthesis of ntrans transforms of one spin group (spin 0: 1 component,
  * spin 2: 2 components (E, B) -> (Q, U), HEALPix convention).
  * alm: transform t, component c at alm + (t * alm_tstride + c * nelem);

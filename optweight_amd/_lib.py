@@ -56,6 +56,8 @@ _SIGNATURES = {
                                             _c_int, _c_ll]),
     'owmi_sht_plan_destroy': (_c_int, [_vp]),
     'owmi_sht_plan_npix': (_c_ll, [_vp]),
+    'owmi_sht_plan_tune': (_c_int, [_vp, _c_ll, _c_int, _c_int]),
+    'owmi_sht_plan_tune_reset': (_c_int, [_vp]),
     'owmi_sht_synthesis': (_c_int, [_vp, _c_int, _c_int, _vp, _c_ll, _vp, _c_ll, _c_ll, _c_int, _vp]),
     'owmi_sht_adjoint': (_c_int, [_vp, _c_int, _c_int, _vp, _c_ll, _vp, _c_ll, _c_ll, _c_int, _vp]),
     'owmi_sht_pixel_sandwich': (_c_int, [_vp, _c_int, _c_int, _vp, _c_ll, _vp, _c_ll, _vp, _c_ll,

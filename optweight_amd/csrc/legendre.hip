@@ -1480,14 +1480,24 @@ static void launch_synth_nb(owmi_plan_impl *p, int spin, const cplx_t<T> *alm, l
                        tb.sp, tb.lfm, tb.sm, tb.coef2, alm, alm_ts, ntrans, G);
 }
 
-// Matrix-core Legendre kernels for float32 batches of at least 8 transforms,
-// unless OWMI_LEGENDRE_VALU=1 selects the vector kernels (for comparisons).
-static bool use_mfma(long long ntrans) {
+// Matrix-core Legendre kernels for float32 batches of at least 8 transforms
+// (or the plan's tuned minimum), unless OWMI_LEGENDRE_VALU=1 selects the
+// vector kernels (for comparisons).
+static bool use_mfma(const owmi_plan_impl *p, long long ntrans) {
   static const bool off = [] {
     const char *e = std::getenv("OWMI_LEGENDRE_VALU");
     return e != nullptr && e[0] == '1';
   }();
-  return !off && ntrans >= 8;
+  const int tuned = p->tuned(OWMI_TUNE_MFMA_MIN, ntrans);
+  return !off && ntrans >= (tuned > 0 ? tuned : 8);
+}
+
+// Largest of 4, 2, 1 not above ntrans and the tuned cap (0: no cap).
+static int nb_for(long long ntrans, int cap) {
+  const int c = cap > 0 ? cap : 4;
+  if (ntrans >= 4 && c >= 4) return 4;
+  if (ntrans >= 2 && c >= 2) return 2;
+  return 1;
 }
 
 static void launch_synth_mfma(owmi_plan_impl *p, int spin, const float2 *alm, long long alm_ts,
@@ -1519,11 +1529,12 @@ void launch_synthesis(owmi_plan_impl *p, int spin, const cplx_t<T> *alm, long lo
                       cplx_t<T> *G, long long ntrans, hipStream_t st) {
   OWMI_REQUIRE(ntrans >= 1 && ntrans < 65536, "ntrans must be in [1, 65535]");
   if constexpr (std::is_same<T, float>::value) {
-    if (use_mfma(ntrans)) return launch_synth_mfma(p, spin, alm, alm_ts, G, ntrans, st);
+    if (use_mfma(p, ntrans)) return launch_synth_mfma(p, spin, alm, alm_ts, G, ntrans, st);
   }
-  if (ntrans >= 4)
+  const int nb = nb_for(ntrans, p->tuned(OWMI_TUNE_SYNTH_NB, ntrans));
+  if (nb == 4)
     launch_synth_nb<T, 4>(p, spin, alm, alm_ts, G, ntrans, st);
-  else if (ntrans >= 2)
+  else if (nb == 2)
     launch_synth_nb<T, 2>(p, spin, alm, alm_ts, G, ntrans, st);
   else
     launch_synth_nb<T, 1>(p, spin, alm, alm_ts, G, ntrans, st);
@@ -1574,10 +1585,11 @@ template <typename T, int SPIN, int K>
 static void launch_adj_nb(owmi_plan_impl *p, const cplx_t<T> *G, cplx_t<T> *alm,
                           long long alm_ts, long long ntrans, hipStream_t st) {
   constexpr int nbmax = SPIN == 0 ? AdjCfg<T>::nb0 : AdjCfg<T>::nb2;
+  const int nb = nb_for(ntrans, p->tuned(OWMI_TUNE_ADJ_NB, ntrans));
   if constexpr (nbmax >= 4) {
-    if (ntrans >= 4) return launch_adj_k<T, SPIN, K, 4>(p, G, alm, alm_ts, ntrans, st);
+    if (nb == 4) return launch_adj_k<T, SPIN, K, 4>(p, G, alm, alm_ts, ntrans, st);
   }
-  if (ntrans >= 2) return launch_adj_k<T, SPIN, K, 2>(p, G, alm, alm_ts, ntrans, st);
+  if (nb >= 2) return launch_adj_k<T, SPIN, K, 2>(p, G, alm, alm_ts, ntrans, st);
   launch_adj_k<T, SPIN, K, 1>(p, G, alm, alm_ts, ntrans, st);
 }
 
@@ -1616,19 +1628,21 @@ static void launch_adj_mfma_w(owmi_plan_impl *p, const float2 *G, float2 *alm, l
 // Column tiles per block of the matrix-core adjoint: 4 for spin 2 from 16
 // transforms (one recurrence per 16 transforms), else 2; OWMI_ADJ_CT
 // (2 or 4) overrides for measurements.
-static int adj_mfma_ct(int spin, long long ntrans) {
+static int adj_mfma_ct(const owmi_plan_impl *p, int spin, long long ntrans) {
   static const int forced = [] {
     const char *e = std::getenv("OWMI_ADJ_CT");
     return e != nullptr ? std::atoi(e) : 0;
   }();
   if (forced == 2 || forced == 4) return forced;
+  const int tuned = p->tuned(spin == 0 ? OWMI_TUNE_ADJ_CT0 : OWMI_TUNE_ADJ_CT2, ntrans);
+  if (tuned == 2 || tuned == 4) return tuned;
   return (spin == 2 && ntrans >= 16) ? 4 : 2;
 }
 
 template <int SPIN, int W>
 static void launch_adj_mfma_ct(owmi_plan_impl *p, const float2 *G, float2 *alm,
                                long long alm_ts, long long ntrans, hipStream_t st) {
-  if (adj_mfma_ct(SPIN, ntrans) == 4)
+  if (adj_mfma_ct(p, SPIN, ntrans) == 4)
     return launch_adj_mfma_w<SPIN, W, 4>(p, G, alm, alm_ts, ntrans, st);
   launch_adj_mfma_w<SPIN, W, 2>(p, G, alm, alm_ts, ntrans, st);
 }
@@ -1649,7 +1663,7 @@ void launch_adjoint(owmi_plan_impl *p, int spin, const cplx_t<T> *G, cplx_t<T> *
                     long long alm_ts, long long ntrans, hipStream_t st) {
   OWMI_REQUIRE(ntrans >= 1 && ntrans < 65536, "ntrans must be in [1, 65535]");
   if constexpr (std::is_same<T, float>::value) {
-    if (use_mfma(ntrans)) {
+    if (use_mfma(p, ntrans)) {
       if (spin == 0) return launch_adj_mfma<0>(p, G, alm, alm_ts, ntrans, st);
       return launch_adj_mfma<2>(p, G, alm, alm_ts, ntrans, st);
     }
@@ -1657,14 +1671,19 @@ void launch_adjoint(owmi_plan_impl *p, int spin, const cplx_t<T> *G, cplx_t<T> *
   // Ring pairs per lane: 2, or 1 on geometries of at most 64 ring pairs
   // (coarse or ring-subset plans), where two would leave most slots empty.
   // More pairs per lane amortise the cross-lane reduction further but cost
-  // registers and waves (measured slower at lmax 1023).
+  // registers and waves (measured slower at lmax 1023); a tuned value
+  // (spin 0: 1, 2, 4; spin 2: 1, 2) replaces the rule.
   const long long np = p->npairs;
   if (spin == 0) {
-    if (np <= 64) return launch_adj_nb<T, 0, 1>(p, G, alm, alm_ts, ntrans, st);
-    if (np <= 128) return launch_adj_nb<T, 0, 2>(p, G, alm, alm_ts, ntrans, st);
+    int k = p->tuned(OWMI_TUNE_ADJ_K0, ntrans);
+    if (k != 1 && k != 2 && k != 4) k = np <= 64 ? 1 : np <= 128 ? 2 : 4;
+    if (k == 1) return launch_adj_nb<T, 0, 1>(p, G, alm, alm_ts, ntrans, st);
+    if (k == 2) return launch_adj_nb<T, 0, 2>(p, G, alm, alm_ts, ntrans, st);
     return launch_adj_nb<T, 0, 4>(p, G, alm, alm_ts, ntrans, st);
   }
-  if (np <= 64) return launch_adj_nb<T, 2, 1>(p, G, alm, alm_ts, ntrans, st);
+  int k = p->tuned(OWMI_TUNE_ADJ_K2, ntrans);
+  if (k != 1 && k != 2) k = np <= 64 ? 1 : 2;
+  if (k == 1) return launch_adj_nb<T, 2, 1>(p, G, alm, alm_ts, ntrans, st);
   launch_adj_nb<T, 2, 2>(p, G, alm, alm_ts, ntrans, st);
 }
 

@@ -92,6 +92,22 @@ struct owmi_plan_impl {
   std::vector<void *> retired;
 
   size_t nchirp = 0, nbhat = 0;  // complex element counts of the tables
+
+  // Launch choices set by an autotuner (owmi_sht_plan_tune): an entry
+  // applies to transforms of at least min_batch right-hand sides (the entry
+  // with the largest such min_batch wins); 0 keeps the library's own rule.
+  struct TuneEntry {
+    long long min_batch = 1;
+    int v[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // by OWMI_TUNE_* parameter
+  };
+  std::vector<TuneEntry> tune;  // sorted by min_batch
+  // value of a parameter for a batch of ntrans (0: library rule)
+  int tuned(int param, long long ntrans) const {
+    int out = 0;
+    for (const TuneEntry &e : tune)
+      if (e.min_batch <= ntrans) out = e.v[param];
+    return out;
+  }
 };
 
 // Plan-time search of the first significant multipole (legendre.hip):

@@ -649,6 +649,50 @@ int owmi_sht_plan_destroy(owmi_sht_plan *plan) {
 
 long long owmi_sht_plan_npix(const owmi_sht_plan *plan) { return plan ? plan->impl.npix : -1; }
 
+int owmi_sht_plan_tune(owmi_sht_plan *plan, long long min_batch, int param, int value) {
+  return guard([&] {
+    OWMI_REQUIRE(plan != nullptr, "plan is NULL");
+    OWMI_REQUIRE(min_batch >= 1, "min_batch must be >= 1");
+    OWMI_REQUIRE(param >= OWMI_TUNE_MFMA_MIN && param <= OWMI_TUNE_ADJ_K2,
+                 "unknown tuning parameter");
+    bool ok = value == 0;
+    switch (param) {
+      case OWMI_TUNE_MFMA_MIN: ok = ok || value >= 1; break;
+      case OWMI_TUNE_ADJ_CT0:
+      case OWMI_TUNE_ADJ_CT2: ok = ok || value == 2 || value == 4; break;
+      case OWMI_TUNE_SYNTH_NB:
+      case OWMI_TUNE_ADJ_NB:
+      case OWMI_TUNE_ADJ_K0: ok = ok || value == 1 || value == 2 || value == 4; break;
+      case OWMI_TUNE_ADJ_K2: ok = ok || value == 1 || value == 2; break;
+    }
+    OWMI_REQUIRE(ok, "value not allowed for this tuning parameter");
+    auto &tune = plan->impl.tune;
+    auto it = std::find_if(tune.begin(), tune.end(), [&](const owmi_plan_impl::TuneEntry &e) {
+      return e.min_batch == min_batch;
+    });
+    if (it == tune.end()) {
+      owmi_plan_impl::TuneEntry e;
+      e.min_batch = min_batch;
+      tune.push_back(e);
+      std::sort(tune.begin(), tune.end(),
+                [](const owmi_plan_impl::TuneEntry &a, const owmi_plan_impl::TuneEntry &b) {
+                  return a.min_batch < b.min_batch;
+                });
+      it = std::find_if(tune.begin(), tune.end(), [&](const owmi_plan_impl::TuneEntry &e) {
+        return e.min_batch == min_batch;
+      });
+    }
+    it->v[param] = value;
+  });
+}
+
+int owmi_sht_plan_tune_reset(owmi_sht_plan *plan) {
+  return guard([&] {
+    OWMI_REQUIRE(plan != nullptr, "plan is NULL");
+    plan->impl.tune.clear();
+  });
+}
+
 }  // extern "C"
 
 template <typename T>

@@ -113,7 +113,11 @@ def get_plan(minfo, lmax, device, rings=None):
             rings = None
     key = (str(device), int(lmax), None if rings is None else rings.tobytes())
     if key not in cache:
-        cache[key] = Plan(minfo, int(lmax), device, rings=rings)
+        plan = Plan(minfo, int(lmax), device, rings=rings)
+        # launch choices an autotuner cached for this geometry
+        from optweight_amd import autotune
+        autotune.apply_cached(plan, minfo, int(lmax), rings)
+        cache[key] = plan
     return cache[key]
 
 
