@@ -1,10 +1,27 @@
 '''
-Preconditioners on device alm, mirroring optweight's preconditioners:
-HarmonicPreconditioner (preconditioners.py:8-124),
-PseudoInvPreconditioner (126-249), PseudoInvPreconditionerWav (251-395),
-MaskedPreconditioner (558-650), MaskedPreconditionerCG (652-789),
-get_2level_prec / get_prec_adef1 / get_prec_adef2 (791-914),
-get_itau_ell_harm (916-948).
+Preconditioners of the Wiener solve on device alm (the callables passed to
+CGWienerMap.add_preconditioner), with the constructors of optweight's
+preconditioners module:
+
+  HarmonicPreconditioner        preconditioners.py:8-124
+  PseudoInvPreconditioner       preconditioners.py:126-249
+  PseudoInvPreconditionerWav    preconditioners.py:251-395
+  PseudoInvPreconditionerFWav   preconditioners.py:397-556
+  MaskedPreconditioner          preconditioners.py:558-650
+  MaskedPreconditionerCG        preconditioners.py:652-789
+  get_2level_prec / get_prec_adef1 / get_prec_adef2   preconditioners.py:791-914
+  get_itau_ell_harm             preconditioners.py:916-948
+
+Every pseudo-inverse here has the form of Seljebotn et al. (2017),
+
+    M = H [ S~^-1 + K^T P K ] H,    H = (S~^-1 + F B T B F)^-1 per l,
+
+with S~^-1 = F S^-1 F (F the optional scale filter, identity by default),
+B the beam, T an isotropic inverse-noise spectrum and K = T B F, or with the
+mask's inverse between T and B for the harmonic-space noise models; P is the
+noise covariance in the space of that model (pixels, needlet maps or 2-D
+Fourier-wavelet maps), applied with quadrature weights. The classes differ
+in T and P only: ``_PseudoInverse`` holds the per-l part, each class its P.
 '''
 import numpy as np
 import torch
@@ -13,8 +30,60 @@ from optweight_amd import (operators, mat_utils, map_utils, multigrid, alm_c_uti
                            alm_utils)
 
 
+class PreconditionerArgumentError(ValueError, AssertionError):
+    '''Inconsistent constructor arguments (a ValueError; also an
+    AssertionError, the type the reference raises for some of them).'''
+
+
+def _ell_sandwich(f, m):
+    '''f_l m_l f_l for every l, (n, n, nell) matrices.'''
+    return np.einsum('ijl,jkl,kol->iol', f, m, f)
+
+
+def _diag_ell(v, npol):
+    '''(npol, nell) diagonal values as (npol, npol, nell) matrices.'''
+    return np.asarray(v) * np.eye(npol)[:, :, None]
+
+
+def _as_matrix_ell(itau):
+    '''An isotropic inverse-noise term as (npol, npol, nell) (a (npol,)
+    vector or an (npol, npol) matrix is taken constant in l).'''
+    itau = np.asarray(itau)
+    if itau.ndim == 1:
+        itau = np.diag(itau)
+    if itau.ndim == 2:
+        itau = itau[:, :, None]
+    return itau
+
+
+def _resolve_tau(itau, minfo, w_ell, mask_pix, icov_pix=None, icov_wav=None,
+                 icov_noise_ell=None):
+    '''The isotropic inverse noise: itau as given, or derived from exactly one
+    noise model (pixel, wavelet or per-l inverse covariance).'''
+    models = {'icov_pix': icov_pix, 'icov_wav': icov_wav, 'icov_noise_ell': icov_noise_ell}
+    given = [name for name, value in models.items() if value is not None]
+    if itau is not None:
+        if given:
+            raise PreconditionerArgumentError(
+                f'itau given together with {", ".join(given)}: pass only one of them')
+        return itau
+    if not given:
+        raise PreconditionerArgumentError(
+            'no noise model: pass itau, icov_pix, icov_wav or icov_noise_ell')
+    if len(given) > 1:
+        raise PreconditionerArgumentError(
+            f'more than one noise model given ({", ".join(given)}): pass exactly one')
+    if icov_pix is not None:
+        return map_utils.get_isotropic_ivar(icov_pix, minfo, mask=mask_pix)
+    if icov_wav is not None:
+        return map_utils.get_ivar_ell(icov_wav, w_ell, mask=mask_pix, minfo_mask=minfo)
+    return get_itau_ell_harm(icov_noise_ell, w_ell)
+
+
 class HarmonicPreconditioner(operators.MatVecAlm):
-    '''M = (S^-1 + B itau B)^-1 per l (eigpow, lim 1e-9).'''
+    '''M = (F S^-1 F + F B T B F)^-1 per l (eigenvalue power, lim 1e-9),
+    T the isotropic inverse noise (given as itau or derived from icov_pix,
+    icov_wav or icov_noise_ell).'''
 
     preserves_input = True
 
@@ -22,390 +91,310 @@ class HarmonicPreconditioner(operators.MatVecAlm):
                  w_ell=None, mask_pix=None, b_ell=None, icov_noise_ell=None, sfilt=None):
         icov_ell = np.asarray(icov_ell)
         npol, nell = icov_ell.shape[-2:]
-        if itau is None:
-            if icov_pix is None and icov_wav is None and icov_noise_ell is None:
-                raise ValueError('provide atleast itau or icov_{pix/wav/noise_ell}')
-            if icov_wav is not None:
-                assert icov_pix is None, 'icov_wav given, icov_pix is not None'
-                assert icov_noise_ell is None, 'icov_wav given, icov_noise_ell is not None'
-                itau = map_utils.get_ivar_ell(icov_wav, w_ell, mask=mask_pix, minfo_mask=minfo)
-            if icov_pix is not None:
-                assert icov_noise_ell is None, 'icov_pix given, icov_noise_ell is not None'
-                itau = map_utils.get_isotropic_ivar(icov_pix, minfo, mask=mask_pix)
-            if icov_noise_ell is not None:
-                itau = get_itau_ell_harm(icov_noise_ell, w_ell)
-        elif icov_pix is not None or icov_wav is not None or icov_noise_ell is not None:
-            raise ValueError('itau is provided but icov_{pix/wav/noise_ell} are not None.')
-        itau = np.asarray(itau)
-        if itau.ndim == 1:
-            itau = itau * np.eye(npol)
-        if itau.ndim == 2:
-            itau = itau[:, :, np.newaxis]
-        if b_ell is None:
-            b_ell = np.ones((npol, nell))
-        b_ell = np.asarray(b_ell) * np.eye(npol)[:, :, np.newaxis]
-        icov_full = mat_utils.full_matrix(icov_ell)
+        tau = _as_matrix_ell(_resolve_tau(itau, minfo, w_ell, mask_pix, icov_pix=icov_pix,
+                                          icov_wav=icov_wav, icov_noise_ell=icov_noise_ell))
+        beam = _diag_ell(np.ones((npol, nell)) if b_ell is None else b_ell, npol)
+        lhs = mat_utils.full_matrix(icov_ell) + _ell_sandwich(beam, tau)
         if sfilt is not None:
-            sfilt = mat_utils.full_matrix(np.asarray(sfilt))
-            op = np.einsum('ijl, jkl, kol -> iol', sfilt, icov_full, sfilt)
-            op2 = np.einsum('ijl, jkl, kol -> iol', b_ell, itau, b_ell)
-            op += np.einsum('ijl, jkl, kol -> iol', sfilt, op2, sfilt)
-        else:
-            op = icov_full + np.einsum('ijl, jkl, kol -> iol', b_ell, itau, b_ell)
-        self.preconditioner = operators.EllMatVecAlm(ainfo, op, -1, inplace=False)
+            lhs = _ell_sandwich(mat_utils.full_matrix(np.asarray(sfilt)), lhs)
+        # the per-l matrix of M (float64), and M as an operator
+        self.lhs = lhs
+        self.preconditioner = operators.EllMatVecAlm(ainfo, lhs, -1, inplace=False)
 
     def call(self, alm):
         return self.preconditioner(alm)
 
 
-class PseudoInvPreconditioner(operators.MatVecAlm):
-    '''Seljebotn et al. (2017) pseudo-inverse:
-    M = H [S^-1 + (tau B) Y^T W N W Y (tau B)] H,  H = (S^-1 + B tau B)^-1.'''
+class _PseudoInverse(operators.MatVecAlm):
+    '''Per-l part shared by the pseudo-inverse preconditioners: the harmonic
+    preconditioner H built from T, the filtered signal term F S^-1 F and the
+    filter F (identity without sfilt); ``call`` runs
+    H [F S^-1 F h + noise_term(h)] with h = H alm.'''
 
     preserves_input = True
+
+    def _setup_ell(self, ainfo, icov_ell, tau, b_ell, sfilt):
+        icov_ell = mat_utils.full_matrix(np.asarray(icov_ell))
+        npol, nell = icov_ell.shape[-2:]
+        self.ainfo = ainfo
+        self.npol = npol
+        self.beam_ell = _diag_ell(np.ones((npol, nell)) if b_ell is None else b_ell, npol)
+        self.filt_ell = None if sfilt is None else mat_utils.full_matrix(np.asarray(sfilt))
+        self.tau_ell = tau
+        self.harmonic = HarmonicPreconditioner(ainfo, icov_ell, itau=tau, b_ell=self.beam_ell,
+                                               sfilt=self.filt_ell)
+        signal = (icov_ell if self.filt_ell is None
+                  else _ell_sandwich(self.filt_ell, icov_ell))
+        self.signal_ell = signal
+        self.signal = operators.EllMatVecAlm(ainfo, signal, inplace=True)
+        self.filt = (operators.EllMatVecAlm(ainfo, self.filt_ell, inplace=True)
+                     if self.filt_ell is not None else (lambda alm: alm))
+        self.beam = operators.EllMatVecAlm(ainfo, self.beam_ell, inplace=True)
+        self.tau = operators.EllMatVecAlm(ainfo, tau, inplace=True)
+
+    def noise_term(self, alm):
+        raise NotImplementedError
+
+    def call(self, alm):
+        h = self.harmonic(alm)
+        total = self.signal(h.clone())
+        total += self.noise_term(h)
+        return self.harmonic(total)
+
+
+class PseudoInvPreconditioner(_PseudoInverse):
+    '''Pixel noise: P = Y^T W N W Y with N = icov_pix^-1 (or cov_pix), T from
+    get_isotropic_ivar; K = T B F (diagonal T only).'''
 
     def __init__(self, ainfo, icov_ell, icov_pix, minfo, spin, itau=None, b_ell=None,
                  cov_pix=None, mask_pix=None, sfilt=None):
         icov_ell = np.asarray(icov_ell)
-        npol, nell = icov_ell.shape[-2:]
+        npol = icov_ell.shape[-2]
         if itau is None:
             itau = map_utils.get_isotropic_ivar(icov_pix, minfo, mask=mask_pix)
-        itau = np.asarray(itau)
-        if itau.ndim == 2:
-            itau = itau[:, :, np.newaxis]
-        if np.count_nonzero(itau - itau * np.eye(npol)[:, :, np.newaxis]):
-            raise NotImplementedError(
-                'Pinv precondition cannot handle off-diagonal itau elements for now.')
-        if b_ell is None:
-            b_ell = np.ones((npol, nell))
-        b_ell = np.asarray(b_ell) * np.eye(npol)[:, :, np.newaxis]
-        if cov_pix is None:
-            cov_pix = mat_utils.matpow(icov_pix, -1)
-        if sfilt is not None:
-            sfilt = mat_utils.full_matrix(np.asarray(sfilt))
-        self.harmonic_prec = HarmonicPreconditioner(ainfo, icov_ell, itau=itau, b_ell=b_ell,
-                                                    sfilt=sfilt)
-        icov_full = mat_utils.full_matrix(icov_ell)
-        if sfilt is not None:
-            self.icov_signal = operators.EllMatVecAlm(
-                ainfo, np.einsum('ijl, jkl, kol -> iol', sfilt, icov_full, sfilt), inplace=True)
-            self.ivar_noise_iso = operators.EllMatVecAlm(
-                ainfo, np.einsum('ijl, jkl, kol -> iol', itau, b_ell, sfilt), inplace=True)
-            self.ivar_noise_iso_adj = operators.EllMatVecAlm(
-                ainfo, np.einsum('ijl, jkl, kol -> iol', sfilt, b_ell, itau), inplace=True)
-        else:
-            self.icov_signal = operators.EllMatVecAlm(ainfo, icov_full, inplace=True)
-            self.ivar_noise_iso = operators.EllMatVecAlm(ainfo, itau * b_ell, inplace=True)
-            self.ivar_noise_iso_adj = self.ivar_noise_iso
-        self.pcov_noise = operators.PixMatVecAlm(ainfo, cov_pix, minfo, spin, inplace=True,
-                                                 adjoint=True)
-        # H (tau B)^T and H S^-1 per multipole (float64, cast to the alm
-        # precision on use): the last three stages of call in one pass
-        h = self.harmonic_prec.preconditioner.m_ell
-        self.ainfo = ainfo
-        self._h_ivar_adj = operators._DeviceCache(
-            np.einsum('ijl, jkl -> ikl', h, self.ivar_noise_iso_adj.m_ell))
-        self._h_icov = operators._DeviceCache(
-            np.einsum('ijl, jkl -> ikl', h, self.icov_signal.m_ell))
+        tau = _as_matrix_ell(itau)
+        if np.count_nonzero(tau * (1 - np.eye(npol)[:, :, None])):
+            raise NotImplementedError('the pixel pseudo-inverse takes a diagonal isotropic '
+                                      'inverse noise only (off-diagonal itau given)')
+        self._setup_ell(ainfo, icov_ell, tau, b_ell, sfilt)
+        filt = (np.broadcast_to(np.eye(npol)[:, :, None], self.beam_ell.shape)
+                if self.filt_ell is None else self.filt_ell)
+        # K = T B F and its transpose, per l
+        k_ell = np.einsum('ijl,jkl,kol->iol', tau, self.beam_ell, filt)
+        kt_ell = np.einsum('ijl,jkl,kol->iol', filt, self.beam_ell, tau)
+        self.k = operators.EllMatVecAlm(ainfo, k_ell, inplace=True)
+        self.noise_cov = operators.PixMatVecAlm(
+            ainfo, mat_utils.matpow(icov_pix, -1) if cov_pix is None else cov_pix, minfo, spin,
+            inplace=True, adjoint=True)
+        # the last stages H K^T n + H S~^-1 h as one per-l pass (float64
+        # products, cast to the alm precision on use)
+        h_ell = self.harmonic.preconditioner.m_ell
+        self._h_kt = operators._DeviceCache(np.einsum('ijl,jkl->ikl', h_ell, kt_ell))
+        self._h_signal = operators._DeviceCache(
+            np.einsum('ijl,jkl->ikl', h_ell, self.signal_ell))
+        self.kt = operators.EllMatVecAlm(ainfo, kt_ell, inplace=True)
+
+    def noise_term(self, alm):
+        return self.kt(self.noise_cov(self.k.apply_copy(alm)))
 
     def call(self, alm):
-        alm = self.harmonic_prec(alm)
         if alm.shape[-2] > 4:
-            # more components than the fused pass takes: stage by stage
-            alm_signal = self.icov_signal.apply_copy(alm)
-            alm_noise = self.pcov_noise(self.ivar_noise_iso.apply_copy(alm))
-            alm_noise = self.ivar_noise_iso_adj(alm_noise)
-            alm_noise += alm_signal
-            return self.harmonic_prec(alm_noise)
-        alm_noise = self.ivar_noise_iso.apply_copy(alm)
-        alm_noise = self.pcov_noise(alm_noise)
-        # H [(tau B)^T noise + S^-1 alm], written over alm (a new array here)
+            # more components than the fused last pass takes
+            return super().call(alm)
+        h = self.harmonic(alm)
+        n = self.noise_cov(self.k.apply_copy(h))
         rdt = operators._real_dtype(alm.dtype)
-        return alm_c_utils.lmul_add(alm_noise, self._h_ivar_adj.get(alm.device, rdt), alm,
-                                    self._h_icov.get(alm.device, rdt), self.ainfo, out=alm)
+        return alm_c_utils.lmul_add(n, self._h_kt.get(alm.device, rdt), h,
+                                    self._h_signal.get(alm.device, rdt), self.ainfo, out=h)
 
 
-class PseudoInvPreconditionerWav(operators.MatVecAlm):
-    """Pseudo-inverse preconditioner for a wavelet-domain noise model
-    (Seljebotn et al. 2017 adapted; preconditioners.py:251-395):
-        M = H [S^-1 + F B M^-1 tau_l N_wav^-1' tau_l M^-1 B F] H
-    with H = (S^-1 + B tau_l B)^-1, tau_l the isotropic inverse-variance
-    spectrum of the wavelet inverse covariance (map_utils.get_ivar_ell, or
-    get_itau_ell_harm from icov_noise_ell), N_wav^-1' the wavelet matrix to
-    the power -1 applied with quadrature weights (WavMatVecAlm(power=-1,
-    adjoint=True)), M^-1 = Y^T W mask^-1 Y and F the optional scale filter.
-    Without sfilt F is the identity (the reference leaves its scale-filter
-    operator unset in that case and fails when applied)."""
+class PseudoInvPreconditionerWav(_PseudoInverse):
+    """Wavelet noise (needlet maps): P = N_wav^-1 to the power -1 with
+    quadrature weights (WavMatVecAlm), between T and the mask's inverse on
+    both sides: noise term F B M^-1 T P T M^-1 B F, with M^-1 = Y^T W
+    mask^-1 Y and T per l from get_ivar_ell (or get_itau_ell_harm for a
+    flattened model with icov_noise_ell). Without sfilt F is the identity
+    (the reference leaves its scale filter unset in that case and fails)."""
 
     def __init__(self, ainfo, icov_ell, icov_wav, w_ell, spin, mask_pix=None, minfo_mask=None,
                  itau_ell=None, b_ell=None, icov_noise_ell=None, sfilt=None):
-        icov_ell = mat_utils.full_matrix(np.asarray(icov_ell))
-        npol, nell = icov_ell.shape[-2:]
         if itau_ell is None:
-            if icov_noise_ell is None:
-                itau_ell = map_utils.get_ivar_ell(icov_wav, w_ell, mask=mask_pix,
-                                                  minfo_mask=minfo_mask)
-            else:
-                itau_ell = get_itau_ell_harm(icov_noise_ell, w_ell)
+            itau_ell = (map_utils.get_ivar_ell(icov_wav, w_ell, mask=mask_pix,
+                                               minfo_mask=minfo_mask)
+                        if icov_noise_ell is None else get_itau_ell_harm(icov_noise_ell, w_ell))
         itau_ell = np.asarray(itau_ell)
         if itau_ell.ndim != 3:
-            raise ValueError(f'Wrong dimensions itau_ell : expected 3, got {itau_ell.ndim}')
-        if b_ell is None:
-            b_ell = np.ones((npol, nell))
-        b_ell = np.asarray(b_ell) * np.eye(npol)[:, :, np.newaxis]
-        if mask_pix is None:
-            self.imask = lambda alm: alm
-        else:
-            self.imask = operators.PixMatVecAlm(ainfo, mask_pix, minfo_mask, spin, power=-1,
+            raise PreconditionerArgumentError(
+                f'itau_ell must be (npol, npol, nell), got {itau_ell.ndim} dimensions')
+        self._setup_ell(ainfo, icov_ell, itau_ell, b_ell, sfilt)
+        self.mask_inv = (operators.PixMatVecAlm(ainfo, mask_pix, minfo_mask, spin, power=-1,
                                                 use_weights=True)
-        if sfilt is not None:
-            sfilt = mat_utils.full_matrix(np.asarray(sfilt))
-        self.harmonic_prec = HarmonicPreconditioner(ainfo, icov_ell, itau=itau_ell, b_ell=b_ell,
-                                                    sfilt=sfilt)
-        if sfilt is not None:
-            self.icov_signal = operators.EllMatVecAlm(
-                ainfo, np.einsum('ijl, jkl, kol -> iol', sfilt, icov_ell, sfilt), inplace=True)
-            self.sfilt = operators.EllMatVecAlm(ainfo, sfilt, inplace=True)
-        else:
-            self.icov_signal = operators.EllMatVecAlm(ainfo, icov_ell, inplace=True)
-            self.sfilt = lambda alm: alm
-        self.beam = operators.EllMatVecAlm(ainfo, b_ell, inplace=True)
-        self.ivar_noise_iso = operators.EllMatVecAlm(ainfo, itau_ell, inplace=True)
+                         if mask_pix is not None else (lambda alm: alm))
         if icov_noise_ell is None:
-            self.pcov_noise = operators.WavMatVecAlm(ainfo, icov_wav, w_ell, spin, power=-1,
-                                                     adjoint=True)
+            self.noise_cov = operators.WavMatVecAlm(ainfo, icov_wav, w_ell, spin, power=-1,
+                                                    adjoint=True)
         else:
-            self.pcov_noise = operators.EllWavEllMatVecAlm(ainfo, icov_wav, w_ell,
-                                                           icov_noise_ell, spin, power_m=-1,
-                                                           power_x=-0.5, adjoint=True)
+            self.noise_cov = operators.EllWavEllMatVecAlm(ainfo, icov_wav, w_ell,
+                                                          icov_noise_ell, spin, power_m=-1,
+                                                          power_x=-0.5, adjoint=True)
 
-    def call(self, alm):
-        alm = self.harmonic_prec(alm.clone())
-        alm_signal = self.icov_signal(alm.clone())
-        alm_noise = self.sfilt(alm)
-        alm_noise = self.beam(alm_noise)
-        alm_noise = self.imask(alm_noise)
-        alm_noise = self.ivar_noise_iso(alm_noise)
-        alm_noise = self.pcov_noise(alm_noise)
-        alm_noise = self.ivar_noise_iso(alm_noise)
-        alm_noise = self.imask(alm_noise)
-        alm_noise = self.beam(alm_noise)
-        alm_noise = self.sfilt(alm_noise)
-        alm_noise += alm_signal
-        return self.harmonic_prec(alm_noise)
+    def noise_term(self, alm):
+        v = self.tau(self.mask_inv(self.beam(self.filt(alm))))
+        v = self.noise_cov(v)
+        return self.filt(self.beam(self.mask_inv(self.tau(v))))
 
 
-class PseudoInvPreconditionerFWav(operators.MatVecAlm):
-    """Pseudo-inverse preconditioner for the 2-D Fourier-wavelet noise
-    model (Seljebotn et al. 2017 adapted; preconditioners.py:397-556):
-        M = H [S^-1 + F B tau_l P tau_l B F] H,  H = (S^-1 + B tau_l B)^-1,
-    P alm = Y^T W mask F^T X^1/2 X^1/2 F mask W Y alm, X = cov_noise_2d (the
-    reference also applies the Fourier-wavelet covariance to the 2-D
-    coefficients between the two X^1/2 but discards its result,
-    preconditioners.py:529-531, so P does not contain it). Without sfilt F
-    is the identity (the reference leaves it unset and fails)."""
+class PseudoInvPreconditionerFWav(_PseudoInverse):
+    """2-D Fourier-wavelet noise: P alm = Y^T W mask F^T X^1/2 X^1/2 F mask
+    W Y alm on the mask's CAR geometry, X = cov_noise_2d (the reference also
+    applies the Fourier-wavelet covariance to the 2-D coefficients between
+    the two X^1/2 and discards its result, preconditioners.py:529-531, so P
+    does not contain it); noise term F B T P T B F. Without sfilt F is the
+    identity (the reference leaves it unset and fails)."""
 
     def __init__(self, ainfo, icov_ell, cov_wav, fkernelset, spin, itau_ell, cov_noise_2d,
                  mask_pix, minfo_mask, b_ell=None, sfilt=None):
-        icov_ell = np.asarray(icov_ell)
-        self.npol, self.nell = icov_ell.shape[-2:]
         itau_ell = np.asarray(itau_ell)
         if itau_ell.ndim != 3:
-            raise ValueError(f'Wrong dimensions itau_ell : expected 3, got {itau_ell.ndim}')
-        if b_ell is None:
-            b_ell = np.ones((self.npol, self.nell))
-        b_ell = np.asarray(b_ell) * np.eye(self.npol)[:, :, np.newaxis]
-        self.ainfo = ainfo
-        self.minfo_mask = minfo_mask
+            raise PreconditionerArgumentError(
+                f'itau_ell must be (npol, npol, nell), got {itau_ell.ndim} dimensions')
+        self._setup_ell(ainfo, icov_ell, itau_ell, b_ell, sfilt)
         self.spin = spin
-        self._mask = operators._DeviceCache(np.asarray(mask_pix))
-        if sfilt is not None:
-            sfilt = mat_utils.full_matrix(np.asarray(sfilt))
-        self.harmonic_prec = HarmonicPreconditioner(ainfo, icov_ell, itau=itau_ell, b_ell=b_ell,
-                                                    sfilt=sfilt)
-        if sfilt is not None:
-            self.icov_signal = operators.EllMatVecAlm(
-                ainfo, np.einsum('ijl, jkl, kol -> iol', sfilt, icov_ell, sfilt), inplace=True)
-            self.sfilt = operators.EllMatVecAlm(ainfo, sfilt, inplace=True)
-        else:
-            self.icov_signal = operators.EllMatVecAlm(ainfo, icov_ell, inplace=True)
-            self.sfilt = lambda alm: alm
-        self.beam = operators.EllMatVecAlm(ainfo, b_ell, inplace=True)
-        self.ivar_noise_iso = operators.EllMatVecAlm(ainfo, itau_ell, inplace=True)
+        self.minfo_mask = minfo_mask
         self.fkernelset = fkernelset
-        self.x_op = operators.FMatVecF(cov_noise_2d, power=0.5, inplace=True)
+        self._mask = operators._DeviceCache(np.asarray(mask_pix))
+        self.x_half = operators.FMatVecF(cov_noise_2d, power=0.5, inplace=True)
 
-    def pcov_noise(self, alm):
-        from optweight_amd import dft, sht
+    def _fourier_noise(self, alm):
+        from optweight_amd import dft
         rdt = operators._real_dtype(alm.dtype)
-        map_tmp = torch.zeros(tuple(alm.shape[:-1]) + (self.minfo_mask.npix,), dtype=rdt,
-                              device=alm.device)
-        sht.alm2map(alm, map_tmp, self.ainfo, self.minfo_mask, self.spin, adjoint=True)
+        omap = torch.zeros(tuple(alm.shape[:-1]) + (self.minfo_mask.npix,), dtype=rdt,
+                           device=alm.device)
+        sht.alm2map(alm, omap, self.ainfo, self.minfo_mask, self.spin, adjoint=True)
         mask = self._mask.get(alm.device, rdt)
-        map_tmp *= mask
-        map2d = map_utils.view_2d(map_tmp, self.minfo_mask)
-        fmap = dft.allocate_fmap(tuple(map2d.shape), rdt, device=alm.device)
-        dft.rfft(map2d, fmap)
-        self.x_op(fmap)
-        self.x_op(fmap)
-        dft.irfft(fmap, map2d)
-        map_tmp *= mask
-        sht.map2alm(map_tmp, alm, self.minfo_mask, self.ainfo, self.spin, adjoint=True)
+        omap *= mask
+        plane = map_utils.view_2d(omap, self.minfo_mask)
+        fmap = dft.allocate_fmap(tuple(plane.shape), rdt, device=alm.device)
+        dft.rfft(plane, fmap)
+        self.x_half(self.x_half(fmap))
+        dft.irfft(fmap, plane)
+        omap *= mask
+        sht.map2alm(omap, alm, self.minfo_mask, self.ainfo, self.spin, adjoint=True)
         return alm
 
-    def call(self, alm):
-        alm = self.harmonic_prec(alm.clone())
-        alm_signal = self.icov_signal(alm.clone())
-        alm_noise = self.sfilt(alm)
-        alm_noise = self.beam(alm_noise)
-        alm_noise = self.ivar_noise_iso(alm_noise)
-        alm_noise = self.pcov_noise(alm_noise)
-        alm_noise = self.ivar_noise_iso(alm_noise)
-        alm_noise = self.beam(alm_noise)
-        alm_noise = self.sfilt(alm_noise)
-        alm_noise += alm_signal
-        return self.harmonic_prec(alm_noise)
+    def noise_term(self, alm):
+        v = self.tau(self.beam(self.filt(alm)))
+        v = self._fourier_noise(v)
+        return self.filt(self.beam(self.tau(v)))
+
+
+def _filtered_signal(icov_ell, sfilt):
+    icov_ell = np.asarray(icov_ell)
+    if icov_ell.ndim == 1:
+        icov_ell = icov_ell[None, :]
+    if sfilt is None:
+        return icov_ell
+    return _ell_sandwich(mat_utils.full_matrix(np.asarray(sfilt)),
+                         mat_utils.full_matrix(icov_ell))
 
 
 class MaskedPreconditioner(operators.MatVecAlm):
     '''Multigrid approximate inverse of the masked-pixel system
-    G = M_u Y d_ell Y^T M_u (Seljebotn et al. 2017), applied as
-    r_ell Y^T M_u V-cycle(M_u Y r_ell alm).'''
+    G = M_u Y d_l Y^T M_u (Seljebotn et al. 2017) on the unobserved pixels
+    M_u, applied as r_l Y^T M_u (V-cycle)^nsteps M_u Y r_l alm with the
+    low-pass r_l (multigrid.lowpass_filter at lmax_r_ell) and the levels of
+    multigrid.get_levels.'''
 
     def __init__(self, ainfo, icov_ell, spin, mask_bool, minfo, min_pix=1000, n_jacobi=1,
                  lmax_r_ell=6000, nsteps=1, sfilt=None, device=None):
-        icov_ell = np.asarray(icov_ell)
-        if icov_ell.ndim == 1:
-            icov_ell = icov_ell[np.newaxis, :]
-        if sfilt is not None:
-            sfilt = mat_utils.full_matrix(np.asarray(sfilt))
-            icov_ell = mat_utils.full_matrix(icov_ell)
-            icov_ell = np.einsum('ijl, jkl, kol -> iol', sfilt, icov_ell, sfilt)
-        self.npol = icov_ell.shape[0]
+        signal = _filtered_signal(icov_ell, sfilt)
+        self.npol = signal.shape[0]
         self.spin = spin
         self.ainfo = ainfo
-        self.n_jacobi = n_jacobi
         self.lmax = ainfo.lmax
+        self.n_jacobi = n_jacobi
         self.nsteps = nsteps
         if device is None:
             device = torch.device('cuda', torch.cuda.current_device())
-        self.levels = multigrid.get_levels(mask_bool, minfo, icov_ell, self.spin,
-                                           min_pix=min_pix, lmax_r_ell=lmax_r_ell, device=device)
+        self.levels = multigrid.get_levels(mask_bool, minfo, signal, spin, min_pix=min_pix,
+                                           lmax_r_ell=lmax_r_ell, device=device)
         self.r_ell = multigrid.lowpass_filter(lmax_r_ell, lmax=self.lmax)
-        self._r = operators._DeviceCache(self.r_ell)
-        self.mask_unobs = self.levels[0].mask_unobs
-        self.minfo = self.levels[0].minfo
+        self._lowpass = operators._DeviceCache(self.r_ell)
+        finest = self.levels[0]
+        self.mask_unobs = finest.mask_unobs
+        self.minfo = finest.minfo
 
     def call(self, alm):
         rdt = operators._real_dtype(alm.dtype)
-        r = self._r.get(alm.device, rdt)
-        alm = alm_c_utils.lmul(alm, r, self.ainfo, alm_out=torch.empty_like(alm))
-        # one right-hand side (nelem,) / (npol, nelem), or a batch (..., npol, nelem)
-        mshape = (self.npol,) if alm.ndim == 1 else tuple(alm.shape[:-1])
-        imap = torch.zeros(mshape + (self.minfo.npix,), dtype=rdt, device=alm.device)
-        rings = self.levels[0].rings
-        sht.alm2map(alm, imap, self.ainfo, self.minfo, self.spin, rings=rings)
-        mu = self.levels[0].mask_dev(alm.device, rdt)
-        imap *= mu
-        omap = None
+        lowpass = self._lowpass.get(alm.device, rdt)
+        finest = self.levels[0]
+        a = alm_c_utils.lmul(alm, lowpass, self.ainfo, alm_out=torch.empty_like(alm))
+        # (nelem,) or (npol, nelem) for one right-hand side, (..., npol, nelem) for a batch
+        lead = (self.npol,) if a.ndim == 1 else tuple(a.shape[:-1])
+        rhs = torch.zeros(lead + (self.minfo.npix,), dtype=rdt, device=a.device)
+        sht.alm2map(a, rhs, self.ainfo, self.minfo, self.spin, rings=finest.rings)
+        unobs = finest.mask_dev(a.device, rdt)
+        rhs *= unobs
+        sol = None
         for _ in range(self.nsteps):
-            omap = multigrid.v_cycle(self.levels, imap, self.spin, n_jacobi=self.n_jacobi, x0=omap)
-        omap *= mu
-        sht.map2alm(omap, alm, self.minfo, self.ainfo, self.spin, adjoint=True, rings=rings)
-        return alm_c_utils.lmul(alm, r, self.ainfo, inplace=True)
+            sol = multigrid.v_cycle(self.levels, rhs, self.spin, n_jacobi=self.n_jacobi, x0=sol)
+        sol *= unobs
+        sht.map2alm(sol, a, self.minfo, self.ainfo, self.spin, adjoint=True, rings=finest.rings)
+        return alm_c_utils.lmul(a, lowpass, self.ainfo, inplace=True)
 
 
 class MaskedPreconditionerCG(operators.MatVecAlm):
-    '''Approximate inverse of the masked-pixel system by nsteps conjugate
-    gradient steps (preconditioners.py:652-789): with alm_lo the l, m <= lmax
-    part of the input (times r_ell if lmax_r_ell is given), solve
-    G x = M_u Y alm_lo with G = M_u Y d_ell Y^T M_u (d_ell = icov_ell r_ell^2)
-    on the equal-area geometry of band limit lmax, inner preconditioner
-    M_u W Y icov_ell^-1 Y^T W M_u, and return r_ell Y^T M_u x in the l, m <=
-    lmax part of a zero alm. Inputs (npol, nelem) or batches
-    (..., npol, nelem): every right-hand side runs its own CG with its own
-    scalars; a right-hand side that is zero gives zero.'''
+    '''Approximate inverse of the masked-pixel system by nsteps of conjugate
+    gradients (preconditioners.py:652-789). The input's l, m <= lmax part
+    (times r_l when lmax_r_ell is given) is synthesised on the equal-area
+    geometry of band limit lmax and masked to the unobserved pixels M_u; CG
+    solves G x = that map with G = M_u Y d_l Y^T M_u (d_l = icov_ell r_l^2,
+    or icov_ell) and inner preconditioner M_u W Y icov_ell^-1 Y^T W M_u;
+    the output is r_l Y^T M_u x in the l, m <= lmax part of a zero alm.
+    Inputs (npol, nelem) or batches (..., npol, nelem): every right-hand
+    side runs its own CG with its own scalars; a zero right-hand side gives
+    zero.'''
 
     def __init__(self, ainfo, icov_ell, spin, mask_bool, minfo, lmax_r_ell=None, lmax=None,
                  nsteps=15, verbose=False, sfilt=None):
-        icov_ell = np.asarray(icov_ell)
+        signal = _filtered_signal(icov_ell, sfilt)
+        self.npol = signal.shape[0]
         self.spin = spin
-        self.npol = icov_ell.shape[0]
         self.ainfo = ainfo
         self.nsteps = nsteps
         self.verbose = verbose
-        if sfilt is not None:
-            sfilt = mat_utils.full_matrix(np.asarray(sfilt))
-            icov_ell = mat_utils.full_matrix(icov_ell)
-            icov_ell = np.einsum('ijl, jkl, kol -> iol', sfilt, icov_ell, sfilt)
-        if lmax is not None:
-            lmax = min(lmax, ainfo.lmax)
-            icov_ell = icov_ell[..., :lmax + 1].copy()
-        else:
-            lmax = ainfo.lmax
-        self.lmax = lmax
-        self.ainfo_lowres = alm_utils.AlmInfo(lmax)
-        if lmax_r_ell is not None:
-            self.r_ell = multigrid.lowpass_filter(lmax_r_ell, lmax=ainfo.lmax)
-            d_ell = icov_ell * self.r_ell[:lmax + 1] ** 2
-            self._r = operators._DeviceCache(np.ascontiguousarray(self.r_ell[:lmax + 1]))
-        else:
-            self.r_ell = None
-            d_ell = icov_ell.copy()
-        mask_bool = np.asarray(mask_bool)
-        if mask_bool.ndim == 1:
-            mask_bool = np.ones((self.npol, mask_bool.size), dtype=mask_bool.dtype) * mask_bool
-        elif mask_bool.shape[0] != self.npol:
-            raise ValueError(f'shape[0] mask != npol {mask_bool.shape[0]} != {self.npol}')
-        if mask_bool.dtype != bool:
-            raise ValueError(f'Input mask must be bool, got {mask_bool.dtype}')
-        mask_bool, minfo = multigrid.get_equal_area_mask_bool(mask_bool, minfo, lmax=lmax)
-        self.mask_unobs = ~mask_bool
-        self.minfo = minfo
-        self._mu = operators._DeviceCache(self.mask_unobs)
-        self.g_op = operators.PixEllPixMatVecMap(self.mask_unobs, d_ell, self.minfo, self.spin,
+        self.lmax = ainfo.lmax if lmax is None else min(lmax, ainfo.lmax)
+        signal = signal[..., :self.lmax + 1].copy()
+        self.ainfo_low = alm_utils.AlmInfo(self.lmax)
+        self.r_ell = (multigrid.lowpass_filter(lmax_r_ell, lmax=ainfo.lmax)
+                      if lmax_r_ell is not None else None)
+        g_ell = signal if self.r_ell is None else signal * self.r_ell[:self.lmax + 1] ** 2
+        if self.r_ell is not None:
+            self._lowpass = operators._DeviceCache(
+                np.ascontiguousarray(self.r_ell[:self.lmax + 1]))
+        observed = np.asarray(mask_bool)
+        if observed.dtype != bool:
+            raise PreconditionerArgumentError(f'mask_bool must be a bool array, not '
+                                              f'{observed.dtype}')
+        if observed.ndim == 1:
+            observed = np.broadcast_to(observed, (self.npol, observed.size)).copy()
+        elif observed.shape[0] != self.npol:
+            raise PreconditionerArgumentError(
+                f'mask_bool has {observed.shape[0]} rows for {self.npol} components')
+        observed, self.minfo = multigrid.get_equal_area_mask_bool(observed, minfo,
+                                                                  lmax=self.lmax)
+        self.mask_unobs = ~observed
+        self._unobs = operators._DeviceCache(self.mask_unobs)
+        self.g_op = operators.PixEllPixMatVecMap(self.mask_unobs, g_ell, self.minfo, spin,
                                                  lmax=self.lmax)
-        self.prec = operators.PixEllPixMatVecMap(self.mask_unobs, icov_ell, self.minfo,
-                                                 self.spin, power_x=-1, lmax=self.lmax,
-                                                 adjoint=True)
-        self.dot = operators.map_dot
+        self.inner_prec = operators.PixEllPixMatVecMap(self.mask_unobs, signal, self.minfo,
+                                                       spin, power_x=-1, lmax=self.lmax,
+                                                       adjoint=True)
 
     def call(self, alm):
         rdt = operators._real_dtype(alm.dtype)
-        alm_lo, _ = alm_utils.trunc_alm(alm, self.ainfo, self.lmax)
+        low, _ = alm_utils.trunc_alm(alm, self.ainfo, self.lmax)
         if self.r_ell is not None:
-            alm_c_utils.lmul(alm_lo, self._r.get(alm.device, rdt), self.ainfo_lowres,
+            alm_c_utils.lmul(low, self._lowpass.get(alm.device, rdt), self.ainfo_low,
                              inplace=True)
-        mshape = (self.npol,) if alm.ndim == 1 else tuple(alm.shape[:-1])
-        imap = torch.zeros(mshape + (self.minfo.npix,), dtype=rdt, device=alm.device)
+        lead = (self.npol,) if alm.ndim == 1 else tuple(alm.shape[:-1])
+        rhs = torch.zeros(lead + (self.minfo.npix,), dtype=rdt, device=alm.device)
         rings = self.g_op.rings
-        sht.alm2map(alm_lo, imap, self.ainfo_lowres, self.minfo, self.spin, rings=rings)
-        mu = self._mu.get(alm.device, torch.bool)
-        imap *= mu
-        omap = operators.inner_cg(self.g_op, self.prec, imap, self.nsteps, dot=self.dot,
-                                  verbose=self.verbose)
-        omap *= mu
-        sht.map2alm(omap, alm_lo, self.minfo, self.ainfo_lowres, self.spin, adjoint=True,
-                    rings=rings)
+        sht.alm2map(low, rhs, self.ainfo_low, self.minfo, self.spin, rings=rings)
+        unobs = self._unobs.get(alm.device, torch.bool)
+        rhs *= unobs
+        sol = operators.inner_cg(self.g_op, self.inner_prec, rhs, self.nsteps,
+                                 dot=operators.map_dot, verbose=self.verbose)
+        sol *= unobs
+        sht.map2alm(sol, low, self.minfo, self.ainfo_low, self.spin, adjoint=True, rings=rings)
         if self.r_ell is not None:
-            alm_c_utils.lmul(alm_lo, self._r.get(alm.device, rdt), self.ainfo_lowres,
+            alm_c_utils.lmul(low, self._lowpass.get(alm.device, rdt), self.ainfo_low,
                              inplace=True)
-        out = torch.zeros_like(alm)
-        return alm_utils.add_to_alm(out, alm_lo, self.ainfo, self.ainfo_lowres)
+        return alm_utils.add_to_alm(torch.zeros_like(alm), low, self.ainfo, self.ainfo_low)
 
 
-def get_2level_prec(prec_main, prec_masked, solver, method, sel_masked=None):
-    '''Two-level preconditioner (preconditioners.py:791-822): 'ADEF-1' or
-    'ADEF-2' combination of the main preconditioner M and the masked one Q
-    through the solver's A; sel_masked restricts Q to a slice.'''
-    if method == 'ADEF-1':
-        return get_prec_adef1(prec_main, prec_masked, solver, sel_masked=sel_masked)
-    if method == 'ADEF-2':
-        return get_prec_adef2(prec_main, prec_masked, solver, sel_masked=sel_masked)
-    raise ValueError(f'method : {method} not supported.')
-
-
-def _apply_sliced(prec, alm, sel):
+def _on_slice(prec, alm, sel):
+    '''prec applied to alm[sel], zero elsewhere (all of alm when sel is None).'''
     if sel is None:
         return prec(alm)
     out = torch.zeros_like(alm)
@@ -414,41 +403,53 @@ def _apply_sliced(prec, alm, sel):
 
 
 def get_prec_adef1(prec_main, prec_masked, solver, sel_masked=None):
-    '''ADEF-1 (preconditioners.py:824-865): M2 b = M (b - A Q b) + Q b.'''
+    '''ADEF-1 two-level combination of a main preconditioner M and a
+    deflation-like one Q (preconditioners.py:824-865):
+    M2 b = M (b - A Q b) + Q b, A the solver's matrix.'''
 
-    def prec_adef1(alm):
-        q_vec = _apply_sliced(prec_masked, alm, sel_masked)
-        out = prec_main(alm - solver.A(q_vec))
-        out += q_vec
+    def adef1(alm):
+        q = _on_slice(prec_masked, alm, sel_masked)
+        out = prec_main(alm - solver.A(q))
+        out += q
         return out
 
-    return prec_adef1
+    return adef1
 
 
 def get_prec_adef2(prec_main, prec_masked, solver, sel_masked=None):
     '''ADEF-2 (preconditioners.py:867-914): M2 b = M b - Q A M b + Q b.'''
 
-    def prec_adef2(alm):
-        m_vec = prec_main(alm)
-        qam_vec = _apply_sliced(prec_masked, solver.A(m_vec), sel_masked)
-        m_vec = m_vec - qam_vec
-        del qam_vec
-        m_vec += _apply_sliced(prec_masked, alm, sel_masked)
-        return m_vec
+    def adef2(alm):
+        mb = prec_main(alm)
+        out = mb - _on_slice(prec_masked, solver.A(mb), sel_masked)
+        out += _on_slice(prec_masked, alm, sel_masked)
+        return out
 
-    return prec_adef2
+    return adef2
+
+
+_TWO_LEVEL = {'ADEF-1': get_prec_adef1, 'ADEF-2': get_prec_adef2}
+
+
+def get_2level_prec(prec_main, prec_masked, solver, method, sel_masked=None):
+    '''The two-level preconditioner named by ``method`` ('ADEF-1' or
+    'ADEF-2', preconditioners.py:791-822).'''
+    try:
+        make = _TWO_LEVEL[method]
+    except KeyError:
+        raise PreconditionerArgumentError(
+            f'unknown two-level method {method!r}; use one of {sorted(_TWO_LEVEL)}') from None
+    return make(prec_main, prec_masked, solver, sel_masked=sel_masked)
 
 
 def get_itau_ell_harm(icov_ell, w_ell):
-    '''Inverse variance per wavelet band, diagonal only (host numpy).'''
+    '''Isotropic inverse noise per l of a per-l noise model restricted to
+    wavelet bands: in band j the w_j^2-weighted mean of each diagonal element,
+    spread back with w_j^2 (host numpy; off-diagonal elements zero).'''
     icov_ell = mat_utils.full_matrix(mat_utils.atleast_nd(np.asarray(icov_ell), 2))
-    itau_ell = np.zeros_like(icov_ell)
-    npol = itau_ell.shape[0]
-    w_ell = np.asarray(w_ell)
-    for widx in range(w_ell.shape[0]):
-        w2 = w_ell[widx] ** 2
-        sw2 = np.sum(w2)
-        for p in range(npol):
-            a_j = np.sum(icov_ell[p, p] * w2) / sw2
-            itau_ell[p, p] += a_j * w2
-    return itau_ell
+    w2 = np.asarray(w_ell) ** 2
+    diag = np.einsum('iil->il', icov_ell)
+    mean = (diag @ w2.T) / np.sum(w2, axis=1)          # (npol, nwav)
+    out = np.zeros_like(icov_ell)
+    np.einsum('iil->il', out)[...] = mean @ w2
+    return out
