@@ -33,8 +33,8 @@ class FKernel:
 
     def __init__(self, fkernel, ly, lx, slices_y=None, slice_x=None, shape_full=None):
         if fkernel.shape[-2:] != (ly.size, lx.size):
-            raise ValueError(f'shape mismatch, fkernel : {fkernel.shape[-2:]}, ly : {ly.size}, '
-                             f'lx : {lx.size}')
+            raise ValueError(f'kernel of shape {fkernel.shape[-2:]} for {ly.size} ly and '
+                             f'{lx.size} lx wavenumbers')
         self.fkernel = fkernel
         self.ly = ly
         self.lx = lx
@@ -52,7 +52,8 @@ class FKernel:
 
     def to_full_array(self):
         if None in (self.slices_y, self.slice_x, self.shape_full):
-            raise ValueError('`to_full_array` requires `slices_y`, `slice_x` and `shape_full`')
+            raise ValueError('this kernel has no place in a full plane (slices_y, slice_x '
+                             'and shape_full unset)')
         full = np.zeros(self.shape_full, dtype=self.dtype)
         dft.add_to_fmap(full, self.fkernel, self.slices_y, self.slice_x)
         return full
@@ -112,16 +113,16 @@ class FKernelSet:
 
     def __setitem__(self, key, fkernel):
         if not isinstance(fkernel, FKernel):
-            raise TypeError(f'Only FKernel instances allowed, got {type(fkernel)}')
+            raise TypeError(f'a kernel set holds FKernel objects, not {type(fkernel).__name__}')
         if int(key) != key:
-            raise TypeError(f'Only integer keys allowed, got {type(key)}')
+            raise TypeError(f'kernel index {key!r} is not an integer')
         if not self.fkernel_dict:
             self.dtype = fkernel.dtype
             self.shape_full = fkernel.shape_full
         if fkernel.dtype != self.dtype:
-            raise TypeError(f'Input is {fkernel.dtype} while set dtype is {self.dtype}')
+            raise TypeError(f'kernel of dtype {fkernel.dtype} added to a set of {self.dtype}')
         if fkernel.shape_full != self.shape_full:
-            raise TypeError(f'Input shape_full : {fkernel.shape_full} while set has '
+            raise TypeError(f'kernel on a {fkernel.shape_full} plane added to a set on '
                             f'{self.shape_full}')
         self.fkernel_dict[int(key)] = fkernel
         self._plans = {}
@@ -142,7 +143,7 @@ class FKernelSet:
 
     def to_full_array(self):
         if not self.fkernel_dict:
-            raise ValueError('Calling `to_full_array` on empty set.')
+            raise ValueError('the kernel set is empty')
         full = np.zeros((len(self),) + tuple(self.shape_full), dtype=self.dtype)
         for key, fk in self:
             dft.add_to_fmap(full[key], fk.fkernel, fk.slices_y, fk.slice_x)
@@ -230,7 +231,7 @@ class FKernelSet:
         with h5py.File(fname, 'r') as hfile:
             for key, val in hfile.items():
                 if not isinstance(val, h5py.Group):
-                    raise ValueError('Contents file not understood.')
+                    raise ValueError(f'{fname}: entry {key!r} is not a kernel group')
                 fks[int(key.split('_')[-1])] = FKernel.from_hdf(val)
         return fks
 
@@ -244,88 +245,83 @@ def _h5py():
 
 
 def digitize_1d(arr):
-    '''On/off approximation of a smooth kernel with values in [0, 1]: a
-    step wherever the rounded running sum changes (fkernel.py:384-410).'''
+    '''On/off approximation of a smooth kernel with values in [0, 1]
+    (fkernel.py:384-410): sample i is on when the rounded running sum of
+    the kernel steps there, so the count of samples that are on tracks the
+    kernel's integral.'''
     arr = np.asarray(arr)
     if arr.min() < 0 or arr.max() > 1:
-        raise ValueError('Input values do not lie between 0 and 1.')
-    csum = np.round(np.cumsum(arr))
-    out = np.concatenate([csum[0:1], csum[1:] != csum[:-1]])
-    return out.astype(bool)
+        raise ValueError(f'a kernel to digitise must lie in [0, 1]; got values in '
+                         f'[{arr.min():.3g}, {arr.max():.3g}]')
+    return np.diff(np.round(np.cumsum(arr)), prepend=0) != 0
+
+
+def _check_disjoint_non_neighbours(w_ell, eps=1e-8):
+    '''Kernels two apart must not overlap: within the even kernels and within
+    the odd ones (groups of two or more), the product of the group stays
+    below eps at every l.'''
+    for parity in (0, 1):
+        group = w_ell[parity::2]
+        if group.shape[0] > 1 and not np.all(np.prod(group, axis=0) < eps):
+            raise ValueError(f'kernels {parity}, {parity + 2}, ... overlap: only neighbouring '
+                             f'kernels may share multipoles')
 
 
 def digitize_kernels(w_ell, upsamp_fact=10):
-    '''Digital kernels on ells_fine = linspace(0, lmax + 1, upsamp_fact
-    (lmax + 1)) (fkernel.py:412-467): even kernels digitised, odd kernels
-    fill the complement (and are off where their smooth value < 1e-5).'''
+    '''Digital (bool) kernels on ells_fine = linspace(0, lmax + 1,
+    upsamp_fact (lmax + 1)) (fkernel.py:412-467): the kernels are linearly
+    interpolated there; the even ones are digitised, the odd ones are on
+    wherever no even one is and their smooth value is at least 1e-5.
+    Returns (d_fine (nwav, nfine) bool, ells_fine).'''
     w_ell = np.asarray(w_ell)
-    nwav = w_ell.shape[0]
-    err = ValueError('Input `w_ell` has overlapping non-neighbouring kernels')
-    eps = 1e-8
-    if nwav > 2 and not np.all(np.prod(w_ell[::2], axis=0) < eps):
-        raise err
-    if nwav > 3 and not np.all(np.prod(w_ell[1::2], axis=0) < eps):
-        raise err
-    lmax = w_ell.shape[-1] - 1
-    ells = np.arange(lmax + 1)
-    ells_fine = np.linspace(0, lmax + 1, upsamp_fact * ells.size)
-    w_fine = np.zeros((nwav, ells_fine.size))
-    d_fine = np.ones_like(w_fine, dtype=bool)
-    for idx in range(nwav):
-        w_fine[idx] = interp1d(ells, w_ell[idx], fill_value='extrapolate',
-                               kind='linear')(ells_fine)
-    for idx in range(0, nwav, 2):
-        d_fine[idx] = digitize_1d(w_fine[idx])
-    d_fine[1::2] &= ~(np.sum(d_fine[::2, :], axis=0)[np.newaxis, :] > 0.5)
-    for idx in range(1, nwav, 2):
-        d_fine[idx, w_fine[idx] < 1e-5] = False
+    _check_disjoint_non_neighbours(w_ell)
+    nell = w_ell.shape[-1]
+    ells_fine = np.linspace(0, nell, upsamp_fact * nell)
+    w_fine = interp1d(np.arange(nell), w_ell, kind='linear', fill_value='extrapolate',
+                      axis=-1)(ells_fine)
+    even = np.array([digitize_1d(w) for w in w_fine[0::2]], dtype=bool)
+    d_fine = np.empty(w_fine.shape, dtype=bool)
+    d_fine[0::2] = even
+    d_fine[1::2] = ~even.any(axis=0) & ~(w_fine[1::2] < 1e-5)
     return d_fine, ells_fine
 
 
-def _find_last_nonzero_idx(arr):
-    arr = np.asarray(arr)
-    if arr.ndim != 1:
-        raise ValueError(f'Input array is not 1D, got ndim : {arr.ndim}')
-    nz = np.nonzero(arr.astype(bool))[0]
-    if nz.size == 0:
-        raise ValueError('Input array is completely zero.')
-    return int(nz[-1])
+def _prefix_len(flags):
+    '''Length of the shortest prefix of ``flags`` holding all its True
+    entries (0 if there is none).'''
+    on = np.flatnonzero(flags)
+    return int(on[-1]) + 1 if on.size else 0
 
 
 def find_kernel_slice(fkernel_arr, minval=1e-7, optimize_len=False):
-    '''Rows (non-negative ly incl. 0, then negative ly) and columns that
-    hold every value >= minval of a kernel on the half plane, the two row
-    ranges symmetric about ly = 0 (fkernel.py:503-598); optimize_len pads
-    the row count to an FFT-friendly length when that stays below ny.'''
+    '''Support of a kernel on the half plane (ny, nlx) (fkernel.py:503-598):
+    ((rows from ly = 0 up, rows of negative ly), columns), holding every
+    value >= minval. Columns start at lx = 0; the rows reach equally far
+    from ly = 0 on both sides (row 0 always kept). With optimize_len the
+    row count grows to an FFT-friendly length (positive side half of the
+    extra, negative side the rest) when that length is below ny.'''
     fkernel_arr = np.asarray(fkernel_arr)
     if fkernel_arr.ndim != 2:
-        raise ValueError(f'Input array must be 2D, got {fkernel_arr.ndim}')
-    nonzero = np.any(fkernel_arr >= minval, axis=0)
-    if not np.any(nonzero):
-        raise ValueError('Input array is completely zero.')
-    slice_x = slice(0, _find_last_nonzero_idx(nonzero) + 1)
-    nonzero = np.any(fkernel_arr >= minval, axis=1)
-    ny = nonzero.size
-    half = int(np.ceil(ny / 2))
-    ypos = nonzero[1:half]
-    yneg = nonzero[half:][::-1]
-    end_pos = _find_last_nonzero_idx(ypos) if ypos.any() else -1
-    end_neg = _find_last_nonzero_idx(yneg) if yneg.any() else -1
-    end = max(end_pos, end_neg)
-    if end == -1:
-        slice_ypos, slice_yneg = slice(0, 1), slice(0, 0)
-    else:
-        slice_ypos = slice(0, min(end + 2, ypos.size + 1))
-        slice_yneg = slice(-min(end + 1, yneg.size), None)
+        raise ValueError(f'a kernel is a 2-D array, got {fkernel_arr.ndim} dimensions')
+    support = fkernel_arr >= minval
+    ncol = _prefix_len(support.any(axis=0))
+    if ncol == 0:
+        raise ValueError(f'the kernel has no value >= {minval}')
+    rows = support.any(axis=1)
+    ny = rows.size
+    mid = -(-ny // 2)
+    above, below = rows[1:mid], rows[mid:][::-1]
+    reach = max(_prefix_len(above), _prefix_len(below))
+    npos = 1 + min(reach, above.size)
+    nneg = min(reach, below.size)
     if optimize_len:
-        ny_cut = slice_ypos.stop - slice_yneg.start
-        ny_opt = dft.get_optimal_fftlen(ny_cut, even=False)
-        if ny_opt < ny:
-            n_pad = ny_opt - ny_cut
-            pad_pos, pad_neg = n_pad // 2, n_pad // 2 + n_pad % 2
-            slice_ypos = slice(slice_ypos.start, slice_ypos.stop + pad_pos)
-            slice_yneg = slice(slice_yneg.start - pad_neg, slice_yneg.stop)
-    return (slice_ypos, slice_yneg), slice_x
+        nopt = dft.get_optimal_fftlen(npos + nneg, even=False)
+        if nopt < ny:
+            extra = nopt - npos - nneg
+            npos += extra // 2
+            nneg += extra - extra // 2
+    slice_yneg = slice(-nneg, None) if nneg else slice(0, 0)
+    return (slice(0, npos), slice_yneg), slice(0, ncol)
 
 
 def w_ell2fkernels(w_ell, ells, ly, lx, interp_kind, optimize_len=False):

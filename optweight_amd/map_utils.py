@@ -147,15 +147,21 @@ def get_minfo(mtype, nrings, nphi, theta_min=None, theta_max=None):
     elif mtype == 'F1':
         minfo = MapInfo.map_info_fejer1(nrings, nphi)
     else:
-        raise ValueError(f'mtype : {mtype} not supported')
-    if theta_min or theta_max:
-        theta_min = theta_min if theta_min else 0
-        theta_max = theta_max if theta_max else np.pi
-        mask = (minfo.theta >= theta_min) & (minfo.theta <= theta_max)
-        offsets = minfo.offsets[mask]
-        minfo = MapInfo(minfo.theta[mask], minfo.weight[mask], nphi=nphi, phi0=0,
-                        offsets=offsets - offsets[0], stride=minfo.stride[mask])
-    return minfo
+        raise ValueError(f"unknown grid {mtype!r}: use 'GL', 'CC' or 'F1'")
+    return _theta_cut(minfo, nphi, theta_min, theta_max)
+
+
+def _theta_cut(minfo, nphi, theta_min, theta_max):
+    '''The rings of a rectangular grid with theta_min <= theta <= theta_max
+    (an unset or zero bound leaves that side open), stored from offset 0.'''
+    if not (theta_min or theta_max):
+        return minfo
+    lo = theta_min or 0
+    hi = theta_max or np.pi
+    keep = np.flatnonzero((minfo.theta >= lo) & (minfo.theta <= hi))
+    return MapInfo(minfo.theta[keep], minfo.weight[keep], nphi=nphi, phi0=0,
+                   offsets=minfo.offsets[keep] - minfo.offsets[keep[0]],
+                   stride=minfo.stride[keep])
 
 
 def fmul_pix(imap, minfo, fmat2d=None, fmat1d=None, ells=None, modlmap=None, inplace=False):
@@ -176,82 +182,74 @@ def fmul_pix(imap, minfo, fmat2d=None, fmat1d=None, ells=None, modlmap=None, inp
 def get_gauss_minfo(lmax, theta_min=None, theta_max=None):
     '''GL grid that supports band limit lmax: floor(lmax/2)+1 rings of
     lmax+1 pixels.'''
-    nrings = int(np.floor(lmax / 2)) + 1
     nphi = lmax + 1
-    minfo = MapInfo.map_info_gauss_legendre(nrings, nphi)
-    if theta_min or theta_max:
-        theta_min = theta_min if theta_min else 0
-        theta_max = theta_max if theta_max else np.pi
-        mask = (minfo.theta >= theta_min) & (minfo.theta <= theta_max)
-        offsets = minfo.offsets[mask]
-        offsets = offsets - offsets[0]
-        minfo = MapInfo(minfo.theta[mask], nphi=nphi, phi0=0, offsets=offsets,
-                        stride=minfo.stride[mask], weight=minfo.weight[mask])
-    return minfo
+    return _theta_cut(MapInfo.map_info_gauss_legendre(lmax // 2 + 1, nphi), nphi, theta_min,
+                      theta_max)
 
 
 def get_mtype(minfo):
-    if minfo.theta.size < 3:
-        raise ValueError(f'Need atleast 3 rings, got {minfo.theta.size=}')
-    diff = np.diff(minfo.theta)
-    if not np.allclose(diff, diff[0]):
+    '''Quadrature rule of a geometry's rings: 'GL' when they are unequally
+    spaced; for equal spacing 'CC' when the last ring lies a whole number
+    of steps from the south pole, 'F1' when half a step more (the fraction
+    rounded to one decimal).'''
+    theta = np.asarray(minfo.theta)
+    if theta.size < 3:
+        raise ValueError(f'the rule of a geometry is found from 3 or more rings, got '
+                         f'{theta.size}')
+    step = np.diff(theta)
+    if not np.allclose(step, step[0]):
         return 'GL'
-    dtheta = diff[0]
-    remain = np.round(np.remainder((np.pi - minfo.theta.max()) / dtheta, 1), 1)
-    if remain == 0.0:
-        return 'CC'
-    if remain == 0.5:
-        return 'F1'
-    raise ValueError(f'Cannot determine mtype, {remain=}')
+    frac = float(np.round(np.remainder((np.pi - theta.max()) / step[0], 1), 1))
+    rule = {0.0: 'CC', 0.5: 'F1'}.get(frac)
+    if rule is None:
+        raise ValueError(f'equally spaced rings ending {frac} of a step short of a whole '
+                         f'number of steps from the pole: neither Clenshaw-Curtis nor Fejer-1')
+    return rule
 
 
 def get_fullsky_geometry(minfo):
-    '''Full-sky GL geometry sharing the rings of a (cut-sky) GL geometry.'''
-    nphi = minfo.nphi[0]
-    if not all(minfo.nphi == nphi):
-        raise ValueError('Geometries with varying nphi not supported')
-    mtype = get_mtype(minfo)
-    diff = np.diff(minfo.theta)
-    if mtype == 'GL':
-        nrings = int(np.round(np.min(np.pi / np.abs(diff) - 0.5)))
-        out = MapInfo.map_info_gauss_legendre(nrings, nphi)
-    elif mtype == 'CC':
-        out = MapInfo.map_info_clenshaw_curtis(int(np.round(np.pi / np.abs(diff[0]) + 1)), nphi)
-    elif mtype == 'F1':
-        out = MapInfo.map_info_fejer1(int(np.round(np.pi / np.abs(diff[0]))), nphi)
+    '''The full-sky grid (same rule, same nphi) of which a geometry's rings
+    are a consecutive run.'''
+    nphi = int(minfo.nphi[0])
+    if np.any(minfo.nphi != nphi):
+        raise ValueError('the rings differ in nphi: no rectangular full-sky grid holds them')
+    rule = get_mtype(minfo)
+    step = np.abs(np.diff(minfo.theta))
+    if rule == 'GL':
+        # GL rings lie about pi / (n + 1/2) apart
+        full = MapInfo.map_info_gauss_legendre(int(np.round(np.min(np.pi / step - 0.5))), nphi)
+    elif rule == 'CC':
+        full = MapInfo.map_info_clenshaw_curtis(int(np.round(np.pi / step[0] + 1)), nphi)
     else:
-        raise ValueError(f'{mtype=} not supported')
-    ts = np.sort(minfo.theta)
-    tf = np.sort(out.theta)
-    idx = np.argwhere(np.isclose(tf, ts[0]))
-    if idx.size != 1:
-        raise ValueError('theta_sort[0] of input not in fullsky output')
-    idx = idx[0][0]
-    if not np.allclose(tf[idx:idx + minfo.nrow], ts):
-        raise ValueError('theta input does not overlap with fullsky output')
-    return out
+        full = MapInfo.map_info_fejer1(int(np.round(np.pi / step[0])), nphi)
+    mine = np.sort(minfo.theta)
+    theirs = np.sort(full.theta)
+    first = np.flatnonzero(np.isclose(theirs, mine[0]))
+    run = theirs[first[0]:first[0] + mine.size] if first.size == 1 else None
+    if run is None or run.size != mine.size or not np.allclose(run, mine):
+        raise ValueError(f'the rings are not a consecutive run of the full-sky {rule} grid of '
+                         f'{full.nrow} rings')
+    return full
 
 
 def get_equal_area_minfo(minfo, orig_band=np.pi / 8, ratio_pow=1.):
-    '''Reduce nphi per ring to nphi * (w_ring / w_equator)^ratio_pow outside
-    the band |theta - pi/2| < orig_band (at least 2 pixels).'''
-    assert orig_band >= 0, f'orig_band must be >= 0, got {orig_band}'
-    delta_theta = np.min(np.abs(np.diff(minfo.theta)))
-    if not np.any(np.abs(minfo.theta - (np.pi / 2)) < 2 * delta_theta):
-        raise ValueError('Input minfo has to include equator.')
-    idx_ref = np.searchsorted(minfo.theta, np.pi / 2, side='left')
-    area_ref = minfo.weight[idx_ref]
-    nphi_r = np.zeros_like(minfo.nphi)
-    w_r = np.zeros_like(minfo.weight)
-    for t in range(minfo.nrow):
-        area = minfo.weight[t]
-        if (np.pi / 2 - orig_band) < minfo.theta[t] < (np.pi / 2 + orig_band):
-            ratio = 1
-        else:
-            ratio = (area / area_ref) ** ratio_pow
-        nphi_r[t] = max(2, int(np.round(minfo.nphi[t] * ratio)))
-        w_r[t] = area * minfo.nphi[t] / nphi_r[t]
-    return MapInfo(minfo.theta, nphi=nphi_r, weight=w_r)
+    '''The same rings with nphi scaled by (w_ring / w_equator)^ratio_pow
+    (rounded, at least 2; weights rescaled to keep each ring's area)
+    outside the band pi/2 - orig_band < theta < pi/2 + orig_band, whose rings
+    keep theirs. w_equator is the weight of the first ring at or south of
+    the equator; some ring must lie within two ring spacings of it.'''
+    if orig_band < 0:
+        raise ValueError(f'orig_band is a half width, >= 0; got {orig_band}')
+    theta = np.asarray(minfo.theta)
+    spacing = np.min(np.abs(np.diff(theta)))
+    if not np.any(np.abs(theta - np.pi / 2) < 2 * spacing):
+        raise ValueError('the geometry has no ring near the equator to take the pixel '
+                         'area from')
+    w_eq = minfo.weight[np.searchsorted(theta, np.pi / 2, side='left')]
+    in_band = ((np.pi / 2 - orig_band) < theta) & (theta < (np.pi / 2 + orig_band))
+    scale = np.where(in_band, 1.0, (minfo.weight / w_eq) ** ratio_pow)
+    nphi = np.maximum(2, np.round(minfo.nphi * scale).astype(minfo.nphi.dtype))
+    return MapInfo(theta, nphi=nphi, weight=minfo.weight * minfo.nphi / nphi)
 
 
 class CarWCS:
@@ -288,12 +286,13 @@ def match_enmap_minfo(shape, wcs, mtype='CC'):
     phi0 + 2 pi j / nx. Raises ValueError if the map does not span the full
     RA range or its rows are not rings of the CC / fejer1 grid.'''
     if mtype not in ('CC', 'fejer1'):
-        raise NotImplementedError('Only "CC" or "fejer1" `mtype` implemented for now.')
+        raise NotImplementedError(f"CAR maps come as 'CC' or 'fejer1', not {mtype!r}")
     yo = 1 if mtype == 'CC' else 0
     cdelt = wcs.wcs.cdelt
     nx = int(np.round(np.abs(360 / cdelt[0])))
     if nx != shape[-1]:
-        raise ValueError(f'Input needs to span full range in phi, {shape[-1]}, {nx} ')
+        raise ValueError(f'the map has {shape[-1]} columns; a full circle at this cdelt '
+                         f'takes {nx}')
     ny = int(np.round(abs(180 / cdelt[1]) + yo))
     phi0 = float(car_pix2sky(wcs, 0, 0)[1])
     if mtype == 'CC':
@@ -306,7 +305,8 @@ def match_enmap_minfo(shape, wcs, mtype='CC'):
     i0 = np.flatnonzero(np.abs(theta[0] - full.theta) < atol)
     i1 = np.flatnonzero(np.abs(theta[-1] - full.theta) < atol)
     if i0.size != 1 or i1.size != 1:
-        raise ValueError(f'Input geometry is not {mtype} at atol = {atol}')
+        raise ValueError(f'the first and last rows of the map are not rings of the {mtype} '
+                         f'grid of {ny} rings (tolerance {atol} rad)')
     keep = np.s_[min(i0[0], i1[0]):max(i0[0], i1[0]) + 1]
     theta_k = full.theta[keep]
     weight_k = full.weight[keep]
@@ -335,33 +335,31 @@ def unit_stride(minfo):
 
 def threshold_icov(icov_pix, q_low=0, q_high=1):
     '''Copy of icov_pix (npol, npix) or (npol, npol, npix) whose diagonal is
-    clipped per component to its [q_low, q_high] quantiles over the nonzero
-    pixels; a 3d matrix is rescaled by sqrt of the clipping factor on both
-    sides, keeping it symmetric.'''
+    clipped per component to its [q_low, q_high] quantiles over the pixels
+    where it is positive (q_low / q_high scalars or one per component); a
+    matrix is rescaled by the square root of each pixel's clipping factor on
+    both sides, so it stays symmetric.'''
     icov_pix = np.asarray(icov_pix)
-    ndim = icov_pix.ndim
-    if ndim not in (2, 3):
-        raise ValueError(f'{icov_pix.ndim=}, expected 2 or 3')
+    if icov_pix.ndim not in (2, 3):
+        raise ValueError(f'icov_pix of {icov_pix.ndim} dimensions: expected (npol, npix) or '
+                         f'(npol, npol, npix)')
     npol = icov_pix.shape[0]
-    q_low = q_low * np.ones(npol)
-    q_high = q_high * np.ones(npol)
-    scaling = np.ones((npol, icov_pix.shape[-1]))
-    for pidx in range(npol):
-        diag = icov_pix[pidx, pidx] if ndim == 3 else icov_pix[pidx]
-        nonzero = diag > 0
-        if np.sum(nonzero) == 0:
+    diags = np.einsum('iip->ip', icov_pix) if icov_pix.ndim == 3 else icov_pix
+    bounds = np.broadcast_to(np.stack(np.broadcast_arrays(q_low, q_high), -1), (npol, 2))
+    factor = np.ones(diags.shape)
+    for comp, (diag, qs) in enumerate(zip(diags, bounds)):
+        positive = diag > 0
+        if not positive.any():
             continue
-        val_low = np.quantile(diag[nonzero], q_low[pidx])
-        val_high = np.quantile(diag[nonzero], q_high[pidx])
-        low = (diag < val_low) & nonzero
-        high = (diag > val_high) & nonzero
-        scaling[pidx][low] = val_low / diag[low]
-        scaling[pidx][high] = val_high / diag[high]
-    if ndim == 2:
-        out = icov_pix * scaling
+        lo, hi = np.quantile(diag[positive], qs[0]), np.quantile(diag[positive], qs[1])
+        clip = np.clip(diag, lo, hi)
+        moved = positive & (clip != diag)
+        factor[comp, moved] = clip[moved] / diag[moved]
+    if icov_pix.ndim == 2:
+        out = icov_pix * factor
     else:
-        root = np.sqrt(scaling)
-        out = root[:, np.newaxis, :] * icov_pix * root[np.newaxis, :, :]
+        root = np.sqrt(factor)
+        out = root[:, None] * icov_pix * root[None, :]
     return out.astype(icov_pix.dtype, copy=False)
 
 

@@ -19,19 +19,48 @@ from scipy import linalg
 from optweight_amd import map_utils, operators, noise_utils, alm_c_utils, sht, alm_utils, map_c_utils
 
 
+def _unobserved(mask, npol=None):
+    '''(npol, npix) True where unobserved, from a bool mask of observed
+    pixels, (npix,) (shared by all components) or (npol, npix).'''
+    mask = np.asarray(mask)
+    if mask.dtype != bool:
+        raise ValueError(f'the mask needs dtype bool (True = observed), not {mask.dtype}')
+    mask = np.atleast_2d(mask)
+    if npol is not None:
+        if mask.shape[0] == 1:
+            mask = np.broadcast_to(mask, (npol, mask.shape[-1]))
+        elif mask.shape[0] != npol:
+            raise ValueError(f'mask has {mask.shape[0]} components, the spectra {npol}')
+    return ~mask
+
+
+def _spectra_matrix(icov_ell):
+    '''(npol, npol, nell) from per-component spectra (nell,) or
+    (npol, nell), or a matrix per l (npol, npol, nell).'''
+    icov_ell = np.asarray(icov_ell)
+    if icov_ell.ndim == 1:
+        icov_ell = icov_ell[np.newaxis]
+    if icov_ell.ndim == 2:
+        return icov_ell * np.eye(icov_ell.shape[0])[:, :, np.newaxis]
+    if icov_ell.ndim != 3 or icov_ell.shape[0] != icov_ell.shape[1]:
+        raise ValueError(f'spectra of shape {icov_ell.shape}: expected (nell,), (npol, nell) '
+                         f'or (npol, npol, nell)')
+    return icov_ell
+
+
 class Level:
+    '''One grid of the multigrid hierarchy: the unobserved pixels of its
+    equal-area geometry, the operator G = M_u Y d_l Y^T M_u there
+    (d_l: (npol, npol, lmax + 1)) and the smoother (Jacobi-like diagonal
+    preconditioner, or a dense pseudo-inverse of G on the coarsest level).'''
 
     def __init__(self, mask, minfo, d_ell, spin, dense_smoothing=False, device=None):
-        mask = np.asarray(mask)
-        assert mask.dtype == bool, f'mask must be bool, got {mask.dtype}'
-        if mask.ndim == 1:
-            mask = mask[np.newaxis, :]
-        self.mask_unobs = ~mask
+        self.mask_unobs = _unobserved(mask)
         self.minfo = minfo
         self.spin = spin
-        assert d_ell.shape == (self.npol, self.npol, self.lmax + 1), (
-            f'Invalid shape d_ell expected {(self.npol, self.npol, self.lmax + 1)} '
-            f'got {d_ell.shape}')
+        want = (self.npol, self.npol, self.lmax + 1)
+        if d_ell.shape != want:
+            raise ValueError(f'd_ell has shape {d_ell.shape}; this level needs {want}')
         self.d_ell = d_ell
         if device is None:
             device = torch.device('cuda', torch.cuda.current_device())
@@ -145,24 +174,9 @@ class Level:
 def get_levels(mask, minfo, icov_ell, spin, min_pix=1000, lmax_r_ell=6000, device=None):
     '''Levels halving lmax until a component has fewer than min_pix masked
     pixels; that level gets the dense smoother.'''
-    icov_ell = np.asarray(icov_ell)
-    if icov_ell.ndim == 1:
-        icov_ell = icov_ell[np.newaxis, :]
-    if icov_ell.ndim == 2:
-        npol, nell = icov_ell.shape
-        icov_ell = icov_ell * np.eye(npol)[:, :, np.newaxis]
-    elif icov_ell.ndim == 3:
-        npol, _, nell = icov_ell.shape
-    else:
-        raise ValueError(f'shape icov_ell : {icov_ell.shape} not supported')
-    mask = np.asarray(mask)
-    if mask.ndim == 1:
-        mask = np.ones((npol, mask.size), dtype=mask.dtype) * mask
-    else:
-        assert mask.shape[0] == npol, (
-            f'shape[0] mask does not match npol {mask.shape[0]} != {npol}')
-    if mask.dtype != bool:
-        raise ValueError(f'Input mask must be bool, got {mask.dtype}')
+    icov_ell = _spectra_matrix(icov_ell)
+    npol, nell = icov_ell.shape[0], icov_ell.shape[-1]
+    mask = ~_unobserved(mask, npol)
     lmax = map_utils.minfo2lmax(minfo)
     levels = []
     for idx in range(16):
@@ -170,13 +184,15 @@ def get_levels(mask, minfo, icov_ell, spin, min_pix=1000, lmax_r_ell=6000, devic
         mask_level, minfo_level = get_equal_area_mask_bool(
             mask, minfo, lmax=lmax_level,
             minfo_fullsky=map_utils.get_fullsky_geometry(minfo) if idx == 0 else None)
-        assert lmax_level <= nell - 1, (
-            f'lmax_level {lmax_level} exceeds lmax of icov_ell {nell - 1}')
+        if lmax_level >= nell:
+            raise ValueError(f'level {idx} has band limit {lmax_level}, beyond the spectra '
+                             f'(lmax {nell - 1})')
         nmasked = np.sum(~mask_level, axis=1)
+        if not np.all(nmasked):
+            raise ValueError(f'level {idx} (lmax {lmax_level}) has no unobserved pixels left '
+                             f'in some component (counts {nmasked.tolist()}); raise min_pix '
+                             f'above {min_pix} or widen the mask holes')
         last = bool(np.any(nmasked < min_pix))
-        if np.any(nmasked == 0):
-            raise ValueError(f'No masked pixels left for min_pix : {min_pix} '
-                             f'in one or more of the masks : {nmasked}')
         if idx == 0:
             r_ell = lowpass_filter(lmax_r_ell, lmax=lmax_level)
             d_ell = icov_ell[:, :, :lmax_level + 1] * r_ell ** 2

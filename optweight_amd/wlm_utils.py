@@ -100,53 +100,59 @@ def lmax_to_j_scale(lmax, lamb):
     return int(np.ceil(np.log10(lmax) / np.log10(lamb)))
 
 
-def get_sd_kernels(lamb, lmax, j0=None, lmin=None, jmax=None, lmax_j=None, return_j=False):
-    '''Scale-discrete kernels w_ell (nwav, lmax + 1) and their band limits
-    (wlm_utils.py:8-109): phi for the first scale, psi in between, omega
-    (one above lambda^J) for the last scale J.'''
+def _scale_range(lamb, lmax, j0, lmin, jmax, lmax_j):
+    '''(first, last) wavelet scale of a kernel set. The last scale is the
+    one reaching lmax, or lower if jmax (or lmax_j, the band limit that
+    sets it) asks; the first is j0, or the scale whose phi ends near lmin,
+    else 0. Only nonzero j0 / lmin (jmax / lmax_j) count as both given.'''
     if j0 and lmin:
-        raise ValueError('Cannot have both j0 and lmin.')
+        raise ValueError(f'give j0 ({j0}) or lmin ({lmin}), not both')
     if jmax and lmax_j:
-        raise ValueError('Cannot have both jmax and lmax_j.')
-    jmax_lmax = lmax_to_j_scale(lmax, lamb)
+        raise ValueError(f'give jmax ({jmax}) or lmax_j ({lmax_j}), not both')
+    top = lmax_to_j_scale(lmax, lamb)
     if lmax_j:
         jmax = lmax_to_j_scale(lmax_j, lamb)
-    jmax = jmax_lmax if jmax is None else min(jmax, jmax_lmax)
-    if j0 is None:
-        if lmin is None:
-            j0 = 0
-        else:
-            if lmin < 0:
-                raise ValueError(f'lmin : {lmin} cannot be negative')
-            j0 = max(0, lmax_to_j_scale(lmin, lamb) - 1)
-    if j0 < 0:
-        raise ValueError(f'j0 : {j0} cannot be negative.')
-    if j0 > jmax:
-        raise ValueError('j0 exceeds jmax, pick lower j0, lower lmin or higher lmax.')
-    js = np.arange(j0, jmax + 1)
+    last = top if jmax is None else min(jmax, top)
+    if j0 is not None:
+        first = j0
+    elif lmin is not None:
+        if lmin < 0:
+            raise ValueError(f'lmin must be >= 0, got {lmin}')
+        first = max(0, lmax_to_j_scale(lmin, lamb) - 1)
+    else:
+        first = 0
+    if first < 0:
+        raise ValueError(f'j0 must be >= 0, got {first}')
+    if first > last:
+        raise ValueError(f'first scale {first} lies above the last scale {last} that lmax '
+                         f'{lmax} (and jmax / lmax_j) allow: lower j0 or lmin, or raise lmax')
+    return first, last
+
+
+def get_sd_kernels(lamb, lmax, j0=None, lmin=None, jmax=None, lmax_j=None, return_j=False):
+    '''Scale-discrete kernels w_ell (nwav, lmax + 1) with their band limits
+    (and scales, with return_j) for the scales j0..J (wlm_utils.py:8-109):
+    the first kernel is phi at scale j0 + 1 (1 from l = 0), the last (for
+    two or more scales) omega at J (1 from lambda^(J+1) on), psi_j between.
+    Band limits: ceil(lambda^(j+1)) capped at lmax, lmax for the last.'''
+    first, last = _scale_range(lamb, lmax, j0, lmin, jmax, lmax_j)
+    js = np.arange(first, last + 1)
     ells = np.arange(lmax + 1)
-    w_ell = np.zeros((js.size, lmax + 1))
-    lmaxs = np.zeros(js.size, dtype=int)
-    for idx, j in enumerate(js):
-        if idx == 0:
-            w_ell[idx] = phi_ell(ells, lamb, j + 1)
-        elif j == jmax:
-            w_ell[idx] = omega_ell(ells, lamb, j)
-        else:
-            w_ell[idx] = psi_ell(ells, lamb, j)
-        lmaxs[idx] = lmax if j == jmax else min(lmax, j_scale_to_lmax(j, lamb))
-    if return_j:
-        return w_ell, lmaxs, js
-    return w_ell, lmaxs
+    kernels = [phi_ell(ells, lamb, first + 1)]
+    kernels += [psi_ell(ells, lamb, j) for j in js[1:-1]]
+    if js.size > 1:
+        kernels.append(omega_ell(ells, lamb, last))
+    w_ell = np.stack(kernels)
+    lmaxs = np.array([min(lmax, j_scale_to_lmax(j, lamb)) for j in js], dtype=int)
+    lmaxs[-1] = lmax
+    return (w_ell, lmaxs, js) if return_j else (w_ell, lmaxs)
 
 
 def get_lmax_array(w_ell, eps=1e-10):
-    '''One past the last multipole where |w_ell| > eps, capped at lmax (0 for
-    an all-zero kernel), as wlm_utils.py:111-142.'''
-    w_ell = np.atleast_2d(w_ell)
-    lmax = w_ell.shape[-1] - 1
-    mask = np.abs(w_ell) > eps
-    lmaxs = lmax - np.argmax(mask[:, ::-1], axis=-1) + 1
-    lmaxs[lmaxs > lmax] = lmax
-    lmaxs[~mask.any(axis=-1)] = 0
-    return lmaxs
+    '''Band limit of each kernel (wlm_utils.py:111-142): one past the last
+    multipole where |w_ell| > eps, at most lmax, 0 for a kernel that never
+    exceeds eps.'''
+    above = np.atleast_2d(np.abs(w_ell)) > eps
+    nell = above.shape[-1]
+    last = nell - 1 - np.argmax(above[:, ::-1], axis=-1)
+    return np.where(above.any(axis=-1), np.minimum(last + 1, nell - 1), 0)

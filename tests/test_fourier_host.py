@@ -118,12 +118,10 @@ def test_find_kernel_slice_errors():
     for bad in (np.zeros((2, 2)), np.ones((1, 2, 2)), np.ones((1, 2, 2)) * 1e-7):
         with pytest.raises(ValueError):
             fkernel.find_kernel_slice(bad)
-    assert fkernel._find_last_nonzero_idx(np.asarray([0, 1, 0])) == 1
-    assert fkernel._find_last_nonzero_idx(np.asarray([0, 1, 1])) == 2
-    assert fkernel._find_last_nonzero_idx(np.asarray([1, 0, 0])) == 0
-    for bad in (np.zeros(3), np.ones((1, 2))):
-        with pytest.raises(ValueError):
-            fkernel._find_last_nonzero_idx(bad)
+    assert fkernel._prefix_len(np.asarray([0, 1, 0])) == 2
+    assert fkernel._prefix_len(np.asarray([0, 1, 1])) == 3
+    assert fkernel._prefix_len(np.asarray([1, 0, 0])) == 1
+    assert fkernel._prefix_len(np.zeros(3)) == 0
 
 
 def test_fkernel_and_set():
