@@ -24,7 +24,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from optweight_amd import (solvers, preconditioners, workloads, alm_utils, sht,  # noqa: E402
-                           mixed, _lib)
+                           mixed, _lib, autotune)
 
 PEAK_FP64_TFLOPS = 78.6
 PEAK_FP32_TFLOPS = 157.3
@@ -634,12 +634,15 @@ def fwav_bench(args, dev):
 
 
 def ragged_bench(args, dev):
-    '''Constrained realisations: per band limit one batched mixed-precision
-    solve of `batch` right-hand sides (data + S^-1/2 w + P^T N^-1/2 w', seeds
-    [k, i]), pinv + multigrid(T) preconditioning, FP64-verified residual.'''
+    '''Constrained realisations of uneven band limit: `batch` right-hand sides
+    at each band limit (data + S^-1/2 w + P^T N^-1/2 w', seeds [k, i]), pinv +
+    multigrid(T) preconditioning, mixed precision, FP64-verified residual.
+    Timed: the lockstep solve (all band limits in every CG phase, one stream
+    each, phases replayed from HIP graphs) and, for comparison, one batched
+    solve per band limit in turn; both give the same iterates.'''
     lmaxs = [int(v) for v in args.ragged_lmaxs.split(',')]
     t0 = time.perf_counter()
-    work, solvers_ = [], []
+    work, solvers_, tuned = [], [], {}
     for k, (prob, data) in enumerate(workloads.ragged(lmaxs, per=args.batch, seed=0,
                                                       device=dev)):
         sol = solvers.CGWienerMap.from_arrays(data, prob.minfo, prob.ainfo, prob.icov_ell,
@@ -651,20 +654,42 @@ def ragged_bench(args, dev):
             sol.add_preconditioner(preconditioners.MaskedPreconditioner(
                 prob.ainfo, prob.icov_ell[0:1, 0:1], 0, prob.mask_bool[0], prob.minfo,
                 min_pix=1000, n_jacobi=1, lmax_r_ell=6000, device=dev), sel=np.s_[0])
+        if args.autotune == 'on':
+            res = autotune.tune(prob.minfo, prob.lmax, torch.float32, args.batch, device=dev)
+            tuned[prob.lmax] = res['params']
         solvers_.append(sol)
         work.append((sol.a_matrix, sol.b0, sol.preconditioner))
         log(f'bucket lmax {prob.lmax}: setup done ({time.perf_counter() - t0:.1f}s)')
-    for w in range(args.warmup):
-        mixed.solve_ragged(work, delta=args.delta)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    for _ in range(args.steps):
-        res = mixed.solve_ragged(work, delta=args.delta)
-    torch.cuda.synchronize()
-    el = time.perf_counter() - t1
-    worst = 0.0
-    for sol, (x, info) in zip(solvers_, res):
+
+    def timed(lockstep):
+        # the lockstep solver records its HIP graphs in the first (warm-up)
+        # solve and replays them in the timed ones
+        lk = mixed.Lockstep(work, delta=args.delta) if lockstep else None
+
+        def run():
+            if lk is not None:
+                return lk.solve()
+            return mixed.solve_ragged(work, delta=args.delta, lockstep=False, graphs=False)
+        for _ in range(max(1, args.warmup)):
+            run()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            res = run()
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t1
+        log(f'{"lockstep" if lockstep else "per band limit"}: {1e3 * el / args.steps:.1f} ms '
+            f'per step')
+        return el, res
+
+    el_seq, res_seq = timed(False)
+    el, res = timed(True)
+    worst, same = 0.0, 0.0
+    for sol, (x, info), (xs, _) in zip(solvers_, res, res_seq):
+        if not info.graphs:
+            raise RuntimeError('the lockstep solve did not replay HIP graphs')
         worst = max(worst, float(np.max(true_residual(sol, x))))
+        same = max(same, float(torch.linalg.vector_norm(x - xs) / torch.linalg.vector_norm(xs)))
         log(f'lmax {sol.sht.ainfo.lmax}: iterations {info.iterations}, '
             f'replacements {info.replacements}')
     n = args.steps * args.batch * len(lmaxs)
@@ -674,9 +699,15 @@ def ragged_bench(args, dev):
            'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
            'dtype': 'f32+f64', 'data': 'synthetic',
            'config': {'workload': f'{args.batch} right-hand sides at each lmax {lmaxs}, '
-                                  'pinv + multigrid(T), mixed precision, one batched solve '
-                                  'per band limit',
-                      'true_residual_max': worst}}
+                                  'pinv + multigrid(T), mixed precision, lockstep CG over '
+                                  'the band limits (one stream each, phases replayed from '
+                                  'HIP graphs recorded in the warm-up solve)',
+                      'true_residual_max': worst,
+                      'per_band_limit_solves_per_s': n / el_seq,
+                      'speedup_lockstep_vs_per_band_limit': el_seq / el,
+                      'rel_l2_lockstep_vs_per_band_limit_max': same,
+                      'iterations': [info.iterations for _, info in res],
+                      'autotune': tuned}}
     print(json.dumps(out))
 
 
@@ -788,7 +819,6 @@ def main():
     log(f'setup {time.perf_counter() - t0:.1f}s')
     tuned = {}
     if args.autotune == 'on':
-        from optweight_amd import autotune
         for dt, nb in ((torch.float32, args.batch), (torch.float64, 1)):
             res = autotune.tune(prob.minfo, prob.lmax, dt, nb, device=dev)
             tuned[f'{str(dt)[6:]}_batch{nb}'] = {k: res[k] for k in ('params', 'cached',

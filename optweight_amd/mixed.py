@@ -47,8 +47,11 @@ import torch
 from optweight_amd import alm_utils, _lib
 
 
-def _convert(src, dtype):
-    out = torch.empty(src.shape, dtype=dtype, device=src.device)
+def _convert(src, dtype, out=None):
+    if out is None:
+        out = torch.empty(src.shape, dtype=dtype, device=src.device)
+    elif tuple(out.shape) != tuple(src.shape) or out.dtype != dtype or not out.is_contiguous():
+        raise ValueError('conversion output must be contiguous, of the input shape and dtype')
     _lib.call('owmi_convert', _lib.dtype_code(src), _lib.ptr(src), _lib.dtype_code(out),
               _lib.ptr(out), src.numel(), _lib.stream_ptr(src.device))
     return out
@@ -58,13 +61,15 @@ class SolveInfo:
     '''iterations: CG iterations of the batch; iterations_rhs: per
     right-hand side; replacements: float64 residual replacements;
     residual: final float64 true relative residual per right-hand side
-    (None when not computed).'''
+    (None when not computed); graphs: whether the iterations were replayed
+    from recorded HIP graphs.'''
 
-    def __init__(self, iterations, iterations_rhs, replacements, residual):
+    def __init__(self, iterations, iterations_rhs, replacements, residual, graphs=False):
         self.iterations = iterations
         self.iterations_rhs = iterations_rhs
         self.replacements = replacements
         self.residual = residual
+        self.graphs = graphs
 
     def as_dict(self):
         return {'iterations': self.iterations,
@@ -97,23 +102,195 @@ def _identity(v):
     return v.clone()
 
 
-class _Phases:
-    '''Device phases of a CG iteration: the first call of each runs eagerly,
-    later calls replay a HIP graph recorded from it (use_graph), else they
-    run eagerly every time.'''
+def _dot(a, b, out=None):
+    return alm_utils.contract_almxblm_batch(a, b, out=out)
 
-    def __init__(self, fns, use_graph):
-        self.fns = fns
+
+class _State:
+    '''One batch of right-hand sides of a batched CG solve. An iteration is
+    phase 0 on the device (q = A p, <p, q>, x += alpha p, r -= alpha q, and
+    <r, r> into ``rr``), the host's look at <r, r> (``check``: convergence,
+    residual replacement, freezing), then phase 1 on the device (z = M r,
+    <r, z>, p = z + beta p). Phases only update the state's buffers in
+    place, so they can be recorded into HIP graphs and replayed, also in
+    later solves: ``reset`` starts a new solve in the same buffers.'''
+
+    def __init__(self, b3, rr, tol, low):
+        self.nb = b3.shape[0]
+        self.dev = b3.device
+        self.tol = tol
+        self.rr = rr
+        self.pq = torch.empty((self.nb,), dtype=torch.float64, device=self.dev)
+        self.rzn = torch.empty_like(self.pq)
+        self.rz = torch.empty_like(self.pq)
+        self.x = torch.zeros_like(b3)
+        self.r = torch.empty(b3.shape, dtype=low, device=self.dev)
+        self.p = torch.empty_like(self.r)
+        self.shape = tuple(b3.shape)
+
+    def reset(self, b3):
+        '''A new solve of b3 (the shape of the first): x = 0, r = b3,
+        p = z = M r, rz = <r, z>.'''
+        if tuple(b3.shape) != self.shape or b3.dtype != self.x.dtype:
+            raise ValueError(f'right-hand sides {tuple(b3.shape)} {b3.dtype}: this solve '
+                             f'was set up for {self.shape} {self.x.dtype}')
+        self.b3 = b3
+        self.bn2 = _dot(b3, b3).cpu().numpy()
+        self.target = self.tol * self.tol * self.bn2
+        self.active = np.ones(self.nb, dtype=bool)
+        self.it = 0
+        self.it_rhs = np.zeros(self.nb, dtype=np.int64)
+        self.x.zero_()
+        if self.r.dtype == b3.dtype:
+            self.r.copy_(b3)
+        else:
+            _convert(b3, self.r.dtype, out=self.r)
+        self.p.copy_(self.m_op(self.r))
+        _dot(self.r, self.p, out=self.rz)
+
+    def phase(self, k):
+        if k == 0:
+            q = self.a_op(self.p)
+            _dot(self.p, q, out=self.pq)
+            _xr(self.x, self.r, self.p, q, self.rz, self.pq, self.nb)
+            _dot(self.r, self.r, out=self.rr)
+        else:
+            z = self.m_op(self.r)
+            _dot(self.r, z, out=self.rzn)
+            _pupd(self.p, z, self.rzn, self.rz, self.nb)
+            self.rz.copy_(self.rzn)
+
+    def _count(self, rr_h):
+        self.it += 1
+        self.it_rhs[self.active] += 1
+        bad = np.flatnonzero(self.active & ~np.isfinite(rr_h))
+        if bad.size:
+            raise FloatingPointError(f'CG: non-finite residual norm of right-hand sides '
+                                     f'{bad.tolist()} at iteration {self.it}')
+
+    def _freeze(self, idx):
+        '''Converged right-hand sides idx: residual and search direction
+        cleared, so every later coefficient of theirs is 0.'''
+        done = torch.as_tensor(idx, device=self.dev)
+        self.r[done] = 0
+        self.p[done] = 0
+
+
+class _Fp64State(_State):
+    '''Float64 CG; a right-hand side stops on its recursive residual
+    ||r|| <= tol ||b||.'''
+
+    def __init__(self, a_matrix, b3, precond, tol, rr):
+        super().__init__(b3, rr, tol, b3.dtype)
+        self.a_op = a_matrix
+        self.m_op = precond
+
+    def check(self, rr_h):
+        self._count(rr_h)
+        newly = self.active & (rr_h <= self.target)
+        if newly.any():
+            self.active &= ~newly
+            if self.active.any():
+                self._freeze(np.flatnonzero(newly))
+        return bool(self.active.any())
+
+    def info(self, verify=False):
+        res = None
+        if verify:
+            r64 = self.b3 - self.a_op(self.x)
+            res = np.sqrt(_dot(r64, r64).cpu().numpy() / self.bn2)
+        return SolveInfo(self.it, self.it_rhs.copy(), 0, res)
+
+
+class _MixedState(_State):
+    '''Float32 (``low``) operators and Krylov vectors, float64 solution;
+    a right-hand side whose recursive residual dropped by delta since its
+    last replacement (or reached the target) gets its float64 true residual
+    b - A64 x, which replaces r and decides convergence.'''
+
+    def __init__(self, a_matrix, b3, precond, tol, delta, low, a_lo, m_lo, rr):
+        super().__init__(b3, rr, tol, low)
+        self.a64 = a_matrix
+        self.a_op = a_lo
+        self.m_op = m_lo
+        self.low = low
+        self.delta2 = delta * delta
+
+    def reset(self, b3):
+        super().reset(b3)
+        self.ref2 = self.bn2.copy()
+        self.nrep = 0
+        self.res = np.full(self.nb, np.nan)
+
+    def check(self, rr_h):
+        self._count(rr_h)
+        trig = self.active & ((rr_h <= self.delta2 * self.ref2) | (rr_h <= self.target))
+        if not trig.any():
+            return True
+        idx = np.flatnonzero(trig)
+        sel = torch.as_tensor(idx, device=self.dev)
+        r64 = self.b3[sel] - self.a64(self.x[sel].contiguous())
+        rr64 = _dot(r64, r64).cpu().numpy()
+        self.nrep += 1
+        self.res[idx] = np.sqrt(rr64 / self.bn2[idx])
+        conv = rr64 <= self.target[idx]
+        self.active[idx[conv]] = False
+        if not self.active.any():
+            return False
+        self.r[sel] = _convert(r64, self.low)
+        self.ref2[idx] = rr64
+        if conv.any():
+            self._freeze(idx[conv])
+        return True
+
+    def info(self):
+        return SolveInfo(self.it, self.it_rhs.copy(), self.nrep, self.res.copy())
+
+
+class _Runner:
+    '''Runs phase k of the running states. One state runs on the current
+    stream; several run as a fork-join, each on a stream of its own (default
+    priority) that waits for the current stream, which then waits for all
+    of them, so their kernels overlap. With use_graph the first run of each
+    (phase, running set) is eager (it sizes every workspace and cache on the
+    streams it uses), the second records it, fork-join included, into one
+    HIP graph (one memory pool shared by all graphs of the runner: the
+    phases never run concurrently and free their temporaries), and later
+    runs replay that graph. A recording failure raises.'''
+
+    def __init__(self, states, use_graph):
+        self.states = states
         self.use_graph = use_graph
-        self.graphs = [None] * len(fns)
-        self.calls = [0] * len(fns)
+        dev = states[0].dev
+        self.streams = ([torch.cuda.Stream(device=dev) for _ in states] if len(states) > 1
+                        else None)
+        self.calls = {}
+        self.graphs = {}
+        self.pool = None
 
-    def run(self, k):
-        self.calls[k] += 1
-        if not self.use_graph or self.calls[k] == 1:
-            self.fns[k]()
+    def _eager(self, k, running):
+        if self.streams is None:
+            for i in running:
+                self.states[i].phase(k)
             return
-        if self.graphs[k] is None:
+        main = torch.cuda.current_stream()
+        for i in running:
+            side = self.streams[i]
+            side.wait_stream(main)
+            with torch.cuda.stream(side):
+                self.states[i].phase(k)
+        for i in running:
+            main.wait_stream(self.streams[i])
+
+    def run(self, k, running):
+        key = (k, tuple(running))
+        self.calls[key] = self.calls.get(key, 0) + 1
+        if not self.use_graph or self.calls[key] == 1:
+            self._eager(k, running)
+            return
+        if key not in self.graphs:
+            if self.pool is None:
+                self.pool = torch.cuda.graph_pool_handle()
             g = torch.cuda.CUDAGraph()
             # no garbage collection inside the recording: a finalizer that
             # frees device memory (an SHT plan of an unreachable geometry)
@@ -122,8 +299,8 @@ class _Phases:
             gc_was_enabled = gc.isenabled()
             gc.disable()
             try:
-                with torch.cuda.graph(g):
-                    self.fns[k]()
+                with torch.cuda.graph(g, pool=self.pool):
+                    self._eager(k, running)
             except RuntimeError as err:
                 # an operator that cannot be recorded (a host copy or sync
                 # inside it): graphs were asked for, so this is an error, not
@@ -133,8 +310,79 @@ class _Phases:
             finally:
                 if gc_was_enabled:
                     gc.enable()
-            self.graphs[k] = g
-        self.graphs[k].replay()
+            self.graphs[key] = g
+        self.graphs[key].replay()
+
+    @property
+    def replayed(self):
+        return bool(self.graphs)
+
+
+class Lockstep:
+    '''Batched CG solves of several buckets of right-hand sides that iterate
+    together (the buckets differ in band limit and operators). Every phase
+    of an iteration runs the running buckets at once, each on a stream of
+    its own, then the host reads all their <r, r> with one copy, each bucket
+    checks convergence, and a bucket leaves once all its right-hand sides
+    have converged. Each bucket follows the recurrence and stopping rule of
+    its own batched solve (solve_fp64 / solve_mixed), so the results equal
+    one solve per bucket.
+
+    buckets: [(a_matrix, b, precond)], b (nbatch, ncomp, nelem) complex128
+    (or (ncomp, nelem)) fixing each bucket's shape; mixed precision
+    (complex64 operators) unless fp64. The buffers and the HIP graphs of
+    the phases (graphs=True: recorded at the second run of each phase and
+    running set) persist: ``solve`` of new right-hand sides of the same
+    shapes replays them.'''
+
+    def __init__(self, buckets, tol=1e-6, delta=0.01, fp64=False, graphs=True, maxiter=1000):
+        if not buckets:
+            raise ValueError('no buckets')
+        b3s = [_as_batch(b)[0] for _, b, _ in buckets]
+        self.squeeze = [b.ndim == 2 for _, b, _ in buckets]
+        self.fp64 = fp64
+        self.maxiter = maxiter
+        self.rr_all = torch.empty((sum(b.shape[0] for b in b3s),), dtype=torch.float64,
+                                  device=b3s[0].device)
+        self.offsets = np.cumsum([0] + [b.shape[0] for b in b3s])
+        self.states = []
+        for k, ((a_matrix, _, precond), b3) in enumerate(zip(buckets, b3s)):
+            rr = self.rr_all[self.offsets[k]:self.offsets[k + 1]]
+            precond = precond or _identity
+            if fp64:
+                self.states.append(_Fp64State(a_matrix, b3, precond, tol, rr))
+            else:
+                self.states.append(_MixedState(a_matrix, b3, precond, tol, delta,
+                                               torch.complex64, a_matrix, precond, rr))
+        self.runner = _Runner(self.states, graphs)
+        self._first = [b for _, b, _ in buckets]
+
+    def solve(self, bs=None, verify=True):
+        '''Solve the buckets' right-hand sides bs (default: those given at
+        construction). Returns [(x, SolveInfo)] in bucket order; x is a view
+        of the solver's buffer, overwritten by the next solve. verify: float64
+        true residuals of a float64 solve.'''
+        bs = self._first if bs is None else bs
+        if len(bs) != len(self.states):
+            raise ValueError(f'{len(bs)} right-hand side batches for {len(self.states)} buckets')
+        for st, b in zip(self.states, bs):
+            st.reset(_as_batch(b)[0])
+        running = [i for i, st in enumerate(self.states) if st.active.any()]
+        it = 0
+        while it < self.maxiter and running:
+            self.runner.run(0, running)
+            it += 1
+            rr_h = self.rr_all.cpu().numpy()
+            running = [i for i in running
+                       if self.states[i].check(rr_h[self.offsets[i]:self.offsets[i + 1]])]
+            if running:
+                self.runner.run(1, running)
+        out = []
+        for st, squeeze in zip(self.states, self.squeeze):
+            info = st.info(verify) if self.fp64 else st.info()
+            info.graphs = self.runner.replayed
+            out.append((st.x[0] if squeeze else st.x, info))
+        return out
 
 
 def _graphs_default(t):
@@ -151,77 +399,12 @@ def solve_mixed(a_matrix, b, precond=None, tol=1e-6, delta=0.1, maxiter=1000,
     environment). Returns (x complex128, SolveInfo).'''
     b3, squeeze = _as_batch(b)
     precond = precond or _identity
-    a_lo = a_matrix_low or a_matrix
-    m_lo = precond_low or precond
     if graphs is None:
         graphs = _graphs_default(b3)
-    dot = alm_utils.contract_almxblm_batch
-    nb = b3.shape[0]
-    dev = b3.device
-    x = torch.zeros_like(b3)
-    bn2 = dot(b3, b3).cpu().numpy()
-    target = tol * tol * bn2
-    r = _convert(b3, low)
-    p = m_lo(r)
-    if p.data_ptr() == r.data_ptr():
-        # a preconditioner that returns its input: p is updated in place
-        p = p.clone()
-    rz = dot(r, p)
-    # CG scalars on the device, updated in place
-    pq = torch.empty((nb,), dtype=torch.float64, device=dev)
-    rr = torch.empty((nb,), dtype=torch.float64, device=dev)
-    rzn = torch.empty((nb,), dtype=torch.float64, device=dev)
-
-    def step_a():
-        # q = A32 p; x += alpha p; r -= alpha q; <r, r>
-        q = a_lo(p)
-        dot(p, q, out=pq)
-        _xr(x, r, p, q, rz, pq, nb)
-        dot(r, r, out=rr)
-
-    def step_m():
-        # z = M32 r; p = z + beta p
-        z = m_lo(r)
-        dot(r, z, out=rzn)
-        _pupd(p, z, rzn, rz, nb)
-        rz.copy_(rzn)
-
-    phases = _Phases((step_a, step_m), graphs)
-    ref2 = bn2.copy()
-    active = np.ones(nb, dtype=bool)
-    it_rhs = np.zeros(nb, dtype=np.int64)
-    res = np.full(nb, np.nan)
-    it = nrep = 0
-    while it < maxiter and active.any():
-        phases.run(0)
-        it += 1
-        it_rhs[active] += 1
-        rr_h = rr.cpu().numpy()
-        trig = active & ((rr_h <= delta * delta * ref2) | (rr_h <= target))
-        if trig.any():
-            # float64 true residual of the triggered right-hand sides only
-            idx = np.nonzero(trig)[0]
-            it_t = torch.as_tensor(idx, device=dev)
-            r64 = b3[it_t] - a_matrix(x[it_t].contiguous())
-            rr64 = dot(r64, r64).cpu().numpy()
-            nrep += 1
-            res[idx] = np.sqrt(rr64 / bn2[idx])
-            conv = rr64 <= target[idx]
-            active[idx[conv]] = False
-            if not active.any():
-                break
-            r[it_t] = _convert(r64, low)
-            del r64
-            ref2[idx] = rr64
-            if conv.any():
-                done = torch.as_tensor(idx[conv], device=dev)
-                r[done] = 0
-                p[done] = 0
-        phases.run(1)
-    info = SolveInfo(it, it_rhs, nrep, res)
-    # whether the iterations were replayed from recorded graphs
-    info.graphs = phases.use_graph and any(g is not None for g in phases.graphs)
-    return (x[0] if squeeze else x), info
+    rr = torch.empty((b3.shape[0],), dtype=torch.float64, device=b3.device)
+    st = _MixedState(a_matrix, b3, precond, tol, delta, low, a_matrix_low or a_matrix,
+                     precond_low or precond, rr)
+    return _solve_one(st, b3, squeeze, graphs, maxiter, st.info)
 
 
 def solve_fp64(a_matrix, b, precond=None, tol=1e-6, maxiter=1000, verify=False):
@@ -230,59 +413,46 @@ def solve_fp64(a_matrix, b, precond=None, tol=1e-6, maxiter=1000, verify=False):
     ||r|| <= tol ||b||. With verify=True the float64 true residual of every
     right-hand side is computed at the end. Returns (x, SolveInfo).'''
     b3, squeeze = _as_batch(b)
-    precond = precond or _identity
-    dot = alm_utils.contract_almxblm_batch
-    nb = b3.shape[0]
-    x = torch.zeros_like(b3)
-    bn2 = dot(b3, b3).cpu().numpy()
-    target = tol * tol * bn2
-    r = b3.clone()
-    z = precond(r)
-    rz = dot(r, z)
-    # p is updated in place: never as the same array as r
-    p = z.clone() if z.data_ptr() == r.data_ptr() else z
-    rr = torch.empty((nb,), dtype=torch.float64, device=b3.device)
-    active = np.ones(nb, dtype=bool)
-    it_rhs = np.zeros(nb, dtype=np.int64)
+    rr = torch.empty((b3.shape[0],), dtype=torch.float64, device=b3.device)
+    st = _Fp64State(a_matrix, b3, precond or _identity, tol, rr)
+    return _solve_one(st, b3, squeeze, False, maxiter, lambda: st.info(verify))
+
+
+def _solve_one(st, b3, squeeze, graphs, maxiter, info_fn):
+    '''One state on the current stream: iterate, then (x, SolveInfo).'''
+    st.reset(b3)
+    runner = _Runner([st], graphs)
     it = 0
-    while it < maxiter and active.any():
-        q = a_matrix(p)
-        pq = dot(p, q)
-        _xr(x, r, p, q, rz, pq, nb)
-        del q
+    while it < maxiter:
+        runner.run(0, [0])
         it += 1
-        it_rhs[active] += 1
-        rr_h = dot(r, r, out=rr).cpu().numpy()
-        newly = active & (rr_h <= target)
-        if newly.any():
-            active &= ~newly
-            if not active.any():
-                break
-            done = torch.as_tensor(~active, device=b3.device)
-            r[done] = 0
-            p[done] = 0
-        z = precond(r)
-        rz_new = dot(r, z)
-        _pupd(p, z, rz_new, rz, nb)
-        del z
-        rz = rz_new
-    res = None
-    if verify:
-        r64 = b3 - a_matrix(x)
-        res = np.sqrt(dot(r64, r64).cpu().numpy() / bn2)
-    info = SolveInfo(it, it_rhs, 0, res)
-    return (x[0] if squeeze else x), info
+        if not st.check(st.rr.cpu().numpy()):
+            break
+        runner.run(1, [0])
+    info = info_fn()
+    info.graphs = runner.replayed
+    return (st.x[0] if squeeze else st.x), info
 
 
-def solve_ragged(buckets, tol=1e-6, delta=0.01, fp64=False):
-    """Right-hand sides of uneven band limit: one batched solve per bucket
-    (the operators differ per band limit). buckets: list of
+def solve_ragged(buckets, tol=1e-6, delta=0.01, fp64=False, lockstep=True, graphs=None,
+                 maxiter=1000):
+    """Right-hand sides of uneven band limit. buckets: list of
     (a_matrix, b, precond) with b (nbatch, ncomp, nelem) at the bucket's
-    lmax. Mixed precision unless fp64. Returns [(x, SolveInfo)]."""
+    lmax; mixed precision unless fp64. Returns [(x, SolveInfo)] in bucket
+    order. lockstep (default): one Lockstep solve of all buckets (graphs
+    default True there; for repeated solves keep a Lockstep object, which
+    records its graphs once); else one solve per bucket in turn (graphs as
+    solve_mixed). Both give the same iterates."""
+    if not buckets:
+        return []
+    if lockstep:
+        return Lockstep(buckets, tol=tol, delta=delta, fp64=fp64,
+                        graphs=True if graphs is None else graphs, maxiter=maxiter).solve()
     out = []
     for a_matrix, b, precond in buckets:
         if fp64:
-            out.append(solve_fp64(a_matrix, b, precond, tol=tol, verify=True))
+            out.append(solve_fp64(a_matrix, b, precond, tol=tol, verify=True, maxiter=maxiter))
         else:
-            out.append(solve_mixed(a_matrix, b, precond, tol=tol, delta=delta))
+            out.append(solve_mixed(a_matrix, b, precond, tol=tol, delta=delta, graphs=graphs,
+                                   maxiter=maxiter))
     return out
