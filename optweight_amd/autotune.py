@@ -13,7 +13,10 @@ keeps the fastest value of each:
   float32, batch >= 8:   matrix cores or vector kernels (OWMI_TUNE_MFMA_MIN),
                          then column tiles of the matrix-core adjoint per
                          spin (OWMI_TUNE_ADJ_CT0 / CT2: 2 or 4);
-  float64, or smaller:   transforms per thread of the vector synthesis and
+  float64, batch >= 4:   float64 matrix cores or vector kernels for the
+                         adjoint (OWMI_TUNE_MFMA64_MIN), then as below (the
+                         vector adjoint's choices only if it won);
+  otherwise:             transforms per thread of the vector synthesis and
                          per block of the vector adjoint (1, 2, 4), ring
                          pairs per lane of the vector adjoint (spin 0: 1, 2,
                          4; spin 2: 1, 2).
@@ -41,10 +44,11 @@ TUNE_SYNTH_NB = 4
 TUNE_ADJ_NB = 5
 TUNE_ADJ_K0 = 6
 TUNE_ADJ_K2 = 7
+TUNE_MFMA64_MIN = 8
 
 PARAMS = {'mfma_min': TUNE_MFMA_MIN, 'adj_ct0': TUNE_ADJ_CT0, 'adj_ct2': TUNE_ADJ_CT2,
           'synth_nb': TUNE_SYNTH_NB, 'adj_nb': TUNE_ADJ_NB, 'adj_k0': TUNE_ADJ_K0,
-          'adj_k2': TUNE_ADJ_K2}
+          'adj_k2': TUNE_ADJ_K2, 'mfma64_min': TUNE_MFMA64_MIN}
 
 _loaded = None
 
@@ -140,14 +144,22 @@ def _timer(plan, dtype, batch, reps):
 
 
 def _candidates(dtype, batch, mfma):
-    if dtype not in (torch.float64, torch.complex128) and batch >= 8:
+    '''The next parameters to search; mfma: None before the matrix-core
+    choice, else whether the matrix cores won.'''
+    f64 = dtype in (torch.float64, torch.complex128)
+    if not f64 and batch >= 8:
         if mfma is None:
             # 1: the matrix cores at this batch; batch + 1: the vector kernels
             return [('mfma_min', [1, batch + 1])]
         if mfma:
             return [('adj_ct2', [2, 4]), ('adj_ct0', [2, 4])]
+    if f64 and batch >= 4 and mfma is None:
+        return [('mfma64_min', [1, batch + 1])]
     nbs = [v for v in (1, 2, 4) if v <= batch]
-    return [('synth_nb', nbs), ('adj_nb', nbs), ('adj_k0', [1, 2, 4]), ('adj_k2', [1, 2])]
+    cands = [('synth_nb', nbs)]
+    if not (f64 and mfma):
+        cands += [('adj_nb', nbs), ('adj_k0', [1, 2, 4]), ('adj_k2', [1, 2])]
+    return cands
 
 
 def tune(minfo, lmax, dtype, batch, device=None, rings=None, reps=3, use_cache=True,
@@ -184,8 +196,8 @@ def tune(minfo, lmax, dtype, batch, device=None, rings=None, reps=3, use_cache=T
             best[name] = v
             best_t = times[v]
             _set(plan, batch, {name: v})
-        if cands and cands[0][0] == 'mfma_min':
-            mfma = best['mfma_min'] == 1
+        if cands and cands[0][0] in ('mfma_min', 'mfma64_min'):
+            mfma = best[cands[0][0]] == 1
             continue
         break
     entry = {'batch': int(batch), 'params': best, 'seconds': best_t}

@@ -1081,10 +1081,11 @@ adj2_kernel(LegArgs g, AdjItems it, const R *__restrict__ xs, const int *__restr
 
 // ---------------------------------------------------- matrix-core adjoint
 //
-// float32 batches: the sums over ring pairs run on the matrix cores as
-// D[i][j] += sum_k A[i][k] B[k][j] (v_mfma_f32_16x16x4f32; A[i][k] in lane
-// i + 16k, B[k][j] in lane j + 16k, D[i][j] in lane j + 16 (i / 4),
-// register i % 4) with
+// Batches: the sums over ring pairs run on the matrix cores as
+// D[i][j] += sum_k A[i][k] B[k][j] (v_mfma_f32_16x16x4f32 for float32,
+// v_mfma_f64_16x16x4f64 for float64; both take A[i][k] in lane i + 16k and
+// B[k][j] in lane j + 16k; float32 returns D[i][j] in lane j + 16 (i / 4),
+// register i % 4, float64 in lane j + 16 (i % 4), register i / 4) with
 //   rows i    : 16 multipoles of one parity of l - m (two row tiles per
 //               chunk of 32 multipoles),
 //   depth k   : spin 0: 4 ring pairs; spin 2: 2 ring pairs x (2lam, s (-2)lam),
@@ -1096,43 +1097,58 @@ adj2_kernel(LegArgs g, AdjItems it, const R *__restrict__ xs, const int *__restr
 // s S- = sum 2lam Zm(S) + s (-2)lam Zm(N), so its weights do not depend on the
 // row parity.
 // Each wave owns 64 ring pairs: its lanes run the float64 recurrences of
-// their pairs over a chunk and write the float32 values to a wave-private
-// LDS tile [multipole][pair]; the matrix products read the A operands from
+// their pairs over a chunk and write the values (float32 or float64) to a
+// wave-private LDS tile [multipole][pair]; the matrix products read the A operands from
 // that tile and keep the B operands (the weights, fixed per m) in registers.
 // The W waves of a block (64 W ring pairs) add their partial tiles through
 // LDS after every chunk; blocks of further ring pairs write partial sums
 // that sum_groups_kernel adds in fixed order.
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ f32x4 mfma16(float a, float b, f32x4 c) {
-  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-}
+template <typename T> struct Mfma16;
+template <> struct Mfma16<float> {
+  using acc = f32x4;
+  static __device__ __forceinline__ acc run(float a, float b, acc c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+template <> struct Mfma16<double> {
+  using acc = f64x4;
+  static __device__ __forceinline__ acc run(double a, double b, acc c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
 
 constexpr int kMaCH = 32;  // multipoles per adjoint chunk
 
-// CT: column tiles of 16 per block (transforms per block: 8 CT for spin 0,
-// 4 CT for spin 2). More tiles share one recurrence and one cross-wave
-// reduction pass between more products, at 32 CT registers of weights.
-template <int SPIN, int W, int CT>
+// E: element type (float: float32 alm and ring coefficients, float64
+// recurrence; double: float64 throughout, float64 matrix cores). CT: column
+// tiles of 16 per block (transforms per block: 8 CT for spin 0, 4 CT for
+// spin 2). More tiles share one recurrence and one cross-wave reduction pass
+// between more products, at 32 CT registers of weights.
+template <typename E, int SPIN, int W, int CT>
 __global__ void __launch_bounds__(64 * W)
 adj_mfma_kernel(LegArgs g, long long grp_ts, const R *__restrict__ xs,
                 const int *__restrict__ lfas, const R *__restrict__ sas,
                 const int *__restrict__ lfbs, const R *__restrict__ sbs,
-                const R *__restrict__ coef, const float2 *__restrict__ G, long long ntrans,
-                float2 *out, long long out_ts) {
+                const R *__restrict__ coef, const cplx_t<E> *__restrict__ G, long long ntrans,
+                cplx_t<E> *out, long long out_ts) {
+  using C = cplx_t<E>;
+  using V4 = typename Mfma16<E>::acc;
   constexpr int NT = (SPIN == 0 ? 8 : 4) * CT;    // transforms per block
   constexpr int NCF = SPIN == 0 ? 2 : 3;    // recurrence coefficients per multipole
-  constexpr int TS = SPIN == 0 ? 66 : 130;  // tile row stride in floats (bank-conflict free)
-  constexpr int TW = kMaCH * TS;            // tile floats per wave
+  constexpr int TS = SPIN == 0 ? 66 : 130;  // tile row stride in elements
+  constexpr int TW = kMaCH * TS;            // tile elements per wave
   static_assert(TW >= 1024, "the reduction buffer overlays the tile");
-  __shared__ __align__(16) float Tall[W * TW];
+  __shared__ __align__(32) E Tall[W * TW];
   __shared__ R cA[3][kMaCH];      // recurrence coefficient A' per chunk buffer
   __shared__ R cB[3][kMaCH];      // B' (spin 2)
-  __shared__ float cN[3][kMaCH];  // normalisation of the output
+  __shared__ E cN[3][kMaCH];      // normalisation of the output
   __shared__ int smin;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  float *T = Tall + wave * TW;
+  E *T = Tall + wave * TW;
   int grp, m;
   xcd_block(gridDim.x, gridDim.y, grp, m);
   const long long wbase = (long long)grp * 64 * W + 64 * wave;
@@ -1149,7 +1165,7 @@ adj_mfma_kernel(LegArgs g, long long grp_ts, const R *__restrict__ xs,
   const int lfb = (SPIN == 2 && active) ? lfbs[si] : INT_MAX;
   const int lmin = block_min(min(lfa, lfb), &smin);
   const R *cf = coef + NCF * ms;
-  float2 *om = out + grp * grp_ts + t0 * out_ts + ms;
+  C *om = out + grp * grp_ts + t0 * out_ts + ms;
   // chunks start at multipoles with l - m even
   const int lb0 = lmin <= lmax ? lmin - ((lmin - m) & 1) : lmax + 1;
   for (int b = 0; b < nt; ++b)
@@ -1165,11 +1181,11 @@ adj_mfma_kernel(LegArgs g, long long grp_ts, const R *__restrict__ xs,
   // (pair, ring, transform[, Q/U]) and lane) into its LDS tile, which the
   // chunks use only later, and builds the operands from there.
   const int bj = lane & 15, bk = lane >> 4;
-  float Bz[32 * CT];
+  E Bz[32 * CT];
   {
     const int rn = active ? g.pair_n[ip] : -1;
     const int rs = active ? g.pair_s[ip] : -1;
-    float2 *S = reinterpret_cast<float2 *>(T);
+    C *S = reinterpret_cast<C *>(T);
     if constexpr (SPIN == 0) {
       // one column tile (8 transforms) at a time: S[(b * 2 + side) * 64 + pair]
       static_assert(8 * 2 * 64 * 2 <= TW, "staging fits the tile");
@@ -1178,7 +1194,7 @@ adj_mfma_kernel(LegArgs g, long long grp_ts, const R *__restrict__ xs,
 #pragma unroll
         for (int bb = 0; bb < 8; ++bb) {
           const int b = 8 * ct + bb;
-          float2 vn = {0, 0}, vs = {0, 0};
+          C vn = {0, 0}, vs = {0, 0};
           if (rn >= 0 && b < nt) {
             const long long t = t0 + b;
             vn = G[(t * g.nrings + rn) * stride + m];
@@ -1189,12 +1205,12 @@ adj_mfma_kernel(LegArgs g, long long grp_ts, const R *__restrict__ xs,
         }
         wave_sync();
         // Bz[(par * 16 + s) * CT + ct]: pair 4s + k, transform 8ct + j / 2
-        const float *Sf = reinterpret_cast<const float *>(S);
+        const E *Sf = reinterpret_cast<const E *>(S);
 #pragma unroll
         for (int sq = 0; sq < 16; ++sq) {
           const int pl = 4 * sq + bk, bb = bj >> 1, comp = bj & 1;
-          const float gn = Sf[2 * ((bb * 2 + 0) * 64 + pl) + comp];
-          const float gs = Sf[2 * ((bb * 2 + 1) * 64 + pl) + comp];
+          const E gn = Sf[2 * ((bb * 2 + 0) * 64 + pl) + comp];
+          const E gs = Sf[2 * ((bb * 2 + 1) * 64 + pl) + comp];
           Bz[(0 * 16 + sq) * CT + ct] = gn + gs;
           Bz[(1 * 16 + sq) * CT + ct] = gn - gs;
         }
@@ -1210,7 +1226,7 @@ adj_mfma_kernel(LegArgs g, long long grp_ts, const R *__restrict__ xs,
       const int f = bk & 1, c = bj & 3, comp = c & 1;
       const int side = (f == 0) == (c < 2) ? 0 : 1;
       const bool minus = (c < 2) == (comp == 0);
-      const float *Sf = reinterpret_cast<const float *>(S);
+      const E *Sf = reinterpret_cast<const E *>(S);
 #pragma unroll
       for (int h = 0; h < CT / 2; ++h) {
 #pragma unroll
@@ -1218,7 +1234,7 @@ adj_mfma_kernel(LegArgs g, long long grp_ts, const R *__restrict__ xs,
           const int b = 8 * h + bb;
 #pragma unroll
           for (int qu = 0; qu < 2; ++qu) {
-            float2 vn = {0, 0}, vs = {0, 0};
+            C vn = {0, 0}, vs = {0, 0};
             if (rn >= 0 && b < nt) {
               const long long row = (t0 + b) * 2 * g.nrings + qu * g.nrings;
               vn = G[(row + rn) * stride + m];
@@ -1237,8 +1253,8 @@ adj_mfma_kernel(LegArgs g, long long grp_ts, const R *__restrict__ xs,
 #pragma unroll
           for (int cq = 0; cq < 2; ++cq) {
             const int bb = 4 * cq + (bj >> 2);
-            const float qv = Sf[2 * (((bb * 2 + 0) * 2 + side) * 64 + pl) + comp];
-            const float uv = Sf[2 * (((bb * 2 + 1) * 2 + side) * 64 + pl) + 1 - comp];
+            const E qv = Sf[2 * (((bb * 2 + 0) * 2 + side) * 64 + pl) + comp];
+            const E uv = Sf[2 * (((bb * 2 + 1) * 2 + side) * 64 + pl) + 1 - comp];
             Bz[sc * CT + 2 * h + cq] = minus ? qv - uv : qv + uv;
           }
         }
@@ -1261,13 +1277,13 @@ adj_mfma_kernel(LegArgs g, long long grp_ts, const R *__restrict__ xs,
   // chunk k and staged in the third LDS buffer, so no chunk waits on them.
   const int nch = (lmax - lb0) / kMaCH + 1;
   R ca = 0, cb = 0;
-  float cn = 0;
+  E cn = 0;
   auto load_coef = [&](int k) {
     const int l = lb0 + k * kMaCH + (int)threadIdx.x;
     const bool in = l <= lmax;
     ca = in ? cf[NCF * l] : (R)0;
     cb = (SPIN == 2 && in) ? cf[NCF * l + 1] : (R)0;
-    cn = in ? (float)(SPIN == 0 ? cf[2 * l + 1] : 0.5 * cf[3 * l + 2]) : 0.0f;
+    cn = in ? (E)(SPIN == 0 ? cf[2 * l + 1] : 0.5 * cf[3 * l + 2]) : (E)0;
   };
   auto store_coef = [&](int buf) {
     cA[buf][threadIdx.x] = ca;
@@ -1294,18 +1310,18 @@ adj_mfma_kernel(LegArgs g, long long grp_ts, const R *__restrict__ xs,
         R a0s, a1s;
         seeds(sas, lfa, lb, a0s, a1s);
         for (int i = 0; i < kMaCH; ++i)
-          T[i * TS + lane] = (float)rec_step(ka[i] * x, a1, a2, lb + i, lfa, a0s, a1s);
+          T[i * TS + lane] = (E)rec_step(ka[i] * x, a1, a2, lb + i, lfa, a0s, a1s);
       } else {
 #pragma unroll 8
         for (int i = 0; i < kMaCH; ++i) {
           const R lam = fma(ka[i] * x, a1, -a2);
           a2 = a1;
           a1 = lam;
-          T[i * TS + lane] = (float)lam;
+          T[i * TS + lane] = (E)lam;
         }
       }
     } else {
-      float2 *T2 = reinterpret_cast<float2 *>(T);
+      C *T2 = reinterpret_cast<C *>(T);
       if (inj) {
         R a0s, a1s, b0s, b1s;
         seeds(sas, lfa, lb, a0s, a1s);
@@ -1314,8 +1330,8 @@ adj_mfma_kernel(LegArgs g, long long grp_ts, const R *__restrict__ xs,
           const R ax = ka[i] * x;
           const R p = rec_step(ax + kb[i], a1, a2, lb + i, lfa, a0s, a1s);
           const R q = rec_step(ax - kb[i], b1, b2, lb + i, lfb, b0s, b1s);
-          const float qs = (i & 1) ? -(float)q : (float)q;
-          T2[i * (TS / 2) + lane] = {(float)p, qs};
+          const E qs = (i & 1) ? -(E)q : (E)q;
+          T2[i * (TS / 2) + lane] = {(E)p, qs};
         }
       } else {
 #pragma unroll 8
@@ -1327,56 +1343,60 @@ adj_mfma_kernel(LegArgs g, long long grp_ts, const R *__restrict__ xs,
           a1 = p;
           b2 = b1;
           b1 = q;
-          const float qs = (i & 1) ? -(float)q : (float)q;
-          T2[i * (TS / 2) + lane] = {(float)p, qs};
+          const E qs = (i & 1) ? -(E)q : (E)q;
+          T2[i * (TS / 2) + lane] = {(E)p, qs};
         }
       }
     }
     wave_sync();
     // products: D[par][ct] over the wave's ring pairs
-    f32x4 D[2][CT];
+    V4 D[2][CT];
 #pragma unroll
-    for (int ct = 0; ct < CT; ++ct) D[0][ct] = D[1][ct] = f32x4{0, 0, 0, 0};
+    for (int ct = 0; ct < CT; ++ct) D[0][ct] = D[1][ct] = V4{0, 0, 0, 0};
     if (SPIN == 0) {
 #pragma unroll
       for (int s = 0; s < 16; ++s) {
-        const float ae = T[(2 * bj) * TS + 4 * s + bk];
-        const float ao = T[(2 * bj + 1) * TS + 4 * s + bk];
+        const E ae = T[(2 * bj) * TS + 4 * s + bk];
+        const E ao = T[(2 * bj + 1) * TS + 4 * s + bk];
 #pragma unroll
         for (int ct = 0; ct < CT; ++ct) {
-          D[0][ct] = mfma16(ae, Bz[(0 * 16 + s) * CT + ct], D[0][ct]);
-          D[1][ct] = mfma16(ao, Bz[(1 * 16 + s) * CT + ct], D[1][ct]);
+          D[0][ct] = Mfma16<E>::run(ae, Bz[(0 * 16 + s) * CT + ct], D[0][ct]);
+          D[1][ct] = Mfma16<E>::run(ao, Bz[(1 * 16 + s) * CT + ct], D[1][ct]);
         }
       }
     } else {
 #pragma unroll
       for (int s = 0; s < 32; ++s) {
-        const float ae = T[(2 * bj) * TS + 4 * s + bk];
-        const float ao = T[(2 * bj + 1) * TS + 4 * s + bk];
+        const E ae = T[(2 * bj) * TS + 4 * s + bk];
+        const E ao = T[(2 * bj + 1) * TS + 4 * s + bk];
 #pragma unroll
         for (int ct = 0; ct < CT; ++ct) {
-          D[0][ct] = mfma16(ae, Bz[s * CT + ct], D[0][ct]);
-          D[1][ct] = mfma16(ao, Bz[s * CT + ct], D[1][ct]);
+          D[0][ct] = Mfma16<E>::run(ae, Bz[s * CT + ct], D[0][ct]);
+          D[1][ct] = Mfma16<E>::run(ao, Bz[s * CT + ct], D[1][ct]);
         }
       }
     }
     // partial tiles -> LDS (over this wave's tile): red[(par * CT + ct) * 16 + j][i]
     static_assert(2 * CT * 16 * 16 <= TW, "partial tiles fit the tile");
-    {
-      const int ib = 4 * bk;
 #pragma unroll
-      for (int par = 0; par < 2; ++par)
+    for (int par = 0; par < 2; ++par)
 #pragma unroll
-        for (int ct = 0; ct < CT; ++ct)
-          *reinterpret_cast<f32x4 *>(&T[((par * CT + ct) * 16 + bj) * 16 + ib]) = D[par][ct];
-    }
+      for (int ct = 0; ct < CT; ++ct) {
+        E *col = &T[((par * CT + ct) * 16 + bj) * 16];
+        if constexpr (std::is_same<E, float>::value) {
+          *reinterpret_cast<V4 *>(col + 4 * bk) = D[par][ct];  // rows 4 bk + r
+        } else {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) col[4 * r + bk] = D[par][ct][r];  // rows 4 r + bk
+        }
+      }
     if (threadIdx.x < kMaCH) store_coef((k + 2) % 3);
     __syncthreads();
     if (SPIN == 0) {
       // items (par, ct, i, column pair)
       for (int it = threadIdx.x; it < 256 * CT; it += 64 * W) {
         const int pc = it >> 7, i = (it >> 3) & 15, jp = it & 7;
-        float re = 0, im = 0;
+        E re = 0, im = 0;
 #pragma unroll
         for (int w = 0; w < W; ++w) {
           re += Tall[w * TW + (pc * 16 + 2 * jp) * 16 + i];
@@ -1385,7 +1405,7 @@ adj_mfma_kernel(LegArgs g, long long grp_ts, const R *__restrict__ xs,
         const int row = pc / CT + 2 * i;
         const int l = lb + row, b = 8 * (pc % CT) + jp;
         if (l <= lmax && b < nt) {
-          const float c = cN[bcur][row];
+          const E c = cN[bcur][row];
           om[b * out_ts + l] = {c * re, c * im};
         }
       }
@@ -1393,7 +1413,7 @@ adj_mfma_kernel(LegArgs g, long long grp_ts, const R *__restrict__ xs,
       // items (par, ct, i, transform of the tile)
       for (int it = threadIdx.x; it < 128 * CT; it += 64 * W) {
         const int pc = it >> 6, i = (it >> 2) & 15, tq = it & 3;
-        float v[4] = {0, 0, 0, 0};
+        E v[4] = {0, 0, 0, 0};
 #pragma unroll
         for (int w = 0; w < W; ++w)
 #pragma unroll
@@ -1402,9 +1422,9 @@ adj_mfma_kernel(LegArgs g, long long grp_ts, const R *__restrict__ xs,
         const int row = par + 2 * i;
         const int l = lb + row, b = 4 * (pc % CT) + tq;
         if (l <= lmax && b < nt) {
-          const float sg = par ? -1.0f : 1.0f;
-          const float2 sp = {v[0], v[1]}, sm = {sg * v[2], sg * v[3]};
-          const float c = cN[bcur][row];
+          const E sg = par ? (E)-1 : (E)1;
+          const C sp = {v[0], v[1]}, sm = {sg * v[2], sg * v[3]};
+          const E c = cN[bcur][row];
           // aE = -(S+ + S-)/2, aB = (i/2)(S+ - S-)
           om[b * out_ts + l] = {-c * (sp.x + sm.x), -c * (sp.y + sm.y)};
           om[b * out_ts + nelem + l] = {-c * (sp.y - sm.y), c * (sp.x - sm.x)};
@@ -1593,40 +1613,41 @@ static void launch_adj_nb(owmi_plan_impl *p, const cplx_t<T> *G, cplx_t<T> *alm,
   launch_adj_k<T, SPIN, K, 1>(p, G, alm, alm_ts, ntrans, st);
 }
 
-template <int SPIN, int W, int CT>
-static void launch_adj_mfma_w(owmi_plan_impl *p, const float2 *G, float2 *alm, long long alm_ts,
-                              long long ntrans, hipStream_t st) {
+template <typename E, int SPIN, int W, int CT>
+static void launch_adj_mfma_w(owmi_plan_impl *p, const cplx_t<E> *G, cplx_t<E> *alm,
+                              long long alm_ts, long long ntrans, hipStream_t st) {
   PlanTables<R> &tb = p->t64;
   const int ngroups = (int)ceil_div(p->npairs, 64LL * W);
   const long long nelem = nelem_of(p->lmax);
   const int nc = SPIN == 0 ? 1 : 2;
-  float2 *dst = alm;
+  cplx_t<E> *dst = alm;
   long long dst_ts = alm_ts, grp_ts = 0;
   if (ngroups > 1) {
     // partial sums [group][transform][comp][nelem] in the plan's second workspace
     dst_ts = nc * nelem;
     grp_ts = ntrans * dst_ts;
-    dst = static_cast<float2 *>(p->ensure_work2((size_t)ngroups * grp_ts * sizeof(float2)));
+    dst = static_cast<cplx_t<E> *>(
+        p->ensure_work2((size_t)ngroups * grp_ts * sizeof(cplx_t<E>)));
   }
   const int nt = (SPIN == 0 ? 8 : 4) * CT;
   dim3 grid(ngroups, (unsigned)(p->mmax + 1), ceil_div(ntrans, nt));
   if constexpr (SPIN == 0)
-    hipLaunchKernelGGL((adj_mfma_kernel<0, W, CT>), grid, dim3(64 * W), 0, st, leg_args(p),
+    hipLaunchKernelGGL((adj_mfma_kernel<E, 0, W, CT>), grid, dim3(64 * W), 0, st, leg_args(p),
                        grp_ts, tb.x, tb.lf0, tb.s0, nullptr, nullptr, tb.coef0, G, ntrans, dst,
                        dst_ts);
   else
-    hipLaunchKernelGGL((adj_mfma_kernel<2, W, CT>), grid, dim3(64 * W), 0, st, leg_args(p),
+    hipLaunchKernelGGL((adj_mfma_kernel<E, 2, W, CT>), grid, dim3(64 * W), 0, st, leg_args(p),
                        grp_ts, tb.x, tb.lfp, tb.sp, tb.lfm, tb.sm, tb.coef2, G, ntrans, dst,
                        dst_ts);
   if (ngroups > 1) {
     dim3 g2(ceil_div(nc * nelem, 256), (unsigned)ntrans);
-    hipLaunchKernelGGL(sum_groups_kernel<float>, g2, dim3(256), 0, st, dst, grp_ts, ngroups,
-                       alm, alm_ts, dst_ts, nelem, (long long)nc);
+    hipLaunchKernelGGL(sum_groups_kernel<E>, g2, dim3(256), 0, st, dst, grp_ts, ngroups, alm,
+                       alm_ts, dst_ts, nelem, (long long)nc);
   }
 }
 
-// Column tiles per block of the matrix-core adjoint: 4 for spin 2 from 16
-// transforms (one recurrence per 16 transforms), else 2; OWMI_ADJ_CT
+// Column tiles per block of the float32 matrix-core adjoint: 4 for spin 2
+// from 16 transforms (one recurrence per 16 transforms), else 2; OWMI_ADJ_CT
 // (2 or 4) overrides for measurements.
 static int adj_mfma_ct(const owmi_plan_impl *p, int spin, long long ntrans) {
   static const int forced = [] {
@@ -1639,23 +1660,42 @@ static int adj_mfma_ct(const owmi_plan_impl *p, int spin, long long ntrans) {
   return (spin == 2 && ntrans >= 16) ? 4 : 2;
 }
 
-template <int SPIN, int W>
-static void launch_adj_mfma_ct(owmi_plan_impl *p, const float2 *G, float2 *alm,
+// float32: CT from adj_mfma_ct; float64: 2 column tiles (the float64
+// weights of 4 tiles would take 256 registers).
+template <typename E, int SPIN, int W>
+static void launch_adj_mfma_ct(owmi_plan_impl *p, const cplx_t<E> *G, cplx_t<E> *alm,
                                long long alm_ts, long long ntrans, hipStream_t st) {
-  if (adj_mfma_ct(p, SPIN, ntrans) == 4)
-    return launch_adj_mfma_w<SPIN, W, 4>(p, G, alm, alm_ts, ntrans, st);
-  launch_adj_mfma_w<SPIN, W, 2>(p, G, alm, alm_ts, ntrans, st);
+  if constexpr (std::is_same<E, float>::value) {
+    if (adj_mfma_ct(p, SPIN, ntrans) == 4)
+      return launch_adj_mfma_w<E, SPIN, W, 4>(p, G, alm, alm_ts, ntrans, st);
+  }
+  launch_adj_mfma_w<E, SPIN, W, 2>(p, G, alm, alm_ts, ntrans, st);
 }
 
-// Waves per block: enough for every ring pair of the plan, at most 8.
-template <int SPIN>
-static void launch_adj_mfma(owmi_plan_impl *p, const float2 *G, float2 *alm, long long alm_ts,
-                            long long ntrans, hipStream_t st) {
+// Waves per block: enough for every ring pair of the plan, at most 8 (at
+// most 4 for float64 spin 2, whose tiles of 4 waves take 133 KB of LDS).
+template <typename E, int SPIN>
+static void launch_adj_mfma(owmi_plan_impl *p, const cplx_t<E> *G, cplx_t<E> *alm,
+                            long long alm_ts, long long ntrans, hipStream_t st) {
   const long long np = p->npairs;
-  if (np <= 64) return launch_adj_mfma_ct<SPIN, 1>(p, G, alm, alm_ts, ntrans, st);
-  if (np <= 128) return launch_adj_mfma_ct<SPIN, 2>(p, G, alm, alm_ts, ntrans, st);
-  if (np <= 256) return launch_adj_mfma_ct<SPIN, 4>(p, G, alm, alm_ts, ntrans, st);
-  launch_adj_mfma_ct<SPIN, 8>(p, G, alm, alm_ts, ntrans, st);
+  if (np <= 64) return launch_adj_mfma_ct<E, SPIN, 1>(p, G, alm, alm_ts, ntrans, st);
+  if (np <= 128) return launch_adj_mfma_ct<E, SPIN, 2>(p, G, alm, alm_ts, ntrans, st);
+  if (np <= 256 || (std::is_same<E, double>::value && SPIN == 2))
+    return launch_adj_mfma_ct<E, SPIN, 4>(p, G, alm, alm_ts, ntrans, st);
+  if constexpr (!(std::is_same<E, double>::value && SPIN == 2))
+    launch_adj_mfma_ct<E, SPIN, 8>(p, G, alm, alm_ts, ntrans, st);
+}
+
+// float64 batches from this size on the float64 matrix cores (tuned value,
+// else 8: at 4 transforms the vector adjoint is still faster at lmax 1023;
+// OWMI_LEGENDRE_VALU=1 keeps every precision on the vector kernels).
+static bool use_mfma64(const owmi_plan_impl *p, long long ntrans) {
+  static const bool off = [] {
+    const char *e = std::getenv("OWMI_LEGENDRE_VALU");
+    return e != nullptr && e[0] == '1';
+  }();
+  const int tuned = p->tuned(OWMI_TUNE_MFMA64_MIN, ntrans);
+  return !off && ntrans >= (tuned > 0 ? tuned : 8);
 }
 
 template <typename T>
@@ -1664,8 +1704,13 @@ void launch_adjoint(owmi_plan_impl *p, int spin, const cplx_t<T> *G, cplx_t<T> *
   OWMI_REQUIRE(ntrans >= 1 && ntrans < 65536, "ntrans must be in [1, 65535]");
   if constexpr (std::is_same<T, float>::value) {
     if (use_mfma(p, ntrans)) {
-      if (spin == 0) return launch_adj_mfma<0>(p, G, alm, alm_ts, ntrans, st);
-      return launch_adj_mfma<2>(p, G, alm, alm_ts, ntrans, st);
+      if (spin == 0) return launch_adj_mfma<float, 0>(p, G, alm, alm_ts, ntrans, st);
+      return launch_adj_mfma<float, 2>(p, G, alm, alm_ts, ntrans, st);
+    }
+  } else {
+    if (use_mfma64(p, ntrans)) {
+      if (spin == 0) return launch_adj_mfma<double, 0>(p, G, alm, alm_ts, ntrans, st);
+      return launch_adj_mfma<double, 2>(p, G, alm, alm_ts, ntrans, st);
     }
   }
   // Ring pairs per lane: 2, or 1 on geometries of at most 64 ring pairs

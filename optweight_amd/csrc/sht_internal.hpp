@@ -98,7 +98,7 @@ struct owmi_plan_impl {
   // with the largest such min_batch wins); 0 keeps the library's own rule.
   struct TuneEntry {
     long long min_batch = 1;
-    int v[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // by OWMI_TUNE_* parameter
+    int v[OWMI_TUNE_MFMA64_MIN + 1] = {};  // by OWMI_TUNE_* parameter (1 ...)
   };
   std::vector<TuneEntry> tune;  // sorted by min_batch
   // value of a parameter for a batch of ntrans (0: library rule)

@@ -653,11 +653,12 @@ int owmi_sht_plan_tune(owmi_sht_plan *plan, long long min_batch, int param, int 
   return guard([&] {
     OWMI_REQUIRE(plan != nullptr, "plan is NULL");
     OWMI_REQUIRE(min_batch >= 1, "min_batch must be >= 1");
-    OWMI_REQUIRE(param >= OWMI_TUNE_MFMA_MIN && param <= OWMI_TUNE_ADJ_K2,
+    OWMI_REQUIRE(param >= OWMI_TUNE_MFMA_MIN && param <= OWMI_TUNE_MFMA64_MIN,
                  "unknown tuning parameter");
     bool ok = value == 0;
     switch (param) {
-      case OWMI_TUNE_MFMA_MIN: ok = ok || value >= 1; break;
+      case OWMI_TUNE_MFMA_MIN:
+      case OWMI_TUNE_MFMA64_MIN: ok = ok || value >= 1; break;
       case OWMI_TUNE_ADJ_CT0:
       case OWMI_TUNE_ADJ_CT2: ok = ok || value == 2 || value == 4; break;
       case OWMI_TUNE_SYNTH_NB:

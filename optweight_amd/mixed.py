@@ -106,6 +106,11 @@ def _dot(a, b, out=None):
     return alm_utils.contract_almxblm_batch(a, b, out=out)
 
 
+def _relative(rr, bb):
+    '''sqrt(rr / bb) per right-hand side; 0 for b = 0 (solved by x = 0).'''
+    return np.sqrt(np.divide(rr, bb, out=np.zeros_like(rr), where=bb > 0))
+
+
 class _State:
     '''One batch of right-hand sides of a batched CG solve. An iteration is
     phase 0 on the device (q = A p, <p, q>, x += alpha p, r -= alpha q, and
@@ -198,7 +203,7 @@ class _Fp64State(_State):
         res = None
         if verify:
             r64 = self.b3 - self.a_op(self.x)
-            res = np.sqrt(_dot(r64, r64).cpu().numpy() / self.bn2)
+            res = _relative(_dot(r64, r64).cpu().numpy(), self.bn2)
         return SolveInfo(self.it, self.it_rhs.copy(), 0, res)
 
 
@@ -232,7 +237,7 @@ class _MixedState(_State):
         r64 = self.b3[sel] - self.a64(self.x[sel].contiguous())
         rr64 = _dot(r64, r64).cpu().numpy()
         self.nrep += 1
-        self.res[idx] = np.sqrt(rr64 / self.bn2[idx])
+        self.res[idx] = _relative(rr64, self.bn2[idx])
         conv = rr64 <= self.target[idx]
         self.active[idx[conv]] = False
         if not self.active.any():

@@ -339,3 +339,49 @@ def test_legendre_flop_counter(dev):
     sht.alm2map(a[0, :1].to(torch.complex64).contiguous(),
                 torch.zeros((1, m.npix), dtype=torch.float32, device=dev), ai, m, 0, rings=rings)
     assert sht.LEGENDRE_FLOPS['float32'] == layout.nelem(lmax) * 3 * 2
+
+
+@pytest.mark.parametrize('lmax', [17, 64])
+@pytest.mark.parametrize('ntrans', [4, 5, 17, 33])
+@pytest.mark.parametrize('adjoint', [False, True])
+def test_adjoint_fp64_batched(dev, lmax, ntrans, adjoint):
+    '''Batches of 4 or more float64 transforms take the adjoint on the
+    float64 matrix cores (partial column tiles at 5, 17 and 33).'''
+    print(f'map2alm fp64 batch lmax={lmax} ntrans={ntrans} adjoint={adjoint}', flush=True)
+    g, m = geoms(lmax)
+    rng = np.random.default_rng(3)
+    f = rng.normal(size=(ntrans, 3, g.npix))
+    ref = np.zeros((ntrans, 3, layout.nelem(lmax)), dtype=np.complex128)
+    for t in range(ntrans):
+        osht.map2alm(f[t], ref[t], g, lmax, [0, 2], adjoint=adjoint)
+    out = torch.zeros(ref.shape, dtype=torch.complex128, device=dev)
+    sht.map2alm(torch.as_tensor(f, device=dev), out, m, alm_utils.AlmInfo(lmax), [0, 2],
+                adjoint=adjoint)
+    assert rel(out.cpu().numpy(), ref) < 1e-12
+
+
+@pytest.mark.parametrize('lmax,rings', [(150, None), (600, None), (600, 'subset')])
+def test_fp64_matrix_cores_match_vector_kernels(dev, lmax, rings):
+    '''The float64 matrix-core adjoint equals the float64 vector adjoint
+    (the plan's threshold raised out of reach) to relative 1e-13 on
+    geometries of 76 and 301 ring pairs (blocks of 2 to 8 waves, several
+    ring-pair groups for spin 2) and on a ring subset.'''
+    from optweight_amd import _lib, autotune
+    print(f'fp64 matrix cores vs vector lmax={lmax} rings={rings}', flush=True)
+    m = map_utils.MapInfo.map_info_gauss_legendre(lmax + 1, 2 * lmax + 1)
+    ring_sel = None if rings is None else np.arange(5, m.nrow - 40, dtype=np.int64)
+    ai = alm_utils.AlmInfo(lmax)
+    rng = np.random.default_rng(4)
+    f = torch.as_tensor(rng.normal(size=(16, 3, m.npix)), device=dev)
+    plan = sht.get_plan(m, lmax, dev, rings=ring_sel)
+    outs = []
+    for vector in (False, True):
+        _lib.call('owmi_sht_plan_tune_reset', plan.handle)
+        if vector:
+            _lib.call('owmi_sht_plan_tune', plan.handle, 1, autotune.TUNE_MFMA64_MIN, 1 << 20)
+        out = torch.zeros((16, 3, ai.nelem), dtype=torch.complex128, device=dev)
+        sht.map2alm(f, out, m, ai, [0, 2], rings=ring_sel)
+        outs.append(out.cpu().numpy())
+    _lib.call('owmi_sht_plan_tune_reset', plan.handle)
+    for c in (slice(0, 1), slice(1, 3)):
+        assert rel(outs[0][:, c], outs[1][:, c]) < 1e-13
