@@ -711,6 +711,74 @@ def ragged_bench(args, dev):
     print(json.dumps(out))
 
 
+def autodiff_bench(args, dev):
+    '''Gradients of L = 1/2 d^T (P S P^T + N)^-1 d + 1/2 <x, x> (x the Wiener
+    solution) with respect to the data map d and 30 log-spaced band
+    amplitudes of S, on the TQU workload at --lmax: one step is the forward
+    float64 solve, the loss and the backward (one adjoint float64 solve).
+    Central finite differences of the loss along one random direction in d
+    and one in the amplitudes check the gradients at this size.'''
+    from optweight_amd import autodiff
+    t0 = time.perf_counter()
+    prob = workloads.polarized(args.lmax, seed=0, device=dev)
+    edges = workloads.log_bands(2, args.lmax, 30)
+    model = autodiff.WienerModel(prob.minfo, prob.ainfo, prob.icov_ell, prob.icov_pix, edges,
+                                 tol=1e-10, device=dev)
+    amps = 1.0 + 0.1 * np.sin(np.arange(30))
+    d0 = torch.as_tensor(prob.imap, device=dev)
+    a0 = torch.as_tensor(amps, device=dev)
+    log(f'setup {time.perf_counter() - t0:.1f}s')
+
+    def step():
+        d = d0.clone().requires_grad_()
+        a = a0.clone().requires_grad_()
+        loss = autodiff.config_loss(d, a, model)
+        loss.backward()
+        return float(loss.item()), d.grad, a.grad
+
+    for _ in range(max(1, args.warmup)):
+        loss, gd, ga = step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        loss, gd, ga = step()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t1
+
+    def value(d, a):
+        with torch.no_grad():
+            return float(autodiff.config_loss(d, a, model).item())
+    rng = np.random.default_rng(1)
+    ed = torch.as_tensor(rng.normal(size=prob.imap.shape) * (prob.icov_pix > 0), device=dev)
+    ed *= torch.linalg.vector_norm(d0) / torch.linalg.vector_norm(ed)
+    ea = torch.as_tensor(rng.normal(size=30), device=dev)
+    # L is quadratic in d (central differences exact up to the solves);
+    # in the amplitudes the step trades truncation against solve error
+    hd, ha = 1e-3, 1e-4
+    fd_d = (value(d0 + hd * ed, a0) - value(d0 - hd * ed, a0)) / (2 * hd)
+    fd_a = (value(d0, a0 + ha * ea) - value(d0, a0 - ha * ea)) / (2 * ha)
+    an_d = float(torch.sum(gd * ed).item())
+    an_a = float(torch.sum(ga * ea).item())
+    err_d = abs(an_d - fd_d) / abs(fd_d)
+    err_a = abs(an_a - fd_a) / abs(fd_a)
+    log(f'fd check: d {an_d:.10e} vs {fd_d:.10e} ({err_d:.1e}); '
+        f'amplitudes {an_a:.10e} vs {fd_a:.10e} ({err_a:.1e})')
+    out = {'metric': 'loss+gradient evaluations/sec (data map and 30 band amplitudes), TQU '
+                     f'lmax={args.lmax}',
+           'value': args.steps / el, 'unit': 'evaluations/s', 'n_gpus': 1, 'steps': args.steps,
+           'warmup': args.warmup, 'ms_per_step': 1e3 * el / args.steps,
+           'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f64',
+           'data': 'synthetic',
+           'config': {'workload': f'differentiable Wiener filter, TQU lmax {args.lmax}, GL '
+                                  f'{prob.minfo.nrow}x{prob.minfo.nphi[0]}, pinv preconditioner, '
+                                  'forward and adjoint float64 CG to 1e-10, 30 log-spaced bands '
+                                  f'{edges.tolist()}',
+                      'loss': loss,
+                      'fd_rel_err_data': err_d, 'fd_rel_err_amplitudes': err_a,
+                      'fd_steps': {'data': hd, 'amplitudes': ha}}}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -761,12 +829,16 @@ def main():
     ap.add_argument('--fwav', action='store_true',
                     help='time the 2-D Fourier-wavelet noise model inverse on the full-sky CAR '
                          'map of band limit --wavelet-lmax; prints its own JSON line')
+    ap.add_argument('--autodiff', action='store_true',
+                    help='time loss + gradient evaluations of the differentiable Wiener filter '
+                         '(data map and 30 log-spaced band amplitudes) on the TQU workload at '
+                         '--lmax, with a finite-difference check; prints its own JSON line')
     ap.add_argument('--roofline-only', action='store_true',
                     help='only build the workload geometry and time the roofline kernel '
                          '(for counter runs), after one copy of known size')
     args = ap.parse_args()
     headline = not (args.ragged or args.wavelet or args.phase1 or args.filter or args.const_cor
-                    or args.noise_est or args.fwav or args.streaming)
+                    or args.noise_est or args.fwav or args.streaming or args.autodiff)
     if args.batch is None:
         args.batch = HEADLINE_BATCH if headline else AUX_BATCH
     if args.gpus != 1:
@@ -778,6 +850,9 @@ def main():
 
     if args.ragged:
         ragged_bench(args, dev)
+        return
+    if args.autodiff:
+        autodiff_bench(args, dev)
         return
     if args.wavelet:
         wavelet_bench(args, dev)

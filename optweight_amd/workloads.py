@@ -263,6 +263,18 @@ def ragged(lmaxs=(255, 511, 1023, 2047), per=16, seed=0, device=None, synth=None
     return out
 
 
+def log_bands(lmin, lmax, nband):
+    """Edges of nband log-spaced multipole bands covering lmin <= l <= lmax
+    (band k: edges[k] <= l < edges[k + 1]): rounded geometric edges, each at
+    least one above the previous."""
+    edges = np.round(np.geomspace(lmin, lmax + 1, nband + 1)).astype(int)
+    for k in range(1, edges.size):
+        edges[k] = max(edges[k], edges[k - 1] + 1)
+    if edges[-1] != lmax + 1:
+        raise ValueError(f'{nband} bands do not fit between l = {lmin} and {lmax}')
+    return edges
+
+
 def needlet_kernels(lmax, nwav=12, lamb=2.0):
     """Cosine-needlet kernels on a log_lamb(l) axis, partition of unity
     sum_j w_j(l)^2 = 1: w_j = cos(pi/2 (t - j)) for |t - j| < 1 with

@@ -108,3 +108,14 @@ def test_workload_mask_and_spectra():
                                np.eye(3)[:, :, None] * np.ones(62), atol=1e-9)
     d = workloads.depth_map(m, 1)
     assert abs(d.min() - 0.5) < 1e-12 and abs(d.max() - 2.0) < 1e-12
+
+
+def test_log_bands():
+    from optweight_amd import workloads
+    e = workloads.log_bands(2, 1023, 30)
+    assert e.size == 31 and e[0] == 2 and e[-1] == 1024 and np.all(np.diff(e) >= 1)
+    # rounding collisions at low l are pushed up by one
+    e = workloads.log_bands(2, 40, 20)
+    assert e.size == 21 and np.all(np.diff(e) >= 1) and e[-1] == 41
+    with pytest.raises(ValueError):
+        workloads.log_bands(2, 20, 30)
