@@ -208,6 +208,18 @@ int owmi_cg_xr_batch(int dtype_x, void *x, int dtype_r, void *r,
                      const double *dev_den, long long n, long long nbatch,
                      void *stream);
 
+/*
+ * owmi_cg_xr_batch on alm (nbatch, nrows, nelem(lmax)) with the real-field
+ * <r_b, r_b> of every new residual written to dev_rr[b] (float64, as
+ * owmi_alm_dot_batch of r with itself would give, summed in another fixed
+ * order): the x/r update and the residual norm of pixell.utils.CG.step in
+ * one pass over r. nbatch <= 1024. Uses the stream's partial-sum buffer of
+ * owmi_alm_dot_batch.
+ */
+int owmi_cg_xr_rr_batch(int dtype_x, void *x, int dtype_r, void *r, const void *p,
+                        const void *q, const double *dev_num, const double *dev_den, int lmax,
+                        long long nrows, long long nbatch, double *dev_rr, void *stream);
+
 /* Batched p_b = z_b + beta_b p_b, beta_b = dev_num[b] / dev_den[b] (0 when
  * dev_den[b] == 0); arrays (nbatch, n). */
 int owmi_cg_p_batch(int dtype, void *p, const void *z, const double *dev_num,

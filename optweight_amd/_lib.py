@@ -40,6 +40,8 @@ _SIGNATURES = {
     'owmi_alm_dot_batch_noalloc': (_c_int, [_c_int, _vp, _vp, _c_int, _c_ll, _c_ll, _vp, _vp]),
     'owmi_cg_xr_batch': (_c_int, [_c_int, _vp, _c_int, _vp, _vp, _vp, _vp, _vp, _c_ll, _c_ll,
                                   _vp]),
+    'owmi_cg_xr_rr_batch': (_c_int, [_c_int, _vp, _c_int, _vp, _vp, _vp, _vp, _vp, _c_int,
+                                     _c_ll, _c_ll, _vp, _vp]),
     'owmi_cg_p_batch': (_c_int, [_c_int, _vp, _vp, _vp, _vp, _c_ll, _c_ll, _vp]),
     'owmi_axpby': (_c_int, [_c_int, _c_dbl, _vp, _c_dbl, _vp, _c_ll, _vp]),
     'owmi_convert': (_c_int, [_c_int, _vp, _c_int, _vp, _c_ll, _vp]),

@@ -634,6 +634,46 @@ __global__ void cg_xr_batch_kernel(cplx_t<X> *x, cplx_t<R> *r, const cplx_t<R> *
   r[o] = {rv.x - a * qv.x, rv.y - a * qv.y};
 }
 
+// The update of cg_xr_batch_kernel with the real-field <r, r> of the new
+// residual folded in: block (x, b) updates a strided slice of every row of
+// right-hand side b (the slices of dot_batch_partial_kernel, m = 0 elements
+// counted once, the rest twice) and writes its partial sum of |r|^2 (of r as
+// stored, in precision R); dot_batch_final_kernel adds the partials of each
+// right-hand side in fixed order.
+template <typename X, typename R>
+__global__ void __launch_bounds__(kDotThreads)
+cg_xr_rr_batch_kernel(cplx_t<X> *x, cplx_t<R> *r, const cplx_t<R> *p, const cplx_t<R> *q,
+                      const double *num, const double *den, int lmax, long long nelem,
+                      long long nrows, double *partial) {
+  __shared__ double red[kDotThreads];
+  const long long b = blockIdx.y;
+  const double alpha = safe_ratio(num, den, b);
+  const R a = (R)alpha;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  const long long j0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  double acc = 0.0;
+  for (long long row = 0; row < nrows; ++row) {
+    const long long base = (b * nrows + row) * nelem;
+    for (long long j = j0; j < nelem; j += stride) {
+      const long long o = base + j;
+      const cplx_t<R> pv = p[o], qv = q[o], rv = r[o];
+      const cplx_t<X> xv = x[o];
+      x[o] = {(X)(xv.x + alpha * (double)pv.x), (X)(xv.y + alpha * (double)pv.y)};
+      const cplx_t<R> rn = {rv.x - a * qv.x, rv.y - a * qv.y};
+      r[o] = rn;
+      const double v = (double)rn.x * (double)rn.x + (double)rn.y * (double)rn.y;
+      acc += (j <= lmax) ? v : 2.0 * v;
+    }
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = kDotThreads / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[b * gridDim.x + blockIdx.x] = red[0];
+}
+
 template <typename T>
 __global__ void cg_p_batch_kernel(cplx_t<T> *p, const cplx_t<T> *z, const double *num,
                                   const double *den, long long n) {
@@ -846,6 +886,39 @@ int owmi_cg_xr_batch(int dtype_x, void *x, int dtype_r, void *r, const void *p, 
                          (float2 *)r, (const float2 *)p, (const float2 *)q, dev_num, dev_den, n);
     else
       throw Error("x must be at least as precise as r");
+  });
+}
+
+int owmi_cg_xr_rr_batch(int dtype_x, void *x, int dtype_r, void *r, const void *p,
+                        const void *q, const double *dev_num, const double *dev_den, int lmax,
+                        long long nrows, long long nbatch, double *dev_rr, void *stream) {
+  return guard([&] {
+    check_dtype(dtype_x);
+    check_dtype(dtype_r);
+    OWMI_REQUIRE(lmax >= 0 && nrows >= 1, "invalid cg_xr_rr_batch sizes");
+    OWMI_REQUIRE(nbatch >= 1 && nbatch <= kDotBlocks, "nbatch must be in [1, 1024]");
+    OWMI_REQUIRE(dev_rr != nullptr, "cg_xr_rr_batch: dev_rr is required");
+    hipStream_t st = as_stream(stream);
+    const long long nelem = nelem_of(lmax);
+    const int bpb = kDotBlocks / (int)nbatch;
+    double *part = dot_scratch(st);
+    dim3 g((unsigned)bpb, (unsigned)nbatch);
+    if (dtype_x == OWMI_F64 && dtype_r == OWMI_F32)
+      hipLaunchKernelGGL((cg_xr_rr_batch_kernel<double, float>), g, dim3(kDotThreads), 0, st,
+                         (double2 *)x, (float2 *)r, (const float2 *)p, (const float2 *)q,
+                         dev_num, dev_den, lmax, nelem, nrows, part);
+    else if (dtype_x == OWMI_F64 && dtype_r == OWMI_F64)
+      hipLaunchKernelGGL((cg_xr_rr_batch_kernel<double, double>), g, dim3(kDotThreads), 0, st,
+                         (double2 *)x, (double2 *)r, (const double2 *)p, (const double2 *)q,
+                         dev_num, dev_den, lmax, nelem, nrows, part);
+    else if (dtype_x == OWMI_F32 && dtype_r == OWMI_F32)
+      hipLaunchKernelGGL((cg_xr_rr_batch_kernel<float, float>), g, dim3(kDotThreads), 0, st,
+                         (float2 *)x, (float2 *)r, (const float2 *)p, (const float2 *)q,
+                         dev_num, dev_den, lmax, nelem, nrows, part);
+    else
+      throw Error("x must be at least as precise as r");
+    hipLaunchKernelGGL(dot_batch_final_kernel, dim3((unsigned)nbatch), dim3(kDotThreads), 0, st,
+                       part, bpb, dev_rr);
   });
 }
 

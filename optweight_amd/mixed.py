@@ -93,6 +93,15 @@ def _xr(x, r, p, q, rz, pq, nb):
               r.numel() // nb, nb, _lib.stream_ptr(r.device))
 
 
+def _xr_rr(x, r, p, q, rz, pq, rr, nb):
+    '''_xr with <r, r> of the new residuals into rr (one pass over r).'''
+    nelem = r.shape[-1]
+    lmax = alm_utils.getlmax(nelem)
+    _lib.call('owmi_cg_xr_rr_batch', _lib.dtype_code(x), _lib.ptr(x), _lib.dtype_code(r),
+              _lib.ptr(r), _lib.ptr(p), _lib.ptr(q), _lib.ptr(rz), _lib.ptr(pq), lmax,
+              r.numel() // (nb * nelem), nb, _lib.ptr(rr), _lib.stream_ptr(r.device))
+
+
 def _pupd(p, z, rz_new, rz, nb):
     _lib.call('owmi_cg_p_batch', _lib.dtype_code(p), _lib.ptr(p), _lib.ptr(z), _lib.ptr(rz_new),
               _lib.ptr(rz), p.numel() // nb, nb, _lib.stream_ptr(p.device))
@@ -157,8 +166,7 @@ class _State:
         if k == 0:
             q = self.a_op(self.p)
             _dot(self.p, q, out=self.pq)
-            _xr(self.x, self.r, self.p, q, self.rz, self.pq, self.nb)
-            _dot(self.r, self.r, out=self.rr)
+            _xr_rr(self.x, self.r, self.p, q, self.rz, self.pq, self.rr, self.nb)
         else:
             z = self.m_op(self.r)
             _dot(self.r, z, out=self.rzn)

@@ -126,6 +126,35 @@ def test_cg_batch_updates(dev):
         assert rel(p[i], exp) < 1e-15
 
 
+@pytest.mark.parametrize('nb,nrows,lmax', [(1, 1, 30), (3, 3, 17), (16, 3, 40)])
+@pytest.mark.parametrize('xdt,rdt', [(torch.complex128, torch.complex128),
+                                     (torch.complex128, torch.complex64),
+                                     (torch.complex64, torch.complex64)])
+def test_cg_xr_rr_fused(dev, nb, nrows, lmax, xdt, rdt):
+    '''The fused x/r update returns the update of owmi_cg_xr_batch bit for
+    bit and <r, r> of the new residuals as the batched dot computes it
+    (relative 1e-13; a right-hand side with a zero denominator keeps r).'''
+    print(f'cg_xr_rr nb={nb} nrows={nrows} lmax={lmax} {xdt} {rdt}', flush=True)
+    rng = np.random.default_rng(5)
+    ne = (lmax + 1) * (lmax + 2) // 2
+    shape = (nb, nrows, ne)
+
+    def rand(dt):
+        v = torch.as_tensor(rng.normal(size=shape) + 1j * rng.normal(size=shape), device=dev)
+        return v.to(dt)
+    x, r, p, q = rand(xdt), rand(rdt), rand(rdt), rand(rdt)
+    num = torch.as_tensor(rng.uniform(1, 2, nb), device=dev)
+    den = torch.as_tensor(rng.uniform(1, 2, nb), device=dev)
+    den[nb // 2] = 0
+    x1, r1 = x.clone(), r.clone()
+    mixed._xr(x1, r1, p, q, num, den, nb)
+    rr1 = alm_utils.contract_almxblm_batch(r1, r1).cpu().numpy()
+    rr = torch.empty(nb, dtype=torch.float64, device=dev)
+    mixed._xr_rr(x, r, p, q, num, den, rr, nb)
+    assert torch.equal(x, x1) and torch.equal(r, r1)
+    np.testing.assert_allclose(rr.cpu().numpy(), rr1, rtol=1e-13)
+
+
 def test_batched_cg_matches_single(dev, prob, data):
     batch = add_precs(solvers.CGWienerMap.from_arrays(
         data, prob.minfo, prob.ainfo, prob.icov_ell, prob.icov_pix, device=dev), prob, dev)
