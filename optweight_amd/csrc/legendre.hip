@@ -1376,18 +1376,22 @@ adj_mfma_kernel(LegArgs g, long long grp_ts, const R *__restrict__ xs,
         }
       }
     }
-    // partial tiles -> LDS (over this wave's tile): red[(par * CT + ct) * 16 + j][i]
-    static_assert(2 * CT * 16 * 16 <= TW, "partial tiles fit the tile");
+    // partial tiles -> LDS (over this wave's tile), row-major with a padded
+    // row stride: red[(par * CT + ct) * 16 + i][j] at ((par * CT + ct) * 16 + i) * RS + j,
+    // so the reduction reads whole runs of a row (one vector load per wave)
+    // and the stores of the 16 columns of a row spread over the banks
+    constexpr int RS = 2 * CT * 16 * 20 <= TW ? 20 : 16;
+    static_assert(2 * CT * 16 * RS <= TW, "partial tiles fit the tile");
 #pragma unroll
     for (int par = 0; par < 2; ++par)
 #pragma unroll
       for (int ct = 0; ct < CT; ++ct) {
-        E *col = &T[((par * CT + ct) * 16 + bj) * 16];
-        if constexpr (std::is_same<E, float>::value) {
-          *reinterpret_cast<V4 *>(col + 4 * bk) = D[par][ct];  // rows 4 bk + r
-        } else {
+        E *tile = &T[(par * CT + ct) * 16 * RS + bj];
 #pragma unroll
-          for (int r = 0; r < 4; ++r) col[4 * r + bk] = D[par][ct][r];  // rows 4 r + bk
+        for (int r = 0; r < 4; ++r) {
+          // float32: rows 4 bk + r; float64: rows 4 r + bk
+          const int i = std::is_same<E, float>::value ? 4 * bk + r : 4 * r + bk;
+          tile[i * RS] = D[par][ct][r];
         }
       }
     if (threadIdx.x < kMaCH) store_coef((k + 2) % 3);
@@ -1399,8 +1403,9 @@ adj_mfma_kernel(LegArgs g, long long grp_ts, const R *__restrict__ xs,
         E re = 0, im = 0;
 #pragma unroll
         for (int w = 0; w < W; ++w) {
-          re += Tall[w * TW + (pc * 16 + 2 * jp) * 16 + i];
-          im += Tall[w * TW + (pc * 16 + 2 * jp + 1) * 16 + i];
+          const C v = *reinterpret_cast<const C *>(&Tall[w * TW + (pc * 16 + i) * RS + 2 * jp]);
+          re += v.x;
+          im += v.y;
         }
         const int row = pc / CT + 2 * i;
         const int l = lb + row, b = 8 * (pc % CT) + jp;
@@ -1415,9 +1420,14 @@ adj_mfma_kernel(LegArgs g, long long grp_ts, const R *__restrict__ xs,
         const int pc = it >> 6, i = (it >> 2) & 15, tq = it & 3;
         E v[4] = {0, 0, 0, 0};
 #pragma unroll
-        for (int w = 0; w < W; ++w)
-#pragma unroll
-          for (int cc = 0; cc < 4; ++cc) v[cc] += Tall[w * TW + (pc * 16 + 4 * tq + cc) * 16 + i];
+        for (int w = 0; w < W; ++w) {
+          const C *src = reinterpret_cast<const C *>(&Tall[w * TW + (pc * 16 + i) * RS + 4 * tq]);
+          const C lo = src[0], hi = src[1];
+          v[0] += lo.x;
+          v[1] += lo.y;
+          v[2] += hi.x;
+          v[3] += hi.y;
+        }
         const int par = pc / CT;
         const int row = par + 2 * i;
         const int l = lb + row, b = 4 * (pc % CT) + tq;
