@@ -1140,7 +1140,11 @@ adj_mfma_kernel(LegArgs g, long long grp_ts, const R *__restrict__ xs,
   constexpr int NT = (SPIN == 0 ? 8 : 4) * CT;    // transforms per block
   constexpr int NCF = SPIN == 0 ? 2 : 3;    // recurrence coefficients per multipole
   constexpr int TS = SPIN == 0 ? 66 : 130;  // tile row stride in elements
-  constexpr int TW = kMaCH * TS;            // tile elements per wave
+  // staging of the weights, rows of SP complex values (64 pairs + 2): the
+  // lanes of one staging read fall on distinct banks
+  constexpr int SP = 66;
+  constexpr int TW = SPIN == 0 ? kMaCH * TS : 2 * 2 * 8 * SP * 2;  // tile elements per wave
+  static_assert(TW >= kMaCH * TS, "the tile fits");
   static_assert(TW >= 1024, "the reduction buffer overlays the tile");
   __shared__ __align__(32) E Tall[W * TW];
   __shared__ R cA[3][kMaCH];      // recurrence coefficient A' per chunk buffer
@@ -1187,8 +1191,8 @@ adj_mfma_kernel(LegArgs g, long long grp_ts, const R *__restrict__ xs,
     const int rs = active ? g.pair_s[ip] : -1;
     C *S = reinterpret_cast<C *>(T);
     if constexpr (SPIN == 0) {
-      // one column tile (8 transforms) at a time: S[(b * 2 + side) * 64 + pair]
-      static_assert(8 * 2 * 64 * 2 <= TW, "staging fits the tile");
+      // one column tile (8 transforms) at a time: S[(b * 2 + side) * SP + pair]
+      static_assert(8 * 2 * SP * 2 <= TW, "staging fits the tile");
 #pragma unroll
       for (int ct = 0; ct < CT; ++ct) {
 #pragma unroll
@@ -1200,8 +1204,8 @@ adj_mfma_kernel(LegArgs g, long long grp_ts, const R *__restrict__ xs,
             vn = G[(t * g.nrings + rn) * stride + m];
             if (rs >= 0) vs = G[(t * g.nrings + rs) * stride + m];
           }
-          S[(bb * 2 + 0) * 64 + lane] = vn;
-          S[(bb * 2 + 1) * 64 + lane] = vs;
+          S[(bb * 2 + 0) * SP + lane] = vn;
+          S[(bb * 2 + 1) * SP + lane] = vs;
         }
         wave_sync();
         // Bz[(par * 16 + s) * CT + ct]: pair 4s + k, transform 8ct + j / 2
@@ -1209,8 +1213,8 @@ adj_mfma_kernel(LegArgs g, long long grp_ts, const R *__restrict__ xs,
 #pragma unroll
         for (int sq = 0; sq < 16; ++sq) {
           const int pl = 4 * sq + bk, bb = bj >> 1, comp = bj & 1;
-          const E gn = Sf[2 * ((bb * 2 + 0) * 64 + pl) + comp];
-          const E gs = Sf[2 * ((bb * 2 + 1) * 64 + pl) + comp];
+          const E gn = Sf[2 * ((bb * 2 + 0) * SP + pl) + comp];
+          const E gs = Sf[2 * ((bb * 2 + 1) * SP + pl) + comp];
           Bz[(0 * 16 + sq) * CT + ct] = gn + gs;
           Bz[(1 * 16 + sq) * CT + ct] = gn - gs;
         }
@@ -1218,8 +1222,8 @@ adj_mfma_kernel(LegArgs g, long long grp_ts, const R *__restrict__ xs,
       }
     } else {
       // two column tiles (8 transforms) at a time:
-      // S[((b * 2 + qu) * 2 + side) * 64 + pair], qu = 0: Q, 1: U
-      static_assert(8 * 2 * 2 * 64 * 2 <= TW, "staging fits the tile");
+      // S[((qu * 2 + side) * 8 + b) * SP + pair], qu = 0: Q, 1: U
+      static_assert(8 * 2 * 2 * SP * 2 <= TW, "staging fits the tile");
       static_assert(CT % 2 == 0, "spin 2 stages column tiles in pairs");
       // f = 0: (Zp(N), Zm(S)), f = 1: (Zp(S), Zm(N)) for c < 2 / c >= 2, with
       // Zp = q + i u = (q.x - u.y, q.y + u.x), Zm = q - i u = (q.x + u.y, q.y - u.x)
@@ -1240,8 +1244,8 @@ adj_mfma_kernel(LegArgs g, long long grp_ts, const R *__restrict__ xs,
               vn = G[(row + rn) * stride + m];
               if (rs >= 0) vs = G[(row + rs) * stride + m];
             }
-            S[((bb * 2 + qu) * 2 + 0) * 64 + lane] = vn;
-            S[((bb * 2 + qu) * 2 + 1) * 64 + lane] = vs;
+            S[((qu * 2 + 0) * 8 + bb) * SP + lane] = vn;
+            S[((qu * 2 + 1) * 8 + bb) * SP + lane] = vs;
           }
         }
         wave_sync();
@@ -1253,8 +1257,8 @@ adj_mfma_kernel(LegArgs g, long long grp_ts, const R *__restrict__ xs,
 #pragma unroll
           for (int cq = 0; cq < 2; ++cq) {
             const int bb = 4 * cq + (bj >> 2);
-            const E qv = Sf[2 * (((bb * 2 + 0) * 2 + side) * 64 + pl) + comp];
-            const E uv = Sf[2 * (((bb * 2 + 1) * 2 + side) * 64 + pl) + 1 - comp];
+            const E qv = Sf[2 * (((0 * 2 + side) * 8 + bb) * SP + pl) + comp];
+            const E uv = Sf[2 * (((1 * 2 + side) * 8 + bb) * SP + pl) + 1 - comp];
             Bz[sc * CT + 2 * h + cq] = minus ? qv - uv : qv + uv;
           }
         }
