@@ -1278,21 +1278,26 @@ adj_mfma_kernel(LegArgs g, long long grp_ts, const R *__restrict__ xs,
     s1 = in ? s[2 * si + 1] : (R)0;
   };
   // The coefficients of chunk k + 2 are loaded from global memory during
-  // chunk k and staged in the third LDS buffer, so no chunk waits on them.
+  // chunk k (between its recurrence and its products) and staged in the
+  // third LDS buffer, so no chunk waits on them.
+  // The loads go to registers as they are (from a valid, clamped index) and
+  // are first used by store_coef after the chunk's products, so the chunk
+  // never waits on them.
   const int nch = (lmax - lb0) / kMaCH + 1;
-  R ca = 0, cb = 0;
-  E cn = 0;
+  R ca = 0, cb = 0, cr = 0;
+  bool cin = false;
   auto load_coef = [&](int k) {
     const int l = lb0 + k * kMaCH + (int)threadIdx.x;
-    const bool in = l <= lmax;
-    ca = in ? cf[NCF * l] : (R)0;
-    cb = (SPIN == 2 && in) ? cf[NCF * l + 1] : (R)0;
-    cn = in ? (E)(SPIN == 0 ? cf[2 * l + 1] : 0.5 * cf[3 * l + 2]) : (E)0;
+    cin = l <= lmax;
+    const int lc = cin ? l : lmax;
+    ca = cf[NCF * lc];
+    if (SPIN == 2) cb = cf[NCF * lc + 1];
+    cr = cf[NCF * lc + NCF - 1];
   };
   auto store_coef = [&](int buf) {
-    cA[buf][threadIdx.x] = ca;
-    cB[buf][threadIdx.x] = cb;
-    cN[buf][threadIdx.x] = cn;
+    cA[buf][threadIdx.x] = cin ? ca : (R)0;
+    cB[buf][threadIdx.x] = cin ? cb : (R)0;
+    cN[buf][threadIdx.x] = cin ? (E)(SPIN == 0 ? cr : 0.5 * cr) : (E)0;
   };
   if (threadIdx.x < kMaCH) {
     load_coef(0);
@@ -1304,7 +1309,6 @@ adj_mfma_kernel(LegArgs g, long long grp_ts, const R *__restrict__ xs,
   for (int k = 0; k < nch; ++k) {
     const int lb = lb0 + k * kMaCH;
     const int bcur = k % 3;
-    if (threadIdx.x < kMaCH) load_coef(k + 2);
     // recurrences of the lane's ring pair over the chunk -> tile
     const R *ka = cA[bcur];
     const R *kb = cB[bcur];
@@ -1353,6 +1357,10 @@ adj_mfma_kernel(LegArgs g, long long grp_ts, const R *__restrict__ xs,
       }
     }
     wave_sync();
+    // coefficients of chunk k + 2, issued after the recurrence (whose start
+    // values of an injection chunk are loaded first) and stored after the
+    // products
+    if (threadIdx.x < kMaCH) load_coef(k + 2);
     // products: D[par][ct] over the wave's ring pairs
     V4 D[2][CT];
 #pragma unroll
