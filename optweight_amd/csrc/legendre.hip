@@ -1,5 +1,7 @@
 // Legendre transforms between alm (triangular, m-major) and ring
-// coefficients G[row][m] for spin 0 and spin 2 (HEALPix convention).
+// coefficients for spin 0 and spin 2 (HEALPix convention): the synthesis
+// writes G[row][m], the adjoint reads the m-blocked pair-major layout of
+// coef_pair_at.
 //
 // Functions: slam_lm(x) = c_l mu_l with the normalised three-term recurrence
 //     mu_l = (A'_l x + B'_l) mu_{l-1} - mu_{l-2}
@@ -55,6 +57,14 @@ struct LegArgs {
 template <typename T>
 __device__ __forceinline__ cplx_t<T> cfma(T a, cplx_t<T> b, cplx_t<T> c) {
   return {fma(a, b.x, c.x), fma(a, b.y, c.y)};
+}
+
+// The (north, south) ring coefficients of pair ip at multipole m in the
+// adjoint's m-blocked pair-major layout (coef_pair_at).
+template <typename T>
+__device__ __forceinline__ const cplx_t<T> *gp_at(const cplx_t<T> *G, long long row, int m,
+                                                  long long ip, const LegArgs &g) {
+  return G + coef_pair_at<T>(row, m, ip, g.mmax, g.npairs);
 }
 
 // ------------------------------------------------------------ seed finder
@@ -362,7 +372,8 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 // Workgroups go to the 8 XCDs (each with its own L2) round-robin by linear
 // block index. Blocks of neighbouring m share the cache lines of the ring
-// coefficients G[row][m] (16 complex64 values of m per 128-byte line), so
+// coefficients (16 complex64 values of m per 128-byte line of G[row][m] in
+// the synthesis, 4 m per 64-byte segment of a pair in the adjoint), so
 // the (x, m) of a block are chosen such that every XCD runs 16 consecutive
 // m in a row (all x of one m together), while each XCD still gets m from the
 // whole range (the work per m falls with m). m beyond the last full group of
@@ -797,10 +808,10 @@ adj0_kernel(LegArgs g, AdjItems it, const R *__restrict__ xs, const int *__restr
 #pragma unroll
           for (int b = 0; b < NB; ++b) {
             if (b < nt) {
-              const long long t = t0 + b;
-              const cplx_t<T> gn = G[(t * g.nrings + rn) * stride + m];
+              const cplx_t<T> *gp = gp_at<T>(G, t0 + b, m, ip, g);
+              const cplx_t<T> gn = gp[0];
               if (rs >= 0) {
-                const cplx_t<T> gs = G[(t * g.nrings + rs) * stride + m];
+                const cplx_t<T> gs = gp[1];
                 He[k][b] = {gn.x + gs.x, gn.y + gs.y};
                 Ho[k][b] = {gn.x - gs.x, gn.y - gs.y};
               } else {
@@ -948,13 +959,13 @@ adj2_kernel(LegArgs g, AdjItems it, const R *__restrict__ xs, const int *__restr
 #pragma unroll
           for (int b = 0; b < NB; ++b) {
             if (b < nt) {
-              const cplx_t<T> *Gq = G + ((t0 + b) * 2 * g.nrings) * stride;
-              const cplx_t<T> *Gu = Gq + g.nrings * stride;
-              const cplx_t<T> qn = Gq[rn * stride + m], un = Gu[rn * stride + m];
+              const cplx_t<T> *Gq = gp_at<T>(G, (t0 + b) * 2, m, ip, g);
+              const cplx_t<T> *Gu = gp_at<T>(G, (t0 + b) * 2 + 1, m, ip, g);
+              const cplx_t<T> qn = Gq[0], un = Gu[0];
               ZpN[k][b] = {qn.x - un.y, qn.y + un.x};
               ZmN[k][b] = {qn.x + un.y, qn.y - un.x};
               if (rs >= 0) {
-                const cplx_t<T> qs = Gq[rs * stride + m], us = Gu[rs * stride + m];
+                const cplx_t<T> qs = Gq[1], us = Gu[1];
                 ZpS[k][b] = {qs.x - us.y, qs.y + us.x};
                 ZmS[k][b] = {qs.x + us.y, qs.y - us.x};
               }
@@ -1180,9 +1191,9 @@ adj_mfma_kernel(LegArgs g, long long grp_ts, const R *__restrict__ xs,
   if (lb0 > lmax) return;  // block-uniform
 
   // B operands: weights of the ring pairs of this wave (lane j + 16k).
-  // The ring coefficients G[row][m] of one m sit one per cache line, so the
-  // wave first gathers them as whole complex values (one 8-byte load per
-  // (pair, ring, transform[, Q/U]) and lane) into its LDS tile, which the
+  // The wave first gathers the ring coefficients of its pairs at this m
+  // (both rings of a pair side by side in the m-blocked pair-major layout:
+  // one load per (transform[, Q/U]) and lane) into its LDS tile, which the
   // chunks use only later, and builds the operands from there.
   const int bj = lane & 15, bk = lane >> 4;
   E Bz[32 * CT];
@@ -1200,9 +1211,9 @@ adj_mfma_kernel(LegArgs g, long long grp_ts, const R *__restrict__ xs,
           const int b = 8 * ct + bb;
           C vn = {0, 0}, vs = {0, 0};
           if (rn >= 0 && b < nt) {
-            const long long t = t0 + b;
-            vn = G[(t * g.nrings + rn) * stride + m];
-            if (rs >= 0) vs = G[(t * g.nrings + rs) * stride + m];
+            const C *gp = gp_at<E>(G, t0 + b, m, ip, g);
+            vn = gp[0];
+            if (rs >= 0) vs = gp[1];
           }
           S[(bb * 2 + 0) * SP + lane] = vn;
           S[(bb * 2 + 1) * SP + lane] = vs;
@@ -1240,9 +1251,9 @@ adj_mfma_kernel(LegArgs g, long long grp_ts, const R *__restrict__ xs,
           for (int qu = 0; qu < 2; ++qu) {
             C vn = {0, 0}, vs = {0, 0};
             if (rn >= 0 && b < nt) {
-              const long long row = (t0 + b) * 2 * g.nrings + qu * g.nrings;
-              vn = G[(row + rn) * stride + m];
-              if (rs >= 0) vs = G[(row + rs) * stride + m];
+              const C *gp = gp_at<E>(G, (t0 + b) * 2 + qu, m, ip, g);
+              vn = gp[0];
+              if (rs >= 0) vs = gp[1];
             }
             S[((qu * 2 + 0) * 8 + bb) * SP + lane] = vn;
             S[((qu * 2 + 1) * 8 + bb) * SP + lane] = vs;

@@ -1,6 +1,10 @@
-// Ring Fourier transforms between the ring coefficients G[row][m]
-// (row = (transform * ncomp + comp) * nrings + ring, m = 0..mmax) and
-// ring-major real maps.
+// Ring Fourier transforms between ring coefficients and ring-major real
+// maps. The synthesis reads G[row][m] (row = (transform * ncomp + comp) *
+// nrings + ring, m = 0..mmax, as the Legendre synthesis writes it); the
+// analysis (adjoint and sandwich modes) writes them for the Legendre adjoint
+// in m-blocked pair-major order (coef_pair_at): a workgroup holding the two
+// rings of one pair writes whole 64-byte segments, and the adjoint's lanes
+// (one pair each) read both rings of their pair with one access.
 //
 // One workgroup transforms two real rings of equal length N packed as one
 // complex sequence (z = a + i b). Any N is handled with Bluestein's
@@ -341,7 +345,9 @@ struct RingArgs {
   const int *ring_logm;
   const long long *chirp_off;
   const long long *bhat_off;
+  const int *ring_slot;  // 2 pair + side of each ring
   long long nrings;
+  long long npairs;
   long long npix;
   int mmax;
   int log_mmax_fft;
@@ -495,8 +501,15 @@ ring_kernel(RingArgs a, const cplx_t<T> *twiddle, const cplx_t<T> *chirp,
       }
     }
     block_sync<GB>();
-    cplx_t<T> *Ga = Gout + G_row_a * (mmax + 1);
-    cplx_t<T> *Gb = Gout + G_row_b * (mmax + 1);
+    // m-blocked pair-major output (coef_pair_at); the two rings of a mirror
+    // pair sit side by side: one 16-byte (float) or 32-byte (double) store per m
+    const int sa = a.ring_slot[ra], sb = rb >= 0 ? a.ring_slot[rb] : 0;
+    const bool joint = rb >= 0 && sb == sa + 1 && (sa & 1) == 0;
+    cplx_t<T> *Ga = Gout + coef_pair_at<T>(tc, 0, sa >> 1, mmax, a.npairs) + (sa & 1);
+    cplx_t<T> *Gb = Gout + coef_pair_at<T>(tc, 0, sb >> 1, mmax, a.npairs) + (sb & 1);
+    constexpr int MB = coef_mb<T>();
+    const long long gp_blk = 2 * MB * a.npairs;  // elements per m block
+    typedef T T4 __attribute__((ext_vector_type(4)));
     for (int m = threadIdx.x; m <= mmax; m += blockDim.x) {
       const long long k = m % N;
       const long long kk = (N - k) % N;
@@ -514,14 +527,17 @@ ring_kernel(RingArgs a, const cplx_t<T> *twiddle, const cplx_t<T> *chirp,
         sincos_t(-(T)m * pa, &sn, &cs);
         xa = cmul<T>(xa, cplx_t<T>{cs, sn});
       }
-      Ga[m] = xa;
-      if (rb >= 0) {
-        if (m > 0 && pb != (T)0) {
-          T sn, cs;
-          sincos_t(-(T)m * pb, &sn, &cs);
-          xb = cmul<T>(xb, cplx_t<T>{cs, sn});
-        }
-        Gb[m] = xb;
+      if (rb >= 0 && m > 0 && pb != (T)0) {
+        T sn, cs;
+        sincos_t(-(T)m * pb, &sn, &cs);
+        xb = cmul<T>(xb, cplx_t<T>{cs, sn});
+      }
+      const long long om = (m / MB) * gp_blk + 2 * (m % MB);
+      if (joint) {
+        *reinterpret_cast<T4 *>(Ga + om) = T4{xa.x, xa.y, xb.x, xb.y};
+      } else {
+        Ga[om] = xa;
+        if (rb >= 0) Gb[om] = xb;
       }
     }
   }
@@ -580,7 +596,9 @@ RingArgs ring_args(owmi_plan_impl *p, int ncomp, int weighted) {
   a.ring_logm = p->ring_logm;
   a.chirp_off = p->ring_chirp_off;
   a.bhat_off = p->ring_bhat_off;
+  a.ring_slot = p->ring_slot;
   a.nrings = p->nrings;
+  a.npairs = p->npairs;
   a.npix = p->npix;
   a.mmax = p->mmax;
   a.log_mmax_fft = p->log2_mmax_fft;

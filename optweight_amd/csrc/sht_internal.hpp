@@ -46,6 +46,28 @@ struct PlanTables {
   bool ready = false;
 };
 
+// Layout of the ring coefficients that the analysis FFTs write for the
+// Legendre adjoint: row = transform * ncomp + component, the m axis cut into
+// blocks of coef_mb<T>() multipoles, and for one (row, m block, ring pair) the
+// (north, south) values of its coef_mb m in one 64-byte segment:
+//   element (((row * nmb + m / MB) * npairs + pair) * MB + m % MB) * 2 + side,
+// nmb = ceil((mmax + 1) / MB). An FFT workgroup (one pair, threads over m)
+// writes whole segments; an adjoint wave (lanes over pairs, one m) reads
+// both rings of a pair with one access per lane, two lanes per 128-byte line.
+// (Whole-line blocks make the adjoint's reads twice as scattered; no m blocks
+// at all make the FFT's writes 16-byte pieces of lines; both measured slower.)
+template <typename T> constexpr int coef_mb() { return 64 / (4 * (int)sizeof(T)); }
+template <typename T>
+__host__ __device__ inline long long coef_nmb(int mmax) {
+  return (mmax + coef_mb<T>()) / coef_mb<T>();
+}
+template <typename T>
+__host__ __device__ inline long long coef_pair_at(long long row, int m, long long pair, int mmax,
+                                                  long long npairs) {
+  constexpr int MB = coef_mb<T>();
+  return (((row * coef_nmb<T>(mmax) + m / MB) * npairs + pair) * MB + m % MB) * 2;
+}
+
 struct owmi_plan_impl {
   int device = 0;
   int lmax = 0, mmax = 0;
@@ -70,6 +92,7 @@ struct owmi_plan_impl {
   long long nfft_jobs = 0;
   int *fft_ring_a = nullptr;
   int *fft_ring_b = nullptr;  // -1 when the job holds one ring
+  int *ring_slot = nullptr;   // 2 pair + side of each ring (side 0: north, 1: south)
 
   PlanTables<double> t64;
   PlanTables<float> t32;
@@ -86,6 +109,11 @@ struct owmi_plan_impl {
   void *work3 = nullptr;
   size_t work3_bytes = 0;
   void *ensure_work3(size_t bytes);
+  // pair-major ring coefficients of the sandwich's analysis half (the
+  // synthesis half reads the ring-major ones from work)
+  void *work4 = nullptr;
+  size_t work4_bytes = 0;
+  void *ensure_work4(size_t bytes);
   // Workspaces outgrown by a larger call stay allocated until the plan is
   // destroyed: kernels recorded into a HIP graph by an earlier call keep
   // their addresses.

@@ -119,6 +119,16 @@ void *owmi_plan_impl::ensure_work2(size_t bytes) {
   return work2;
 }
 
+void *owmi_plan_impl::ensure_work4(size_t bytes) {
+  if (bytes <= work4_bytes) return work4;
+  if (work4) retired.push_back(work4);
+  work4 = nullptr;
+  work4_bytes = 0;
+  OWMI_HIP(hipMalloc(&work4, bytes));
+  work4_bytes = bytes;
+  return work4;
+}
+
 void *owmi_plan_impl::ensure_work3(size_t bytes) {
   if (bytes <= work3_bytes) return work3;
   if (work3) retired.push_back(work3);
@@ -575,6 +585,18 @@ int create_plan(owmi_sht_plan **out, long long nrings, const double *theta,
       p.ring_bhat_off = upload(ring_boff);
       p.fft_ring_a = upload(ja);
       p.fft_ring_b = upload(jb);
+      {
+        // slot of each ring in the pair-major coefficients: every ring is
+        // the north or the south ring of exactly one pair
+        std::vector<int> slot(nrings, -1);
+        for (long long k = 0; k < p.npairs; ++k) {
+          slot[p.h_pair_n[k]] = (int)(2 * k);
+          if (p.h_pair_s[k] >= 0) slot[p.h_pair_s[k]] = (int)(2 * k + 1);
+        }
+        for (long long r = 0; r < nrings; ++r)
+          OWMI_REQUIRE(slot[r] >= 0, "internal: a ring belongs to no pair");
+        p.ring_slot = upload(slot);
+      }
 
       HostTables ht;
       build_host_tables(p, thn, xs, ht);
@@ -638,7 +660,8 @@ int owmi_sht_plan_destroy(owmi_sht_plan *plan) {
   free_tables(p.t64);
   free_tables(p.t32);
   void *ptrs[] = {p.pair_n, p.pair_s, p.nphi, p.offsets, p.ring_logm, p.ring_chirp_off,
-                  p.ring_bhat_off, p.fft_ring_a, p.fft_ring_b, p.work, p.work2, p.work3};
+                  p.ring_bhat_off, p.fft_ring_a, p.fft_ring_b, p.ring_slot, p.work, p.work2,
+                  p.work3, p.work4};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   for (void *q : p.retired) (void)hipFree(q);
@@ -696,6 +719,17 @@ int owmi_sht_plan_tune_reset(owmi_sht_plan *plan) {
 
 }  // extern "C"
 
+// Bytes of ring coefficients for `rows` (transform, component) rows, large
+// enough for the ring-major layout (nrings rings, mmax + 1 multipoles) and
+// the m-blocked pair-major one (2 npairs slots, more than nrings when some
+// pairs have no south ring; m rounded up to whole blocks).
+template <typename T>
+static size_t coef_bytes(const owmi_plan_impl &p, long long rows) {
+  const long long ring_major = p.nrings * (p.mmax + 1);
+  const long long pair_major = 2 * p.npairs * coef_nmb<T>(p.mmax) * coef_mb<T>();
+  return (size_t)rows * std::max(ring_major, pair_major) * sizeof(cplx_t<T>);
+}
+
 template <typename T>
 static void check_fft_fits(owmi_plan_impl &p) {
   // M / 16 threads per ring workgroup, at most 1024; the buffer lives in
@@ -717,8 +751,7 @@ static int do_synthesis(owmi_sht_plan *plan, int spin, const void *alm, long lon
     check_fft_fits<T>(p);
     if (ntrans == 0) return;
     int nc = spin == 0 ? 1 : 2;
-    size_t rows = (size_t)ntrans * nc * p.nrings;
-    auto *G = static_cast<cplx_t<T> *>(p.ensure_work(rows * (p.mmax + 1) * sizeof(cplx_t<T>)));
+    auto *G = static_cast<cplx_t<T> *>(p.ensure_work(coef_bytes<T>(p, ntrans * nc)));
     hipStream_t st = as_stream(stream);
     launch_synthesis<T>(&p, spin, static_cast<const cplx_t<T> *>(alm), alm_ts, G, ntrans, st);
     launch_ring_synth<T>(&p, G, static_cast<T *>(map), map_ts, nc, ntrans, weighted, st);
@@ -737,8 +770,7 @@ static int do_adjoint(owmi_sht_plan *plan, int spin, const void *map, long long 
     check_fft_fits<T>(p);
     if (ntrans == 0) return;
     int nc = spin == 0 ? 1 : 2;
-    size_t rows = (size_t)ntrans * nc * p.nrings;
-    auto *G = static_cast<cplx_t<T> *>(p.ensure_work(rows * (p.mmax + 1) * sizeof(cplx_t<T>)));
+    auto *G = static_cast<cplx_t<T> *>(p.ensure_work(coef_bytes<T>(p, ntrans * nc)));
     hipStream_t st = as_stream(stream);
     launch_ring_adjoint<T>(&p, static_cast<const T *>(map), map_ts, G, nc, ntrans, weighted, st);
     launch_adjoint<T>(&p, spin, G, static_cast<cplx_t<T> *>(alm), alm_ts, ntrans, st);
@@ -757,12 +789,12 @@ static int do_sandwich(owmi_sht_plan *plan, int spin, const void *alm_in, long l
     check_fft_fits<T>(p);
     if (ntrans == 0) return;
     int nc = spin == 0 ? 1 : 2;
-    size_t rows = (size_t)ntrans * nc * p.nrings;
-    auto *G = static_cast<cplx_t<T> *>(p.ensure_work(rows * (p.mmax + 1) * sizeof(cplx_t<T>)));
+    auto *G = static_cast<cplx_t<T> *>(p.ensure_work(coef_bytes<T>(p, ntrans * nc)));
+    auto *GP = static_cast<cplx_t<T> *>(p.ensure_work4(coef_bytes<T>(p, ntrans * nc)));
     hipStream_t st = as_stream(stream);
     launch_synthesis<T>(&p, spin, static_cast<const cplx_t<T> *>(alm_in), in_ts, G, ntrans, st);
-    launch_ring_sandwich<T>(&p, G, G, static_cast<const T *>(diag), diag_ts, nc, ntrans, st);
-    launch_adjoint<T>(&p, spin, G, static_cast<cplx_t<T> *>(alm_out), out_ts, ntrans, st);
+    launch_ring_sandwich<T>(&p, G, GP, static_cast<const T *>(diag), diag_ts, nc, ntrans, st);
+    launch_adjoint<T>(&p, spin, GP, static_cast<cplx_t<T> *>(alm_out), out_ts, ntrans, st);
   });
 }
 
@@ -776,8 +808,7 @@ static int do_legendre(owmi_sht_plan *plan, int spin, int direction, void *alm,
     OWMI_REQUIRE(direction == 0 || direction == 1, "direction must be 0 or 1");
     if (ntrans == 0) return;
     int nc = spin == 0 ? 1 : 2;
-    size_t rows = (size_t)ntrans * nc * p.nrings;
-    auto *G = static_cast<cplx_t<T> *>(p.ensure_work(rows * (p.mmax + 1) * sizeof(cplx_t<T>)));
+    auto *G = static_cast<cplx_t<T> *>(p.ensure_work(coef_bytes<T>(p, ntrans * nc)));
     hipStream_t st = as_stream(stream);
     if (direction == 0)
       launch_synthesis<T>(&p, spin, static_cast<const cplx_t<T> *>(alm), alm_ts, G, ntrans, st);
