@@ -1,7 +1,7 @@
 // Legendre transforms between alm (triangular, m-major) and ring
 // coefficients for spin 0 and spin 2 (HEALPix convention): the synthesis
-// writes G[row][m], the adjoint reads the m-blocked pair-major layout of
-// coef_pair_at.
+// writes the pair-major layout of coef_syn_at, the adjoint reads the
+// m-blocked pair-major layout of coef_pair_at.
 //
 // Functions: slam_lm(x) = c_l mu_l with the normalised three-term recurrence
 //     mu_l = (A'_l x + B'_l) mu_{l-1} - mu_{l-2}
@@ -65,6 +65,17 @@ template <typename T>
 __device__ __forceinline__ const cplx_t<T> *gp_at(const cplx_t<T> *G, long long row, int m,
                                                   long long ip, const LegArgs &g) {
   return G + coef_pair_at<T>(row, m, ip, g.mmax, g.npairs);
+}
+
+// Stores the (north, south) values of one pair at Gp (coef_syn_at): one 16-
+// or 32-byte store, or the north value alone for a pair without a south ring.
+template <typename T>
+__device__ __forceinline__ void store_pair(cplx_t<T> *Gp, cplx_t<T> n, cplx_t<T> s, bool has_s) {
+  typedef T T4 __attribute__((ext_vector_type(4)));
+  if (has_s)
+    *reinterpret_cast<T4 *>(Gp) = T4{n.x, n.y, s.x, s.y};
+  else
+    Gp[0] = n;
 }
 
 // ------------------------------------------------------------ seed finder
@@ -208,15 +219,13 @@ synth0_kernel(LegArgs g, const R *__restrict__ xs, const int *__restrict__ lfs,
     }
   }
   if (!active) return;
-  const int rn = g.pair_n[ip], rs = g.pair_s[ip];
-  const long long stride = g.mmax + 1;
+  const bool has_s = g.pair_s[ip] >= 0;
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
-    if (b < nt) {
-      const long long t = t0 + b;
-      G[(t * g.nrings + rn) * stride + m] = {E[b].x + O[b].x, E[b].y + O[b].y};
-      if (rs >= 0) G[(t * g.nrings + rs) * stride + m] = {E[b].x - O[b].x, E[b].y - O[b].y};
-    }
+    if (b < nt)
+      store_pair<T>(G + coef_syn_at(t0 + b, m, ip, g.mmax, g.npairs),
+                    {E[b].x + O[b].x, E[b].y + O[b].y}, {E[b].x - O[b].x, E[b].y - O[b].y},
+                    has_s);
   }
 }
 
@@ -323,26 +332,21 @@ synth2_kernel(LegArgs g, const R *__restrict__ xs, const int *__restrict__ lfps,
     }
   }
   if (!active) return;
-  const int rn = g.pair_n[ip], rs = g.pair_s[ip];
-  const long long stride = g.mmax + 1;
+  const bool has_s = g.pair_s[ip] >= 0;
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
     if (b >= nt) continue;
     const long long t = t0 + b;
-    cplx_t<T> *Gq = G + (t * 2 * g.nrings) * stride;
-    cplx_t<T> *Gu = Gq + g.nrings * stride;
     // North: A+ = sum P 2lam, A- = sum Mm (-2)lam; Q = A+ + A-, U = -i (A+ - A-).
     const cplx_t<T> ApN = {PPe[b].x + PPo[b].x, PPe[b].y + PPo[b].y};
     const cplx_t<T> AmN = {MMe[b].x + MMo[b].x, MMe[b].y + MMo[b].y};
-    Gq[rn * stride + m] = {ApN.x + AmN.x, ApN.y + AmN.y};
-    Gu[rn * stride + m] = {ApN.y - AmN.y, -(ApN.x - AmN.x)};
-    if (rs >= 0) {
-      // South: A+ = sum (-1)^(l+m) P (-2)lam, A- = sum (-1)^(l+m) Mm 2lam.
-      const cplx_t<T> ApS = {PMe[b].x - PMo[b].x, PMe[b].y - PMo[b].y};
-      const cplx_t<T> AmS = {MPe[b].x - MPo[b].x, MPe[b].y - MPo[b].y};
-      Gq[rs * stride + m] = {ApS.x + AmS.x, ApS.y + AmS.y};
-      Gu[rs * stride + m] = {ApS.y - AmS.y, -(ApS.x - AmS.x)};
-    }
+    // South: A+ = sum (-1)^(l+m) P (-2)lam, A- = sum (-1)^(l+m) Mm 2lam.
+    const cplx_t<T> ApS = {PMe[b].x - PMo[b].x, PMe[b].y - PMo[b].y};
+    const cplx_t<T> AmS = {MPe[b].x - MPo[b].x, MPe[b].y - MPo[b].y};
+    store_pair<T>(G + coef_syn_at(2 * t, m, ip, g.mmax, g.npairs),
+                  {ApN.x + AmN.x, ApN.y + AmN.y}, {ApS.x + AmS.x, ApS.y + AmS.y}, has_s);
+    store_pair<T>(G + coef_syn_at(2 * t + 1, m, ip, g.mmax, g.npairs),
+                  {ApN.y - AmN.y, -(ApN.x - AmN.x)}, {ApS.y - AmS.y, -(ApS.x - AmS.x)}, has_s);
   }
 }
 
@@ -371,9 +375,8 @@ synth2_kernel(LegArgs g, const R *__restrict__ xs, const int *__restrict__ lfps,
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 // Workgroups go to the 8 XCDs (each with its own L2) round-robin by linear
-// block index. Blocks of neighbouring m share the cache lines of the ring
-// coefficients (16 complex64 values of m per 128-byte line of G[row][m] in
-// the synthesis, 4 m per 64-byte segment of a pair in the adjoint), so
+// block index. Blocks of neighbouring m of the adjoint share the cache
+// lines of the ring coefficients (4 m per 64-byte segment of a pair), so
 // the (x, m) of a block are chosen such that every XCD runs 16 consecutive
 // m in a row (all x of one m together), while each XCD still gets m from the
 // whole range (the work per m falls with m). m beyond the last full group of
@@ -662,7 +665,6 @@ synth_mfma_kernel(LegArgs g, const R *__restrict__ xs, const int *__restrict__ l
   // difference with the sign of (-1)^(lmin - m)
   const float sgn0 = ((lmin - m) & 1) ? -1.0f : 1.0f;
   const long long wbase = (long long)bx * blockDim.x + (threadIdx.x & ~63);
-  const long long stride = g.mmax + 1;
   const bool odd = jc & 1;
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
@@ -684,7 +686,7 @@ synth_mfma_kernel(LegArgs g, const R *__restrict__ xs, const int *__restrict__ l
         const int ring = odd ? g.pair_s[pp] : g.pair_n[pp];
         if (ring < 0) continue;
         const float2 val = odd ? float2{recv, vs} : float2{vn, recv};
-        G[(t * g.nrings + ring) * stride + m] = val;
+        G[coef_syn_at(t, m, pp, g.mmax, g.npairs) + odd] = val;
       } else {
         // column c = jc % 4 of transform tt of each set: q component c % 2
         // and u component 1 - c % 2 of ring (c < 2: north, else south); the
@@ -701,11 +703,10 @@ synth_mfma_kernel(LegArgs g, const R *__restrict__ xs, const int *__restrict__ l
           const int b = 8 * set + tt;
           if (ring < 0 || b >= nt) continue;
           const long long t = t0 + b;
-          const long long qi = (t * 2 * g.nrings + ring) * stride + m;
-          if (!odd)
-            G[qi] = float2{q, recv};                    // Q = (q of c, q of c + 1)
-          else
-            G[qi + g.nrings * stride] = float2{u, recv};  // U = (u of c + 1, u of c)
+          // side c / 2 of the pair in row 2t (Q) or 2t + 1 (U)
+          const long long qi = coef_syn_at(2 * t + odd, m, pp, g.mmax, g.npairs) + (c >> 1);
+          G[qi] = odd ? float2{u, recv}    // U = (u of c + 1, u of c)
+                      : float2{q, recv};   // Q = (q of c, q of c + 1)
         }
       }
     }

@@ -46,26 +46,41 @@ struct PlanTables {
   bool ready = false;
 };
 
-// Layout of the ring coefficients that the analysis FFTs write for the
-// Legendre adjoint: row = transform * ncomp + component, the m axis cut into
-// blocks of coef_mb<T>() multipoles, and for one (row, m block, ring pair) the
-// (north, south) values of its coef_mb m in one 64-byte segment:
-//   element (((row * nmb + m / MB) * npairs + pair) * MB + m % MB) * 2 + side,
-// nmb = ceil((mmax + 1) / MB). An FFT workgroup (one pair, threads over m)
-// writes whole segments; an adjoint wave (lanes over pairs, one m) reads
-// both rings of a pair with one access per lane, two lanes per 128-byte line.
-// (Whole-line blocks make the adjoint's reads twice as scattered; no m blocks
-// at all make the FFT's writes 16-byte pieces of lines; both measured slower.)
+// Layouts of the ring coefficients between the Legendre and the FFT stages,
+// row = transform * ncomp + component, side 0 / 1 the north / south ring of
+// a ring pair:
+//  * synthesis (the Legendre synthesis writes, the FFTs read), pair-major:
+//      element ((row * (mmax + 1) + m) * npairs + pair) * 2 + side.
+//    A Legendre wave (lanes over pairs, one m) writes whole lines; an FFT
+//    workgroup (one pair, threads over m) reads both rings of its pair with
+//    one access per m.
+//  * adjoint (the FFTs write, the Legendre adjoint reads), m-blocked
+//    pair-major with blocks of MB = coef_mb<T>() multipoles:
+//      element (((row * nmb + m / MB) * npairs + pair) * MB + m % MB) * 2 + side,
+//    nmb = ceil((mmax + 1) / MB): the MB values of a pair fill one 64-byte
+//    segment, which an FFT workgroup writes whole; an adjoint wave reads both
+//    rings of a pair with one access per lane, two lanes per 128-byte line.
+//    (Whole-line blocks make the adjoint's reads twice as scattered; no m
+//    blocks make the FFT's writes 16-byte pieces of lines; both measured
+//    slower. The synthesis needs no blocks: its FFT reads of 16 bytes per
+//    line are served by L2, as the workgroups of one XCD take neighbouring
+//    pairs.)
 template <typename T> constexpr int coef_mb() { return 64 / (4 * (int)sizeof(T)); }
 template <typename T>
 __host__ __device__ inline long long coef_nmb(int mmax) {
   return (mmax + coef_mb<T>()) / coef_mb<T>();
 }
+// element index of (row, m, pair, side 0) in the adjoint layout
 template <typename T>
 __host__ __device__ inline long long coef_pair_at(long long row, int m, long long pair, int mmax,
                                                   long long npairs) {
   constexpr int MB = coef_mb<T>();
   return (((row * coef_nmb<T>(mmax) + m / MB) * npairs + pair) * MB + m % MB) * 2;
+}
+// element index of (row, m, pair, side 0) in the synthesis layout
+__host__ __device__ inline long long coef_syn_at(long long row, int m, long long pair, int mmax,
+                                                 long long npairs) {
+  return ((row * (mmax + 1) + m) * npairs + pair) * 2;
 }
 
 struct owmi_plan_impl {

@@ -720,14 +720,12 @@ int owmi_sht_plan_tune_reset(owmi_sht_plan *plan) {
 }  // extern "C"
 
 // Bytes of ring coefficients for `rows` (transform, component) rows, large
-// enough for the ring-major layout (nrings rings, mmax + 1 multipoles) and
-// the m-blocked pair-major one (2 npairs slots, more than nrings when some
-// pairs have no south ring; m rounded up to whole blocks).
+// enough for the synthesis and the adjoint layouts (2 npairs slots per m,
+// the adjoint's m rounded up to whole blocks).
 template <typename T>
 static size_t coef_bytes(const owmi_plan_impl &p, long long rows) {
-  const long long ring_major = p.nrings * (p.mmax + 1);
-  const long long pair_major = 2 * p.npairs * coef_nmb<T>(p.mmax) * coef_mb<T>();
-  return (size_t)rows * std::max(ring_major, pair_major) * sizeof(cplx_t<T>);
+  const long long per_row = 2 * p.npairs * coef_nmb<T>(p.mmax) * coef_mb<T>();
+  return (size_t)rows * per_row * sizeof(cplx_t<T>);
 }
 
 template <typename T>
