@@ -894,7 +894,8 @@ def main():
     log(f'setup {time.perf_counter() - t0:.1f}s')
     tuned = {}
     if args.autotune == 'on':
-        for dt, nb in ((torch.float32, args.batch), (torch.float64, 1)):
+        for dt, nb in ((torch.float32, args.batch), (torch.float64, 1),
+                       (torch.float64, args.batch)):
             res = autotune.tune(prob.minfo, prob.lmax, dt, nb, device=dev)
             tuned[f'{str(dt)[6:]}_batch{nb}'] = {k: res[k] for k in ('params', 'cached',
                                                                         'tune_seconds')}

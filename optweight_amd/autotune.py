@@ -162,6 +162,14 @@ def _candidates(dtype, batch, mfma):
     return cands
 
 
+def _searched(cands):
+    '''The candidates with a choice to make. A parameter with one candidate
+    value at this batch (transforms per thread at batch 1) is left out: the
+    entry also covers larger batches, where the library's own rule applies
+    to it.'''
+    return [(name, values) for name, values in cands if len(values) > 1]
+
+
 def tune(minfo, lmax, dtype, batch, device=None, rings=None, reps=3, use_cache=True,
          force=False):
     '''Tune the plan of (minfo, lmax[, rings]) on ``device`` for ``batch``
@@ -187,7 +195,7 @@ def tune(minfo, lmax, dtype, batch, device=None, rings=None, reps=3, use_cache=T
     best_t = measure()
     for stage in range(3):
         cands = _candidates(dtype, batch, mfma)
-        for name, values in cands:
+        for name, values in _searched(cands):
             times = {}
             for v in values:
                 _set(plan, batch, {name: v})

@@ -1116,6 +1116,26 @@ adj2_kernel(LegArgs g, AdjItems it, const R *__restrict__ xs, const int *__restr
 // LDS after every chunk; blocks of further ring pairs write partial sums
 // that sum_groups_kernel adds in fixed order.
 
+// Copies 16 bytes per lane from global memory (src, per lane) to LDS at
+// dst + 16 lane (dst wave-uniform) without passing through registers
+// (global_load_lds_dwordx4); complete after a wait for vmcnt 0.
+__device__ __forceinline__ void lds_dma16(const void *src, void *dst) {
+  __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void *)dst, 16, 0, 0);
+}
+// s_waitcnt vmcnt(0) / lgkmcnt(0) with the other counters left alone
+constexpr int kWaitVm0 = 0x0F70;
+constexpr int kWaitLgkm0 = 0xC07F;
+
+// Minimum of v over the block with one LDS atomic per wave.
+__device__ __forceinline__ int block_min_waves(int v, int *slot) {
+  v = wave_min(v);
+  if (threadIdx.x == 0) *slot = INT_MAX;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) atomicMin(slot, v);
+  __syncthreads();
+  return *slot;
+}
+
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
@@ -1152,16 +1172,17 @@ adj_mfma_kernel(LegArgs g, long long grp_ts, const R *__restrict__ xs,
   constexpr int NT = (SPIN == 0 ? 8 : 4) * CT;    // transforms per block
   constexpr int NCF = SPIN == 0 ? 2 : 3;    // recurrence coefficients per multipole
   constexpr int TS = SPIN == 0 ? 66 : 130;  // tile row stride in elements
-  // staging of the weights, rows of SP complex values (64 pairs + 2): the
-  // lanes of one staging read fall on distinct banks
-  constexpr int SP = 66;
-  constexpr int TW = SPIN == 0 ? kMaCH * TS : 2 * 2 * 8 * SP * 2;  // tile elements per wave
+  // tile elements per wave (spin 2: room for the staged weights of 8
+  // transforms, 66 pair slots per row)
+  constexpr int TW = SPIN == 0 ? kMaCH * TS : 2 * 2 * 8 * 66 * 2;
   static_assert(TW >= kMaCH * TS, "the tile fits");
   static_assert(TW >= 1024, "the reduction buffer overlays the tile");
   __shared__ __align__(32) E Tall[W * TW];
-  __shared__ R cA[3][kMaCH];      // recurrence coefficient A' per chunk buffer
-  __shared__ R cB[3][kMaCH];      // B' (spin 2)
-  __shared__ E cN[3][kMaCH];      // normalisation of the output
+  // recurrence coefficients per chunk buffer and multipole: (A', B') for
+  // spin 2, (A', c) for spin 0; (B', c) for spin 2 (c: the output's
+  // normalisation), copied from the plan's table without registers
+  __shared__ __align__(16) cplx_t<R> cAB[3][kMaCH];
+  __shared__ __align__(16) cplx_t<R> cBC[SPIN == 2 ? 3 : 1][kMaCH];
   __shared__ int smin;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   E *T = Tall + wave * TW;
@@ -1177,90 +1198,150 @@ adj_mfma_kernel(LegArgs g, long long grp_ts, const R *__restrict__ xs,
   const long long ms = mstart_of(m, lmax);
   const long long stride = g.mmax + 1;
   const long long si = (long long)m * g.npairs + ip;
+  // Loads of inactive lanes read the last pair (their Legendre values stay 0).
+  const long long ipc = active ? ip : g.npairs - 1;
+  const long long sic = (long long)m * g.npairs + ipc;
   const int lfa = active ? lfas[si] : INT_MAX;
   const int lfb = (SPIN == 2 && active) ? lfbs[si] : INT_MAX;
-  const int lmin = block_min(min(lfa, lfb), &smin);
+  const R x = active ? xs[ip] : (R)0;
+  const int wu = __builtin_amdgcn_readfirstlane(wave);
+  // The start values (mu_{lf-1}, mu_lf) of the lane's functions go to LDS
+  // once (no registers held through the kernel) and are read in the chunk
+  // that holds the lane's start multipole.
+  __shared__ __align__(16) cplx_t<R> sSeed[W][SPIN == 0 ? 1 : 2][64];
+  lds_dma16(sas + 2 * sic, &sSeed[wu][0][0]);
+  if (SPIN == 2) lds_dma16(sbs + 2 * sic, &sSeed[wu][SPIN == 2 ? 1 : 0][0]);
+
+  // B operands: weights of the ring pairs of this wave (lane j + 16k).
+  // The wave copies the ring coefficients of its pairs at this m (both rings
+  // of a pair side by side in the m-blocked pair-major layout) from global
+  // memory into its LDS tile, which the chunks use only later, 16 bytes per
+  // lane and (transform[, Q/U][, ring]) without passing through registers,
+  // and builds the operands from there. Staging rows of 64 lanes x 16 bytes
+  // are SR bytes apart (32 bytes of padding: the lanes of one operand read
+  // fall on distinct banks). Rows of transforms beyond ntrans repeat the
+  // last transform (their columns are never stored).
+  constexpr int SR = 1056;
+  static_assert((SPIN == 0 ? 8 : 16) * (sizeof(E) == 4 ? 1 : 2) * SR <= TW * sizeof(E),
+                "staging fits the tile");
+  char *Sb = reinterpret_cast<char *>(Tall + wu * TW);
+  auto stage = [&](int h) {
+    // The copies write LDS outside the LDS queue: the reads of the previous
+    // round's rows must have completed before they are issued.
+    if (h > 0) __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
+    // float: row (qu * 8 + bb) holds (north, south) per lane; double: row
+    // ((qu * 2 + side) * 8 + bb) holds one ring per lane
+#pragma unroll
+    for (int bb = 0; bb < 8; ++bb) {
+      const long long t = t0 + min(8 * h + bb, nt - 1);
+#pragma unroll
+      for (int qu = 0; qu < (SPIN == 0 ? 1 : 2); ++qu) {
+        const C *gp = gp_at<E>(G, SPIN == 0 ? t : 2 * t + qu, m, ipc, g);
+        if (sizeof(E) == 4) {
+          lds_dma16(gp, Sb + (qu * 8 + bb) * SR);
+        } else {
+          lds_dma16(gp, Sb + ((qu * 2 + 0) * 8 + bb) * SR);
+          lds_dma16(gp + 1, Sb + ((qu * 2 + 1) * 8 + bb) * SR);
+        }
+      }
+    }
+  };
+  stage(0);
+  const int lmin = block_min_waves(min(lfa, lfb), &smin);
   const R *cf = coef + NCF * ms;
   C *om = out + grp * grp_ts + t0 * out_ts + ms;
   // chunks start at multipoles with l - m even
   const int lb0 = lmin <= lmax ? lmin - ((lmin - m) & 1) : lmax + 1;
+  // coefficients of chunk k into buffer buf (wave 0, lanes below kMaCH);
+  // multipoles past lmax repeat lmax's (their rows are never stored)
+  auto fetch_coef = [&](int k, int buf) {
+    const int l = min(lb0 + k * kMaCH + lane, lmax);
+    lds_dma16(cf + NCF * l, &cAB[buf][0]);
+    if (SPIN == 2) lds_dma16(cf + NCF * l + 1, &cBC[SPIN == 2 ? buf : 0][0]);
+  };
+  // the first two chunks' coefficients, copied together with the last
+  // staged weights
+  auto fetch_first_coefs = [&]() {
+    if (threadIdx.x < kMaCH) {
+      fetch_coef(0, 0);
+      fetch_coef(1, 1);
+    }
+  };
   for (int b = 0; b < nt; ++b)
     for (int l = m + threadIdx.x; l < lb0; l += blockDim.x) {
       om[b * out_ts + l] = {0, 0};
       if (SPIN == 2) om[b * out_ts + nelem + l] = {0, 0};
     }
-  if (lb0 > lmax) return;  // block-uniform
+  if (lb0 > lmax) {  // block-uniform; no copy into LDS may outlive the block
+    __builtin_amdgcn_s_waitcnt(kWaitVm0);
+    return;
+  }
 
-  // B operands: weights of the ring pairs of this wave (lane j + 16k).
-  // The wave first gathers the ring coefficients of its pairs at this m
-  // (both rings of a pair side by side in the m-blocked pair-major layout:
-  // one load per (transform[, Q/U]) and lane) into its LDS tile, which the
-  // chunks use only later, and builds the operands from there.
   const int bj = lane & 15, bk = lane >> 4;
   E Bz[32 * CT];
   {
-    const int rn = active ? g.pair_n[ip] : -1;
-    const int rs = active ? g.pair_s[ip] : -1;
-    C *S = reinterpret_cast<C *>(T);
+    // The staged slots of a lane beyond the last pair (both rings) and the
+    // south slots of a pair without a south ring (the ring-coefficient
+    // layout leaves them unwritten) are set to 0: a product of 0 with
+    // unwritten memory could be NaN.
+    const bool zero_n = !active, zero_s = !active || g.pair_s[ip] < 0;
+    auto finish_stage = [&]() {
+      __builtin_amdgcn_s_waitcnt(kWaitVm0);
+      if (zero_s) {
+#pragma unroll
+        for (int r = 0; r < (SPIN == 0 ? 8 : 16); ++r) {
+          // r: staging row of one ring side (float: qu * 8 + bb; double: the
+          // (qu, bb) rows of the north ring are r + 8 (r / 8) + {0, 8})
+          if (sizeof(E) == 4) {
+            C *row = reinterpret_cast<C *>(Sb + r * SR);
+            row[2 * lane + 1] = C{0, 0};
+            if (zero_n) row[2 * lane] = C{0, 0};
+          } else {
+            const int rn = r + (r >= 8 ? 8 : 0);
+            reinterpret_cast<C *>(Sb + (rn + 8) * SR)[lane] = C{0, 0};
+            if (zero_n) reinterpret_cast<C *>(Sb + rn * SR)[lane] = C{0, 0};
+          }
+        }
+      }
+      wave_sync();
+    };
+    const E *Sf = reinterpret_cast<const E *>(Sb);
+    // element of (staging row, pair, ring side, component)
+    auto sv = [&](int row, int qu, int pl, int side, int comp) -> E {
+      if (sizeof(E) == 4) return Sf[(qu * 8 + row) * (SR / 4) + pl * 4 + side * 2 + comp];
+      return Sf[((qu * 2 + side) * 8 + row) * (SR / 8) + pl * 2 + comp];
+    };
     if constexpr (SPIN == 0) {
-      // one column tile (8 transforms) at a time: S[(b * 2 + side) * SP + pair]
-      static_assert(8 * 2 * SP * 2 <= TW, "staging fits the tile");
+      // one column tile (8 transforms) at a time
 #pragma unroll
       for (int ct = 0; ct < CT; ++ct) {
-#pragma unroll
-        for (int bb = 0; bb < 8; ++bb) {
-          const int b = 8 * ct + bb;
-          C vn = {0, 0}, vs = {0, 0};
-          if (rn >= 0 && b < nt) {
-            const C *gp = gp_at<E>(G, t0 + b, m, ip, g);
-            vn = gp[0];
-            if (rs >= 0) vs = gp[1];
-          }
-          S[(bb * 2 + 0) * SP + lane] = vn;
-          S[(bb * 2 + 1) * SP + lane] = vs;
-        }
-        wave_sync();
+        if (ct > 0) stage(ct);
+        if (ct == CT - 1) fetch_first_coefs();
+        finish_stage();
         // Bz[(par * 16 + s) * CT + ct]: pair 4s + k, transform 8ct + j / 2
-        const E *Sf = reinterpret_cast<const E *>(S);
 #pragma unroll
         for (int sq = 0; sq < 16; ++sq) {
           const int pl = 4 * sq + bk, bb = bj >> 1, comp = bj & 1;
-          const E gn = Sf[2 * ((bb * 2 + 0) * SP + pl) + comp];
-          const E gs = Sf[2 * ((bb * 2 + 1) * SP + pl) + comp];
+          const E gn = sv(bb, 0, pl, 0, comp);
+          const E gs = sv(bb, 0, pl, 1, comp);
           Bz[(0 * 16 + sq) * CT + ct] = gn + gs;
           Bz[(1 * 16 + sq) * CT + ct] = gn - gs;
         }
         wave_sync();
       }
     } else {
-      // two column tiles (8 transforms) at a time:
-      // S[((qu * 2 + side) * 8 + b) * SP + pair], qu = 0: Q, 1: U
-      static_assert(8 * 2 * 2 * SP * 2 <= TW, "staging fits the tile");
+      // two column tiles (8 transforms) at a time, qu = 0: Q, 1: U
       static_assert(CT % 2 == 0, "spin 2 stages column tiles in pairs");
       // f = 0: (Zp(N), Zm(S)), f = 1: (Zp(S), Zm(N)) for c < 2 / c >= 2, with
       // Zp = q + i u = (q.x - u.y, q.y + u.x), Zm = q - i u = (q.x + u.y, q.y - u.x)
       const int f = bk & 1, c = bj & 3, comp = c & 1;
       const int side = (f == 0) == (c < 2) ? 0 : 1;
       const bool minus = (c < 2) == (comp == 0);
-      const E *Sf = reinterpret_cast<const E *>(S);
 #pragma unroll
       for (int h = 0; h < CT / 2; ++h) {
-#pragma unroll
-        for (int bb = 0; bb < 8; ++bb) {
-          const int b = 8 * h + bb;
-#pragma unroll
-          for (int qu = 0; qu < 2; ++qu) {
-            C vn = {0, 0}, vs = {0, 0};
-            if (rn >= 0 && b < nt) {
-              const C *gp = gp_at<E>(G, (t0 + b) * 2 + qu, m, ip, g);
-              vn = gp[0];
-              if (rs >= 0) vs = gp[1];
-            }
-            S[((qu * 2 + 0) * 8 + bb) * SP + lane] = vn;
-            S[((qu * 2 + 1) * 8 + bb) * SP + lane] = vs;
-          }
-        }
-        wave_sync();
+        if (h > 0) stage(h);
+        if (h == CT / 2 - 1) fetch_first_coefs();
+        finish_stage();
         // Bz[s * CT + ct]: pair 2s + k / 2, function k % 2, transform
         // 4ct + j / 4 (ct = 2h, 2h + 1)
 #pragma unroll
@@ -1269,8 +1350,8 @@ adj_mfma_kernel(LegArgs g, long long grp_ts, const R *__restrict__ xs,
 #pragma unroll
           for (int cq = 0; cq < 2; ++cq) {
             const int bb = 4 * cq + (bj >> 2);
-            const E qv = Sf[2 * (((0 * 2 + side) * 8 + bb) * SP + pl) + comp];
-            const E uv = Sf[2 * (((1 * 2 + side) * 8 + bb) * SP + pl) + 1 - comp];
+            const E qv = sv(bb, 0, pl, side, comp);
+            const E uv = sv(bb, 1, pl, side, 1 - comp);
             Bz[sc * CT + 2 * h + cq] = minus ? qv - uv : qv + uv;
           }
         }
@@ -1279,62 +1360,35 @@ adj_mfma_kernel(LegArgs g, long long grp_ts, const R *__restrict__ xs,
     }
   }
 
-  const R x = active ? xs[ip] : (R)0;
   R a1 = 0, a2 = 0, b1 = 0, b2 = 0;
-  // The start values (mu_{lf-1}, mu_lf) are read from global memory only in
-  // the chunk that holds the lane's start multipole, not kept in registers
-  // through the kernel.
-  auto seeds = [&](const R *s, int lf, int lb, R &s0, R &s1) {
+  auto seeds = [&](int fn, int lf, int lb, R &s0, R &s1) {
     const bool in = active && lf >= lb && lf < lb + kMaCH;
-    s0 = in ? s[2 * si] : (R)0;
-    s1 = in ? s[2 * si + 1] : (R)0;
+    const cplx_t<R> v = sSeed[wu][fn][lane];
+    s0 = in ? v.x : (R)0;
+    s1 = in ? v.y : (R)0;
   };
-  // The coefficients of chunk k + 2 are loaded from global memory during
-  // chunk k (between its recurrence and its products) and staged in the
-  // third LDS buffer, so no chunk waits on them.
-  // The loads go to registers as they are (from a valid, clamped index) and
-  // are first used by store_coef after the chunk's products, so the chunk
-  // never waits on them.
+  // The coefficients of chunk k + 2 are copied during chunk k (after its
+  // recurrence) into the buffer chunk k - 1 used; the barriers of chunk k
+  // complete the copy.
   const int nch = (lmax - lb0) / kMaCH + 1;
-  R ca = 0, cb = 0, cr = 0;
-  bool cin = false;
-  auto load_coef = [&](int k) {
-    const int l = lb0 + k * kMaCH + (int)threadIdx.x;
-    cin = l <= lmax;
-    const int lc = cin ? l : lmax;
-    ca = cf[NCF * lc];
-    if (SPIN == 2) cb = cf[NCF * lc + 1];
-    cr = cf[NCF * lc + NCF - 1];
-  };
-  auto store_coef = [&](int buf) {
-    cA[buf][threadIdx.x] = cin ? ca : (R)0;
-    cB[buf][threadIdx.x] = cin ? cb : (R)0;
-    cN[buf][threadIdx.x] = cin ? (E)(SPIN == 0 ? cr : 0.5 * cr) : (E)0;
-  };
-  if (threadIdx.x < kMaCH) {
-    load_coef(0);
-    store_coef(0);
-    load_coef(1);
-    store_coef(1);
-  }
   __syncthreads();
   for (int k = 0; k < nch; ++k) {
     const int lb = lb0 + k * kMaCH;
     const int bcur = k % 3;
     // recurrences of the lane's ring pair over the chunk -> tile
-    const R *ka = cA[bcur];
-    const R *kb = cB[bcur];
+    const cplx_t<R> *kab = cAB[bcur];
     const bool inj = __any((lfa >= lb && lfa < lb + kMaCH) || (lfb >= lb && lfb < lb + kMaCH));
     if (SPIN == 0) {
       if (inj) {
         R a0s, a1s;
-        seeds(sas, lfa, lb, a0s, a1s);
+        seeds(0, lfa, lb, a0s, a1s);
+#pragma unroll 8
         for (int i = 0; i < kMaCH; ++i)
-          T[i * TS + lane] = (E)rec_step(ka[i] * x, a1, a2, lb + i, lfa, a0s, a1s);
+          T[i * TS + lane] = (E)rec_step(kab[i].x * x, a1, a2, lb + i, lfa, a0s, a1s);
       } else {
 #pragma unroll 8
         for (int i = 0; i < kMaCH; ++i) {
-          const R lam = fma(ka[i] * x, a1, -a2);
+          const R lam = fma(kab[i].x * x, a1, -a2);
           a2 = a1;
           a1 = lam;
           T[i * TS + lane] = (E)lam;
@@ -1344,21 +1398,24 @@ adj_mfma_kernel(LegArgs g, long long grp_ts, const R *__restrict__ xs,
       C *T2 = reinterpret_cast<C *>(T);
       if (inj) {
         R a0s, a1s, b0s, b1s;
-        seeds(sas, lfa, lb, a0s, a1s);
-        seeds(sbs, lfb, lb, b0s, b1s);
+        seeds(0, lfa, lb, a0s, a1s);
+        seeds(SPIN == 2 ? 1 : 0, lfb, lb, b0s, b1s);
+#pragma unroll 8
         for (int i = 0; i < kMaCH; ++i) {
-          const R ax = ka[i] * x;
-          const R p = rec_step(ax + kb[i], a1, a2, lb + i, lfa, a0s, a1s);
-          const R q = rec_step(ax - kb[i], b1, b2, lb + i, lfb, b0s, b1s);
+          const cplx_t<R> k2 = kab[i];
+          const R ax = k2.x * x;
+          const R p = rec_step(ax + k2.y, a1, a2, lb + i, lfa, a0s, a1s);
+          const R q = rec_step(ax - k2.y, b1, b2, lb + i, lfb, b0s, b1s);
           const E qs = (i & 1) ? -(E)q : (E)q;
           T2[i * (TS / 2) + lane] = {(E)p, qs};
         }
       } else {
 #pragma unroll 8
         for (int i = 0; i < kMaCH; ++i) {
-          const R ax = ka[i] * x;
-          const R p = fma(ax + kb[i], a1, -a2);
-          const R q = fma(ax - kb[i], b1, -b2);
+          const cplx_t<R> k2 = kab[i];
+          const R ax = k2.x * x;
+          const R p = fma(ax + k2.y, a1, -a2);
+          const R q = fma(ax - k2.y, b1, -b2);
           a2 = a1;
           a1 = p;
           b2 = b1;
@@ -1372,7 +1429,7 @@ adj_mfma_kernel(LegArgs g, long long grp_ts, const R *__restrict__ xs,
     // coefficients of chunk k + 2, issued after the recurrence (whose start
     // values of an injection chunk are loaded first) and stored after the
     // products
-    if (threadIdx.x < kMaCH) load_coef(k + 2);
+    if (threadIdx.x < kMaCH && k + 2 < nch) fetch_coef(k + 2, (k + 2) % 3);
     // products: D[par][ct] over the wave's ring pairs
     V4 D[2][CT];
 #pragma unroll
@@ -1418,7 +1475,6 @@ adj_mfma_kernel(LegArgs g, long long grp_ts, const R *__restrict__ xs,
           tile[i * RS] = D[par][ct][r];
         }
       }
-    if (threadIdx.x < kMaCH) store_coef((k + 2) % 3);
     __syncthreads();
     if (SPIN == 0) {
       // items (par, ct, i, column pair)
@@ -1434,7 +1490,7 @@ adj_mfma_kernel(LegArgs g, long long grp_ts, const R *__restrict__ xs,
         const int row = pc / CT + 2 * i;
         const int l = lb + row, b = 8 * (pc % CT) + jp;
         if (l <= lmax && b < nt) {
-          const E c = cN[bcur][row];
+          const E c = (E)cAB[bcur][row].y;
           om[b * out_ts + l] = {c * re, c * im};
         }
       }
@@ -1458,7 +1514,7 @@ adj_mfma_kernel(LegArgs g, long long grp_ts, const R *__restrict__ xs,
         if (l <= lmax && b < nt) {
           const E sg = par ? (E)-1 : (E)1;
           const C sp = {v[0], v[1]}, sm = {sg * v[2], sg * v[3]};
-          const E c = cN[bcur][row];
+          const E c = (E)(0.5 * cBC[SPIN == 2 ? bcur : 0][row].y);
           // aE = -(S+ + S-)/2, aB = (i/2)(S+ - S-)
           om[b * out_ts + l] = {-c * (sp.x + sm.x), -c * (sp.y + sm.y)};
           om[b * out_ts + nelem + l] = {-c * (sp.y - sm.y), c * (sp.x - sm.x)};

@@ -41,6 +41,17 @@ def test_geometry_key_distinguishes_rings():
     assert k1.startswith('lmax=63;nrings=64;')
 
 
+def test_single_value_parameters_are_not_set():
+    # at batch 1 the transforms-per-thread choices have one candidate; a
+    # tuned entry applies to every larger batch, so they must stay unset
+    for dt in (torch.float32, torch.float64):
+        names = [n for n, _ in autotune._searched(autotune._candidates(dt, 1, None))]
+        assert 'synth_nb' not in names and 'adj_nb' not in names
+        assert 'adj_k0' in names and 'adj_k2' in names
+    names = [n for n, _ in autotune._searched(autotune._candidates(torch.float64, 2, False))]
+    assert 'synth_nb' in names and 'adj_nb' in names
+
+
 def rand_alm(ntrans, lmax, seed):
     rng = np.random.default_rng(seed)
     n = (lmax + 1) * (lmax + 2) // 2
