@@ -931,14 +931,17 @@ def main():
     # plain float64 CG on the same batch of right-hand sides (batching
     # without mixed precision), and the mixed solutions' error against it
     # per right-hand side
-    x64b, el64b, inf64b = time_solves('fp64', bsolver, args.fp64_batch_steps, 1)
-    fp64b_sps = args.fp64_batch_steps * args.batch / el64b
-    errs = (torch.linalg.vector_norm((xmx - x64b).reshape(args.batch, -1), dim=1)
-            / torch.linalg.vector_norm(x64b.reshape(args.batch, -1), dim=1)).cpu().numpy()
-    res64b = true_residual(bsolver, x64b)
-    log(f'fp64 batch {args.batch}: {fp64b_sps:.3f} solves/s, max true residual '
-        f'{res64b.max():.3e}; mixed vs fp64 per RHS rel L2 max {errs.max():.3e} '
-        f'median {np.median(errs):.3e}')
+    # (--fp64-batch-steps 0 skips it, e.g. for a profile of the mixed solve)
+    fp64b_sps, res64b, errs = None, np.array([np.nan]), np.array([np.nan])
+    if args.fp64_batch_steps > 0:
+        x64b, el64b, inf64b = time_solves('fp64', bsolver, args.fp64_batch_steps, 1)
+        fp64b_sps = args.fp64_batch_steps * args.batch / el64b
+        errs = (torch.linalg.vector_norm((xmx - x64b).reshape(args.batch, -1), dim=1)
+                / torch.linalg.vector_norm(x64b.reshape(args.batch, -1), dim=1)).cpu().numpy()
+        res64b = true_residual(bsolver, x64b)
+        log(f'fp64 batch {args.batch}: {fp64b_sps:.3f} solves/s, max true residual '
+            f'{res64b.max():.3e}; mixed vs fp64 per RHS rel L2 max {errs.max():.3e} '
+            f'median {np.median(errs):.3e}')
 
     cpu = None
     if args.cpu_baseline == 'on':
@@ -981,8 +984,10 @@ def main():
                    # the mixed solutions to those float64 solutions
                    'fp64_batched_solves_per_s': fp64b_sps,
                    'fp64_batched_true_residual_max': float(res64b.max()),
-                   'speedup_fp64_batched_vs_fp64': fp64b_sps / fp64_sps,
-                   'speedup_mixed_vs_fp64_batched': mixed_sps / fp64b_sps,
+                   'speedup_fp64_batched_vs_fp64': (fp64b_sps / fp64_sps
+                                                    if fp64b_sps else None),
+                   'speedup_mixed_vs_fp64_batched': (mixed_sps / fp64b_sps
+                                                     if fp64b_sps else None),
                    'rel_l2_vs_fp64_batched_max': float(errs.max()),
                    'rel_l2_vs_fp64_batched_median': float(np.median(errs)),
                    'autotune': tuned,

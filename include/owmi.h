@@ -410,6 +410,35 @@ int owmi_sht_pixel_sandwich(owmi_sht_plan *plan, int dtype, int spin,
                             long long ntrans, void *stream);
 
 /*
+ * Spectral form of owmi_sht_pixel_sandwich for a pixel diagonal D shared by
+ * every transform: the ring FFT stage becomes a convolution over m with
+ * per-ring real multipliers of D (two FFTs of length M per ring pair
+ * instead of four Bluestein FFTs); same result up to rounding.
+ *
+ * owmi_sht_sandwich_length: M (the smallest power of two >= 4 mmax + 1),
+ *   or 0 when the plan's ring FFTs do not cover it (rings shorter than
+ *   2 mmax + 1); the spectral entry points then fail.
+ * owmi_sht_sandwich_multipliers: mult[c][ring][k] (ncomp x nrings x M
+ *   values of dtype) from pixdiag (ncomp x npix of dtype), computed in
+ *   float64; once per diagonal.
+ * owmi_sht_pixel_sandwich_spectral: alm_out = Y^T D Y alm_in for one spin
+ *   group with the multipliers of its components (mult points at the
+ *   group's first component; component stride nrings * M).
+ * Same reference interfaces as owmi_sht_pixel_sandwich
+ * (optweight/operators.py:614-643, solvers.py:217-242).
+ */
+long long owmi_sht_sandwich_length(owmi_sht_plan *plan);
+int owmi_sht_sandwich_multipliers(owmi_sht_plan *plan, int dtype,
+                                  const void *pixdiag, int ncomp, void *mult,
+                                  void *stream);
+int owmi_sht_pixel_sandwich_spectral(owmi_sht_plan *plan, int dtype, int spin,
+                                     const void *alm_in,
+                                     long long alm_in_tstride, void *alm_out,
+                                     long long alm_out_tstride,
+                                     const void *mult, long long ntrans,
+                                     void *stream);
+
+/*
  * The Legendre stage alone, between alm and the plan's ring-coefficient
  * workspace (direction 0: alm -> workspace, the first half of synthesis;
  * direction 1: workspace -> alm, the second half of the adjoint). Used to

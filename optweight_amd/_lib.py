@@ -65,6 +65,10 @@ _SIGNATURES = {
     'owmi_sht_pixel_sandwich': (_c_int, [_vp, _c_int, _c_int, _vp, _c_ll, _vp, _c_ll, _vp, _c_ll,
                                          _c_ll, _vp]),
     'owmi_sht_legendre': (_c_int, [_vp, _c_int, _c_int, _c_int, _vp, _c_ll, _c_ll, _vp]),
+    'owmi_sht_sandwich_length': (_c_ll, [_vp]),
+    'owmi_sht_sandwich_multipliers': (_c_int, [_vp, _c_int, _vp, _c_int, _vp, _vp]),
+    'owmi_sht_pixel_sandwich_spectral': (_c_int, [_vp, _c_int, _c_int, _vp, _c_ll, _vp, _c_ll,
+                                                  _vp, _c_ll, _vp]),
 }
 
 _lib = None
@@ -119,6 +123,11 @@ def dtype_code(t):
     if t.dtype in (torch.float32, torch.complex64):
         return F32
     raise ValueError(f'unsupported dtype {t.dtype}')
+
+
+def real_dtype(dtype):
+    '''The real dtype of a (complex) alm dtype.'''
+    return {torch.complex64: torch.float32, torch.complex128: torch.float64}.get(dtype, dtype)
 
 
 def require_cuda(*ts):

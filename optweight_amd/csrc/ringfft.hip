@@ -574,6 +574,209 @@ ring_kernel(RingArgs a, const cplx_t<T> *twiddle, const cplx_t<T> *chirp,
   }
 }
 
+// ------------------------------------------------------ spectral sandwich
+//
+// The pixel sandwich of a fixed pixel diagonal as a convolution over m.
+// Ring a with spectrum C_m (Hermitian-extended to |m| <= mmax) and pixel
+// diagonal D_j gives H_m = sum_j D_j x_j exp(-i m phi_j)
+//                        = sum_m' K(m - m') C_m',
+// K(d) = sum_j D_j exp(-i d phi_j), |d| <= 2 mmax: a linear convolution,
+// exact as a cyclic one of length M >= 4 mmax + 1 with C placed at index
+// m mod M. Two rings a, b packed as Z = C_a + i C_b: the M-point DFT of a
+// Hermitian sequence is real, so DFT(Z) = DFT(C_a) + i DFT(C_b) splits into
+// its real and imaginary parts, and the sandwich is
+//   F = IDFT_M(Re DFT_M(Z) * S_a + i Im DFT_M(Z) * S_b),
+//   S_r[k] = DFT_M(K_r)[k] / M (real),
+// H_a,m = (F_m + conj F_-m) / 2, H_b,m = (F_m - conj F_-m) / (2 i): two FFTs
+// of length M per ring pair instead of four Bluestein FFTs, no chirps, any
+// ring length and phi0. The multipliers S_r depend on the diagonal only and
+// are computed once (conv_mult_kernel).
+
+// One workgroup per (ring pair, component, group of TPB transforms): the
+// multipliers are read once per group (TPB 8 for workgroups of up to 256
+// threads; 1 beyond, where the loop's registers would spill).
+template <typename T, int MAXT, bool GB, int TPB>
+__global__ void __launch_bounds__(MAXT)
+ring_conv_kernel(RingArgs a, int logm, const cplx_t<T> *twiddle, const T *mult,
+                 long long mult_cs, const cplx_t<T> *Gin, cplx_t<T> *Gout, long long ntrans,
+                 cplx_t<T> *gbuf, long long region) {
+  extern __shared__ unsigned char smem_raw[];
+  cplx_t<T> *buf = GB ? gbuf + ((long long)blockIdx.y * gridDim.x + blockIdx.x) * region
+                      : reinterpret_cast<cplx_t<T> *>(smem_raw);
+  __shared__ cplx_t<T> tw_lo[kTwTab], tw_hi[kTwTab];
+  const Twiddles<T> tws = load_twiddles<T>(twiddle, a.log_mmax_fft, tw_lo, tw_hi);
+  // workgroups of one XCD take a contiguous range of ring pairs (they share
+  // the cache lines of the pair-major coefficients)
+  const long long gx = gridDim.x;
+  const long long job = (gx & 7) == 0 ? (long long)(blockIdx.x & 7) * (gx >> 3) + (blockIdx.x >> 3)
+                                      : (long long)blockIdx.x;
+  const int c = (int)(blockIdx.y % a.ncomp);
+  const long long t_lo = (long long)(blockIdx.y / a.ncomp) * TPB;
+  const long long t_hi = min(ntrans, t_lo + TPB);
+  const int ra = a.fft_a[job];
+  const int rb = a.fft_b[job];
+  const int M = 1 << logm;
+  const int Q = M >> 4;
+  const int q = threadIdx.x;
+  const bool on = q < Q;
+  const int mmax = a.mmax;
+  const int sa = a.ring_slot[ra], sb = rb >= 0 ? a.ring_slot[rb] : 0;
+  const bool joint = rb >= 0 && sb == sa + 1 && (sa & 1) == 0;
+  // The multipliers stay in registers across the group where the register
+  // budget allows (float, 256 threads), else they are read (from L2 after
+  // the first transform) at the multiply.
+  constexpr bool KEEP = sizeof(T) == 4 && TPB > 1;
+  const T *pma = mult + c * mult_cs + (long long)ra * M;
+  const T *pmb = mult + c * mult_cs + (long long)(rb >= 0 ? rb : ra) * M;
+  T ma[16], mb[16];
+  auto load_mult = [&]() {
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+      ma[s] = pma[q + s * Q];
+      mb[s] = rb >= 0 ? pmb[q + s * Q] : (T)0;
+    }
+  };
+  if (KEEP && on) load_mult();
+  const long long fs = 2 * a.npairs;  // m stride of the synthesis layout
+  constexpr int MB = coef_mb<T>();
+  const long long gp_blk = 2 * MB * a.npairs;  // elements per m block of the adjoint layout
+  typedef T T4 __attribute__((ext_vector_type(4)));
+  for (long long t = t_lo; t < t_hi; ++t) {
+    const long long tc = t * a.ncomp + c;
+    const cplx_t<T> *Fa = Gin + coef_syn_at(tc, 0, sa >> 1, mmax, a.npairs) + (sa & 1);
+    const cplx_t<T> *Fb = Gin + coef_syn_at(tc, 0, sb >> 1, mmax, a.npairs) + (sb & 1);
+    cplx_t<T> v[16];
+    if (on) {
+#pragma unroll
+      for (int s = 0; s < 16; ++s) {
+        const int k = q + s * Q;
+        // slot k holds m = k (k <= mmax) or m = k - M (k >= M - mmax)
+        const int am = k <= mmax ? k : (k >= M - mmax ? M - k : -1);
+        cplx_t<T> za = {0, 0}, zb = {0, 0};
+        if (am >= 0) {
+          if (joint) {
+            const T4 w4 = *reinterpret_cast<const T4 *>(Fa + am * fs);
+            za = {w4.x, w4.y};
+            zb = {w4.z, w4.w};
+          } else {
+            za = Fa[am * fs];
+            if (rb >= 0) zb = Fb[am * fs];
+          }
+          if (am == 0) {
+            za.y = 0;
+            zb.y = 0;
+          }
+          if (k > mmax) {  // C_-m = conj C_m
+            za.y = -za.y;
+            zb.y = -zb.y;
+          }
+        }
+        v[s] = {za.x - zb.y, za.y + zb.x};
+      }
+    }
+    fft16<T, -1, GB>(v, buf, logm, q, tws, a.log_mmax_fft);
+    if (on) {
+      if (!KEEP) load_mult();
+#pragma unroll
+      for (int s = 0; s < 16; ++s) v[s] = {v[s].x * ma[s], v[s].y * mb[s]};
+    }
+    fft16<T, 1, GB>(v, buf, logm, q, tws, a.log_mmax_fft);
+    block_sync<GB>();
+    if (on) {
+#pragma unroll
+      for (int s = 0; s < 16; ++s) {
+        const int k = q + s * Q;
+        if (k <= mmax || k >= M - mmax) buf[pad16(k)] = v[s];
+      }
+    }
+    block_sync<GB>();
+    cplx_t<T> *Ga = Gout + coef_pair_at<T>(tc, 0, sa >> 1, mmax, a.npairs) + (sa & 1);
+    cplx_t<T> *Gb = Gout + coef_pair_at<T>(tc, 0, sb >> 1, mmax, a.npairs) + (sb & 1);
+    for (int m = threadIdx.x; m <= mmax; m += blockDim.x) {
+      const cplx_t<T> X = buf[pad16(m)];
+      const cplx_t<T> Xc = cconj<T>(buf[pad16(m == 0 ? 0 : M - m)]);
+      cplx_t<T> xa = {(T)0.5 * (X.x + Xc.x), (T)0.5 * (X.y + Xc.y)};
+      const cplx_t<T> d = {(T)0.5 * (X.x - Xc.x), (T)0.5 * (X.y - Xc.y)};
+      cplx_t<T> xb = {d.y, -d.x};  // d / i
+      if (m == 0) {
+        xa.y = 0;
+        xb.y = 0;
+      }
+      const long long om = (m / MB) * gp_blk + 2 * (m % MB);
+      if (joint) {
+        *reinterpret_cast<T4 *>(Ga + om) = T4{xa.x, xa.y, xb.x, xb.y};
+      } else {
+        Ga[om] = xa;
+        if (rb >= 0) Gb[om] = xb;
+      }
+    }
+  }
+}
+
+// Multipliers of the spectral sandwich, in float64: for ring r and
+// component c, K(d) = exp(-i d phi0) sum_j D_j exp(-2 pi i (d j mod N) / N)
+// for d = 0 .. 2 mmax (into LDS), then
+// S[k] = (K(0) + 2 sum_{d >= 1} Re(K(d) exp(-2 pi i d k / M))) / M.
+// The phase factors advance by one complex product per step and restart
+// from sincospi every 32 steps.
+template <typename T>
+__global__ void __launch_bounds__(256)
+conv_mult_kernel(const T *diag, long long diag_cs, const long long *nphi,
+                 const long long *offsets, const double *phi0, int mmax, int logm, T *mult,
+                 long long mult_cs) {
+  extern __shared__ unsigned char smem_raw[];
+  double2 *K = reinterpret_cast<double2 *>(smem_raw);  // 2 mmax + 1
+  const long long r = blockIdx.x;
+  const int c = blockIdx.y;
+  const long long N = nphi[r];
+  const T *D = diag + c * diag_cs + offsets[r];
+  const double ph = phi0[r];
+  const int nd = 2 * mmax + 1;
+  for (int d = threadIdx.x; d < nd; d += blockDim.x) {
+    const long long dm = d % N;
+    double re = 0, im = 0, zr = 1, zi = 0, wr, wi;
+    sincospi(-2.0 * (double)dm / (double)N, &wi, &wr);
+    long long idx = 0;  // d j mod N
+    for (long long j = 0; j < N; ++j) {
+      if ((j & 31) == 0) sincospi(-2.0 * (double)idx / (double)N, &zi, &zr);
+      const double x = (double)D[j];
+      re = fma(x, zr, re);
+      im = fma(x, zi, im);
+      const double nr = zr * wr - zi * wi;
+      zi = zr * wi + zi * wr;
+      zr = nr;
+      idx += dm;
+      if (idx >= N) idx -= N;
+    }
+    double sp, cp;
+    sincos(-(double)d * ph, &sp, &cp);
+    K[d] = {re * cp - im * sp, re * sp + im * cp};
+  }
+  __syncthreads();
+  const long long M = 1LL << logm;
+  const double invm = 1.0 / (double)M;
+  T *out = mult + c * mult_cs + r * M;
+  for (long long k = threadIdx.x; k < M; k += blockDim.x) {
+    double acc = 0, zr = 1, zi = 0, wr, wi;
+    sincospi(-2.0 * (double)k / (double)M, &wi, &wr);
+    long long idx = 0;  // d k mod M
+    for (int d = 1; d < nd; ++d) {
+      idx += k;
+      if (idx >= M) idx -= M;
+      if (((d - 1) & 31) == 0) {
+        sincospi(-2.0 * (double)idx / (double)M, &zi, &zr);
+      } else {
+        const double nr = zr * wr - zi * wi;
+        zi = zr * wi + zi * wr;
+        zr = nr;
+      }
+      acc = fma(K[d].x, zr, acc);
+      acc = fma(-K[d].y, zi, acc);
+    }
+    out[k] = (T)((K[0].x + 2.0 * acc) * invm);
+  }
+}
+
 // bhat for one distinct ring length per block.
 __global__ void __launch_bounds__(256)
 bluestein_init_kernel(const long long *lens, const int *logms, const long long *chirp_off,
@@ -679,6 +882,40 @@ void launch_ring(owmi_plan_impl *p, const cplx_t<T> *Gin, cplx_t<T> *Gout, const
                      map_in, map_out, map_ts, diag, diag_ts, gbuf, region);
 }
 
+template <typename T>
+void launch_conv(owmi_plan_impl *p, const cplx_t<T> *G, cplx_t<T> *Gout, const T *mult,
+                 long long mult_cs, int ncomp, long long ntrans, hipStream_t st) {
+  const int logm = conv_logm(p);
+  OWMI_REQUIRE(logm > 0, "the spectral sandwich does not apply to this plan");
+  const int M = 1 << logm;
+  // transforms per workgroup: the multipliers are read once per group
+  const int threads = std::max(64, M / 16);
+  const int tpb = threads <= 256 ? 8 : 1;
+  const long long ngroups = (ntrans + tpb - 1) / tpb;
+  const long long nrows = ngroups * ncomp;
+  OWMI_REQUIRE(nrows < 65536, "too many rows for one ring-FFT launch");
+  const size_t lds = ring_lds_bytes<T>(M);
+  PlanTables<T> &tb = tab<T>(p);
+  dim3 grid((unsigned)p->nfft_jobs, (unsigned)nrows);
+  const bool gb = lds + 2 * kTwTab * sizeof(cplx_t<T>) > kMaxLds;
+  cplx_t<T> *gbuf = nullptr;
+  const long long region = M + M / 16;
+  if (gb) {
+    const size_t bytes = (size_t)p->nfft_jobs * nrows * region * sizeof(cplx_t<T>);
+    OWMI_REQUIRE(bytes <= (size_t)32 << 30, "ring-FFT scratch above 32 GiB: split the batch");
+    gbuf = static_cast<cplx_t<T> *>(p->ensure_work3(bytes));
+  }
+  auto kern = gb ? ring_conv_kernel<T, 1024, true, 1>
+              : threads <= 256 ? ring_conv_kernel<T, 256, false, 8>
+              : threads <= 512 ? ring_conv_kernel<T, 512, false, 1>
+                               : ring_conv_kernel<T, 1024, false, 1>;
+  const size_t dyn = gb ? 0 : lds;
+  allow_lds(kern, dyn);
+  hipLaunchKernelGGL(kern, grid, dim3(threads), dyn, st, ring_args<T>(p, ncomp, 0), logm,
+                     reinterpret_cast<const cplx_t<T> *>(tb.twiddle), mult, mult_cs, G, Gout,
+                     ntrans, gbuf, region);
+}
+
 }  // namespace
 
 template <typename T>
@@ -700,6 +937,33 @@ void launch_ring_sandwich(owmi_plan_impl *p, const cplx_t<T> *G, cplx_t<T> *Gout
                           long long diag_tstride, int ncomp, long long ntrans, hipStream_t st) {
   launch_ring<T, kSandwich>(p, G, Gout, nullptr, nullptr, 0, diag, diag_tstride, ncomp,
                             ntrans, 0, st);
+}
+
+int conv_logm(const owmi_plan_impl *p) {
+  int lm = 4;
+  while ((1LL << lm) < 4LL * p->mmax + 1) ++lm;
+  // the plan's twiddle table and FFT workgroup size cover M
+  return (1 << lm) <= p->mmax_fft ? lm : 0;
+}
+
+template <typename T>
+void launch_ring_conv(owmi_plan_impl *p, const cplx_t<T> *G, cplx_t<T> *Gout, const T *mult,
+                      long long mult_cs, int ncomp, long long ntrans, hipStream_t st) {
+  launch_conv<T>(p, G, Gout, mult, mult_cs, ncomp, ntrans, st);
+}
+
+template <typename T>
+void launch_conv_mult(owmi_plan_impl *p, const T *diag, long long diag_cs, int ncomp, T *mult,
+                      long long mult_cs, hipStream_t st) {
+  const int logm = conv_logm(p);
+  OWMI_REQUIRE(logm > 0, "the spectral sandwich does not apply to this plan");
+  const size_t lds = (size_t)(2 * p->mmax + 1) * sizeof(double2);
+  OWMI_REQUIRE(lds <= kMaxLds, "mmax too large for the sandwich multipliers");
+  allow_lds(conv_mult_kernel<T>, lds);
+  OWMI_REQUIRE(p->nrings < (1LL << 31) && ncomp < 65536, "too many rings or components");
+  hipLaunchKernelGGL(conv_mult_kernel<T>, dim3((unsigned)p->nrings, (unsigned)ncomp), dim3(256),
+                     lds, st, diag, diag_cs, p->nphi, p->offsets,
+                     reinterpret_cast<const double *>(p->t64.phi0), p->mmax, logm, mult, mult_cs);
 }
 
 void launch_bluestein_init(owmi_plan_impl *p, const std::vector<long long> &lens,
@@ -747,5 +1011,13 @@ template void launch_ring_sandwich<double>(owmi_plan_impl *, const double2 *, do
                                            const double *, long long, int, long long, hipStream_t);
 template void launch_ring_sandwich<float>(owmi_plan_impl *, const float2 *, float2 *,
                                           const float *, long long, int, long long, hipStream_t);
+template void launch_ring_conv<double>(owmi_plan_impl *, const double2 *, double2 *,
+                                       const double *, long long, int, long long, hipStream_t);
+template void launch_ring_conv<float>(owmi_plan_impl *, const float2 *, float2 *, const float *,
+                                      long long, int, long long, hipStream_t);
+template void launch_conv_mult<double>(owmi_plan_impl *, const double *, long long, int, double *,
+                                       long long, hipStream_t);
+template void launch_conv_mult<float>(owmi_plan_impl *, const float *, long long, int, float *,
+                                      long long, hipStream_t);
 
 }  // namespace owmi

@@ -182,6 +182,18 @@ template <typename T>
 void launch_ring_sandwich(owmi_plan_impl *p, const cplx_t<T> *G, cplx_t<T> *Gout,
                           const T *diag, long long diag_tstride, int ncomp, long long ntrans,
                           hipStream_t st);
+// Spectral sandwich (ringfft.hip): log2 of its FFT length M (the smallest
+// power of two >= 4 mmax + 1, at least 16), or 0 when the plan's FFT tables
+// do not cover it; the multipliers [comp][ring][M] of a pixel diagonal
+// (comp, npix) for one precision, and the sandwich of the ring
+// coefficients with them (synthesis layout in, adjoint layout out).
+int conv_logm(const owmi_plan_impl *p);
+template <typename T>
+void launch_conv_mult(owmi_plan_impl *p, const T *diag, long long diag_cs, int ncomp, T *mult,
+                      long long mult_cs, hipStream_t st);
+template <typename T>
+void launch_ring_conv(owmi_plan_impl *p, const cplx_t<T> *G, cplx_t<T> *Gout, const T *mult,
+                      long long mult_cs, int ncomp, long long ntrans, hipStream_t st);
 // Fill bhat for every distinct ring length from the chirp table (plan setup).
 void launch_bluestein_init(owmi_plan_impl *p, const std::vector<long long> &lens,
                            const std::vector<int> &logms, const std::vector<long long> &chirp_off,

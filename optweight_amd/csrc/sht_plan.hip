@@ -797,6 +797,43 @@ static int do_sandwich(owmi_sht_plan *plan, int spin, const void *alm_in, long l
 }
 
 template <typename T>
+static int do_sandwich_mult(owmi_sht_plan *plan, const void *diag, int ncomp, void *mult,
+                            void *stream) {
+  return guard([&] {
+    OWMI_REQUIRE(plan != nullptr, "plan is NULL");
+    owmi_plan_impl &p = plan->impl;
+    OWMI_REQUIRE(ncomp >= 1, "ncomp must be >= 1");
+    OWMI_REQUIRE(conv_logm(&p) > 0, "the spectral sandwich does not apply to this plan");
+    const long long M = 1LL << conv_logm(&p);
+    launch_conv_mult<T>(&p, static_cast<const T *>(diag), p.npix, ncomp, static_cast<T *>(mult),
+                        p.nrings * M, as_stream(stream));
+  });
+}
+
+template <typename T>
+static int do_sandwich_spectral(owmi_sht_plan *plan, int spin, const void *alm_in,
+                                long long in_ts, void *alm_out, long long out_ts,
+                                const void *mult, long long ntrans, void *stream) {
+  return guard([&] {
+    OWMI_REQUIRE(plan != nullptr, "plan is NULL");
+    owmi_plan_impl &p = plan->impl;
+    OWMI_REQUIRE(spin == 0 || spin == 2, "spin must be 0 or 2");
+    OWMI_REQUIRE(spin <= p.lmax, "Spin exceeds lmax");
+    OWMI_REQUIRE(conv_logm(&p) > 0, "the spectral sandwich does not apply to this plan");
+    check_fft_fits<T>(p);
+    if (ntrans == 0) return;
+    const int nc = spin == 0 ? 1 : 2;
+    const long long M = 1LL << conv_logm(&p);
+    auto *G = static_cast<cplx_t<T> *>(p.ensure_work(coef_bytes<T>(p, ntrans * nc)));
+    auto *GP = static_cast<cplx_t<T> *>(p.ensure_work4(coef_bytes<T>(p, ntrans * nc)));
+    hipStream_t st = as_stream(stream);
+    launch_synthesis<T>(&p, spin, static_cast<const cplx_t<T> *>(alm_in), in_ts, G, ntrans, st);
+    launch_ring_conv<T>(&p, G, GP, static_cast<const T *>(mult), p.nrings * M, nc, ntrans, st);
+    launch_adjoint<T>(&p, spin, GP, static_cast<cplx_t<T> *>(alm_out), out_ts, ntrans, st);
+  });
+}
+
+template <typename T>
 static int do_legendre(owmi_sht_plan *plan, int spin, int direction, void *alm,
                        long long alm_ts, long long ntrans, void *stream) {
   return guard([&] {
@@ -849,6 +886,34 @@ int owmi_sht_adjoint(owmi_sht_plan *plan, int dtype, int spin, const void *map,
   if (dtype == OWMI_F32)
     return do_adjoint<float>(plan, spin, map, map_tstride, alm, alm_tstride, ntrans, weighted,
                              stream);
+  set_error("dtype must be OWMI_F32 or OWMI_F64");
+  return 1;
+}
+
+long long owmi_sht_sandwich_length(owmi_sht_plan *plan) {
+  if (plan == nullptr) return 0;
+  const int lm = conv_logm(&plan->impl);
+  return lm > 0 ? 1LL << lm : 0;
+}
+
+int owmi_sht_sandwich_multipliers(owmi_sht_plan *plan, int dtype, const void *pixdiag, int ncomp,
+                                  void *mult, void *stream) {
+  if (dtype == OWMI_F64) return do_sandwich_mult<double>(plan, pixdiag, ncomp, mult, stream);
+  if (dtype == OWMI_F32) return do_sandwich_mult<float>(plan, pixdiag, ncomp, mult, stream);
+  set_error("dtype must be OWMI_F32 or OWMI_F64");
+  return 1;
+}
+
+int owmi_sht_pixel_sandwich_spectral(owmi_sht_plan *plan, int dtype, int spin,
+                                     const void *alm_in, long long alm_in_tstride, void *alm_out,
+                                     long long alm_out_tstride, const void *mult,
+                                     long long ntrans, void *stream) {
+  if (dtype == OWMI_F64)
+    return do_sandwich_spectral<double>(plan, spin, alm_in, alm_in_tstride, alm_out,
+                                        alm_out_tstride, mult, ntrans, stream);
+  if (dtype == OWMI_F32)
+    return do_sandwich_spectral<float>(plan, spin, alm_in, alm_in_tstride, alm_out,
+                                       alm_out_tstride, mult, ntrans, stream);
   set_error("dtype must be OWMI_F32 or OWMI_F64");
   return 1;
 }

@@ -14,6 +14,8 @@ other rings untouched; the adjoint reads only the given rings, which is the
 full adjoint for a map that vanishes on all other rings (masked operators).
 '''
 import ctypes
+import os
+import weakref
 
 import numpy as np
 import torch
@@ -216,9 +218,45 @@ def map2alm(imap, alm, minfo, ainfo, spin, adjoint=False, rings=None):
     return alm
 
 
-def pixel_sandwich(alm, oalm, ainfo, minfo, spin, pixdiag):
+def spectral_enabled():
+    '''The spectral sandwich is used unless OWMI_SANDWICH=bluestein.'''
+    return os.environ.get('OWMI_SANDWICH', 'spectral') != 'bluestein'
+
+
+def sandwich_multipliers(pixdiag, minfo, lmax):
+    '''Per-ring multipliers (npol, nrings, M) of the spectral sandwich for a
+    pixel diagonal (npol, npix) shared by every transform, or None where the
+    plan's ring FFTs do not cover the length M (owmi_sht_sandwich_length).
+    They are computed once (float64 sums, stored in the diagonal's dtype)
+    and kept on the diagonal tensor with its version counter, so an in-place
+    change of the diagonal recomputes them.'''
+    plan = get_plan(minfo, lmax, pixdiag.device)
+    M = int(plan._lib.owmi_sht_sandwich_length(plan.handle))
+    if M == 0:
+        return None
+    key = (int(lmax), M, pixdiag.dtype, str(pixdiag.device))
+    cached = getattr(pixdiag, '_owmi_mult', None)
+    if (cached is not None and cached[0] == pixdiag._version and cached[1] == key
+            and cached[2]() is minfo):
+        return cached[3]
+    d = pixdiag.reshape((-1, minfo.npix))
+    if not d.is_contiguous():
+        raise ValueError('pixdiag must be contiguous')
+    mult = torch.empty((d.shape[0], minfo.nrow, M), dtype=d.dtype, device=d.device)
+    _lib.check(plan._lib.owmi_sht_sandwich_multipliers(
+        plan.handle, _lib.dtype_code(d), _lib.ptr(d), d.shape[0], _lib.ptr(mult),
+        _lib.stream_ptr(d.device)))
+    pixdiag._owmi_mult = (pixdiag._version, key, weakref.ref(minfo), mult)
+    return mult
+
+
+def pixel_sandwich(alm, oalm, ainfo, minfo, spin, pixdiag, spectral=None):
     '''oalm = Y^T D Y alm with D = pixdiag (npol, npix) per component; the
-    map stays on chip between the two transforms.'''
+    map stays on chip between the two transforms. A diagonal shared by all
+    transforms runs as the spectral sandwich (per-ring multipliers from
+    sandwich_multipliers, two FFTs of length M per ring pair) unless
+    spectral is False or OWMI_SANDWICH=bluestein; a per-transform diagonal,
+    or a plan without the spectral length, runs the Bluestein sandwich.'''
     lmax = _lmax_of(ainfo)
     _lib.require_cuda(alm, oalm, pixdiag)
     if tuple(alm.shape) != tuple(oalm.shape) or alm.dtype != oalm.dtype:
@@ -233,6 +271,18 @@ def pixel_sandwich(alm, oalm, ainfo, minfo, spin, pixdiag):
     plan = get_plan(minfo, lmax, alm.device)
     code = _lib.dtype_code(alm)
     st = _lib.stream_ptr(alm.device)
+    if spectral is None:
+        spectral = spectral_enabled()
+    mult = None
+    if spectral and pix_ts == 0 and pixdiag.dtype == _lib.real_dtype(alm.dtype):
+        mult = sandwich_multipliers(pixdiag, minfo, lmax)
+    if mult is not None:
+        for s, i1, i2 in spin_groups(spin, npol):
+            _lib.check(plan._lib.owmi_sht_pixel_sandwich_spectral(
+                plan.handle, code, s, _lib.ptr(a3[:, i1]), npol * a3.shape[-1],
+                _lib.ptr(o3[:, i1]), npol * o3.shape[-1], _lib.ptr(mult[i1]), a3.shape[0], st))
+            _count(plan, s, i2 - i1, a3.shape[0], alm.dtype, directions=2)
+        return oalm
     for s, i1, i2 in spin_groups(spin, npol):
         _lib.check(plan._lib.owmi_sht_pixel_sandwich(
             plan.handle, code, s, _lib.ptr(a3[:, i1]), npol * a3.shape[-1], _lib.ptr(o3[:, i1]),
