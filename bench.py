@@ -150,12 +150,12 @@ def legendre_roofline(prob, dev, batch, reps=10):
     return {'kernel': kernel, 'ms': ms, 'flops': flops, 'achieved': achieved}
 
 
-# HBM bytes per launch of the roofline kernel (adj_mfma_kernel<2, 8, 4>, 128
-# right-hand sides): FETCH_SIZE (x 2.0) + WRITE_SIZE (x 1.0), factors from a
-# 256 MiB copy in the same process, measured with `rocprofv3 --pmc` on
+# HBM bytes per launch of the roofline kernel (adj_mfma_kernel<float, 2, 8,
+# 4>, 128 right-hand sides): FETCH_SIZE (x 2.0) + WRITE_SIZE (x 1.0), factors
+# from a 256 MiB copy in the same process, measured with `rocprofv3 --pmc` on
 # `python bench.py --roofline-only`
-# (profiles/adj_mfma2_f32_tqu1023_b128_roofline.md): 2.756 GB + 1.476 GB.
-ROOFLINE_TRAFFIC_BYTES = 4.232e9
+# (profiles/adj_mfma2_f32_tqu1023_b128_roofline.md): 2.763 GB + 1.452 GB.
+ROOFLINE_TRAFFIC_BYTES = 4.215e9
 ROOFLINE_TRAFFIC_SHAPE = (1023, 128)
 
 # right-hand sides per batch: the headline solve, and the batched timings of
