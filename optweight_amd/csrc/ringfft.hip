@@ -21,6 +21,7 @@
 #include "sht_internal.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 
 namespace owmi {
 
@@ -647,30 +648,35 @@ ring_conv_kernel(RingArgs a, int logm, const cplx_t<T> *twiddle, const T *mult,
     const cplx_t<T> *Fb = Gin + coef_syn_at(tc, 0, sb >> 1, mmax, a.npairs) + (sb & 1);
     cplx_t<T> v[16];
     if (on) {
+      // slot k holds m = k (k <= mmax) or m = k - M (k >= M - mmax), zero
+      // between; the loads of all 16 slots are issued together (from a
+      // clamped, valid m) and the slot's value selected afterwards
+      T4 w[16];
+      if (joint) {
+#pragma unroll
+        for (int s = 0; s < 16; ++s) {
+          const int k = q + s * Q;
+          const int am = k <= mmax ? k : (k >= M - mmax ? M - k : 0);
+          w[s] = *reinterpret_cast<const T4 *>(Fa + (long long)am * fs);
+        }
+      } else {
+#pragma unroll
+        for (int s = 0; s < 16; ++s) {
+          const int k = q + s * Q;
+          const int am = k <= mmax ? k : (k >= M - mmax ? M - k : 0);
+          const cplx_t<T> fa = Fa[(long long)am * fs];
+          const cplx_t<T> fb = rb >= 0 ? Fb[(long long)am * fs] : cplx_t<T>{0, 0};
+          w[s] = T4{fa.x, fa.y, fb.x, fb.y};
+        }
+      }
 #pragma unroll
       for (int s = 0; s < 16; ++s) {
         const int k = q + s * Q;
-        // slot k holds m = k (k <= mmax) or m = k - M (k >= M - mmax)
-        const int am = k <= mmax ? k : (k >= M - mmax ? M - k : -1);
-        cplx_t<T> za = {0, 0}, zb = {0, 0};
-        if (am >= 0) {
-          if (joint) {
-            const T4 w4 = *reinterpret_cast<const T4 *>(Fa + am * fs);
-            za = {w4.x, w4.y};
-            zb = {w4.z, w4.w};
-          } else {
-            za = Fa[am * fs];
-            if (rb >= 0) zb = Fb[am * fs];
-          }
-          if (am == 0) {
-            za.y = 0;
-            zb.y = 0;
-          }
-          if (k > mmax) {  // C_-m = conj C_m
-            za.y = -za.y;
-            zb.y = -zb.y;
-          }
-        }
+        const bool in = k <= mmax || k >= M - mmax;
+        // C_0 real; C_-m = conj C_m
+        const T sg = k == 0 ? (T)0 : (k > mmax ? (T)-1 : (T)1);
+        const cplx_t<T> za = {in ? w[s].x : (T)0, in ? sg * w[s].y : (T)0};
+        const cplx_t<T> zb = {in ? w[s].z : (T)0, in ? sg * w[s].w : (T)0};
         v[s] = {za.x - zb.y, za.y + zb.x};
       }
     }
@@ -889,8 +895,13 @@ void launch_conv(owmi_plan_impl *p, const cplx_t<T> *G, cplx_t<T> *Gout, const T
   OWMI_REQUIRE(logm > 0, "the spectral sandwich does not apply to this plan");
   const int M = 1 << logm;
   // transforms per workgroup: the multipliers are read once per group
+  // (OWMI_CONV_TPB = 1, 2, 4 or 8 overrides for measurements)
   const int threads = std::max(64, M / 16);
-  const int tpb = threads <= 256 ? 8 : 1;
+  static const int forced = [] {
+    const char *e = std::getenv("OWMI_CONV_TPB");
+    return e != nullptr ? std::atoi(e) : 0;
+  }();
+  const int tpb = threads > 256 ? 1 : (forced == 1 || forced == 2 || forced == 4) ? forced : 8;
   const long long ngroups = (ntrans + tpb - 1) / tpb;
   const long long nrows = ngroups * ncomp;
   OWMI_REQUIRE(nrows < 65536, "too many rows for one ring-FFT launch");
@@ -906,9 +917,12 @@ void launch_conv(owmi_plan_impl *p, const cplx_t<T> *G, cplx_t<T> *Gout, const T
     gbuf = static_cast<cplx_t<T> *>(p->ensure_work3(bytes));
   }
   auto kern = gb ? ring_conv_kernel<T, 1024, true, 1>
-              : threads <= 256 ? ring_conv_kernel<T, 256, false, 8>
-              : threads <= 512 ? ring_conv_kernel<T, 512, false, 1>
-                               : ring_conv_kernel<T, 1024, false, 1>;
+              : threads > 512 ? ring_conv_kernel<T, 1024, false, 1>
+              : threads > 256 ? ring_conv_kernel<T, 512, false, 1>
+              : tpb == 1 ? ring_conv_kernel<T, 256, false, 1>
+              : tpb == 2 ? ring_conv_kernel<T, 256, false, 2>
+              : tpb == 4 ? ring_conv_kernel<T, 256, false, 4>
+                         : ring_conv_kernel<T, 256, false, 8>;
   const size_t dyn = gb ? 0 : lds;
   allow_lds(kern, dyn);
   hipLaunchKernelGGL(kern, grid, dim3(threads), dyn, st, ring_args<T>(p, ncomp, 0), logm,
