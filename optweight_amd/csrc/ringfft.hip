@@ -719,67 +719,111 @@ ring_conv_kernel(RingArgs a, int logm, const cplx_t<T> *twiddle, const T *mult,
   }
 }
 
-// Multipliers of the spectral sandwich, in float64: for ring r and
-// component c, K(d) = exp(-i d phi0) sum_j D_j exp(-2 pi i (d j mod N) / N)
-// for d = 0 .. 2 mmax (into LDS), then
-// S[k] = (K(0) + 2 sum_{d >= 1} Re(K(d) exp(-2 pi i d k / M))) / M.
-// The phase factors advance by one complex product per step and restart
-// from sincospi every 32 steps.
-template <typename T>
-__global__ void __launch_bounds__(256)
-conv_mult_kernel(const T *diag, long long diag_cs, const long long *nphi,
-                 const long long *offsets, const double *phi0, int mmax, int logm, T *mult,
-                 long long mult_cs) {
+// Multipliers of the spectral sandwich, in float64, per FFT job (rings a,
+// b) and component c: the Bluestein DFT of z = D_a + i D_b gives X_k,
+// k < N, and
+//   K_a(d) = exp(-i d phi_a) (X_d + conj X_-d) / 2,
+//   K_b(d) = exp(-i d phi_b) (X_d - conj X_-d) / (2 i)   (indices mod N),
+// for d = 0 .. 2 mmax; Z = K_a + i K_b at index d mod M (|d| <= 2 mmax,
+// K(-d) = conj K(d)), and S_a = Re DFT_M(Z) / M, S_b = Im DFT_M(Z) / M
+// (the DFTs of the two Hermitian sequences are real). Two Bluestein FFTs and
+// one FFT of length M per job and component.
+template <typename T, int MAXT, bool GB>
+__global__ void __launch_bounds__(MAXT)
+conv_mult_kernel(RingArgs a, int logmc, const double2 *twiddle, const double2 *chirp,
+                 const double2 *bhat_all, const double *phi0s, const T *diag, long long diag_cs,
+                 T *mult, long long mult_cs, double2 *gbuf, long long region) {
   extern __shared__ unsigned char smem_raw[];
-  double2 *K = reinterpret_cast<double2 *>(smem_raw);  // 2 mmax + 1
-  const long long r = blockIdx.x;
+  double2 *buf = GB ? gbuf + ((long long)blockIdx.y * gridDim.x + blockIdx.x) * region
+                    : reinterpret_cast<double2 *>(smem_raw);
+  __shared__ double2 tw_lo[kTwTab], tw_hi[kTwTab];
+  const Twiddles<double> tws = load_twiddles<double>(twiddle, a.log_mmax_fft, tw_lo, tw_hi);
+  const long long job = blockIdx.x;
   const int c = blockIdx.y;
-  const long long N = nphi[r];
-  const T *D = diag + c * diag_cs + offsets[r];
-  const double ph = phi0[r];
-  const int nd = 2 * mmax + 1;
-  for (int d = threadIdx.x; d < nd; d += blockDim.x) {
-    const long long dm = d % N;
-    double re = 0, im = 0, zr = 1, zi = 0, wr, wi;
-    sincospi(-2.0 * (double)dm / (double)N, &wi, &wr);
-    long long idx = 0;  // d j mod N
-    for (long long j = 0; j < N; ++j) {
-      if ((j & 31) == 0) sincospi(-2.0 * (double)idx / (double)N, &zi, &zr);
-      const double x = (double)D[j];
-      re = fma(x, zr, re);
-      im = fma(x, zi, im);
-      const double nr = zr * wr - zi * wi;
-      zi = zr * wi + zi * wr;
-      zr = nr;
-      idx += dm;
-      if (idx >= N) idx -= N;
-    }
-    double sp, cp;
-    sincos(-(double)d * ph, &sp, &cp);
-    K[d] = {re * cp - im * sp, re * sp + im * cp};
-  }
-  __syncthreads();
-  const long long M = 1LL << logm;
-  const double invm = 1.0 / (double)M;
-  T *out = mult + c * mult_cs + r * M;
-  for (long long k = threadIdx.x; k < M; k += blockDim.x) {
-    double acc = 0, zr = 1, zi = 0, wr, wi;
-    sincospi(-2.0 * (double)k / (double)M, &wi, &wr);
-    long long idx = 0;  // d k mod M
-    for (int d = 1; d < nd; ++d) {
-      idx += k;
-      if (idx >= M) idx -= M;
-      if (((d - 1) & 31) == 0) {
-        sincospi(-2.0 * (double)idx / (double)M, &zi, &zr);
-      } else {
-        const double nr = zr * wr - zi * wi;
-        zi = zr * wi + zi * wr;
-        zr = nr;
+  const int ra = a.fft_a[job];
+  const int rb = a.fft_b[job];
+  const long long N = a.nphi[ra];
+  const int logm = a.ring_logm[ra];
+  const int Q = 1 << (logm - 4);
+  const int q = threadIdx.x;
+  const bool on = q < Q;
+  const double2 *w = chirp + a.chirp_off[ra];
+  const double2 *bhat = bhat_all + a.bhat_off[ra];
+  const double pa = phi0s[ra], pb = rb >= 0 ? phi0s[rb] : 0.0;
+  const T *Da = diag + c * diag_cs + a.offsets[ra];
+  const T *Db = diag + c * diag_cs + (rb >= 0 ? a.offsets[rb] : a.offsets[ra]);
+  double2 v[16];
+  if (on) {
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+      const long long j = q + s * Q;
+      double2 val = {0, 0};
+      if (j < N) {
+        const double da = (double)Da[j];
+        const double db = rb >= 0 ? (double)Db[j] : 0.0;
+        val = cmul<double>(double2{da, db}, w[j]);
       }
-      acc = fma(K[d].x, zr, acc);
-      acc = fma(-K[d].y, zi, acc);
+      v[s] = val;
     }
-    out[k] = (T)((K[0].x + 2.0 * acc) * invm);
+  }
+  bluestein_conv<double, GB>(v, buf, logm, q, tws, a.log_mmax_fft, bhat);
+  block_sync<GB>();
+  if (on) {
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+      const long long k = q + s * Q;
+      if (k < N) buf[pad16((int)k)] = cmul<double>(w[k], v[s]);
+    }
+  }
+  block_sync<GB>();
+  const int Mc = 1 << logmc;
+  const int Qc = Mc >> 4;
+  const bool onc = q < Qc;
+  const int dmax = 2 * a.mmax;
+  if (onc) {
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+      const int k = q + s * Qc;
+      const bool in = k <= dmax || k >= Mc - dmax;
+      const int ad = k <= dmax ? k : Mc - k;  // |d|
+      double2 z = {0, 0};
+      if (in) {
+        const long long i1 = ad % N, i2 = (N - i1) % N;
+        const double2 X = buf[pad16((int)i1)];
+        const double2 Xc = cconj<double>(buf[pad16((int)i2)]);
+        double2 ka = {0.5 * (X.x + Xc.x), 0.5 * (X.y + Xc.y)};
+        double2 kb = {0.5 * (X.y - Xc.y), -0.5 * (X.x - Xc.x)};
+        if (pa != 0.0) {
+          double sn, cs;
+          sincos(-(double)ad * pa, &sn, &cs);
+          ka = cmul<double>(ka, double2{cs, sn});
+        }
+        if (pb != 0.0) {
+          double sn, cs;
+          sincos(-(double)ad * pb, &sn, &cs);
+          kb = cmul<double>(kb, double2{cs, sn});
+        }
+        if (rb < 0) kb = {0, 0};
+        if (k > dmax) {  // d < 0: K(d) = conj K(-d)
+          ka.y = -ka.y;
+          kb.y = -kb.y;
+        }
+        z = {ka.x - kb.y, ka.y + kb.x};
+      }
+      v[s] = z;
+    }
+  }
+  fft16<double, -1, GB>(v, buf, logmc, q, tws, a.log_mmax_fft);
+  if (onc) {
+    const double inv = 1.0 / (double)Mc;
+    T *oa = mult + c * mult_cs + (long long)ra * Mc;
+    T *ob = mult + c * mult_cs + (long long)(rb >= 0 ? rb : ra) * Mc;
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+      const int k = q + s * Qc;
+      oa[k] = (T)(v[s].x * inv);
+      if (rb >= 0) ob[k] = (T)(v[s].y * inv);
+    }
   }
 }
 
@@ -971,13 +1015,30 @@ void launch_conv_mult(owmi_plan_impl *p, const T *diag, long long diag_cs, int n
                       long long mult_cs, hipStream_t st) {
   const int logm = conv_logm(p);
   OWMI_REQUIRE(logm > 0, "the spectral sandwich does not apply to this plan");
-  const size_t lds = (size_t)(2 * p->mmax + 1) * sizeof(double2);
-  OWMI_REQUIRE(lds <= kMaxLds, "mmax too large for the sandwich multipliers");
-  allow_lds(conv_mult_kernel<T>, lds);
-  OWMI_REQUIRE(p->nrings < (1LL << 31) && ncomp < 65536, "too many rings or components");
-  hipLaunchKernelGGL(conv_mult_kernel<T>, dim3((unsigned)p->nrings, (unsigned)ncomp), dim3(256),
-                     lds, st, diag, diag_cs, p->nphi, p->offsets,
-                     reinterpret_cast<const double *>(p->t64.phi0), p->mmax, logm, mult, mult_cs);
+  OWMI_REQUIRE(ncomp < 65536, "too many components");
+  const size_t lds = ring_lds_bytes<double>(p->mmax_fft);
+  const int threads = std::max(64, p->mmax_fft / 16);
+  const bool gb = lds + 2 * kTwTab * sizeof(double2) > kMaxLds;
+  double2 *gbuf = nullptr;
+  const long long region = p->mmax_fft + p->mmax_fft / 16;
+  if (gb) {
+    const size_t bytes = (size_t)p->nfft_jobs * ncomp * region * sizeof(double2);
+    OWMI_REQUIRE(bytes <= (size_t)32 << 30, "ring-FFT scratch above 32 GiB");
+    gbuf = static_cast<double2 *>(p->ensure_work3(bytes));
+  }
+  auto kern = gb ? conv_mult_kernel<T, 1024, true>
+              : threads <= 256 ? conv_mult_kernel<T, 256, false>
+              : threads <= 512 ? conv_mult_kernel<T, 512, false>
+                               : conv_mult_kernel<T, 1024, false>;
+  const size_t dyn = gb ? 0 : lds;
+  allow_lds(kern, dyn);
+  hipLaunchKernelGGL(kern, dim3((unsigned)p->nfft_jobs, (unsigned)ncomp), dim3(threads), dyn, st,
+                     ring_args<double>(p, ncomp, 0), logm,
+                     reinterpret_cast<const double2 *>(p->t64.twiddle),
+                     reinterpret_cast<const double2 *>(p->t64.chirp),
+                     reinterpret_cast<const double2 *>(p->t64.bhat),
+                     reinterpret_cast<const double *>(p->t64.phi0), diag, diag_cs, mult, mult_cs,
+                     gbuf, region);
 }
 
 void launch_bluestein_init(owmi_plan_impl *p, const std::vector<long long> &lens,

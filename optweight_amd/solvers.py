@@ -259,10 +259,20 @@ class CGWienerMap(CG):
               and (self._mask_op is None or isinstance(self._mask_op, operators.PixMatVecMap)))
         if ok:
             rdt = operators._real_dtype(alm.dtype)
-            d = self.icov_noise.diagonal(alm.device, rdt)
-            if d is not None and self._mask_op is not None:
-                md = self._mask_op.diagonal(alm.device, rdt)
-                d = None if md is None else (md * d * md).contiguous()
+            # the pixel diagonal depends on the noise and mask operators only:
+            # it is kept across resets of the plan (a new S^-1, as the
+            # differentiable solve sets per evaluation), and with it the
+            # spectral-sandwich multipliers cached on it
+            dkey = (key, id(self.icov_noise), id(self._mask_op))
+            cached = getattr(self, '_fast_diag', None)
+            if cached is not None and cached[0] == dkey:
+                d = cached[1]
+            else:
+                d = self.icov_noise.diagonal(alm.device, rdt)
+                if d is not None and self._mask_op is not None:
+                    md = self._mask_op.diagonal(alm.device, rdt)
+                    d = None if md is None else (md * d * md).contiguous()
+                self._fast_diag = (dkey, d, self.icov_noise, self._mask_op)
             if d is not None:
                 beam = (self._beam_op.device_matrix(alm.device, alm.dtype)
                         if self._beam_op is not None else None)
