@@ -364,7 +364,7 @@ long long owmi_sht_plan_npix(const owmi_sht_plan *plan);
 #define OWMI_TUNE_ADJ_NB 5   /* vector adjoint: at most 1, 2 or 4 transforms per block row */
 #define OWMI_TUNE_ADJ_K0 6   /* vector adjoint, spin 0: 1, 2 or 4 ring pairs per lane */
 #define OWMI_TUNE_ADJ_K2 7   /* vector adjoint, spin 2: 1 or 2 ring pairs per lane */
-#define OWMI_TUNE_MFMA64_MIN 8 /* float64 batches from this size take the adjoint on the float64 matrix cores (>= 1; default 8) */
+#define OWMI_TUNE_MFMA64_MIN 8 /* float64 batches from this size take the Legendre synthesis and adjoint on the float64 matrix cores (>= 1; default 8) */
 int owmi_sht_plan_tune(owmi_sht_plan *plan, long long min_batch, int param, int value);
 
 /* Drop every tuned setting of a plan (back to the library's rules). */

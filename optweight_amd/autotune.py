@@ -14,8 +14,8 @@ keeps the fastest value of each:
                          then column tiles of the matrix-core adjoint per
                          spin (OWMI_TUNE_ADJ_CT0 / CT2: 2 or 4);
   float64, batch >= 4:   float64 matrix cores or vector kernels for the
-                         adjoint (OWMI_TUNE_MFMA64_MIN), then as below (the
-                         vector adjoint's choices only if it won);
+                         synthesis and the adjoint (OWMI_TUNE_MFMA64_MIN),
+                         then as below if the vector kernels won;
   otherwise:             transforms per thread of the vector synthesis and
                          per block of the vector adjoint (1, 2, 4), ring
                          pairs per lane of the vector adjoint (spin 0: 1, 2,
@@ -153,13 +153,13 @@ def _candidates(dtype, batch, mfma):
             return [('mfma_min', [1, batch + 1])]
         if mfma:
             return [('adj_ct2', [2, 4]), ('adj_ct0', [2, 4])]
-    if f64 and batch >= 4 and mfma is None:
-        return [('mfma64_min', [1, batch + 1])]
+    if f64 and batch >= 4:
+        if mfma is None:
+            return [('mfma64_min', [1, batch + 1])]
+        if mfma:
+            return []  # both directions run on the float64 matrix cores
     nbs = [v for v in (1, 2, 4) if v <= batch]
-    cands = [('synth_nb', nbs)]
-    if not (f64 and mfma):
-        cands += [('adj_nb', nbs), ('adj_k0', [1, 2, 4]), ('adj_k2', [1, 2])]
-    return cands
+    return [('synth_nb', nbs), ('adj_nb', nbs), ('adj_k0', [1, 2, 4]), ('adj_k2', [1, 2])]
 
 
 def _searched(cands):

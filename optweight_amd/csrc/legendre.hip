@@ -1153,6 +1153,235 @@ template <> struct Mfma16<double> {
   }
 };
 
+// ------------------------------------------- float64 synthesis, matrix cores
+//
+// D[pair][column] += A[pair][depth] B[depth][column] on v_mfma_f64_16x16x4f64
+// tiles: A holds the float64 Legendre values of 16 ring pairs at 4
+// multipoles, B the normalised alm of those multipoles (spin 0: columns
+// (transform, re/im) of 8 transforms per tile; spin 2: 4 transforms per tile
+// with columns (P, s Mm) / (Mm, s P) as the float32 kernel). Every wave is
+// independent (no block barrier): it runs the recurrence of its own 64 ring
+// pairs, 8 multipoles at a time, into a wave-private LDS tile whose 16-pair
+// column blocks are swizzled by depth (conflict-free reads of the 4 depths),
+// and loads its B operands straight from global memory one group of
+// multipoles ahead (the 4 waves of a block read the same alm, so the loads
+// of 3 of them hit L1). Groups start at l - m even: the two depth sets of a
+// group (l - m even / odd) are the spin-0 sums E and O (north E + O, south
+// E - O); at spin 2 both sets accumulate into the tile of their function.
+// Four row tiles (64 pairs) and NCT column tiles: 64 NCT accumulator
+// registers per lane.
+constexpr int kS64G = 8;  // multipoles per group
+
+template <int SPIN, int NCT>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+synth_mfma64_kernel(LegArgs g, const R *__restrict__ xs, const int *__restrict__ lfas,
+                    const R *__restrict__ sas, const int *__restrict__ lfbs,
+                    const R *__restrict__ sbs, const R *__restrict__ coef,
+                    const double2 *__restrict__ alm, long long alm_ts, long long ntrans,
+                    double2 *G) {
+  using V4 = f64x4;
+  constexpr int NF = SPIN == 0 ? 1 : 2;           // Legendre functions per pair
+  constexpr int NT = (SPIN == 0 ? 8 : 4) * NCT;   // transforms per block
+  constexpr int NCF = SPIN == 0 ? 2 : 3;          // recurrence coefficients per multipole
+  constexpr int NACC = SPIN == 0 ? 2 : 2;         // spin 0: E / O; spin 2: functions
+  __shared__ __align__(16) double Tall[4][NF][kS64G][64];
+  const int lane = threadIdx.x & 63;
+  const int wu = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  double(*T)[kS64G][64] = Tall[wu];
+  int bx, m;
+  xcd_block(gridDim.x, gridDim.y, bx, m);
+  const long long wbase = (long long)bx * blockDim.x + 64 * wu;
+  const long long ip = wbase + lane;
+  const long long t0 = (long long)blockIdx.z * NT;
+  const int nt = (int)min((long long)NT, ntrans - t0);
+  const bool active = ip < g.npairs;
+  const int lmax = g.lmax;
+  const long long nelem = nelem_of(lmax);
+  const long long ms = mstart_of(m, lmax);
+  const long long si = (long long)m * g.npairs + ip;
+  const int lfa = active ? lfas[si] : INT_MAX;
+  const int lfb = (SPIN == 2 && active) ? lfbs[si] : INT_MAX;
+  const int lmin = __builtin_amdgcn_readfirstlane(wave_min(min(lfa, lfb)));
+  const int bj = lane & 15, bk = lane >> 4;
+  V4 D[4][NCT][NACC];
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int ct = 0; ct < NCT; ++ct) D[r][ct][0] = D[r][ct][1] = V4{0, 0, 0, 0};
+  if (lmin <= lmax) {  // wave-uniform
+    const R x = active ? xs[ip] : (R)0;
+    const R a0s = active ? sas[2 * si] : (R)0, a1s = active ? sas[2 * si + 1] : (R)0;
+    R b0s = 0, b1s = 0;
+    if (SPIN == 2 && active) {
+      b0s = sbs[2 * si];
+      b1s = sbs[2 * si + 1];
+    }
+    const R *cf = coef + NCF * ms;
+    const double2 *a = alm + t0 * alm_ts + ms;
+    const int lb0 = lmin - ((lmin - m) & 1);
+    // B-operand column of this lane: transform tcol[ct] of the block,
+    // component comp of its alm (spin 2: of the E alm; the B alm's other
+    // component pairs with it), ring side (spin 2) from the column
+    const int comp = bj & 1;
+    int tcol[NCT];
+    bool tin[NCT];
+#pragma unroll
+    for (int ct = 0; ct < NCT; ++ct) {
+      const int tl = SPIN == 0 ? 8 * ct + (bj >> 1) : 4 * ct + (bj >> 2);
+      tin[ct] = tl < nt;
+      tcol[ct] = min(tl, nt - 1);
+    }
+    // raw values of a group: per depth set the normalisation (rc) and per
+    // column tile the alm component(s) (ra: E or the spin-0 alm, rb: B)
+    R rc[2], ra[2][NCT], rb[2][NCT];
+    auto load_raw = [&](int lg) {
+#pragma unroll
+      for (int par = 0; par < 2; ++par) {
+        const int lc = min(lg + 2 * bk + par, lmax);
+        rc[par] = cf[NCF * lc + NCF - 1];
+#pragma unroll
+        for (int ct = 0; ct < NCT; ++ct) {
+          const double *ae = reinterpret_cast<const double *>(a + tcol[ct] * alm_ts + lc);
+          ra[par][ct] = ae[comp];
+          if (SPIN == 2) {
+            const double *ab =
+                reinterpret_cast<const double *>(a + tcol[ct] * alm_ts + nelem + lc);
+            rb[par][ct] = ab[1 - comp];
+          }
+        }
+      }
+    };
+    R a1 = 0, a2 = 0, b1 = 0, b2 = 0;
+    load_raw(lb0);
+    for (int lg = lb0; lg <= lmax; lg += kS64G) {
+      // B operands of this group: Bv[f][par][ct]
+      R Bv[NF][2][NCT];
+#pragma unroll
+      for (int par = 0; par < 2; ++par) {
+        const bool lin = lg + 2 * bk + par <= lmax;
+#pragma unroll
+        for (int ct = 0; ct < NCT; ++ct) {
+          const bool in = lin && tin[ct];
+          if (SPIN == 0) {
+            Bv[0][par][ct] = in ? rc[par] * ra[par][ct] : 0.0;
+          } else {
+            // P = -(e + i b) / 2, Mm = -(e - i b) / 2 (the -1/2 with the
+            // normalisation): component comp of P is e_c - b_{1-c} (c = 0)
+            // or e_c + b_{1-c} (c = 1); columns 2, 3 of a transform take the
+            // south ring's sign s = (-1)^(l - m)
+            const R cl = -0.5 * rc[par];
+            const R sb = comp ? rb[par][ct] : -rb[par][ct];
+            const R P = cl * (ra[par][ct] + sb), Mm = cl * (ra[par][ct] - sb);
+            const bool south = (bj & 2) != 0;
+            const R sg = par ? -1.0 : 1.0;
+            Bv[0][par][ct] = in ? (south ? sg * Mm : P) : 0.0;
+            Bv[SPIN == 2 ? 1 : 0][par][ct] = in ? (south ? sg * P : Mm) : 0.0;
+          }
+        }
+      }
+      if (lg + kS64G <= lmax) load_raw(lg + kS64G);
+      // recurrence of the lane's ring pair over the group -> tile rows
+      // (column block swizzled by depth: row i, lane L at L ^ 16 ((i / 2) % 4))
+      const bool inj = __any((lfa >= lg && lfa < lg + kS64G) || (lfb >= lg && lfb < lg + kS64G));
+#pragma unroll
+      for (int i = 0; i < kS64G; ++i) {
+        const int l = lg + i;
+        const int lc = min(l, lmax);
+        const int col = lane ^ (16 * ((i >> 1) & 3));
+        const bool lin = l <= lmax;
+        if (SPIN == 0) {
+          const R ax = cf[2 * lc] * x;
+          R v;
+          if (inj) {
+            v = rec_step(ax, a1, a2, l, lfa, a0s, a1s);
+          } else {
+            v = fma(ax, a1, -a2);
+            a2 = a1;
+            a1 = v;
+          }
+          T[0][i][col] = lin ? v : 0.0;
+        } else {
+          const R ax = cf[3 * lc] * x, bc = cf[3 * lc + 1];
+          R p, q;
+          if (inj) {
+            p = rec_step(ax + bc, a1, a2, l, lfa, a0s, a1s);
+            q = rec_step(ax - bc, b1, b2, l, lfb, b0s, b1s);
+          } else {
+            p = fma(ax + bc, a1, -a2);
+            q = fma(ax - bc, b1, -b2);
+            a2 = a1;
+            a1 = p;
+            b2 = b1;
+            b1 = q;
+          }
+          T[0][i][col] = lin ? p : 0.0;
+          T[SPIN == 2 ? 1 : 0][i][col] = lin ? q : 0.0;
+        }
+      }
+      wave_sync();
+      // products: depth k = bk of set par is row 2k + par
+#pragma unroll
+      for (int par = 0; par < 2; ++par) {
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+          double A[4];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) A[r] = T[f][2 * bk + par][(16 * r + bj) ^ (16 * bk)];
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int ct = 0; ct < NCT; ++ct) {
+              const int c = SPIN == 0 ? par : f;
+              D[r][ct][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[r], Bv[f][par][ct],
+                                                                 D[r][ct][c], 0, 0, 0);
+            }
+        }
+      }
+      wave_sync();
+    }
+  }
+  // D[i][j] of row tile r: register i / 4 at lane j + 16 (i % 4); pair
+  // wbase + 16 r + 4 reg + bk, column bj
+  const bool odd = bj & 1;
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int reg = 0; reg < 4; ++reg) {
+      const long long pp = wbase + 16 * r + 4 * reg + bk;
+#pragma unroll
+      for (int ct = 0; ct < NCT; ++ct) {
+        const double v0 = D[r][ct][0][reg], v1 = D[r][ct][1][reg];
+        if (SPIN == 0) {
+          // column bj: transform 8 ct + bj / 2, component bj % 2; the even
+          // lane stores the north value E + O, the odd lane the south E - O
+          const double vn = v0 + v1, vs = v0 - v1;
+          const double recv = __shfl_xor(odd ? vn : vs, 1);
+          const int b = 8 * ct + (bj >> 1);
+          if (pp >= g.npairs || b >= nt) continue;
+          const int ring = odd ? g.pair_s[pp] : g.pair_n[pp];
+          if (ring < 0) continue;
+          G[coef_syn_at(t0 + b, m, pp, g.mmax, g.npairs) + odd] =
+              odd ? double2{recv, vs} : double2{vn, recv};
+        } else {
+          // column c = bj % 4 of transform 4 ct + bj / 4: q component c % 2
+          // and u component 1 - c % 2 of ring side c / 2; the even lane
+          // stores Q, the odd lane U
+          const int c = bj & 3;
+          const double q = v0 + v1;
+          const double u = (c == 0 || c == 3) ? v1 - v0 : v0 - v1;
+          const double recv = __shfl_xor(odd ? q : u, 1);
+          const int b = 4 * ct + (bj >> 2);
+          if (pp >= g.npairs || b >= nt) continue;
+          const int ring = c < 2 ? g.pair_n[pp] : g.pair_s[pp];
+          if (ring < 0) continue;
+          const long long qi =
+              coef_syn_at(2 * (t0 + b) + odd, m, pp, g.mmax, g.npairs) + (c >> 1);
+          G[qi] = odd ? double2{u, recv} : double2{q, recv};
+        }
+      }
+    }
+}
+
 constexpr int kMaCH = 32;  // multipoles per adjoint chunk
 
 // E: element type (float: float32 alm and ring coefficients, float64
@@ -1633,6 +1862,31 @@ static void launch_synth_mfma(owmi_plan_impl *p, int spin, const float2 *alm, lo
   }
 }
 
+// float64: NCT column tiles of 8 (spin 0) or 4 (spin 2) transforms; two
+// once the batch fills them.
+template <int SPIN>
+static void launch_synth_mfma64(owmi_plan_impl *p, const double2 *alm, long long alm_ts,
+                                double2 *G, long long ntrans, hipStream_t st) {
+  PlanTables<R> &tb = p->t64;
+  const int bs = (int)std::min<long long>(256, ((p->npairs + 63) / 64) * 64);
+  const int per = SPIN == 0 ? 8 : 4;
+  const int nct = ntrans >= 2 * per ? 2 : 1;
+  dim3 grid(ceil_div(p->npairs, bs), (unsigned)(p->mmax + 1), ceil_div(ntrans, per * nct));
+  const int *lfa = SPIN == 0 ? tb.lf0 : tb.lfp;
+  const R *sa = SPIN == 0 ? tb.s0 : tb.sp;
+  const int *lfb = SPIN == 0 ? nullptr : tb.lfm;
+  const R *sb = SPIN == 0 ? nullptr : tb.sm;
+  const R *cf = SPIN == 0 ? tb.coef0 : tb.coef2;
+  if (nct == 2)
+    hipLaunchKernelGGL((synth_mfma64_kernel<SPIN, 2>), grid, dim3(bs), 0, st, leg_args(p), tb.x,
+                       lfa, sa, lfb, sb, cf, alm, alm_ts, ntrans, G);
+  else
+    hipLaunchKernelGGL((synth_mfma64_kernel<SPIN, 1>), grid, dim3(bs), 0, st, leg_args(p), tb.x,
+                       lfa, sa, lfb, sb, cf, alm, alm_ts, ntrans, G);
+}
+
+static bool use_mfma64(const owmi_plan_impl *p, long long ntrans);
+
 // Transforms per block: the widest of 4, 2, 1 not exceeding ntrans.
 template <typename T>
 void launch_synthesis(owmi_plan_impl *p, int spin, const cplx_t<T> *alm, long long alm_ts,
@@ -1640,6 +1894,11 @@ void launch_synthesis(owmi_plan_impl *p, int spin, const cplx_t<T> *alm, long lo
   OWMI_REQUIRE(ntrans >= 1 && ntrans < 65536, "ntrans must be in [1, 65535]");
   if constexpr (std::is_same<T, float>::value) {
     if (use_mfma(p, ntrans)) return launch_synth_mfma(p, spin, alm, alm_ts, G, ntrans, st);
+  } else {
+    if (use_mfma64(p, ntrans)) {
+      if (spin == 0) return launch_synth_mfma64<0>(p, alm, alm_ts, G, ntrans, st);
+      return launch_synth_mfma64<2>(p, alm, alm_ts, G, ntrans, st);
+    }
   }
   const int nb = nb_for(ntrans, p->tuned(OWMI_TUNE_SYNTH_NB, ntrans));
   if (nb == 4)
@@ -1776,9 +2035,10 @@ static void launch_adj_mfma(owmi_plan_impl *p, const cplx_t<E> *G, cplx_t<E> *al
     launch_adj_mfma_ct<E, SPIN, 8>(p, G, alm, alm_ts, ntrans, st);
 }
 
-// float64 batches from this size on the float64 matrix cores (tuned value,
-// else 8: at 4 transforms the vector adjoint is still faster at lmax 1023;
-// OWMI_LEGENDRE_VALU=1 keeps every precision on the vector kernels).
+// float64 batches from this size on the float64 matrix cores, synthesis and
+// adjoint (tuned value, else 8: at 4 transforms the vector adjoint is still
+// faster at lmax 1023; OWMI_LEGENDRE_VALU=1 keeps every precision on the
+// vector kernels).
 static bool use_mfma64(const owmi_plan_impl *p, long long ntrans) {
   static const bool off = [] {
     const char *e = std::getenv("OWMI_LEGENDRE_VALU");

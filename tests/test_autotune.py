@@ -50,6 +50,12 @@ def test_single_value_parameters_are_not_set():
         assert 'adj_k0' in names and 'adj_k2' in names
     names = [n for n, _ in autotune._searched(autotune._candidates(torch.float64, 2, False))]
     assert 'synth_nb' in names and 'adj_nb' in names
+    # float64 batches: the matrix-core choice covers both directions; the
+    # vector kernels' choices follow only when the vector kernels win
+    assert autotune._candidates(torch.float64, 128, None) == [('mfma64_min', [1, 129])]
+    assert autotune._candidates(torch.float64, 128, True) == []
+    names = [n for n, _ in autotune._searched(autotune._candidates(torch.float64, 128, False))]
+    assert 'synth_nb' in names and 'adj_k2' in names
 
 
 def rand_alm(ntrans, lmax, seed):

@@ -360,12 +360,50 @@ def test_adjoint_fp64_batched(dev, lmax, ntrans, adjoint):
     assert rel(out.cpu().numpy(), ref) < 1e-12
 
 
+@pytest.mark.parametrize('lmax,nrings', [(2, 3), (3, 4), (5, 6), (17, 18), (70, 71),
+                                         (200, 201), (60, 401), (60, 1101)])
+@pytest.mark.parametrize('ntrans', [8, 13, 16, 24, 33])
+def test_synthesis_fp64_matrix_cores(dev, lmax, nrings, ntrans):
+    '''float64 batches of at least 8 transforms take the synthesis on the
+    float64 matrix cores (spin 0: 8 or 16 transforms per block, spin 2: 4 or
+    8; partial column tiles at 13, 24 and 33): every transform of a TQU
+    batch against the float64 vector synthesis of the same plan (relative
+    1e-13), and the first, eighth and last transforms against the oracle
+    (1e-12). Band limits below one group of 8 multipoles (lmax 2, 3, 5), an
+    odd ring count (an equator ring without a mirror), one partial wave of
+    ring pairs (9, 36), two waves (101), four-wave blocks (201 pairs) and
+    several blocks (551 pairs).'''
+    from optweight_amd import _lib, autotune
+    print(f'fp64 matrix-core synthesis lmax={lmax} nrings={nrings} ntrans={ntrans}', flush=True)
+    nphi = 2 * lmax + 1
+    g = og.gauss_legendre(nrings, nphi)
+    m = map_utils.MapInfo.map_info_gauss_legendre(nrings, nphi)
+    ai = alm_utils.AlmInfo(lmax)
+    a = np.stack([rand_alm(lmax, 3, 60 + s) for s in range(ntrans)])
+    plan = sht.get_plan(m, lmax, dev)
+    got = []
+    for vector in (False, True):
+        _lib.call('owmi_sht_plan_tune_reset', plan.handle)
+        if vector:
+            _lib.call('owmi_sht_plan_tune', plan.handle, 1, autotune.TUNE_MFMA64_MIN, 1 << 20)
+        out = torch.zeros((ntrans, 3, m.npix), dtype=torch.float64, device=dev)
+        sht.alm2map(torch.as_tensor(a, device=dev), out, ai, m, [0, 2])
+        got.append(out.cpu().numpy())
+    _lib.call('owmi_sht_plan_tune_reset', plan.handle)
+    for t in range(ntrans):
+        assert rel(got[0][t], got[1][t]) < 1e-13, t
+    for t in (0, 7, ntrans - 1):
+        ref = np.zeros((3, g.npix))
+        osht.alm2map(a[t], ref, lmax, g, [0, 2])
+        assert rel(got[0][t], ref) < 1e-12, t
+
+
 @pytest.mark.parametrize('lmax,rings', [(150, None), (600, None), (600, 'subset')])
 def test_fp64_matrix_cores_match_vector_kernels(dev, lmax, rings):
-    '''The float64 matrix-core adjoint equals the float64 vector adjoint
-    (the plan's threshold raised out of reach) to relative 1e-13 on
-    geometries of 76 and 301 ring pairs (blocks of 2 to 8 waves, several
-    ring-pair groups for spin 2) and on a ring subset.'''
+    '''The float64 matrix-core synthesis and adjoint equal the float64
+    vector kernels (the plan's threshold raised out of reach) to relative
+    1e-13 on geometries of 76 and 301 ring pairs (blocks of 2 to 8 waves,
+    several ring-pair groups for spin 2) and on a ring subset.'''
     from optweight_amd import _lib, autotune
     print(f'fp64 matrix cores vs vector lmax={lmax} rings={rings}', flush=True)
     m = map_utils.MapInfo.map_info_gauss_legendre(lmax + 1, 2 * lmax + 1)
@@ -374,7 +412,8 @@ def test_fp64_matrix_cores_match_vector_kernels(dev, lmax, rings):
     rng = np.random.default_rng(4)
     f = torch.as_tensor(rng.normal(size=(16, 3, m.npix)), device=dev)
     plan = sht.get_plan(m, lmax, dev, rings=ring_sel)
-    outs = []
+    a = torch.as_tensor(np.stack([rand_alm(lmax, 3, 70 + s) for s in range(16)]), device=dev)
+    outs, maps = [], []
     for vector in (False, True):
         _lib.call('owmi_sht_plan_tune_reset', plan.handle)
         if vector:
@@ -382,6 +421,10 @@ def test_fp64_matrix_cores_match_vector_kernels(dev, lmax, rings):
         out = torch.zeros((16, 3, ai.nelem), dtype=torch.complex128, device=dev)
         sht.map2alm(f, out, m, ai, [0, 2], rings=ring_sel)
         outs.append(out.cpu().numpy())
+        mp = torch.zeros((16, 3, m.npix), dtype=torch.float64, device=dev)
+        sht.alm2map(a, mp, ai, m, [0, 2], rings=ring_sel)
+        maps.append(mp.cpu().numpy())
     _lib.call('owmi_sht_plan_tune_reset', plan.handle)
     for c in (slice(0, 1), slice(1, 3)):
         assert rel(outs[0][:, c], outs[1][:, c]) < 1e-13
+        assert rel(maps[0][:, c], maps[1][:, c]) < 1e-13
