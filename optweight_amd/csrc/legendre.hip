@@ -437,7 +437,8 @@ synth_mfma_kernel(LegArgs g, const R *__restrict__ xs, const int *__restrict__ l
   // recurrence runs one group of multipoles ahead of the products)
   __shared__ R sA[2][kMfCH];
   __shared__ R sC[2][kMfCH];
-  __shared__ __align__(16) float sB[NCB][kMfCH][32];
+  // 4 padding rows: the B reads of the group after a chunk's last
+  __shared__ __align__(16) float sB[NCB][kMfCH + 4][32];
   __shared__ int smin;
   const int lane = threadIdx.x & 63;
   int bx, m;
@@ -609,9 +610,9 @@ synth_mfma_kernel(LegArgs g, const R *__restrict__ xs, const int *__restrict__ l
     // group g + 1, the coefficients of group g + 2 and the B operands of
     // group g + 1 in flight from LDS
     R kc[4];
-    float alo, ahi, blo, bhi;
+    float av[4];  // A operands (alo, ahi, blo, bhi) of the current group
     load_coefs(0, 0, kc);
-    operands(std::true_type{}, kc, lmin, alo, ahi, blo, bhi);
+    operands(std::true_type{}, kc, lmin, av[0], av[1], av[2], av[3]);
     load_coefs(STEP, 0, kc);
     // start multipoles of this lane within [lo, lo + n)
     auto starts_in = [&](int lo, int n) {
@@ -634,28 +635,36 @@ synth_mfma_kernel(LegArgs g, const R *__restrict__ xs, const int *__restrict__ l
       const int nls = (min(kMfCH, lmax - l0 + 1) + STEP - 1) & ~(STEP - 1);
       float bv[NACC];
       load_bv(0, bv);
-      for (int i = 0; i < nls; i += STEP) {
-        float nalo, nahi, nblo, nbhi;
+      // one group: the products of the current operands (a, b) while the
+      // next group's operands (na, nb) are formed; the B rows of the group
+      // after a chunk's last read the padding rows of sB (never used)
+      auto group = [&](int i, const float(&a)[4], const float(&b)[NACC], float(&na)[4],
+                       float(&nb)[NACC]) {
         if (slow)
-          operands(std::true_type{}, kc, l0 + i + STEP, nalo, nahi, nblo, nbhi);
+          operands(std::true_type{}, kc, l0 + i + STEP, na[0], na[1], na[2], na[3]);
         else
-          operands(std::false_type{}, kc, l0 + i + STEP, nalo, nahi, nblo, nbhi);
+          operands(std::false_type{}, kc, l0 + i + STEP, na[0], na[1], na[2], na[3]);
         load_coefs(i + 2 * STEP, cb, kc);
-        float nbv[NACC];
-        load_bv(min(i + STEP, kMfCH - STEP), nbv);
+        load_bv(i + STEP, nb);
 #pragma unroll
         for (int c = 0; c < NACC; c += 2) {
-          acc[c][0] = mfma32(alo, bv[c], acc[c][0]);
-          acc[c][1] = mfma32(ahi, bv[c], acc[c][1]);
-          acc[c + 1][0] = mfma32(blo, bv[c + 1], acc[c + 1][0]);
-          acc[c + 1][1] = mfma32(bhi, bv[c + 1], acc[c + 1][1]);
+          acc[c][0] = mfma32(a[0], b[c], acc[c][0]);
+          acc[c][1] = mfma32(a[1], b[c], acc[c][1]);
+          acc[c + 1][0] = mfma32(a[2], b[c + 1], acc[c + 1][0]);
+          acc[c + 1][1] = mfma32(a[3], b[c + 1], acc[c + 1][1]);
         }
-        alo = nalo;
-        ahi = nahi;
-        blo = nblo;
-        bhi = nbhi;
+      };
+      // two groups per iteration, the operand sets alternating (no copies)
+      float a2[4], b2[NACC];
+      int i = 0;
+      for (; i + 2 * STEP <= nls; i += 2 * STEP) {
+        group(i, av, bv, a2, b2);
+        group(i + STEP, a2, b2, av, bv);
+      }
+      if (i < nls) {
+        group(i, av, bv, a2, b2);
 #pragma unroll
-        for (int c = 0; c < NACC; ++c) bv[c] = nbv[c];
+        for (int k = 0; k < 4; ++k) av[k] = a2[k];
       }
     }
   }
