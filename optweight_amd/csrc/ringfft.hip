@@ -280,6 +280,27 @@ struct Twiddles {
   }
 };
 
+// Twiddle W_{16 Ns}^{r k} of the radix-16 pass with Ns = 2^lns.
+template <typename T>
+__device__ __forceinline__ cplx_t<T> pass_twiddle(const Twiddles<T> &tw, int lns, int log_mmax,
+                                                  int r, int k) {
+  return tw(r * (k << (log_mmax - lns - 4)));  // W_Mmax^(r k Mmax / (16 Ns))
+}
+
+// Direct tables for FFTs of length 4096 (passes Ns = 16 and 256): entries
+// of W_256 and W_4096 read as they are, no product.
+template <typename T>
+struct Twiddles4096 {
+  const cplx_t<T> *t256;
+  const cplx_t<T> *t4096;
+};
+
+template <typename T>
+__device__ __forceinline__ cplx_t<T> pass_twiddle(const Twiddles4096<T> &tw, int lns, int, int r,
+                                                  int k) {
+  return lns == 4 ? tw.t256[r * k] : tw.t4096[r * k];
+}
+
 // Fill the tables of a Twiddles from the plan's table tw (Mmax entries);
 // the caller's next barrier makes them visible.
 template <typename T>
@@ -292,14 +313,18 @@ __device__ __forceinline__ Twiddles<T> load_twiddles(const cplx_t<T> *tw, int lo
 }
 
 // Full FFT of the thread's 16 slots (see above). All threads of the block
-// call it (barriers); threads with q >= Q pass through.
-template <typename T, int DIR, bool GB>
-__device__ __forceinline__ void fft16(cplx_t<T> *v, cplx_t<T> *buf, int logm, int q,
-                                      const Twiddles<T> &tw, int log_mmax) {
+// call it (barriers); threads with q >= Q pass through. LOGM > 0 fixes the
+// length at compile time (logm is then ignored): the passes unroll and the
+// slot offsets fold into the LDS instructions. For a stride that is a
+// multiple of 16 the padded index is linear: pad16(b + r s) = pad16(b) +
+// r (s + s / 16).
+template <typename T, int DIR, bool GB, int LOGM = 0, typename TW = Twiddles<T>>
+__device__ __forceinline__ void fft16(cplx_t<T> *v, cplx_t<T> *buf, int logm_rt, int q,
+                                      const TW &tw, int log_mmax) {
+  const int logm = LOGM > 0 ? LOGM : logm_rt;
   const int Q = 1 << (logm - 4);
   const bool on = q < Q;
   const int r0 = logm & 3;
-  int Ns;
   block_sync<GB>();  // the previous user of buf has finished reading it
   if (r0 == 0) {
     if (logm == 4) {
@@ -309,9 +334,8 @@ __device__ __forceinline__ void fft16(cplx_t<T> *v, cplx_t<T> *buf, int logm, in
     if (on) {
       Dft<T, DIR, 16>::run(v, 0, 1);
 #pragma unroll
-      for (int r = 0; r < 16; ++r) buf[pad16(q * 16 + r)] = v[r];
+      for (int r = 0; r < 16; ++r) buf[17 * q + r] = v[r];  // pad16(16 q + r)
     }
-    Ns = 16;
   } else {
     if (on) {
       if (r0 == 1)
@@ -321,22 +345,29 @@ __device__ __forceinline__ void fft16(cplx_t<T> *v, cplx_t<T> *buf, int logm, in
       else
         first_pass_small<T, DIR, 8>(v, buf, q, Q);
     }
-    Ns = 1 << r0;
   }
   // radix-16 passes; the last one keeps its outputs in registers
-  int lns = r0 == 0 ? 4 : r0;  // log2 Ns
-  while (true) {
-    block_sync<GB>();
-    const bool last = lns + 4 == logm;
-    const int k = q & (Ns - 1);
-    if (on) {
+  const int lns0 = r0 == 0 ? 4 : r0;  // log2 Ns of the first of them
+  const int npass = (logm - lns0) >> 2;
 #pragma unroll
-      for (int r = 0; r < 16; ++r) v[r] = buf[pad16(q + r * Q)];
+  for (int pass = 0; pass < npass; ++pass) {
+    const int lns = lns0 + 4 * pass;
+    block_sync<GB>();
+    const bool last = pass == npass - 1;
+    const int k = q & ((1 << lns) - 1);
+    if (on) {
+      if (Q >= 16) {
+        const cplx_t<T> *src = buf + pad16(q);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) v[r] = src[r * (Q + (Q >> 4))];
+      } else {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) v[r] = buf[pad16(q + r * Q)];
+      }
       if (k != 0) {
-        const int step = k << (log_mmax - lns - 4);  // k Mmax / (16 Ns)
 #pragma unroll
         for (int r = 1; r < 16; ++r) {
-          cplx_t<T> w = tw(r * step);
+          cplx_t<T> w = pass_twiddle(tw, lns, log_mmax, r, k);
           if (DIR > 0) w.y = -w.y;
           v[r] = cmul<T>(v[r], w);
         }
@@ -347,11 +378,16 @@ __device__ __forceinline__ void fft16(cplx_t<T> *v, cplx_t<T> *buf, int logm, in
     block_sync<GB>();
     if (on) {
       const int base = ((q >> lns) << (lns + 4)) + k;
+      if (lns >= 4) {
+        cplx_t<T> *dst = buf + pad16(base);
+        const int st = (1 << lns) + (1 << (lns - 4));
 #pragma unroll
-      for (int r = 0; r < 16; ++r) buf[pad16(base + (r << lns))] = v[r];
+        for (int r = 0; r < 16; ++r) dst[r * st] = v[r];
+      } else {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) buf[pad16(base + (r << lns))] = v[r];
+      }
     }
-    Ns <<= 4;
-    lns += 4;
   }
 }
 
@@ -595,10 +631,11 @@ ring_kernel(RingArgs a, const cplx_t<T> *twiddle, const cplx_t<T> *chirp,
 
 // One workgroup per (ring pair, component, group of TPB transforms): the
 // multipliers are read once per group (TPB 8 for workgroups of up to 256
-// threads; 1 beyond, where the loop's registers would spill).
-template <typename T, int MAXT, bool GB, int TPB>
+// threads; 1 beyond, where the loop's registers would spill). LOGM > 0: the
+// FFT length fixed at compile time (logm_rt ignored).
+template <typename T, int MAXT, bool GB, int TPB, int LOGM = 0>
 __global__ void __launch_bounds__(MAXT)
-ring_conv_kernel(RingArgs a, int logm, const cplx_t<T> *twiddle, const T *mult,
+ring_conv_kernel(RingArgs a, int logm_rt, const cplx_t<T> *twiddle, const T *mult,
                  long long mult_cs, const cplx_t<T> *Gin, cplx_t<T> *Gout, long long ntrans,
                  cplx_t<T> *gbuf, long long region) {
   extern __shared__ unsigned char smem_raw[];
@@ -606,6 +643,16 @@ ring_conv_kernel(RingArgs a, int logm, const cplx_t<T> *twiddle, const T *mult,
                       : reinterpret_cast<cplx_t<T> *>(smem_raw);
   __shared__ cplx_t<T> tw_lo[kTwTab], tw_hi[kTwTab];
   const Twiddles<T> tws = load_twiddles<T>(twiddle, a.log_mmax_fft, tw_lo, tw_hi);
+  // float32 at M = 4096: direct twiddle tables (34 KiB of LDS; the
+  // workgroups per CU are bound by registers, not LDS, at this size)
+  constexpr bool DIRECT = LOGM == 12 && sizeof(T) == 4;
+  __shared__ __align__(16) cplx_t<T> tdir[DIRECT ? 4096 + 256 : 1];
+  if constexpr (DIRECT) {
+    const int sh = a.log_mmax_fft - 12;
+    for (int i = threadIdx.x; i < 4096; i += blockDim.x) tdir[i] = twiddle[i << sh];
+    for (int i = threadIdx.x; i < 256; i += blockDim.x) tdir[4096 + i] = twiddle[i << (sh + 4)];
+  }
+  const Twiddles4096<T> twd = {tdir + 4096, tdir};
   // workgroups of one XCD take a contiguous range of ring pairs (they share
   // the cache lines of the pair-major coefficients)
   const long long gx = gridDim.x;
@@ -616,6 +663,7 @@ ring_conv_kernel(RingArgs a, int logm, const cplx_t<T> *twiddle, const T *mult,
   const long long t_hi = min(ntrans, t_lo + TPB);
   const int ra = a.fft_a[job];
   const int rb = a.fft_b[job];
+  const int logm = LOGM > 0 ? LOGM : logm_rt;
   const int M = 1 << logm;
   const int Q = M >> 4;
   const int q = threadIdx.x;
@@ -680,19 +728,32 @@ ring_conv_kernel(RingArgs a, int logm, const cplx_t<T> *twiddle, const T *mult,
         v[s] = {za.x - zb.y, za.y + zb.x};
       }
     }
-    fft16<T, -1, GB>(v, buf, logm, q, tws, a.log_mmax_fft);
+    if constexpr (DIRECT)
+      fft16<T, -1, GB, LOGM>(v, buf, logm, q, twd, a.log_mmax_fft);
+    else
+      fft16<T, -1, GB, LOGM>(v, buf, logm, q, tws, a.log_mmax_fft);
     if (on) {
       if (!KEEP) load_mult();
 #pragma unroll
       for (int s = 0; s < 16; ++s) v[s] = {v[s].x * ma[s], v[s].y * mb[s]};
     }
-    fft16<T, 1, GB>(v, buf, logm, q, tws, a.log_mmax_fft);
+    if constexpr (DIRECT)
+      fft16<T, 1, GB, LOGM>(v, buf, logm, q, twd, a.log_mmax_fft);
+    else
+      fft16<T, 1, GB, LOGM>(v, buf, logm, q, tws, a.log_mmax_fft);
     block_sync<GB>();
     if (on) {
+      // pad16(q + s Q) = pad16(q) + s (Q + Q / 16) for Q >= 16
+      cplx_t<T> *dst = buf + pad16(q);
 #pragma unroll
       for (int s = 0; s < 16; ++s) {
         const int k = q + s * Q;
-        if (k <= mmax || k >= M - mmax) buf[pad16(k)] = v[s];
+        if (k <= mmax || k >= M - mmax) {
+          if (Q >= 16)
+            dst[s * (Q + (Q >> 4))] = v[s];
+          else
+            buf[pad16(k)] = v[s];
+        }
       }
     }
     block_sync<GB>();
@@ -960,13 +1021,15 @@ void launch_conv(owmi_plan_impl *p, const cplx_t<T> *G, cplx_t<T> *Gout, const T
     OWMI_REQUIRE(bytes <= (size_t)32 << 30, "ring-FFT scratch above 32 GiB: split the batch");
     gbuf = static_cast<cplx_t<T> *>(p->ensure_work3(bytes));
   }
+  // M = 4096 (band limits 512 .. 1023) with its length fixed at compile time
   auto kern = gb ? ring_conv_kernel<T, 1024, true, 1>
               : threads > 512 ? ring_conv_kernel<T, 1024, false, 1>
               : threads > 256 ? ring_conv_kernel<T, 512, false, 1>
               : tpb == 1 ? ring_conv_kernel<T, 256, false, 1>
               : tpb == 2 ? ring_conv_kernel<T, 256, false, 2>
               : tpb == 4 ? ring_conv_kernel<T, 256, false, 4>
-                         : ring_conv_kernel<T, 256, false, 8>;
+              : logm == 12 ? ring_conv_kernel<T, 256, false, 8, 12>
+                           : ring_conv_kernel<T, 256, false, 8>;
   const size_t dyn = gb ? 0 : lds;
   allow_lds(kern, dyn);
   hipLaunchKernelGGL(kern, grid, dim3(threads), dyn, st, ring_args<T>(p, ncomp, 0), logm,
