@@ -413,24 +413,25 @@ struct RingArgs {
 
 // Cyclic convolution with the ring's Bluestein kernel, in the thread's
 // slots: v <- IDFT_M(DFT_M(v) * bhat) (bhat includes 1/M).
-template <typename T, bool GB>
+template <typename T, bool GB, int LOGM = 0>
 __device__ __forceinline__ void bluestein_conv(cplx_t<T> *v, cplx_t<T> *buf, int logm, int q,
                                                const Twiddles<T> &tw, int log_mmax,
                                                const cplx_t<T> *bhat) {
   const int Q = 1 << (logm - 4);
-  fft16<T, -1, GB>(v, buf, logm, q, tw, log_mmax);
+  fft16<T, -1, GB, LOGM>(v, buf, logm, q, tw, log_mmax);
   if (q < Q) {
 #pragma unroll
     for (int s = 0; s < 16; ++s) v[s] = cmul<T>(v[s], bhat[q + s * Q]);
   }
-  fft16<T, 1, GB>(v, buf, logm, q, tw, log_mmax);
+  fft16<T, 1, GB, LOGM>(v, buf, logm, q, tw, log_mmax);
 }
 
 // MAXT: the launch's workgroup size bound (M / 16 threads rounded up to
 // 256, 512 or 1024), which sets the register budget per thread. GB: the
 // FFT buffer of each workgroup is a region of gbuf (region elements) in
-// global memory instead of LDS.
-template <typename T, int MODE, int MAXT, bool GB>
+// global memory instead of LDS. LOGM > 0: every ring of the plan has the
+// Bluestein length 2^LOGM, fixed at compile time.
+template <typename T, int MODE, int MAXT, bool GB, int LOGM = 0>
 __global__ void __launch_bounds__(MAXT)
 ring_kernel(RingArgs a, const cplx_t<T> *twiddle, const cplx_t<T> *chirp,
             const cplx_t<T> *bhat_all, const T *ringw, const T *phi0s, const cplx_t<T> *Gin,
@@ -457,7 +458,7 @@ ring_kernel(RingArgs a, const cplx_t<T> *twiddle, const cplx_t<T> *chirp,
   const int ra = a.fft_a[job];
   const int rb = a.fft_b[job];
   const long long N = a.nphi[ra];
-  const int logm = a.ring_logm[ra];
+  const int logm = LOGM > 0 ? LOGM : a.ring_logm[ra];
   const int Q = 1 << (logm - 4);
   const int q = threadIdx.x;
   const bool on = q < Q;
@@ -505,7 +506,7 @@ ring_kernel(RingArgs a, const cplx_t<T> *twiddle, const cplx_t<T> *chirp,
         v[s] = val;
       }
     }
-    bluestein_conv<T, GB>(v, buf, logm, q, tws, a.log_mmax_fft, bhat);
+    bluestein_conv<T, GB, LOGM>(v, buf, logm, q, tws, a.log_mmax_fft, bhat);
     // DFT(conj Z)_j = w_j c_j and x = conj of it: ring a gets Re(w_j c_j),
     // ring b gets -Im(w_j c_j).
     if (on) {
@@ -560,7 +561,7 @@ ring_kernel(RingArgs a, const cplx_t<T> *twiddle, const cplx_t<T> *chirp,
   }
 
   if (MODE == kAdjoint || MODE == kSandwich) {
-    bluestein_conv<T, GB>(v, buf, logm, q, tws, a.log_mmax_fft, bhat);
+    bluestein_conv<T, GB, LOGM>(v, buf, logm, q, tws, a.log_mmax_fft, bhat);
     // X_k = w_k c_k, gathered in the buffer (natural order) for the Hermitian split
     block_sync<GB>();
     if (on) {
@@ -980,7 +981,10 @@ void launch_ring(owmi_plan_impl *p, const cplx_t<T> *Gin, cplx_t<T> *Gout, const
     OWMI_REQUIRE(bytes <= (size_t)32 << 30, "ring-FFT scratch above 32 GiB: split the batch");
     gbuf = static_cast<cplx_t<T> *>(p->ensure_work3(bytes));
   }
+  // every ring of Bluestein length 4096 (nphi 1025 .. 2048, e.g. the
+  // Gauss-Legendre rings of lmax 1023): length fixed at compile time
   auto kern = gb ? ring_kernel<T, MODE, 1024, true>
+              : (threads == 256 && p->uniform_logm == 12) ? ring_kernel<T, MODE, 256, false, 12>
               : threads <= 256 ? ring_kernel<T, MODE, 256, false>
               : threads <= 512 ? ring_kernel<T, MODE, 512, false>
                                : ring_kernel<T, MODE, 1024, false>;

@@ -89,6 +89,7 @@ struct owmi_plan_impl {
   long long nrings = 0, npix = 0, npairs = 0;
   int log2_mmax_fft = 0;  // log2 of the largest Bluestein length
   int mmax_fft = 0;       // largest Bluestein length M
+  int uniform_logm = 0;   // log2 of the Bluestein length when all rings share it, else 0
 
   // host copies
   std::vector<long long> h_nphi, h_offsets;

@@ -545,6 +545,7 @@ int create_plan(owmi_sht_plan **out, long long nrings, const double *theta,
       p.nchirp = chirp.size() / 2;
       p.nbhat = nbhat;
       p.log2_mmax_fft = ilog2_ceil(p.mmax_fft);
+      p.uniform_logm = logms.size() == 1 ? logms[0] : 0;
 
       // FFT jobs: mirror pairs share nphi on symmetric grids; other rings
       // are paired with the next ring of equal nphi, leftovers run alone.
