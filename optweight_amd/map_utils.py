@@ -378,6 +378,20 @@ def ring_support(mask, minfo):
     return np.nonzero(hit)[0].astype(np.int64)
 
 
+def compact_rings(minfo, rings):
+    '''The geometry of the rings ``rings`` (sorted indices) of minfo alone,
+    stored one after another in ring order, and for each of its pixels the
+    index of the same pixel in a map of minfo: (MapInfo, int64 array).'''
+    rings = np.asarray(rings, dtype=np.int64)
+    nphi = np.asarray(minfo.nphi, dtype=np.int64)[rings]
+    off = np.asarray(minfo.offsets, dtype=np.int64)[rings]
+    sub = MapInfo(np.asarray(minfo.theta)[rings], np.asarray(minfo.weight)[rings], nphi=nphi,
+                  phi0=np.asarray(minfo.phi0)[rings])
+    pix = (np.concatenate([o + np.arange(n) for o, n in zip(off, nphi)]) if rings.size
+           else np.zeros(0, dtype=np.int64))
+    return sub, pix.astype(np.int64)
+
+
 def view_2d(imap, minfo):
     nphi = minfo.nphi[0]
     if not np.all(minfo.nphi == nphi):

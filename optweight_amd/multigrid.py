@@ -52,7 +52,16 @@ class Level:
     '''One grid of the multigrid hierarchy: the unobserved pixels of its
     equal-area geometry, the operator G = M_u Y d_l Y^T M_u there
     (d_l: (npol, npol, lmax + 1)) and the smoother (Jacobi-like diagonal
-    preconditioner, or a dense pseudo-inverse of G on the coarsest level).'''
+    preconditioner, or a dense pseudo-inverse of G on the coarsest level).
+
+    mask_unobs and minfo describe the level's full grid. Every map of the
+    level vanishes off the rings that hold unobserved pixels (inputs are
+    masked, outputs are masked after), so the level stores its maps on
+    those rings alone: the compact geometry minfo_c (the same rings in the
+    same order, pixel i of a compact map is pixel pix_c[i] of the full
+    grid) with the mask mask_c. v_cycle, restrict and the smoothers work on
+    compact maps; G, the transforms and the pixel passes touch no pixel off
+    those rings.'''
 
     def __init__(self, mask, minfo, d_ell, spin, dense_smoothing=False, device=None):
         self.mask_unobs = _unobserved(mask)
@@ -65,11 +74,13 @@ class Level:
         if device is None:
             device = torch.device('cuda', torch.cuda.current_device())
         self.device = device
-        self._mask = operators._DeviceCache(self.mask_unobs)
         # rings holding unobserved pixels: every transform of this level acts
         # on maps that vanish elsewhere (input) or are masked after (output)
         self.rings = map_utils.ring_support(self.mask_unobs, minfo)
-        self.g_op = operators.PixEllPixMatVecMap(self.mask_unobs, self.d_ell, self.minfo,
+        self.minfo_c, self.pix_c = map_utils.compact_rings(minfo, self.rings)
+        self.mask_c = np.ascontiguousarray(self.mask_unobs[:, self.pix_c])
+        self._mask = operators._DeviceCache(self.mask_c)
+        self.g_op = operators.PixEllPixMatVecMap(self.mask_c, self.d_ell, self.minfo_c,
                                                  self.spin)
         self.pinv_g = None
         if dense_smoothing:
@@ -104,8 +115,9 @@ class Level:
 
     def _init_sparse_smoother(self):
         m_prec = np.zeros(self.npol)
-        sample_idx = np.argmax(self.mask_unobs, axis=-1)
-        test = torch.zeros((self.npol, self.minfo.npix), dtype=torch.float64, device=self.device)
+        sample_idx = np.argmax(self.mask_c, axis=-1)
+        test = torch.zeros((self.npol, self.minfo_c.npix), dtype=torch.float64,
+                           device=self.device)
         for p in range(self.npol):
             test[p, sample_idx[p]] = 1.
             omap = self.g_op(test)
@@ -121,8 +133,9 @@ class Level:
         return smoother
 
     def dense_g(self, batch=512):
-        '''G restricted to unobserved pixels, (n_unobs, n_unobs) float64 host.'''
-        mu = self.mask_unobs
+        '''G restricted to unobserved pixels, (n_unobs, n_unobs) float64 host
+        (pixels in the order of the full grid).'''
+        mu = self.mask_c
         idx = np.nonzero(mu.reshape(-1))[0]
         n = idx.size
         npol, npix = mu.shape
@@ -139,9 +152,9 @@ class Level:
               torch.as_tensor(cols, device=self.device)] = 1
             u = u.reshape(nb, npol, npix) * mdev
             alm = torch.zeros((nb, npol, ainfo.nelem), dtype=torch.complex128, device=self.device)
-            sht.map2alm(u, alm, self.minfo, ainfo, self.spin, adjoint=True)
+            sht.map2alm(u, alm, self.minfo_c, ainfo, self.spin, adjoint=True)
             alm = alm_c_utils.lmul(alm, d, ainfo, inplace=True)
-            sht.alm2map(alm, u, ainfo, self.minfo, self.spin)
+            sht.alm2map(alm, u, ainfo, self.minfo_c, self.spin)
             u = (u * mdev).reshape(nb, npol * npix)
             out[:, start:start + nb] = u[:, idx_t].T.cpu().numpy()
         return out
@@ -149,10 +162,10 @@ class Level:
     def _init_dense_smoother(self):
         mat = self.dense_g()
         pinv_mat = linalg.pinvh(mat)
-        mu_flat = torch.as_tensor(np.nonzero(self.mask_unobs.reshape(-1))[0], device=self.device)
+        mu_flat = torch.as_tensor(np.nonzero(self.mask_c.reshape(-1))[0], device=self.device)
         pcache = operators._DeviceCache(pinv_mat)
 
-        npp = self.mask_unobs.size
+        npp = self.mask_c.size
 
         def smoother(imap):
             # rows: right-hand sides (one for an unbatched map)
@@ -164,9 +177,9 @@ class Level:
         return smoother, pinv_mat
 
     def _g_op_unobs(self, imap_unobs):
-        imap = torch.zeros(self.mask_unobs.shape, dtype=imap_unobs.dtype,
+        imap = torch.zeros(self.mask_c.shape, dtype=imap_unobs.dtype,
                            device=imap_unobs.device)
-        m = torch.as_tensor(self.mask_unobs, device=imap_unobs.device)
+        m = torch.as_tensor(self.mask_c, device=imap_unobs.device)
         imap[m] = imap_unobs
         return self.g_op(imap)[m]
 
@@ -222,9 +235,9 @@ def get_equal_area_mask_bool(mask_bool, minfo, lmax=None, minfo_fullsky=None):
 
 def restrict(imap, level_in, level_out, spin, adjoint=False, premasked=False, acc=None):
     '''Y R Y^T W between levels (W Y R Y^T if adjoint), R a Gaussian beam
-    at the coarser band limit, masks on both sides. premasked: imap already
-    carries level_in's mask. acc: add the result to acc (in place) and
-    return acc.'''
+    at the coarser band limit, masks on both sides, on the levels' compact
+    maps. premasked: imap already carries level_in's mask. acc: add the
+    result to acc (in place) and return acc.'''
     lmax_in = level_in.lmax
     lmax_out = level_out.lmax
     rdt = imap.dtype
@@ -235,16 +248,16 @@ def restrict(imap, level_in, level_out, spin, adjoint=False, premasked=False, ac
     ainfo = alm_utils.AlmInfo(lmax_t)
     # the analysis overwrites every coefficient
     alm = torch.empty(tuple(imap.shape[:-1]) + (ainfo.nelem,), dtype=cdt, device=imap.device)
-    sht.map2alm(imap, alm, level_in.minfo, ainfo, spin, adjoint=adjoint, rings=level_in.rings)
+    sht.map2alm(imap, alm, level_in.minfo_c, ainfo, spin, adjoint=adjoint)
     if not adjoint:
         r_ell = noise_utils._band_limit_gauss_beam(lmax_out, fwhm=2 * np.pi / lmax_out)
     else:
         r_ell = noise_utils._band_limit_gauss_beam(lmax_in, fwhm=2 * np.pi / lmax_in)
     alm_c_utils.lmul(alm, r_ell[:lmax_t + 1], ainfo, inplace=True)
-    omap = torch.zeros(tuple(imap.shape[:-1]) + (level_out.minfo.npix,), dtype=rdt,
+    # the synthesis writes every pixel of the compact geometry
+    omap = torch.empty(tuple(imap.shape[:-1]) + (level_out.minfo_c.npix,), dtype=rdt,
                        device=imap.device)
-    sht.alm2map(alm, omap, ainfo, level_out.minfo, spin, adjoint=adjoint,
-                rings=level_out.rings)
+    sht.alm2map(alm, omap, ainfo, level_out.minfo_c, spin, adjoint=adjoint)
     if acc is not None:
         return map_c_utils.map_fused(omap, x=acc, mask=level_out.mask_dev(imap.device, rdt),
                                      out=acc)

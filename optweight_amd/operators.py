@@ -354,8 +354,11 @@ class PixEllPixMatVecMap(MatVecMap):
         self.adjoint = adjoint
         self.use_weights = use_weights
         # M vanishes off these rings: the transforms skip the others exactly
+        # (None: M touches every ring, the plan of the whole geometry)
         self.rings = (map_utils.ring_support(m_pix, minfo) if m_pix is not None
                       and np.asarray(m_pix).shape[-1] == minfo.npix else None)
+        if self.rings is not None and self.rings.size == minfo.nrow:
+            self.rings = None
 
     def _apply_m(self, out):
         if self._m is None:

@@ -311,15 +311,17 @@ class MaskedPreconditioner(operators.MatVecAlm):
         a = alm_c_utils.lmul(alm, lowpass, self.ainfo, alm_out=torch.empty_like(alm))
         # (nelem,) or (npol, nelem) for one right-hand side, (..., npol, nelem) for a batch
         lead = (self.npol,) if a.ndim == 1 else tuple(a.shape[:-1])
-        rhs = torch.zeros(lead + (self.minfo.npix,), dtype=rdt, device=a.device)
-        sht.alm2map(a, rhs, self.ainfo, self.minfo, self.spin, rings=finest.rings)
+        # maps on the finest level's compact geometry (its rings that hold
+        # unobserved pixels), every pixel written by the synthesis
+        rhs = torch.empty(lead + (finest.minfo_c.npix,), dtype=rdt, device=a.device)
+        sht.alm2map(a, rhs, self.ainfo, finest.minfo_c, self.spin)
         unobs = finest.mask_dev(a.device, rdt)
         rhs *= unobs
         sol = None
         for _ in range(self.nsteps):
             sol = multigrid.v_cycle(self.levels, rhs, self.spin, n_jacobi=self.n_jacobi, x0=sol)
         sol *= unobs
-        sht.map2alm(sol, a, self.minfo, self.ainfo, self.spin, adjoint=True, rings=finest.rings)
+        sht.map2alm(sol, a, finest.minfo_c, self.ainfo, self.spin, adjoint=True)
         return alm_c_utils.lmul(a, lowpass, self.ainfo, inplace=True)
 
 
