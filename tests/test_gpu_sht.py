@@ -253,10 +253,11 @@ def test_batched_legendre_tails(dev, ntrans, dtype):
 
 @pytest.mark.parametrize('lmax,nrings', [(2, 3), (3, 4), (5, 6), (70, 71), (200, 201),
                                          (60, 401), (60, 1101)])
-@pytest.mark.parametrize('ntrans', [8, 13, 16, 24])
+@pytest.mark.parametrize('ntrans', [8, 13, 16, 24, 40])
 def test_matrix_core_legendre(dev, lmax, nrings, ntrans):
     '''float32 batches of at least 8 transforms take the matrix-core Legendre
-    kernels (16 transforms per block, partial last blocks at 13 and 24):
+    kernels (16 transforms per block, partial last blocks at 13, 24 and
+    40):
     synthesis and adjoint of every transform against the float64 transforms
     of the same input (relative 1e-5, or exactly zero where the float64
     result is), and the last transform's synthesis against the oracle
