@@ -433,7 +433,7 @@ __device__ __forceinline__ void bluestein_conv(cplx_t<T> *v, cplx_t<T> *buf, int
 // Bluestein length 2^LOGM, fixed at compile time.
 // The float32 synthesis mode in 256-thread workgroups is held to 128
 // registers: four workgroups per CU (the LDS bound) instead of three, 9 %
-// faster at Bluestein length 4096 (the kernel waits on memory latency).
+// faster at Bluestein length 4096 (more waves to fill the issue slots).
 template <typename T, int MODE, int MAXT, bool GB, int LOGM = 0>
 __global__ void __launch_bounds__(MAXT)
 __attribute__((amdgpu_waves_per_eu(MAXT == 256 && sizeof(T) == 4 && MODE == 0 ? 4 : 1)))
