@@ -695,33 +695,42 @@ ring_conv_kernel(RingArgs a, int logm_rt, const cplx_t<T> *twiddle, const T *mul
   constexpr int MB = coef_mb<T>();
   const long long gp_blk = 2 * MB * a.npairs;  // elements per m block of the adjoint layout
   typedef T T4 __attribute__((ext_vector_type(4)));
-  for (long long t = t_lo; t < t_hi; ++t) {
+  // slot k holds m = k (k <= mmax) or m = k - M (k >= M - mmax), zero
+  // between; the loads of all 16 slots are issued together (from a clamped,
+  // valid m) and the slot's value selected afterwards. float32 with the
+  // length fixed (LOGM): the next transform's loads are issued before this
+  // one's FFTs, which then hide their latency.
+  T4 w[16];
+  auto issue = [&](long long t) {
     const long long tc = t * a.ncomp + c;
     const cplx_t<T> *Fa = Gin + coef_syn_at(tc, 0, sa >> 1, mmax, a.npairs) + (sa & 1);
     const cplx_t<T> *Fb = Gin + coef_syn_at(tc, 0, sb >> 1, mmax, a.npairs) + (sb & 1);
-    cplx_t<T> v[16];
-    if (on) {
-      // slot k holds m = k (k <= mmax) or m = k - M (k >= M - mmax), zero
-      // between; the loads of all 16 slots are issued together (from a
-      // clamped, valid m) and the slot's value selected afterwards
-      T4 w[16];
-      if (joint) {
+    if (!on) return;
+    if (joint) {
 #pragma unroll
-        for (int s = 0; s < 16; ++s) {
-          const int k = q + s * Q;
-          const int am = k <= mmax ? k : (k >= M - mmax ? M - k : 0);
-          w[s] = *reinterpret_cast<const T4 *>(Fa + (long long)am * fs);
-        }
-      } else {
-#pragma unroll
-        for (int s = 0; s < 16; ++s) {
-          const int k = q + s * Q;
-          const int am = k <= mmax ? k : (k >= M - mmax ? M - k : 0);
-          const cplx_t<T> fa = Fa[(long long)am * fs];
-          const cplx_t<T> fb = rb >= 0 ? Fb[(long long)am * fs] : cplx_t<T>{0, 0};
-          w[s] = T4{fa.x, fa.y, fb.x, fb.y};
-        }
+      for (int s = 0; s < 16; ++s) {
+        const int k = q + s * Q;
+        const int am = k <= mmax ? k : (k >= M - mmax ? M - k : 0);
+        w[s] = *reinterpret_cast<const T4 *>(Fa + (long long)am * fs);
       }
+    } else {
+#pragma unroll
+      for (int s = 0; s < 16; ++s) {
+        const int k = q + s * Q;
+        const int am = k <= mmax ? k : (k >= M - mmax ? M - k : 0);
+        const cplx_t<T> fa = Fa[(long long)am * fs];
+        const cplx_t<T> fb = rb >= 0 ? Fb[(long long)am * fs] : cplx_t<T>{0, 0};
+        w[s] = T4{fa.x, fa.y, fb.x, fb.y};
+      }
+    }
+  };
+  constexpr bool AHEAD = LOGM > 0 && sizeof(T) == 4;
+  if (AHEAD) issue(t_lo);
+  for (long long t = t_lo; t < t_hi; ++t) {
+    const long long tc = t * a.ncomp + c;
+    cplx_t<T> v[16];
+    if (!AHEAD) issue(t);
+    if (on) {
 #pragma unroll
       for (int s = 0; s < 16; ++s) {
         const int k = q + s * Q;
@@ -733,6 +742,7 @@ ring_conv_kernel(RingArgs a, int logm_rt, const cplx_t<T> *twiddle, const T *mul
         v[s] = {za.x - zb.y, za.y + zb.x};
       }
     }
+    if (AHEAD && t + 1 < t_hi) issue(t + 1);
     if constexpr (DIRECT)
       fft16<T, -1, GB, LOGM>(v, buf, logm, q, twd, a.log_mmax_fft);
     else
