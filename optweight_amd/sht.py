@@ -223,18 +223,21 @@ def spectral_enabled():
     return os.environ.get('OWMI_SANDWICH', 'spectral') != 'bluestein'
 
 
-def sandwich_multipliers(pixdiag, minfo, lmax):
+def sandwich_multipliers(pixdiag, minfo, lmax, rings=None):
     '''Per-ring multipliers (npol, nrings, M) of the spectral sandwich for a
     pixel diagonal (npol, npix) shared by every transform, or None where the
-    plan's ring FFTs do not cover the length M (owmi_sht_sandwich_length).
-    They are computed once (float64 sums, stored in the diagonal's dtype)
-    and kept on the diagonal tensor with its version counter, so an in-place
-    change of the diagonal recomputes them.'''
-    plan = get_plan(minfo, lmax, pixdiag.device)
+    plan's ring FFTs do not cover the length M (owmi_sht_sandwich_length);
+    rings: the plan restricted to these rings (nrings of them). They are
+    computed once (float64 FFTs, stored in the diagonal's dtype) and kept on
+    the diagonal tensor with its version counter, so an in-place change of
+    the diagonal recomputes them.'''
+    plan = get_plan(minfo, lmax, pixdiag.device, rings=rings)
     M = int(plan._lib.owmi_sht_sandwich_length(plan.handle))
     if M == 0:
         return None
-    key = (int(lmax), M, pixdiag.dtype, str(pixdiag.device))
+    nr = minfo.nrow if rings is None else len(rings)
+    rkey = None if rings is None or nr == minfo.nrow else np.asarray(rings, np.int64).tobytes()
+    key = (int(lmax), M, pixdiag.dtype, str(pixdiag.device), rkey)
     cached = getattr(pixdiag, '_owmi_mult', None)
     if (cached is not None and cached[0] == pixdiag._version and cached[1] == key
             and cached[2]() is minfo):
@@ -242,7 +245,7 @@ def sandwich_multipliers(pixdiag, minfo, lmax):
     d = pixdiag.reshape((-1, minfo.npix))
     if not d.is_contiguous():
         raise ValueError('pixdiag must be contiguous')
-    mult = torch.empty((d.shape[0], minfo.nrow, M), dtype=d.dtype, device=d.device)
+    mult = torch.empty((d.shape[0], nr, M), dtype=d.dtype, device=d.device)
     _lib.check(plan._lib.owmi_sht_sandwich_multipliers(
         plan.handle, _lib.dtype_code(d), _lib.ptr(d), d.shape[0], _lib.ptr(mult),
         _lib.stream_ptr(d.device)))
@@ -250,13 +253,15 @@ def sandwich_multipliers(pixdiag, minfo, lmax):
     return mult
 
 
-def pixel_sandwich(alm, oalm, ainfo, minfo, spin, pixdiag, spectral=None):
+def pixel_sandwich(alm, oalm, ainfo, minfo, spin, pixdiag, spectral=None, rings=None):
     '''oalm = Y^T D Y alm with D = pixdiag (npol, npix) per component; the
     map stays on chip between the two transforms. A diagonal shared by all
     transforms runs as the spectral sandwich (per-ring multipliers from
     sandwich_multipliers, two FFTs of length M per ring pair) unless
     spectral is False or OWMI_SANDWICH=bluestein; a per-transform diagonal,
-    or a plan without the spectral length, runs the Bluestein sandwich.'''
+    or a plan without the spectral length, runs the Bluestein sandwich.
+    rings: sorted ring indices outside which D vanishes (ring_support of
+    the diagonal); the others contribute nothing and are skipped.'''
     lmax = _lmax_of(ainfo)
     _lib.require_cuda(alm, oalm, pixdiag)
     if tuple(alm.shape) != tuple(oalm.shape) or alm.dtype != oalm.dtype:
@@ -268,14 +273,14 @@ def pixel_sandwich(alm, oalm, ainfo, minfo, spin, pixdiag, spectral=None):
         raise ValueError(f'pixdiag shape {tuple(pixdiag.shape)} != {(npol, minfo.npix)}')
     p3 = pixdiag.reshape((-1, npol, minfo.npix))
     pix_ts = 0 if p3.shape[0] == 1 else npol * minfo.npix
-    plan = get_plan(minfo, lmax, alm.device)
+    plan = get_plan(minfo, lmax, alm.device, rings=rings)
     code = _lib.dtype_code(alm)
     st = _lib.stream_ptr(alm.device)
     if spectral is None:
         spectral = spectral_enabled()
     mult = None
     if spectral and pix_ts == 0 and pixdiag.dtype == _lib.real_dtype(alm.dtype):
-        mult = sandwich_multipliers(pixdiag, minfo, lmax)
+        mult = sandwich_multipliers(pixdiag, minfo, lmax, rings=rings)
     if mult is not None:
         for s, i1, i2 in spin_groups(spin, npol):
             _lib.check(plan._lib.owmi_sht_pixel_sandwich_spectral(

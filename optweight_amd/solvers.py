@@ -246,8 +246,10 @@ class CGWienerMap(CG):
         '''(pixel diagonal, beam matrix, S^-1 matrix, Y op) when A can run
         fused, else None.'''
         key = (str(alm.device), alm.dtype)
-        if self._fast is not None and self._fast[0] == key:
-            return self._fast[1]
+        if self._fast is None:
+            self._fast = {}
+        if key in self._fast:
+            return self._fast[key]
         plan = None
         y, yt = self.sht, self.sht_adjoint
         ok = (not self.swap_bm and self._sfilt_op is None and self._filt_op is None
@@ -263,21 +265,27 @@ class CGWienerMap(CG):
             # it is kept across resets of the plan (a new S^-1, as the
             # differentiable solve sets per evaluation), and with it the
             # spectral-sandwich multipliers cached on it
-            dkey = (key, id(self.icov_noise), id(self._mask_op))
+            # one entry per (device, dtype) and operator pair, so that the
+            # float32 and float64 operators of a mixed solve do not evict each
+            # other (a recording HIP graph finds its entry made eagerly)
+            okey = (id(self.icov_noise), id(self._mask_op))
             cached = getattr(self, '_fast_diag', None)
-            if cached is not None and cached[0] == dkey:
-                d = cached[1]
-            else:
+            if cached is None or cached[0] != okey:
+                cached = (okey, {}, self.icov_noise, self._mask_op)
+                self._fast_diag = cached
+            diags = cached[1]
+            if key not in diags:
                 d = self.icov_noise.diagonal(alm.device, rdt)
                 if d is not None and self._mask_op is not None:
                     md = self._mask_op.diagonal(alm.device, rdt)
                     d = None if md is None else (md * d * md).contiguous()
-                self._fast_diag = (dkey, d, self.icov_noise, self._mask_op)
+                diags[key] = d
+            d = diags[key]
             if d is not None:
                 beam = (self._beam_op.device_matrix(alm.device, alm.dtype)
                         if self._beam_op is not None else None)
                 plan = (d, beam, self.icov_signal.device_matrix(alm.device, alm.dtype), y)
-        self._fast = (key, plan)
+        self._fast[key] = plan
         return plan
 
     def a_matrix(self, alm):

@@ -65,10 +65,11 @@ def test_b_vec_and_a_matrix(dev, pol_problem):
     v[:, :prob.lmax + 1] = v[:, :prob.lmax + 1].real
     ref = osol.a_matrix(v)
     got = solver.a_matrix(torch.as_tensor(v, device=dev))
-    assert solver._fast is not None and solver._fast[1] is not None, 'fused A path not taken'
+    assert solver._fast and all(v is not None for v in solver._fast.values()), \
+        'fused A path not taken'
     assert rel(got, ref) < 1e-12
     # the unfused (general callable) path gives the same result
-    solver._fast = (solver._fast[0], None)
+    solver._fast = {k: None for k in solver._fast}
     got2 = solver.a_matrix(torch.as_tensor(v, device=dev))
     assert rel(got2, ref) < 1e-12
 
