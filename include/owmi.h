@@ -83,6 +83,19 @@ int owmi_lmul_add(int dtype, const void *lmat_a, const void *a,
                   int ncomp, long long ntrans, void *stream);
 
 /*
+ * owmi_lmul_add with the real-field <b_t, out_t> of every transform t
+ * written to dev_dot[t] (float64; as owmi_alm_dot_batch of b with out
+ * would give, summed in another fixed order): the last pass of the fused
+ * CGWienerMap.a_matrix (solvers.py:217-242) with the CG coefficient
+ * <p, A p> of pixell.utils.CG.step folded in (b = p, out = A p), so p and
+ * A p are not read again. ntrans <= 1024; b may not alias out. Uses the
+ * stream's partial-sum buffer of owmi_alm_dot_batch.
+ */
+int owmi_lmul_add_dot(int dtype, const void *lmat_a, const void *a,
+                      const void *lmat_b, const void *b, void *out, int lmax,
+                      int ncomp, long long ntrans, double *dev_dot, void *stream);
+
+/*
  * Copy l <= lmax_out, m <= lmax_out of (ntrans*ncomp) alm rows at lmax into
  * rows at lmax_out. Replaces trunc_alm_dp / trunc_alm_sp
  * (include/optweight_alm_c_utils.h:92-101, 142-146).

@@ -27,6 +27,8 @@ _SIGNATURES = {
     'owmi_lmul': (_c_int, [_c_int, _vp, _c_int, _vp, _vp, _c_int, _c_int, _c_ll, _vp]),
     'owmi_trunc_alm': (_c_int, [_c_int, _vp, _vp, _c_int, _c_int, _c_ll, _vp]),
     'owmi_lmul_add': (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _c_ll, _vp]),
+    'owmi_lmul_add_dot': (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _c_ll, _vp,
+                                   _vp]),
     'owmi_map_fused': (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _vp, _c_int, _c_ll, _c_ll,
                                 _vp]),
     'owmi_wlm2alm': (_c_int, [_c_int, _vp, _vp, _vp, _c_int, _c_int, _c_ll, _vp]),

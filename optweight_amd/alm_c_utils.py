@@ -97,10 +97,12 @@ def lmul(alm, lmat, ainfo, alm_out=None, inplace=False):
     return out.reshape(shape_in)
 
 
-def lmul_add(a, lmat_a, b, lmat_b, ainfo, out):
+def lmul_add(a, lmat_a, b, lmat_b, ainfo, out, dot_out=None):
     '''out = lmat_a a + lmat_b b (lmat_a None: a itself) in one pass, full
     (npol, npol, nell) device matrices of the alm precision; a or b may be
-    out.'''
+    out. dot_out: float64 device tensor (ntrans,) that receives the
+    real-field <b, out> of every transform in the same pass (b may then not
+    be out).'''
     _lib.require_cuda(a, b, out)
     if not (a.shape == b.shape == out.shape) or not (a.dtype == b.dtype == out.dtype):
         raise ValueError('a, b and out must have one shape and dtype')
@@ -122,6 +124,15 @@ def lmul_add(a, lmat_a, b, lmat_b, ainfo, out):
     if mats[1] is None:
         raise ValueError('lmat_b is required')
     ntrans = a.numel() // (npol * nelem)
+    if dot_out is not None:
+        _lib.require_cuda(dot_out)
+        if dot_out.dtype != torch.float64 or dot_out.numel() != ntrans:
+            raise ValueError(f'dot_out must be float64 with {ntrans} elements')
+        _lib.call('owmi_lmul_add_dot', _lib.dtype_code(a),
+                  None if mats[0] is None else _lib.ptr(mats[0]), _lib.ptr(a),
+                  _lib.ptr(mats[1]), _lib.ptr(b), _lib.ptr(out), lmax, npol, ntrans,
+                  _lib.ptr(dot_out), _lib.stream_ptr(a.device))
+        return out
     _lib.call('owmi_lmul_add', _lib.dtype_code(a),
               None if mats[0] is None else _lib.ptr(mats[0]), _lib.ptr(a), _lib.ptr(mats[1]),
               _lib.ptr(b), _lib.ptr(out), lmax, npol, ntrans, _lib.stream_ptr(a.device))

@@ -634,6 +634,70 @@ __global__ void cg_xr_batch_kernel(cplx_t<X> *x, cplx_t<R> *r, const cplx_t<R> *
   r[o] = {rv.x - a * qv.x, rv.y - a * qv.y};
 }
 
+// lmul_add_kernel with the real-field <b, out> of every transform folded in
+// (the CG coefficient <p, A p> when b = p and out = A p): block (x, t)
+// takes a strided slice of the elements of transform t (all components of
+// an element in one thread, m = 0 elements counted once, the rest twice)
+// and writes its partial sum; dot_batch_final_kernel adds the partials of
+// each transform in fixed order. a may alias out (every input of an element
+// is read before it is written).
+template <typename T, int NC, bool HASA>
+__global__ void __launch_bounds__(kDotThreads)
+lmul_add_dot_kernel(const T *__restrict__ la, const cplx_t<T> *a, const T *__restrict__ lb,
+                    const cplx_t<T> *b, cplx_t<T> *out, int lmax, long long nelem,
+                    double *partial) {
+  __shared__ double red[kDotThreads];
+  const long long t = blockIdx.y;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  const int nell = lmax + 1;
+  double acc = 0.0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nelem; i += stride) {
+    int l, m;
+    lm_of(i, lmax, l, m);
+    const long long base = t * NC * nelem + i;
+    cplx_t<T> va[NC], vb[NC];
+#pragma unroll
+    for (int j = 0; j < NC; ++j) {
+      va[j] = a[base + j * nelem];
+      vb[j] = b[base + j * nelem];
+    }
+    double v = 0.0;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      T ra = 0, ia = 0;
+      if (HASA) {
+#pragma unroll
+        for (int j = 0; j < NC; ++j) {
+          const T w = la[(c * NC + j) * nell + l];
+          ra += w * va[j].x;
+          ia += w * va[j].y;
+        }
+      } else {
+        ra = va[c].x;
+        ia = va[c].y;
+      }
+      T rb = 0, ib = 0;
+#pragma unroll
+      for (int j = 0; j < NC; ++j) {
+        const T w = lb[(c * NC + j) * nell + l];
+        rb += w * vb[j].x;
+        ib += w * vb[j].y;
+      }
+      const cplx_t<T> o = {ra + rb, ia + ib};
+      out[base + c * nelem] = o;
+      v += (double)vb[c].x * (double)o.x + (double)vb[c].y * (double)o.y;
+    }
+    acc += (i <= lmax) ? v : 2.0 * v;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = kDotThreads / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[t * gridDim.x + blockIdx.x] = red[0];
+}
+
 // The update of cg_xr_batch_kernel with the real-field <r, r> of the new
 // residual folded in: block (x, b) updates a strided slice of every row of
 // right-hand side b (the slices of dot_batch_partial_kernel, m = 0 elements
@@ -708,6 +772,41 @@ convert_kernel(const cplx_t<S> *src, cplx_t<D> *dst, long long n) {
 
 using namespace owmi;
 
+template <typename T, int NC>
+static void lmul_add_dot_nc(const T *la, const cplx_t<T> *a, const T *lb, const cplx_t<T> *b,
+                            cplx_t<T> *out, int lmax, long long ntrans, double *dev_dot,
+                            hipStream_t st) {
+  const long long nelem = nelem_of(lmax);
+  const int bpb = kDotBlocks / (int)ntrans;
+  double *part = dot_scratch(st);
+  dim3 g((unsigned)bpb, (unsigned)ntrans);
+  if (la)
+    hipLaunchKernelGGL((lmul_add_dot_kernel<T, NC, true>), g, dim3(kDotThreads), 0, st, la, a,
+                       lb, b, out, lmax, nelem, part);
+  else
+    hipLaunchKernelGGL((lmul_add_dot_kernel<T, NC, false>), g, dim3(kDotThreads), 0, st, la, a,
+                       lb, b, out, lmax, nelem, part);
+  hipLaunchKernelGGL(dot_batch_final_kernel, dim3((unsigned)ntrans), dim3(kDotThreads), 0, st,
+                     part, bpb, dev_dot);
+}
+
+template <typename T>
+static void lmul_add_dot_impl(const void *la, const void *a, const void *lb, const void *b,
+                              void *out, int lmax, int ncomp, long long ntrans, double *dev_dot,
+                              hipStream_t st) {
+  using C = cplx_t<T>;
+  const T *pla = (const T *)la, *plb = (const T *)lb;
+  const C *pa = (const C *)a, *pb = (const C *)b;
+  C *po = (C *)out;
+  switch (ncomp) {
+    case 1: return lmul_add_dot_nc<T, 1>(pla, pa, plb, pb, po, lmax, ntrans, dev_dot, st);
+    case 2: return lmul_add_dot_nc<T, 2>(pla, pa, plb, pb, po, lmax, ntrans, dev_dot, st);
+    case 3: return lmul_add_dot_nc<T, 3>(pla, pa, plb, pb, po, lmax, ntrans, dev_dot, st);
+    case 4: return lmul_add_dot_nc<T, 4>(pla, pa, plb, pb, po, lmax, ntrans, dev_dot, st);
+    default: throw Error("lmul_add_dot: ncomp must be in 1..4");
+  }
+}
+
 extern "C" {
 
 const char *owmi_last_error(void) { return owmi::g_last_error.c_str(); }
@@ -761,6 +860,26 @@ int owmi_lmul_add(int dtype, const void *lmat_a, const void *a, const void *lmat
       lmul_add_impl<double>(lmat_a, a, lmat_b, b, out, lmax, ncomp, ntrans, as_stream(stream));
     else
       lmul_add_impl<float>(lmat_a, a, lmat_b, b, out, lmax, ncomp, ntrans, as_stream(stream));
+  });
+}
+
+int owmi_lmul_add_dot(int dtype, const void *lmat_a, const void *a, const void *lmat_b,
+                      const void *b, void *out, int lmax, int ncomp, long long ntrans,
+                      double *dev_dot, void *stream) {
+  return guard([&] {
+    check_dtype(dtype);
+    OWMI_REQUIRE(lmax >= 0 && ncomp >= 1 && ntrans >= 1 && ntrans <= kDotBlocks,
+                 "invalid lmul_add_dot sizes (ntrans must be in [1, 1024])");
+    OWMI_REQUIRE(lmat_b != nullptr && a != nullptr && b != nullptr && out != nullptr &&
+                     dev_dot != nullptr,
+                 "lmul_add_dot: lmat_b, a, b, out and dev_dot are required");
+    OWMI_REQUIRE(out != b, "lmul_add_dot: b (the dot's first factor) may not alias out");
+    if (dtype == OWMI_F64)
+      lmul_add_dot_impl<double>(lmat_a, a, lmat_b, b, out, lmax, ncomp, ntrans, dev_dot,
+                                as_stream(stream));
+    else
+      lmul_add_dot_impl<float>(lmat_a, a, lmat_b, b, out, lmax, ncomp, ntrans, dev_dot,
+                               as_stream(stream));
   });
 }
 

@@ -39,6 +39,7 @@ the launches already overlap the kernels and the recording costs more than
 it saves (49.4 against 50.7 solves/s), hence off by default.
 '''
 import gc
+import inspect
 import os
 
 import numpy as np
@@ -115,6 +116,16 @@ def _dot(a, b, out=None):
     return alm_utils.contract_almxblm_batch(a, b, out=out)
 
 
+def _takes_dot_out(op):
+    '''Whether the operator callable accepts dot_out= (it then writes the
+    real-field <p, A p> of every right-hand side itself, as
+    CGWienerMap.a_matrix does).'''
+    try:
+        return 'dot_out' in inspect.signature(op).parameters
+    except (TypeError, ValueError):
+        return False
+
+
 def _relative(rr, bb):
     '''sqrt(rr / bb) per right-hand side; 0 for b = 0 (solved by x = 0).'''
     return np.sqrt(np.divide(rr, bb, out=np.zeros_like(rr), where=bb > 0))
@@ -164,8 +175,13 @@ class _State:
 
     def phase(self, k):
         if k == 0:
-            q = self.a_op(self.p)
-            _dot(self.p, q, out=self.pq)
+            if getattr(self, '_a_dot', None) is None:
+                self._a_dot = _takes_dot_out(self.a_op)
+            if self._a_dot:
+                q = self.a_op(self.p, dot_out=self.pq)  # <p, A p> in A's last pass
+            else:
+                q = self.a_op(self.p)
+                _dot(self.p, q, out=self.pq)
             _xr_rr(self.x, self.r, self.p, q, self.rz, self.pq, self.rr, self.nb)
         else:
             z = self.m_op(self.r)

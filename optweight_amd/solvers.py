@@ -288,7 +288,10 @@ class CGWienerMap(CG):
         self._fast[key] = plan
         return plan
 
-    def a_matrix(self, alm):
+    def a_matrix(self, alm, dot_out=None):
+        '''A alm. dot_out (batched alm, device tensors): float64 (nbatch,)
+        that receives the real-field <alm, A alm> per right-hand side (on
+        the fused path in the operator's last pass).'''
         fast = self._fast_plan(alm) if isinstance(alm, torch.Tensor) else None
         if fast is not None:
             d, beam, icov, y = fast
@@ -296,19 +299,24 @@ class CGWienerMap(CG):
             t = alm_c_utils.lmul(alm, beam, ainfo) if beam is not None else alm
             u = torch.empty_like(alm)
             sht_tools.pixel_sandwich(t, u, ainfo, y.minfo, y.spin, d)
-            if alm.shape[-2] > 4:
-                if beam is not None:
-                    alm_c_utils.lmul(u, beam, ainfo, inplace=True)
-                u += alm_c_utils.lmul(alm, icov, ainfo)
-                return u
-            # B u + S^-1 alm in one pass
-            return alm_c_utils.lmul_add(u, beam, alm, icov, ainfo, out=u)
-        alm = self.scale_filter(alm)
-        omap = self.proj(alm)
-        omap = self.icov_noise(omap)
-        oalm = self.proj_adjoint(omap)
-        oalm = self.scale_filter(oalm)
-        oalm += self.scale_filter(self.icov_signal(alm))
+            if alm.shape[-2] <= 4:
+                # B u + S^-1 alm in one pass (and <alm, A alm> with it)
+                return alm_c_utils.lmul_add(u, beam, alm, icov, ainfo, out=u, dot_out=dot_out)
+            if beam is not None:
+                alm_c_utils.lmul(u, beam, ainfo, inplace=True)
+            u += alm_c_utils.lmul(alm, icov, ainfo)
+            oalm = u
+        else:
+            a0 = alm
+            alm = self.scale_filter(alm)
+            omap = self.proj(alm)
+            omap = self.icov_noise(omap)
+            oalm = self.proj_adjoint(omap)
+            oalm = self.scale_filter(oalm)
+            oalm += self.scale_filter(self.icov_signal(alm))
+            alm = a0
+        if dot_out is not None:
+            alm_utils.contract_almxblm_batch(alm, oalm, out=dot_out)
         return oalm
 
     def get_b_vec(self, imap):

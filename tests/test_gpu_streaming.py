@@ -251,6 +251,42 @@ def test_lmul_add(dev, dtype, npol, case):
 
 
 @pytest.mark.parametrize('dtype', ['float64', 'float32'])
+@pytest.mark.parametrize('npol', [1, 3])
+@pytest.mark.parametrize('ntrans', [1, 5, 128])
+@pytest.mark.parametrize('alias_a', [False, True])
+def test_lmul_add_dot(dev, dtype, npol, ntrans, alias_a):
+    '''owmi_lmul_add_dot: out equal to owmi_lmul_add's bit for bit, and the
+    real-field <b, out> of every transform equal to owmi_alm_dot_batch of
+    b with out (relative 1e-13 in float64, 1e-12 in float32: the products
+    are summed in float64 in another fixed order), with out aliasing a.'''
+    print(f'lmul_add_dot npol={npol} ntrans={ntrans} {dtype} alias_a={alias_a}', flush=True)
+    from optweight_amd import alm_c_utils
+    lmax = 37
+    rng = np.random.default_rng(10 * npol + ntrans)
+    n = layout.nelem(lmax)
+    a = rng.normal(size=(ntrans, npol, n)) + 1j * rng.normal(size=(ntrans, npol, n))
+    b = rng.normal(size=(ntrans, npol, n)) + 1j * rng.normal(size=(ntrans, npol, n))
+    a[..., :lmax + 1] = a[..., :lmax + 1].real
+    b[..., :lmax + 1] = b[..., :lmax + 1].real
+    la = rng.normal(size=(npol, npol, lmax + 1))
+    lb = rng.normal(size=(npol, npol, lmax + 1))
+    cdt = torch.complex128 if dtype == 'float64' else torch.complex64
+    ai = alm_utils.AlmInfo(lmax)
+    at, bt = torch.as_tensor(a, device=dev).to(cdt), torch.as_tensor(b, device=dev).to(cdt)
+    ref = alm_c_utils.lmul_add(at, la, bt, lb, ai, out=torch.empty_like(at))
+    dref = alm_utils.contract_almxblm_batch(bt, ref).cpu().numpy()
+    a2 = at.clone()
+    out = a2 if alias_a else torch.empty_like(at)
+    dot = torch.empty(ntrans, dtype=torch.float64, device=dev)
+    got = alm_c_utils.lmul_add(a2, la, bt, lb, ai, out=out, dot_out=dot)
+    assert torch.equal(got, ref)
+    tol = 1e-13 if dtype == 'float64' else 1e-12
+    assert np.all(np.abs(dot.cpu().numpy() - dref) <= tol * np.abs(dref))
+    with pytest.raises(Exception):
+        alm_c_utils.lmul_add(at, la, bt, lb, ai, out=bt, dot_out=dot)
+
+
+@pytest.mark.parametrize('dtype', ['float64', 'float32'])
 @pytest.mark.parametrize('case', ['jacobi', 'masked_residual', 'masked_add', 'scale'])
 @pytest.mark.parametrize('npix', [1000, 1001])
 def test_map_fused(dev, dtype, case, npix):
